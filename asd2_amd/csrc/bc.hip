@@ -1,0 +1,241 @@
+// bc.hip -- ghost-cell fill of the staggered grids (reference src/bc2d.cpp).
+#include "internal.h"
+
+namespace asd2 {
+
+struct BcArgs {
+    int kind[4];
+    double c[4];
+};
+
+// ------------------------------------------------------------------ P grid
+// One thread per boundary line; each thread fills the g ghosts of its line in the
+// reference's sequential order (needed by the LS-free extrapolation, bc2d.cpp:896-934).
+// W/E lines only cover interior j and S/N lines interior i: corner ghosts are never
+// written (bc2d.cpp:810-832), so the four sides are independent.
+__global__ __launch_bounds__(256) void k_bc_p(Geom G, BcArgs A, double* __restrict__ a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = G.nx, ny = G.ny, g = G.g;
+    int side, line;
+    if (t < ny) { side = 0; line = t; }
+    else if (t < 2 * ny) { side = 1; line = t - ny; }
+    else if (t < 2 * ny + nx) { side = 2; line = t - 2 * ny; }
+    else if (t < 2 * ny + 2 * nx) { side = 3; line = t - 2 * ny - nx; }
+    else return;
+    const int k = A.kind[side];
+    const double c = A.c[side];
+    const bool neu = (k == ASD2_BC_NEUMANN || k == ASD2_BC_AXISYM || k == ASD2_BC_WALL);
+    const bool dir = (k == ASD2_BC_DIRICHLET || k == ASD2_BC_INLET || k == ASD2_BC_OUTLET ||
+                      k == ASD2_BC_PRESSURE);
+    const bool lsf = (k == ASD2_BC_LSFREE);
+    if (side == 0) {
+        const int j = g + line;
+        if (lsf) {
+            for (int i = g - 1; i >= 0; i--)
+                a[G.idx(i, j)] = a[G.idx(i + 1, j)] + G.dx * (a[G.idx(i + 1, j)] - a[G.idx(i + 2, j)]) / G.dx;
+        } else if (neu || dir) {
+            for (int i = 0; i < g; i++) {
+                const double m = a[G.idx(2 * g - i - 1, j)];
+                a[G.idx(i, j)] = neu ? m : (-m + 2.0 * c);
+            }
+        }
+    } else if (side == 1) {
+        const int j = g + line;
+        if (lsf) {
+            for (int i = 0; i < g; i++)
+                a[G.idx(nx + g + i, j)] = a[G.idx(nx + g + i - 1, j)]
+                    + G.dx * (a[G.idx(nx + g + i - 1, j)] - a[G.idx(nx + g + i - 2, j)]) / G.dx;
+        } else if (neu || dir) {
+            for (int i = 0; i < g; i++) {
+                const double m = a[G.idx(nx + i, j)];
+                a[G.idx(nx + 2 * g - i - 1, j)] = neu ? m : (-m + 2.0 * c);
+            }
+        }
+    } else if (side == 2) {
+        const int i = g + line;
+        if (lsf) {
+            for (int j = g - 1; j >= 0; j--)
+                a[G.idx(i, j)] = a[G.idx(i, j + 1)] + G.dy * (a[G.idx(i, j + 1)] - a[G.idx(i, j + 2)]) / G.dy;
+        } else if (neu || dir) {
+            for (int j = 0; j < g; j++) {
+                const double m = a[G.idx(i, 2 * g - j - 1)];
+                a[G.idx(i, j)] = neu ? m : (-m + 2.0 * c);
+            }
+        }
+    } else {
+        const int i = g + line;
+        if (lsf) {
+            for (int j = 0; j < g; j++)
+                a[G.idx(i, ny + g + j)] = a[G.idx(i, ny + g + j - 1)]
+                    + G.dy * (a[G.idx(i, ny + g + j - 1)] - a[G.idx(i, ny + g + j - 2)]) / G.dy;
+        } else if (neu || dir) {
+            for (int j = 0; j < g; j++) {
+                const double m = a[G.idx(i, ny + j)];
+                a[G.idx(i, ny + 2 * g - j - 1)] = neu ? m : (-m + 2.0 * c);
+            }
+        }
+    }
+}
+
+int bc_apply_p(asd2_ctx* ctx, double* d) {
+    for (int s = 0; s < 4; s++) {
+        if (ctx->bc[ASD2_GRID_P][s] == ASD2_BC_PERIODIC) {
+            set_error("periodic P boundaries are not supported on the GPU path");
+            return ASD2_ERR_UNSUPPORTED;
+        }
+    }
+    BcArgs A;
+    for (int s = 0; s < 4; s++) { A.kind[s] = ctx->bc[ASD2_GRID_P][s]; A.c[s] = ctx->bcc[ASD2_GRID_P][s]; }
+    const int lines = 2 * (ctx->geo.nx + ctx->geo.ny);
+    hipLaunchKernelGGL(k_bc_p, dim3(cdiv(lines, 256)), dim3(256), 0, ctx->stream, ctx->geo, A, d);
+    ASD2_LAUNCH_CHECK();
+    return ASD2_OK;
+}
+
+// ------------------------------------------------------------------ U / V grids
+// One workgroup walks the sides W, E, S, N in the reference order with a barrier
+// between sides: the U/V rules overlap at corners (bc2d.cpp:599-627,705-733), so the
+// side order is part of the result.  Within a side all writes are independent.
+#define PFOR(count) for (int q = threadIdx.x; q < (count); q += blockDim.x)
+
+__device__ void u_side(const Geom& G, int side, int k, double c, double* a) {
+    const int nx = G.nx, ny = G.ny, g = G.g;
+    const int H = ny + 2 * g;
+    if (side == 0) {
+        if (k == ASD2_BC_PERIODIC) {
+            PFOR((g + 1) * ny) { const int i = q / ny, j = g + q % ny; a[G.idx(i, j)] = a[G.idx(nx - 1 + i, j)]; }
+        } else if (k == ASD2_BC_NEUMANN || k == ASD2_BC_OUTLET) {
+            PFOR(H) a[G.idx(g, q)] = a[G.idx(g + 1, q)];
+            PFOR(g * (ny + 1)) { const int i = q / (ny + 1), j = g + q % (ny + 1); a[G.idx(i, j)] = a[G.idx(2 * g - i, j)]; }
+        } else if (k == ASD2_BC_AXISYM || k == ASD2_BC_WALL || k == ASD2_BC_DIRICHLET || k == ASD2_BC_INLET) {
+            PFOR(H) a[G.idx(g, q)] = c;
+            PFOR(g * ny) { const int i = q / ny, j = g + q % ny; a[G.idx(i, j)] = -a[G.idx(2 * g - i, j)] + 2.0 * c; }
+        }
+    } else if (side == 1) {
+        if (k == ASD2_BC_PERIODIC) {
+            PFOR(g * ny) { const int i = q / ny, j = g + q % ny; a[G.idx(nx + i + g, j)] = a[G.idx(i + g, j)]; }
+        } else if (k == ASD2_BC_NEUMANN || k == ASD2_BC_OUTLET) {
+            PFOR(H) a[G.idx(nx + g, q)] = a[G.idx(nx + g - 1, q)];
+            PFOR((g - 1) * (ny + 1)) {
+                const int i = 1 + q / (ny + 1), j = g + q % (ny + 1);
+                a[G.idx(nx + 2 * g - i, j)] = a[G.idx(nx + i, j)];
+            }
+        } else if (k == ASD2_BC_AXISYM || k == ASD2_BC_WALL || k == ASD2_BC_DIRICHLET || k == ASD2_BC_INLET) {
+            PFOR(H) a[G.idx(nx + g, q)] = c;
+            PFOR((g - 1) * ny) {
+                const int i = 1 + q / ny, j = g + q % ny;
+                a[G.idx(nx + 2 * g - i, j)] = -a[G.idx(nx + i, j)] + 2.0 * c;
+            }
+        }
+    } else if (side == 2) {
+        if (k == ASD2_BC_PERIODIC) {
+            PFOR((nx - 1) * g) { const int i = g + 1 + q / g, j = q % g; a[G.idx(i, j)] = a[G.idx(i, ny + j)]; }
+        } else if (k == ASD2_BC_NEUMANN || k == ASD2_BC_AXISYM || k == ASD2_BC_OUTLET) {
+            PFOR((nx - 1) * g) { const int i = g + 1 + q / g, j = q % g; a[G.idx(i, j)] = a[G.idx(i, 2 * g - j - 1)]; }
+        } else if (k == ASD2_BC_WALL || k == ASD2_BC_DIRICHLET || k == ASD2_BC_INLET) {
+            PFOR(nx * g) { const int i = g + q / g, j = q % g; a[G.idx(i, j)] = -a[G.idx(i, 2 * g - j - 1)] + 2.0 * c; }
+        }
+    } else {
+        if (k == ASD2_BC_PERIODIC) {
+            PFOR((nx - 1) * g) { const int i = g + 1 + q / g, j = q % g; a[G.idx(i, ny + j + g)] = a[G.idx(i, j + g)]; }
+        } else if (k == ASD2_BC_NEUMANN || k == ASD2_BC_AXISYM || k == ASD2_BC_OUTLET) {
+            PFOR((nx - 1) * g) {
+                const int i = g + 1 + q / g, j = q % g;
+                a[G.idx(i, ny + 2 * g - j - 1)] = a[G.idx(i, ny + j)];
+            }
+        } else if (k == ASD2_BC_WALL || k == ASD2_BC_DIRICHLET || k == ASD2_BC_INLET) {
+            PFOR(nx * g) {
+                const int i = g + q / g, j = q % g;
+                a[G.idx(i, ny + 2 * g - j - 1)] = -a[G.idx(i, ny + j)] + 2.0 * c;
+            }
+        }
+    }
+}
+
+__device__ void v_side(const Geom& G, int side, int k, double c, double* a) {
+    const int nx = G.nx, ny = G.ny, g = G.g;
+    const int W = nx + 2 * g;
+    if (side == 0) {
+        if (k == ASD2_BC_PERIODIC) {
+            PFOR(g * (ny - 1)) { const int i = q / (ny - 1), j = g + 1 + q % (ny - 1); a[G.idx(i, j)] = a[G.idx(nx + i, j)]; }
+        } else if (k == ASD2_BC_NEUMANN || k == ASD2_BC_AXISYM || k == ASD2_BC_OUTLET) {
+            PFOR(g * (ny - 1)) { const int i = q / (ny - 1), j = g + 1 + q % (ny - 1); a[G.idx(i, j)] = a[G.idx(2 * g - i - 1, j)]; }
+        } else if (k == ASD2_BC_WALL || k == ASD2_BC_DIRICHLET || k == ASD2_BC_INLET) {
+            PFOR(g * (ny - 1)) {
+                const int i = q / (ny - 1), j = g + 1 + q % (ny - 1);
+                a[G.idx(i, j)] = -a[G.idx(2 * g - i - 1, j)] + 2.0 * c;
+            }
+        }
+    } else if (side == 1) {
+        if (k == ASD2_BC_PERIODIC) {
+            PFOR(g * (ny - 1)) { const int i = q / (ny - 1), j = g + 1 + q % (ny - 1); a[G.idx(nx + i + g, j)] = a[G.idx(i + g, j)]; }
+        } else if (k == ASD2_BC_NEUMANN || k == ASD2_BC_AXISYM || k == ASD2_BC_OUTLET) {
+            PFOR(g * (ny - 1)) {
+                const int i = q / (ny - 1), j = g + 1 + q % (ny - 1);
+                a[G.idx(nx + 2 * g - i - 1, j)] = a[G.idx(nx + i, j)];
+            }
+        } else if (k == ASD2_BC_WALL || k == ASD2_BC_DIRICHLET || k == ASD2_BC_INLET) {
+            PFOR(g * (ny - 1)) {
+                const int i = q / (ny - 1), j = g + 1 + q % (ny - 1);
+                a[G.idx(nx + 2 * g - i - 1, j)] = -a[G.idx(nx + i, j)] + 2.0 * c;
+            }
+        }
+    } else if (side == 2) {
+        if (k == ASD2_BC_PERIODIC) {
+            PFOR(nx * (g + 1)) { const int i = g + q / (g + 1), j = q % (g + 1); a[G.idx(i, j)] = a[G.idx(i, ny - 1 + j)]; }
+        } else if (k == ASD2_BC_NEUMANN || k == ASD2_BC_OUTLET) {
+            PFOR(W) a[G.idx(q, g)] = a[G.idx(q, g + 1)];
+            PFOR(nx * g) { const int i = g + q / g, j = q % g; a[G.idx(i, j)] = a[G.idx(i, 2 * g - j)]; }
+        } else if (k == ASD2_BC_AXISYM || k == ASD2_BC_WALL || k == ASD2_BC_DIRICHLET || k == ASD2_BC_INLET) {
+            PFOR(W) a[G.idx(q, g)] = c;
+            PFOR(nx * g) { const int i = g + q / g, j = q % g; a[G.idx(i, j)] = -a[G.idx(i, 2 * g - j)] + 2.0 * c; }
+        }
+    } else {
+        if (k == ASD2_BC_PERIODIC) {
+            PFOR(nx * g) { const int i = g + q / g, j = q % g; a[G.idx(i, ny + j + g)] = a[G.idx(i, j + g)]; }
+        } else if (k == ASD2_BC_NEUMANN || k == ASD2_BC_OUTLET) {
+            PFOR(W) a[G.idx(q, ny + g)] = a[G.idx(q, ny + g - 1)];
+            PFOR(nx * (g - 1)) {
+                const int i = g + q / (g - 1), j = 1 + q % (g - 1);
+                a[G.idx(i, ny + 2 * g - j)] = a[G.idx(i, ny + j)];
+            }
+        } else if (k == ASD2_BC_AXISYM || k == ASD2_BC_WALL || k == ASD2_BC_DIRICHLET || k == ASD2_BC_INLET) {
+            PFOR(W) a[G.idx(q, ny + g)] = c;
+            PFOR(nx * (g - 1)) {
+                const int i = g + q / (g - 1), j = 1 + q % (g - 1);
+                a[G.idx(i, ny + 2 * g - j)] = -a[G.idx(i, ny + j)] + 2.0 * c;
+            }
+        }
+    }
+}
+#undef PFOR
+
+__global__ __launch_bounds__(1024) void k_bc_uv(Geom G, BcArgs A, int is_v, double* __restrict__ a) {
+    for (int side = 0; side < 4; side++) {
+        if (is_v) v_side(G, side, A.kind[side], A.c[side], a);
+        else u_side(G, side, A.kind[side], A.c[side], a);
+        __syncthreads();
+    }
+}
+
+int bc_apply_uv(asd2_ctx* ctx, int grid_kind, double* d) {
+    BcArgs A;
+    for (int s = 0; s < 4; s++) { A.kind[s] = ctx->bc[grid_kind][s]; A.c[s] = ctx->bcc[grid_kind][s]; }
+    hipLaunchKernelGGL(k_bc_uv, dim3(1), dim3(1024), 0, ctx->stream, ctx->geo, A,
+                       grid_kind == ASD2_GRID_V ? 1 : 0, d);
+    ASD2_LAUNCH_CHECK();
+    return ASD2_OK;
+}
+
+}  // namespace asd2
+
+using namespace asd2;
+
+extern "C" int asd2_bc_apply(asd2_ctx* ctx, int grid_kind, double* d_arr) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!d_arr) { set_error("null array"); return ASD2_ERR_ARG; }
+    if (grid_kind == ASD2_GRID_P) return bc_apply_p(ctx, d_arr);
+    if (grid_kind == ASD2_GRID_U || grid_kind == ASD2_GRID_V) return bc_apply_uv(ctx, grid_kind, d_arr);
+    set_error("bad grid kind %d", grid_kind);
+    return ASD2_ERR_ARG;
+}

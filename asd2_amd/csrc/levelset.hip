@@ -1,0 +1,586 @@
+// levelset.hip -- HJ-WENO5 + SSP-RK3 advection and reinitialisation of the level set on
+// gfx950.  Reference: src/levelset2d.cpp.
+//
+// Kernels keep the reference's FP64 arithmetic.  Branch-deciding quantities (MinMod
+// choices, the subcell interface position and its cutoffs, the sign functions) are
+// evaluated in the reference's operation order with contraction off, so every branch
+// agrees with the reference; the bulk WENO/ENO algebra is reorganised (one division per
+// WENO side, reciprocal multiplies) and agrees to rounding.
+#include "internal.h"
+#include "weno.h"
+
+namespace asd2 {
+
+// ------------------------------------------------------------------ WENO5
+constexpr int WJ = 64;   // tile width along j (contiguous)
+constexpr int WI = 16;   // tile height along i
+constexpr int WTH = 256; // 64 x 4 threads
+
+// cell-centred velocity of Solve_LevelSet_2D with the P BC on its ghosts
+// (src/levelset2d.cpp:47-57); CUSTOM sides leave the ghost at 0.
+__device__ __forceinline__ double uc_face(const Geom& G, const GhostRule& R, const double* u, int i, int j) {
+    const int g = G.g;
+    if (i >= g && i < G.nx + g) return (u[G.idx(i, j)] + u[G.idx(i + 1, j)]) * 0.5;
+    const int s = (i < g) ? 0 : 1;
+    const int m = (i < g) ? (2 * g - 1 - i) : (2 * G.nx + 2 * g - 1 - i);
+    if (R.kind[s] == 2) return 0.0;
+    const double x = (u[G.idx(m, j)] + u[G.idx(m + 1, j)]) * 0.5;
+    return R.kind[s] == 0 ? x : (-x + R.two_c[s]);
+}
+
+__device__ __forceinline__ double vc_face(const Geom& G, const GhostRule& R, const double* v, int i, int j) {
+    const int g = G.g;
+    if (j >= g && j < G.ny + g) return (v[G.idx(i, j)] + v[G.idx(i, j + 1)]) * 0.5;
+    const int s = (j < g) ? 2 : 3;
+    const int m = (j < g) ? (2 * g - 1 - j) : (2 * G.ny + 2 * g - 1 - j);
+    if (R.kind[s] == 2) return 0.0;
+    const double x = (v[G.idx(i, m)] + v[G.idx(i, m + 1)]) * 0.5;
+    return R.kind[s] == 0 ? x : (-x + R.two_c[s]);
+}
+
+// STAGE 0: write L (HJWENO5_LS_2D); 1..3: SSP-RK3 stages of Solve_LevelSet_2D.
+// CENTRED: velocities given cell-centred (with ghosts) instead of on faces.
+template <int STAGE, bool CENTRED>
+__global__ __launch_bounds__(WTH) void k_weno(Geom G, GhostRule R, const double* __restrict__ phi,
+                                              const double* phin, const double* __restrict__ u,
+                                              const double* __restrict__ v, double* out, double dt,
+                                              int ghosts) {
+    __shared__ double sFx[WI + 6][WJ];
+    __shared__ double sFy[WI][WJ + 6];
+    const int tj = threadIdx.x & (WJ - 1);
+    const int ti = threadIdx.x / WJ;
+    const int g = G.g;
+    const int j0 = g + blockIdx.x * WJ;
+    const int i0 = g + blockIdx.y * WI;
+    const int iend = G.nx + g, jend = G.ny + g;
+    const int ipad = G.nx + 2 * g, jpad = G.ny + 2 * g;
+
+    for (int r = ti; r < WI + 6; r += WTH / WJ) {
+        const int i = i0 - 3 + r, j = j0 + tj;
+        double f = 0.0;
+        if (j < jend && i < ipad) {
+            const double vel = CENTRED ? u[G.idx(i, j)] : uc_face(G, R, u, i, j);
+            f = phi[G.idx(i, j)] * vel;
+        }
+        sFx[r][tj] = f;
+    }
+    for (int e = threadIdx.x; e < WI * (WJ + 6); e += WTH) {
+        const int r = e / (WJ + 6), c = e - r * (WJ + 6);
+        const int i = i0 + r, j = j0 - 3 + c;
+        double f = 0.0;
+        if (i < iend && j < jpad) {
+            const double vel = CENTRED ? v[G.idx(i, j)] : vc_face(G, R, v, i, j);
+            f = phi[G.idx(i, j)] * vel;
+        }
+        sFy[r][c] = f;
+    }
+    __syncthreads();
+
+    const double idx_ = 1.0 / G.dx, idy_ = 1.0 / G.dy;
+    const double epsx = 1e-6 * G.dx * G.dx, epsy = 1e-6 * G.dy * G.dy;
+    for (int r = ti; r < WI; r += WTH / WJ) {
+        const int i = i0 + r, j = j0 + tj;
+        if (i >= iend || j >= jend) continue;
+        double lxp, lxm, lyp, lym;
+        {
+            const double f0 = sFx[r][tj], f1 = sFx[r + 1][tj], f2 = sFx[r + 2][tj], f3 = sFx[r + 3][tj];
+            const double f4 = sFx[r + 4][tj], f5 = sFx[r + 5][tj], f6 = sFx[r + 6][tj];
+            weno5(f1 - f0, f2 - f1, f3 - f2, f4 - f3, f5 - f4, f6 - f5, idx_, epsx, lxp, lxm);
+        }
+        {
+            const double f0 = sFy[r][tj], f1 = sFy[r][tj + 1], f2 = sFy[r][tj + 2], f3 = sFy[r][tj + 3];
+            const double f4 = sFy[r][tj + 4], f5 = sFy[r][tj + 5], f6 = sFy[r][tj + 6];
+            weno5(f1 - f0, f2 - f1, f3 - f2, f4 - f3, f5 - f4, f6 - f5, idy_, epsy, lyp, lym);
+        }
+        const int64_t c = G.idx(i, j);
+        double ax, ay;
+        if (CENTRED) { ax = u[c]; ay = v[c]; }
+        else {
+            ax = (u[c] + u[G.idx(i + 1, j)]) * 0.5;
+            ay = (v[c] + v[G.idx(i, j + 1)]) * 0.5;
+        }
+        const double L = 0.5 * (lxp + lxm) + 0.5 * (lyp + lym) - ax * (0.5 * (lxp - lxm)) - ay * (0.5 * (lyp - lym));
+        double val;
+        if (STAGE == 0) val = L;
+        else if (STAGE == 1) val = phi[c] - dt * L;
+        else if (STAGE == 2) val = 0.75 * phin[c] + 0.25 * phi[c] - 0.25 * dt * L;
+        else val = 1.0 / 3.0 * phin[c] + 2.0 / 3.0 * phi[c] - 2.0 / 3.0 * dt * L;
+        out[c] = val;
+        if (ghosts) write_p_ghosts(G, R, out, i, j, val);
+    }
+}
+
+// interior cell-centred velocities of Solve_LevelSet_2D (src/levelset2d.cpp:50-54); used
+// when a side's ghost rule cannot be evaluated inside the stage kernel (LS-free sides)
+__global__ __launch_bounds__(256) void k_centre_velocity(Geom G, const double* __restrict__ u,
+                                                         const double* __restrict__ v, double* __restrict__ uc,
+                                                         double* __restrict__ vc) {
+    const int j = G.g + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = G.g + blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= G.nx + G.g || j >= G.ny + G.g) return;
+    const int64_t c = G.idx(i, j);
+    uc[c] = (u[c] + u[G.idx(i + 1, j)]) * 0.5;
+    vc[c] = (v[c] + v[G.idx(i, j + 1)]) * 0.5;
+}
+
+// ------------------------------------------------------------------ ENO / subcell
+constexpr int RJ = 64, RI = 16, RH = 2;  // reinit tile and halo
+constexpr int RW = RJ + 2 * RH;
+
+__device__ __forceinline__ void load_tile2(const Geom& G, const double* __restrict__ src,
+                                           double (*s)[RW], int i0, int j0) {
+    const int ipad = G.nx + 2 * G.g, jpad = G.ny + 2 * G.g;
+    for (int e = threadIdx.x; e < (RI + 2 * RH) * RW; e += WTH) {
+        const int r = e / RW, c = e - r * RW;
+        const int i = i0 - RH + r, j = j0 - RH + c;
+        s[r][c] = (i < ipad && j < jpad) ? src[G.idx(i, j)] : 0.0;
+    }
+}
+
+// 2nd-order ENO one-sided differences with MinMod (src/levelset2d.cpp:573-593):
+// the MinMod picks on undivided second differences (same choice as on divided ones)
+// and the result is scaled once.  x0..x4 = phi[k-2..k+2] along one axis.
+__device__ __forceinline__ void eno2(double xm2, double xm1, double x0, double xp1, double xp2,
+                                     double inv_h, double& dp, double& dm) {
+    const double dd0 = xm1 - 2.0 * x0 + xp1;
+    const double ddp = x0 - 2.0 * xp1 + xp2;
+    const double ddm = xm2 - 2.0 * xm1 + x0;
+    dp = inv_h * ((xp1 - x0) - 0.5 * minmod(dd0, ddp));
+    dm = inv_h * ((x0 - xm1) + 0.5 * minmod(dd0, ddm));
+}
+
+// Min-Gibou subcell interface distance and one-sided derivative
+// (src/levelset2d.cpp:655-707), reference operation order, no contraction.
+// I* = phi0 along the axis (k-2..k+2), l0 = current phi at k; plus = derivative towards k+1.
+__device__ __noinline__ double subcell_plus(double Im1, double I0, double Ip1, double Ip2, double l0, double h) {
+#pragma clang fp contract(off)
+    const double DD = minmod((Im1 - 2.0 * I0 + Ip1) / (h * h), (I0 - 2.0 * Ip1 + Ip2) / (h * h));
+    const double c2 = 0.5 * DD;
+    const double c1 = (Ip1 - I0) / h;
+    const double c0 = (Ip1 + I0) * 0.5 - c2 * h * h * 0.25;
+    double nd;
+    if (fabs(c2) > 1.0e-10) nd = 0.5 * h + (-c1 - isign(I0) * sqrt(c1 * c1 - 4.0 * c2 * c0)) / (2.0 * c2);
+    else nd = 0.5 * h - c0 / c1;
+    double d = -l0 / nd - 0.5 * nd * DD;
+    if (fabs(nd) < h * 0.01) d = -0.5 * nd * DD;
+    return d;
+}
+
+__device__ __noinline__ double subcell_minus(double Im2, double Im1, double I0, double Ip1, double l0, double h) {
+#pragma clang fp contract(off)
+    const double DD = minmod((Im1 - 2.0 * I0 + Ip1) / (h * h), (Im2 - 2.0 * Im1 + I0) / (h * h));
+    const double c2 = 0.5 * DD;
+    const double c1 = (I0 - Im1) / h;
+    const double c0 = (I0 + Im1) * 0.5 - c2 * h * h * 0.25;
+    double nd;
+    if (fabs(c2) > 1.0e-10) nd = 0.5 * h - (-c1 + isign(I0) * sqrt(c1 * c1 - 4.0 * c2 * c0)) / (2.0 * c2);
+    else nd = 0.5 * h + c0 / c1;
+    double d = l0 / nd + 0.5 * nd * DD;
+    if (fabs(nd) < h * 0.01) d = 0.5 * nd * DD;
+    return d;
+}
+
+// Godunov |grad phi| (src/levelset2d.cpp:768-782)
+__device__ __forceinline__ double godunov(double s0, double dPX, double dMX, double dPY, double dMY) {
+    double a, b;
+    if (s0 >= 0) {
+        const double px = smin(dPX, 0.0), mx = smax(dMX, 0.0), py = smin(dPY, 0.0), my = smax(dMY, 0.0);
+        a = smax(px * px, mx * mx);
+        b = smax(py * py, my * my);
+    } else {
+        const double px = smax(dPX, 0.0), mx = smin(dMX, 0.0), py = smax(dPY, 0.0), my = smin(dMY, 0.0);
+        a = smax(px * px, mx * mx);
+        b = smax(py * py, my * my);
+    }
+    return sqrt(a + b);
+}
+
+// |grad phi| at tile position (r, c) (offset by the halo) from tiles of phi and phi0.
+template <bool SUBCELL>
+__device__ __forceinline__ double absgrad_at(const double (*sL)[RW], const double (*sI)[RW], int r, int c,
+                                             double dx, double dy, double idx_, double idy_) {
+    const int R = r + RH, C = c + RH;
+    double dPX, dMX, dPY, dMY;
+    const double l0 = sL[R][C];
+    eno2(sL[R - 2][C], sL[R - 1][C], l0, sL[R + 1][C], sL[R + 2][C], idx_, dPX, dMX);
+    eno2(sL[R][C - 2], sL[R][C - 1], l0, sL[R][C + 1], sL[R][C + 2], idy_, dPY, dMY);
+    const double I0 = sI[R][C];
+    if (SUBCELL) {
+        const double Ixm2 = sI[R - 2][C], Ixm1 = sI[R - 1][C], Ixp1 = sI[R + 1][C], Ixp2 = sI[R + 2][C];
+        const double Iym2 = sI[R][C - 2], Iym1 = sI[R][C - 1], Iyp1 = sI[R][C + 1], Iyp2 = sI[R][C + 2];
+        if (I0 * Ixp1 < 0) dPX = subcell_plus(Ixm1, I0, Ixp1, Ixp2, l0, dx);
+        if (I0 * Ixm1 < 0) dMX = subcell_minus(Ixm2, Ixm1, I0, Ixp1, l0, dx);
+        if (I0 * Iyp1 < 0) dPY = subcell_plus(Iym1, I0, Iyp1, Iyp2, l0, dy);
+        if (I0 * Iym1 < 0) dMY = subcell_minus(Iym2, Iym1, I0, Iyp1, l0, dy);
+    }
+    return godunov(I0, dPX, dMX, dPY, dMY);
+}
+
+// smoothed sign of phi0 (src/levelset2d.cpp:914-923), reference operation order
+__device__ __forceinline__ double smoothed_sign(double x, double eps) {
+#pragma clang fp contract(off)
+    return x / sqrt(x * x + eps * eps);
+}
+
+// mode 0: out = |grad phi| (subcell) ; 1: out = |grad phi| (ENO)
+template <bool SUBCELL>
+__global__ __launch_bounds__(WTH) void k_absgrad(Geom G, const double* __restrict__ ls,
+                                                 const double* __restrict__ ls0, double* __restrict__ out) {
+    __shared__ double sL[RI + 2 * RH][RW];
+    __shared__ double sI[RI + 2 * RH][RW];
+    const int j0 = G.g + blockIdx.x * RJ, i0 = G.g + blockIdx.y * RI;
+    load_tile2(G, ls, sL, i0, j0);
+    load_tile2(G, ls0, sI, i0, j0);
+    __syncthreads();
+    const int tj = threadIdx.x & (RJ - 1), ti = threadIdx.x / RJ;
+    const double idx_ = 1.0 / G.dx, idy_ = 1.0 / G.dy;
+    for (int r = ti; r < RI; r += WTH / RJ) {
+        const int i = i0 + r, j = j0 + tj;
+        if (i >= G.nx + G.g || j >= G.ny + G.g) continue;
+        out[G.idx(i, j)] = absgrad_at<SUBCELL>(sL, sI, r, tj, G.dx, G.dy, idx_, idy_);
+    }
+}
+
+// MinRK2 substage A: LS1 = phi - dtau S(phi0) (|grad phi|_sub - 1)   (levelset2d.cpp:453-460)
+// MinRK2 substage B: LS2 = LS1 - dtau S (|grad LS1|_sub - 1); phi = (phi + LS2)/2  (:462-481)
+// Sussman (SUBCELL = false, sign(phi0)): A writes LS1 without ghosts (:344-348),
+// B writes tmp = (phi + LS2)/2 with ghosts (:351-363).
+template <bool SUBCELL, int SUB>
+__global__ __launch_bounds__(WTH) void k_reinit(Geom G, GhostRule R, const double* __restrict__ src,
+                                                const double* __restrict__ ls0, const double* phi_c,
+                                                double* out, int ghosts) {
+    __shared__ double sL[RI + 2 * RH][RW];
+    __shared__ double sI[RI + 2 * RH][RW];
+    const int j0 = G.g + blockIdx.x * RJ, i0 = G.g + blockIdx.y * RI;
+    load_tile2(G, src, sL, i0, j0);
+    load_tile2(G, ls0, sI, i0, j0);
+    __syncthreads();
+    const int tj = threadIdx.x & (RJ - 1), ti = threadIdx.x / RJ;
+    const double idx_ = 1.0 / G.dx, idy_ = 1.0 / G.dy;
+    const double eps = smin(G.dx, G.dy), adt = eps * 0.5;
+    for (int r = ti; r < RI; r += WTH / RJ) {
+        const int i = i0 + r, j = j0 + tj;
+        if (i >= G.nx + G.g || j >= G.ny + G.g) continue;
+        const double a = absgrad_at<SUBCELL>(sL, sI, r, tj, G.dx, G.dy, idx_, idy_);
+        const double I0 = sI[r + RH][tj + RH];
+        double val;
+        {
+#pragma clang fp contract(off)
+            const double S = SUBCELL ? smoothed_sign(I0, eps) : static_cast<double>(isign(I0));
+            const double x = sL[r + RH][tj + RH] - adt * S * (a - 1.0);
+            if (SUB == 0) val = x;
+            else val = 0.5 * (phi_c[G.idx(i, j)] + x);
+        }
+        out[G.idx(i, j)] = val;
+        if (ghosts) write_p_ghosts(G, R, out, i, j, val);
+    }
+}
+
+// Sussman's volume constraint (src/levelset2d.cpp:27-41 mask, :787-893 correction),
+// reference operation order; phi += correction, ghosts rewritten.
+constexpr int SJ = 64, SI = 16, SHL = 1, SW = SJ + 2 * SHL;
+
+__device__ __noinline__ double sussman_correction(const double (*T)[SW], const double (*I)[SW], int R, int C,
+                                                  double dx, double dy) {
+#pragma clang fp contract(off)
+    const double eps = smin(dx, dy), thick = 1.5 * eps, adt = eps * 0.5;
+    double F[3][3], IF[3][3], dA[3][3], Hp[3][3], Lm[3][3], f[3][3];
+#define L_(a, b) T[R + (a)][C + (b)]
+#define I_(a, b) I[R + (a)][C + (b)]
+    F[0][0] = 0.25 * (L_(0, 0) + L_(-1, 0) + L_(-1, -1) + L_(0, -1));
+    F[0][1] = 0.5 * (L_(0, -1) + L_(0, 0));
+    F[0][2] = 0.25 * (L_(0, 0) + L_(-1, 0) + L_(-1, 1) + L_(0, 1));
+    F[1][0] = 0.5 * (L_(0, -1) + L_(0, 0));
+    F[1][1] = L_(0, 0);
+    F[1][2] = 0.5 * (L_(0, 0) + L_(0, 1));
+    F[2][0] = 0.25 * (L_(0, 0) + L_(1, 0) + L_(1, -1) + L_(0, -1));
+    F[2][1] = 0.5 * (L_(0, 0) + L_(1, 0));
+    F[2][2] = 0.25 * (L_(0, 0) + L_(1, 0) + L_(1, 1) + L_(0, 1));
+    IF[0][0] = 0.25 * (I_(0, 0) + I_(-1, 0) + I_(-1, -1) + I_(0, -1));
+    IF[0][1] = 0.5 * (I_(0, -1) + I_(0, 0));
+    IF[0][2] = 0.25 * (I_(0, 0) + I_(-1, 0) + I_(-1, 1) + I_(0, 1));
+    IF[1][0] = 0.5 * (I_(0, -1) + I_(0, 0));
+    IF[1][1] = I_(0, 0);
+    IF[1][2] = 0.5 * (I_(0, 0) + I_(0, 1));
+    IF[2][0] = 0.25 * (I_(0, 0) + I_(1, 0) + I_(1, -1) + I_(0, -1));
+    IF[2][1] = 0.5 * (I_(0, 0) + I_(1, 0));
+    IF[2][2] = 0.25 * (I_(0, 0) + I_(1, 0) + I_(1, 1) + I_(0, 1));
+    auto sq = [](double x) { return x * x; };
+    dA[0][0] = sqrt(sq(((I_(0, 0) + I_(0, -1)) * 0.5 - 0.5 * (I_(-1, 0) + I_(-1, -1))) / dx)
+                  + sq((IF[0][1] - 0.5 * (I_(0, -1) + I_(-1, -1))) / dy));
+    dA[0][1] = sqrt(sq((IF[1][1] - I_(-1, 0)) / dx) + sq((IF[2][0] - IF[0][0]) / dy));
+    dA[0][2] = sqrt(sq((IF[1][2] - -0.5 * (I_(-1, 0) + I_(-1, 1))) / dx)
+                  + sq((0.5 * (I_(-1, 1) + I_(0, 1)) - IF[0][1]) / dy));
+    dA[1][0] = sqrt(sq((IF[2][0] - IF[0][0]) / dx) + sq((IF[1][1] - 0.5 * (I_(0, -1) + I_(0, 0))) / dy));
+    dA[1][1] = sqrt(sq((IF[2][1] - IF[0][1]) / dx) + sq((IF[1][2] - IF[1][0]) / dy));
+    dA[1][2] = sqrt(sq((IF[2][2] - IF[0][2]) / dx) + sq((I_(0, 1) - IF[1][1]) / dy));
+    dA[2][0] = sqrt(sq((0.5 * (I_(1, 0) + I_(1, -1)) - IF[1][0]) / dx)
+                  + sq((IF[2][1] - 0.5 * (I_(0, -1) + I_(1, -1))) / dy));
+    dA[2][1] = sqrt(sq((I_(1, 0) - IF[1][1]) / (2.0 * dx)) + sq((IF[2][2] - IF[2][0]) / dy));
+    dA[2][2] = sqrt(sq((0.5 * (I_(1, 0) + I_(1, 1)) - IF[1][1]) / dx)
+                  + sq((0.5 * (I_(0, 1) + I_(1, 1)) - IF[2][1]) / dy));
+#undef L_
+#undef I_
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            if (fabs(F[a][b]) > thick) Hp[a][b] = 0.0;
+            else Hp[a][b] = 0.5 / eps + 0.5 / eps * cos(M_PI * F[a][b] / eps);
+            Lm[a][b] = (F[a][b] - IF[a][b]) / adt;
+            f[a][b] = Hp[a][b] * dA[a][b];
+        }
+    double s1 = 0.0, s2 = 0.0;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            s1 += Hp[a][b] * Lm[a][b];
+            s2 += Hp[a][b] * f[a][b];
+        }
+    s1 += 15.0 * Hp[1][1] * Lm[1][1];
+    s2 += 15.0 * Hp[1][1] * f[1][1];
+    return 0.0 - adt * f[1][1] * s1 / s2;
+}
+
+__global__ __launch_bounds__(WTH) void k_sussman_constraint(Geom G, GhostRule R, const double* __restrict__ tmp,
+                                                            const double* __restrict__ ls0, double* ls, int ghosts) {
+    __shared__ double sT[SI + 2 * SHL][SW];
+    __shared__ double sI[SI + 2 * SHL][SW];
+    const int j0 = G.g + blockIdx.x * SJ, i0 = G.g + blockIdx.y * SI;
+    const int ipad = G.nx + 2 * G.g, jpad = G.ny + 2 * G.g;
+    for (int e = threadIdx.x; e < (SI + 2 * SHL) * SW; e += WTH) {
+        const int r = e / SW, c = e - r * SW;
+        const int i = i0 - SHL + r, j = j0 - SHL + c;
+        const bool in = (i < ipad && j < jpad);
+        sT[r][c] = in ? tmp[G.idx(i, j)] : 0.0;
+        sI[r][c] = in ? ls0[G.idx(i, j)] : 0.0;
+    }
+    __syncthreads();
+    const int tj = threadIdx.x & (SJ - 1), ti = threadIdx.x / SJ;
+    const double eps = smin(G.dx, G.dy);
+    for (int r = ti; r < SI; r += WTH / SJ) {
+        const int i = i0 + r, j = j0 + tj;
+        if (i >= G.nx + G.g || j >= G.ny + G.g) continue;
+        const double t = sT[r + SHL][tj + SHL];
+        double cons = 0.0;
+        if (!(fabs(t) > eps)) {
+            double hd;
+            {
+#pragma clang fp contract(off)
+                hd = 0.5 * (1.0 / eps + cos(M_PI * t / eps));
+            }
+            if (hd != 0.0) cons = sussman_correction(sT, sI, r + SHL, tj + SHL, G.dx, G.dy);
+        }
+        const int64_t c = G.idx(i, j);
+        const double val = ls[c] + cons;
+        ls[c] = val;
+        if (ghosts) write_p_ghosts(G, R, ls, i, j, val);
+    }
+}
+
+struct Launch {
+    dim3 grid, block;
+};
+
+static Launch tiles(const Geom& G, int tj, int ti) {
+    return Launch{dim3(cdiv(G.ny, tj), cdiv(G.nx, ti)), dim3(WTH)};
+}
+
+}  // namespace asd2
+
+using namespace asd2;
+
+namespace {
+
+// after a kernel that could not write the ghosts itself (LS-free sides)
+int finish_ghosts(asd2_ctx* ctx, bool fused, double* d) {
+    if (fused) return ASD2_OK;
+    return bc_apply_p(ctx, d);
+}
+
+int p_rule(asd2_ctx* ctx, GhostRule* R, bool* fused) {
+    for (int s = 0; s < 4; s++) {
+        if (ctx->bc[ASD2_GRID_P][s] == ASD2_BC_PERIODIC) {
+            set_error("periodic P boundaries are not supported on the GPU level-set path");
+            return ASD2_ERR_UNSUPPORTED;
+        }
+    }
+    *R = ghost_rule_p(ctx, fused);
+    return ASD2_OK;
+}
+
+bool has_custom(const asd2_ctx* ctx) {
+    for (int s = 0; s < 4; s++)
+        if (ctx->bc[ASD2_GRID_P][s] == ASD2_BC_CUSTOM) return true;
+    return false;
+}
+
+}  // namespace
+
+extern "C" {
+
+int asd2_ls_advect(asd2_ctx* ctx, double* d_ls, const double* d_u, const double* d_v, double dt) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!d_ls || !d_u || !d_v) { set_error("null array"); return ASD2_ERR_ARG; }
+    GhostRule R;
+    bool fused;
+    ASD2_TRY(p_rule(ctx, &R, &fused));
+    const Geom& G = ctx->geo;
+    const size_t n = static_cast<size_t>(G.size());
+    double *p1, *p2;
+    ASD2_TRY(ws(ctx, "ls_stage1", n, &p1));
+    ASD2_TRY(ws(ctx, "ls_stage2", n, &p2));
+    if (has_custom(ctx)) {  // fresh stage arrays keep zero ghosts on CUSTOM sides
+        ASD2_TRY(zero_async(ctx, p1, n * 8));
+        ASD2_TRY(zero_async(ctx, p2, n * 8));
+    }
+    ASD2_TRY(bc_apply_p(ctx, d_ls));
+    const Launch L = tiles(G, WJ, WI);
+    const int gh = fused ? 1 : 0;
+    if (!fused) {
+        // LS-free sides: build tmpU/tmpV with the full P BC, then run the centred variant
+        double *uc, *vc;
+        ASD2_TRY(ws(ctx, "ls_uc", n, &uc));
+        ASD2_TRY(ws(ctx, "ls_vc", n, &vc));
+        ASD2_TRY(zero_async(ctx, uc, n * 8));
+        ASD2_TRY(zero_async(ctx, vc, n * 8));
+        hipLaunchKernelGGL(k_centre_velocity, dim3(cdiv(G.ny, 64), cdiv(G.nx, 4)), dim3(256), 0, ctx->stream, G,
+                           d_u, d_v, uc, vc);
+        ASD2_LAUNCH_CHECK();
+        ASD2_TRY(bc_apply_p(ctx, uc));
+        ASD2_TRY(bc_apply_p(ctx, vc));
+        hipLaunchKernelGGL((k_weno<1, true>), L.grid, L.block, 0, ctx->stream, G, R, d_ls, d_ls, uc, vc, p1, dt, 0);
+        ASD2_LAUNCH_CHECK();
+        ASD2_TRY(bc_apply_p(ctx, p1));
+        hipLaunchKernelGGL((k_weno<2, true>), L.grid, L.block, 0, ctx->stream, G, R, p1, d_ls, uc, vc, p2, dt, 0);
+        ASD2_LAUNCH_CHECK();
+        ASD2_TRY(bc_apply_p(ctx, p2));
+        hipLaunchKernelGGL((k_weno<3, true>), L.grid, L.block, 0, ctx->stream, G, R, p2, d_ls, uc, vc, d_ls, dt, 0);
+        ASD2_LAUNCH_CHECK();
+        return bc_apply_p(ctx, d_ls);
+    }
+    prof_begin(ctx, ASD2_KID_WENO_STAGE);
+    hipLaunchKernelGGL((k_weno<1, false>), L.grid, L.block, 0, ctx->stream, G, R, d_ls, d_ls, d_u, d_v, p1, dt, gh);
+    prof_end(ctx, ASD2_KID_WENO_STAGE);
+    ASD2_LAUNCH_CHECK();
+    ASD2_TRY(finish_ghosts(ctx, fused, p1));
+    prof_begin(ctx, ASD2_KID_WENO_STAGE);
+    hipLaunchKernelGGL((k_weno<2, false>), L.grid, L.block, 0, ctx->stream, G, R, p1, d_ls, d_u, d_v, p2, dt, gh);
+    prof_end(ctx, ASD2_KID_WENO_STAGE);
+    ASD2_LAUNCH_CHECK();
+    ASD2_TRY(finish_ghosts(ctx, fused, p2));
+    prof_begin(ctx, ASD2_KID_WENO_STAGE);
+    hipLaunchKernelGGL((k_weno<3, false>), L.grid, L.block, 0, ctx->stream, G, R, p2, d_ls, d_u, d_v, d_ls, dt, gh);
+    prof_end(ctx, ASD2_KID_WENO_STAGE);
+    ASD2_LAUNCH_CHECK();
+    return finish_ghosts(ctx, fused, d_ls);
+}
+
+int asd2_ls_weno_rhs(asd2_ctx* ctx, double* d_ls, const double* d_uc, const double* d_vc, double* d_L) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!d_ls || !d_uc || !d_vc || !d_L) { set_error("null array"); return ASD2_ERR_ARG; }
+    GhostRule R;
+    bool fused;
+    ASD2_TRY(p_rule(ctx, &R, &fused));
+    const Geom& G = ctx->geo;
+    ASD2_TRY(bc_apply_p(ctx, d_ls));
+    ASD2_TRY(zero_async(ctx, d_L, static_cast<size_t>(G.size()) * 8));
+    const Launch L = tiles(G, WJ, WI);
+    hipLaunchKernelGGL((k_weno<0, true>), L.grid, L.block, 0, ctx->stream, G, R, d_ls, d_ls, d_uc, d_vc, d_L, 0.0, 0);
+    ASD2_LAUNCH_CHECK();
+    return ASD2_OK;
+}
+
+int asd2_ls_subcell_absgrad(asd2_ctx* ctx, const double* d_ls, const double* d_ls0, double* d_out) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!d_ls || !d_ls0 || !d_out) { set_error("null array"); return ASD2_ERR_ARG; }
+    const Geom& G = ctx->geo;
+    ASD2_TRY(zero_async(ctx, d_out, static_cast<size_t>(G.size()) * 8));
+    const Launch L = tiles(G, RJ, RI);
+    hipLaunchKernelGGL(k_absgrad<true>, L.grid, L.block, 0, ctx->stream, G, d_ls, d_ls0, d_out);
+    ASD2_LAUNCH_CHECK();
+    return ASD2_OK;
+}
+
+int asd2_ls_eno_absgrad(asd2_ctx* ctx, const double* d_ls, const double* d_ls0, double* d_out) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!d_ls || !d_ls0 || !d_out) { set_error("null array"); return ASD2_ERR_ARG; }
+    const Geom& G = ctx->geo;
+    ASD2_TRY(zero_async(ctx, d_out, static_cast<size_t>(G.size()) * 8));
+    const Launch L = tiles(G, RJ, RI);
+    hipLaunchKernelGGL(k_absgrad<false>, L.grid, L.block, 0, ctx->stream, G, d_ls, d_ls0, d_out);
+    ASD2_LAUNCH_CHECK();
+    return ASD2_OK;
+}
+
+int asd2_ls_reinit_iterations(double eps, double max_time) {
+    const double adt = eps * 0.5;
+    if (!(adt > 0)) return 0;
+    double t = 0.0;
+    int k = 0;
+    while (t < max_time) { t += adt; k++; }
+    return k;
+}
+
+int asd2_ls_reinit_minrk2(asd2_ctx* ctx, double* d_ls, int iterations) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!d_ls || iterations < 0) { set_error("bad argument"); return ASD2_ERR_ARG; }
+    GhostRule R;
+    bool fused;
+    ASD2_TRY(p_rule(ctx, &R, &fused));
+    const Geom& G = ctx->geo;
+    const size_t n = static_cast<size_t>(G.size());
+    double *ls0, *ls1;
+    ASD2_TRY(ws(ctx, "ri_ls0", n, &ls0));
+    ASD2_TRY(ws(ctx, "ri_ls1", n, &ls1));
+    if (has_custom(ctx)) ASD2_TRY(zero_async(ctx, ls1, n * 8));
+    ASD2_TRY(bc_apply_p(ctx, d_ls));
+    ASD2_HIP(hipMemcpyAsync(ls0, d_ls, n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    const Launch L = tiles(G, RJ, RI);
+    const int gh = fused ? 1 : 0;
+    for (int it = 0; it < iterations; it++) {
+        prof_begin(ctx, ASD2_KID_REINIT);
+        hipLaunchKernelGGL((k_reinit<true, 0>), L.grid, L.block, 0, ctx->stream, G, R, d_ls, ls0, d_ls, ls1, gh);
+        prof_end(ctx, ASD2_KID_REINIT);
+        ASD2_LAUNCH_CHECK();
+        ASD2_TRY(finish_ghosts(ctx, fused, ls1));
+        prof_begin(ctx, ASD2_KID_REINIT);
+        hipLaunchKernelGGL((k_reinit<true, 1>), L.grid, L.block, 0, ctx->stream, G, R, ls1, ls0, d_ls, d_ls, gh);
+        prof_end(ctx, ASD2_KID_REINIT);
+        ASD2_LAUNCH_CHECK();
+        ASD2_TRY(finish_ghosts(ctx, fused, d_ls));
+    }
+    return ASD2_OK;
+}
+
+int asd2_ls_reinit_sussman(asd2_ctx* ctx, double* d_ls, int iterations) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!d_ls || iterations < 0) { set_error("bad argument"); return ASD2_ERR_ARG; }
+    GhostRule R;
+    bool fused;
+    ASD2_TRY(p_rule(ctx, &R, &fused));
+    const Geom& G = ctx->geo;
+    const size_t n = static_cast<size_t>(G.size());
+    double *ls0, *ls1, *tmp;
+    ASD2_TRY(ws(ctx, "ri_ls0", n, &ls0));
+    ASD2_TRY(ws(ctx, "ri_ls1", n, &ls1));
+    ASD2_TRY(ws(ctx, "ri_tmp", n, &tmp));
+    // LS1 never gets a BC (its ghosts stay 0) and tmp's corners stay 0 (levelset2d.cpp:326-333)
+    ASD2_TRY(zero_async(ctx, ls1, n * 8));
+    ASD2_TRY(zero_async(ctx, tmp, n * 8));
+    ASD2_HIP(hipMemcpyAsync(ls0, d_ls, n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    ASD2_TRY(bc_apply_p(ctx, d_ls));
+    const Launch L = tiles(G, RJ, RI);
+    const int gh = fused ? 1 : 0;
+    for (int it = 0; it < iterations; it++) {
+        hipLaunchKernelGGL((k_reinit<false, 0>), L.grid, L.block, 0, ctx->stream, G, R, d_ls, ls0, d_ls, ls1, 0);
+        ASD2_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_reinit<false, 1>), L.grid, L.block, 0, ctx->stream, G, R, ls1, ls0, d_ls, tmp, gh);
+        ASD2_LAUNCH_CHECK();
+        ASD2_TRY(finish_ghosts(ctx, fused, tmp));
+        hipLaunchKernelGGL(k_sussman_constraint, dim3(cdiv(G.ny, SJ), cdiv(G.nx, SI)), dim3(WTH), 0, ctx->stream,
+                           G, R, tmp, ls0, d_ls, gh);
+        ASD2_LAUNCH_CHECK();
+        ASD2_TRY(finish_ghosts(ctx, fused, d_ls));
+    }
+    return ASD2_OK;
+}
+
+}  // extern "C"

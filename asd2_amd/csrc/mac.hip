@@ -1,0 +1,463 @@
+// mac.hip -- staggered-grid pieces of the two-phase step around the pressure solve
+// (reference MACSolver2D, src/mac2d.cpp): curvature, Heaviside functions, divergence,
+// GFM-consistent projection, CFL time step, and the momentum predictor.
+#include <algorithm>
+#include <cmath>
+#include <limits>
+
+#include "internal.h"
+#include "weno.h"
+
+namespace asd2 {
+
+inline dim3 rows_grid(const Geom& G) { return dim3(cdiv(G.ny + 2 * G.g, 64), cdiv(G.nx + 2 * G.g, 4)); }
+
+// ------------------------------------------------------------------ curvature
+// src/mac2d.cpp:221-252: central differences; phi_xy from differences of the stored
+// phi_x at j +- 1, which is 0 outside the interior rows (reference boundary artefact).
+__device__ __forceinline__ double dlsdx_at(const Geom& G, const double* ls, int i, int j) {
+#pragma clang fp contract(off)
+    if (j < G.g || j >= G.ny + G.g) return 0.0;
+    return (ls[G.idx(i + 1, j)] - ls[G.idx(i - 1, j)]) / (2.0 * G.dx);
+}
+
+__global__ __launch_bounds__(256) void k_kappa(Geom G, const double* __restrict__ ls, double* __restrict__ kap) {
+#pragma clang fp contract(off)
+    const int j = G.g + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = G.g + blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= G.nx + G.g || j >= G.ny + G.g) return;
+    const int64_t c = G.idx(i, j);
+    const double dx = G.dx, dy = G.dy;
+    const double dX = (ls[G.idx(i + 1, j)] - ls[G.idx(i - 1, j)]) / (2.0 * dx);
+    const double dY = (ls[G.idx(i, j + 1)] - ls[G.idx(i, j - 1)]) / (2.0 * dy);
+    const double S = sqrt(dX * dX + dY * dY + 1.0e-100);
+    const double k = (dX * dX * (ls[G.idx(i, j - 1)] - 2.0 * ls[c] + ls[G.idx(i, j + 1)]) / (dy * dy)
+                    - 2.0 * dX * dY * (dlsdx_at(G, ls, i, j + 1) - dlsdx_at(G, ls, i, j - 1)) / (2.0 * dy)
+                    + dY * dY * (ls[G.idx(i - 1, j)] - 2.0 * ls[c] + ls[G.idx(i + 1, j)]) / (dx * dx))
+                   / pow(S, 3.0);
+    kap[c] = isign(k) * fabs(smin(fabs(k), 1.0 / smin(dx, dy)));
+}
+
+// src/mac2d.cpp:87-122 over the padded array
+__global__ __launch_bounds__(256) void k_heaviside(int64_t n, double eps, int smooth, const double* __restrict__ ls,
+                                                   double* __restrict__ H) {
+#pragma clang fp contract(off)
+    for (int64_t k = blockIdx.x * 256ll + threadIdx.x; k < n; k += static_cast<int64_t>(gridDim.x) * 256) {
+        const double x = ls[k];
+        double h;
+        if (!smooth) h = (x >= 0.0) ? 1.0 : 0.0;
+        else if (x > eps) h = 1.0;
+        else if (x < -eps) h = 0.0;
+        else h = 0.5 * (1.0 + x / eps + 1.0 / M_PI * sin(M_PI * x / eps));
+        H[k] = h;
+    }
+}
+
+// src/mac2d.cpp:1729-1738 (zero outside the interior)
+__global__ __launch_bounds__(256) void k_divergence(Geom G, const double* __restrict__ u, const double* __restrict__ v,
+                                                    double* __restrict__ div) {
+#pragma clang fp contract(off)
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= G.nx + 2 * G.g || j >= G.ny + 2 * G.g) return;
+    double d = 0.0;
+    if (i >= G.g && i < G.nx + G.g && j >= G.g && j < G.ny + G.g)
+        d = (u[G.idx(i + 1, j)] - u[G.idx(i, j)]) / G.dx + (v[G.idx(i, j + 1)] - v[G.idx(i, j)]) / G.dy;
+    div[G.idx(i, j)] = d;
+}
+
+// ------------------------------------------------------------------ projection
+// face between a (west/south cell) and b (the cell itself), src/mac2d.cpp:1775-1788
+__device__ __forceinline__ void face_theta_w(double a, double b, double& thH, double& th) {
+#pragma clang fp contract(off)
+    if (a >= 0.0 && b >= 0.0) { thH = 1.0; th = 0.0; }
+    else if (a < 0.0 && b < 0.0) { thH = 0.0; th = 0.0; }
+    else if (a >= 0.0 && b < 0.0) { thH = fabs(a) / (fabs(a) + fabs(b)); th = fabs(a) / (fabs(a) + fabs(b)); }
+    else { thH = fabs(b) / (fabs(a) + fabs(b)); th = fabs(a) / (fabs(a) + fabs(b)); }
+}
+
+struct ProjArgs {
+    double rho_h, rho_l, sigma, dt;
+};
+
+// u on faces i in [g+1, nx+g), v on faces j in [g+1, ny+g); src/mac2d.cpp:1768-1843
+__global__ __launch_bounds__(256) void k_update_vel(Geom G, ProjArgs A, double* __restrict__ u, double* __restrict__ v,
+                                                    const double* __restrict__ us, const double* __restrict__ vs,
+                                                    const double* __restrict__ ps, const double* __restrict__ ls,
+                                                    const double* __restrict__ kap, const double* __restrict__ H) {
+#pragma clang fp contract(off)
+    const int j = G.g + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = G.g + blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= G.nx + G.g || j >= G.ny + G.g) return;
+    const int64_t c = G.idx(i, j);
+    if (i >= G.g + 1) {
+        const int64_t w = G.idx(i - 1, j);
+        double thH, th;
+        face_theta_w(ls[w], ls[c], thH, th);
+        const double iRho = 1.0 / (A.rho_h * thH + A.rho_l * (1.0 - thH));
+        const double kE = kap[c] * th + kap[w] * (1.0 - th);
+        double val = A.dt * (-A.sigma * kE) * (H[c] - H[w]);
+        val *= iRho / G.dx;
+        val += us[c] - iRho * (ps[c] - ps[w]) / G.dx;
+        u[c] = val;
+    }
+    if (j >= G.g + 1) {
+        const int64_t s = G.idx(i, j - 1);
+        double thH, th;
+        face_theta_w(ls[s], ls[c], thH, th);
+        const double iRho = 1.0 / (A.rho_h * thH + A.rho_l * (1.0 - thH));
+        const double kE = kap[c] * th + kap[s] * (1.0 - th);
+        double val = A.dt * (-A.sigma * kE) * (H[c] - H[s]);
+        val *= iRho / G.dy;
+        val += vs[c] - iRho * (ps[c] - ps[s]) / G.dy;
+        v[c] = val;
+    }
+}
+
+// ------------------------------------------------------------------ CFL
+// max over the interior of |u_{i+1} u_i / 2| and |v_{j+1} v_j / 2| (src/mac2d.cpp:1858-1862)
+__global__ __launch_bounds__(256) void k_velmax(Geom G, const double* __restrict__ u, const double* __restrict__ v,
+                                                double* __restrict__ partials, unsigned* ticket, double* out) {
+    double um = 0.0, vm = 0.0;
+    const int64_t cells = static_cast<int64_t>(G.nx) * G.ny;
+    for (int64_t t = blockIdx.x * 256ll + threadIdx.x; t < cells; t += static_cast<int64_t>(gridDim.x) * 256) {
+        const int i = G.g + static_cast<int>(t / G.ny), j = G.g + static_cast<int>(t % G.ny);
+        um = smax(um, fabs((u[G.idx(i + 1, j)] * u[G.idx(i, j)]) * 0.5));
+        vm = smax(vm, fabs((v[G.idx(i, j + 1)] * v[G.idx(i, j)]) * 0.5));
+    }
+    __shared__ double su[256], sv[256];
+    su[threadIdx.x] = um;
+    sv[threadIdx.x] = vm;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            su[threadIdx.x] = smax(su[threadIdx.x], su[threadIdx.x + s]);
+            sv[threadIdx.x] = smax(sv[threadIdx.x], sv[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    __shared__ int last;
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = su[0];
+        partials[2 * blockIdx.x + 1] = sv[0];
+        __threadfence();
+        last = (atomicAdd(ticket, 1u) == gridDim.x - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (unsigned k = 0; k < gridDim.x; k++) {
+            a = smax(a, partials[2 * k]);
+            b = smax(b, partials[2 * k + 1]);
+        }
+        out[0] = a;
+        out[1] = b;
+        *ticket = 0u;
+    }
+}
+
+// ------------------------------------------------------------------ momentum predictor
+// mu = mu_L + (mu_H - mu_L) H on the interior, 0 on ghosts (src/mac2d.cpp:606-608)
+__device__ __forceinline__ double mu_at(const Geom& G, const double* Hs, double muH, double muL, int i, int j) {
+#pragma clang fp contract(off)
+    if (i < G.g || i >= G.nx + G.g || j < G.g || j >= G.ny + G.g) return 0.0;
+    return muL + (muH - muL) * Hs[G.idx(i, j)];
+}
+
+struct MomArgs {
+    double rho_h, rho_l, mu_h, mu_l, dt;
+    double gu, gv;    // gravity source on U faces / V faces
+    int gv_from_i;    // V gravity only where i >= g + 1 (the U-grid array of mac2d.cpp:294)
+    int viscous;      // kRe > 0
+    int mode;         // 0: s + dt F; 1: 0.75 u0 + 0.25 (s + dt F); 2: u0/3 + 2/3 (s + dt F); 3: u0/2 + (s + dt F)/2
+};
+
+// tmpV of AddConvectionFU at a U face (src/mac2d.cpp:314-316), 0 elsewhere
+__device__ __forceinline__ double vbar_u(const Geom& G, const double* v, int i, int j) {
+    if (j < G.g || j >= G.ny + G.g || i < G.g + 1 || i >= G.nx + G.g) return 0.0;
+    return (v[G.idx(i, j)] + v[G.idx(i - 1, j)] + v[G.idx(i - 1, j + 1)] + v[G.idx(i, j + 1)]) * 0.25;
+}
+
+// tmpU of AddConvectionFV at a V face (src/mac2d.cpp:395-397), 0 elsewhere
+__device__ __forceinline__ double ubar_v(const Geom& G, const double* u, int i, int j) {
+    if (j < G.g + 1 || j >= G.ny + G.g || i < G.g || i >= G.nx + G.g) return 0.0;
+    return (u[G.idx(i, j)] + u[G.idx(i, j - 1)] + u[G.idx(i + 1, j - 1)] + u[G.idx(i + 1, j)]) * 0.25;
+}
+
+__device__ __forceinline__ double irho_harmonic(double rH, double rL, double a, double b) {
+#pragma clang fp contract(off)
+    if (a >= 0 && b >= 0) return 1.0 / rH;
+    else if (a <= 0 && b <= 0) return 1.0 / rL;
+    else if (a > 0 && b <= 0) {
+        const double th = fabs(a) / (fabs(a) + fabs(b));
+        return 1.0 / (rL * rH) / (1.0 / rL * th + 1.0 / rH * (1.0 - th));
+    } else {
+        const double th = fabs(a) / (fabs(a) + fabs(b));
+        return 1.0 / (rH * rL) / (1.0 / rH * th + 1.0 / rL * (1.0 - th));
+    }
+}
+
+__device__ __forceinline__ double rho_arith(double rH, double rL, double a, double b) {
+#pragma clang fp contract(off)
+    if (a >= 0 && b >= 0) return rH;
+    else if (a <= 0 && b <= 0) return rL;
+    else if (a > 0 && b <= 0) {
+        const double th = fabs(a) / (fabs(a) + fabs(b));
+        return rH * th + rL * (1.0 - th);
+    } else {
+        const double th = fabs(a) / (fabs(a) + fabs(b));
+        return rL * th + rH * (1.0 - th);
+    }
+}
+
+// -cU + vU + gU at a U face and the RK combination (src/mac2d.cpp:254-279, 308-387, 570-809)
+__global__ __launch_bounds__(256) void k_mom_u(Geom G, MomArgs A, const double* __restrict__ u,
+                                               const double* __restrict__ v, const double* __restrict__ ls,
+                                               const double* __restrict__ Hs, const double* __restrict__ u0,
+                                               double* __restrict__ out) {
+    const int j = G.g + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = G.g + 1 + blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= G.nx + G.g || j >= G.ny + G.g) return;
+    const int64_t c = G.idx(i, j);
+    // convection: WENO of u^2 along i and of u * vbar along j
+    double lxp, lxm, lyp, lym;
+    {
+        double f[7];
+        for (int k = 0; k < 7; k++) { const double w = u[G.idx(i - 3 + k, j)]; f[k] = w * w; }
+        weno5(f[1] - f[0], f[2] - f[1], f[3] - f[2], f[4] - f[3], f[5] - f[4], f[6] - f[5], 1.0 / G.dx,
+              1e-6 * G.dx * G.dx, lxp, lxm);
+    }
+    {
+        double f[7];
+        for (int k = 0; k < 7; k++) f[k] = u[G.idx(i, j - 3 + k)] * vbar_u(G, v, i, j - 3 + k);
+        weno5(f[1] - f[0], f[2] - f[1], f[3] - f[2], f[4] - f[3], f[5] - f[4], f[6] - f[5], 1.0 / G.dy,
+              1e-6 * G.dy * G.dy, lyp, lym);
+    }
+    const double aX = u[c], aY = vbar_u(G, v, i, j);
+    const double cU = 0.5 * (lxp + lxm) + 0.5 * (lyp + lym) - aX * (0.5 * (lxp - lxm)) - aY * (0.5 * (lyp - lym));
+    double dU = 0.0;
+    if (A.viscous) {
+#pragma clang fp contract(off)
+        const double dx = G.dx, dy = G.dy;
+        const double uM = u[c], uW = u[G.idx(i - 1, j)], uE = u[G.idx(i + 1, j)];
+        const double uS = u[G.idx(i, j - 1)], uN = u[G.idx(i, j + 1)];
+        const double vW = v[G.idx(i - 1, j)], vW_N = v[G.idx(i - 1, j + 1)], vM = v[c], vN = v[G.idx(i, j + 1)];
+        const double lsW = ls[G.idx(i - 1, j)], lsM = ls[c];
+        const double lsUNHalf = 0.25 * (ls[c] + ls[G.idx(i - 1, j)] + ls[G.idx(i - 1, j + 1)] + ls[G.idx(i, j + 1)]);
+        const double lsUSHalf = 0.25 * (ls[c] + ls[G.idx(i - 1, j)] + ls[G.idx(i - 1, j - 1)] + ls[G.idx(i, j - 1)]);
+        const double iWE = irho_harmonic(A.rho_h, A.rho_l, lsW, lsM);
+        const double iSN = irho_harmonic(A.rho_h, A.rho_l, lsUSHalf, lsUNHalf);
+#define MU(a, b) mu_at(G, Hs, A.mu_h, A.mu_l, a, b)
+        const double muU_X_W = MU(i - 1, j) * (uM - uW) / dx;
+        const double muU_X_E = MU(i, j) * (uE - uM) / dx;
+        const double muU_Y_S = 0.25 * (MU(i, j) + MU(i - 1, j) + MU(i - 1, j - 1) + MU(i, j - 1)) * (uM - uS) / dy;
+        const double muU_Y_N = 0.25 * (MU(i, j) + MU(i - 1, j) + MU(i - 1, j + 1) + MU(i, j + 1)) * (uN - uM) / dy;
+        const double muV_X_S = 0.25 * (MU(i, j) + MU(i - 1, j) + MU(i - 1, j - 1) + MU(i, j - 1)) * (vM - vW) / dx;
+        const double muV_X_N = 0.25 * (MU(i, j) + MU(i - 1, j) + MU(i - 1, j + 1) + MU(i, j + 1)) * (vN - vW_N) / dx;
+#undef MU
+        const double visX = 2.0 * (muU_X_E - muU_X_W) / dx;
+        const double visY = (muU_Y_N - muU_Y_S) / dy + (muV_X_N - muV_X_S) / dy;
+        dU = visX * iWE + visY * iSN;
+    }
+    const double F = -cU + dU + A.gu;
+    const double t = u[c] + A.dt * F;
+    double o;
+    if (A.mode == 0) o = t;
+    else if (A.mode == 1) o = 0.75 * u0[c] + 0.25 * t;
+    else if (A.mode == 2) o = 1.0 / 3.0 * u0[c] + 2.0 / 3.0 * t;
+    else o = 0.5 * u0[c] + 0.5 * t;
+    out[c] = o;
+}
+
+// -cV + vV + gV at a V face and the RK combination (src/mac2d.cpp:281-306, 389-463, 811-1041)
+__global__ __launch_bounds__(256) void k_mom_v(Geom G, MomArgs A, const double* __restrict__ u,
+                                               const double* __restrict__ v, const double* __restrict__ ls,
+                                               const double* __restrict__ Hs, const double* __restrict__ v0,
+                                               double* __restrict__ out) {
+    const int j = G.g + 1 + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = G.g + blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= G.nx + G.g || j >= G.ny + G.g) return;
+    const int64_t c = G.idx(i, j);
+    double lxp, lxm, lyp, lym;
+    {
+        double f[7];
+        for (int k = 0; k < 7; k++) f[k] = v[G.idx(i - 3 + k, j)] * ubar_v(G, u, i - 3 + k, j);
+        weno5(f[1] - f[0], f[2] - f[1], f[3] - f[2], f[4] - f[3], f[5] - f[4], f[6] - f[5], 1.0 / G.dx,
+              1e-6 * G.dx * G.dx, lxp, lxm);
+    }
+    {
+        double f[7];
+        for (int k = 0; k < 7; k++) { const double w = v[G.idx(i, j - 3 + k)]; f[k] = w * w; }
+        weno5(f[1] - f[0], f[2] - f[1], f[3] - f[2], f[4] - f[3], f[5] - f[4], f[6] - f[5], 1.0 / G.dy,
+              1e-6 * G.dy * G.dy, lyp, lym);
+    }
+    const double aX = ubar_v(G, u, i, j), aY = v[c];
+    const double cV = 0.5 * (lxp + lxm) + 0.5 * (lyp + lym) - aX * (0.5 * (lxp - lxm)) - aY * (0.5 * (lyp - lym));
+    double dV = 0.0;
+    if (A.viscous) {
+#pragma clang fp contract(off)
+        const double dx = G.dx, dy = G.dy;
+        const double vW = v[G.idx(i - 1, j)], vE = v[G.idx(i + 1, j)], vS = v[G.idx(i, j - 1)];
+        const double vN = v[G.idx(i, j + 1)], vM = v[c];
+        const double uM = u[c], uE = u[G.idx(i + 1, j)], uS = u[G.idx(i, j - 1)], uE_S = u[G.idx(i + 1, j - 1)];
+        const double lsVWHalf = 0.25 * (ls[c] + ls[G.idx(i - 1, j)] + ls[G.idx(i - 1, j - 1)] + ls[G.idx(i, j - 1)]);
+        const double lsVEHalf = 0.25 * (ls[c] + ls[G.idx(i + 1, j)] + ls[G.idx(i + 1, j - 1)] + ls[G.idx(i, j - 1)]);
+        const double iWE = 1.0 / rho_arith(A.rho_h, A.rho_l, lsVWHalf, lsVEHalf);
+        const double iSN = 1.0 / rho_arith(A.rho_h, A.rho_l, ls[G.idx(i, j - 1)], ls[c]);
+#define MU(a, b) mu_at(G, Hs, A.mu_h, A.mu_l, a, b)
+        const double muU_Y_W = 0.25 * (MU(i, j) + MU(i - 1, j) + MU(i - 1, j - 1) + MU(i, j - 1)) * (uM - uS) / dy;
+        const double muU_Y_E = 0.25 * (MU(i, j) + MU(i + 1, j) + MU(i + 1, j - 1) + MU(i, j - 1)) * (uE - uE_S) / dy;
+        const double muV_X_W = 0.25 * (MU(i, j) + MU(i - 1, j) + MU(i - 1, j - 1) + MU(i, j - 1)) * (vM - vW) / dx;
+        const double muV_X_E = 0.25 * (MU(i, j) + MU(i + 1, j) + MU(i + 1, j - 1) + MU(i, j - 1)) * (vE - vM) / dx;
+        const double muV_Y_S = MU(i, j - 1) * (vM - vS) / dy;
+        const double muV_Y_N = MU(i, j) * (vN - vM) / dy;
+#undef MU
+        const double visX = (muU_Y_E - muU_Y_W) / dx + (muV_X_E - muV_X_W) / dx;
+        const double visY = 2.0 * (muV_Y_N - muV_Y_S) / dy;
+        dV = visX * iWE + visY * iSN;
+    }
+    const double gv = (A.gv_from_i && i < G.g + 1) ? 0.0 : A.gv;
+    const double F = -cV + dV + gv;
+    const double t = v[c] + A.dt * F;
+    double o;
+    if (A.mode == 0) o = t;
+    else if (A.mode == 1) o = 0.75 * v0[c] + 0.25 * t;
+    else if (A.mode == 2) o = 1.0 / 3.0 * v0[c] + 2.0 / 3.0 * t;
+    else o = 0.5 * v0[c] + 0.5 * t;
+    out[c] = o;
+}
+
+}  // namespace asd2
+
+using namespace asd2;
+
+extern "C" {
+
+int asd2_mac_kappa(asd2_ctx* ctx, const double* d_ls, double* d_kappa) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!d_ls || !d_kappa) { set_error("null array"); return ASD2_ERR_ARG; }
+    const Geom& G = ctx->geo;
+    hipLaunchKernelGGL(k_kappa, dim3(cdiv(G.ny, 64), cdiv(G.nx, 4)), dim3(256), 0, ctx->stream, G, d_ls, d_kappa);
+    ASD2_LAUNCH_CHECK();
+    return ASD2_OK;
+}
+
+int asd2_mac_heaviside(asd2_ctx* ctx, const double* d_ls, double* d_H, int smooth) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!d_ls || !d_H) { set_error("null array"); return ASD2_ERR_ARG; }
+    const Geom& G = ctx->geo;
+    const double eps = std::min(G.dx, G.dy) * 1.5;
+    hipLaunchKernelGGL(k_heaviside, dim3(2048), dim3(256), 0, ctx->stream, G.size(), eps, smooth ? 1 : 0, d_ls, d_H);
+    ASD2_LAUNCH_CHECK();
+    return ASD2_OK;
+}
+
+int asd2_mac_divergence(asd2_ctx* ctx, const double* d_u, const double* d_v, double* d_div) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!d_u || !d_v || !d_div) { set_error("null array"); return ASD2_ERR_ARG; }
+    const Geom& G = ctx->geo;
+    hipLaunchKernelGGL(k_divergence, rows_grid(G), dim3(256), 0, ctx->stream, G, d_u, d_v, d_div);
+    ASD2_LAUNCH_CHECK();
+    return ASD2_OK;
+}
+
+int asd2_mac_update_vel(asd2_ctx* ctx, const asd2_fluid* fl, double* d_u, double* d_v, const double* d_us,
+                        const double* d_vs, const double* d_ps, const double* d_ls, const double* d_kappa,
+                        const double* d_H) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!fl || !d_u || !d_v || !d_us || !d_vs || !d_ps || !d_ls || !d_kappa || !d_H) {
+        set_error("null argument");
+        return ASD2_ERR_ARG;
+    }
+    const Geom& G = ctx->geo;
+    ProjArgs A{fl->rho_h, fl->rho_l, fl->sigma, fl->dt};
+    hipLaunchKernelGGL(k_update_vel, dim3(cdiv(G.ny, 64), cdiv(G.nx, 4)), dim3(256), 0, ctx->stream, G, A, d_u, d_v,
+                       d_us, d_vs, d_ps, d_ls, d_kappa, d_H);
+    ASD2_LAUNCH_CHECK();
+    ASD2_TRY(bc_apply_uv(ctx, ASD2_GRID_U, d_u));
+    ASD2_TRY(bc_apply_uv(ctx, ASD2_GRID_V, d_v));
+    return ASD2_OK;
+}
+
+int asd2_mac_update_dt(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_u, const double* d_v, double* dt_out) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!fl || !d_u || !d_v || !dt_out) { set_error("null argument"); return ASD2_ERR_ARG; }
+    const Geom& G = ctx->geo;
+    double *partials, *out;
+    unsigned* ticket;
+    ASD2_TRY(ws(ctx, "dt_partials", 2 * 1024, &partials));
+    ASD2_TRY(ws(ctx, "dt_out", 2, &out));
+    ASD2_TRY(ws(ctx, "dt_ticket", 2, &ticket));
+    hipLaunchKernelGGL(k_velmax, dim3(1024), dim3(256), 0, ctx->stream, G, d_u, d_v, partials, ticket, out);
+    ASD2_LAUNCH_CHECK();
+    double h[2];
+    ASD2_HIP(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    ASD2_HIP(hipStreamSynchronize(ctx->stream));
+    const double dx = G.dx, dy = G.dy;
+    const double C = h[0] / dx + h[1] / dy;
+    const double V = std::max(fl->mu_h, fl->mu_l) * (2.0 / (dx * dx + dy * dy));
+    const double Gf = std::max(std::sqrt(std::fabs(fl->g) / dy), std::sqrt(std::fabs(fl->g) / dx));
+    const double S = std::sqrt((fl->sigma / std::min(dx, dy)) /
+                               (std::min(fl->rho_h, fl->rho_l) * std::pow(std::min(dx, dy), 2.0)));
+    double dt = std::numeric_limits<double>::max();
+    dt = std::min(dt, fl->cfl / (0.5 * (C + V + std::sqrt(std::pow(C + V, 2.0) + 4.0 * Gf * Gf + 4.0 * S * S))));
+    *dt_out = dt;
+    if (!std::isfinite(dt)) { set_error("dt nan/inf"); return ASD2_ERR_NONFINITE; }
+    return ASD2_OK;
+}
+
+int asd2_mac_intermediate_vel(asd2_ctx* ctx, const asd2_fluid* fl, int time_order, const double* d_ls,
+                              const double* d_u, const double* d_v, double* d_uhat, double* d_vhat,
+                              const double* d_Hs) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!fl || !d_ls || !d_u || !d_v || !d_uhat || !d_vhat || !d_Hs || time_order < 0 || time_order > 2) {
+        set_error("bad argument");
+        return ASD2_ERR_ARG;
+    }
+    const Geom& G = ctx->geo;
+    const size_t n = static_cast<size_t>(G.size());
+    MomArgs A{};
+    A.rho_h = fl->rho_h; A.rho_l = fl->rho_l; A.mu_h = fl->mu_h; A.mu_l = fl->mu_l; A.dt = fl->dt;
+    const bool fr_ok = (fl->fr != 0 || fl->g != 0.0) && !std::isnan(fl->fr) && !std::isinf(fl->fr);
+    A.gu = (fr_ok && fl->g_axis == 0) ? -fl->g : 0.0;
+    if (fl->gravity_fix) { A.gv = (fr_ok && fl->g_axis == 1) ? -fl->g : 0.0; A.gv_from_i = 0; }
+    else { A.gv = A.gu; A.gv_from_i = 1; }  // mac2d.cpp:294 adds AddGravityFU() to the V equation
+    A.viscous = fl->re > 0.0 ? 1 : 0;
+    double *u1, *v1, *u2, *v2;
+    ASD2_TRY(ws(ctx, "mom_u1", n, &u1));
+    ASD2_TRY(ws(ctx, "mom_v1", n, &v1));
+    ASD2_TRY(ws(ctx, "mom_u2", n, &u2));
+    ASD2_TRY(ws(ctx, "mom_v2", n, &v2));
+    // fresh stage arrays (GetIntermediateVel allocates zero vectors, mac2d.cpp:1146-1147)
+    for (double* p : {u1, v1, u2, v2}) ASD2_TRY(zero_async(ctx, p, n * 8));
+    const dim3 gu(cdiv(G.ny, 64), cdiv(G.nx - 1, 4)), gv(cdiv(G.ny - 1, 64), cdiv(G.nx, 4));
+    auto stage = [&](const double* us, const double* vs, const double* u0, const double* v0, double* uo, double* vo,
+                     int mode) -> int {
+        A.mode = mode;
+        hipLaunchKernelGGL(k_mom_u, gu, dim3(256), 0, ctx->stream, G, A, us, vs, d_ls, d_Hs, u0, uo);
+        hipLaunchKernelGGL(k_mom_v, gv, dim3(256), 0, ctx->stream, G, A, us, vs, d_ls, d_Hs, v0, vo);
+        ASD2_LAUNCH_CHECK();
+        return ASD2_OK;
+    };
+    if (time_order == 0) {
+        ASD2_TRY(stage(d_u, d_v, d_u, d_v, d_uhat, d_vhat, 0));
+    } else if (time_order == 1) {
+        ASD2_TRY(stage(d_u, d_v, d_u, d_v, u1, v1, 0));
+        ASD2_TRY(bc_apply_uv(ctx, ASD2_GRID_U, u1));
+        ASD2_TRY(bc_apply_uv(ctx, ASD2_GRID_V, v1));
+        ASD2_TRY(stage(u1, v1, d_u, d_v, d_uhat, d_vhat, 3));
+    } else {
+        ASD2_TRY(stage(d_u, d_v, d_u, d_v, u1, v1, 0));
+        ASD2_TRY(bc_apply_uv(ctx, ASD2_GRID_U, u1));
+        ASD2_TRY(bc_apply_uv(ctx, ASD2_GRID_V, v1));
+        ASD2_TRY(stage(u1, v1, d_u, d_v, u2, v2, 1));
+        ASD2_TRY(bc_apply_uv(ctx, ASD2_GRID_U, u2));
+        ASD2_TRY(bc_apply_uv(ctx, ASD2_GRID_V, v2));
+        ASD2_TRY(stage(u2, v2, d_u, d_v, d_uhat, d_vhat, 2));
+    }
+    return ASD2_OK;
+}
+
+}  // extern "C"

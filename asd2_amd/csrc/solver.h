@@ -1,0 +1,39 @@
+// solver.h -- matrix-free pressure system and Krylov state shared by the solver sources.
+#pragma once
+
+#include "internal.h"
+
+namespace asd2 {
+
+// Matrix-free 5-point operator on the padded layout (interior cells are unknowns; all
+// ghost entries are 0 so ghost neighbours contribute nothing):
+//   (A p)_c = diag_c p_c + ax_c p_W + ax_E p_E + ay_c p_S + ay_N p_N
+struct System {
+    double *ax = nullptr, *ay = nullptr, *diag = nullptr, *rhs = nullptr, *dinv = nullptr;
+    float *ax32 = nullptr, *ay32 = nullptr, *diag32 = nullptr, *dinv32 = nullptr;
+};
+
+int system_arrays(asd2_ctx* ctx, System* S, bool want32);
+int ensure_system32(asd2_ctx* ctx, System* S);
+bool all_neumann_family(const asd2_ctx* ctx);
+
+// Device-resident Krylov scalars; updated by the last block of each reducing kernel.
+struct KrylovState {
+    double rho, rho_old, alpha, beta, omega;  // recurrence scalars
+    double rr;                                // ||r||^2 of the recurrence
+    double bnorm;                             // ||b|| (1 if b = 0)
+    double tol;
+    double rmax;                              // mixed: ||r|| at the last reliable update
+    double delta;                             // mixed: replacement factor
+    double pq, aux0, aux1, aux2;              // scratch reductions
+    int iter, maxit, mode, done;              // done: 0 running, 1 converged, 2 maxit, 3 breakdown, 4 paused
+    int updates, fallbacks, flags, pad;
+    unsigned int ticket[4];
+};
+
+// flexible PCG with the learned preconditioner (learned.hip)
+int learned_solve(asd2_ctx* ctx, const asd2_solve_opts* opts, double* d_ps, asd2_solve_stats* stats);
+
+constexpr int DONE_RUNNING = 0, DONE_CONVERGED = 1, DONE_MAXIT = 2, DONE_BREAKDOWN = 3, DONE_PAUSED = 4;
+
+}  // namespace asd2

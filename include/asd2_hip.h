@@ -1,0 +1,214 @@
+/*
+ * asd2_hip.h -- C ABI of the MI355X (gfx950) two-phase hot path.
+ *
+ * The drop-in C++ classes in levelset2d.h / poisson2d.h / mac2d.h / bc2d.h are
+ * implemented on top of these entry points; other hosts (ctypes, cgo, JNI) can bind
+ * them directly.  Plain pointers and sizes only.  Field pointers named d_* are DEVICE
+ * pointers to FP64 arrays in the reference layout: (nx + 2g) x (ny + 2g) doubles, g = 3
+ * ghost layers, j fastest, idx(i, j) = j + (ny + 2g) * i (reference
+ * src/levelset2d.cpp:1048-1050).  U values sit on west faces, V values on south faces
+ * of the cell with the same index (reference src/mac2d.cpp:1729-1735).
+ *
+ * Every function returns ASD2_OK or an error code; asd2_last_error() gives the message
+ * of the last failure on the calling thread.  Work is enqueued on the context's stream;
+ * functions that return host scalars synchronise that stream.
+ */
+#ifndef ASD2_HIP_H
+#define ASD2_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ASD2_OK 0
+#define ASD2_ERR_ARG 1
+#define ASD2_ERR_UNSUPPORTED 2
+#define ASD2_ERR_HIP 3
+#define ASD2_ERR_NOT_CONVERGED 4
+#define ASD2_ERR_NONFINITE 5
+#define ASD2_ERR_STATE 6
+
+/* boundary-condition kinds, same order as enum class BC2D (reference include/common.h:5) */
+#define ASD2_BC_PERIODIC 0
+#define ASD2_BC_NEUMANN 1
+#define ASD2_BC_DIRICHLET 2
+#define ASD2_BC_AXISYM 3
+#define ASD2_BC_WALL 4
+#define ASD2_BC_INLET 5
+#define ASD2_BC_OUTLET 6
+#define ASD2_BC_PRESSURE 7
+#define ASD2_BC_LSFREE 8
+#define ASD2_BC_CUSTOM 9
+
+/* staggered grid a boundary condition applies to */
+#define ASD2_GRID_P 0
+#define ASD2_GRID_U 1
+#define ASD2_GRID_V 2
+
+/* Krylov iteration variants */
+#define ASD2_CG_REFERENCE 0   /* reference CG recurrence, src/poisson2d.cpp:345-396 */
+#define ASD2_CG_STANDARD 1    /* textbook Jacobi-PCG */
+
+/* arithmetic of the pressure solve */
+#define ASD2_PREC_FP64 0      /* FP64 Jacobi-PCG */
+#define ASD2_PREC_MIXED 1     /* FP32 PCG with FP64 reliable updates of x and r */
+
+/* preconditioner of the pressure solve */
+#define ASD2_PRECOND_JACOBI 0
+#define ASD2_PRECOND_LEARNED 1 /* U-Net correction inside flexible PCG, Jacobi fallback */
+
+typedef struct asd2_ctx asd2_ctx;
+
+/* grid description, reference LevelSetSolver2D / MACSolver2D constructor arguments */
+typedef struct asd2_grid {
+    int nx, ny, num_bc_grid;
+    double base_x, base_y, dx, dy;
+} asd2_grid;
+
+/* two-phase fluid parameters, reference MACSolver2D members (include/mac2d.h:33-51) */
+typedef struct asd2_fluid {
+    double rho_h, rho_l, mu_h, mu_l;
+    double sigma, dt;
+    double g;          /* kG */
+    int g_axis;        /* 0 = X, 1 = Y (GAXISENUM2D) */
+    int gravity_fix;   /* 1: V momentum gets the V-grid gravity (reference mac2d.cpp:294 uses U's) */
+    double re, we, fr; /* kRe, kWe, kFr */
+    double cfl;
+} asd2_fluid;
+
+typedef struct asd2_solve_opts {
+    double tol;          /* stop when ||r||_2 / ||b||_2 <= tol */
+    int max_iter;        /* iteration cap (inner iterations for MIXED) */
+    int mode;            /* ASD2_CG_* */
+    int precision;       /* ASD2_PREC_* */
+    int precond;         /* ASD2_PRECOND_* */
+    double reliable_delta; /* MIXED: FP64 residual replacement when ||r|| drops by this factor (0 -> 0.1) */
+    int check_every;     /* iterations enqueued between host convergence polls (0 -> 64) */
+} asd2_solve_opts;
+
+typedef struct asd2_solve_stats {
+    int iterations;          /* Krylov iterations performed */
+    int reliable_updates;    /* MIXED: FP64 residual replacements */
+    int fallbacks;           /* LEARNED: switches to the Jacobi preconditioner */
+    int converged;           /* 1 when the tolerance was met */
+    double rel_residual;     /* ||r||/||b|| of the recurrence at exit */
+    double true_rel_residual;/* ||b - A x||/||b|| evaluated in FP64 at exit */
+} asd2_solve_stats;
+
+/* ---------------------------------------------------------------- runtime plumbing */
+const char* asd2_last_error(void);
+int asd2_version(void);
+int asd2_device_count(int* count);
+int asd2_device_name(char* buf, int len);
+int asd2_malloc(void** d_ptr, size_t bytes);           /* zero-filled device allocation */
+int asd2_free(void* d_ptr);
+int asd2_memcpy_h2d(void* d_dst, const void* h_src, size_t bytes);
+int asd2_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes);
+int asd2_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes);
+int asd2_device_synchronize(void);
+
+/* ---------------------------------------------------------------- context */
+/* replaces the solver constructors, reference src/levelset2d.cpp:3-25,
+ * src/mac2d.cpp:28-49, src/poisson2d.cpp:3-5; num_bc_grid must be 3 */
+int asd2_create(const asd2_grid* grid, asd2_ctx** out);
+int asd2_destroy(asd2_ctx* ctx);
+int asd2_synchronize(asd2_ctx* ctx);
+/* reference BoundaryCondition2D::SetBC_{U,V,P}_2D + SetBCConstant* (src/bc2d.cpp:16-344,630-849);
+ * kinds/consts ordered W, E, S, N */
+int asd2_set_bc(asd2_ctx* ctx, int grid_kind, const int kinds[4], const double consts[4]);
+
+/* ---------------------------------------------------------------- boundary conditions */
+/* BoundaryCondition2D::ApplyBC_{P,U,V}_2D, src/bc2d.cpp:123-130,239-246,346-353 */
+int asd2_bc_apply(asd2_ctx* ctx, int grid_kind, double* d_arr);
+
+/* ---------------------------------------------------------------- level set */
+/* LevelSetSolver2D::Solve_LevelSet_2D, src/levelset2d.cpp:43-86 (in place) */
+int asd2_ls_advect(asd2_ctx* ctx, double* d_ls, const double* d_u, const double* d_v, double dt);
+/* LevelSetSolver2D::HJWENO5_LS_2D, src/levelset2d.cpp:88-157 (applies the P BC to d_ls) */
+int asd2_ls_weno_rhs(asd2_ctx* ctx, double* d_ls, const double* d_uc, const double* d_vc, double* d_L);
+/* LevelSetSolver2D::Reinit_MinRK2_2D, src/levelset2d.cpp:433-486; iterations from
+ * asd2_ls_reinit_iterations(min(dx,dy), kMaxATime) */
+int asd2_ls_reinit_minrk2(asd2_ctx* ctx, double* d_ls, int iterations);
+/* LevelSetSolver2D::Reinit_Sussman_2D, src/levelset2d.cpp:318-375 */
+int asd2_ls_reinit_sussman(asd2_ctx* ctx, double* d_ls, int iterations);
+/* LevelSetSolver2D::SubcellENO_Min_DerivAbsLS_2D, src/levelset2d.cpp:615-785 */
+int asd2_ls_subcell_absgrad(asd2_ctx* ctx, const double* d_ls, const double* d_ls0, double* d_out);
+/* LevelSetSolver2D::ENO_DerivAbsLS_2D, src/levelset2d.cpp:559-613 */
+int asd2_ls_eno_absgrad(asd2_ctx* ctx, const double* d_ls, const double* d_ls0, double* d_out);
+/* the pseudo-time loop count of src/levelset2d.cpp:447-448 (host only) */
+int asd2_ls_reinit_iterations(double eps, double max_time);
+
+/* ---------------------------------------------------------------- MAC helpers */
+/* MACSolver2D::UpdateKappa, src/mac2d.cpp:221-252 (writes the interior) */
+int asd2_mac_kappa(asd2_ctx* ctx, const double* d_ls, double* d_kappa);
+/* MACSolver2D::UpdateHeavisideFunc (smooth = 0, src/mac2d.cpp:87-101) and
+ * UpdateSmoothHeavisideFunc (smooth = 1, :103-122) */
+int asd2_mac_heaviside(asd2_ctx* ctx, const double* d_ls, double* d_H, int smooth);
+/* MACSolver2D::GetDivergence, src/mac2d.cpp:1729-1738 */
+int asd2_mac_divergence(asd2_ctx* ctx, const double* d_u, const double* d_v, double* d_div);
+/* MACSolver2D::UpdateVel, src/mac2d.cpp:1740-1849 (then U/V BCs) */
+int asd2_mac_update_vel(asd2_ctx* ctx, const asd2_fluid* fl, double* d_u, double* d_v,
+                        const double* d_us, const double* d_vs, const double* d_ps,
+                        const double* d_ls, const double* d_kappa, const double* d_H);
+/* MACSolver2D::UpdateDt, src/mac2d.cpp:1851-1880 */
+int asd2_mac_update_dt(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_u, const double* d_v,
+                       double* dt_out);
+/* MACSolver2D::GetIntermediateVel, src/mac2d.cpp:1081-1239 (time_order 0/1/2 =
+ * EULER/RK2/RK3; d_Hs is the smoothed Heaviside) */
+int asd2_mac_intermediate_vel(asd2_ctx* ctx, const asd2_fluid* fl, int time_order,
+                              const double* d_ls, const double* d_u, const double* d_v,
+                              double* d_uhat, double* d_vhat, const double* d_Hs);
+
+/* ---------------------------------------------------------------- GFM pressure Poisson */
+/* Assembly half of MACSolver2D::SolvePoisson, src/mac2d.cpp:1249-1656: face
+ * coefficients, jump right-hand side, BC and pin handling, stored matrix-free in the
+ * context.  d_H may be NULL (then H = [ls >= 0]).  The all-Neumann pin uses the
+ * intended operator (the reference leaves the centre column unshifted, :1601-1604). */
+int asd2_gfm_assemble(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_ls,
+                      const double* d_kappa, const double* d_H, const double* d_div);
+/* copies the assembled system out (any pointer may be NULL): ax/ay = CSR value of the
+ * west/south neighbour of each cell, diag = CSR centre value, rhs = right-hand side */
+int asd2_gfm_export(asd2_ctx* ctx, double* d_ax, double* d_ay, double* d_diag, double* d_rhs);
+/* installs a caller-built system of the same form */
+int asd2_gfm_import(asd2_ctx* ctx, const double* d_ax, const double* d_ay, const double* d_diag,
+                    const double* d_rhs);
+/* y = A x with the assembled operator (interior cells) */
+int asd2_gfm_apply(asd2_ctx* ctx, const double* d_x, double* d_y);
+/* Solve half of SolvePoisson: PoissonSolver2D::CG_2FUniformP_2D, src/poisson2d.cpp:230-401
+ * (x0 = 0; the solution is written to the interior of d_ps, ghosts untouched) */
+int asd2_poisson_solve(asd2_ctx* ctx, const asd2_solve_opts* opts, double* d_ps,
+                       asd2_solve_stats* stats);
+/* PoissonSolver2D::BiCGStab_2FUniform_2D, src/poisson2d.cpp:403-575 (textbook
+ * recurrence, pin-aware) */
+int asd2_poisson_bicgstab(asd2_ctx* ctx, const asd2_solve_opts* opts, double* d_ps,
+                          asd2_solve_stats* stats);
+/* CG_2FUniform_2D on a caller-built zero-based CSR matrix (src/poisson2d.cpp:285-401):
+ * n rows, Jacobi preconditioner from d_mdiag, x0 = 0 */
+int asd2_csr_pcg(asd2_ctx* ctx, int n, const int* d_rowptr, const int* d_cols,
+                 const double* d_vals, const double* d_mdiag, const double* d_b, double* d_x,
+                 const asd2_solve_opts* opts, asd2_solve_stats* stats);
+
+/* ---------------------------------------------------------------- timing */
+/* records the start/stop of a region on the context's stream (hipEvents) */
+int asd2_timer_start(asd2_ctx* ctx);
+int asd2_timer_stop(asd2_ctx* ctx, double* ms);
+/* per-launch timing of one kernel family (hipEvents around each of its launches):
+ * kernel_id is one of ASD2_KID_*, 0 = off; reading resets the counters */
+#define ASD2_KID_WENO_STAGE 1     /* fused HJ-WENO5 + SSP-RK3 stage */
+#define ASD2_KID_REINIT 2         /* MinRK2 substages */
+#define ASD2_KID_PCG64_SPMV 3     /* FP64 stencil SpMV + direction update + dot */
+#define ASD2_KID_PCG64_UPDATE 4   /* FP64 x/r/z update + dots */
+#define ASD2_KID_PCG32_SPMV 5     /* FP32 counterparts */
+#define ASD2_KID_PCG32_UPDATE 6
+#define ASD2_KID_GFM 7            /* GFM assembly */
+#define ASD2_KID_UNET 8           /* learned preconditioner application */
+int asd2_profile_kernel(asd2_ctx* ctx, int kernel_id);
+int asd2_profile_read(asd2_ctx* ctx, int* launches, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,165 @@
+"""GPU parity of the level-set path (reference src/levelset2d.cpp, src/bc2d.cpp) against the
+FP64 oracle, called through the C ABI.  Tolerance: max |delta phi| <= 1e-12 after a step
+(the parity bar of the level set); BC fills are exact."""
+import numpy as np
+import pytest
+
+import asd2_amd as A
+from oracle import make_grid
+from tests import scenarios as S
+
+pytestmark = pytest.mark.gpu
+
+TOL_PHI = 1e-12
+
+GRIDS = [(64, 64), (67, 45), (128, 128)]
+P_BCS = [("neumann",) * 4, ("dirichlet",) * 4, ("neumann", "dirichlet", "wall", "outlet"),
+         ("neumann", "neumann", "custom", "lsfree")]
+
+
+def _ctx(nx, ny, dx, dy, bc_p=("neumann",) * 4, c_p=(0.0,) * 4):
+    c = A.Context(nx, ny, dx, dy)
+    c.set_bc(A.GRID_P, bc_p, c_p)
+    return c
+
+
+def _level_set(nx, ny, seed=0):
+    dx = 1.0 / max(nx, ny)
+    phi = S.circle_sdf(nx, ny, dx, dx, 0.5 * nx * dx, 0.45 * ny * dx, 0.22 * min(nx, ny) * dx)
+    phi += 0.02 * dx * S.random_field(seed, phi.shape)
+    return np.ascontiguousarray(phi), dx
+
+
+@pytest.mark.parametrize("bc", P_BCS, ids=lambda b: "-".join(b))
+def test_bc_p_exact(oracle, bc):
+    nx, ny, dx = 37, 29, 0.03
+    gr = make_grid(nx, ny, dx, dx, bc_p=bc, c_p=(0.1, -0.2, 0.3, 0.4))
+    a = S.random_field(1, (nx + 6, ny + 6))
+    ctx = _ctx(nx, ny, dx, dx, bc, (0.1, -0.2, 0.3, 0.4))
+    d = ctx.array(a)
+    ctx.bc_apply(A.GRID_P, d)
+    np.testing.assert_array_equal(d.download(), oracle.bc_p(gr, a.copy()))
+
+
+@pytest.mark.parametrize("kind", ["neumann", "dirichlet", "periodic", "axisym", "wall", "inlet", "outlet"])
+def test_bc_uv_exact(oracle, kind):
+    nx, ny, dx = 31, 26, 0.03
+    cu, cv = (0.1, 0.2, 0.3, 0.4), (0.5, -0.6, 0.7, 0.8)
+    gr = make_grid(nx, ny, dx, dx, bc_u=(kind,) * 4, c_u=cu, bc_v=(kind,) * 4, c_v=cv)
+    ctx = A.Context(nx, ny, dx, dx)
+    ctx.set_bc(A.GRID_U, (kind,) * 4, cu)
+    ctx.set_bc(A.GRID_V, (kind,) * 4, cv)
+    for gk, seed, fn in ((A.GRID_U, 2, oracle.bc_u), (A.GRID_V, 3, oracle.bc_v)):
+        a = S.random_field(seed, (nx + 6, ny + 6))
+        d = ctx.array(a)
+        ctx.bc_apply(gk, d)
+        np.testing.assert_array_equal(d.download(), fn(gr, a.copy()))
+
+
+@pytest.mark.parametrize("nx,ny", GRIDS)
+@pytest.mark.parametrize("bc", P_BCS[:3], ids=lambda b: "-".join(b))
+def test_advect_one_step(oracle, nx, ny, bc):
+    phi, dx = _level_set(nx, ny)
+    u, v = S.single_vortex(nx, ny, dx, dx)
+    dt = 0.5 * dx
+    gr = make_grid(nx, ny, dx, dx, bc_p=bc, c_p=(0.05, 0.0, -0.05, 0.1))
+    ref = oracle.solve_levelset(gr, phi.copy(), u, v, dt)
+    ctx = _ctx(nx, ny, dx, dx, bc, (0.05, 0.0, -0.05, 0.1))
+    d_phi, d_u, d_v = ctx.array(phi), ctx.array(u), ctx.array(v)
+    ctx.ls_advect(d_phi, d_u, d_v, dt)
+    out = d_phi.download()
+    err = np.abs(out - ref).max()
+    print(f"advect {nx}x{ny} {bc}: max|dphi| = {err:.3e}")
+    assert err <= TOL_PHI
+    assert np.abs(out - phi).max() > 1e-4  # the step moved the level set
+
+
+def test_advect_custom_and_lsfree(oracle):
+    nx, ny = 48, 40
+    phi, dx = _level_set(nx, ny, 3)
+    u, v = S.single_vortex(nx, ny, dx, dx)
+    bc = P_BCS[3]
+    gr = make_grid(nx, ny, dx, dx, bc_p=bc)
+    ref = oracle.solve_levelset(gr, phi.copy(), u, v, 0.3 * dx)
+    ctx = _ctx(nx, ny, dx, dx, bc)
+    d_phi = ctx.array(phi)
+    ctx.ls_advect(d_phi, ctx.array(u), ctx.array(v), 0.3 * dx)
+    assert np.abs(d_phi.download() - ref).max() <= TOL_PHI
+
+
+def test_weno_rhs(oracle):
+    nx, ny = 53, 47
+    phi, dx = _level_set(nx, ny, 4)
+    uc = S.random_field(5, phi.shape)
+    vc = S.random_field(6, phi.shape)
+    gr = make_grid(nx, ny, dx, dx)
+    ref = oracle.hjweno5_ls(gr, phi.copy(), uc, vc)
+    ctx = _ctx(nx, ny, dx, dx)
+    d_phi, d_L = ctx.array(phi), ctx.array()
+    ctx.ls_weno_rhs(d_phi, ctx.array(uc), ctx.array(vc), d_L)
+    L = d_L.download()
+    scale = np.abs(ref).max()
+    assert np.abs(L - ref).max() <= 1e-13 * scale
+
+
+@pytest.mark.parametrize("subcell", [True, False])
+def test_absgrad(oracle, subcell):
+    nx, ny = 61, 58
+    phi, dx = _level_set(nx, ny, 7)
+    gr = make_grid(nx, ny, dx, dx)
+    oracle.bc_p(gr, phi)
+    phi0 = phi.copy()
+    phi1 = phi + 0.3 * dx * S.random_field(8, phi.shape)
+    fn = oracle.subcell_absgrad if subcell else oracle.eno_absgrad
+    ref = fn(gr, phi1, phi0)
+    ctx = _ctx(nx, ny, dx, dx)
+    d_out = ctx.array()
+    if subcell:
+        ctx.ls_subcell_absgrad(ctx.array(phi1), ctx.array(phi0), d_out)
+    else:
+        ctx.ls_eno_absgrad(ctx.array(phi1), ctx.array(phi0), d_out)
+    assert np.abs(d_out.download() - ref).max() <= 1e-12
+
+
+@pytest.mark.parametrize("iters", [3, 5])
+@pytest.mark.parametrize("nx,ny", GRIDS)
+def test_reinit_minrk2(oracle, nx, ny, iters):
+    phi, dx = _level_set(nx, ny, 9)
+    phi = np.ascontiguousarray(phi * (1.0 + 0.3 * np.sin(7 * phi / dx)))  # not a distance function
+    gr = make_grid(nx, ny, dx, dx)
+    ref = oracle.reinit_minrk2(gr, phi.copy(), iters)
+    ctx = _ctx(nx, ny, dx, dx)
+    d = ctx.array(phi)
+    ctx.ls_reinit_minrk2(d, iters)
+    err = np.abs(d.download() - ref).max()
+    print(f"minrk2 {nx}x{ny} K={iters}: max|dphi| = {err:.3e}")
+    assert err <= TOL_PHI
+
+
+@pytest.mark.parametrize("nx,ny", GRIDS[:2])
+def test_reinit_sussman(oracle, nx, ny):
+    phi, dx = _level_set(nx, ny, 10)
+    phi = np.ascontiguousarray(phi * (1.0 + 0.2 * np.cos(5 * phi / dx)))
+    gr = make_grid(nx, ny, dx, dx)
+    ref = oracle.reinit_sussman(gr, phi.copy(), 5)
+    ctx = _ctx(nx, ny, dx, dx)
+    d = ctx.array(phi)
+    ctx.ls_reinit_sussman(d, 5)
+    err = np.abs(d.download() - ref).max()
+    print(f"sussman {nx}x{ny}: max|dphi| = {err:.3e}")
+    assert err <= TOL_PHI
+
+
+def test_reinit_iterations_match_time_loop(oracle):
+    lib = A.load()
+    for eps in (1.0 / 128, 0.04 / 128, 1.0 / 2048):
+        for mt in (1.25 * eps, 2.25 * eps, 1.5 * 1.5 * eps, 256 * eps):
+            assert lib.asd2_ls_reinit_iterations(eps, mt) == oracle.reinit_iterations(eps, mt)
+
+
+def test_periodic_p_refused():
+    ctx = _ctx(16, 16, 0.1, 0.1, ("periodic",) * 4)
+    d = ctx.array(np.zeros(ctx.shape))
+    with pytest.raises(A.Asd2Error) as e:
+        ctx.ls_advect(d, ctx.array(), ctx.array(), 0.01)
+    assert e.value.code == A.ERR_UNSUPPORTED
