@@ -24,6 +24,7 @@ GRID_P, GRID_U, GRID_V = 0, 1, 2
 CG_REFERENCE, CG_STANDARD = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 PRECOND_JACOBI, PRECOND_LEARNED = 0, 1
+MOM_RHS, MOM_CONVECTION, MOM_VISCOSITY = 0, 1, 2
 KID = {"weno_stage": 1, "reinit": 2, "pcg64_spmv": 3, "pcg64_update": 4, "pcg32_spmv": 5,
        "pcg32_update": 6, "gfm": 7, "unet": 8}
 
@@ -122,6 +123,14 @@ def load() -> C.CDLL:
         "asd2_poisson_solve": ([_VP, P(SolveOpts), _VP, P(SolveStats)], C.c_int),
         "asd2_poisson_bicgstab": ([_VP, P(SolveOpts), _VP, P(SolveStats)], C.c_int),
         "asd2_csr_pcg": ([_VP, C.c_int, _VP, _VP, _VP, _VP, _VP, _VP, P(SolveOpts), P(SolveStats)], C.c_int),
+        "asd2_csr_bicgstab": ([_VP, C.c_int, _VP, _VP, _VP, _VP, _VP, _VP, P(SolveOpts), P(SolveStats)],
+                              C.c_int),
+        "asd2_ls_reinit_original": ([_VP, _VP, C.c_int], C.c_int),
+        "asd2_ls_heaviside_deriv": ([_VP, _VP, _VP], C.c_int),
+        "asd2_ls_smoothed_sign": ([_VP, _VP, _VP], C.c_int),
+        "asd2_ls_sussman_constraint": ([_VP, _VP, _VP, _VP, _VP], C.c_int),
+        "asd2_ls_unit_weno5": ([_VP, _VP, _VP, _VP, C.c_double, C.c_int, C.c_int], C.c_int),
+        "asd2_mac_momentum_term": ([_VP, P(Fluid), C.c_int, C.c_int, _VP, _VP, _VP, _VP, _VP], C.c_int),
         "asd2_timer_start": ([_VP], C.c_int),
         "asd2_timer_stop": ([_VP, P(C.c_double)], C.c_int),
         "asd2_profile_kernel": ([_VP, C.c_int], C.c_int),
@@ -264,6 +273,26 @@ class Context:
     def ls_eno_absgrad(self, ls, ls0, out):
         check(self.lib.asd2_ls_eno_absgrad(self.h, ls.ptr, ls0.ptr, out.ptr))
 
+    def ls_reinit_original(self, ls, iters):
+        check(self.lib.asd2_ls_reinit_original(self.h, ls.ptr, iters))
+
+    def ls_heaviside_deriv(self, ls, out):
+        check(self.lib.asd2_ls_heaviside_deriv(self.h, ls.ptr, out.ptr))
+
+    def ls_smoothed_sign(self, ls, out):
+        check(self.lib.asd2_ls_smoothed_sign(self.h, ls.ptr, out.ptr))
+
+    def ls_sussman_constraint(self, ls, ls0, hd, out):
+        check(self.lib.asd2_ls_sussman_constraint(self.h, ls.ptr, ls0.ptr, hd.ptr if hd is not None else None,
+                                                  out.ptr))
+
+    def ls_unit_weno5(self, F, d, n, g):
+        """One line of n + 2g values -> (FP, FM) on the same padded line."""
+        dF = DeviceArray.from_host(np.asarray(F, np.float64))
+        dP, dM = DeviceArray((n + 2 * g,)), DeviceArray((n + 2 * g,))
+        check(self.lib.asd2_ls_unit_weno5(self.h, dF.ptr, dP.ptr, dM.ptr, d, n, g))
+        return dP.download(), dM.download()
+
     # MAC ---------------------------------------------------------------------------
     def kappa(self, ls, kap):
         check(self.lib.asd2_mac_kappa(self.h, ls.ptr, kap.ptr))
@@ -286,6 +315,10 @@ class Context:
     def intermediate_vel(self, fl, order, ls, u, v, uhat, vhat, Hs):
         check(self.lib.asd2_mac_intermediate_vel(self.h, C.byref(fl), order, ls.ptr, u.ptr, v.ptr,
                                                  uhat.ptr, vhat.ptr, Hs.ptr))
+
+    def momentum_term(self, fl, grid_kind, term, ls, u, v, Hs, out):
+        check(self.lib.asd2_mac_momentum_term(self.h, C.byref(fl), grid_kind, term, ls.ptr, u.ptr, v.ptr,
+                                              Hs.ptr, out.ptr))
 
     # GFM / Poisson -----------------------------------------------------------------
     def gfm_assemble(self, fl, ls, kap, H, div):
@@ -324,8 +357,9 @@ class Context:
             check(code)
         return Stats(**st.as_dict())
 
-    def csr_pcg(self, rowptr, cols, vals, mdiag, b, opts: SolveOpts):
+    def csr_pcg(self, rowptr, cols, vals, mdiag, b, opts: SolveOpts, bicgstab=False):
         n = len(b)
+        fn = self.lib.asd2_csr_bicgstab if bicgstab else self.lib.asd2_csr_pcg
         d_rp = DeviceArray.from_host(np.asarray(rowptr, np.int32))
         d_c = DeviceArray.from_host(np.asarray(cols, np.int32))
         d_v = DeviceArray.from_host(np.asarray(vals, np.float64))
@@ -333,8 +367,7 @@ class Context:
         d_b = DeviceArray.from_host(np.asarray(b, np.float64))
         d_x = DeviceArray((n,))
         st = SolveStats()
-        check(self.lib.asd2_csr_pcg(self.h, n, d_rp.ptr, d_c.ptr, d_v.ptr, d_m.ptr, d_b.ptr, d_x.ptr,
-                                    C.byref(opts), C.byref(st)))
+        check(fn(self.h, n, d_rp.ptr, d_c.ptr, d_v.ptr, d_m.ptr, d_b.ptr, d_x.ptr, C.byref(opts), C.byref(st)))
         return d_x.download(), Stats(**st.as_dict())
 
     # timing --------------------------------------------------------------------------

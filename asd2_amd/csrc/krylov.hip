@@ -455,6 +455,34 @@ __global__ __launch_bounds__(KB) void k_csr_spmv(int n, const int* __restrict__ 
     }
 }
 
+// CSR y = A x with the BiCGStab dot products of k_stencil_dot (phase 1: alpha, phase 2: omega)
+__global__ __launch_bounds__(KB) void k_csr_dot(int n, const int* __restrict__ rp, const int* __restrict__ cols,
+                                                const double* __restrict__ vals, const double* __restrict__ x,
+                                                double* __restrict__ y, const double* __restrict__ w,
+                                                KrylovState* st, double* partials, int phase) {
+    if (st->done != DONE_RUNNING) return;
+    if (phase == 2 && (st->flags & 1)) return;
+    double a0 = 0.0, a1 = 0.0;
+    for (int row = blockIdx.x * KB + threadIdx.x; row < n; row += gridDim.x * KB) {
+        double s = 0.0;
+        for (int e = rp[row]; e < rp[row + 1]; e++) s += vals[e] * x[cols[e]];
+        y[row] = s;
+        a0 += w[row] * s;
+        a1 += s * s;
+    }
+    double v[2] = {a0, a1};
+    block_sum(v);
+    if (publish(v, partials, &st->ticket[1]) && threadIdx.x == 0) {
+        if (phase == 1) {
+            st->alpha = st->rho / v[0];
+            if (!isfinite(st->alpha)) st->done = DONE_BREAKDOWN;
+        } else {
+            st->omega = v[0] / v[1];
+            if (!isfinite(st->omega)) st->done = DONE_BREAKDOWN;
+        }
+    }
+}
+
 __global__ void k_recip(int64_t n, const double* __restrict__ d, double* __restrict__ di) {
     for (int64_t k = blockIdx.x * static_cast<int64_t>(KB) + threadIdx.x; k < n;
          k += static_cast<int64_t>(gridDim.x) * KB)
@@ -791,6 +819,54 @@ int asd2_csr_pcg(asd2_ctx* ctx, int n, const int* d_rowptr, const int* d_cols, c
     };
     KrylovState fs;
     ASD2_TRY(drive(&D, o->check_every > 0 ? o->check_every : 32, batch, []() { return ASD2_OK; }, false, &fs));
+    fs.aux2 = -1.0;
+    fill_stats(fs, stats);
+    return status_of(fs);
+}
+
+int asd2_csr_bicgstab(asd2_ctx* ctx, int n, const int* d_rowptr, const int* d_cols, const double* d_vals,
+                      const double* d_mdiag, const double* d_b, double* d_x, const asd2_solve_opts* o,
+                      asd2_solve_stats* stats) {
+    ASD2_TRY(check_ctx(ctx));
+    if (n <= 0 || !d_rowptr || !d_cols || !d_vals || !d_mdiag || !d_b || !d_x || !o) {
+        set_error("bad CSR argument");
+        return ASD2_ERR_ARG;
+    }
+    double *r, *rt, *p, *v, *ph, *s, *sh, *t, *dinv;
+    ASD2_TRY(ws(ctx, "csr_r", n, &r));
+    ASD2_TRY(ws(ctx, "csr_rt", n, &rt));
+    ASD2_TRY(ws(ctx, "csr_p", n, &p));
+    ASD2_TRY(ws(ctx, "csr_v", n, &v));
+    ASD2_TRY(ws(ctx, "csr_ph", n, &ph));
+    ASD2_TRY(ws(ctx, "csr_s", n, &s));
+    ASD2_TRY(ws(ctx, "csr_sh", n, &sh));
+    ASD2_TRY(ws(ctx, "csr_t", n, &t));
+    ASD2_TRY(ws(ctx, "csr_dinv", n, &dinv));
+    const int blocks = static_cast<int>(std::min<int64_t>(FLAT_BLOCKS, cdiv(n, KB)));
+    const int64_t n64 = n;
+    Driver D;
+    ASD2_TRY(driver_init(ctx, &D));
+    ASD2_TRY(set_state(&D, o));
+    hipLaunchKernelGGL(k_recip, dim3(blocks), dim3(KB), 0, ctx->stream, n64, d_mdiag, dinv);
+    hipLaunchKernelGGL(k_bicg_init, dim3(blocks), dim3(KB), 0, ctx->stream, n64, d_b, d_x, r, rt, D.st, D.partials);
+    ASD2_LAUNCH_CHECK();
+    auto batch = [&](int count) -> int {
+        for (int k = 0; k < count; k++) {
+            hipLaunchKernelGGL(k_bicg_dir, dim3(blocks), dim3(KB), 0, ctx->stream, n64, r, p, v, dinv, ph, D.st);
+            hipLaunchKernelGGL(k_csr_dot, dim3(blocks), dim3(KB), 0, ctx->stream, n, d_rowptr, d_cols, d_vals, ph, v,
+                               rt, D.st, D.partials, 1);
+            hipLaunchKernelGGL(k_bicg_s, dim3(blocks), dim3(KB), 0, ctx->stream, n64, r, v, dinv, s, sh, D.st,
+                               D.partials);
+            hipLaunchKernelGGL(k_csr_dot, dim3(blocks), dim3(KB), 0, ctx->stream, n, d_rowptr, d_cols, d_vals, sh, t,
+                               s, D.st, D.partials, 2);
+            hipLaunchKernelGGL(k_bicg_x, dim3(blocks), dim3(KB), 0, ctx->stream, n64, d_x, ph, sh, r, s, t, rt, D.st,
+                               D.partials);
+        }
+        ASD2_LAUNCH_CHECK();
+        return ASD2_OK;
+    };
+    KrylovState fs;
+    ASD2_TRY(drive(&D, o->check_every > 0 ? o->check_every : 16, batch, []() { return ASD2_OK; }, false, &fs));
     fs.aux2 = -1.0;
     fill_stats(fs, stats);
     return status_of(fs);

@@ -339,8 +339,11 @@ __device__ __noinline__ double sussman_correction(const double (*T)[SW], const d
     return 0.0 - adt * f[1][1] * s1 / s2;
 }
 
+// add = 1: ls += correction (in place) with ghosts rewritten (one Sussman iteration);
+// add = 0: out = correction, with the mask taken from hd (GetSussmanReinitConstraint).
 __global__ __launch_bounds__(WTH) void k_sussman_constraint(Geom G, GhostRule R, const double* __restrict__ tmp,
-                                                            const double* __restrict__ ls0, double* ls, int ghosts) {
+                                                            const double* __restrict__ ls0, const double* hd,
+                                                            double* out, int add, int ghosts) {
     __shared__ double sT[SI + 2 * SHL][SW];
     __shared__ double sI[SI + 2 * SHL][SW];
     const int j0 = G.g + blockIdx.x * SJ, i0 = G.g + blockIdx.y * SI;
@@ -358,21 +361,50 @@ __global__ __launch_bounds__(WTH) void k_sussman_constraint(Geom G, GhostRule R,
     for (int r = ti; r < SI; r += WTH / SJ) {
         const int i = i0 + r, j = j0 + tj;
         if (i >= G.nx + G.g || j >= G.ny + G.g) continue;
-        const double t = sT[r + SHL][tj + SHL];
-        double cons = 0.0;
-        if (!(fabs(t) > eps)) {
-            double hd;
-            {
-#pragma clang fp contract(off)
-                hd = 0.5 * (1.0 / eps + cos(M_PI * t / eps));
-            }
-            if (hd != 0.0) cons = sussman_correction(sT, sI, r + SHL, tj + SHL, G.dx, G.dy);
-        }
         const int64_t c = G.idx(i, j);
-        const double val = ls[c] + cons;
-        ls[c] = val;
-        if (ghosts) write_p_ghosts(G, R, ls, i, j, val);
+        const double t = sT[r + SHL][tj + SHL];
+        double mask;
+        if (hd) mask = hd[c];
+        else if (fabs(t) > eps) mask = 0.0;
+        else {
+#pragma clang fp contract(off)
+            mask = 0.5 * (1.0 / eps + cos(M_PI * t / eps));
+        }
+        const double cons = (mask != 0.0) ? sussman_correction(sT, sI, r + SHL, tj + SHL, G.dx, G.dy) : 0.0;
+        if (add) {
+            const double val = out[c] + cons;
+            out[c] = val;
+            if (ghosts) write_p_ghosts(G, R, out, i, j, val);
+        } else {
+            out[c] = cons;
+        }
     }
+}
+
+// UpdateHeavisideFuncDeriv (what = 0) and GetSmoothedSignFunc (what = 1) over the padded array
+__global__ __launch_bounds__(256) void k_pointwise(int64_t n, int what, double eps, const double* __restrict__ in,
+                                                   double* __restrict__ out) {
+#pragma clang fp contract(off)
+    for (int64_t k = blockIdx.x * 256ll + threadIdx.x; k < n; k += static_cast<int64_t>(gridDim.x) * 256) {
+        const double x = in[k];
+        double y;
+        if (what == 0) y = (fabs(x) > eps) ? 0.0 : 0.5 * (1.0 / eps + cos(M_PI * x / eps));
+        else y = smoothed_sign(x, eps);
+        out[k] = y;
+    }
+}
+
+// one WENO5 line: FP, FM on [g, n + g) from F[0, n + 2g)
+__global__ __launch_bounds__(256) void k_unit_weno5(const double* __restrict__ F, double* __restrict__ FP,
+                                                    double* __restrict__ FM, double d, int n, int g) {
+    const int i = g + blockIdx.x * 256 + threadIdx.x;
+    if (i >= n + g) return;
+    double f[7];
+    for (int k = 0; k < 7; k++) f[k] = F[i - 3 + k];
+    double fp, fm;
+    weno5(f[1] - f[0], f[2] - f[1], f[3] - f[2], f[4] - f[3], f[5] - f[4], f[6] - f[5], 1.0 / d, 1e-6 * d * d, fp, fm);
+    FP[i] = fp;
+    FM[i] = fm;
 }
 
 struct Launch {
@@ -405,6 +437,8 @@ int p_rule(asd2_ctx* ctx, GhostRule* R, bool* fused) {
     *R = ghost_rule_p(ctx, fused);
     return ASD2_OK;
 }
+
+double smin_host(double a, double b) { return (b < a) ? b : a; }
 
 bool has_custom(const asd2_ctx* ctx) {
     for (int s = 0; s < 4; s++)
@@ -576,10 +610,80 @@ int asd2_ls_reinit_sussman(asd2_ctx* ctx, double* d_ls, int iterations) {
         ASD2_LAUNCH_CHECK();
         ASD2_TRY(finish_ghosts(ctx, fused, tmp));
         hipLaunchKernelGGL(k_sussman_constraint, dim3(cdiv(G.ny, SJ), cdiv(G.nx, SI)), dim3(WTH), 0, ctx->stream,
-                           G, R, tmp, ls0, d_ls, gh);
+                           G, R, tmp, ls0, static_cast<const double*>(nullptr), d_ls, 1, gh);
         ASD2_LAUNCH_CHECK();
         ASD2_TRY(finish_ghosts(ctx, fused, d_ls));
     }
+    return ASD2_OK;
+}
+
+// Reinit_Original_2D: Sussman's ENO2 substages with BC on LS1 and no constraint; lsInit is
+// copied before the first BC (src/levelset2d.cpp:264-316)
+int asd2_ls_reinit_original(asd2_ctx* ctx, double* d_ls, int iterations) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!d_ls || iterations < 0) { set_error("bad argument"); return ASD2_ERR_ARG; }
+    GhostRule R;
+    bool fused;
+    ASD2_TRY(p_rule(ctx, &R, &fused));
+    const Geom& G = ctx->geo;
+    const size_t n = static_cast<size_t>(G.size());
+    double *ls0, *ls1;
+    ASD2_TRY(ws(ctx, "ri_ls0", n, &ls0));
+    ASD2_TRY(ws(ctx, "ri_ls1", n, &ls1));
+    ASD2_TRY(zero_async(ctx, ls1, n * 8));
+    ASD2_HIP(hipMemcpyAsync(ls0, d_ls, n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    ASD2_TRY(bc_apply_p(ctx, d_ls));
+    const Launch L = tiles(G, RJ, RI);
+    const int gh = fused ? 1 : 0;
+    for (int it = 0; it < iterations; it++) {
+        hipLaunchKernelGGL((k_reinit<false, 0>), L.grid, L.block, 0, ctx->stream, G, R, d_ls, ls0, d_ls, ls1, gh);
+        ASD2_LAUNCH_CHECK();
+        ASD2_TRY(finish_ghosts(ctx, fused, ls1));
+        hipLaunchKernelGGL((k_reinit<false, 1>), L.grid, L.block, 0, ctx->stream, G, R, ls1, ls0, d_ls, d_ls, gh);
+        ASD2_LAUNCH_CHECK();
+        ASD2_TRY(finish_ghosts(ctx, fused, d_ls));
+    }
+    return ASD2_OK;
+}
+
+int asd2_ls_heaviside_deriv(asd2_ctx* ctx, const double* d_ls, double* d_out) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!d_ls || !d_out) { set_error("null array"); return ASD2_ERR_ARG; }
+    const Geom& G = ctx->geo;
+    hipLaunchKernelGGL(k_pointwise, dim3(1024), dim3(256), 0, ctx->stream, G.size(), 0, smin_host(G.dx, G.dy),
+                       d_ls, d_out);
+    ASD2_LAUNCH_CHECK();
+    return ASD2_OK;
+}
+
+int asd2_ls_smoothed_sign(asd2_ctx* ctx, const double* d_ls, double* d_out) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!d_ls || !d_out) { set_error("null array"); return ASD2_ERR_ARG; }
+    const Geom& G = ctx->geo;
+    hipLaunchKernelGGL(k_pointwise, dim3(1024), dim3(256), 0, ctx->stream, G.size(), 1, smin_host(G.dx, G.dy),
+                       d_ls, d_out);
+    ASD2_LAUNCH_CHECK();
+    return ASD2_OK;
+}
+
+int asd2_ls_sussman_constraint(asd2_ctx* ctx, const double* d_ls, const double* d_ls0, const double* d_hd,
+                               double* d_out) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!d_ls || !d_ls0 || !d_hd || !d_out) { set_error("null array"); return ASD2_ERR_ARG; }
+    const Geom& G = ctx->geo;
+    ASD2_TRY(zero_async(ctx, d_out, static_cast<size_t>(G.size()) * 8));
+    GhostRule R{};
+    hipLaunchKernelGGL(k_sussman_constraint, dim3(cdiv(G.ny, SJ), cdiv(G.nx, SI)), dim3(WTH), 0, ctx->stream, G, R,
+                       d_ls, d_ls0, d_hd, d_out, 0, 0);
+    ASD2_LAUNCH_CHECK();
+    return ASD2_OK;
+}
+
+int asd2_ls_unit_weno5(asd2_ctx* ctx, const double* d_F, double* d_FP, double* d_FM, double d, int n, int g) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!d_F || !d_FP || !d_FM || n <= 0 || g != 3 || !(d > 0)) { set_error("bad argument"); return ASD2_ERR_ARG; }
+    hipLaunchKernelGGL(k_unit_weno5, dim3(cdiv(n, 256)), dim3(256), 0, ctx->stream, d_F, d_FP, d_FM, d, n, g);
+    ASD2_LAUNCH_CHECK();
     return ASD2_OK;
 }
 

@@ -172,6 +172,7 @@ struct MomArgs {
     int gv_from_i;    // V gravity only where i >= g + 1 (the U-grid array of mac2d.cpp:294)
     int viscous;      // kRe > 0
     int mode;         // 0: s + dt F; 1: 0.75 u0 + 0.25 (s + dt F); 2: u0/3 + 2/3 (s + dt F); 3: u0/2 + (s + dt F)/2
+                      // 4: F; 5: convection term; 6: viscous term
 };
 
 // tmpV of AddConvectionFU at a U face (src/mac2d.cpp:314-316), 0 elsewhere
@@ -267,7 +268,10 @@ __global__ __launch_bounds__(256) void k_mom_u(Geom G, MomArgs A, const double* 
     if (A.mode == 0) o = t;
     else if (A.mode == 1) o = 0.75 * u0[c] + 0.25 * t;
     else if (A.mode == 2) o = 1.0 / 3.0 * u0[c] + 2.0 / 3.0 * t;
-    else o = 0.5 * u0[c] + 0.5 * t;
+    else if (A.mode == 3) o = 0.5 * u0[c] + 0.5 * t;
+    else if (A.mode == 4) o = F;
+    else if (A.mode == 5) o = cU;
+    else o = dU;
     out[c] = o;
 }
 
@@ -325,7 +329,10 @@ __global__ __launch_bounds__(256) void k_mom_v(Geom G, MomArgs A, const double* 
     if (A.mode == 0) o = t;
     else if (A.mode == 1) o = 0.75 * v0[c] + 0.25 * t;
     else if (A.mode == 2) o = 1.0 / 3.0 * v0[c] + 2.0 / 3.0 * t;
-    else o = 0.5 * v0[c] + 0.5 * t;
+    else if (A.mode == 3) o = 0.5 * v0[c] + 0.5 * t;
+    else if (A.mode == 4) o = F;
+    else if (A.mode == 5) o = cV;
+    else o = dV;
     out[c] = o;
 }
 
@@ -457,6 +464,34 @@ int asd2_mac_intermediate_vel(asd2_ctx* ctx, const asd2_fluid* fl, int time_orde
         ASD2_TRY(bc_apply_uv(ctx, ASD2_GRID_V, v2));
         ASD2_TRY(stage(u2, v2, d_u, d_v, d_uhat, d_vhat, 2));
     }
+    return ASD2_OK;
+}
+
+int asd2_mac_momentum_term(asd2_ctx* ctx, const asd2_fluid* fl, int grid_kind, int term, const double* d_ls,
+                           const double* d_u, const double* d_v, const double* d_Hs, double* d_out) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!fl || !d_ls || !d_u || !d_v || !d_Hs || !d_out || term < 0 || term > 2 ||
+        (grid_kind != ASD2_GRID_U && grid_kind != ASD2_GRID_V)) {
+        set_error("bad argument");
+        return ASD2_ERR_ARG;
+    }
+    const Geom& G = ctx->geo;
+    MomArgs A{};
+    A.rho_h = fl->rho_h; A.rho_l = fl->rho_l; A.mu_h = fl->mu_h; A.mu_l = fl->mu_l; A.dt = fl->dt;
+    const bool fr_ok = (fl->fr != 0 || fl->g != 0.0) && !std::isnan(fl->fr) && !std::isinf(fl->fr);
+    A.gu = (fr_ok && fl->g_axis == 0) ? -fl->g : 0.0;
+    if (fl->gravity_fix) { A.gv = (fr_ok && fl->g_axis == 1) ? -fl->g : 0.0; A.gv_from_i = 0; }
+    else { A.gv = A.gu; A.gv_from_i = 1; }
+    A.viscous = fl->re > 0.0 ? 1 : 0;
+    A.mode = 4 + term;
+    ASD2_TRY(zero_async(ctx, d_out, static_cast<size_t>(G.size()) * 8));
+    if (grid_kind == ASD2_GRID_U)
+        hipLaunchKernelGGL(k_mom_u, dim3(cdiv(G.ny, 64), cdiv(G.nx - 1, 4)), dim3(256), 0, ctx->stream, G, A, d_u,
+                           d_v, d_ls, d_Hs, d_u, d_out);
+    else
+        hipLaunchKernelGGL(k_mom_v, dim3(cdiv(G.ny - 1, 64), cdiv(G.nx, 4)), dim3(256), 0, ctx->stream, G, A, d_u,
+                           d_v, d_ls, d_Hs, d_v, d_out);
+    ASD2_LAUNCH_CHECK();
     return ASD2_OK;
 }
 

@@ -140,6 +140,18 @@ int asd2_ls_subcell_absgrad(asd2_ctx* ctx, const double* d_ls, const double* d_l
 int asd2_ls_eno_absgrad(asd2_ctx* ctx, const double* d_ls, const double* d_ls0, double* d_out);
 /* the pseudo-time loop count of src/levelset2d.cpp:447-448 (host only) */
 int asd2_ls_reinit_iterations(double eps, double max_time);
+/* LevelSetSolver2D::Reinit_Original_2D, src/levelset2d.cpp:264-316 (ENO2, sign(phi0)) */
+int asd2_ls_reinit_original(asd2_ctx* ctx, double* d_ls, int iterations);
+/* LevelSetSolver2D::UpdateHeavisideFuncDeriv, src/levelset2d.cpp:27-41 (padded array) */
+int asd2_ls_heaviside_deriv(asd2_ctx* ctx, const double* d_ls, double* d_out);
+/* LevelSetSolver2D::GetSmoothedSignFunc, src/levelset2d.cpp:914-923 (padded array) */
+int asd2_ls_smoothed_sign(asd2_ctx* ctx, const double* d_ls, double* d_out);
+/* LevelSetSolver2D::GetSussmanReinitConstraint, src/levelset2d.cpp:787-893 */
+int asd2_ls_sussman_constraint(asd2_ctx* ctx, const double* d_ls, const double* d_ls0, const double* d_hd,
+                               double* d_out);
+/* LevelSetSolver2D::UnitHJWENO5 on one line of n + 2g values, src/levelset2d.cpp:159-262
+ * (FP, FM written on [g, n + g)) */
+int asd2_ls_unit_weno5(asd2_ctx* ctx, const double* d_F, double* d_FP, double* d_FM, double d, int n, int g);
 
 /* ---------------------------------------------------------------- MAC helpers */
 /* MACSolver2D::UpdateKappa, src/mac2d.cpp:221-252 (writes the interior) */
@@ -161,6 +173,15 @@ int asd2_mac_update_dt(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_u, c
 int asd2_mac_intermediate_vel(asd2_ctx* ctx, const asd2_fluid* fl, int time_order,
                               const double* d_ls, const double* d_u, const double* d_v,
                               double* d_uhat, double* d_vhat, const double* d_Hs);
+/* one momentum term on the U (grid_kind = ASD2_GRID_U) or V faces, interior faces only:
+ * term 0 = UpdateFU/FV (-convection + viscosity + gravity, src/mac2d.cpp:254-306),
+ * 1 = AddConvectionFU/FV (:308-463), 2 = AddViscosityFU/FV (:570-1041) */
+#define ASD2_MOM_RHS 0
+#define ASD2_MOM_CONVECTION 1
+#define ASD2_MOM_VISCOSITY 2
+int asd2_mac_momentum_term(asd2_ctx* ctx, const asd2_fluid* fl, int grid_kind, int term,
+                           const double* d_ls, const double* d_u, const double* d_v,
+                           const double* d_Hs, double* d_out);
 
 /* ---------------------------------------------------------------- GFM pressure Poisson */
 /* Assembly half of MACSolver2D::SolvePoisson, src/mac2d.cpp:1249-1656: face
@@ -190,6 +211,11 @@ int asd2_poisson_bicgstab(asd2_ctx* ctx, const asd2_solve_opts* opts, double* d_
 int asd2_csr_pcg(asd2_ctx* ctx, int n, const int* d_rowptr, const int* d_cols,
                  const double* d_vals, const double* d_mdiag, const double* d_b, double* d_x,
                  const asd2_solve_opts* opts, asd2_solve_stats* stats);
+/* BiCGStab_2FUniform_2D on a caller-built zero-based CSR matrix (src/poisson2d.cpp:403-575,
+ * textbook recurrence), Jacobi preconditioner from d_mdiag, x0 = 0 */
+int asd2_csr_bicgstab(asd2_ctx* ctx, int n, const int* d_rowptr, const int* d_cols,
+                      const double* d_vals, const double* d_mdiag, const double* d_b, double* d_x,
+                      const asd2_solve_opts* opts, asd2_solve_stats* stats);
 
 /* ---------------------------------------------------------------- timing */
 /* records the start/stop of a region on the context's stream (hipEvents) */

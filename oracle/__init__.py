@@ -107,6 +107,7 @@ class Oracle:
             "or_smoothed_sign": [G, _P, _P], "or_heaviside_deriv": [G, _P, _P],
             "or_sussman_constraint": [G, _P, _P, _P, _P],
             "or_reinit_minrk2": [G, _P, C.c_int], "or_reinit_sussman": [G, _P, C.c_int],
+            "or_reinit_original": [G, _P, C.c_int],
             "or_update_kappa": [G, _P, _P], "or_heaviside": [G, _P, _P],
             "or_smooth_heaviside": [G, _P, _P], "or_divergence": [G, _P, _P, _P],
             "or_update_vel": [G, F, _P, _P, _P, _P, _P, _P, _P, _P],
@@ -179,6 +180,9 @@ class Oracle:
 
     def reinit_sussman(self, gr, ls, iters):
         self.lib.or_reinit_sussman(C.byref(gr), _ptr(ls), iters); return ls
+
+    def reinit_original(self, gr, ls, iters):
+        self.lib.or_reinit_original(C.byref(gr), _ptr(ls), iters); return ls
 
     def reinit_iterations(self, eps, max_time):
         return self.lib.or_reinit_iterations(eps, max_time)
@@ -278,6 +282,7 @@ class Reference:
             "ref_smoothed_sign": [G, _P, _P], "ref_heaviside_deriv": [G, _P, _P],
             "ref_sussman_constraint": [G, _P, _P, _P, _P],
             "ref_reinit_minrk2": [G, _P, C.c_double], "ref_reinit_sussman": [G, _P, C.c_double],
+            "ref_reinit_original": [G, _P, C.c_double],
         }
         for name, args in sig.items():
             fn = getattr(self.lib, name)
@@ -332,3 +337,6 @@ class Reference:
 
     def reinit_sussman(self, gr, ls, max_time):
         self.lib.ref_reinit_sussman(C.byref(gr), _ptr(ls), max_time); return ls
+
+    def reinit_original(self, gr, ls, max_time):
+        self.lib.ref_reinit_original(C.byref(gr), _ptr(ls), max_time); return ls

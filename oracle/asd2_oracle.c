@@ -594,6 +594,41 @@ void or_reinit_minrk2(const or_grid* gr, double* ls, int iters) {
     free(ls0); free(S); free(a1); free(a2); free(LS1); free(LS2); free(tmp);
 }
 
+/* src/levelset2d.cpp:264-316: Heun with plain ENO2 and the integer sign of the input
+ * (captured before any BC), BC on every stage; the Heaviside-derivative mask computed there
+ * (:307) does not change the result (its test is commented out). */
+void or_reinit_original(const or_grid* gr, double* ls, int iters) {
+    const int nx = gr->nx, ny = gr->ny, g = gr->g;
+    const double eps = smin(gr->dx, gr->dy), adt = eps * 0.5;
+    const int64_t n = or_size(gr);
+    double* ls0 = zalloc(gr); double* a1 = zalloc(gr); double* a2 = zalloc(gr);
+    double* LS1 = zalloc(gr); double* LS2 = zalloc(gr);
+    memcpy(ls0, ls, sizeof(double) * (size_t)n);
+    for (int it = 0; it < iters; it++) {
+        or_bc_p(gr, ls);
+        or_eno_absgrad(gr, ls, ls0, a1);
+        for (int i = g; i < nx + g; i++)
+            for (int j = g; j < ny + g; j++) {
+                const int64_t c = IDX(gr, i, j);
+                LS1[c] = ls[c] - adt * isign(ls0[c]) * (a1[c] - 1.0);
+            }
+        or_bc_p(gr, LS1);
+        or_eno_absgrad(gr, LS1, ls0, a2);
+        for (int i = g; i < nx + g; i++)
+            for (int j = g; j < ny + g; j++) {
+                const int64_t c = IDX(gr, i, j);
+                LS2[c] = LS1[c] - adt * isign(ls0[c]) * (a2[c] - 1.0);
+            }
+        for (int i = g; i < nx + g; i++)
+            for (int j = g; j < ny + g; j++) {
+                const int64_t c = IDX(gr, i, j);
+                ls[c] = 0.5 * (ls[c] + LS2[c]);
+            }
+    }
+    or_bc_p(gr, ls);
+    free(ls0); free(a1); free(a2); free(LS1); free(LS2);
+}
+
 /* src/levelset2d.cpp:318-375 (LS1 gets no BC: its ghosts stay 0) */
 void or_reinit_sussman(const or_grid* gr, double* ls, int iters) {
     const int nx = gr->nx, ny = gr->ny, g = gr->g;

@@ -11,8 +11,8 @@
  * (src/levelset2d.cpp:1048-1050).  BC kinds follow enum BC2D
  * (include/common.h:5).
  *
- * Parity status of each function (how it is pinned) is listed in DESIGN.md
- * and in the header comment of asd2_oracle.c.
+ * How each function is pinned (golden vectors, the compiled reference, closed
+ * forms) is listed in the header comment of asd2_oracle.c.
  */
 #ifndef ASD2_ORACLE_H
 #define ASD2_ORACLE_H
@@ -65,6 +65,7 @@ void or_heaviside_deriv(const or_grid* gr, const double* ls, double* out);
 void or_sussman_constraint(const or_grid* gr, const double* ls, const double* ls0,
                            const double* hd, double* out);
 void or_reinit_minrk2(const or_grid* gr, double* ls, int iters);
+void or_reinit_original(const or_grid* gr, double* ls, int iters);
 void or_reinit_sussman(const or_grid* gr, double* ls, int iters);
 int  or_reinit_iterations(double eps, double max_time);
 

@@ -149,6 +149,14 @@ int ref_reinit_minrk2(const or_grid* gr, double* ls, double max_time) {
     return 0;
 }
 
+int ref_reinit_original(const or_grid* gr, double* ls, double max_time) {
+    auto s = make_ls(gr, max_time);
+    std::vector<double> l = vec(ls, grid_size(gr));
+    s->Reinit_Original_2D(l);
+    out(l, ls);
+    return 0;
+}
+
 int ref_reinit_sussman(const or_grid* gr, double* ls, double max_time) {
     auto s = make_ls(gr, max_time);
     std::vector<double> l = vec(ls, grid_size(gr));

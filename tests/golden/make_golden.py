@@ -93,6 +93,8 @@ def main():
                                     out=R.reinit_minrk2(gr, phi.copy(), (K - 0.5) * 0.5 * dx))
     phi = level_set(nx, ny, dx, 120)
     cases["sussman_5"] = dict(grid=gr, phi=phi, iters=np.array(5), out=R.reinit_sussman(gr, phi.copy(), 2.25 * dx))
+    cases["original_5"] = dict(grid=gr, phi=phi, iters=np.array(5),
+                               out=R.reinit_original(gr, phi.copy(), 2.25 * dx))
 
     # a 64^2 droplet (config-1 geometry) after one advection step + 3 MinRK2 iterations
     n64 = 64

@@ -150,6 +150,67 @@ def test_reinit_sussman(oracle, nx, ny):
     assert err <= TOL_PHI
 
 
+@pytest.mark.parametrize("bc", P_BCS[:3], ids=lambda b: "-".join(b))
+def test_reinit_original(oracle, bc):
+    nx, ny = 67, 45
+    phi, dx = _level_set(nx, ny, 12)
+    phi = np.ascontiguousarray(phi * (1.0 + 0.25 * np.sin(3 * phi / dx)))
+    gr = make_grid(nx, ny, dx, dx, bc_p=bc, c_p=(0.02, -0.01, 0.0, 0.03))
+    ref = oracle.reinit_original(gr, phi.copy(), 5)
+    ctx = _ctx(nx, ny, dx, dx, bc, (0.02, -0.01, 0.0, 0.03))
+    d = ctx.array(phi)
+    ctx.ls_reinit_original(d, 5)
+    err = np.abs(d.download() - ref).max()
+    print(f"original {bc}: max|dphi| = {err:.3e}")
+    assert err <= TOL_PHI
+
+
+def test_pointwise_sign_and_delta(oracle):
+    nx, ny = 50, 41
+    phi, dx = _level_set(nx, ny, 13)
+    gr = make_grid(nx, ny, dx, dx)
+    ctx = _ctx(nx, ny, dx, dx)
+    d_phi, d_out = ctx.array(phi), ctx.array()
+    ctx.ls_smoothed_sign(d_phi, d_out)
+    ref = oracle.smoothed_sign(gr, phi)
+    assert np.abs(d_out.download() - ref).max() <= 1e-15
+    ctx.ls_heaviside_deriv(d_phi, d_out)
+    ref = oracle.heaviside_deriv(gr, phi)
+    out = d_out.download()
+    np.testing.assert_array_equal(out == 0.0, ref == 0.0)  # the mask the constraint uses
+    assert np.abs(out - ref).max() <= 1e-14 * np.abs(ref).max()
+
+
+def test_sussman_constraint(oracle):
+    nx, ny = 47, 52
+    phi, dx = _level_set(nx, ny, 14)
+    gr = make_grid(nx, ny, dx, dx)
+    oracle.bc_p(gr, phi)
+    phi0 = phi.copy()
+    phi1 = np.ascontiguousarray(phi + 0.2 * dx * S.random_field(15, phi.shape))
+    oracle.bc_p(gr, phi1)
+    hd = oracle.heaviside_deriv(gr, phi1)
+    ref = oracle.sussman_constraint(gr, phi1, phi0, hd)
+    ctx = _ctx(nx, ny, dx, dx)
+    out = ctx.array()
+    ctx.ls_sussman_constraint(ctx.array(phi1), ctx.array(phi0), ctx.array(hd), out)
+    err = np.abs(out.download() - ref).max()
+    print(f"sussman constraint: max err {err:.3e} (scale {np.abs(ref).max():.3e})")
+    assert err <= 1e-13
+
+
+@pytest.mark.parametrize("n", [7, 64, 301])
+def test_unit_weno5(oracle, n):
+    g, d = 3, 0.37
+    F = np.ascontiguousarray(np.sin(np.linspace(0, 7, n + 2 * g)) + 0.1 * S.random_field(n, (n + 2 * g,)))
+    rp, rm = oracle.unit_hjweno5(F, d, n, g)
+    ctx = _ctx(8, 8, 0.1, 0.1)
+    fp, fm = ctx.ls_unit_weno5(F, d, n, g)
+    scale = max(np.abs(rp).max(), np.abs(rm).max())
+    assert np.abs(fp - rp).max() <= 1e-13 * scale
+    assert np.abs(fm - rm).max() <= 1e-13 * scale
+
+
 def test_reinit_iterations_match_time_loop(oracle):
     lib = A.load()
     for eps in (1.0 / 128, 0.04 / 128, 1.0 / 2048):

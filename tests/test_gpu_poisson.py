@@ -136,6 +136,22 @@ def test_csr_pcg(oracle, mode, tol):
     assert abs(stats.iterations - it_ref) <= max(3, 0.05 * it_ref)
 
 
+def test_csr_bicgstab(oracle):
+    gr, fl, phi, kap, H, div = gfm_case(oracle, 40, 36, seed=52)
+    ax, ay, dg, rhs = oracle.gfm_assemble(gr, fl, phi, kap, H, div)
+    Acsr, ids = stencil_to_csr(gr, ax, ay, dg)
+    b = rhs_vector(gr, rhs, ids)
+    x_direct = spla.spsolve(Acsr.tocsc(), b)
+    ctx = A.Context(gr.nx, gr.ny, gr.dx, gr.dy)
+    x, stats = ctx.csr_pcg(Acsr.indptr, Acsr.indices, Acsr.data, Acsr.diagonal(), b,
+                           ctx.opts(tol=1e-11, check_every=4), bicgstab=True)
+    err = _relerr(x, x_direct)
+    print(f"csr bicgstab: iters {stats.iterations}, rel res {stats.rel_residual:.2e}, rel L2 {err:.2e}")
+    assert stats.converged == 1 and stats.rel_residual <= 1e-11
+    assert np.linalg.norm(Acsr @ x - b) <= 1e-10 * np.linalg.norm(b)
+    assert err <= 1e-7
+
+
 def test_constant_curvature_gives_piecewise_constant_pressure(oracle):
     """GFM closed form: with kappa = k0 everywhere and div = 0 the solution is
     p* = -dt sigma k0 H (zero at the pinned outside cell)."""

@@ -34,7 +34,7 @@ def c(a):
 
 def test_all_fixtures_present():
     names = {os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLD, "*.npz"))}
-    assert {"advect_neumann", "minrk2_3", "sussman_5", "droplet64_step", "bc_p_mixed"} <= names
+    assert {"advect_neumann", "minrk2_3", "sussman_5", "original_5", "droplet64_step", "bc_p_mixed"} <= names
 
 
 @pytest.mark.parametrize("name", ["bc_p_neumann", "bc_p_dirichlet", "bc_p_mixed"])
@@ -97,6 +97,11 @@ def test_sussman(oracle):
     np.testing.assert_array_equal(oracle.reinit_sussman(gr, c(d["phi"]), int(d["iters"])), d["out"])
 
 
+def test_original(oracle):
+    gr, d = load("original_5")
+    np.testing.assert_array_equal(oracle.reinit_original(gr, c(d["phi"]), int(d["iters"])), d["out"])
+
+
 def test_droplet_step(oracle):
     gr, d = load("droplet64_step")
     out = oracle.solve_levelset(gr, c(d["phi"]), c(d["u"]), c(d["v"]), float(d["dt"]))
@@ -119,6 +124,7 @@ def test_live_levelset_path(oracle, reference, seed, kinds):
                                   reference.solve_levelset(gr, b, u, v, 0.4 * dx))
     np.testing.assert_array_equal(oracle.reinit_minrk2(gr, a, 5), reference.reinit_minrk2(gr, b, 2.25 * dx))
     np.testing.assert_array_equal(oracle.reinit_sussman(gr, a, 3), reference.reinit_sussman(gr, b, 1.25 * dx))
+    np.testing.assert_array_equal(oracle.reinit_original(gr, a, 5), reference.reinit_original(gr, b, 2.25 * dx))
 
 
 def test_live_iteration_count(oracle):
