@@ -126,6 +126,8 @@ def load() -> C.CDLL:
         "asd2_csr_bicgstab": ([_VP, C.c_int, _VP, _VP, _VP, _VP, _VP, _VP, P(SolveOpts), P(SolveStats)],
                               C.c_int),
         "asd2_ls_reinit_original": ([_VP, _VP, C.c_int], C.c_int),
+        "asd2_ls_reinit_first_time": ([_VP, _VP, C.c_int], C.c_int),
+        "asd2_ls_weno_absgrad": ([_VP, _VP, _VP, _VP], C.c_int),
         "asd2_ls_heaviside_deriv": ([_VP, _VP, _VP], C.c_int),
         "asd2_ls_smoothed_sign": ([_VP, _VP, _VP], C.c_int),
         "asd2_ls_sussman_constraint": ([_VP, _VP, _VP, _VP, _VP], C.c_int),
@@ -275,6 +277,12 @@ class Context:
 
     def ls_reinit_original(self, ls, iters):
         check(self.lib.asd2_ls_reinit_original(self.h, ls.ptr, iters))
+
+    def ls_reinit_first_time(self, ls, iters):
+        check(self.lib.asd2_ls_reinit_first_time(self.h, ls.ptr, iters))
+
+    def ls_weno_absgrad(self, ls, ls0, out):
+        check(self.lib.asd2_ls_weno_absgrad(self.h, ls.ptr, ls0.ptr, out.ptr))
 
     def ls_heaviside_deriv(self, ls, out):
         check(self.lib.asd2_ls_heaviside_deriv(self.h, ls.ptr, out.ptr))

@@ -1,29 +1,39 @@
 // krylov.hip -- Jacobi-preconditioned Krylov solvers for the matrix-free GFM system
 // (reference PoissonSolver2D, src/poisson2d.cpp:230-575) on gfx950.
 //
-// Each CG iteration is two kernels: (1) direction update p = z + beta p fused with the
-// 5-point SpMV q = A p and the reduction p.q, (2) x/r/z update fused with the reductions
-// r.z and r.r.  Reductions are single pass: wave shuffles, then LDS across the 4 waves,
-// then one partial per block; the last block to finish (atomic ticket) sums the partials
-// in block order (deterministic) and updates the device-resident Krylov scalars, so the
-// host never waits inside an iteration.  The host enqueues iterations in batches and
-// polls a pinned copy of the scalars one batch behind; kernels of a finished solve exit
-// on the `done` flag.
+// Each CG iteration is two streaming kernels: (1) direction update p = z + beta p fused
+// with the 5-point SpMV q = A p and the block partials of p.q, (2) the x/r/z update fused
+// with the block partials of r.z and r.r.  Every reduction finishes in a one-block "tail"
+// kernel that sums the partials in block order (deterministic) and advances the
+// device-resident Krylov scalars, so the host never waits inside an iteration.  A grid-wide
+// "last block" reduction would need a device-scope release fence per block, which on this
+// chip writes back the XCD's L2 each time; the tail kernel costs about 2 us instead.
+// Reducing kernels use at most MAX_BLOCKS blocks; each block walks a contiguous range of
+// tiles, and consecutive ranges are placed on the same XCD (workgroup b runs on XCD b % 8)
+// so stencil halos shared by neighbouring tiles hit the same L2.  The host enqueues
+// iterations in batches and polls a pinned copy of the scalars one batch behind; kernels
+// of a finished solve exit on the `done` flag.
 #include <algorithm>
 
 #include "solver.h"
 
 namespace asd2 {
 
-constexpr int KB = 256;       // threads per block of every reducing kernel
-constexpr int TJ = 64, TI = 16;  // SpMV tile
+constexpr int KB = 256;            // threads per block of every kernel here
+constexpr int MAX_BLOCKS = 1024;   // blocks of a reducing kernel (= partials per value)
+constexpr int TJ = 64, TI = 16;    // SpMV tile (j fastest)
+constexpr int NXCD = 8;
 
+__host__ __device__ inline int ceil_div(int64_t a, int64_t b) { return static_cast<int>((a + b - 1) / b); }
+
+// ------------------------------------------------------------------ reductions
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     return v;
 }
 
+// block sum of NV values; the totals land in thread 0
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV]) {
     __shared__ double sh[NV][KB / 64];
@@ -47,59 +57,100 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
     __syncthreads();
 }
 
-// Publishes this block's sums; in the last block to arrive, v receives the grid totals
-// (summed in block order) in thread 0 and the function returns true for all threads.
+// block partials of a 1-D grid: partials[k * gridDim.x + blockIdx.x]
 template <int NV>
-__device__ __forceinline__ bool publish(double (&v)[NV], double* partials, unsigned* ticket) {
-    __shared__ int last;
-    const int nblocks = gridDim.x * gridDim.y;
-    const int bid = blockIdx.x + blockIdx.y * gridDim.x;
+__device__ __forceinline__ void put_partials(double (&v)[NV], double* partials) {
+    block_sum(v);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < NV; k++) partials[k * nblocks + bid] = v[k];
-        __threadfence();
-        const unsigned t = atomicAdd(ticket, 1u);
-        last = (t == static_cast<unsigned>(nblocks - 1)) ? 1 : 0;
+        for (int k = 0; k < NV; k++) partials[k * gridDim.x + blockIdx.x] = v[k];
     }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
+}
+
+// in a one-block tail kernel: totals of nb partials per value, in thread 0
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double* partials, int nb, double (&v)[NV]) {
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         double s = 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += KB) s += partials[k * nblocks + b];
+        for (int b = threadIdx.x; b < nb; b += KB) s += partials[k * nb + b];
         v[k] = s;
     }
     block_sum(v);
-    if (threadIdx.x == 0) *ticket = 0u;
-    return true;
 }
 
-// ------------------------------------------------------------------ CG kernels
-template <typename T>
-__global__ __launch_bounds__(KB) void k_cg_init(int64_t n, const double* __restrict__ b, T* __restrict__ x,
-                                                T* __restrict__ r, double* __restrict__ r64, double* __restrict__ x64,
-                                                const T* __restrict__ dinv, T* __restrict__ z, T* __restrict__ p,
-                                                KrylovState* st, double* partials) {
-    double bb = 0.0, rz = 0.0, rr = 0.0;
-    for (int64_t k = blockIdx.x * static_cast<int64_t>(KB) + threadIdx.x; k < n;
-         k += static_cast<int64_t>(gridDim.x) * KB) {
-        const double bv = b[k];
-        const T rv = static_cast<T>(bv);
-        const T zv = dinv[k] * rv;
-        x[k] = T(0);
-        r[k] = rv;
-        z[k] = zv;
-        p[k] = rv;
-        if (r64) r64[k] = bv;
-        if (x64) x64[k] = 0.0;
-        bb += bv * bv;
-        rz += static_cast<double>(rv) * static_cast<double>(zv);
-        rr += static_cast<double>(rv) * static_cast<double>(rv);
+// logical block index: consecutive logical indices run on the same XCD
+__device__ __forceinline__ int logical_block() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    if (nb % NXCD) return b;
+    return (b % NXCD) * (nb / NXCD) + b / NXCD;
+}
+
+// the contiguous tile range [t0, t1) of this block
+__device__ __forceinline__ void tile_range(int ntiles, int& t0, int& t1) {
+    const int per = ceil_div(ntiles, gridDim.x);
+    t0 = logical_block() * per;
+    t1 = min(ntiles, t0 + per);
+}
+
+// Visits every interior cell once: 64 x 4 tiles, stacked along i within a block's range.
+template <class F>
+__device__ __forceinline__ void for_interior(const Geom& G, F f) {
+    const int tiles_i = ceil_div(G.nx, 4), ntiles = tiles_i * ceil_div(G.ny, 64);
+    int t0, t1;
+    tile_range(ntiles, t0, t1);
+    for (int t = t0; t < t1; t++) {
+        const int tj = t / tiles_i, ti = t - tj * tiles_i;
+        const int j = G.g + tj * 64 + (threadIdx.x & 63);
+        const int i = G.g + ti * 4 + (threadIdx.x >> 6);
+        if (i < G.nx + G.g && j < G.ny + G.g) f(i, j, G.idx(i, j));
     }
-    double v[3] = {bb, rz, rr};
-    block_sum(v);
-    if (publish(v, partials, &st->ticket[0]) && threadIdx.x == 0) {
+}
+
+#define FLAT_LOOP(k, n) \
+    for (int64_t k = blockIdx.x * static_cast<int64_t>(KB) + threadIdx.x; k < (n); k += static_cast<int64_t>(gridDim.x) * KB)
+
+// ------------------------------------------------------------------ tail kernels
+enum TailOp {
+    T_CG_INIT,          // b.b, r.z, r.r  -> bnorm, rho, convergence
+    T_CG_ALPHA,         // p.q            -> alpha
+    T_CG_BETA,          // r.z, r.r       -> iter++, beta, convergence
+    T_CG_BETA_MIXED,    //                   ... pausing for reliable updates
+    T_TRUE_RES,         // ||b - A x||^2  -> reliable update bookkeeping
+    T_TRUE_RES_REPORT,  //                -> reported true residual
+    T_MX_RESUME,        // r.z, r.r       -> beta after a reliable update
+    T_BICG_INIT,        // b.b            -> bnorm, rho
+    T_BICG_ALPHA,       // rt.v           -> alpha
+    T_BICG_S,           // s.s            -> early exit on ||s||
+    T_BICG_OMEGA,       // t.s, t.t       -> omega
+    T_BICG_X,           // r.r, rt.r      -> iter++, beta, convergence
+};
+
+template <int OP> struct TailNV { static constexpr int value = 1; };
+template <> struct TailNV<T_CG_INIT> { static constexpr int value = 3; };
+template <> struct TailNV<T_CG_BETA> { static constexpr int value = 2; };
+template <> struct TailNV<T_CG_BETA_MIXED> { static constexpr int value = 2; };
+template <> struct TailNV<T_MX_RESUME> { static constexpr int value = 2; };
+template <> struct TailNV<T_BICG_OMEGA> { static constexpr int value = 2; };
+template <> struct TailNV<T_BICG_X> { static constexpr int value = 2; };
+
+// whether the producer of OP's partials ran (the tail mirrors its guard)
+template <int OP>
+__device__ __forceinline__ bool tail_active(const KrylovState* st) {
+    if (OP == T_CG_INIT || OP == T_BICG_INIT || OP == T_TRUE_RES_REPORT) return true;
+    if (OP == T_TRUE_RES || OP == T_MX_RESUME) return st->done == DONE_PAUSED;
+    if (OP == T_BICG_OMEGA) return st->done == DONE_RUNNING && !(st->flags & 1);
+    return st->done == DONE_RUNNING;
+}
+
+template <int OP>
+__global__ __launch_bounds__(KB) void k_tail(int nb, const double* __restrict__ partials, KrylovState* st) {
+    if (!tail_active<OP>(st)) return;
+    constexpr int NV = TailNV<OP>::value;
+    double v[NV];
+    sum_partials(partials, nb, v);
+    if (threadIdx.x != 0) return;
+    if (OP == T_CG_INIT) {
         double bn = sqrt(v[0]);
         if (bn == 0.0) bn = 1.0;
         st->bnorm = bn;
@@ -114,7 +165,105 @@ __global__ __launch_bounds__(KB) void k_cg_init(int64_t n, const double* __restr
         // reference iteration 0: beta = rho / rho1 with rho1 = 1 and p0 = r0 (poisson2d.cpp:331-360)
         st->beta = (st->mode == ASD2_CG_REFERENCE) ? v[1] / 1.0 : 0.0;
         st->done = (sqrt(v[2]) / bn <= st->tol) ? DONE_CONVERGED : DONE_RUNNING;
+    } else if (OP == T_CG_ALPHA) {
+        const double pq = v[0];
+        st->pq = pq;
+        // the reference adds err_tol^2 = 1e-12 to the denominator (poisson2d.cpp:373)
+        const double alpha = (st->mode == ASD2_CG_REFERENCE) ? st->rho / (pq + 1.0e-6 * 1.0e-6) : st->rho / pq;
+        st->alpha = alpha;
+        if (!isfinite(alpha)) st->done = DONE_BREAKDOWN;
+    } else if (OP == T_CG_BETA || OP == T_CG_BETA_MIXED) {
+        const bool mixed = (OP == T_CG_BETA_MIXED);
+        st->iter += 1;
+        const int it = st->iter;
+        st->rr = v[1];
+        st->rho_old = st->rho;
+        st->rho = v[0];
+        // reference iteration 1 restarts with p = z (poisson2d.cpp:350-352)
+        if (st->mode == ASD2_CG_REFERENCE) st->beta = (it == 1) ? 0.0 : st->rho / st->rho_old;
+        else st->beta = st->rho / st->rho_old;
+        const double rn = sqrt(v[1]);
+        const double resid = rn / st->bnorm;
+        if (!isfinite(resid) || !isfinite(st->beta)) st->done = DONE_BREAKDOWN;
+        else if (mixed && (resid <= st->tol || rn < st->delta * st->rmax)) st->done = DONE_PAUSED;
+        else if (!mixed && resid <= st->tol) st->done = DONE_CONVERGED;
+        else if (it >= st->maxit) st->done = mixed ? DONE_PAUSED : DONE_MAXIT;
+    } else if (OP == T_TRUE_RES) {
+        const double rn = sqrt(v[0]);
+        st->aux0 = rn;
+        st->rmax = rn;
+        st->updates += 1;
+        if (rn / st->bnorm <= st->tol) st->done = DONE_CONVERGED;
+        else if (st->iter >= st->maxit) st->done = DONE_MAXIT;
+    } else if (OP == T_TRUE_RES_REPORT) {
+        st->aux2 = sqrt(v[0]) / st->bnorm;
+    } else if (OP == T_MX_RESUME) {
+        st->rho = v[0];
+        st->rr = v[1];
+        st->beta = st->rho / st->rho_old;
+        st->done = isfinite(st->beta) ? DONE_RUNNING : DONE_BREAKDOWN;
+    } else if (OP == T_BICG_INIT) {
+        double bn = sqrt(v[0]);
+        const double rn = bn;
+        if (bn == 0.0) bn = 1.0;
+        st->bnorm = bn;
+        st->rr = v[0];
+        st->rho = v[0];  // rtilde . r with rtilde = r = b
+        st->alpha = 1.0;
+        st->omega = 1.0;
+        st->iter = 0;
+        st->flags = 0;
+        st->done = (rn / bn <= st->tol || v[0] == 0.0) ? DONE_CONVERGED : DONE_RUNNING;
+    } else if (OP == T_BICG_ALPHA) {
+        st->alpha = st->rho / v[0];
+        if (!isfinite(st->alpha)) st->done = DONE_BREAKDOWN;
+    } else if (OP == T_BICG_S) {
+        st->aux1 = sqrt(v[0]);
+        if (st->aux1 / st->bnorm <= st->tol) {
+            st->flags |= 1;
+            st->omega = 0.0;
+        }
+    } else if (OP == T_BICG_OMEGA) {
+        st->omega = v[0] / v[1];
+        if (!isfinite(st->omega)) st->done = DONE_BREAKDOWN;
+    } else if (OP == T_BICG_X) {
+        const bool half = (st->flags & 1) != 0;
+        st->iter += 1;
+        st->rr = v[0];
+        const double rho_new = v[1];
+        const double resid = sqrt(v[0]) / st->bnorm;
+        if (half || resid <= st->tol) st->done = DONE_CONVERGED;
+        else if (!isfinite(resid)) st->done = DONE_BREAKDOWN;
+        else if (rho_new == 0.0) st->done = DONE_BREAKDOWN;
+        else if (st->iter >= st->maxit) st->done = DONE_MAXIT;
+        st->beta = (rho_new / st->rho) * (st->alpha / st->omega);
+        st->rho = rho_new;
     }
+}
+
+// ------------------------------------------------------------------ CG kernels
+template <typename T>
+__global__ __launch_bounds__(KB) void k_cg_init(int64_t n, const double* __restrict__ b, T* __restrict__ x,
+                                                T* __restrict__ r, double* __restrict__ r64, double* __restrict__ x64,
+                                                const T* __restrict__ dinv, T* __restrict__ z, T* __restrict__ p,
+                                                double* partials) {
+    double bb = 0.0, rz = 0.0, rr = 0.0;
+    FLAT_LOOP(k, n) {
+        const double bv = b[k];
+        const T rv = static_cast<T>(bv);
+        const T zv = dinv[k] * rv;
+        x[k] = T(0);
+        r[k] = rv;
+        z[k] = zv;
+        p[k] = rv;
+        if (r64) r64[k] = bv;
+        if (x64) x64[k] = 0.0;
+        bb += bv * bv;
+        rz += static_cast<double>(rv) * static_cast<double>(zv);
+        rr += static_cast<double>(rv) * static_cast<double>(rv);
+    }
+    double v[3] = {bb, rz, rr};
+    put_partials(v, partials);
 }
 
 // p_new = z + beta p_old on a tile with a one-cell halo, q = A p_new, partial p_new . q.
@@ -131,57 +280,56 @@ __global__ __launch_bounds__(KB) void k_cg_spmv(Geom G, const T* __restrict__ ax
     const T* __restrict__ pold = odd ? p1 : p0;
     T* __restrict__ pnew = odd ? p0 : p1;
     __shared__ T sP[TI + 2][TJ + 2];
-    const int j0 = G.g + blockIdx.x * TJ, i0 = G.g + blockIdx.y * TI;
     const int ipad = G.nx + 2 * G.g, jpad = G.ny + 2 * G.g;
-    for (int e = threadIdx.x; e < (TI + 2) * (TJ + 2); e += KB) {
-        const int r = e / (TJ + 2), c = e - r * (TJ + 2);
-        const int i = i0 - 1 + r, j = j0 - 1 + c;
-        T val = T(0);
-        if (i < ipad && j < jpad) {
-            const int64_t k = G.idx(i, j);
-            val = z[k] + beta * pold[k];
-        }
-        sP[r][c] = val;
-    }
-    __syncthreads();
+    const int tiles_i = ceil_div(G.nx, TI), ntiles = tiles_i * ceil_div(G.ny, TJ);
+    int t0, t1;
+    tile_range(ntiles, t0, t1);
     const int tj = threadIdx.x & (TJ - 1), ti = threadIdx.x / TJ;
     double acc = 0.0;
-    for (int r = ti; r < TI; r += KB / TJ) {
-        const int i = i0 + r, j = j0 + tj;
-        if (i >= G.nx + G.g || j >= G.ny + G.g) continue;
-        const int64_t k = G.idx(i, j);
-        const T pc = sP[r + 1][tj + 1];
-        const T qv = dg[k] * pc + ax[k] * sP[r][tj + 1] + ax[k + G.sy] * sP[r + 2][tj + 1] + ay[k] * sP[r + 1][tj] +
-                     ay[k + 1] * sP[r + 1][tj + 2];
-        pnew[k] = pc;
-        q[k] = qv;
-        acc += static_cast<double>(pc) * static_cast<double>(qv);
+    for (int t = t0; t < t1; t++) {
+        const int bj = t / tiles_i, bi = t - bj * tiles_i;
+        const int j0 = G.g + bj * TJ, i0 = G.g + bi * TI;
+        for (int e = threadIdx.x; e < (TI + 2) * (TJ + 2); e += KB) {
+            const int r = e / (TJ + 2), c = e - r * (TJ + 2);
+            const int i = i0 - 1 + r, j = j0 - 1 + c;
+            T val = T(0);
+            if (i < ipad && j < jpad) {
+                const int64_t k = G.idx(i, j);
+                val = z[k] + beta * pold[k];
+            }
+            sP[r][c] = val;
+        }
+        __syncthreads();
+        for (int r = ti; r < TI; r += KB / TJ) {
+            const int i = i0 + r, j = j0 + tj;
+            if (i >= G.nx + G.g || j >= G.ny + G.g) continue;
+            const int64_t k = G.idx(i, j);
+            const T pc = sP[r + 1][tj + 1];
+            const T qv = dg[k] * pc + ax[k] * sP[r][tj + 1] + ax[k + G.sy] * sP[r + 2][tj + 1] +
+                         ay[k] * sP[r + 1][tj] + ay[k + 1] * sP[r + 1][tj + 2];
+            pnew[k] = pc;
+            q[k] = qv;
+            acc += static_cast<double>(pc) * static_cast<double>(qv);
+        }
+        __syncthreads();
     }
     double v[1] = {acc};
-    block_sum(v);
-    if (publish(v, partials, &st->ticket[1]) && threadIdx.x == 0) {
-        const double pq = v[0];
-        st->pq = pq;
-        const double alpha = (st->mode == ASD2_CG_REFERENCE) ? st->rho / (pq + 1.0e-6 * 1.0e-6) : st->rho / pq;
-        st->alpha = alpha;
-        if (!isfinite(alpha)) st->done = DONE_BREAKDOWN;
-    }
+    put_partials(v, partials);
 }
 
-// x += alpha p, r -= alpha q, z = M^-1 r, partials r.z and r.r; the last block advances
-// the recurrence (beta, convergence) -- poisson2d.cpp:346-395.  p is the direction just
-// written by k_cg_spmv (P[(iter + 1) & 1]); pass p0 == p1 for a single buffer.
+// x += alpha p, r -= alpha q, z = M^-1 r, partials r.z and r.r (poisson2d.cpp:346-395).
+// p is the direction just written by k_cg_spmv (P[(iter + 1) & 1]); pass p0 == p1 for a
+// single buffer.
 template <typename T>
 __global__ __launch_bounds__(KB) void k_cg_update(int64_t n, T* __restrict__ x, const T* p0, const T* p1,
                                                   T* __restrict__ r, const T* __restrict__ q,
                                                   const T* __restrict__ dinv, T* __restrict__ z,
-                                                  KrylovState* st, double* partials, int mixed) {
+                                                  const KrylovState* st, double* partials) {
     if (st->done != DONE_RUNNING) return;
     const T a = static_cast<T>(st->alpha);
     const T* __restrict__ p = (st->iter & 1) ? p0 : p1;
     double rz = 0.0, rr = 0.0;
-    for (int64_t k = blockIdx.x * static_cast<int64_t>(KB) + threadIdx.x; k < n;
-         k += static_cast<int64_t>(gridDim.x) * KB) {
+    FLAT_LOOP(k, n) {
         const T xv = a * p[k] + x[k];
         const T rv = -a * q[k] + r[k];
         const T zv = dinv[k] * rv;
@@ -192,22 +340,7 @@ __global__ __launch_bounds__(KB) void k_cg_update(int64_t n, T* __restrict__ x, 
         rr += static_cast<double>(rv) * static_cast<double>(rv);
     }
     double v[2] = {rz, rr};
-    block_sum(v);
-    if (publish(v, partials, &st->ticket[2]) && threadIdx.x == 0) {
-        st->iter += 1;
-        const int it = st->iter;
-        st->rr = v[1];
-        st->rho_old = st->rho;
-        st->rho = v[0];
-        if (st->mode == ASD2_CG_REFERENCE) st->beta = (it == 1) ? 0.0 : st->rho / st->rho_old;
-        else st->beta = st->rho / st->rho_old;
-        const double rn = sqrt(v[1]);
-        const double resid = rn / st->bnorm;
-        if (!isfinite(resid) || !isfinite(st->beta)) st->done = DONE_BREAKDOWN;
-        else if (mixed && (resid <= st->tol || rn < st->delta * st->rmax)) st->done = DONE_PAUSED;
-        else if (!mixed && resid <= st->tol) st->done = DONE_CONVERGED;
-        else if (it >= st->maxit) st->done = mixed ? DONE_PAUSED : DONE_MAXIT;
-    }
+    put_partials(v, partials);
 }
 
 // ------------------------------------------------------------------ mixed-precision reliable updates
@@ -215,56 +348,39 @@ __global__ __launch_bounds__(KB) void k_cg_update(int64_t n, T* __restrict__ x, 
 __global__ __launch_bounds__(KB) void k_mx_accum(int64_t n, double* __restrict__ x64, float* __restrict__ x32,
                                                  const KrylovState* st, int force) {
     if (!force && st->done != DONE_PAUSED) return;
-    for (int64_t k = blockIdx.x * static_cast<int64_t>(KB) + threadIdx.x; k < n;
-         k += static_cast<int64_t>(gridDim.x) * KB) {
+    FLAT_LOOP(k, n) {
         x64[k] += static_cast<double>(x32[k]);
         x32[k] = 0.0f;
     }
 }
 
-// r64 = b - A x64 in FP64 (interior), partial ||r64||^2.  mode 0: reliable update
-// (sets CONVERGED or keeps PAUSED); mode 1: report only (writes aux2).
+// r64 = b - A x64 in FP64 (interior), partial ||r64||^2.  mode 0: reliable update (runs
+// while paused), mode 1: report only.
 __global__ __launch_bounds__(KB) void k_true_residual(Geom G, const double* __restrict__ ax,
                                                       const double* __restrict__ ay, const double* __restrict__ dg,
                                                       const double* __restrict__ b, const double* __restrict__ x,
-                                                      double* __restrict__ r64, KrylovState* st, double* partials,
-                                                      int mode) {
+                                                      double* __restrict__ r64, const KrylovState* st,
+                                                      double* partials, int mode) {
     if (mode == 0 && st->done != DONE_PAUSED) return;
-    const int j = G.g + blockIdx.x * 64 + (threadIdx.x & 63);
-    const int i = G.g + blockIdx.y * 4 + (threadIdx.x >> 6);
     double acc = 0.0;
-    if (i < G.nx + G.g && j < G.ny + G.g) {
-        const int64_t k = G.idx(i, j);
+    for_interior(G, [&](int, int, int64_t k) {
         const double ax_ = dg[k] * x[k] + ax[k] * x[k - G.sy] + ax[k + G.sy] * x[k + G.sy] + ay[k] * x[k - 1] +
                            ay[k + 1] * x[k + 1];
         const double rv = b[k] - ax_;
         if (r64) r64[k] = rv;
-        acc = rv * rv;
-    }
+        acc += rv * rv;
+    });
     double v[1] = {acc};
-    block_sum(v);
-    if (publish(v, partials, &st->ticket[3]) && threadIdx.x == 0) {
-        const double rn = sqrt(v[0]);
-        if (mode == 1) {
-            st->aux2 = rn / st->bnorm;
-        } else {
-            st->aux0 = rn;
-            st->rmax = rn;
-            st->updates += 1;
-            if (rn / st->bnorm <= st->tol) st->done = DONE_CONVERGED;
-            else if (st->iter >= st->maxit) st->done = DONE_MAXIT;
-        }
-    }
+    put_partials(v, partials);
 }
 
-// r32 = r64, z32 = M^-1 r32, rho = r.z; beta = rho_true / rho_prev; resume the iteration
+// r32 = r64, z32 = M^-1 r32, partials r.z and r.r (the tail resumes the iteration)
 __global__ __launch_bounds__(KB) void k_mx_resume(int64_t n, const double* __restrict__ r64, float* __restrict__ r32,
                                                   const float* __restrict__ dinv, float* __restrict__ z32,
-                                                  KrylovState* st, double* partials) {
+                                                  const KrylovState* st, double* partials) {
     if (st->done != DONE_PAUSED) return;
     double rz = 0.0, rr = 0.0;
-    for (int64_t k = blockIdx.x * static_cast<int64_t>(KB) + threadIdx.x; k < n;
-         k += static_cast<int64_t>(gridDim.x) * KB) {
+    FLAT_LOOP(k, n) {
         const float rv = static_cast<float>(r64[k]);
         const float zv = dinv[k] * rv;
         r32[k] = rv;
@@ -273,67 +389,47 @@ __global__ __launch_bounds__(KB) void k_mx_resume(int64_t n, const double* __res
         rr += static_cast<double>(rv) * static_cast<double>(rv);
     }
     double v[2] = {rz, rr};
-    block_sum(v);
-    if (publish(v, partials, &st->ticket[0]) && threadIdx.x == 0) {
-        st->rho = v[0];
-        st->rr = v[1];
-        st->beta = st->rho / st->rho_old;
-        st->done = isfinite(st->beta) ? DONE_RUNNING : DONE_BREAKDOWN;
-    }
+    put_partials(v, partials);
 }
 
 // ------------------------------------------------------------------ generic pieces (BiCGStab, CSR)
 __global__ __launch_bounds__(KB) void k_stencil_apply(Geom G, const double* __restrict__ ax,
                                                       const double* __restrict__ ay, const double* __restrict__ dg,
                                                       const double* __restrict__ x, double* __restrict__ y) {
-    const int j = G.g + blockIdx.x * 64 + (threadIdx.x & 63);
-    const int i = G.g + blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (i >= G.nx + G.g || j >= G.ny + G.g) return;
-    const int64_t k = G.idx(i, j);
-    y[k] = dg[k] * x[k] + ax[k] * x[k - G.sy] + ax[k + G.sy] * x[k + G.sy] + ay[k] * x[k - 1] + ay[k + 1] * x[k + 1];
+    for_interior(G, [&](int, int, int64_t k) {
+        y[k] = dg[k] * x[k] + ax[k] * x[k - G.sy] + ax[k + G.sy] * x[k + G.sy] + ay[k] * x[k - 1] +
+               ay[k + 1] * x[k + 1];
+    });
 }
 
-// y = A x (stencil) with partial dot(w, y) -> st->aux[slot]
+// y = A x (stencil) with partials (w . y, y . y); phase 2 is skipped after the half-step exit
 __global__ __launch_bounds__(KB) void k_stencil_dot(Geom G, const double* __restrict__ ax,
                                                     const double* __restrict__ ay, const double* __restrict__ dg,
                                                     const double* __restrict__ x, double* __restrict__ y,
-                                                    const double* __restrict__ w, const double* __restrict__ w2,
-                                                    KrylovState* st, double* partials, int phase) {
+                                                    const double* __restrict__ w, const KrylovState* st,
+                                                    double* partials, int phase) {
     if (st->done != DONE_RUNNING) return;
     if (phase == 2 && (st->flags & 1)) return;
-    const int j = G.g + blockIdx.x * 64 + (threadIdx.x & 63);
-    const int i = G.g + blockIdx.y * 4 + (threadIdx.x >> 6);
     double a0 = 0.0, a1 = 0.0;
-    if (i < G.nx + G.g && j < G.ny + G.g) {
-        const int64_t k = G.idx(i, j);
+    for_interior(G, [&](int, int, int64_t k) {
         const double yv = dg[k] * x[k] + ax[k] * x[k - G.sy] + ax[k + G.sy] * x[k + G.sy] + ay[k] * x[k - 1] +
                           ay[k + 1] * x[k + 1];
         y[k] = yv;
-        a0 = w[k] * yv;
-        a1 = w2 ? yv * yv : 0.0;
-    }
+        a0 += w[k] * yv;
+        a1 += yv * yv;
+    });
     double v[2] = {a0, a1};
-    block_sum(v);
-    if (publish(v, partials, &st->ticket[1]) && threadIdx.x == 0) {
-        if (phase == 1) {  // v = A phat: alpha = rho / (rtilde . v)
-            st->alpha = st->rho / v[0];
-            if (!isfinite(st->alpha)) st->done = DONE_BREAKDOWN;
-        } else {           // t = A shat: omega = (t . s) / (t . t)
-            st->omega = v[0] / v[1];
-            if (!isfinite(st->omega)) st->done = DONE_BREAKDOWN;
-        }
-    }
+    put_partials(v, partials);
 }
 
 // BiCGStab vector phases (van der Vorst 1992, IML++ ordering)
 __global__ __launch_bounds__(KB) void k_bicg_dir(int64_t n, const double* __restrict__ r, double* __restrict__ p,
                                                  const double* __restrict__ v, const double* __restrict__ dinv,
-                                                 double* __restrict__ ph, KrylovState* st) {
+                                                 double* __restrict__ ph, const KrylovState* st) {
     if (st->done != DONE_RUNNING) return;
     const bool first = (st->iter == 0);
     const double beta = st->beta, om = st->omega;
-    for (int64_t k = blockIdx.x * static_cast<int64_t>(KB) + threadIdx.x; k < n;
-         k += static_cast<int64_t>(gridDim.x) * KB) {
+    FLAT_LOOP(k, n) {
         const double pv = first ? r[k] : r[k] + beta * (p[k] - om * v[k]);
         p[k] = pv;
         ph[k] = dinv[k] * pv;
@@ -342,35 +438,30 @@ __global__ __launch_bounds__(KB) void k_bicg_dir(int64_t n, const double* __rest
 
 __global__ __launch_bounds__(KB) void k_bicg_s(int64_t n, const double* __restrict__ r, const double* __restrict__ v,
                                                const double* __restrict__ dinv, double* __restrict__ s,
-                                               double* __restrict__ sh, KrylovState* st, double* partials) {
+                                               double* __restrict__ sh, const KrylovState* st, double* partials) {
     if (st->done != DONE_RUNNING) return;
     const double a = st->alpha;
     double ss = 0.0;
-    for (int64_t k = blockIdx.x * static_cast<int64_t>(KB) + threadIdx.x; k < n;
-         k += static_cast<int64_t>(gridDim.x) * KB) {
+    FLAT_LOOP(k, n) {
         const double sv = r[k] - a * v[k];
         s[k] = sv;
         sh[k] = dinv[k] * sv;
         ss += sv * sv;
     }
     double vv[1] = {ss};
-    block_sum(vv);
-    if (publish(vv, partials, &st->ticket[2]) && threadIdx.x == 0) {
-        st->aux1 = sqrt(vv[0]);
-        if (st->aux1 / st->bnorm <= st->tol) { st->flags |= 1; st->omega = 0.0; }
-    }
+    put_partials(vv, partials);
 }
 
 __global__ __launch_bounds__(KB) void k_bicg_x(int64_t n, double* __restrict__ x, const double* __restrict__ ph,
                                                const double* __restrict__ sh, double* __restrict__ r,
                                                const double* __restrict__ s, const double* __restrict__ t,
-                                               const double* __restrict__ rt, KrylovState* st, double* partials) {
+                                               const double* __restrict__ rt, const KrylovState* st,
+                                               double* partials) {
     if (st->done != DONE_RUNNING) return;
     const bool half = (st->flags & 1) != 0;
     const double a = st->alpha, om = half ? 0.0 : st->omega;
     double rr = 0.0, rtr = 0.0;
-    for (int64_t k = blockIdx.x * static_cast<int64_t>(KB) + threadIdx.x; k < n;
-         k += static_cast<int64_t>(gridDim.x) * KB) {
+    FLAT_LOOP(k, n) {
         const double xv = x[k] + a * ph[k] + om * sh[k];
         const double rv = half ? s[k] : s[k] - om * t[k];
         x[k] = xv;
@@ -379,27 +470,13 @@ __global__ __launch_bounds__(KB) void k_bicg_x(int64_t n, double* __restrict__ x
         rtr += rt[k] * rv;
     }
     double v[2] = {rr, rtr};
-    block_sum(v);
-    if (publish(v, partials, &st->ticket[3]) && threadIdx.x == 0) {
-        st->iter += 1;
-        st->rr = v[0];
-        const double rho_new = v[1];
-        const double resid = sqrt(v[0]) / st->bnorm;
-        if (half || resid <= st->tol) st->done = DONE_CONVERGED;
-        else if (!isfinite(resid)) st->done = DONE_BREAKDOWN;
-        else if (rho_new == 0.0) st->done = DONE_BREAKDOWN;
-        else if (st->iter >= st->maxit) st->done = DONE_MAXIT;
-        st->beta = (rho_new / st->rho) * (st->alpha / st->omega);
-        st->rho = rho_new;
-    }
+    put_partials(v, partials);
 }
 
 __global__ __launch_bounds__(KB) void k_bicg_init(int64_t n, const double* __restrict__ b, double* __restrict__ x,
-                                                  double* __restrict__ r, double* __restrict__ rt,
-                                                  KrylovState* st, double* partials) {
+                                                  double* __restrict__ r, double* __restrict__ rt, double* partials) {
     double bb = 0.0;
-    for (int64_t k = blockIdx.x * static_cast<int64_t>(KB) + threadIdx.x; k < n;
-         k += static_cast<int64_t>(gridDim.x) * KB) {
+    FLAT_LOOP(k, n) {
         const double bv = b[k];
         x[k] = 0.0;
         r[k] = bv;
@@ -407,20 +484,7 @@ __global__ __launch_bounds__(KB) void k_bicg_init(int64_t n, const double* __res
         bb += bv * bv;
     }
     double v[1] = {bb};
-    block_sum(v);
-    if (publish(v, partials, &st->ticket[0]) && threadIdx.x == 0) {
-        double bn = sqrt(v[0]);
-        const double rn = bn;
-        if (bn == 0.0) bn = 1.0;
-        st->bnorm = bn;
-        st->rr = v[0];
-        st->rho = v[0];  // rtilde . r with rtilde = r = b
-        st->alpha = 1.0;
-        st->omega = 1.0;
-        st->iter = 0;
-        st->flags = 0;
-        st->done = (rn / bn <= st->tol) ? DONE_CONVERGED : (v[0] == 0.0 ? DONE_CONVERGED : DONE_RUNNING);
-    }
+    put_partials(v, partials);
 }
 
 // CSR pieces (caller-built matrices, PoissonSolver2D::CG_2FUniform_2D)
@@ -428,42 +492,34 @@ __global__ __launch_bounds__(KB) void k_csr_dir(int64_t n, const double* __restr
                                                 const KrylovState* st) {
     if (st->done != DONE_RUNNING) return;
     const double beta = st->beta;
-    for (int64_t k = blockIdx.x * static_cast<int64_t>(KB) + threadIdx.x; k < n;
-         k += static_cast<int64_t>(gridDim.x) * KB)
-        p[k] = z[k] + beta * p[k];
+    FLAT_LOOP(k, n) p[k] = z[k] + beta * p[k];
 }
 
+// q = A p, partial p.q
 __global__ __launch_bounds__(KB) void k_csr_spmv(int n, const int* __restrict__ rp, const int* __restrict__ cols,
                                                  const double* __restrict__ vals, const double* __restrict__ p,
-                                                 double* __restrict__ q, KrylovState* st, double* partials) {
+                                                 double* __restrict__ q, const KrylovState* st, double* partials) {
     if (st->done != DONE_RUNNING) return;
     double acc = 0.0;
-    for (int row = blockIdx.x * KB + threadIdx.x; row < n; row += gridDim.x * KB) {
+    FLAT_LOOP(row, n) {
         double s = 0.0;
         for (int e = rp[row]; e < rp[row + 1]; e++) s += vals[e] * p[cols[e]];
         q[row] = s;
         acc += p[row] * s;
     }
     double v[1] = {acc};
-    block_sum(v);
-    if (publish(v, partials, &st->ticket[1]) && threadIdx.x == 0) {
-        const double pq = v[0];
-        st->pq = pq;
-        const double alpha = (st->mode == ASD2_CG_REFERENCE) ? st->rho / (pq + 1.0e-6 * 1.0e-6) : st->rho / pq;
-        st->alpha = alpha;
-        if (!isfinite(alpha)) st->done = DONE_BREAKDOWN;
-    }
+    put_partials(v, partials);
 }
 
-// CSR y = A x with the BiCGStab dot products of k_stencil_dot (phase 1: alpha, phase 2: omega)
+// y = A x with partials (w . y, y . y) for BiCGStab (phase 1: alpha, phase 2: omega)
 __global__ __launch_bounds__(KB) void k_csr_dot(int n, const int* __restrict__ rp, const int* __restrict__ cols,
                                                 const double* __restrict__ vals, const double* __restrict__ x,
                                                 double* __restrict__ y, const double* __restrict__ w,
-                                                KrylovState* st, double* partials, int phase) {
+                                                const KrylovState* st, double* partials, int phase) {
     if (st->done != DONE_RUNNING) return;
     if (phase == 2 && (st->flags & 1)) return;
     double a0 = 0.0, a1 = 0.0;
-    for (int row = blockIdx.x * KB + threadIdx.x; row < n; row += gridDim.x * KB) {
+    FLAT_LOOP(row, n) {
         double s = 0.0;
         for (int e = rp[row]; e < rp[row + 1]; e++) s += vals[e] * x[cols[e]];
         y[row] = s;
@@ -471,30 +527,16 @@ __global__ __launch_bounds__(KB) void k_csr_dot(int n, const int* __restrict__ r
         a1 += s * s;
     }
     double v[2] = {a0, a1};
-    block_sum(v);
-    if (publish(v, partials, &st->ticket[1]) && threadIdx.x == 0) {
-        if (phase == 1) {
-            st->alpha = st->rho / v[0];
-            if (!isfinite(st->alpha)) st->done = DONE_BREAKDOWN;
-        } else {
-            st->omega = v[0] / v[1];
-            if (!isfinite(st->omega)) st->done = DONE_BREAKDOWN;
-        }
-    }
+    put_partials(v, partials);
 }
 
 __global__ void k_recip(int64_t n, const double* __restrict__ d, double* __restrict__ di) {
-    for (int64_t k = blockIdx.x * static_cast<int64_t>(KB) + threadIdx.x; k < n;
-         k += static_cast<int64_t>(gridDim.x) * KB)
-        di[k] = 1.0 / d[k];
+    FLAT_LOOP(k, n) di[k] = 1.0 / d[k];
 }
 
 // copy the interior of x into the interior of ps
-__global__ void k_copy_interior(Geom G, const double* __restrict__ x, double* __restrict__ ps) {
-    const int j = G.g + blockIdx.x * 64 + (threadIdx.x & 63);
-    const int i = G.g + blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (i >= G.nx + G.g || j >= G.ny + G.g) return;
-    ps[G.idx(i, j)] = x[G.idx(i, j)];
+__global__ __launch_bounds__(KB) void k_copy_interior(Geom G, const double* __restrict__ x, double* __restrict__ ps) {
+    for_interior(G, [&](int, int, int64_t k) { ps[k] = x[k]; });
 }
 
 // ------------------------------------------------------------------ host driver
@@ -506,19 +548,26 @@ struct Driver {
     double* partials = nullptr;
 };
 
-inline dim3 grid_tiles(const Geom& G) { return dim3(cdiv(G.ny, TJ), cdiv(G.nx, TI)); }
-inline dim3 grid_rows(const Geom& G) { return dim3(cdiv(G.ny, 64), cdiv(G.nx, 4)); }
-constexpr int FLAT_BLOCKS = 2048;
+int flat_blocks(int64_t n) { return std::max(1, std::min<int>(MAX_BLOCKS, ceil_div(n, KB))); }
+int spmv_blocks(const Geom& G) {
+    return std::max(1, std::min<int>(MAX_BLOCKS, ceil_div(G.nx, TI) * ceil_div(G.ny, TJ)));
+}
+int interior_blocks(const Geom& G) {
+    return std::max(1, std::min<int>(MAX_BLOCKS, ceil_div(G.nx, 4) * ceil_div(G.ny, 64)));
+}
+
+template <int OP>
+int tail(Driver* D, int nb) {
+    hipLaunchKernelGGL(k_tail<OP>, dim3(1), dim3(KB), 0, D->ctx->stream, nb, D->partials, D->st);
+    ASD2_LAUNCH_CHECK();
+    return ASD2_OK;
+}
 
 int driver_init(asd2_ctx* ctx, Driver* D) {
     D->ctx = ctx;
-    const Geom& G = ctx->geo;
-    const dim3 a = grid_tiles(G), b = grid_rows(G);
-    const size_t max_blocks = std::max<size_t>({static_cast<size_t>(a.x) * a.y, static_cast<size_t>(b.x) * b.y,
-                                                static_cast<size_t>(FLAT_BLOCKS)});
     static_assert(sizeof(KrylovState) % 8 == 0, "state size");
     ASD2_TRY(ws(ctx, "kr_state", sizeof(KrylovState) / 8, reinterpret_cast<double**>(&D->st)));
-    ASD2_TRY(ws(ctx, "kr_partials", 4 * max_blocks, &D->partials));
+    ASD2_TRY(ws(ctx, "kr_partials", 4 * MAX_BLOCKS, &D->partials));
     if (!ctx->kr_host) ASD2_HIP(hipHostMalloc(&ctx->kr_host, 2 * sizeof(KrylovState), hipHostMallocDefault));
     for (int k = 0; k < 2; k++)
         if (!ctx->kr_ev[k]) ASD2_HIP(hipEventCreateWithFlags(&ctx->kr_ev[k], hipEventDisableTiming));
@@ -597,9 +646,11 @@ int status_of(const KrylovState& s) {
 
 int report_true_residual(asd2_ctx* ctx, Driver* D, const System& S, const double* x, KrylovState* fs) {
     const Geom& G = ctx->geo;
-    hipLaunchKernelGGL(k_true_residual, grid_rows(G), dim3(KB), 0, ctx->stream, G, S.ax, S.ay, S.diag, S.rhs, x,
+    const int nb = interior_blocks(G);
+    hipLaunchKernelGGL(k_true_residual, dim3(nb), dim3(KB), 0, ctx->stream, G, S.ax, S.ay, S.diag, S.rhs, x,
                        static_cast<double*>(nullptr), D->st, D->partials, 1);
     ASD2_LAUNCH_CHECK();
+    ASD2_TRY(tail<T_TRUE_RES_REPORT>(D, nb));
     return read_state(D, fs);
 }
 
@@ -615,30 +666,33 @@ int solve_fp64(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2_solve
     ASD2_TRY(ws(ctx, "cg_p0", n, &p0));
     ASD2_TRY(ws(ctx, "cg_p1", n, &p1));
     ASD2_TRY(ws(ctx, "cg_q", n, &q));
-    const dim3 gt = grid_tiles(G);
+    const int nbs = spmv_blocks(G), nbf = flat_blocks(n);
     Driver D;
     ASD2_TRY(driver_init(ctx, &D));
     ASD2_TRY(set_state(&D, o));
-    hipLaunchKernelGGL(k_cg_init<double>, dim3(FLAT_BLOCKS), dim3(KB), 0, ctx->stream, n, S.rhs, x, r,
-                       static_cast<double*>(nullptr), static_cast<double*>(nullptr), S.dinv, z, p0, D.st, D.partials);
+    hipLaunchKernelGGL(k_cg_init<double>, dim3(nbf), dim3(KB), 0, ctx->stream, n, S.rhs, x, r,
+                       static_cast<double*>(nullptr), static_cast<double*>(nullptr), S.dinv, z, p0, D.partials);
     ASD2_LAUNCH_CHECK();
+    ASD2_TRY(tail<T_CG_INIT>(&D, nbf));
     auto batch = [&](int count) -> int {
         for (int k = 0; k < count; k++) {
             prof_begin(ctx, ASD2_KID_PCG64_SPMV);
-            hipLaunchKernelGGL(k_cg_spmv<double>, gt, dim3(KB), 0, ctx->stream, G, S.ax, S.ay, S.diag, z, p0, p1,
-                               q, D.st, D.partials);
+            hipLaunchKernelGGL(k_cg_spmv<double>, dim3(nbs), dim3(KB), 0, ctx->stream, G, S.ax, S.ay, S.diag, z, p0,
+                               p1, q, D.st, D.partials);
             prof_end(ctx, ASD2_KID_PCG64_SPMV);
+            ASD2_TRY(tail<T_CG_ALPHA>(&D, nbs));
             prof_begin(ctx, ASD2_KID_PCG64_UPDATE);
-            hipLaunchKernelGGL(k_cg_update<double>, dim3(FLAT_BLOCKS), dim3(KB), 0, ctx->stream, n, x, p0, p1, r, q,
-                               S.dinv, z, D.st, D.partials, 0);
+            hipLaunchKernelGGL(k_cg_update<double>, dim3(nbf), dim3(KB), 0, ctx->stream, n, x, p0, p1, r, q, S.dinv,
+                               z, D.st, D.partials);
             prof_end(ctx, ASD2_KID_PCG64_UPDATE);
+            ASD2_TRY(tail<T_CG_BETA>(&D, nbf));
         }
         ASD2_LAUNCH_CHECK();
         return ASD2_OK;
     };
     KrylovState fs;
     ASD2_TRY(drive(&D, o->check_every > 0 ? o->check_every : 32, batch, []() { return ASD2_OK; }, false, &fs));
-    hipLaunchKernelGGL(k_copy_interior, grid_rows(G), dim3(256), 0, ctx->stream, G, x, d_ps);
+    hipLaunchKernelGGL(k_copy_interior, dim3(interior_blocks(G)), dim3(KB), 0, ctx->stream, G, x, d_ps);
     ASD2_LAUNCH_CHECK();
     const int st = status_of(fs);
     ASD2_TRY(report_true_residual(ctx, &D, S, x, &fs));
@@ -662,41 +716,46 @@ int solve_mixed(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2_solv
     ASD2_TRY(ws(ctx, "mx_p0", n, &p0));
     ASD2_TRY(ws(ctx, "mx_p1", n, &p1));
     ASD2_TRY(ws(ctx, "mx_q", n, &q));
-    const dim3 gt = grid_tiles(G);
+    const int nbs = spmv_blocks(G), nbf = flat_blocks(n), nbi = interior_blocks(G);
     Driver D;
     ASD2_TRY(driver_init(ctx, &D));
     ASD2_TRY(set_state(&D, o));
-    hipLaunchKernelGGL(k_cg_init<float>, dim3(FLAT_BLOCKS), dim3(KB), 0, ctx->stream, n, S.rhs, x, r, r64, x64,
-                       S.dinv32, z, p0, D.st, D.partials);
+    hipLaunchKernelGGL(k_cg_init<float>, dim3(nbf), dim3(KB), 0, ctx->stream, n, S.rhs, x, r, r64, x64, S.dinv32, z,
+                       p0, D.partials);
     ASD2_LAUNCH_CHECK();
+    ASD2_TRY(tail<T_CG_INIT>(&D, nbf));
     auto batch = [&](int count) -> int {
         for (int k = 0; k < count; k++) {
             prof_begin(ctx, ASD2_KID_PCG32_SPMV);
-            hipLaunchKernelGGL(k_cg_spmv<float>, gt, dim3(KB), 0, ctx->stream, G, S.ax32, S.ay32, S.diag32, z, p0,
-                               p1, q, D.st, D.partials);
+            hipLaunchKernelGGL(k_cg_spmv<float>, dim3(nbs), dim3(KB), 0, ctx->stream, G, S.ax32, S.ay32, S.diag32, z,
+                               p0, p1, q, D.st, D.partials);
             prof_end(ctx, ASD2_KID_PCG32_SPMV);
+            ASD2_TRY(tail<T_CG_ALPHA>(&D, nbs));
             prof_begin(ctx, ASD2_KID_PCG32_UPDATE);
-            hipLaunchKernelGGL(k_cg_update<float>, dim3(FLAT_BLOCKS), dim3(KB), 0, ctx->stream, n, x, p0, p1, r, q,
-                               S.dinv32, z, D.st, D.partials, 1);
+            hipLaunchKernelGGL(k_cg_update<float>, dim3(nbf), dim3(KB), 0, ctx->stream, n, x, p0, p1, r, q,
+                               S.dinv32, z, D.st, D.partials);
             prof_end(ctx, ASD2_KID_PCG32_UPDATE);
+            ASD2_TRY(tail<T_CG_BETA_MIXED>(&D, nbf));
         }
         ASD2_LAUNCH_CHECK();
         return ASD2_OK;
     };
     auto pause = [&]() -> int {
-        hipLaunchKernelGGL(k_mx_accum, dim3(FLAT_BLOCKS), dim3(KB), 0, ctx->stream, n, x64, x, D.st, 0);
-        hipLaunchKernelGGL(k_true_residual, grid_rows(G), dim3(KB), 0, ctx->stream, G, S.ax, S.ay, S.diag, S.rhs,
-                           x64, r64, D.st, D.partials, 0);
-        hipLaunchKernelGGL(k_mx_resume, dim3(FLAT_BLOCKS), dim3(KB), 0, ctx->stream, n, r64, r, S.dinv32, z, D.st,
+        hipLaunchKernelGGL(k_mx_accum, dim3(nbf), dim3(KB), 0, ctx->stream, n, x64, x, D.st, 0);
+        hipLaunchKernelGGL(k_true_residual, dim3(nbi), dim3(KB), 0, ctx->stream, G, S.ax, S.ay, S.diag, S.rhs, x64,
+                           r64, D.st, D.partials, 0);
+        ASD2_TRY(tail<T_TRUE_RES>(&D, nbi));
+        hipLaunchKernelGGL(k_mx_resume, dim3(nbf), dim3(KB), 0, ctx->stream, n, r64, r, S.dinv32, z, D.st,
                            D.partials);
+        ASD2_TRY(tail<T_MX_RESUME>(&D, nbf));
         ASD2_LAUNCH_CHECK();
         return ASD2_OK;
     };
     KrylovState fs;
     ASD2_TRY(drive(&D, o->check_every > 0 ? o->check_every : 16, batch, pause, true, &fs));
     // fold any FP32 progress not yet accumulated (max-iteration exit)
-    hipLaunchKernelGGL(k_mx_accum, dim3(FLAT_BLOCKS), dim3(KB), 0, ctx->stream, n, x64, x, D.st, 1);
-    hipLaunchKernelGGL(k_copy_interior, grid_rows(G), dim3(256), 0, ctx->stream, G, x64, d_ps);
+    hipLaunchKernelGGL(k_mx_accum, dim3(nbf), dim3(KB), 0, ctx->stream, n, x64, x, D.st, 1);
+    hipLaunchKernelGGL(k_copy_interior, dim3(nbi), dim3(KB), 0, ctx->stream, G, x64, d_ps);
     ASD2_LAUNCH_CHECK();
     const int st = status_of(fs);
     ASD2_TRY(report_true_residual(ctx, &D, S, x64, &fs));
@@ -717,7 +776,8 @@ int asd2_gfm_apply(asd2_ctx* ctx, const double* d_x, double* d_y) {
     System S;
     ASD2_TRY(system_arrays(ctx, &S, false));
     const Geom& G = ctx->geo;
-    hipLaunchKernelGGL(k_stencil_apply, grid_rows(G), dim3(KB), 0, ctx->stream, G, S.ax, S.ay, S.diag, d_x, d_y);
+    hipLaunchKernelGGL(k_stencil_apply, dim3(interior_blocks(G)), dim3(KB), 0, ctx->stream, G, S.ax, S.ay, S.diag,
+                       d_x, d_y);
     ASD2_LAUNCH_CHECK();
     return ASD2_OK;
 }
@@ -753,30 +813,35 @@ int asd2_poisson_bicgstab(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps,
     ASD2_TRY(ws(ctx, "bi_s", n, &s));
     ASD2_TRY(ws(ctx, "bi_sh", n, &sh));
     ASD2_TRY(ws(ctx, "bi_t", n, &t));
-    const dim3 gr = grid_rows(G);
+    const int nbf = flat_blocks(n), nbi = interior_blocks(G);
     Driver D;
     ASD2_TRY(driver_init(ctx, &D));
     ASD2_TRY(set_state(&D, o));
-    hipLaunchKernelGGL(k_bicg_init, dim3(FLAT_BLOCKS), dim3(KB), 0, ctx->stream, n, S.rhs, x, r, rt, D.st, D.partials);
+    hipLaunchKernelGGL(k_bicg_init, dim3(nbf), dim3(KB), 0, ctx->stream, n, S.rhs, x, r, rt, D.partials);
     ASD2_LAUNCH_CHECK();
+    ASD2_TRY(tail<T_BICG_INIT>(&D, nbf));
     auto batch = [&](int count) -> int {
         for (int k = 0; k < count; k++) {
-            hipLaunchKernelGGL(k_bicg_dir, dim3(FLAT_BLOCKS), dim3(KB), 0, ctx->stream, n, r, p, v, S.dinv, ph, D.st);
-            hipLaunchKernelGGL(k_stencil_dot, gr, dim3(KB), 0, ctx->stream, G, S.ax, S.ay, S.diag, ph, v, rt,
-                               static_cast<const double*>(nullptr), D.st, D.partials, 1);
-            hipLaunchKernelGGL(k_bicg_s, dim3(FLAT_BLOCKS), dim3(KB), 0, ctx->stream, n, r, v, S.dinv, s, sh, D.st,
+            hipLaunchKernelGGL(k_bicg_dir, dim3(nbf), dim3(KB), 0, ctx->stream, n, r, p, v, S.dinv, ph, D.st);
+            hipLaunchKernelGGL(k_stencil_dot, dim3(nbi), dim3(KB), 0, ctx->stream, G, S.ax, S.ay, S.diag, ph, v, rt,
+                               D.st, D.partials, 1);
+            ASD2_TRY(tail<T_BICG_ALPHA>(&D, nbi));
+            hipLaunchKernelGGL(k_bicg_s, dim3(nbf), dim3(KB), 0, ctx->stream, n, r, v, S.dinv, s, sh, D.st,
                                D.partials);
-            hipLaunchKernelGGL(k_stencil_dot, gr, dim3(KB), 0, ctx->stream, G, S.ax, S.ay, S.diag, sh, t, s, t, D.st,
-                               D.partials, 2);
-            hipLaunchKernelGGL(k_bicg_x, dim3(FLAT_BLOCKS), dim3(KB), 0, ctx->stream, n, x, ph, sh, r, s, t, rt,
-                               D.st, D.partials);
+            ASD2_TRY(tail<T_BICG_S>(&D, nbf));
+            hipLaunchKernelGGL(k_stencil_dot, dim3(nbi), dim3(KB), 0, ctx->stream, G, S.ax, S.ay, S.diag, sh, t, s,
+                               D.st, D.partials, 2);
+            ASD2_TRY(tail<T_BICG_OMEGA>(&D, nbi));
+            hipLaunchKernelGGL(k_bicg_x, dim3(nbf), dim3(KB), 0, ctx->stream, n, x, ph, sh, r, s, t, rt, D.st,
+                               D.partials);
+            ASD2_TRY(tail<T_BICG_X>(&D, nbf));
         }
         ASD2_LAUNCH_CHECK();
         return ASD2_OK;
     };
     KrylovState fs;
     ASD2_TRY(drive(&D, o->check_every > 0 ? o->check_every : 16, batch, []() { return ASD2_OK; }, false, &fs));
-    hipLaunchKernelGGL(k_copy_interior, gr, dim3(256), 0, ctx->stream, G, x, d_ps);
+    hipLaunchKernelGGL(k_copy_interior, dim3(nbi), dim3(KB), 0, ctx->stream, G, x, d_ps);
     ASD2_LAUNCH_CHECK();
     const int st = status_of(fs);
     ASD2_TRY(report_true_residual(ctx, &D, S, x, &fs));
@@ -792,27 +857,35 @@ int asd2_csr_pcg(asd2_ctx* ctx, int n, const int* d_rowptr, const int* d_cols, c
         set_error("bad CSR argument");
         return ASD2_ERR_ARG;
     }
+    if (o->mode != ASD2_CG_REFERENCE && o->mode != ASD2_CG_STANDARD) {
+        set_error("bad CG mode %d", o->mode);
+        return ASD2_ERR_ARG;
+    }
     double *r, *z, *p, *q, *dinv;
     ASD2_TRY(ws(ctx, "csr_r", n, &r));
     ASD2_TRY(ws(ctx, "csr_z", n, &z));
     ASD2_TRY(ws(ctx, "csr_p", n, &p));
     ASD2_TRY(ws(ctx, "csr_q", n, &q));
     ASD2_TRY(ws(ctx, "csr_dinv", n, &dinv));
-    const int blocks = static_cast<int>(std::min<int64_t>(FLAT_BLOCKS, cdiv(n, KB)));
+    const int nb = flat_blocks(n);
+    const int64_t n64 = n;
     Driver D;
     ASD2_TRY(driver_init(ctx, &D));
     ASD2_TRY(set_state(&D, o));
-    hipLaunchKernelGGL(k_recip, dim3(blocks), dim3(KB), 0, ctx->stream, static_cast<int64_t>(n), d_mdiag, dinv);
-    hipLaunchKernelGGL(k_cg_init<double>, dim3(blocks), dim3(KB), 0, ctx->stream, static_cast<int64_t>(n), d_b, d_x,
-                       r, static_cast<double*>(nullptr), static_cast<double*>(nullptr), dinv, z, p, D.st, D.partials);
+    hipLaunchKernelGGL(k_recip, dim3(nb), dim3(KB), 0, ctx->stream, n64, d_mdiag, dinv);
+    hipLaunchKernelGGL(k_cg_init<double>, dim3(nb), dim3(KB), 0, ctx->stream, n64, d_b, d_x, r,
+                       static_cast<double*>(nullptr), static_cast<double*>(nullptr), dinv, z, p, D.partials);
     ASD2_LAUNCH_CHECK();
+    ASD2_TRY(tail<T_CG_INIT>(&D, nb));
     auto batch = [&](int count) -> int {
         for (int k = 0; k < count; k++) {
-            hipLaunchKernelGGL(k_csr_dir, dim3(blocks), dim3(KB), 0, ctx->stream, static_cast<int64_t>(n), z, p, D.st);
-            hipLaunchKernelGGL(k_csr_spmv, dim3(blocks), dim3(KB), 0, ctx->stream, n, d_rowptr, d_cols, d_vals, p, q,
+            hipLaunchKernelGGL(k_csr_dir, dim3(nb), dim3(KB), 0, ctx->stream, n64, z, p, D.st);
+            hipLaunchKernelGGL(k_csr_spmv, dim3(nb), dim3(KB), 0, ctx->stream, n, d_rowptr, d_cols, d_vals, p, q,
                                D.st, D.partials);
-            hipLaunchKernelGGL(k_cg_update<double>, dim3(blocks), dim3(KB), 0, ctx->stream, static_cast<int64_t>(n),
-                               d_x, p, p, r, q, dinv, z, D.st, D.partials, 0);
+            ASD2_TRY(tail<T_CG_ALPHA>(&D, nb));
+            hipLaunchKernelGGL(k_cg_update<double>, dim3(nb), dim3(KB), 0, ctx->stream, n64, d_x, p, p, r, q, dinv,
+                               z, D.st, D.partials);
+            ASD2_TRY(tail<T_CG_BETA>(&D, nb));
         }
         ASD2_LAUNCH_CHECK();
         return ASD2_OK;
@@ -842,25 +915,30 @@ int asd2_csr_bicgstab(asd2_ctx* ctx, int n, const int* d_rowptr, const int* d_co
     ASD2_TRY(ws(ctx, "csr_sh", n, &sh));
     ASD2_TRY(ws(ctx, "csr_t", n, &t));
     ASD2_TRY(ws(ctx, "csr_dinv", n, &dinv));
-    const int blocks = static_cast<int>(std::min<int64_t>(FLAT_BLOCKS, cdiv(n, KB)));
+    const int nb = flat_blocks(n);
     const int64_t n64 = n;
     Driver D;
     ASD2_TRY(driver_init(ctx, &D));
     ASD2_TRY(set_state(&D, o));
-    hipLaunchKernelGGL(k_recip, dim3(blocks), dim3(KB), 0, ctx->stream, n64, d_mdiag, dinv);
-    hipLaunchKernelGGL(k_bicg_init, dim3(blocks), dim3(KB), 0, ctx->stream, n64, d_b, d_x, r, rt, D.st, D.partials);
+    hipLaunchKernelGGL(k_recip, dim3(nb), dim3(KB), 0, ctx->stream, n64, d_mdiag, dinv);
+    hipLaunchKernelGGL(k_bicg_init, dim3(nb), dim3(KB), 0, ctx->stream, n64, d_b, d_x, r, rt, D.partials);
     ASD2_LAUNCH_CHECK();
+    ASD2_TRY(tail<T_BICG_INIT>(&D, nb));
     auto batch = [&](int count) -> int {
         for (int k = 0; k < count; k++) {
-            hipLaunchKernelGGL(k_bicg_dir, dim3(blocks), dim3(KB), 0, ctx->stream, n64, r, p, v, dinv, ph, D.st);
-            hipLaunchKernelGGL(k_csr_dot, dim3(blocks), dim3(KB), 0, ctx->stream, n, d_rowptr, d_cols, d_vals, ph, v,
-                               rt, D.st, D.partials, 1);
-            hipLaunchKernelGGL(k_bicg_s, dim3(blocks), dim3(KB), 0, ctx->stream, n64, r, v, dinv, s, sh, D.st,
+            hipLaunchKernelGGL(k_bicg_dir, dim3(nb), dim3(KB), 0, ctx->stream, n64, r, p, v, dinv, ph, D.st);
+            hipLaunchKernelGGL(k_csr_dot, dim3(nb), dim3(KB), 0, ctx->stream, n, d_rowptr, d_cols, d_vals, ph, v, rt,
+                               D.st, D.partials, 1);
+            ASD2_TRY(tail<T_BICG_ALPHA>(&D, nb));
+            hipLaunchKernelGGL(k_bicg_s, dim3(nb), dim3(KB), 0, ctx->stream, n64, r, v, dinv, s, sh, D.st,
                                D.partials);
-            hipLaunchKernelGGL(k_csr_dot, dim3(blocks), dim3(KB), 0, ctx->stream, n, d_rowptr, d_cols, d_vals, sh, t,
-                               s, D.st, D.partials, 2);
-            hipLaunchKernelGGL(k_bicg_x, dim3(blocks), dim3(KB), 0, ctx->stream, n64, d_x, ph, sh, r, s, t, rt, D.st,
+            ASD2_TRY(tail<T_BICG_S>(&D, nb));
+            hipLaunchKernelGGL(k_csr_dot, dim3(nb), dim3(KB), 0, ctx->stream, n, d_rowptr, d_cols, d_vals, sh, t, s,
+                               D.st, D.partials, 2);
+            ASD2_TRY(tail<T_BICG_OMEGA>(&D, nb));
+            hipLaunchKernelGGL(k_bicg_x, dim3(nb), dim3(KB), 0, ctx->stream, n64, d_x, ph, sh, r, s, t, rt, D.st,
                                D.partials);
+            ASD2_TRY(tail<T_BICG_X>(&D, nb));
         }
         ASD2_LAUNCH_CHECK();
         return ASD2_OK;

@@ -245,6 +245,8 @@ __global__ __launch_bounds__(WTH) void k_absgrad(Geom G, const double* __restric
 // MinRK2 substage B: LS2 = LS1 - dtau S (|grad LS1|_sub - 1); phi = (phi + LS2)/2  (:462-481)
 // Sussman (SUBCELL = false, sign(phi0)): A writes LS1 without ghosts (:344-348),
 // B writes tmp = (phi + LS2)/2 with ghosts (:351-363).
+// SUB 2 (first-time Sussman, :405-421): LS2 = phi - dtau sign(phi0) (|grad LS1| - 1),
+// phi = (phi + LS2)/2, i.e. the second stage restarts from phi instead of LS1.
 template <bool SUBCELL, int SUB>
 __global__ __launch_bounds__(WTH) void k_reinit(Geom G, GhostRule R, const double* __restrict__ src,
                                                 const double* __restrict__ ls0, const double* phi_c,
@@ -267,7 +269,8 @@ __global__ __launch_bounds__(WTH) void k_reinit(Geom G, GhostRule R, const doubl
         {
 #pragma clang fp contract(off)
             const double S = SUBCELL ? smoothed_sign(I0, eps) : static_cast<double>(isign(I0));
-            const double x = sL[r + RH][tj + RH] - adt * S * (a - 1.0);
+            const double base = (SUB == 2) ? phi_c[G.idx(i, j)] : sL[r + RH][tj + RH];
+            const double x = base - adt * S * (a - 1.0);
             if (SUB == 0) val = x;
             else val = 0.5 * (phi_c[G.idx(i, j)] + x);
         }
@@ -405,6 +408,24 @@ __global__ __launch_bounds__(256) void k_unit_weno5(const double* __restrict__ F
     weno5(f[1] - f[0], f[2] - f[1], f[3] - f[2], f[4] - f[3], f[5] - f[4], f[6] - f[5], 1.0 / d, 1e-6 * d * d, fp, fm);
     FP[i] = fp;
     FM[i] = fm;
+}
+
+// HJWENO5_DerivAbsLS_2D (src/levelset2d.cpp:488-557): WENO5 one-sided derivatives of phi
+// along both axes, Godunov |grad phi| with the branch chosen by phi0 >= 0.  Off the hot path
+// (the reference's reinitialisations use ENO2); one thread per interior cell.
+__global__ __launch_bounds__(256) void k_weno_absgrad(Geom G, const double* __restrict__ ls,
+                                                      const double* __restrict__ ls0, double* __restrict__ out) {
+    const int j = G.g + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = G.g + blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= G.nx + G.g || j >= G.ny + G.g) return;
+    double f[7], dPX, dMX, dPY, dMY;
+    for (int k = 0; k < 7; k++) f[k] = ls[G.idx(i - 3 + k, j)];
+    weno5(f[1] - f[0], f[2] - f[1], f[3] - f[2], f[4] - f[3], f[5] - f[4], f[6] - f[5], 1.0 / G.dx,
+          1e-6 * G.dx * G.dx, dPX, dMX);
+    for (int k = 0; k < 7; k++) f[k] = ls[G.idx(i, j - 3 + k)];
+    weno5(f[1] - f[0], f[2] - f[1], f[3] - f[2], f[4] - f[3], f[5] - f[4], f[6] - f[5], 1.0 / G.dy,
+          1e-6 * G.dy * G.dy, dPY, dMY);
+    out[G.idx(i, j)] = godunov(ls0[G.idx(i, j)], dPX, dMX, dPY, dMY);
 }
 
 struct Launch {
@@ -643,6 +664,46 @@ int asd2_ls_reinit_original(asd2_ctx* ctx, double* d_ls, int iterations) {
         ASD2_LAUNCH_CHECK();
         ASD2_TRY(finish_ghosts(ctx, fused, d_ls));
     }
+    return ASD2_OK;
+}
+
+int asd2_ls_reinit_first_time(asd2_ctx* ctx, double* d_ls, int iterations) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!d_ls || iterations < 0) { set_error("bad argument"); return ASD2_ERR_ARG; }
+    GhostRule R;
+    bool fused;
+    ASD2_TRY(p_rule(ctx, &R, &fused));
+    const Geom& G = ctx->geo;
+    const size_t n = static_cast<size_t>(G.size());
+    double *ls0, *ls1;
+    ASD2_TRY(ws(ctx, "ri_ls0", n, &ls0));
+    ASD2_TRY(ws(ctx, "ri_ls1", n, &ls1));
+    ASD2_TRY(zero_async(ctx, ls1, n * 8));
+    // phi0 is taken before any BC; its normalisation by |grad phi0| (:389-396) keeps its sign
+    // and its zero set, the only properties the loop reads
+    ASD2_HIP(hipMemcpyAsync(ls0, d_ls, n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    ASD2_TRY(bc_apply_p(ctx, d_ls));
+    const Launch L = tiles(G, RJ, RI);
+    const int gh = fused ? 1 : 0;
+    for (int it = 0; it < iterations; it++) {
+        hipLaunchKernelGGL((k_reinit<false, 0>), L.grid, L.block, 0, ctx->stream, G, R, d_ls, ls0, d_ls, ls1, gh);
+        ASD2_LAUNCH_CHECK();
+        ASD2_TRY(finish_ghosts(ctx, fused, ls1));
+        hipLaunchKernelGGL((k_reinit<false, 2>), L.grid, L.block, 0, ctx->stream, G, R, ls1, ls0, d_ls, d_ls, gh);
+        ASD2_LAUNCH_CHECK();
+        ASD2_TRY(finish_ghosts(ctx, fused, d_ls));
+    }
+    return ASD2_OK;
+}
+
+int asd2_ls_weno_absgrad(asd2_ctx* ctx, const double* d_ls, const double* d_ls0, double* d_out) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!d_ls || !d_ls0 || !d_out) { set_error("null array"); return ASD2_ERR_ARG; }
+    const Geom& G = ctx->geo;
+    ASD2_TRY(zero_async(ctx, d_out, static_cast<size_t>(G.size()) * 8));
+    hipLaunchKernelGGL(k_weno_absgrad, dim3(cdiv(G.ny, 64), cdiv(G.nx, 4)), dim3(256), 0, ctx->stream, G, d_ls,
+                       d_ls0, d_out);
+    ASD2_LAUNCH_CHECK();
     return ASD2_OK;
 }
 

@@ -17,7 +17,8 @@ int system_arrays(asd2_ctx* ctx, System* S, bool want32);
 int ensure_system32(asd2_ctx* ctx, System* S);
 bool all_neumann_family(const asd2_ctx* ctx);
 
-// Device-resident Krylov scalars; updated by the last block of each reducing kernel.
+// Device-resident Krylov scalars; advanced by the one-block tail kernel that follows each
+// reducing kernel.
 struct KrylovState {
     double rho, rho_old, alpha, beta, omega;  // recurrence scalars
     double rr;                                // ||r||^2 of the recurrence
@@ -28,7 +29,6 @@ struct KrylovState {
     double pq, aux0, aux1, aux2;              // scratch reductions
     int iter, maxit, mode, done;              // done: 0 running, 1 converged, 2 maxit, 3 breakdown, 4 paused
     int updates, fallbacks, flags, pad;
-    unsigned int ticket[4];
 };
 
 // flexible PCG with the learned preconditioner (learned.hip)

@@ -26,16 +26,16 @@ sys.path.insert(0, ROOT)
 METRIC = "2048² two-phase step ms + PCG iters: learned/mixed vs FP64 PCG @1e-8 res"
 
 PEAK_HBM_GBS = 8000.0
-# algorithmic bytes per interior cell of each kernel family (each array read once and
-# written once per launch; halos free)
+# Algorithmic (compulsory) bytes per interior cell and launch: every array the unit of work
+# must read once and write once; halos, ghosts and re-reads are free.
 BYTES_PER_CELL = {
-    "pcg64_spmv": 7 * 8,     # z, p_old, ax, ay, diag -> p_new, q
+    "pcg64_spmv": 6 * 8,     # z, p_old, beta_x, beta_y -> p_new, q (the diagonal follows from the betas)
     "pcg64_update": 8 * 8,   # x, p, r, q, dinv -> x, r, z
-    "pcg32_spmv": 7 * 4,
+    "pcg32_spmv": 6 * 4,
     "pcg32_update": 8 * 4,
-    "weno_stage": 112 / 3,   # 32 + 40 + 40 bytes over the 3 stages of a step
-    "reinit": 28,            # substage A 24 B, substage B 32 B
-    "gfm": 9 * 8,            # ls, kappa, div -> ax, ay, diag, rhs, dinv + FP32 copies (4x4 B)
+    "weno_stage": 112 / 3,   # 32 + 40 + 40 bytes over the 3 stages of an advection step
+    "reinit": 28,            # MinRK2 substage A 24 B, substage B 32 B
+    "gfm": 7 * 8,            # phi, kappa, div -> beta_x, beta_y, rhs, dinv
 }
 
 

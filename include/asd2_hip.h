@@ -142,6 +142,12 @@ int asd2_ls_eno_absgrad(asd2_ctx* ctx, const double* d_ls, const double* d_ls0, 
 int asd2_ls_reinit_iterations(double eps, double max_time);
 /* LevelSetSolver2D::Reinit_Original_2D, src/levelset2d.cpp:264-316 (ENO2, sign(phi0)) */
 int asd2_ls_reinit_original(asd2_ctx* ctx, double* d_ls, int iterations);
+/* LevelSetSolver2D::FirstTimeOnlyReinit_Sussman_2D, src/levelset2d.cpp:377-431 (ENO2,
+ * sign(phi0), second stage restarted from phi; `iterations` from the end time
+ * max(dx nx, dy ny), see asd2_ls_reinit_iterations) */
+int asd2_ls_reinit_first_time(asd2_ctx* ctx, double* d_ls, int iterations);
+/* LevelSetSolver2D::HJWENO5_DerivAbsLS_2D, src/levelset2d.cpp:488-557 (interior written) */
+int asd2_ls_weno_absgrad(asd2_ctx* ctx, const double* d_ls, const double* d_ls0, double* d_out);
 /* LevelSetSolver2D::UpdateHeavisideFuncDeriv, src/levelset2d.cpp:27-41 (padded array) */
 int asd2_ls_heaviside_deriv(asd2_ctx* ctx, const double* d_ls, double* d_out);
 /* LevelSetSolver2D::GetSmoothedSignFunc, src/levelset2d.cpp:914-923 (padded array) */

@@ -100,8 +100,10 @@ public:
     asd2_ctx* Context();
 
 private:
+    struct Device;  // device context, workspace buffers
     void EnsureBC();
-    asd2_ctx* m_ctx = nullptr;
+    Device& Dev();
+    std::unique_ptr<Device> m_dev;
 };
 
 #endif

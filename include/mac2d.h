@@ -164,8 +164,10 @@ public:
     asd2_ctx* Context();
 
 private:
+    struct Device;  // device context and named workspace buffers
+    Device& Dev();
     void FluidParams(void* out) const;
-    asd2_ctx* m_ctx = nullptr;
+    std::unique_ptr<Device> m_dev;
     double m_tol = 1.0e-6;
     int m_mode = 0, m_precision = 0, m_precond = 0;
     bool m_gravityFix = false;

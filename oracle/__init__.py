@@ -107,7 +107,8 @@ class Oracle:
             "or_smoothed_sign": [G, _P, _P], "or_heaviside_deriv": [G, _P, _P],
             "or_sussman_constraint": [G, _P, _P, _P, _P],
             "or_reinit_minrk2": [G, _P, C.c_int], "or_reinit_sussman": [G, _P, C.c_int],
-            "or_reinit_original": [G, _P, C.c_int],
+            "or_reinit_original": [G, _P, C.c_int], "or_reinit_first_time": [G, _P, C.c_int],
+            "or_weno_absgrad": [G, _P, _P, _P],
             "or_update_kappa": [G, _P, _P], "or_heaviside": [G, _P, _P],
             "or_smooth_heaviside": [G, _P, _P], "or_divergence": [G, _P, _P, _P],
             "or_update_vel": [G, F, _P, _P, _P, _P, _P, _P, _P, _P],
@@ -183,6 +184,12 @@ class Oracle:
 
     def reinit_original(self, gr, ls, iters):
         self.lib.or_reinit_original(C.byref(gr), _ptr(ls), iters); return ls
+
+    def reinit_first_time(self, gr, ls, iters):
+        self.lib.or_reinit_first_time(C.byref(gr), _ptr(ls), iters); return ls
+
+    def weno_absgrad(self, gr, ls, ls0):
+        o = np.zeros_like(ls); self.lib.or_weno_absgrad(C.byref(gr), _ptr(ls), _ptr(ls0), _ptr(o)); return o
 
     def reinit_iterations(self, eps, max_time):
         return self.lib.or_reinit_iterations(eps, max_time)
@@ -282,7 +289,8 @@ class Reference:
             "ref_smoothed_sign": [G, _P, _P], "ref_heaviside_deriv": [G, _P, _P],
             "ref_sussman_constraint": [G, _P, _P, _P, _P],
             "ref_reinit_minrk2": [G, _P, C.c_double], "ref_reinit_sussman": [G, _P, C.c_double],
-            "ref_reinit_original": [G, _P, C.c_double],
+            "ref_reinit_original": [G, _P, C.c_double], "ref_reinit_first_time": [G, _P],
+            "ref_weno_absgrad": [G, _P, _P, _P],
         }
         for name, args in sig.items():
             fn = getattr(self.lib, name)
@@ -340,3 +348,9 @@ class Reference:
 
     def reinit_original(self, gr, ls, max_time):
         self.lib.ref_reinit_original(C.byref(gr), _ptr(ls), max_time); return ls
+
+    def reinit_first_time(self, gr, ls):
+        self.lib.ref_reinit_first_time(C.byref(gr), _ptr(ls)); return ls
+
+    def weno_absgrad(self, gr, ls, ls0):
+        o = np.zeros_like(ls); self.lib.ref_weno_absgrad(C.byref(gr), _ptr(ls), _ptr(ls0), _ptr(o)); return o

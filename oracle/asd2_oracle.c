@@ -368,6 +368,36 @@ void or_eno_absgrad(const or_grid* gr, const double* ls, const double* ls0, doub
         }
 }
 
+/* src/levelset2d.cpp:488-557: UnitHJWENO5 along every interior row and column of phi,
+ * Godunov |grad phi| with the branch of phi0 >= 0 */
+void or_weno_absgrad(const or_grid* gr, const double* ls, const double* ls0, double* out) {
+    const int nx = gr->nx, ny = gr->ny, g = gr->g;
+    const int lx = nx + 2 * g, ly = ny + 2 * g;
+    double* dPX = zalloc(gr); double* dMX = zalloc(gr);
+    double* dPY = zalloc(gr); double* dMY = zalloc(gr);
+    const int lmax = lx > ly ? lx : ly;
+    double* F = (double*)calloc((size_t)lmax, sizeof(double));
+    double* FP = (double*)calloc((size_t)lmax, sizeof(double));
+    double* FM = (double*)calloc((size_t)lmax, sizeof(double));
+    for (int64_t k = 0; k < or_size(gr); k++) out[k] = 0.0;
+    for (int j = g; j < ny + g; j++) {
+        for (int i = 0; i < lx; i++) { F[i] = ls[IDX(gr, i, j)]; FP[i] = 0.0; FM[i] = 0.0; }
+        or_unit_hjweno5(F, FP, FM, gr->dx, nx, g);
+        for (int i = 0; i < lx; i++) { dPX[IDX(gr, i, j)] = FP[i]; dMX[IDX(gr, i, j)] = FM[i]; }
+    }
+    for (int i = g; i < nx + g; i++) {
+        for (int j = 0; j < ly; j++) { F[j] = ls[IDX(gr, i, j)]; FP[j] = 0.0; FM[j] = 0.0; }
+        or_unit_hjweno5(F, FP, FM, gr->dy, ny, g);
+        for (int j = 0; j < ly; j++) { dPY[IDX(gr, i, j)] = FP[j]; dMY[IDX(gr, i, j)] = FM[j]; }
+    }
+    for (int i = g; i < nx + g; i++)
+        for (int j = g; j < ny + g; j++) {
+            const int64_t c = IDX(gr, i, j);
+            out[c] = godunov(ls0[c], dPX[c], dMX[c], dPY[c], dMY[c]);
+        }
+    free(dPX); free(dMX); free(dPY); free(dMY); free(F); free(FP); free(FM);
+}
+
 /* src/levelset2d.cpp:615-785 (Min-Gibou subcell fix on top of ENO2) */
 void or_subcell_absgrad(const or_grid* gr, const double* ls, const double* ls0, double* out) {
     const int nx = gr->nx, ny = gr->ny, g = gr->g;
@@ -618,6 +648,48 @@ void or_reinit_original(const or_grid* gr, double* ls, int iters) {
             for (int j = g; j < ny + g; j++) {
                 const int64_t c = IDX(gr, i, j);
                 LS2[c] = LS1[c] - adt * isign(ls0[c]) * (a2[c] - 1.0);
+            }
+        for (int i = g; i < nx + g; i++)
+            for (int j = g; j < ny + g; j++) {
+                const int64_t c = IDX(gr, i, j);
+                ls[c] = 0.5 * (ls[c] + LS2[c]);
+            }
+    }
+    or_bc_p(gr, ls);
+    free(ls0); free(a1); free(a2); free(LS1); free(LS2);
+}
+
+/* src/levelset2d.cpp:377-431: phi0 normalised by its ENO2 |grad phi0| where that is
+ * positive (then BC), Heun whose second stage restarts from phi, until the pseudo time
+ * reaches max(dx nx, dy ny) (`iters` = that loop count). */
+void or_reinit_first_time(const or_grid* gr, double* ls, int iters) {
+    const int nx = gr->nx, ny = gr->ny, g = gr->g;
+    const double eps = smin(gr->dx, gr->dy), adt = eps * 0.5;
+    const int64_t n = or_size(gr);
+    double* ls0 = zalloc(gr); double* a1 = zalloc(gr); double* a2 = zalloc(gr);
+    double* LS1 = zalloc(gr); double* LS2 = zalloc(gr);
+    memcpy(ls0, ls, sizeof(double) * (size_t)n);
+    or_eno_absgrad(gr, ls0, ls0, a1);
+    for (int i = g; i < nx + g; i++)
+        for (int j = g; j < ny + g; j++) {
+            const int64_t c = IDX(gr, i, j);
+            if (a1[c] > 0.0) ls0[c] = ls0[c] / a1[c];
+        }
+    or_bc_p(gr, ls0);
+    for (int it = 0; it < iters; it++) {
+        or_bc_p(gr, ls);
+        or_eno_absgrad(gr, ls, ls0, a1);
+        for (int i = g; i < nx + g; i++)
+            for (int j = g; j < ny + g; j++) {
+                const int64_t c = IDX(gr, i, j);
+                LS1[c] = ls[c] - adt * isign(ls0[c]) * (a1[c] - 1.0);
+            }
+        or_bc_p(gr, LS1);
+        or_eno_absgrad(gr, LS1, ls0, a2);
+        for (int i = g; i < nx + g; i++)
+            for (int j = g; j < ny + g; j++) {
+                const int64_t c = IDX(gr, i, j);
+                LS2[c] = ls[c] - adt * isign(ls0[c]) * (a2[c] - 1.0);
             }
         for (int i = g; i < nx + g; i++)
             for (int j = g; j < ny + g; j++) {

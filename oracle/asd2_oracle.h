@@ -66,6 +66,8 @@ void or_sussman_constraint(const or_grid* gr, const double* ls, const double* ls
                            const double* hd, double* out);
 void or_reinit_minrk2(const or_grid* gr, double* ls, int iters);
 void or_reinit_original(const or_grid* gr, double* ls, int iters);
+void or_reinit_first_time(const or_grid* gr, double* ls, int iters);
+void or_weno_absgrad(const or_grid* gr, const double* ls, const double* ls0, double* out);
 void or_reinit_sussman(const or_grid* gr, double* ls, int iters);
 int  or_reinit_iterations(double eps, double max_time);
 

@@ -157,6 +157,21 @@ int ref_reinit_original(const or_grid* gr, double* ls, double max_time) {
     return 0;
 }
 
+int ref_reinit_first_time(const or_grid* gr, double* ls) {
+    auto s = make_ls(gr, 0.0);
+    std::vector<double> l = vec(ls, grid_size(gr));
+    s->FirstTimeOnlyReinit_Sussman_2D(l);
+    out(l, ls);
+    return 0;
+}
+
+int ref_weno_absgrad(const or_grid* gr, const double* ls, const double* ls0, double* o) {
+    auto s = make_ls(gr, 0.0);
+    const int64_t n = grid_size(gr);
+    out(s->HJWENO5_DerivAbsLS_2D(vec(ls, n), vec(ls0, n)), o);
+    return 0;
+}
+
 int ref_reinit_sussman(const or_grid* gr, double* ls, double max_time) {
     auto s = make_ls(gr, max_time);
     std::vector<double> l = vec(ls, grid_size(gr));

@@ -82,6 +82,7 @@ def main():
     cases["absgrad"] = dict(grid=gr, phi=phi1, phi0=phi0, eno=R.eno_absgrad(gr, phi1, phi0),
                             subcell=R.subcell_absgrad(gr, phi1, phi0), sign=R.smoothed_sign(gr, phi0),
                             hderiv=R.heaviside_deriv(gr, phi1))
+    cases["weno_absgrad"] = dict(grid=gr, phi=phi1, phi0=phi0, out=R.weno_absgrad(gr, phi1, phi0))
     hd = R.heaviside_deriv(gr, phi1)
     cases["sussman_constraint"] = dict(grid=gr, phi=phi1, phi0=phi0, hd=hd,
                                        out=R.sussman_constraint(gr, phi1, phi0, hd))
@@ -95,6 +96,8 @@ def main():
     cases["sussman_5"] = dict(grid=gr, phi=phi, iters=np.array(5), out=R.reinit_sussman(gr, phi.copy(), 2.25 * dx))
     cases["original_5"] = dict(grid=gr, phi=phi, iters=np.array(5),
                                out=R.reinit_original(gr, phi.copy(), 2.25 * dx))
+    # first-time Sussman: the loop runs until the pseudo time reaches max(dx nx, dy ny)
+    cases["first_time"] = dict(grid=gr, phi=phi, out=R.reinit_first_time(gr, phi.copy()))
 
     # a 64^2 droplet (config-1 geometry) after one advection step + 3 MinRK2 iterations
     n64 = 64

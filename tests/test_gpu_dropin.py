@@ -1,0 +1,183 @@
+"""The drop-in C++ classes (include/levelset2d.h, mac2d.h, poisson2d.h, bc2d.h) driven the
+way the reference's own drivers drive them (tests/cpp/dropin_driver.cpp, linked against
+libasd2_hip.so only), checked against the FP64 oracle running the same sequence.
+
+Bars: level set 1e-12 (max abs); velocities and pressure relative 1e-9 after two full
+stationary-bubble steps with the pressure solved to 1e-10 (both sides then differ only by
+reduction order); with the reference CG, which stalls on this case, 1e-5."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from oracle import make_fluid, make_grid
+from tests import scenarios as S
+from tests.gfm_common import gfm_case, interior, rhs_vector, scatter, stencil_to_csr
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DRIVER = os.path.join(HERE, "cpp", "dropin_driver")
+
+
+def _run(*args):
+    if not os.path.exists(DRIVER):
+        pytest.fail(f"{DRIVER} is not built (python tests/cpp/build.py)")
+    print("dropin_driver", *args, flush=True)
+    res = subprocess.run([DRIVER, *map(str, args)], capture_output=True, text=True, timeout=100)
+    if res.returncode != 0:
+        raise AssertionError(f"driver failed ({res.returncode}):\n{res.stdout[-2000:]}\n{res.stderr[-2000:]}")
+    return res.stdout
+
+
+def _load(d, name, shape=None, dtype=np.float64):
+    a = np.fromfile(os.path.join(d, name), dtype=dtype)
+    return a.reshape(shape) if shape is not None else a
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("mode", [1, 0], ids=["tol1e-10", "reference-cg"])
+def test_stationary_bubble_steps(oracle, tmp_path, mode):
+    """mode 1: pressure to 1e-10 (textbook PCG).  mode 0: the reference CG as configured by
+    the reference driver (1e-6, poissonMaxIter 20000).  Its alpha denominator carries
+    +1e-12 (poisson2d.cpp:373), which on this bubble stalls the residual near 1.5e-6, so
+    the reference itself runs all 20000 iterations; both sides must do the same."""
+    n = 64
+    steps = 2 if mode == 1 else 1
+    tol, maxit = (1e-10, 20000) if mode == 1 else (1e-6, 20000)
+    _run("step", tmp_path, n, steps, mode)
+    shp = (n + 6, n + 6)
+    lenx = 0.04
+    dx = lenx / n
+    gr = make_grid(n, n, dx, dx, bc_p=("neumann",) * 4, bc_u=("dirichlet",) * 4, bc_v=("dirichlet",) * 4)
+    fl = make_fluid(1000.0, 1.226, 1.137e-3, 1.78e-5, 0.0728, 0.0, g=0.0, g_axis=1, gravity_fix=0, cfl=0.3)
+    g = 3
+    xc = (np.arange(n + 6) + 0.5 - g) * dx
+    X, Y = np.meshgrid(xc, xc, indexing="ij")
+    ls = np.ascontiguousarray(np.sqrt((X - 0.02) ** 2 + (Y - 0.02) ** 2) - 0.01)
+    K = oracle.reinit_iterations(dx, 1.5 * (1.5 * dx))
+    oracle.bc_p(gr, ls)
+    oracle.reinit_minrk2(gr, ls, K)
+    oracle.bc_p(gr, ls)
+    np.testing.assert_allclose(_load(tmp_path, "ls0.bin", shp), ls, rtol=0, atol=1e-12)
+    u = np.zeros(shp); v = np.zeros(shp); ps = np.zeros(shp); kappa = np.zeros(shp)
+    oracle.bc_u(gr, u); oracle.bc_v(gr, v); oracle.bc_p(gr, ps)
+    dt = 0.3 * dx
+    dts, iters = [], []
+    for _ in range(steps):
+        fl.dt = dt
+        ls = oracle.solve_levelset(gr, ls, u, v, dt)
+        oracle.bc_p(gr, ls)
+        oracle.reinit_minrk2(gr, ls, K)
+        oracle.bc_p(gr, ls)
+        oracle.update_kappa(gr, ls, kappa)
+        H = oracle.heaviside(gr, ls)
+        Hs = oracle.smooth_heaviside(gr, ls)
+        uh, vh = oracle.intermediate_vel(gr, fl, 2, ls, u, v, Hs)
+        oracle.bc_u(gr, uh); oracle.bc_v(gr, vh)
+        div = oracle.divergence(gr, uh, vh)
+        oracle.bc_p(gr, ps); oracle.bc_p(gr, ls)
+        oracle.update_kappa(gr, ls, kappa)
+        oracle.bc_p(gr, kappa)
+        ax, ay, dg, rhs = oracle.gfm_assemble(gr, fl, ls, kappa, H, div)
+        x, it, _, st = oracle.pcg(gr, ax, ay, dg, rhs, tol, maxit, 1 if mode == 1 else 0)
+        assert st == 0 if mode == 1 else (st == 1 and it == maxit)
+        ps = np.zeros(shp)
+        interior(gr, ps)[:] = interior(gr, x)
+        oracle.bc_p(gr, ps)
+        oracle.update_vel(gr, fl, u, v, uh, vh, ps, ls, kappa, H)
+        oracle.bc_u(gr, u); oracle.bc_v(gr, v)
+        dts.append(dt); iters.append(it)
+        dt = oracle.update_dt(gr, fl, u, v)
+    dts.append(dt)
+    got = {k: _load(tmp_path, f"{k}.bin", shp) for k in ("ls", "u", "v", "ps", "p", "kappa")}
+    print("iterations driver", _load(tmp_path, "iters.bin", dtype=np.int32), "oracle", iters)
+    assert np.abs(got["ls"] - ls).max() <= 1e-12
+    # a stalled recurrence keeps moving x at the 1e-6 level; the converged one is exact
+    bar = 1e-9 if mode == 1 else 1e-5
+    for k, ref in (("u", u), ("v", v), ("ps", ps), ("kappa", kappa)):
+        print(k, _rel(got[k], ref))
+        assert _rel(got[k], ref) <= bar, k
+    p_ref = np.zeros(shp)
+    interior(gr, p_ref)[:] = interior(gr, ps) / dts[-2]
+    assert _rel(interior(gr, got["p"]), interior(gr, p_ref)) <= bar
+    np.testing.assert_allclose(_load(tmp_path, "dt.bin"), dts, rtol=1e-12)
+    assert np.all(np.abs(_load(tmp_path, "iters.bin", dtype=np.int32) - np.array(iters)) <= 1)
+
+
+def test_levelset_class_methods(oracle, tmp_path):
+    nx, ny = 48, 40
+    dx = 1.0 / nx
+    phi = S.circle_sdf(nx, ny, dx, dx, 0.5, 0.45 * ny * dx, 0.25)
+    phi = np.ascontiguousarray(phi * (1.0 + 0.2 * np.sin(3 * phi / dx)))
+    u, v = S.smooth_velocity(5, nx, ny, dx, dx)
+    for name, a in (("phi", phi), ("u", u), ("v", v)):
+        a.tofile(tmp_path / f"{name}.bin")
+    _run("ls", tmp_path, nx, ny)
+    shp = phi.shape
+    gr = make_grid(nx, ny, dx, dx, bc_p=("neumann", "dirichlet", "wall", "outlet"), c_p=(0.0, 0.1, 0.0, -0.05))
+    K = oracle.reinit_iterations(dx, 1.5 * (1.5 * dx))
+    a = oracle.solve_levelset(gr, phi.copy(), u, v, 0.4 * dx)
+    checks = {
+        "advect": a,
+        "minrk2": oracle.reinit_minrk2(gr, phi.copy(), K),
+        "sussman": oracle.reinit_sussman(gr, phi.copy(), K),
+        "original": oracle.reinit_original(gr, phi.copy(), K),
+        "first_time": oracle.reinit_first_time(gr, phi.copy(), oracle.reinit_iterations(dx, max(dx * nx, dx * ny))),
+        "subcell": oracle.subcell_absgrad(gr, a, phi),
+        "eno": oracle.eno_absgrad(gr, a, phi),
+        "weno_absgrad": oracle.weno_absgrad(gr, a, phi),
+        "sign": oracle.smoothed_sign(gr, phi),
+    }
+    for k, ref in checks.items():
+        err = np.abs(_load(tmp_path, f"{k}.bin", shp) - ref).max()
+        print(k, err)
+        # the first-time reinitialisation runs max(dx nx, dy ny) / dtau ~ 2 nx pseudo-time
+        # iterations; rounding differences of the reorganised ENO2 accumulate over them
+        assert err <= (1e-10 if k == "first_time" else 1e-12), k
+    hd = oracle.heaviside_deriv(gr, a)
+    assert _rel(_load(tmp_path, "hderiv.bin", shp), hd) <= 1e-14
+    p = phi.copy()
+    L = oracle.hjweno5_ls(gr, p, u, v)
+    assert _rel(_load(tmp_path, "weno_rhs.bin", shp), L) <= 1e-13
+    np.testing.assert_array_equal(_load(tmp_path, "weno_rhs_phi.bin", shp), p)
+    np.testing.assert_array_equal(_load(tmp_path, "bc_p.bin", shp), oracle.bc_p(gr, u.copy()))
+    np.testing.assert_array_equal(_load(tmp_path, "scalars.bin"), [K, -1.0, -1.0, 0.0, 0.5])
+
+
+@pytest.mark.parametrize("all_neumann", [1, 0], ids=["pinned", "dirichlet-south"])
+def test_poisson_class_csr(oracle, tmp_path, all_neumann):
+    bc = ("neumann",) * 4 if all_neumann else ("neumann", "neumann", "dirichlet", "neumann")
+    gr, fl, phi, kap, H, div = gfm_case(oracle, 36, 30, seed=71, bc_p=bc)
+    ax, ay, dg, rhs = oracle.gfm_assemble(gr, fl, phi, kap, H, div)
+    A, ids = stencil_to_csr(gr, ax, ay, dg, pin=bool(all_neumann))
+    n = A.shape[0]
+    A.data.astype(np.float64).tofile(tmp_path / "AVals.bin")
+    A.indices.astype(np.int32).tofile(tmp_path / "ACols.bin")
+    A.indptr.astype(np.int32).tofile(tmp_path / "ARowIdx.bin")
+    A.diagonal().astype(np.float64).tofile(tmp_path / "MVals.bin")
+    np.arange(n, dtype=np.int32).tofile(tmp_path / "MCols.bin")
+    np.arange(n + 1, dtype=np.int32).tofile(tmp_path / "MRowIdx.bin")
+    rhs.tofile(tmp_path / "rhs.bin")
+    _run("csr", tmp_path, gr.nx, gr.ny, all_neumann)
+    shp = rhs.shape
+    b = rhs_vector(gr, rhs, ids)
+    x_ref, it_ref, _, st = oracle.pcg_csr(A.indptr, A.indices, A.data, A.diagonal(), b, 1e-6, 100000, 0)
+    assert st == 0
+    iters = _load(tmp_path, "iters.bin", dtype=np.int32)
+    assert abs(int(iters[0]) - it_ref) <= 1
+    got = _load(tmp_path, "ps_cg.bin", shp)
+    assert _rel(interior(gr, got), interior(gr, scatter(gr, x_ref, ids))) <= 1e-9
+    x_direct = spla.spsolve(A.tocsc(), b)
+    for name in ("ps_cg_tight", "ps_bicg"):
+        got = _load(tmp_path, f"{name}.bin", shp)
+        err = _rel(interior(gr, got), interior(gr, scatter(gr, x_direct, ids)))
+        print(name, err)
+        assert err <= 1e-7, name
+    np.testing.assert_array_equal(_load(tmp_path, "minv.bin"), 1.0 / A.diagonal())
+    assert _load(tmp_path, "sym.bin", dtype=np.int32)[0] == 1

@@ -165,6 +165,35 @@ def test_reinit_original(oracle, bc):
     assert err <= TOL_PHI
 
 
+def test_reinit_first_time(oracle):
+    nx, ny = 40, 36
+    phi, dx = _level_set(nx, ny, 16)
+    phi = np.ascontiguousarray(phi * 1.8)
+    gr = make_grid(nx, ny, dx, dx)
+    iters = oracle.reinit_iterations(dx, max(dx * nx, dx * ny))
+    ref = oracle.reinit_first_time(gr, phi.copy(), iters)
+    ctx = _ctx(nx, ny, dx, dx)
+    d = ctx.array(phi)
+    ctx.ls_reinit_first_time(d, iters)
+    err = np.abs(d.download() - ref).max()
+    print(f"first-time sussman ({iters} iterations): max|dphi| = {err:.3e}")
+    # ~2 nx pseudo-time iterations: rounding differences of the reorganised ENO2 accumulate
+    assert err <= 1e-10
+
+
+def test_weno_absgrad(oracle):
+    nx, ny = 59, 62
+    phi, dx = _level_set(nx, ny, 17)
+    gr = make_grid(nx, ny, dx, dx)
+    oracle.bc_p(gr, phi)
+    phi1 = np.ascontiguousarray(phi + 0.3 * dx * S.random_field(18, phi.shape))
+    ref = oracle.weno_absgrad(gr, phi1, phi)
+    ctx = _ctx(nx, ny, dx, dx)
+    out = ctx.array()
+    ctx.ls_weno_absgrad(ctx.array(phi1), ctx.array(phi), out)
+    assert np.abs(out.download() - ref).max() <= 1e-12
+
+
 def test_pointwise_sign_and_delta(oracle):
     nx, ny = 50, 41
     phi, dx = _level_set(nx, ny, 13)

@@ -34,7 +34,7 @@ def c(a):
 
 def test_all_fixtures_present():
     names = {os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLD, "*.npz"))}
-    assert {"advect_neumann", "minrk2_3", "sussman_5", "original_5", "droplet64_step", "bc_p_mixed"} <= names
+    assert {"advect_neumann", "minrk2_3", "sussman_5", "original_5", "first_time", "weno_absgrad", "droplet64_step", "bc_p_mixed"} <= names
 
 
 @pytest.mark.parametrize("name", ["bc_p_neumann", "bc_p_dirichlet", "bc_p_mixed"])
@@ -102,6 +102,17 @@ def test_original(oracle):
     np.testing.assert_array_equal(oracle.reinit_original(gr, c(d["phi"]), int(d["iters"])), d["out"])
 
 
+def test_first_time(oracle):
+    gr, d = load("first_time")
+    iters = oracle.reinit_iterations(min(gr.dx, gr.dy), max(gr.dx * gr.nx, gr.dy * gr.ny))
+    np.testing.assert_array_equal(oracle.reinit_first_time(gr, c(d["phi"]), iters), d["out"])
+
+
+def test_weno_absgrad(oracle):
+    gr, d = load("weno_absgrad")
+    np.testing.assert_array_equal(oracle.weno_absgrad(gr, c(d["phi"]), c(d["phi0"])), d["out"])
+
+
 def test_droplet_step(oracle):
     gr, d = load("droplet64_step")
     out = oracle.solve_levelset(gr, c(d["phi"]), c(d["u"]), c(d["v"]), float(d["dt"]))
@@ -125,6 +136,9 @@ def test_live_levelset_path(oracle, reference, seed, kinds):
     np.testing.assert_array_equal(oracle.reinit_minrk2(gr, a, 5), reference.reinit_minrk2(gr, b, 2.25 * dx))
     np.testing.assert_array_equal(oracle.reinit_sussman(gr, a, 3), reference.reinit_sussman(gr, b, 1.25 * dx))
     np.testing.assert_array_equal(oracle.reinit_original(gr, a, 5), reference.reinit_original(gr, b, 2.25 * dx))
+    np.testing.assert_array_equal(oracle.weno_absgrad(gr, a, phi), reference.weno_absgrad(gr, b, phi))
+    iters = oracle.reinit_iterations(dx, max(dx * nx, dx * ny))
+    np.testing.assert_array_equal(oracle.reinit_first_time(gr, a, iters), reference.reinit_first_time(gr, b))
 
 
 def test_live_iteration_count(oracle):
