@@ -99,6 +99,7 @@ def load() -> C.CDLL:
         "asd2_memcpy_d2h": ([_VP, _VP, C.c_size_t], C.c_int),
         "asd2_memcpy_d2d": ([_VP, _VP, C.c_size_t], C.c_int),
         "asd2_device_synchronize": ([], C.c_int),
+        "asd2_stream_copy": ([_VP, _VP, _VP, C.c_size_t], C.c_int),
         "asd2_create": ([P(Grid), P(_VP)], C.c_int), "asd2_destroy": ([_VP], C.c_int),
         "asd2_synchronize": ([_VP], C.c_int),
         "asd2_set_bc": ([_VP, C.c_int, P(C.c_int), P(C.c_double)], C.c_int),
@@ -249,6 +250,10 @@ class Context:
 
     def sync(self):
         check(self.lib.asd2_synchronize(self.h))
+
+    def stream_copy(self, dst: DeviceArray, src: DeviceArray):
+        assert dst.nbytes == src.nbytes and dst.dtype == np.float64
+        check(self.lib.asd2_stream_copy(self.h, dst.ptr, src.ptr, dst.nbytes // 8))
 
     def set_bc(self, grid_kind, kinds, consts=(0.0, 0.0, 0.0, 0.0)):
         check(self.lib.asd2_set_bc(self.h, grid_kind, _bc_codes(kinds), (C.c_double * 4)(*consts)))

@@ -52,6 +52,14 @@ __global__ void k_zero(uint64_t* p, size_t n64, unsigned char* tail, size_t ntai
     if (blockIdx.x == 0 && threadIdx.x < ntail) tail[threadIdx.x] = 0;
 }
 
+// dst = src over `count` doubles: the reference streaming kernel for calibrating HBM byte
+// counters (one read and one write per element)
+__global__ __launch_bounds__(256) void k_stream_copy(double* __restrict__ dst, const double* __restrict__ src,
+                                                     int64_t count) {
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < count; i += static_cast<int64_t>(gridDim.x) * 256)
+        dst[i] = src[i];
+}
+
 int zero_async(asd2_ctx* ctx, void* p, size_t bytes) {
     if (bytes == 0) return ASD2_OK;
     const size_t n64 = bytes / 8;
@@ -157,6 +165,16 @@ int asd2_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes) {
 
 int asd2_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes) {
     ASD2_HIP(hipMemcpy(d_dst, d_src, bytes, hipMemcpyDeviceToDevice));
+    return ASD2_OK;
+}
+
+int asd2_stream_copy(asd2_ctx* ctx, double* d_dst, const double* d_src, size_t count) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!d_dst || !d_src) { set_error("null array"); return ASD2_ERR_ARG; }
+    const int blocks = static_cast<int>(std::min<size_t>(4096, (count + 255) / 256));
+    hipLaunchKernelGGL(k_stream_copy, dim3(std::max(blocks, 1)), dim3(256), 0, ctx->stream, d_dst, d_src,
+                       static_cast<int64_t>(count));
+    ASD2_LAUNCH_CHECK();
     return ASD2_OK;
 }
 

@@ -7,7 +7,7 @@ FP64 residual of 1e-8, projection, CFL time step.  The step is timed twice from 
 state: with the plain FP64 Jacobi-PCG and with the FP32 PCG + FP64 reliable updates.
 `value` is the ms/step of the faster pressure path (inputs resident in HBM).
 
-Usage: python bench.py [--steps K] [--warmup W] [--n 2048] [--no-cpu]
+Usage: python bench.py [--steps K] [--warmup W] [--n 2048] [--no-cpu] [--pmc-workload]
 """
 from __future__ import annotations
 
@@ -189,6 +189,35 @@ def cpu_baseline(n, gpu_iters_fp64, budget_s=20.0):
             "cpu": cpu_info()[0], "nproc": cpu_info()[1]}
 
 
+def pmc_workload(n):
+    """A short fixed workload for hardware-counter runs (rocprofv3 --pmc): three plain copies
+    of 2^25 doubles (the calibration kernel), the stencil part of one step and 32 iterations
+    of each pressure path."""
+    import asd2_amd as A
+    sim = Droplet(n, A)
+    copy_n = 1 << 25
+    src, dst = A.DeviceArray((copy_n,)), A.DeviceArray((copy_n,))
+    for _ in range(3):
+        sim.ctx.stream_copy(dst, src)
+    fl = sim.fluid()
+    c = sim.ctx
+    c.ls_advect(sim.ls, sim.u, sim.v, sim.dt)
+    c.ls_reinit_minrk2(sim.ls, 3)
+    c.kappa(sim.ls, sim.kap)
+    c.heaviside(sim.ls, sim.H)
+    c.heaviside(sim.ls, sim.Hs, smooth=True)
+    c.intermediate_vel(fl, 2, sim.ls, sim.u, sim.v, sim.uh, sim.vh, sim.Hs)
+    c.divergence(sim.uh, sim.vh, sim.div)
+    c.bc_apply(A.GRID_P, sim.kap)
+    c.gfm_assemble(fl, sim.ls, sim.kap, sim.H, sim.div)
+    for prec in (A.PREC_FP64, A.PREC_MIXED):
+        c.poisson_solve(sim.ps, A.Context.opts(tol=1e-30, max_iter=32, precision=prec, mode=A.CG_STANDARD,
+                                               check_every=8), raise_on_fail=False)
+    c.sync()
+    print(json.dumps({"pmc_workload": {"grid": n, "cells": n * n, "copy_doubles": copy_n,
+                                       "copy_bytes_per_launch": 16 * copy_n}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,12 +227,17 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--pmc-workload", action="store_true",
+                    help="run the short fixed workload used for hardware-counter passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
                                    "(halo exchange + scalar all-reduce per PCG dot)"}))
         sys.exit(2)
 
+    if args.pmc_workload:
+        pmc_workload(args.n)
+        return
     import asd2_amd as A
     sim = Droplet(args.n, A)
     snap = sim.snapshot()

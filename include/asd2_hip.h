@@ -108,6 +108,9 @@ int asd2_free(void* d_ptr);
 int asd2_memcpy_h2d(void* d_dst, const void* h_src, size_t bytes);
 int asd2_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes);
 int asd2_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes);
+/* d_dst[0:count] = d_src[0:count] by a plain streaming kernel on the context's stream.  No
+ * reference counterpart: it is the known-size copy that HBM byte counters are calibrated on. */
+int asd2_stream_copy(asd2_ctx* ctx, double* d_dst, const double* d_src, size_t count);
 int asd2_device_synchronize(void);
 
 /* ---------------------------------------------------------------- context */
