@@ -3,11 +3,15 @@
 //
 // Each CG iteration is two streaming kernels: (1) direction update p = z + beta p fused
 // with the 5-point SpMV q = A p and the block partials of p.q, (2) the x/r/z update fused
-// with the block partials of r.z and r.r.  Every reduction finishes in a one-block "tail"
-// kernel that sums the partials in block order (deterministic) and advances the
-// device-resident Krylov scalars, so the host never waits inside an iteration.  A grid-wide
-// "last block" reduction would need a device-scope release fence per block, which on this
-// chip writes back the XCD's L2 each time; the tail kernel costs about 2 us instead.
+// with the block partials of r.z and r.r.  Each kernel starts by summing the partials of
+// the kernel before it: every block reads the same <= 1024 partials in the same order, so
+// all blocks derive bit-identical scalars (alpha, beta, the convergence decision), and
+// block 0 records them in the device-resident state, into slots indexed by the iteration's
+// parity so that no block reads a slot written in the same launch.  A grid-wide "last
+// block" reduction would need a device-scope release fence per block, which on this chip
+// writes back the XCD's L2 each time (measured: 124 us instead of 16 us for an axpy-dot
+// over 4096 blocks); a separate one-block reduction kernel costs about 4 us per launch.
+// BiCGStab and the rare bookkeeping steps still use such one-block "tail" kernels.
 // Reducing kernels use at most MAX_BLOCKS blocks; each block walks a contiguous range of
 // tiles, and consecutive ranges are placed on the same XCD (workgroup b runs on XCD b % 8)
 // so stencil halos shared by neighbouring tiles hit the same L2.  The host enqueues
@@ -79,6 +83,21 @@ __device__ __forceinline__ void sum_partials(const double* partials, int nb, dou
     block_sum(v);
 }
 
+// totals of nb partials per value, in every thread; the summation order is fixed, so every
+// block of a launch obtains the same bits
+template <int NV>
+__device__ __forceinline__ void sum_partials_all(const double* partials, int nb, double (&v)[NV]) {
+    __shared__ double bc[NV];
+    sum_partials(partials, nb, v);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) bc[k] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = bc[k];
+}
+
 // logical block index: consecutive logical indices run on the same XCD
 __device__ __forceinline__ int logical_block() {
     const int nb = gridDim.x, b = blockIdx.x;
@@ -112,13 +131,10 @@ __device__ __forceinline__ void for_interior(const Geom& G, F f) {
 
 // ------------------------------------------------------------------ tail kernels
 enum TailOp {
-    T_CG_INIT,          // b.b, r.z, r.r  -> bnorm, rho, convergence
-    T_CG_ALPHA,         // p.q            -> alpha
-    T_CG_BETA,          // r.z, r.r       -> iter++, beta, convergence
-    T_CG_BETA_MIXED,    //                   ... pausing for reliable updates
+    T_CG_INIT,          // r.z, r.r, b.b  -> bnorm, initial convergence
     T_TRUE_RES,         // ||b - A x||^2  -> reliable update bookkeeping
     T_TRUE_RES_REPORT,  //                -> reported true residual
-    T_MX_RESUME,        // r.z, r.r       -> beta after a reliable update
+    T_MX_RESUME,        // (no sum)       -> back to RUNNING after a reliable update
     T_BICG_INIT,        // b.b            -> bnorm, rho
     T_BICG_ALPHA,       // rt.v           -> alpha
     T_BICG_S,           // s.s            -> early exit on ||s||
@@ -128,9 +144,6 @@ enum TailOp {
 
 template <int OP> struct TailNV { static constexpr int value = 1; };
 template <> struct TailNV<T_CG_INIT> { static constexpr int value = 3; };
-template <> struct TailNV<T_CG_BETA> { static constexpr int value = 2; };
-template <> struct TailNV<T_CG_BETA_MIXED> { static constexpr int value = 2; };
-template <> struct TailNV<T_MX_RESUME> { static constexpr int value = 2; };
 template <> struct TailNV<T_BICG_OMEGA> { static constexpr int value = 2; };
 template <> struct TailNV<T_BICG_X> { static constexpr int value = 2; };
 
@@ -146,48 +159,25 @@ __device__ __forceinline__ bool tail_active(const KrylovState* st) {
 template <int OP>
 __global__ __launch_bounds__(KB) void k_tail(int nb, const double* __restrict__ partials, KrylovState* st) {
     if (!tail_active<OP>(st)) return;
+    if (OP == T_MX_RESUME) {
+        if (threadIdx.x == 0) st->done = DONE_RUNNING;
+        return;
+    }
     constexpr int NV = TailNV<OP>::value;
     double v[NV];
     sum_partials(partials, nb, v);
     if (threadIdx.x != 0) return;
     if (OP == T_CG_INIT) {
-        double bn = sqrt(v[0]);
+        double bn = sqrt(v[2]);
         if (bn == 0.0) bn = 1.0;
         st->bnorm = bn;
-        st->rho = v[1];
-        st->rho_old = 1.0;
-        st->rr = v[2];
-        st->rmax = sqrt(v[2]);
+        st->rr = v[1];
+        st->rmax = sqrt(v[1]);
         st->iter = 0;
         st->updates = 0;
         st->fallbacks = 0;
         st->flags = 0;
-        // reference iteration 0: beta = rho / rho1 with rho1 = 1 and p0 = r0 (poisson2d.cpp:331-360)
-        st->beta = (st->mode == ASD2_CG_REFERENCE) ? v[1] / 1.0 : 0.0;
-        st->done = (sqrt(v[2]) / bn <= st->tol) ? DONE_CONVERGED : DONE_RUNNING;
-    } else if (OP == T_CG_ALPHA) {
-        const double pq = v[0];
-        st->pq = pq;
-        // the reference adds err_tol^2 = 1e-12 to the denominator (poisson2d.cpp:373)
-        const double alpha = (st->mode == ASD2_CG_REFERENCE) ? st->rho / (pq + 1.0e-6 * 1.0e-6) : st->rho / pq;
-        st->alpha = alpha;
-        if (!isfinite(alpha)) st->done = DONE_BREAKDOWN;
-    } else if (OP == T_CG_BETA || OP == T_CG_BETA_MIXED) {
-        const bool mixed = (OP == T_CG_BETA_MIXED);
-        st->iter += 1;
-        const int it = st->iter;
-        st->rr = v[1];
-        st->rho_old = st->rho;
-        st->rho = v[0];
-        // reference iteration 1 restarts with p = z (poisson2d.cpp:350-352)
-        if (st->mode == ASD2_CG_REFERENCE) st->beta = (it == 1) ? 0.0 : st->rho / st->rho_old;
-        else st->beta = st->rho / st->rho_old;
-        const double rn = sqrt(v[1]);
-        const double resid = rn / st->bnorm;
-        if (!isfinite(resid) || !isfinite(st->beta)) st->done = DONE_BREAKDOWN;
-        else if (mixed && (resid <= st->tol || rn < st->delta * st->rmax)) st->done = DONE_PAUSED;
-        else if (!mixed && resid <= st->tol) st->done = DONE_CONVERGED;
-        else if (it >= st->maxit) st->done = mixed ? DONE_PAUSED : DONE_MAXIT;
+        st->done = (sqrt(v[1]) / bn <= st->tol) ? DONE_CONVERGED : DONE_RUNNING;
     } else if (OP == T_TRUE_RES) {
         const double rn = sqrt(v[0]);
         st->aux0 = rn;
@@ -197,11 +187,6 @@ __global__ __launch_bounds__(KB) void k_tail(int nb, const double* __restrict__ 
         else if (st->iter >= st->maxit) st->done = DONE_MAXIT;
     } else if (OP == T_TRUE_RES_REPORT) {
         st->aux2 = sqrt(v[0]) / st->bnorm;
-    } else if (OP == T_MX_RESUME) {
-        st->rho = v[0];
-        st->rr = v[1];
-        st->beta = st->rho / st->rho_old;
-        st->done = isfinite(st->beta) ? DONE_RUNNING : DONE_BREAKDOWN;
     } else if (OP == T_BICG_INIT) {
         double bn = sqrt(v[0]);
         const double rn = bn;
@@ -262,21 +247,74 @@ __global__ __launch_bounds__(KB) void k_cg_init(int64_t n, const double* __restr
         rz += static_cast<double>(rv) * static_cast<double>(zv);
         rr += static_cast<double>(rv) * static_cast<double>(rv);
     }
-    double v[3] = {bb, rz, rr};
+    double v[3] = {rz, rr, bb};
     put_partials(v, partials);
+}
+
+// The CG recurrence at the start of iteration `it` (poisson2d.cpp:345-395), from the
+// partials (r.z, r.r) of the kernel that produced r_it: the convergence, pause (mixed
+// precision) or iteration-limit decision and beta.  The reference's iteration 0 uses
+// p = rho r + z (beta = rho / 1 with p0 = r0, :331-360) and its iteration 1 restarts with
+// p = z (:350-352).  Every block computes the same values; block 0 records them.  Returns
+// false when the iteration must not run.
+template <bool MIXED>
+__device__ __forceinline__ bool cg_begin(KrylovState* st, const double* pu, int nbu, int it, double& beta) {
+    double v[2];
+    sum_partials_all(pu, nbu, v);
+    const double rz = v[0], rr = v[1];
+    const double rn = sqrt(rr), resid = rn / st->bnorm;
+    if (st->mode == ASD2_CG_REFERENCE) beta = (it == 0) ? rz / 1.0 : (it == 1 ? 0.0 : rz / st->rz[(it - 1) & 1]);
+    else beta = (it == 0) ? 0.0 : rz / st->rz[(it - 1) & 1];
+    int done = DONE_RUNNING;
+    if (!isfinite(resid) || !isfinite(beta)) done = DONE_BREAKDOWN;
+    else if (MIXED && (resid <= st->tol || rn < st->delta * st->rmax)) done = DONE_PAUSED;
+    else if (!MIXED && resid <= st->tol) done = DONE_CONVERGED;
+    else if (it >= st->maxit) done = MIXED ? DONE_PAUSED : DONE_MAXIT;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->rr = rr;
+        st->iter = it;
+        if (done == DONE_RUNNING) {
+            st->rz[it & 1] = rz;
+            st->beta = beta;
+        } else {
+            st->done = done;
+        }
+    }
+    return done == DONE_RUNNING;
+}
+
+// alpha of iteration `it` from the p.q partials; false (and BREAKDOWN recorded) if not finite
+__device__ __forceinline__ bool cg_alpha(KrylovState* st, const double* ps, int nbs, int it, double& alpha) {
+    double v[1];
+    sum_partials_all(ps, nbs, v);
+    // the reference adds err_tol^2 = 1e-12 to the denominator (poisson2d.cpp:373)
+    alpha = (st->mode == ASD2_CG_REFERENCE) ? st->rz[it & 1] / (v[0] + 1.0e-6 * 1.0e-6) : st->rz[it & 1] / v[0];
+    const bool ok = isfinite(alpha);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->pq = v[0];
+        st->alpha = alpha;
+        if (!ok) {
+            st->done = DONE_BREAKDOWN;
+            st->iter = it;
+        }
+    }
+    return ok;
 }
 
 // p_new = z + beta p_old on a tile with a one-cell halo, q = A p_new, partial p_new . q.
 // The two direction buffers alternate with the device iteration count (iteration k reads
 // P[k & 1] and writes P[(k + 1) & 1]), so iterations skipped while a solve is paused do
 // not shift the alternation.
-template <typename T>
+template <typename T, bool MIXED>
 __global__ __launch_bounds__(KB) void k_cg_spmv(Geom G, const T* __restrict__ ax, const T* __restrict__ ay,
                                                 const T* __restrict__ dg, const T* __restrict__ z, T* p0, T* p1,
-                                                T* __restrict__ q, KrylovState* st, double* partials) {
+                                                T* __restrict__ q, KrylovState* st, const double* pu, int nbu,
+                                                double* ps, int it) {
     if (st->done != DONE_RUNNING) return;
-    const T beta = static_cast<T>(st->beta);
-    const bool odd = (st->iter & 1) != 0;
+    double beta64;
+    if (!cg_begin<MIXED>(st, pu, nbu, it, beta64)) return;
+    const T beta = static_cast<T>(beta64);
+    const bool odd = (it & 1) != 0;
     const T* __restrict__ pold = odd ? p1 : p0;
     T* __restrict__ pnew = odd ? p0 : p1;
     __shared__ T sP[TI + 2][TJ + 2];
@@ -314,20 +352,22 @@ __global__ __launch_bounds__(KB) void k_cg_spmv(Geom G, const T* __restrict__ ax
         __syncthreads();
     }
     double v[1] = {acc};
-    put_partials(v, partials);
+    put_partials(v, ps);
 }
 
 // x += alpha p, r -= alpha q, z = M^-1 r, partials r.z and r.r (poisson2d.cpp:346-395).
-// p is the direction just written by k_cg_spmv (P[(iter + 1) & 1]); pass p0 == p1 for a
-// single buffer.
+// p is the direction written by k_cg_spmv of iteration `it` (P[(it + 1) & 1]); pass
+// p0 == p1 for a single buffer.
 template <typename T>
 __global__ __launch_bounds__(KB) void k_cg_update(int64_t n, T* __restrict__ x, const T* p0, const T* p1,
                                                   T* __restrict__ r, const T* __restrict__ q,
-                                                  const T* __restrict__ dinv, T* __restrict__ z,
-                                                  const KrylovState* st, double* partials) {
+                                                  const T* __restrict__ dinv, T* __restrict__ z, KrylovState* st,
+                                                  const double* ps, int nbs, double* pu, int it) {
     if (st->done != DONE_RUNNING) return;
-    const T a = static_cast<T>(st->alpha);
-    const T* __restrict__ p = (st->iter & 1) ? p0 : p1;
+    double alpha;
+    if (!cg_alpha(st, ps, nbs, it, alpha)) return;
+    const T a = static_cast<T>(alpha);
+    const T* __restrict__ p = (it & 1) ? p0 : p1;
     double rz = 0.0, rr = 0.0;
     FLAT_LOOP(k, n) {
         const T xv = a * p[k] + x[k];
@@ -340,7 +380,8 @@ __global__ __launch_bounds__(KB) void k_cg_update(int64_t n, T* __restrict__ x, 
         rr += static_cast<double>(rv) * static_cast<double>(rv);
     }
     double v[2] = {rz, rr};
-    put_partials(v, partials);
+    put_partials(v, pu);
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->iter = it + 1;
 }
 
 // ------------------------------------------------------------------ mixed-precision reliable updates
@@ -374,7 +415,8 @@ __global__ __launch_bounds__(KB) void k_true_residual(Geom G, const double* __re
     put_partials(v, partials);
 }
 
-// r32 = r64, z32 = M^-1 r32, partials r.z and r.r (the tail resumes the iteration)
+// r32 = r64, z32 = M^-1 r32, partials r.z and r.r in the layout of k_cg_update (the next
+// k_cg_spmv derives beta = rho_true / rho_prev from them, keeping the direction)
 __global__ __launch_bounds__(KB) void k_mx_resume(int64_t n, const double* __restrict__ r64, float* __restrict__ r32,
                                                   const float* __restrict__ dinv, float* __restrict__ z32,
                                                   const KrylovState* st, double* partials) {
@@ -487,15 +529,16 @@ __global__ __launch_bounds__(KB) void k_bicg_init(int64_t n, const double* __res
     put_partials(v, partials);
 }
 
-// CSR pieces (caller-built matrices, PoissonSolver2D::CG_2FUniform_2D)
+// CSR pieces (caller-built matrices, PoissonSolver2D::CG_2FUniform_2D): the recurrence
+// step and p = z + beta p in place, then q = A p with partial p.q
 __global__ __launch_bounds__(KB) void k_csr_dir(int64_t n, const double* __restrict__ z, double* __restrict__ p,
-                                                const KrylovState* st) {
+                                                KrylovState* st, const double* pu, int nbu, int it) {
     if (st->done != DONE_RUNNING) return;
-    const double beta = st->beta;
+    double beta;
+    if (!cg_begin<false>(st, pu, nbu, it, beta)) return;
     FLAT_LOOP(k, n) p[k] = z[k] + beta * p[k];
 }
 
-// q = A p, partial p.q
 __global__ __launch_bounds__(KB) void k_csr_spmv(int n, const int* __restrict__ rp, const int* __restrict__ cols,
                                                  const double* __restrict__ vals, const double* __restrict__ p,
                                                  double* __restrict__ q, const KrylovState* st, double* partials) {
@@ -545,7 +588,8 @@ struct Driver {
     KrylovState* st = nullptr;       // device
     KrylovState* host = nullptr;     // pinned, 2 slots
     hipEvent_t ev[2] = {nullptr, nullptr};
-    double* partials = nullptr;
+    double* pu = nullptr;            // partials of the residual producers (r.z, r.r[, b.b]); tail kernels
+    double* ps = nullptr;            // partials of the SpMV producers (p.q [, ...])
 };
 
 int flat_blocks(int64_t n) { return std::max(1, std::min<int>(MAX_BLOCKS, ceil_div(n, KB))); }
@@ -557,8 +601,8 @@ int interior_blocks(const Geom& G) {
 }
 
 template <int OP>
-int tail(Driver* D, int nb) {
-    hipLaunchKernelGGL(k_tail<OP>, dim3(1), dim3(KB), 0, D->ctx->stream, nb, D->partials, D->st);
+int tail(Driver* D, int nb, const double* partials) {
+    hipLaunchKernelGGL(k_tail<OP>, dim3(1), dim3(KB), 0, D->ctx->stream, nb, partials, D->st);
     ASD2_LAUNCH_CHECK();
     return ASD2_OK;
 }
@@ -567,7 +611,10 @@ int driver_init(asd2_ctx* ctx, Driver* D) {
     D->ctx = ctx;
     static_assert(sizeof(KrylovState) % 8 == 0, "state size");
     ASD2_TRY(ws(ctx, "kr_state", sizeof(KrylovState) / 8, reinterpret_cast<double**>(&D->st)));
-    ASD2_TRY(ws(ctx, "kr_partials", 4 * MAX_BLOCKS, &D->partials));
+    double* part = nullptr;
+    ASD2_TRY(ws(ctx, "kr_partials", 5 * MAX_BLOCKS, &part));
+    D->pu = part;
+    D->ps = part + 3 * MAX_BLOCKS;
     if (!ctx->kr_host) ASD2_HIP(hipHostMalloc(&ctx->kr_host, 2 * sizeof(KrylovState), hipHostMallocDefault));
     for (int k = 0; k < 2; k++)
         if (!ctx->kr_ev[k]) ASD2_HIP(hipEventCreateWithFlags(&ctx->kr_ev[k], hipEventDisableTiming));
@@ -597,7 +644,7 @@ int read_state(Driver* D, KrylovState* out) {
 }
 
 // Runs `batch(count)` repeatedly, polling one batch behind, until done is neither
-// RUNNING nor (when on_pause is given) PAUSED after on_pause ran.
+// RUNNING nor (when has_pause) PAUSED after on_pause(state) ran.
 template <class Batch, class Pause>
 int drive(Driver* D, int check_every, Batch batch, Pause on_pause, bool has_pause, KrylovState* final_state) {
     asd2_ctx* ctx = D->ctx;
@@ -614,7 +661,7 @@ int drive(Driver* D, int check_every, Batch batch, Pause on_pause, bool has_paus
         ASD2_HIP(hipEventSynchronize(D->ev[slot]));
         const KrylovState s = D->host[slot];
         if (s.done == DONE_PAUSED && has_pause) {
-            ASD2_TRY(on_pause());
+            ASD2_TRY(on_pause(s));
         } else if (s.done != DONE_RUNNING) {
             break;
         }
@@ -648,9 +695,9 @@ int report_true_residual(asd2_ctx* ctx, Driver* D, const System& S, const double
     const Geom& G = ctx->geo;
     const int nb = interior_blocks(G);
     hipLaunchKernelGGL(k_true_residual, dim3(nb), dim3(KB), 0, ctx->stream, G, S.ax, S.ay, S.diag, S.rhs, x,
-                       static_cast<double*>(nullptr), D->st, D->partials, 1);
+                       static_cast<double*>(nullptr), D->st, D->ps, 1);
     ASD2_LAUNCH_CHECK();
-    ASD2_TRY(tail<T_TRUE_RES_REPORT>(D, nb));
+    ASD2_TRY(tail<T_TRUE_RES_REPORT>(D, nb, D->ps));
     return read_state(D, fs);
 }
 
@@ -671,27 +718,27 @@ int solve_fp64(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2_solve
     ASD2_TRY(driver_init(ctx, &D));
     ASD2_TRY(set_state(&D, o));
     hipLaunchKernelGGL(k_cg_init<double>, dim3(nbf), dim3(KB), 0, ctx->stream, n, S.rhs, x, r,
-                       static_cast<double*>(nullptr), static_cast<double*>(nullptr), S.dinv, z, p0, D.partials);
+                       static_cast<double*>(nullptr), static_cast<double*>(nullptr), S.dinv, z, p0, D.pu);
     ASD2_LAUNCH_CHECK();
-    ASD2_TRY(tail<T_CG_INIT>(&D, nbf));
+    ASD2_TRY(tail<T_CG_INIT>(&D, nbf, D.pu));
+    int it = 0;
     auto batch = [&](int count) -> int {
-        for (int k = 0; k < count; k++) {
+        for (int k = 0; k < count; k++, it++) {
             prof_begin(ctx, ASD2_KID_PCG64_SPMV);
-            hipLaunchKernelGGL(k_cg_spmv<double>, dim3(nbs), dim3(KB), 0, ctx->stream, G, S.ax, S.ay, S.diag, z, p0,
-                               p1, q, D.st, D.partials);
+            hipLaunchKernelGGL((k_cg_spmv<double, false>), dim3(nbs), dim3(KB), 0, ctx->stream, G, S.ax, S.ay,
+                               S.diag, z, p0, p1, q, D.st, D.pu, nbf, D.ps, it);
             prof_end(ctx, ASD2_KID_PCG64_SPMV);
-            ASD2_TRY(tail<T_CG_ALPHA>(&D, nbs));
             prof_begin(ctx, ASD2_KID_PCG64_UPDATE);
             hipLaunchKernelGGL(k_cg_update<double>, dim3(nbf), dim3(KB), 0, ctx->stream, n, x, p0, p1, r, q, S.dinv,
-                               z, D.st, D.partials);
+                               z, D.st, D.ps, nbs, D.pu, it);
             prof_end(ctx, ASD2_KID_PCG64_UPDATE);
-            ASD2_TRY(tail<T_CG_BETA>(&D, nbf));
         }
         ASD2_LAUNCH_CHECK();
         return ASD2_OK;
     };
     KrylovState fs;
-    ASD2_TRY(drive(&D, o->check_every > 0 ? o->check_every : 32, batch, []() { return ASD2_OK; }, false, &fs));
+    ASD2_TRY(drive(&D, o->check_every > 0 ? o->check_every : 32, batch,
+                   [](const KrylovState&) { return ASD2_OK; }, false, &fs));
     hipLaunchKernelGGL(k_copy_interior, dim3(interior_blocks(G)), dim3(KB), 0, ctx->stream, G, x, d_ps);
     ASD2_LAUNCH_CHECK();
     const int st = status_of(fs);
@@ -721,33 +768,34 @@ int solve_mixed(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2_solv
     ASD2_TRY(driver_init(ctx, &D));
     ASD2_TRY(set_state(&D, o));
     hipLaunchKernelGGL(k_cg_init<float>, dim3(nbf), dim3(KB), 0, ctx->stream, n, S.rhs, x, r, r64, x64, S.dinv32, z,
-                       p0, D.partials);
+                       p0, D.pu);
     ASD2_LAUNCH_CHECK();
-    ASD2_TRY(tail<T_CG_INIT>(&D, nbf));
+    ASD2_TRY(tail<T_CG_INIT>(&D, nbf, D.pu));
+    int it = 0;
     auto batch = [&](int count) -> int {
-        for (int k = 0; k < count; k++) {
+        for (int k = 0; k < count; k++, it++) {
             prof_begin(ctx, ASD2_KID_PCG32_SPMV);
-            hipLaunchKernelGGL(k_cg_spmv<float>, dim3(nbs), dim3(KB), 0, ctx->stream, G, S.ax32, S.ay32, S.diag32, z,
-                               p0, p1, q, D.st, D.partials);
+            hipLaunchKernelGGL((k_cg_spmv<float, true>), dim3(nbs), dim3(KB), 0, ctx->stream, G, S.ax32, S.ay32,
+                               S.diag32, z, p0, p1, q, D.st, D.pu, nbf, D.ps, it);
             prof_end(ctx, ASD2_KID_PCG32_SPMV);
-            ASD2_TRY(tail<T_CG_ALPHA>(&D, nbs));
             prof_begin(ctx, ASD2_KID_PCG32_UPDATE);
             hipLaunchKernelGGL(k_cg_update<float>, dim3(nbf), dim3(KB), 0, ctx->stream, n, x, p0, p1, r, q,
-                               S.dinv32, z, D.st, D.partials);
+                               S.dinv32, z, D.st, D.ps, nbs, D.pu, it);
             prof_end(ctx, ASD2_KID_PCG32_UPDATE);
-            ASD2_TRY(tail<T_CG_BETA_MIXED>(&D, nbf));
         }
         ASD2_LAUNCH_CHECK();
         return ASD2_OK;
     };
-    auto pause = [&]() -> int {
+    // Reliable update at the paused iteration k: fold x32 into x64, r64 = b - A x64, replace
+    // the FP32 residual, and re-run iteration k (its SpMV derives beta from the new r.z).
+    auto pause = [&](const KrylovState& s) -> int {
+        it = s.iter;
         hipLaunchKernelGGL(k_mx_accum, dim3(nbf), dim3(KB), 0, ctx->stream, n, x64, x, D.st, 0);
         hipLaunchKernelGGL(k_true_residual, dim3(nbi), dim3(KB), 0, ctx->stream, G, S.ax, S.ay, S.diag, S.rhs, x64,
-                           r64, D.st, D.partials, 0);
-        ASD2_TRY(tail<T_TRUE_RES>(&D, nbi));
-        hipLaunchKernelGGL(k_mx_resume, dim3(nbf), dim3(KB), 0, ctx->stream, n, r64, r, S.dinv32, z, D.st,
-                           D.partials);
-        ASD2_TRY(tail<T_MX_RESUME>(&D, nbf));
+                           r64, D.st, D.ps, 0);
+        ASD2_TRY(tail<T_TRUE_RES>(&D, nbi, D.ps));
+        hipLaunchKernelGGL(k_mx_resume, dim3(nbf), dim3(KB), 0, ctx->stream, n, r64, r, S.dinv32, z, D.st, D.pu);
+        ASD2_TRY(tail<T_MX_RESUME>(&D, 0, D.pu));
         ASD2_LAUNCH_CHECK();
         return ASD2_OK;
     };
@@ -817,30 +865,31 @@ int asd2_poisson_bicgstab(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps,
     Driver D;
     ASD2_TRY(driver_init(ctx, &D));
     ASD2_TRY(set_state(&D, o));
-    hipLaunchKernelGGL(k_bicg_init, dim3(nbf), dim3(KB), 0, ctx->stream, n, S.rhs, x, r, rt, D.partials);
+    hipLaunchKernelGGL(k_bicg_init, dim3(nbf), dim3(KB), 0, ctx->stream, n, S.rhs, x, r, rt, D.pu);
     ASD2_LAUNCH_CHECK();
-    ASD2_TRY(tail<T_BICG_INIT>(&D, nbf));
+    ASD2_TRY(tail<T_BICG_INIT>(&D, nbf, D.pu));
     auto batch = [&](int count) -> int {
         for (int k = 0; k < count; k++) {
             hipLaunchKernelGGL(k_bicg_dir, dim3(nbf), dim3(KB), 0, ctx->stream, n, r, p, v, S.dinv, ph, D.st);
             hipLaunchKernelGGL(k_stencil_dot, dim3(nbi), dim3(KB), 0, ctx->stream, G, S.ax, S.ay, S.diag, ph, v, rt,
-                               D.st, D.partials, 1);
-            ASD2_TRY(tail<T_BICG_ALPHA>(&D, nbi));
+                               D.st, D.pu, 1);
+            ASD2_TRY(tail<T_BICG_ALPHA>(&D, nbi, D.pu));
             hipLaunchKernelGGL(k_bicg_s, dim3(nbf), dim3(KB), 0, ctx->stream, n, r, v, S.dinv, s, sh, D.st,
-                               D.partials);
-            ASD2_TRY(tail<T_BICG_S>(&D, nbf));
+                               D.pu);
+            ASD2_TRY(tail<T_BICG_S>(&D, nbf, D.pu));
             hipLaunchKernelGGL(k_stencil_dot, dim3(nbi), dim3(KB), 0, ctx->stream, G, S.ax, S.ay, S.diag, sh, t, s,
-                               D.st, D.partials, 2);
-            ASD2_TRY(tail<T_BICG_OMEGA>(&D, nbi));
+                               D.st, D.pu, 2);
+            ASD2_TRY(tail<T_BICG_OMEGA>(&D, nbi, D.pu));
             hipLaunchKernelGGL(k_bicg_x, dim3(nbf), dim3(KB), 0, ctx->stream, n, x, ph, sh, r, s, t, rt, D.st,
-                               D.partials);
-            ASD2_TRY(tail<T_BICG_X>(&D, nbf));
+                               D.pu);
+            ASD2_TRY(tail<T_BICG_X>(&D, nbf, D.pu));
         }
         ASD2_LAUNCH_CHECK();
         return ASD2_OK;
     };
     KrylovState fs;
-    ASD2_TRY(drive(&D, o->check_every > 0 ? o->check_every : 16, batch, []() { return ASD2_OK; }, false, &fs));
+    ASD2_TRY(drive(&D, o->check_every > 0 ? o->check_every : 16, batch,
+                   [](const KrylovState&) { return ASD2_OK; }, false, &fs));
     hipLaunchKernelGGL(k_copy_interior, dim3(nbi), dim3(KB), 0, ctx->stream, G, x, d_ps);
     ASD2_LAUNCH_CHECK();
     const int st = status_of(fs);
@@ -874,24 +923,24 @@ int asd2_csr_pcg(asd2_ctx* ctx, int n, const int* d_rowptr, const int* d_cols, c
     ASD2_TRY(set_state(&D, o));
     hipLaunchKernelGGL(k_recip, dim3(nb), dim3(KB), 0, ctx->stream, n64, d_mdiag, dinv);
     hipLaunchKernelGGL(k_cg_init<double>, dim3(nb), dim3(KB), 0, ctx->stream, n64, d_b, d_x, r,
-                       static_cast<double*>(nullptr), static_cast<double*>(nullptr), dinv, z, p, D.partials);
+                       static_cast<double*>(nullptr), static_cast<double*>(nullptr), dinv, z, p, D.pu);
     ASD2_LAUNCH_CHECK();
-    ASD2_TRY(tail<T_CG_INIT>(&D, nb));
+    ASD2_TRY(tail<T_CG_INIT>(&D, nb, D.pu));
+    int it = 0;
     auto batch = [&](int count) -> int {
-        for (int k = 0; k < count; k++) {
-            hipLaunchKernelGGL(k_csr_dir, dim3(nb), dim3(KB), 0, ctx->stream, n64, z, p, D.st);
+        for (int k = 0; k < count; k++, it++) {
+            hipLaunchKernelGGL(k_csr_dir, dim3(nb), dim3(KB), 0, ctx->stream, n64, z, p, D.st, D.pu, nb, it);
             hipLaunchKernelGGL(k_csr_spmv, dim3(nb), dim3(KB), 0, ctx->stream, n, d_rowptr, d_cols, d_vals, p, q,
-                               D.st, D.partials);
-            ASD2_TRY(tail<T_CG_ALPHA>(&D, nb));
+                               D.st, D.ps);
             hipLaunchKernelGGL(k_cg_update<double>, dim3(nb), dim3(KB), 0, ctx->stream, n64, d_x, p, p, r, q, dinv,
-                               z, D.st, D.partials);
-            ASD2_TRY(tail<T_CG_BETA>(&D, nb));
+                               z, D.st, D.ps, nb, D.pu, it);
         }
         ASD2_LAUNCH_CHECK();
         return ASD2_OK;
     };
     KrylovState fs;
-    ASD2_TRY(drive(&D, o->check_every > 0 ? o->check_every : 32, batch, []() { return ASD2_OK; }, false, &fs));
+    ASD2_TRY(drive(&D, o->check_every > 0 ? o->check_every : 32, batch,
+                   [](const KrylovState&) { return ASD2_OK; }, false, &fs));
     fs.aux2 = -1.0;
     fill_stats(fs, stats);
     return status_of(fs);
@@ -921,30 +970,31 @@ int asd2_csr_bicgstab(asd2_ctx* ctx, int n, const int* d_rowptr, const int* d_co
     ASD2_TRY(driver_init(ctx, &D));
     ASD2_TRY(set_state(&D, o));
     hipLaunchKernelGGL(k_recip, dim3(nb), dim3(KB), 0, ctx->stream, n64, d_mdiag, dinv);
-    hipLaunchKernelGGL(k_bicg_init, dim3(nb), dim3(KB), 0, ctx->stream, n64, d_b, d_x, r, rt, D.partials);
+    hipLaunchKernelGGL(k_bicg_init, dim3(nb), dim3(KB), 0, ctx->stream, n64, d_b, d_x, r, rt, D.pu);
     ASD2_LAUNCH_CHECK();
-    ASD2_TRY(tail<T_BICG_INIT>(&D, nb));
+    ASD2_TRY(tail<T_BICG_INIT>(&D, nb, D.pu));
     auto batch = [&](int count) -> int {
         for (int k = 0; k < count; k++) {
             hipLaunchKernelGGL(k_bicg_dir, dim3(nb), dim3(KB), 0, ctx->stream, n64, r, p, v, dinv, ph, D.st);
             hipLaunchKernelGGL(k_csr_dot, dim3(nb), dim3(KB), 0, ctx->stream, n, d_rowptr, d_cols, d_vals, ph, v, rt,
-                               D.st, D.partials, 1);
-            ASD2_TRY(tail<T_BICG_ALPHA>(&D, nb));
+                               D.st, D.pu, 1);
+            ASD2_TRY(tail<T_BICG_ALPHA>(&D, nb, D.pu));
             hipLaunchKernelGGL(k_bicg_s, dim3(nb), dim3(KB), 0, ctx->stream, n64, r, v, dinv, s, sh, D.st,
-                               D.partials);
-            ASD2_TRY(tail<T_BICG_S>(&D, nb));
+                               D.pu);
+            ASD2_TRY(tail<T_BICG_S>(&D, nb, D.pu));
             hipLaunchKernelGGL(k_csr_dot, dim3(nb), dim3(KB), 0, ctx->stream, n, d_rowptr, d_cols, d_vals, sh, t, s,
-                               D.st, D.partials, 2);
-            ASD2_TRY(tail<T_BICG_OMEGA>(&D, nb));
+                               D.st, D.pu, 2);
+            ASD2_TRY(tail<T_BICG_OMEGA>(&D, nb, D.pu));
             hipLaunchKernelGGL(k_bicg_x, dim3(nb), dim3(KB), 0, ctx->stream, n64, d_x, ph, sh, r, s, t, rt, D.st,
-                               D.partials);
-            ASD2_TRY(tail<T_BICG_X>(&D, nb));
+                               D.pu);
+            ASD2_TRY(tail<T_BICG_X>(&D, nb, D.pu));
         }
         ASD2_LAUNCH_CHECK();
         return ASD2_OK;
     };
     KrylovState fs;
-    ASD2_TRY(drive(&D, o->check_every > 0 ? o->check_every : 16, batch, []() { return ASD2_OK; }, false, &fs));
+    ASD2_TRY(drive(&D, o->check_every > 0 ? o->check_every : 16, batch,
+                   [](const KrylovState&) { return ASD2_OK; }, false, &fs));
     fs.aux2 = -1.0;
     fill_stats(fs, stats);
     return status_of(fs);

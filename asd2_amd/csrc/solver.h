@@ -17,8 +17,9 @@ int system_arrays(asd2_ctx* ctx, System* S, bool want32);
 int ensure_system32(asd2_ctx* ctx, System* S);
 bool all_neumann_family(const asd2_ctx* ctx);
 
-// Device-resident Krylov scalars; advanced by the one-block tail kernel that follows each
-// reducing kernel.
+// Device-resident Krylov scalars.  CG kernels advance them themselves (every block
+// re-sums the producer's partials; block 0 records the result); BiCGStab and the rare
+// bookkeeping steps use a one-block tail kernel.
 struct KrylovState {
     double rho, rho_old, alpha, beta, omega;  // recurrence scalars
     double rr;                                // ||r||^2 of the recurrence
@@ -27,6 +28,7 @@ struct KrylovState {
     double rmax;                              // mixed: ||r|| at the last reliable update
     double delta;                             // mixed: replacement factor
     double pq, aux0, aux1, aux2;              // scratch reductions
+    double rz[2];                             // CG: r.z of iteration k in rz[k & 1]
     int iter, maxit, mode, done;              // done: 0 running, 1 converged, 2 maxit, 3 breakdown, 4 paused
     int updates, fallbacks, flags, pad;
 };
