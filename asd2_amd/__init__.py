@@ -23,7 +23,7 @@ BC = {"periodic": 0, "neumann": 1, "dirichlet": 2, "axisym": 3, "wall": 4,
 GRID_P, GRID_U, GRID_V = 0, 1, 2
 CG_REFERENCE, CG_STANDARD = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
-PRECOND_JACOBI, PRECOND_LEARNED = 0, 1
+PRECOND_JACOBI, PRECOND_LEARNED, PRECOND_MULTILEVEL = 0, 1, 2
 MOM_RHS, MOM_CONVECTION, MOM_VISCOSITY = 0, 1, 2
 KID = {"weno_stage": 1, "reinit": 2, "pcg64_spmv": 3, "pcg64_update": 4, "pcg32_spmv": 5,
        "pcg32_update": 6, "gfm": 7, "unet": 8}
@@ -134,6 +134,19 @@ def load() -> C.CDLL:
         "asd2_ls_sussman_constraint": ([_VP, _VP, _VP, _VP, _VP], C.c_int),
         "asd2_ls_unit_weno5": ([_VP, _VP, _VP, _VP, C.c_double, C.c_int, C.c_int], C.c_int),
         "asd2_mac_momentum_term": ([_VP, P(Fluid), C.c_int, C.c_int, _VP, _VP, _VP, _VP, _VP], C.c_int),
+        "asd2_unet_create": ([C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, P(_VP)], C.c_int),
+        "asd2_unet_destroy": ([_VP], C.c_int),
+        "asd2_unet_param_count": ([_VP, P(C.c_longlong)], C.c_int),
+        "asd2_unet_set_params": ([_VP, _VP, C.c_longlong], C.c_int),
+        "asd2_unet_get_params": ([_VP, _VP, C.c_longlong], C.c_int),
+        "asd2_unet_get_grads": ([_VP, _VP, C.c_longlong], C.c_int),
+        "asd2_unet_forward": ([_VP, C.c_int, _VP, _VP], C.c_int),
+        "asd2_unet_backward": ([_VP, C.c_int, _VP], C.c_int),
+        "asd2_unet_forward_linear": ([_VP, C.c_int, _VP, _VP], C.c_int),
+        "asd2_unet_backward_linear": ([_VP, C.c_int, _VP], C.c_int),
+        "asd2_unet_train_step": ([_VP, C.c_int, _VP, _VP, _VP, _VP, _VP, C.c_double, C.c_double, C.c_float,
+                                  P(C.c_float)], C.c_int),
+        "asd2_set_preconditioner": ([_VP, _VP], C.c_int),
         "asd2_timer_start": ([_VP], C.c_int),
         "asd2_timer_stop": ([_VP, P(C.c_double)], C.c_int),
         "asd2_profile_kernel": ([_VP, C.c_int], C.c_int),
@@ -217,6 +230,83 @@ class Stats:
     converged: int
     rel_residual: float
     true_rel_residual: float
+
+
+class UNet:
+    """The learned preconditioner's network (asd2_unet) for nx x ny grids coarsened by c x c
+    blocks: its images are the (nx / c) x (ny / c) coarse grids."""
+
+    def __init__(self, nx, ny, max_batch=1, seed=0, coarsen=1):
+        self.lib = load()
+        h = _VP()
+        check(self.lib.asd2_unet_create(nx, ny, coarsen, max_batch, seed, C.byref(h)))
+        self.h = h.value
+        self.nx, self.ny, self.max_batch, self.coarsen = nx, ny, max_batch, coarsen
+        self.H, self.W = nx // coarsen, ny // coarsen
+        cnt = C.c_longlong(0)
+        check(self.lib.asd2_unet_param_count(self.h, C.byref(cnt)))
+        self.nparams = cnt.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.asd2_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, v):
+        v = np.ascontiguousarray(v, np.float32)
+        check(self.lib.asd2_unet_set_params(self.h, v.ctypes.data, v.size))
+
+    def get_params(self):
+        v = np.empty(self.nparams, np.float32)
+        check(self.lib.asd2_unet_get_params(self.h, v.ctypes.data, v.size))
+        return v
+
+    def get_grads(self):
+        v = np.empty(self.nparams, np.float32)
+        check(self.lib.asd2_unet_get_grads(self.h, v.ctypes.data, v.size))
+        return v
+
+    def forward(self, feat):
+        """feat (B, H, W, 4) -> y (B, H, W) on the coarse grid"""
+        feat = np.ascontiguousarray(feat, np.float32)
+        b = feat.shape[0]
+        d_f = DeviceArray.from_host(feat)
+        d_y = DeviceArray((b, self.H, self.W), np.float32)
+        check(self.lib.asd2_unet_forward(self.h, b, d_f.ptr, d_y.ptr))
+        return d_y.download()
+
+    def backward(self, dy):
+        dy = np.ascontiguousarray(dy, np.float32)
+        d_dy = DeviceArray.from_host(dy)
+        check(self.lib.asd2_unet_backward(self.h, dy.shape[0], d_dy.ptr))
+
+    def forward_linear(self, feat):
+        """the preconditioner's linear form: feat (B, H, W, 4) -> y (B, H, W), linear in
+        channel 0 with the ReLU pattern of channels 1..3"""
+        feat = np.ascontiguousarray(feat, np.float32)
+        b = feat.shape[0]
+        d_f = DeviceArray.from_host(feat)
+        d_y = DeviceArray((b, self.H, self.W), np.float32)
+        check(self.lib.asd2_unet_forward_linear(self.h, b, d_f.ptr, d_y.ptr))
+        return d_y.download()
+
+    def backward_linear(self, dy):
+        dy = np.ascontiguousarray(dy, np.float32)
+        d_dy = DeviceArray.from_host(dy)
+        check(self.lib.asd2_unet_backward_linear(self.h, dy.shape[0], d_dy.ptr))
+
+    def train_step(self, batch, d_r, d_dinv, d_ax, d_ay, d_dg, dx, dy, lr):
+        """one Adam step; d_* are device arrays of `batch` FP32 padded systems; returns the loss"""
+        loss = C.c_float(0.0)
+        check(self.lib.asd2_unet_train_step(self.h, batch, d_r.ptr, d_dinv.ptr, d_ax.ptr, d_ay.ptr, d_dg.ptr,
+                                            dx, dy, lr, C.byref(loss)))
+        return loss.value
 
 
 class Context:
@@ -347,6 +437,9 @@ class Context:
         ds = [self.array(a) for a in (ax, ay, diag, rhs)]
         check(self.lib.asd2_gfm_import(self.h, *[d.ptr for d in ds]))
         self.sync()
+
+    def set_preconditioner(self, net):
+        check(self.lib.asd2_set_preconditioner(self.h, net.h if net is not None else None))
 
     def gfm_apply(self, x, y):
         check(self.lib.asd2_gfm_apply(self.h, x.ptr, y.ptr))

@@ -73,6 +73,8 @@ struct asd2_ctx {
 
     // assembled Poisson system state
     bool system_ready = false;
+    // network of the learned preconditioner (not owned)
+    struct asd2_unet* net = nullptr;
 
     // Krylov polling: pinned host copies of the device scalars and their events
     void* kr_host = nullptr;

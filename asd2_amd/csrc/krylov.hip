@@ -20,6 +20,7 @@
 #include <algorithm>
 
 #include "solver.h"
+#include "unet.h"
 
 namespace asd2 {
 
@@ -132,6 +133,7 @@ __device__ __forceinline__ void for_interior(const Geom& G, F f) {
 // ------------------------------------------------------------------ tail kernels
 enum TailOp {
     T_CG_INIT,          // r.z, r.r, b.b  -> bnorm, initial convergence
+    T_FCG_INIT,         // r.r, (D^-1 r)^2, b.b -> bnorm, initial convergence
     T_TRUE_RES,         // ||b - A x||^2  -> reliable update bookkeeping
     T_TRUE_RES_REPORT,  //                -> reported true residual
     T_MX_RESUME,        // (no sum)       -> back to RUNNING after a reliable update
@@ -144,13 +146,14 @@ enum TailOp {
 
 template <int OP> struct TailNV { static constexpr int value = 1; };
 template <> struct TailNV<T_CG_INIT> { static constexpr int value = 3; };
+template <> struct TailNV<T_FCG_INIT> { static constexpr int value = 3; };
 template <> struct TailNV<T_BICG_OMEGA> { static constexpr int value = 2; };
 template <> struct TailNV<T_BICG_X> { static constexpr int value = 2; };
 
 // whether the producer of OP's partials ran (the tail mirrors its guard)
 template <int OP>
 __device__ __forceinline__ bool tail_active(const KrylovState* st) {
-    if (OP == T_CG_INIT || OP == T_BICG_INIT || OP == T_TRUE_RES_REPORT) return true;
+    if (OP == T_CG_INIT || OP == T_FCG_INIT || OP == T_BICG_INIT || OP == T_TRUE_RES_REPORT) return true;
     if (OP == T_TRUE_RES || OP == T_MX_RESUME) return st->done == DONE_PAUSED;
     if (OP == T_BICG_OMEGA) return st->done == DONE_RUNNING && !(st->flags & 1);
     return st->done == DONE_RUNNING;
@@ -167,7 +170,9 @@ __global__ __launch_bounds__(KB) void k_tail(int nb, const double* __restrict__ 
     double v[NV];
     sum_partials(partials, nb, v);
     if (threadIdx.x != 0) return;
-    if (OP == T_CG_INIT) {
+    if (OP == T_CG_INIT || OP == T_FCG_INIT) {
+        // CG partials are (r.z, r.r, b.b), FCG partials (r.r, (D^-1 r)^2, b.b)
+        if (OP == T_FCG_INIT) v[1] = v[0];
         double bn = sqrt(v[2]);
         if (bn == 0.0) bn = 1.0;
         st->bnorm = bn;
@@ -590,6 +595,7 @@ struct Driver {
     hipEvent_t ev[2] = {nullptr, nullptr};
     double* pu = nullptr;            // partials of the residual producers (r.z, r.r[, b.b]); tail kernels
     double* ps = nullptr;            // partials of the SpMV producers (p.q [, ...])
+    double* pz = nullptr;            // partials of the preconditioner output (z.r, z.q)
 };
 
 int flat_blocks(int64_t n) { return std::max(1, std::min<int>(MAX_BLOCKS, ceil_div(n, KB))); }
@@ -612,9 +618,10 @@ int driver_init(asd2_ctx* ctx, Driver* D) {
     static_assert(sizeof(KrylovState) % 8 == 0, "state size");
     ASD2_TRY(ws(ctx, "kr_state", sizeof(KrylovState) / 8, reinterpret_cast<double**>(&D->st)));
     double* part = nullptr;
-    ASD2_TRY(ws(ctx, "kr_partials", 5 * MAX_BLOCKS, &part));
+    ASD2_TRY(ws(ctx, "kr_partials", 7 * MAX_BLOCKS, &part));
     D->pu = part;
     D->ps = part + 3 * MAX_BLOCKS;
+    D->pz = part + 5 * MAX_BLOCKS;
     if (!ctx->kr_host) ASD2_HIP(hipHostMalloc(&ctx->kr_host, 2 * sizeof(KrylovState), hipHostMallocDefault));
     for (int k = 0; k < 2; k++)
         if (!ctx->kr_ev[k]) ASD2_HIP(hipEventCreateWithFlags(&ctx->kr_ev[k], hipEventDisableTiming));
@@ -644,11 +651,14 @@ int read_state(Driver* D, KrylovState* out) {
 }
 
 // Runs `batch(count)` repeatedly, polling one batch behind, until done is neither
-// RUNNING nor (when has_pause) PAUSED after on_pause(state) ran.
+// RUNNING nor (when has_pause) PAUSED after on_pause(state) ran.  The batch in the other slot
+// was enqueued before on_pause's kernels: it ran as no-ops on the paused state and its
+// snapshot is stale, so that snapshot is skipped once.
 template <class Batch, class Pause>
 int drive(Driver* D, int check_every, Batch batch, Pause on_pause, bool has_pause, KrylovState* final_state) {
     asd2_ctx* ctx = D->ctx;
     int slot = 0;
+    bool stale[2] = {false, false};
     auto enqueue = [&](int s) -> int {
         ASD2_TRY(batch(check_every));
         ASD2_HIP(hipMemcpyAsync(&D->host[s], D->st, sizeof(KrylovState), hipMemcpyDeviceToHost, ctx->stream));
@@ -660,8 +670,11 @@ int drive(Driver* D, int check_every, Batch batch, Pause on_pause, bool has_paus
     for (int guard = 0; guard < 100000000; guard++) {
         ASD2_HIP(hipEventSynchronize(D->ev[slot]));
         const KrylovState s = D->host[slot];
-        if (s.done == DONE_PAUSED && has_pause) {
+        if (stale[slot]) {
+            stale[slot] = false;
+        } else if (s.done == DONE_PAUSED && has_pause) {
             ASD2_TRY(on_pause(s));
+            stale[slot ^ 1] = true;
         } else if (s.done != DONE_RUNNING) {
             break;
         }
@@ -811,6 +824,834 @@ int solve_mixed(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2_solv
     return st;
 }
 
+
+// ------------------------------------------------------------------ flexible PCG, learned preconditioner
+// FP32 flexible CG (Notay 2000): beta_k = (z_k, r_k - r_{k-1}) / (z_{k-1}, r_{k-1})
+// = -alpha_{k-1} (z_k, q_{k-1}) / rho_{k-1}, with the preconditioner
+//     z = D^-1 r + gamma P y,  y = net(g),  g = P^T (r - A D^-1 r),  gamma = (y.g) / (y.A_c y)
+// on c x c blocks (P piecewise constant, A_c = P^T A P), or z = D^-1 r (Jacobi fallback).  The
+// network runs in its linear form: the ReLU pattern of a pass on the coarse coefficients, taken
+// once per solve, then per iteration a bias-free masked pass on g, so g -> y is one linear map
+// for the whole solve and only the scalar gamma varies (a network that is nonlinear in r makes
+// every iteration's preconditioner different, and the truncated flexible recurrence then loses
+// the Krylov acceleration for every component the network does not capture).  FP64
+// reliable updates and the FP64 true residual as in the mixed-precision PCG.  Every scalar of z
+// splits into a fine Jacobi part and a coarse part (z.r = r.D^-1 r + gamma y.P^T r, likewise
+// z.q_prev), so z is never stored: the SpMV forms it on the fly.  Each iteration: begin
+// (decision; one fine pass for g, P^T r, P^T q_prev and the Jacobi dots), pack (network input),
+// network, coarse dots (y.g, y.A_c y, y.P^T r, y.P^T q_prev), direction/SpMV, x/r update.
+
+// coarse data in FP64: a coarse diagonal is the sum of c^2 fine diagonals minus twice the
+// faces inside the block, a cancellation that loses about log2(c) bits per level, and the
+// smooth modes the coarse correction is for sit at the bottom of the spectrum
+struct Coarse {
+    int c, Hc, Wc;
+    double *acx, *acy, *dc;  // coarse operator, (Hc + 1) x (Wc + 1)
+    double *g, *rc, *qc;     // P^T (r - A D^-1 r), P^T r, P^T q_prev, Hc x Wc
+};
+
+// fine region of the begin kernel: whole c x c blocks, at most 32 x 64 cells
+__host__ __device__ inline int lp_ri(int c) { return c > 16 ? c : 16; }
+constexpr int LP_RJ = 64;
+
+__global__ __launch_bounds__(KB) void k_fcg_init(int64_t n, const double* __restrict__ b, float* __restrict__ x,
+                                                 float* __restrict__ r, double* __restrict__ r64,
+                                                 double* __restrict__ x64, const float* __restrict__ dinv,
+                                                 float* __restrict__ p0, float* __restrict__ p1,
+                                                 float* __restrict__ q, double* partials) {
+    double rr = 0.0, zz = 0.0, bb = 0.0;
+    FLAT_LOOP(k, n) {
+        const double bv = b[k];
+        const float rv = static_cast<float>(bv);
+        const float zv = dinv[k] * rv;
+        x[k] = 0.0f;
+        r[k] = rv;
+        r64[k] = bv;
+        x64[k] = 0.0;
+        p0[k] = 0.0f;
+        p1[k] = 0.0f;
+        q[k] = 0.0f;
+        rr += static_cast<double>(rv) * rv;
+        zz += static_cast<double>(zv) * zv;
+        bb += bv * bv;
+    }
+    double v[3] = {rr, zz, bb};
+    put_partials(v, partials);
+}
+
+__global__ __launch_bounds__(KB) void k_lp_coarse_op(Geom G, const double* __restrict__ ax,
+                                                     const double* __restrict__ ay, const double* __restrict__ dg,
+                                                     Coarse C) {
+    const int64_t cs = static_cast<int64_t>(C.Hc + 1) * (C.Wc + 1);
+    FLAT_LOOP(e, cs) {
+        const int I = static_cast<int>(e / (C.Wc + 1)), J = static_cast<int>(e - static_cast<int64_t>(I) * (C.Wc + 1));
+        coarse_cell(ax, ay, dg, G.sy, G.g, C.c, C.Hc, C.Wc, I, J, C.acx, C.acy, C.dc);
+    }
+}
+
+// ReLU-pass input (0, log coarse W-face, S-face coefficients, log coarse diagonal) and the
+// linear-pass input cleared (its channel 0 is written per iteration)
+__global__ __launch_bounds__(KB) void k_lp_static_features(Coarse C, uint16_t* __restrict__ feat,
+                                                           uint16_t* __restrict__ flin, float h2x, float h2y) {
+    const int64_t Pc = static_cast<int64_t>(C.Hc) * C.Wc;
+    const float h2 = fminf(h2x, h2y);
+    FLAT_LOOP(p, Pc) {
+        const int I = static_cast<int>(p / C.Wc), J = static_cast<int>(p - static_cast<int64_t>(I) * C.Wc);
+        const int k = I * (C.Wc + 1) + J;
+        const uint32_t lo = static_cast<uint32_t>(f2bf(log_coeff(static_cast<float>(C.acx[k]), h2x))) << 16;
+        const uint32_t hi = static_cast<uint32_t>(f2bf(log_coeff(static_cast<float>(C.acy[k]), h2y))) |
+                            (static_cast<uint32_t>(f2bf(log_diag(static_cast<float>(C.dc[k]), h2))) << 16);
+        *reinterpret_cast<uint2*>(feat + p * 4) = make_uint2(lo, hi);
+        *reinterpret_cast<uint2*>(flin + p * 4) = make_uint2(0u, 0u);
+    }
+}
+
+// sum of v over the c consecutive lanes of a block row (c a power of two)
+__device__ __forceinline__ double seg_sum(double v, int c) {
+    for (int m = 1; m < c && m < 64; m <<= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// Start of iteration `it`: the decision (converged / pause for a reliable update / limit) from
+// the partials (r.r, .) of the kernel that produced r; then one pass over regions of whole
+// blocks: g, P^T r and P^T q_prev into the coarse arrays, partials (r.D^-1 r,
+// (D^-1 r).q_prev, sum (g / dc)^2).
+__global__ __launch_bounds__(KB) void k_lp_begin(Geom G, Coarse C, const float* __restrict__ r,
+                                                 const float* __restrict__ dinv, const float* __restrict__ q,
+                                                 const float* __restrict__ ax, const float* __restrict__ ay,
+                                                 const float* __restrict__ dg, KrylovState* st, const double* pu,
+                                                 int nbu, int it, double* pb) {
+    if (st->done != DONE_RUNNING) return;
+    {
+        double v[2];
+        sum_partials_all(pu, nbu, v);
+        const double rn = sqrt(v[0]), resid = rn / st->bnorm;
+        int done = DONE_RUNNING;
+        if (!isfinite(resid)) done = DONE_BREAKDOWN;
+        else if (resid <= st->tol || rn < st->delta * st->rmax || it >= st->maxit) done = DONE_PAUSED;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->rr = v[0];
+            st->iter = it;
+            if (done != DONE_RUNNING) st->done = done;
+        }
+        if (done != DONE_RUNNING) return;
+    }
+    __shared__ float sZ[32 + 2][LP_RJ + 2];
+    __shared__ double sB[3][1024];
+    const int c = C.c, RI = lp_ri(c), nbi = RI / c, nbj = LP_RJ / c, nblk = nbi * nbj;
+    const int reg_i = ceil_div(G.nx, RI), nreg = reg_i * ceil_div(G.ny, LP_RJ);
+    const int ilo = G.g, ihi = G.g + G.nx, jlo = G.g, jhi = G.g + G.ny;
+    const int lane = threadIdx.x & 63;
+    int t0, t1;
+    tile_range(nreg, t0, t1);
+    double a_rz = 0.0, a_zq = 0.0, a_uu = 0.0;
+    for (int t = t0; t < t1; t++) {
+        const int rj = t / reg_i, ri = t - rj * reg_i;
+        const int i0 = ilo + ri * RI, j0 = jlo + rj * LP_RJ;
+        for (int e = threadIdx.x; e < (RI + 2) * (LP_RJ + 2); e += KB) {
+            const int a = e / (LP_RJ + 2), b = e - a * (LP_RJ + 2);
+            const int i = i0 - 1 + a, j = j0 - 1 + b;
+            float v = 0.0f;
+            if (i >= ilo && i < ihi && j >= jlo && j < jhi) {
+                const int64_t k = G.idx(i, j);
+                v = dinv[k] * r[k];
+            }
+            sZ[a][b] = v;
+        }
+        for (int e = threadIdx.x; e < nblk; e += KB) sB[0][e] = sB[1][e] = sB[2][e] = 0.0;
+        __syncthreads();
+        for (int e = threadIdx.x; e < RI * LP_RJ; e += KB) {
+            const int a = e / LP_RJ, b = e - a * LP_RJ;  // a wave covers one region row
+            const int i = i0 + a, j = j0 + b;
+            const bool in = i < ihi && j < jhi;
+            double w = 0.0, rv = 0.0, qv = 0.0;  // FP64 sums of FP32 data (the inputs are exact)
+            if (in) {
+                const int64_t k = G.idx(i, j);
+                const double zc = sZ[a + 1][b + 1];
+                const double az = dg[k] * zc + ax[k] * static_cast<double>(sZ[a][b + 1]) +
+                                  ax[k + G.sy] * static_cast<double>(sZ[a + 2][b + 1]) +
+                                  ay[k] * static_cast<double>(sZ[a + 1][b]) + ay[k + 1] * static_cast<double>(sZ[a + 1][b + 2]);
+                rv = r[k];
+                qv = q[k];
+                w = rv - az;
+                a_rz += zc * rv;
+                a_zq += zc * qv;
+            }
+            w = seg_sum(w, c);
+            rv = seg_sum(rv, c);
+            qv = seg_sum(qv, c);
+            if ((lane & (c - 1)) == 0 || c >= 64) {
+                const int blk = (a / c) * nbj + b / c;
+                atomicAdd(&sB[0][blk], w);
+                atomicAdd(&sB[1][blk], rv);
+                atomicAdd(&sB[2][blk], qv);
+            }
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < nblk; e += KB) {
+            const int bi = e / nbj, bj = e - bi * nbj;
+            const int I = (i0 - ilo) / c + bi, J = (j0 - jlo) / c + bj;
+            if (I < C.Hc && J < C.Wc) {
+                const int64_t pc = static_cast<int64_t>(I) * C.Wc + J;
+                const double gv = sB[0][e];
+                C.g[pc] = gv;
+                C.rc[pc] = sB[1][e];
+                C.qc[pc] = sB[2][e];
+                const double u = gv / C.dc[I * (C.Wc + 1) + J];
+                a_uu += u * u;
+            }
+        }
+        __syncthreads();
+    }
+    double v[3] = {a_rz, a_zq, a_uu};
+    put_partials(v, pb);
+}
+
+// linear-pass input channel 0 = g / dc / s, s = rms over the coarse cells of g / dc
+__global__ __launch_bounds__(KB) void k_lp_pack(Coarse C, uint16_t* __restrict__ flin, const KrylovState* st,
+                                                const double* pb, int nbb) {
+    if (st->done != DONE_RUNNING) return;
+    double v[3];
+    sum_partials_all(pb, nbb, v);
+    const int64_t Pc = static_cast<int64_t>(C.Hc) * C.Wc;
+    double s = sqrt(v[2] / static_cast<double>(Pc));
+    if (!(s > 0.0) || !isfinite(s)) s = 1.0;
+    const float is = static_cast<float>(1.0 / s);
+    FLAT_LOOP(p, Pc) {
+        const int I = static_cast<int>(p / C.Wc), J = static_cast<int>(p - static_cast<int64_t>(I) * C.Wc);
+        flin[p * 4] = f2bf(static_cast<float>(C.g[p] / C.dc[I * (C.Wc + 1) + J]) * is);
+    }
+}
+
+// partials (y.g, y.A_c y, y.P^T r, y.P^T q_prev)
+__global__ __launch_bounds__(KB) void k_lp_coarse_dots(Coarse C, const float* __restrict__ y, const KrylovState* st,
+                                                       double* pc) {
+    if (st->done != DONE_RUNNING) return;
+    const int64_t Pc = static_cast<int64_t>(C.Hc) * C.Wc;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    FLAT_LOOP(p, Pc) {
+        const int I = static_cast<int>(p / C.Wc), J = static_cast<int>(p - static_cast<int64_t>(I) * C.Wc);
+        const double yv = y[p];
+        a0 += yv * C.g[p];
+        a1 += yv * coarse_apply(C.acx, C.acy, C.dc, y, C.Hc, C.Wc, I, J);
+        a2 += yv * C.rc[p];
+        a3 += yv * C.qc[p];
+    }
+    double v[4] = {a0, a1, a2, a3};
+    put_partials(v, pc);
+}
+
+// gamma, rho = z.r, beta, then p = z + beta p_old on a tile with halo (z = D^-1 r + gamma P y
+// formed here), q = A p, partial p.q.  gamma: the energy-optimal step along P y, or gamma_fixed
+// when positive (a y that already carries its scale: a scalar that varies with r makes the
+// preconditioner nonlinear, which costs a linear y about half its benefit).  beta: 0 at it == 0 and at restart_it (the preconditioner
+// changed); rho / rho_prev at fr_it (the residual was replaced by a reliable update, so
+// r_k - r_{k-1} = -alpha q no longer holds; the direction is kept); otherwise the flexible form
+// -alpha_prev (z.q_prev) / rho_prev.  rho <= 0 records BREAKDOWN; the host then falls back to
+// Jacobi.  y == nullptr: Jacobi (gamma = 0).
+__global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* __restrict__ ax,
+                                                const float* __restrict__ ay, const float* __restrict__ dg,
+                                                const float* __restrict__ r, const float* __restrict__ dinv,
+                                                const float* __restrict__ y, float* p0, float* p1,
+                                                float* __restrict__ q, KrylovState* st, const double* pb, int nbb,
+                                                const double* pc, int nbc, double* ps, int it, int restart_it,
+                                                int fr_it, double gamma_fixed) {
+    if (st->done != DONE_RUNNING) return;
+    double vb[3];
+    sum_partials_all(pb, nbb, vb);
+    double gm = 0.0, yr = 0.0, yq = 0.0;
+    if (y) {
+        double vc[4];
+        sum_partials_all(pc, nbc, vc);
+        const double t = vc[0] / vc[1];
+        if (gamma_fixed > 0.0) gm = gamma_fixed;
+        else if (vc[1] > 0.0 && isfinite(t)) gm = t;
+        yr = vc[2];
+        yq = vc[3];
+    }
+    const double rho = vb[0] + gm * yr;
+    double beta64 = 0.0;
+    if (it > 0 && it != restart_it) {
+        const double rho_prev = st->rz[(it - 1) & 1];
+        beta64 = (it == fr_it) ? rho / rho_prev : -st->alpha * (vb[1] + gm * yq) / rho_prev;
+    }
+    const bool ok = isfinite(beta64) && isfinite(rho) && rho > 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (ok) {
+            st->rz[it & 1] = rho;
+            st->beta = beta64;
+            st->omega = gm;
+        } else {
+            st->done = DONE_BREAKDOWN;
+            st->iter = it;
+        }
+    }
+    if (!ok) return;
+    const float beta = static_cast<float>(beta64), gmf = static_cast<float>(gm);
+    const bool odd = (it & 1) != 0;
+    const float* __restrict__ pold = odd ? p1 : p0;
+    float* __restrict__ pnew = odd ? p0 : p1;
+    __shared__ float sP[TI + 2][TJ + 2];
+    const int ipad = G.nx + 2 * G.g, jpad = G.ny + 2 * G.g;
+    const int ilo = G.g, ihi = G.g + G.nx, jlo = G.g, jhi = G.g + G.ny;
+    const int tiles_i = ceil_div(G.nx, TI), ntiles = tiles_i * ceil_div(G.ny, TJ);
+    int t0, t1;
+    tile_range(ntiles, t0, t1);
+    const int tj = threadIdx.x & (TJ - 1), ti = threadIdx.x / TJ;
+    double acc = 0.0;
+    for (int t = t0; t < t1; t++) {
+        const int bj = t / tiles_i, bi = t - bj * tiles_i;
+        const int j0 = G.g + bj * TJ, i0 = G.g + bi * TI;
+        for (int e = threadIdx.x; e < (TI + 2) * (TJ + 2); e += KB) {
+            const int rr_ = e / (TJ + 2), cc = e - rr_ * (TJ + 2);
+            const int i = i0 - 1 + rr_, j = j0 - 1 + cc;
+            float val = 0.0f;
+            if (i < ipad && j < jpad) {
+                const int64_t k = G.idx(i, j);
+                float zv = 0.0f;
+                if (i >= ilo && i < ihi && j >= jlo && j < jhi) {
+                    zv = dinv[k] * r[k];
+                    if (y) zv += gmf * y[static_cast<int64_t>((i - ilo) / C.c) * C.Wc + (j - jlo) / C.c];
+                }
+                val = zv + beta * pold[k];
+            }
+            sP[rr_][cc] = val;
+        }
+        __syncthreads();
+        for (int rr_ = ti; rr_ < TI; rr_ += KB / TJ) {
+            const int i = i0 + rr_, j = j0 + tj;
+            if (i >= ihi || j >= jhi) continue;
+            const int64_t k = G.idx(i, j);
+            const float pcv = sP[rr_ + 1][tj + 1];
+            const float qv = dg[k] * pcv + ax[k] * sP[rr_][tj + 1] + ax[k + G.sy] * sP[rr_ + 2][tj + 1] +
+                             ay[k] * sP[rr_ + 1][tj] + ay[k + 1] * sP[rr_ + 1][tj + 2];
+            pnew[k] = pcv;
+            q[k] = qv;
+            acc += static_cast<double>(pcv) * qv;
+        }
+        __syncthreads();
+    }
+    double v[1] = {acc};
+    put_partials(v, ps);
+}
+
+// alpha = rho / p.q, x += alpha p, r -= alpha q, partials (r.r, (D^-1 r)^2)
+__global__ __launch_bounds__(KB) void k_fcg_update(int64_t n, float* __restrict__ x, const float* p0, const float* p1,
+                                                   float* __restrict__ r, const float* __restrict__ q,
+                                                   const float* __restrict__ dinv, KrylovState* st, const double* ps,
+                                                   int nbs, double* pu, int it) {
+    if (st->done != DONE_RUNNING) return;
+    double alpha;
+    if (!cg_alpha(st, ps, nbs, it, alpha)) return;
+    const float a = static_cast<float>(alpha);
+    const float* __restrict__ p = (it & 1) ? p0 : p1;
+    double rr = 0.0, zz = 0.0;
+    FLAT_LOOP(k, n) {
+        const float xv = a * p[k] + x[k];
+        const float rv = -a * q[k] + r[k];
+        const float zv = dinv[k] * rv;
+        x[k] = xv;
+        r[k] = rv;
+        rr += static_cast<double>(rv) * rv;
+        zz += static_cast<double>(zv) * zv;
+    }
+    double v[2] = {rr, zz};
+    put_partials(v, pu);
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->iter = it + 1;
+}
+
+// r32 = r64 after a reliable update, partials (r.r, (D^-1 r)^2)
+__global__ __launch_bounds__(KB) void k_fcg_resume(int64_t n, const double* __restrict__ r64, float* __restrict__ r32,
+                                                   const float* __restrict__ dinv, const KrylovState* st,
+                                                   double* partials) {
+    if (st->done != DONE_PAUSED) return;
+    double rr = 0.0, zz = 0.0;
+    FLAT_LOOP(k, n) {
+        const float rv = static_cast<float>(r64[k]);
+        const float zv = dinv[k] * rv;
+        r32[k] = rv;
+        rr += static_cast<double>(rv) * rv;
+        zz += static_cast<double>(zv) * zv;
+    }
+    double v[2] = {rr, zz};
+    put_partials(v, partials);
+}
+
+// done: from -> to (single thread)
+__global__ void k_set_done(KrylovState* st, int from, int to) {
+    if (threadIdx.x == 0 && st->done == from) st->done = to;
+}
+
+// ------------------------------------------------------------------ multilevel coarse correction
+// The same flexible PCG with y from a V-cycle instead of the network: the coarse operators of
+// a hierarchy of block aggregations (level 0 = the c x c blocks of the fine grid, each next
+// level factor 4 or 2 coarser, down to at most 16 x 16), one Jacobi sweep before and after the
+// coarse correction on every level, and the coarsest system solved exactly with its inverse,
+// formed once per solve.  Everything is linear in g, so only gamma varies between iterations.
+// Level 0 runs on the whole chip (one thread per cell); the levels below it, at most a few
+// thousand cells, run in one workgroup, which keeps the V-cycle at three launches.
+
+constexpr int ML_MAX_LEVELS = 8;
+constexpr int ML_MAX_DENSE = 256;   // unknowns of the coarsest level
+constexpr int ML_LOWER_MAX = 16384; // cells of level 1 handled by the one-workgroup kernel
+constexpr int ML_T = 16;            // level-0 tile of the restriction kernel (ML_T x ML_T cells)
+
+struct Level {
+    int H, W, c;              // c: factor to the next level (0 on the coarsest)
+    double *acx, *acy, *dc;   // (H + 1) x (W + 1)
+    double *g, *y1, *r1, *y;  // H x W
+};
+
+struct Hierarchy {
+    int n;
+    Level L[ML_MAX_LEVELS];
+};
+
+// operator of level `to` from level `from` (same aggregation rule as the first coarsening;
+// the level arrays have the padded layout with no ghost layer and stride W + 1)
+__global__ __launch_bounds__(KB) void k_ml_coarsen(Level from, Level to) {
+    const int64_t cs = static_cast<int64_t>(to.H + 1) * (to.W + 1);
+    FLAT_LOOP(e, cs) {
+        const int I = static_cast<int>(e / (to.W + 1)), J = static_cast<int>(e - static_cast<int64_t>(I) * (to.W + 1));
+        coarse_cell(from.acx, from.acy, from.dc, from.W + 1, 0, from.c, to.H, to.W, I, J, to.acx, to.acy, to.dc);
+    }
+}
+
+// dense matrix of the coarsest level (row-major m x m, m = H W); one block
+__global__ __launch_bounds__(KB) void k_ml_dense_build(Level L, double* __restrict__ M) {
+    const int m = L.H * L.W;
+    for (int64_t e = threadIdx.x; e < static_cast<int64_t>(m) * m; e += KB) M[e] = 0.0;
+    __syncthreads();
+    for (int p = threadIdx.x; p < m; p += KB) {
+        const int I = p / L.W, J = p - I * L.W, k = I * (L.W + 1) + J;
+        double* row = M + static_cast<int64_t>(p) * m;
+        row[p] = L.dc[k];
+        if (I > 0) row[p - L.W] = L.acx[k];
+        if (I + 1 < L.H) row[p + L.W] = L.acx[k + L.W + 1];
+        if (J > 0) row[p - 1] = L.acy[k];
+        if (J + 1 < L.W) row[p + 1] = L.acy[k + 1];
+    }
+}
+
+// in-place Gauss-Jordan inverse of an SPD m x m matrix (no pivoting needed), one block of 1024
+__global__ __launch_bounds__(1024) void k_ml_invert(double* __restrict__ M, int m) {
+    __shared__ double f[ML_MAX_DENSE];
+    for (int k = 0; k < m; k++) {
+        const double piv = 1.0 / M[static_cast<int64_t>(k) * m + k];
+        for (int i = threadIdx.x; i < m; i += 1024) f[i] = M[static_cast<int64_t>(i) * m + k];
+        __syncthreads();
+        for (int j = threadIdx.x; j < m; j += 1024) {
+            double* rk = M + static_cast<int64_t>(k) * m;
+            rk[j] = (j == k) ? piv : rk[j] * piv;
+        }
+        __syncthreads();
+        for (int64_t e = threadIdx.x; e < static_cast<int64_t>(m) * m; e += 1024) {
+            const int i = static_cast<int>(e / m), j = static_cast<int>(e - static_cast<int64_t>(i) * m);
+            if (i == k) continue;
+            const double rkj = M[static_cast<int64_t>(k) * m + j];
+            M[e] = (j == k) ? -f[i] * piv : M[e] - f[i] * rkj;
+        }
+        __syncthreads();
+    }
+}
+
+// v of level cell (i, j), 0 outside the level grid
+__device__ __forceinline__ double ml_at(const Level& L, const double* v, int i, int j) {
+    return (i < 0 || i >= L.H || j < 0 || j >= L.W) ? 0.0 : v[static_cast<int64_t>(i) * L.W + j];
+}
+__device__ __forceinline__ double ml_apply(const Level& L, const double* v, int i, int j) {
+    const int sc = L.W + 1, k = i * sc + j;
+    return L.dc[k] * ml_at(L, v, i, j) + L.acx[k] * ml_at(L, v, i - 1, j) + L.acx[k + sc] * ml_at(L, v, i + 1, j) +
+           L.acy[k] * ml_at(L, v, i, j - 1) + L.acy[k + 1] * ml_at(L, v, i, j + 1);
+}
+
+// level 0, pre-smoothing and restriction: y1 = g / dc, next.g = block sums of g - A y1.
+// A block owns an ML_T x ML_T tile (whole blocks of the next level: c divides ML_T).
+__global__ __launch_bounds__(KB) void k_ml_down0(Level L, Level next, const KrylovState* st) {
+    if (st->done != DONE_RUNNING) return;
+    __shared__ double sR[ML_T][ML_T];
+    const int tiles_j = ceil_div(L.W, ML_T), ntiles = ceil_div(L.H, ML_T) * tiles_j;
+    const int c = L.c, nb = ML_T / c;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int ti = t / tiles_j, tj = t - ti * tiles_j;
+        const int a = threadIdx.x / ML_T, b = threadIdx.x - a * ML_T;  // KB = ML_T * ML_T
+        const int i = ti * ML_T + a, j = tj * ML_T + b;
+        double rv = 0.0;
+        if (i < L.H && j < L.W) {
+            auto jac = [&](int ii, int jj) {
+                return (ii < 0 || ii >= L.H || jj < 0 || jj >= L.W)
+                           ? 0.0
+                           : L.g[static_cast<int64_t>(ii) * L.W + jj] / L.dc[ii * (L.W + 1) + jj];
+            };
+            const int sc = L.W + 1, k = i * sc + j;
+            const double y1 = jac(i, j);
+            const double ay = L.dc[k] * y1 + L.acx[k] * jac(i - 1, j) + L.acx[k + sc] * jac(i + 1, j) +
+                              L.acy[k] * jac(i, j - 1) + L.acy[k + 1] * jac(i, j + 1);
+            L.y1[static_cast<int64_t>(i) * L.W + j] = y1;
+            rv = L.g[static_cast<int64_t>(i) * L.W + j] - ay;
+        }
+        sR[a][b] = rv;
+        __syncthreads();
+        if (static_cast<int>(threadIdx.x) < nb * nb) {
+            const int bi = threadIdx.x / nb, bj = threadIdx.x - bi * nb;
+            const int I = ti * nb + bi, J = tj * nb + bj;
+            if (I < next.H && J < next.W) {
+                double s = 0.0;
+                for (int u = 0; u < c; u++)
+                    for (int v = 0; v < c; v++) s += sR[bi * c + u][bj * c + v];
+                next.g[static_cast<int64_t>(I) * next.W + J] = s;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// levels 1 .. n-1 in one workgroup: the V-cycle below level 0 (g of level 1 given), y of level 1
+__global__ __launch_bounds__(1024) void k_ml_lower(Hierarchy Hh, const double* __restrict__ Minv,
+                                                   const KrylovState* st) {
+    if (st->done != DONE_RUNNING) return;
+    const int n = Hh.n;
+    for (int l = 1; l + 1 < n; l++) {
+        const Level& L = Hh.L[l];
+        const Level& N = Hh.L[l + 1];
+        const int P = L.H * L.W;
+        for (int p = threadIdx.x; p < P; p += 1024) {
+            const int i = p / L.W, j = p - i * L.W;
+            L.y1[p] = L.g[p] / L.dc[i * (L.W + 1) + j];
+        }
+        __syncthreads();
+        for (int p = threadIdx.x; p < P; p += 1024) {
+            const int i = p / L.W, j = p - i * L.W;
+            L.r1[p] = L.g[p] - ml_apply(L, L.y1, i, j);
+        }
+        __syncthreads();
+        for (int p = threadIdx.x; p < N.H * N.W; p += 1024) {
+            const int I = p / N.W, J = p - I * N.W;
+            double s = 0.0;
+            for (int u = 0; u < L.c; u++)
+                for (int v = 0; v < L.c; v++) s += L.r1[static_cast<int64_t>(I * L.c + u) * L.W + J * L.c + v];
+            N.g[p] = s;
+        }
+        __syncthreads();
+    }
+    {
+        const Level& L = Hh.L[n - 1];
+        const int m = L.H * L.W;
+        for (int p = threadIdx.x; p < m; p += 1024) {
+            const double* row = Minv + static_cast<int64_t>(p) * m;
+            double s = 0.0;
+            for (int q = 0; q < m; q++) s += row[q] * L.g[q];
+            L.y[p] = s;
+        }
+        __syncthreads();
+    }
+    for (int l = n - 2; l >= 1; l--) {
+        const Level& L = Hh.L[l];
+        const Level& N = Hh.L[l + 1];
+        const int P = L.H * L.W;
+        for (int p = threadIdx.x; p < P; p += 1024) {  // y2 into r1
+            const int i = p / L.W, j = p - i * L.W;
+            L.r1[p] = L.y1[p] + N.y[static_cast<int64_t>(i / L.c) * N.W + j / L.c];
+        }
+        __syncthreads();
+        for (int p = threadIdx.x; p < P; p += 1024) {
+            const int i = p / L.W, j = p - i * L.W;
+            L.y[p] = L.r1[p] + (L.g[p] - ml_apply(L, L.r1, i, j)) / L.dc[i * (L.W + 1) + j];
+        }
+        __syncthreads();
+    }
+}
+
+// y2 = y1 + P y_next of level-0 cell (i, j), 0 outside the level grid
+__device__ __forceinline__ double ml_y2(const Level& L, const Level& next, int i, int j) {
+    if (i < 0 || i >= L.H || j < 0 || j >= L.W) return 0.0;
+    return L.y1[static_cast<int64_t>(i) * L.W + j] + next.y[static_cast<int64_t>(i / L.c) * next.W + j / L.c];
+}
+
+// level 0, prolongation and post-smoothing: y = y2 + (g - A y2) / dc (FP32 output)
+__global__ __launch_bounds__(KB) void k_ml_up0(Level L, Level next, float* __restrict__ yout, const KrylovState* st) {
+    if (st->done != DONE_RUNNING) return;
+    const int64_t P = static_cast<int64_t>(L.H) * L.W;
+    const int sc = L.W + 1;
+    FLAT_LOOP(e, P) {
+        const int i = static_cast<int>(e / L.W), j = static_cast<int>(e - static_cast<int64_t>(i) * L.W), k = i * sc + j;
+        const double y2 = ml_y2(L, next, i, j);
+        const double ay = L.dc[k] * y2 + L.acx[k] * ml_y2(L, next, i - 1, j) + L.acx[k + sc] * ml_y2(L, next, i + 1, j) +
+                          L.acy[k] * ml_y2(L, next, i, j - 1) + L.acy[k + 1] * ml_y2(L, next, i, j + 1);
+        yout[e] = static_cast<float>(y2 + (L.g[e] - ay) / L.dc[k]);
+    }
+}
+
+// a single level (the first coarsening already small): y = A^-1 g, FP32 output; one block
+__global__ __launch_bounds__(KB) void k_ml_dense_only(Level L, const double* __restrict__ Minv, float* __restrict__ yout,
+                                                      const KrylovState* st) {
+    if (st->done != DONE_RUNNING) return;
+    const int m = L.H * L.W;
+    for (int p = threadIdx.x; p < m; p += KB) {
+        const double* row = Minv + static_cast<int64_t>(p) * m;
+        double s = 0.0;
+        for (int q = 0; q < m; q++) s += row[q] * L.g[q];
+        yout[p] = static_cast<float>(s);
+    }
+}
+
+// the hierarchy below a first coarsening of H0 x W0 cells
+int ml_plan(int H0, int W0, int* H, int* W, int* c, int* nlev) {
+    int n = 0;
+    H[0] = H0;
+    W[0] = W0;
+    while ((H[n] > 16 || W[n] > 16) && H[n] % 2 == 0 && W[n] % 2 == 0 && n + 1 < ML_MAX_LEVELS) {
+        const int f = (H[n] % 4 == 0 && W[n] % 4 == 0 && H[n] >= 8 && W[n] >= 8) ? 4 : 2;
+        c[n] = f;
+        H[n + 1] = H[n] / f;
+        W[n + 1] = W[n] / f;
+        n++;
+    }
+    c[n] = 0;
+    *nlev = n + 1;
+    if (H[n] * W[n] > ML_MAX_DENSE) {
+        set_error("multilevel: the coarsest level (%d x %d) exceeds %d unknowns", H[n], W[n], ML_MAX_DENSE);
+        return ASD2_ERR_UNSUPPORTED;
+    }
+    if (n >= 1 && H[1] * W[1] > ML_LOWER_MAX) {
+        set_error("multilevel: level 1 (%d x %d) exceeds %d cells", H[1], W[1], ML_LOWER_MAX);
+        return ASD2_ERR_UNSUPPORTED;
+    }
+    return ASD2_OK;
+}
+
+// Flexible PCG with a coarse correction: learned (the network's linear form) or multilevel
+// (a V-cycle on Galerkin aggregations).  Shared: the fine begin pass, the coarse dots, the
+// direction/SpMV and update kernels, FP64 reliable updates and the Jacobi fallback.
+int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2_solve_stats* stats, bool learned) {
+    asd2_unet* N = ctx->net;
+    const Geom& G = ctx->geo;
+    int c0 = 1;
+    if (learned) {
+        if (!N) {
+            set_error("no network attached to the context (asd2_set_preconditioner)");
+            return ASD2_ERR_UNSUPPORTED;
+        }
+        if (N->H * N->coarsen != G.nx || N->W * N->coarsen != G.ny) {
+            set_error("the network was built for %d x %d, the grid is %d x %d", N->H * N->coarsen, N->W * N->coarsen,
+                      G.nx, G.ny);
+            return ASD2_ERR_UNSUPPORTED;
+        }
+        c0 = N->coarsen;
+    } else {
+        while (c0 < 8 && G.nx % (2 * c0) == 0 && G.ny % (2 * c0) == 0) c0 *= 2;
+    }
+    const int64_t n = G.size();
+    System S;
+    ASD2_TRY(system_arrays(ctx, &S, false));
+    ASD2_TRY(ensure_system32(ctx, &S));
+    double *x64, *r64, *lp_part;
+    float *x, *r, *p0, *p1, *q;
+    ASD2_TRY(ws(ctx, "cg_x", n, &x64));
+    ASD2_TRY(ws(ctx, "cg_r", n, &r64));
+    ASD2_TRY(ws(ctx, "mx_x", n, &x));
+    ASD2_TRY(ws(ctx, "mx_r", n, &r));
+    ASD2_TRY(ws(ctx, "mx_p0", n, &p0));
+    ASD2_TRY(ws(ctx, "mx_p1", n, &p1));
+    ASD2_TRY(ws(ctx, "mx_q", n, &q));
+    ASD2_TRY(ws(ctx, "lp_partials", 7 * MAX_BLOCKS, &lp_part));
+    Coarse C{c0, G.nx / c0, G.ny / c0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    const int64_t cs = static_cast<int64_t>(C.Hc + 1) * (C.Wc + 1), Pc = static_cast<int64_t>(C.Hc) * C.Wc;
+    ASD2_TRY(ws(ctx, "lp_acx", cs, &C.acx));
+    ASD2_TRY(ws(ctx, "lp_acy", cs, &C.acy));
+    ASD2_TRY(ws(ctx, "lp_dc", cs, &C.dc));
+    ASD2_TRY(ws(ctx, "lp_g", Pc, &C.g));
+    ASD2_TRY(ws(ctx, "lp_rc", Pc, &C.rc));
+    ASD2_TRY(ws(ctx, "lp_qc", Pc, &C.qc));
+    // multilevel hierarchy: level 0 shares the first coarsening's arrays
+    Hierarchy Hh{};
+    double* Minv = nullptr;
+    float* yml = nullptr;  // the V-cycle's output on level 0 (FP32, read by the SpMV)
+    if (!learned) {
+        int Hl[ML_MAX_LEVELS], Wl[ML_MAX_LEVELS], cl[ML_MAX_LEVELS];
+        ASD2_TRY(ml_plan(C.Hc, C.Wc, Hl, Wl, cl, &Hh.n));
+        static const char* names[ML_MAX_LEVELS][7] = {
+            {"ml0_acx", "ml0_acy", "ml0_dc", "ml0_g", "ml0_y1", "ml0_r1", "ml0_y"},
+            {"ml1_acx", "ml1_acy", "ml1_dc", "ml1_g", "ml1_y1", "ml1_r1", "ml1_y"},
+            {"ml2_acx", "ml2_acy", "ml2_dc", "ml2_g", "ml2_y1", "ml2_r1", "ml2_y"},
+            {"ml3_acx", "ml3_acy", "ml3_dc", "ml3_g", "ml3_y1", "ml3_r1", "ml3_y"},
+            {"ml4_acx", "ml4_acy", "ml4_dc", "ml4_g", "ml4_y1", "ml4_r1", "ml4_y"},
+            {"ml5_acx", "ml5_acy", "ml5_dc", "ml5_g", "ml5_y1", "ml5_r1", "ml5_y"},
+            {"ml6_acx", "ml6_acy", "ml6_dc", "ml6_g", "ml6_y1", "ml6_r1", "ml6_y"},
+            {"ml7_acx", "ml7_acy", "ml7_dc", "ml7_g", "ml7_y1", "ml7_r1", "ml7_y"}};
+        for (int l = 0; l < Hh.n; l++) {
+            Level& L = Hh.L[l];
+            L.H = Hl[l];
+            L.W = Wl[l];
+            L.c = cl[l];
+            const int64_t lc = static_cast<int64_t>(L.H + 1) * (L.W + 1), lp = static_cast<int64_t>(L.H) * L.W;
+            if (l == 0) {
+                L.acx = C.acx;
+                L.acy = C.acy;
+                L.dc = C.dc;
+                L.g = C.g;
+            } else {
+                ASD2_TRY(ws(ctx, names[l][0], lc, &L.acx));
+                ASD2_TRY(ws(ctx, names[l][1], lc, &L.acy));
+                ASD2_TRY(ws(ctx, names[l][2], lc, &L.dc));
+                ASD2_TRY(ws(ctx, names[l][3], lp, &L.g));
+                ASD2_TRY(ws(ctx, names[l][5], lp, &L.r1));
+            }
+            ASD2_TRY(ws(ctx, names[l][4], lp, &L.y1));
+            ASD2_TRY(ws(ctx, names[l][6], lp, &L.y));
+        }
+        const int m = Hh.L[Hh.n - 1].H * Hh.L[Hh.n - 1].W;
+        ASD2_TRY(ws(ctx, "ml_dense", static_cast<int64_t>(m) * m, &Minv));
+        ASD2_TRY(ws(ctx, "ml_yout", Pc, &yml));
+    }
+    double* pb = lp_part;                   // begin: r.D^-1 r, (D^-1 r).q_prev, sum (g / dc)^2
+    double* pc = lp_part + 3 * MAX_BLOCKS;  // coarse dots: y.g, y.A_c y, y.P^T r, y.P^T q_prev
+    const int nbs = spmv_blocks(G), nbf = flat_blocks(n), nbi = interior_blocks(G);
+    const int RI = lp_ri(C.c);
+    const int nbb = std::max(1, std::min<int>(MAX_BLOCKS, ceil_div(G.nx, RI) * ceil_div(G.ny, LP_RJ)));
+    const int nbc = flat_blocks(Pc);
+    asd2_solve_opts oo = *o;
+    oo.mode = ASD2_CG_STANDARD;
+    Driver D;
+    ASD2_TRY(driver_init(ctx, &D));
+    ASD2_TRY(set_state(&D, &oo));
+    hipStream_t saved = learned ? N->stream : nullptr;
+    if (learned) N->stream = ctx->stream;
+    struct Restore {
+        asd2_unet* N;
+        hipStream_t s;
+        ~Restore() {
+            if (N) N->stream = s;
+        }
+    } restore{learned ? N : nullptr, saved};
+    hipLaunchKernelGGL(k_lp_coarse_op, dim3(flat_blocks(cs)), dim3(KB), 0, ctx->stream, G, S.ax, S.ay, S.diag, C);
+    if (learned) {
+        hipLaunchKernelGGL(k_lp_static_features, dim3(nbc), dim3(KB), 0, ctx->stream, C, N->feat, N->flin,
+                           static_cast<float>(G.dx * G.dx / C.c), static_cast<float>(G.dy * G.dy / C.c));
+        ASD2_TRY(unet_forward(N, 1));  // the ReLU pattern of this system, fixed for the solve
+    } else {
+        for (int l = 1; l < Hh.n; l++)
+            hipLaunchKernelGGL(k_ml_coarsen, dim3(flat_blocks(static_cast<int64_t>(Hh.L[l].H + 1) * (Hh.L[l].W + 1))),
+                               dim3(KB), 0, ctx->stream, Hh.L[l - 1], Hh.L[l]);
+        const Level& Lc = Hh.L[Hh.n - 1];
+        hipLaunchKernelGGL(k_ml_dense_build, dim3(1), dim3(KB), 0, ctx->stream, Lc, Minv);
+        hipLaunchKernelGGL(k_ml_invert, dim3(1), dim3(1024), 0, ctx->stream, Minv, Lc.H * Lc.W);
+    }
+    hipLaunchKernelGGL(k_fcg_init, dim3(nbf), dim3(KB), 0, ctx->stream, n, S.rhs, x, r, r64, x64, S.dinv32, p0, p1, q,
+                       D.pu);
+    ASD2_LAUNCH_CHECK();
+    ASD2_TRY(tail<T_FCG_INIT>(&D, nbf, D.pu));
+    int it = 0, restart_it = 0, fr_it = -1, fallbacks = 0;
+    bool use_net = true;  // the coarse correction is in use (false after a fallback to Jacobi)
+    const float* ycoarse = learned ? static_cast<const float*>(N->y) : yml;
+    const int ntiles0 = ceil_div(C.Hc, ML_T) * ceil_div(C.Wc, ML_T);
+    auto batch = [&](int count) -> int {
+        for (int k = 0; k < count; k++, it++) {
+            hipLaunchKernelGGL(k_lp_begin, dim3(nbb), dim3(KB), 0, ctx->stream, G, C, r, S.dinv32, q, S.ax32, S.ay32,
+                               S.diag32, D.st, D.pu, nbf, it, pb);
+            if (use_net) {
+                prof_begin(ctx, ASD2_KID_UNET);
+                if (learned) {
+                    hipLaunchKernelGGL(k_lp_pack, dim3(nbc), dim3(KB), 0, ctx->stream, C, N->flin, D.st, pb, nbb);
+                    ASD2_TRY(unet_forward_linear(N, 1));
+                } else if (Hh.n == 1) {
+                    hipLaunchKernelGGL(k_ml_dense_only, dim3(1), dim3(KB), 0, ctx->stream, Hh.L[0], Minv, yml, D.st);
+                } else {
+                    hipLaunchKernelGGL(k_ml_down0, dim3(std::min(ntiles0, 4096)), dim3(KB), 0, ctx->stream, Hh.L[0],
+                                       Hh.L[1], D.st);
+                    hipLaunchKernelGGL(k_ml_lower, dim3(1), dim3(1024), 0, ctx->stream, Hh, Minv, D.st);
+                    hipLaunchKernelGGL(k_ml_up0, dim3(nbc), dim3(KB), 0, ctx->stream, Hh.L[0], Hh.L[1], yml, D.st);
+                }
+                hipLaunchKernelGGL(k_lp_coarse_dots, dim3(nbc), dim3(KB), 0, ctx->stream, C, ycoarse, D.st, pc);
+                prof_end(ctx, ASD2_KID_UNET);
+            }
+            prof_begin(ctx, ASD2_KID_PCG32_SPMV);
+            hipLaunchKernelGGL(k_lp_spmv, dim3(nbs), dim3(KB), 0, ctx->stream, G, C, S.ax32, S.ay32, S.diag32, r,
+                               S.dinv32, use_net ? ycoarse : nullptr, p0, p1, q, D.st, pb, nbb, pc, nbc, D.ps, it,
+                               restart_it, fr_it, learned ? 0.0 : 1.0);
+            prof_end(ctx, ASD2_KID_PCG32_SPMV);
+            prof_begin(ctx, ASD2_KID_PCG32_UPDATE);
+            hipLaunchKernelGGL(k_fcg_update, dim3(nbf), dim3(KB), 0, ctx->stream, n, x, p0, p1, r, q, S.dinv32, D.st,
+                               D.ps, nbs, D.pu, it);
+            prof_end(ctx, ASD2_KID_PCG32_UPDATE);
+        }
+        ASD2_LAUNCH_CHECK();
+        return ASD2_OK;
+    };
+    // reliable update at the paused iteration k: fold x32 into x64, r64 = b - A x64, r32 = r64,
+    // and re-run iteration k keeping the direction (beta = rho_true / rho_prev); after a
+    // switch of preconditioner the recurrence restarts instead (beta = 0)
+    auto reliable = [&](int k, bool restart) -> int {
+        hipLaunchKernelGGL(k_mx_accum, dim3(nbf), dim3(KB), 0, ctx->stream, n, x64, x, D.st, 0);
+        hipLaunchKernelGGL(k_true_residual, dim3(nbi), dim3(KB), 0, ctx->stream, G, S.ax, S.ay, S.diag, S.rhs, x64, r64,
+                           D.st, D.ps, 0);
+        ASD2_TRY(tail<T_TRUE_RES>(&D, nbi, D.ps));
+        hipLaunchKernelGGL(k_fcg_resume, dim3(nbf), dim3(KB), 0, ctx->stream, n, r64, r, S.dinv32, D.st, D.pu);
+        ASD2_TRY(tail<T_MX_RESUME>(&D, 0, D.pu));
+        ASD2_LAUNCH_CHECK();
+        it = k;
+        if (restart) restart_it = k;
+        else fr_it = k;
+        return ASD2_OK;
+    };
+    const int check_every = o->check_every > 0 ? o->check_every : 4;
+    // stall: ||r||^2 has not halved within this many iterations (CG residuals plateau for a
+    // while even when the method works, so the window is generous)
+    const int stall_window = 400;
+    double best_rr = -1.0;
+    int it_best = 0;
+    int slot = 0;
+    bool stale[2] = {false, false};  // snapshot of a batch enqueued before a host intervention
+    auto enqueue = [&](int sl) -> int {
+        ASD2_TRY(batch(check_every));
+        ASD2_HIP(hipMemcpyAsync(&D.host[sl], D.st, sizeof(KrylovState), hipMemcpyDeviceToHost, ctx->stream));
+        ASD2_HIP(hipEventRecord(D.ev[sl], ctx->stream));
+        return ASD2_OK;
+    };
+    ASD2_TRY(enqueue(0));
+    ASD2_TRY(enqueue(1));
+    for (int guard = 0; guard < 100000000; guard++) {
+        ASD2_HIP(hipEventSynchronize(D.ev[slot]));
+        const KrylovState s = D.host[slot];
+        if (stale[slot]) {
+            stale[slot] = false;
+        } else if (s.done == DONE_BREAKDOWN && use_net) {
+            // the network direction failed: continue from the current iterate with Jacobi
+            use_net = false;
+            fallbacks++;
+            hipLaunchKernelGGL(k_set_done, dim3(1), dim3(64), 0, ctx->stream, D.st, DONE_BREAKDOWN, DONE_PAUSED);
+            ASD2_TRY(reliable(s.iter, true));
+            stale[slot ^ 1] = true;
+        } else if (s.done == DONE_PAUSED) {
+            ASD2_TRY(reliable(s.iter, false));
+            stale[slot ^ 1] = true;
+        } else if (s.done != DONE_RUNNING) {
+            break;
+        } else if (use_net) {
+            if (best_rr < 0.0 || s.rr < 0.5 * best_rr) {
+                best_rr = s.rr;
+                it_best = s.iter;
+            } else if (s.iter - it_best > stall_window) {
+                use_net = false;  // stalled: the remaining iterations use Jacobi, restarting the recurrence
+                fallbacks++;
+                restart_it = it;
+            }
+        }
+        ASD2_TRY(enqueue(slot));
+        slot ^= 1;
+    }
+    KrylovState fs;
+    ASD2_TRY(read_state(&D, &fs));
+    hipLaunchKernelGGL(k_mx_accum, dim3(nbf), dim3(KB), 0, ctx->stream, n, x64, x, D.st, 1);
+    hipLaunchKernelGGL(k_copy_interior, dim3(nbi), dim3(KB), 0, ctx->stream, G, x64, d_ps);
+    ASD2_LAUNCH_CHECK();
+    const int st = status_of(fs);
+    ASD2_TRY(report_true_residual(ctx, &D, S, x64, &fs));
+    fill_stats(fs, stats);
+    if (stats) stats->fallbacks = fallbacks;
+    return st;
+}
+
 }  // namespace asd2
 
 using namespace asd2;
@@ -838,9 +1679,21 @@ int asd2_poisson_solve(asd2_ctx* ctx, const asd2_solve_opts* opts, double* d_ps,
         set_error("bad CG mode %d", opts->mode);
         return ASD2_ERR_ARG;
     }
-    if (opts->precond == ASD2_PRECOND_LEARNED) return learned_solve(ctx, opts, d_ps, stats);
+    if (opts->precond == ASD2_PRECOND_LEARNED || opts->precond == ASD2_PRECOND_MULTILEVEL) {
+        if (opts->precision != ASD2_PREC_MIXED) {
+            set_error("the learned and multilevel preconditioners run in the mixed-precision solver");
+            return ASD2_ERR_UNSUPPORTED;
+        }
+        return coarse_fcg_solve(ctx, opts, d_ps, stats, opts->precond == ASD2_PRECOND_LEARNED);
+    }
     if (opts->precision == ASD2_PREC_MIXED) return solve_mixed(ctx, opts, d_ps, stats);
     return solve_fp64(ctx, opts, d_ps, stats);
+}
+
+int asd2_set_preconditioner(asd2_ctx* ctx, asd2_unet* net) {
+    ASD2_TRY(check_ctx(ctx));
+    ctx->net = net;
+    return ASD2_OK;
 }
 
 int asd2_poisson_bicgstab(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2_solve_stats* stats) {

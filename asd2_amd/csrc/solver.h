@@ -34,7 +34,7 @@ struct KrylovState {
 };
 
 // flexible PCG with the learned preconditioner (learned.hip)
-int learned_solve(asd2_ctx* ctx, const asd2_solve_opts* opts, double* d_ps, asd2_solve_stats* stats);
+int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* opts, double* d_ps, asd2_solve_stats* stats, bool learned);
 
 constexpr int DONE_RUNNING = 0, DONE_CONVERGED = 1, DONE_MAXIT = 2, DONE_BREAKDOWN = 3, DONE_PAUSED = 4;
 

@@ -1,0 +1,147 @@
+"""Training of the learned pressure preconditioner on the GPU.
+
+The training systems are the GFM pressure systems of random droplet configurations, built by
+the library itself: the level set of a union of circles (heavy phase inside, phi = max over
+the circles of r_k - |x - c_k|), its curvature and Heaviside, the divergence of a smooth random
+velocity, assembled by asd2_gfm_assemble (the path of MACSolver2D::SolvePoisson, reference
+src/mac2d.cpp:1249) and exported in FP64.  The residual samples are the right-hand side and the
+residuals b - A x_k after k iterations of the mixed-precision Jacobi-PCG.  numpy only draws the
+random fields and stacks host copies into the FP32 padded layout of asd2_unet_train_step; the
+network's forward and backward passes and the Adam steps are the library's kernels.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import (GRID_P, GRID_U, GRID_V, PREC_MIXED, Context, DeviceArray, UNet, make_fluid)
+
+G = 3
+
+
+def _centres(n, dx, g=G):
+    x = (np.arange(n + 2 * g) + 0.5 - g) * dx
+    return np.meshgrid(x, x, indexing="ij")
+
+
+def droplets(rng, n, dx, count=(2, 8), radius=(0.05, 0.25)):
+    """Signed distance of `count` non-overlapping circles in the unit box (padded layout)."""
+    k = int(rng.integers(count[0], count[1] + 1))
+    circles = []
+    for _ in range(10000):
+        if len(circles) == k:
+            break
+        r = rng.uniform(*radius)
+        c = rng.uniform([r, r], [1 - r, 1 - r])
+        if all(np.hypot(*(c - c2)) > r + r2 + 2 * dx for c2, r2 in circles):
+            circles.append((c, r))
+    X, Y = _centres(n, dx)
+    phi = np.full(X.shape, -np.inf)
+    for c, r in circles:
+        phi = np.maximum(phi, r - np.hypot(X - c[0], Y - c[1]))
+    return np.ascontiguousarray(phi)
+
+
+def smooth_velocity(rng, n, dx, modes=8):
+    """A sum of low Fourier modes on the MAC faces (u on W faces, v on S faces)."""
+    xc = (np.arange(n + 2 * G) + 0.5 - G) * dx
+    xf = (np.arange(n + 2 * G) - G) * dx
+    Xu, Yu = np.meshgrid(xf, xc, indexing="ij")
+    Xv, Yv = np.meshgrid(xc, xf, indexing="ij")
+    u = np.zeros_like(Xu)
+    v = np.zeros_like(Xv)
+    for _ in range(modes):
+        kx, ky = rng.integers(1, 6, size=2)
+        a, b = rng.standard_normal(2)
+        px, py = rng.uniform(0, 2 * np.pi, size=2)
+        u += a * np.sin(np.pi * kx * Xu + px) * np.cos(np.pi * ky * Yu + py)
+        v += b * np.cos(np.pi * kx * Xv + px) * np.sin(np.pi * ky * Yv + py)
+    return np.ascontiguousarray(u / modes), np.ascontiguousarray(v / modes)
+
+
+class Samples:
+    """Training systems on the device: (S, n + 6, n + 6) FP32 stacks of residual, Jacobi
+    inverse, W-face and S-face coefficients and diagonal."""
+
+    def __init__(self, arrays, n):
+        self.n = n
+        self.count = arrays[0].shape[0]
+        self.dev = [DeviceArray.from_host(np.ascontiguousarray(a, np.float32)) for a in arrays]
+        self.stride = (n + 2 * G) ** 2 * 4  # bytes per sample
+
+    def batch(self, start):
+        """device views of samples start, start + 1, ... (contiguous)"""
+        return [_View(d.ptr + start * self.stride) for d in self.dev]
+
+
+class _View:
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+
+def make_samples(n, systems=6, snapshots=(0, 10, 40, 120), seed=0, rho_h=1000.0, rho_l=(1.0, 10.0),
+                 sigma=(0.3, 1.0), log=None):
+    """`systems` random droplet systems on an n x n grid, one sample per snapshot k (the
+    residual after k mixed-precision Jacobi-PCG iterations; k = 0: the right-hand side)."""
+    rng = np.random.default_rng(seed)
+    dx = 1.0 / n
+    cols = [[] for _ in range(5)]
+    ctx = Context(n, n, dx, dx)
+    ctx.set_bc(GRID_P, ("neumann",) * 4)
+    ctx.set_bc(GRID_U, ("dirichlet",) * 4)
+    ctx.set_bc(GRID_V, ("dirichlet",) * 4)
+    ls, kap, H, div = ctx.array(), ctx.array(), ctx.array(), ctx.array()
+    u, v, ps, y = ctx.array(), ctx.array(), ctx.array(), ctx.array()
+    t0 = time.perf_counter()
+    for s in range(systems):
+        ls.upload(droplets(rng, n, dx))
+        ctx.bc_apply(GRID_P, ls)
+        ctx.kappa(ls, kap)
+        ctx.bc_apply(GRID_P, kap)
+        ctx.heaviside(ls, H)
+        uu, vv = smooth_velocity(rng, n, dx)
+        u.upload(uu)
+        v.upload(vv)
+        ctx.bc_apply(GRID_U, u)
+        ctx.bc_apply(GRID_V, v)
+        ctx.divergence(u, v, div)
+        rl = float(np.exp(rng.uniform(np.log(rho_l[0]), np.log(rho_l[1]))))
+        fl = make_fluid(rho_h, rl, 1.0, 0.01, float(rng.uniform(*sigma)), 0.3 * dx)
+        ctx.gfm_assemble(fl, ls, kap, H, div)
+        ax, ay, dg, rhs = ctx.gfm_export()
+        dinv = np.zeros_like(dg)
+        inner = (slice(G, G + n), slice(G, G + n))
+        dinv[inner] = 1.0 / dg[inner]
+        for k in snapshots:
+            if k == 0:
+                r = rhs.copy()
+            else:
+                ps.upload(np.zeros_like(rhs))
+                ctx.poisson_solve(ps, Context.opts(tol=1e-30, max_iter=k, precision=PREC_MIXED), raise_on_fail=False)
+                ctx.gfm_apply(ps, y)
+                r = rhs - y.download()
+            rm = np.zeros_like(r)
+            rm[inner] = r[inner]
+            for col, a in zip(cols, (rm, dinv, ax, ay, dg)):
+                col.append(a.astype(np.float32))
+        if log:
+            log(f"system {s + 1}/{systems} ({time.perf_counter() - t0:.1f} s)")
+    order = rng.permutation(len(cols[0]))
+    return Samples([np.stack([col[i] for i in order]) for col in cols], n)
+
+
+def train(net: UNet, samples: Samples, steps=300, batch=4, lr=2e-3, lr_final=5e-4, seed=0, log=None):
+    """Adam on the energy loss over random contiguous windows of the (shuffled) samples;
+    returns the per-step losses."""
+    rng = np.random.default_rng(seed)
+    batch = min(batch, samples.count, net.max_batch)
+    dx = 1.0 / samples.n
+    losses = []
+    for step in range(steps):
+        start = int(rng.integers(0, samples.count - batch + 1))
+        rate = lr if step < 0.75 * steps else lr_final
+        losses.append(net.train_step(batch, *samples.batch(start), dx, dx, rate))
+        if log and (step % 50 == 0 or step == steps - 1):
+            log(f"step {step}: loss {losses[-1]:.3f}")
+    return losses

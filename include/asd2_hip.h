@@ -58,7 +58,8 @@ extern "C" {
 
 /* preconditioner of the pressure solve */
 #define ASD2_PRECOND_JACOBI 0
-#define ASD2_PRECOND_LEARNED 1 /* U-Net correction inside flexible PCG, Jacobi fallback */
+#define ASD2_PRECOND_LEARNED 1    /* U-Net coarse correction inside flexible PCG, Jacobi fallback */
+#define ASD2_PRECOND_MULTILEVEL 2 /* Galerkin multilevel coarse correction inside flexible PCG */
 
 typedef struct asd2_ctx asd2_ctx;
 
@@ -225,6 +226,48 @@ int asd2_csr_pcg(asd2_ctx* ctx, int n, const int* d_rowptr, const int* d_cols,
 int asd2_csr_bicgstab(asd2_ctx* ctx, int n, const int* d_rowptr, const int* d_cols,
                       const double* d_vals, const double* d_mdiag, const double* d_b, double* d_x,
                       const asd2_solve_opts* opts, asd2_solve_stats* stats);
+
+/* ---------------------------------------------------------------- learned preconditioner
+ * No reference counterpart: the reference preconditions its CG with the Jacobi diagonal only
+ * (src/poisson2d.cpp:346-347).  A 4-level U-Net (base 16 channels, 3x3, BF16 weights and
+ * activations, FP32 accumulation) runs on the coarse grid of c x c blocks and proposes the
+ * direction y of a coarse correction (its linear form, asd2_unet_forward_linear, on the coarse
+ * residual); inside flexible PCG the preconditioner is
+ *     z = D^-1 r + gamma P y,  gamma = (y.g) / (y.A_c y),  g = P^T (r - A D^-1 r),  A_c = P^T A P
+ * (P: piecewise-constant prolongation), i.e. Jacobi plus the energy-optimal step along the
+ * network's coarse correction.  Images are coarse grids (nx / c) x (ny / c), channels last. */
+typedef struct asd2_unet asd2_unet;
+/* network for fine grids nx x ny coarsened by c (a power of two <= 32, nx and ny divisible by
+ * 8 c), buffers for up to max_batch images; He-uniform weights from `seed` */
+int asd2_unet_create(int nx, int ny, int coarsen, int max_batch, unsigned long long seed, asd2_unet** out);
+int asd2_unet_destroy(asd2_unet* net);
+int asd2_unet_param_count(const asd2_unet* net, long long* count);
+/* FP32 parameter vector (host): per conv layer weights [9][c_out][c_in] (tap = 3 (di + 1) +
+ * (dj + 1)), then all biases, then the 16 output weights and the output bias */
+int asd2_unet_set_params(asd2_unet* net, const float* host, long long count);
+int asd2_unet_get_params(asd2_unet* net, float* host, long long count);
+int asd2_unet_get_grads(asd2_unet* net, float* host, long long count);
+/* network output y (FP32 [batch][H][W], device) of features (FP32 [batch][H][W][4], device),
+ * H x W the coarse grid */
+int asd2_unet_forward(asd2_unet* net, int batch, const float* d_feat, float* d_y);
+/* parameter gradients (asd2_unet_get_grads) for dL/dy = d_dy after asd2_unet_forward */
+int asd2_unet_backward(asd2_unet* net, int batch, const float* d_dy);
+/* the preconditioner's form of the network: the ReLU pattern of a pass on channels 1..3 (channel
+ * 0 zeroed) frozen, then a pass without biases on channel 0 alone, each conv output multiplied
+ * by (pattern > 0); for fixed coefficient channels y is linear in channel 0 */
+int asd2_unet_forward_linear(asd2_unet* net, int batch, const float* d_feat, float* d_y);
+/* parameter gradients of that pass for dL/dy = d_dy after asd2_unet_forward_linear (the
+ * pattern held fixed; bias gradients zero) */
+int asd2_unet_backward_linear(asd2_unet* net, int batch, const float* d_dy);
+/* one Adam step on the mean energy loss E(z) / r.D^-1 r, E(z) = z.A z - 2 z.r (the A-norm error
+ * of z as a solve of A e = r, shifted by a constant of the system), of `batch` fine systems given
+ * in the solver's FP32 padded layout ((nx + 6) (ny + 6) values per system, concatenated):
+ * residual, Jacobi inverse, W-face and S-face coefficients, diagonal; loss_out = loss before the
+ * step */
+int asd2_unet_train_step(asd2_unet* net, int batch, const float* d_r, const float* d_dinv, const float* d_ax,
+                         const float* d_ay, const float* d_dg, double dx, double dy, float lr, float* loss_out);
+/* network used by asd2_poisson_solve with precond = ASD2_PRECOND_LEARNED (not owned; NULL detaches) */
+int asd2_set_preconditioner(asd2_ctx* ctx, asd2_unet* net);
 
 /* ---------------------------------------------------------------- timing */
 /* records the start/stop of a region on the context's stream (hipEvents) */
