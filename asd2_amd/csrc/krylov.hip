@@ -1547,10 +1547,12 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
     const int ntiles0 = ceil_div(C.Hc, ML_T) * ceil_div(C.Wc, ML_T);
     auto batch = [&](int count) -> int {
         for (int k = 0; k < count; k++, it++) {
+            prof_begin(ctx, ASD2_KID_FCG_BEGIN);
             hipLaunchKernelGGL(k_lp_begin, dim3(nbb), dim3(KB), 0, ctx->stream, G, C, r, S.dinv32, q, S.ax32, S.ay32,
                                S.diag32, D.st, D.pu, nbf, it, pb);
+            prof_end(ctx, ASD2_KID_FCG_BEGIN);
             if (use_net) {
-                prof_begin(ctx, ASD2_KID_UNET);
+                prof_begin(ctx, ASD2_KID_COARSE);
                 if (learned) {
                     hipLaunchKernelGGL(k_lp_pack, dim3(nbc), dim3(KB), 0, ctx->stream, C, N->flin, D.st, pb, nbb);
                     ASD2_TRY(unet_forward_linear(N, 1));
@@ -1563,17 +1565,17 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
                     hipLaunchKernelGGL(k_ml_up0, dim3(nbc), dim3(KB), 0, ctx->stream, Hh.L[0], Hh.L[1], yml, D.st);
                 }
                 hipLaunchKernelGGL(k_lp_coarse_dots, dim3(nbc), dim3(KB), 0, ctx->stream, C, ycoarse, D.st, pc);
-                prof_end(ctx, ASD2_KID_UNET);
+                prof_end(ctx, ASD2_KID_COARSE);
             }
-            prof_begin(ctx, ASD2_KID_PCG32_SPMV);
+            prof_begin(ctx, ASD2_KID_FCG_SPMV);
             hipLaunchKernelGGL(k_lp_spmv, dim3(nbs), dim3(KB), 0, ctx->stream, G, C, S.ax32, S.ay32, S.diag32, r,
                                S.dinv32, use_net ? ycoarse : nullptr, p0, p1, q, D.st, pb, nbb, pc, nbc, D.ps, it,
                                restart_it, fr_it, learned ? 0.0 : 1.0);
-            prof_end(ctx, ASD2_KID_PCG32_SPMV);
-            prof_begin(ctx, ASD2_KID_PCG32_UPDATE);
+            prof_end(ctx, ASD2_KID_FCG_SPMV);
+            prof_begin(ctx, ASD2_KID_FCG_UPDATE);
             hipLaunchKernelGGL(k_fcg_update, dim3(nbf), dim3(KB), 0, ctx->stream, n, x, p0, p1, r, q, S.dinv32, D.st,
                                D.ps, nbs, D.pu, it);
-            prof_end(ctx, ASD2_KID_PCG32_UPDATE);
+            prof_end(ctx, ASD2_KID_FCG_UPDATE);
         }
         ASD2_LAUNCH_CHECK();
         return ASD2_OK;

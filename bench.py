@@ -3,11 +3,13 @@
 One step = the reference's stationary-droplet loop (tests/LargeTests/test_mac2d_large.cpp:
 496-535): HJ-WENO5/SSP-RK3 advection, 3 MinRK2 reinit iterations, curvature and
 Heaviside, RK3 momentum predictor, divergence, GFM assembly, pressure solve to a relative
-FP64 residual of 1e-8, projection, CFL time step.  The step is timed twice from the same
-state: with the plain FP64 Jacobi-PCG and with the FP32 PCG + FP64 reliable updates.
-`value` is the ms/step of the faster pressure path (inputs resident in HBM).
+FP64 residual of 1e-8, projection, CFL time step.  The step is timed from the same state with
+four pressure paths: the plain FP64 Jacobi-PCG, the FP32 PCG with FP64 reliable updates, and
+that flexible FP32 PCG with a coarse correction from the learned network (trained first, on
+the GPU, on systems the library builds from random droplet fields) or from the multilevel
+V-cycle.  `value` is the ms/step of the fastest path (inputs resident in HBM).
 
-Usage: python bench.py [--steps K] [--warmup W] [--n 2048] [--no-cpu] [--pmc-workload]
+Usage: python bench.py [--steps K] [--warmup W] [--n 2048] [--no-cpu] [--no-learned] [--pmc-workload]
 """
 from __future__ import annotations
 
@@ -33,6 +35,9 @@ BYTES_PER_CELL = {
     "pcg64_update": 8 * 8,   # x, p, r, q, dinv -> x, r, z
     "pcg32_spmv": 6 * 4,
     "pcg32_update": 8 * 4,
+    "fcg_begin": 6 * 4,      # r, dinv, q, beta_x, beta_y, diag -> (coarse sums only)
+    "fcg_spmv": 8 * 4,       # r, dinv, p_old, beta_x, beta_y, diag -> p_new, q (z formed on the fly)
+    "fcg_update": 7 * 4,     # x, p, r, q, dinv -> x, r
     "weno_stage": 112 / 3,   # 32 + 40 + 40 bytes over the 3 stages of an advection step
     "reinit": 28,            # MinRK2 substage A 24 B, substage B 32 B
     "gfm": 7 * 8,            # phi, kappa, div -> beta_x, beta_y, rhs, dinv
@@ -131,7 +136,27 @@ def run_path(sim, opts, steps, warmup, snap):
     wall_ms = (time.perf_counter() - t0) * 1e3
     return {"ms_per_step": wall_ms / steps, "event_ms_per_step": gpu_ms / steps,
             "pcg_iters_per_step": float(np.mean(iters)), "pcg_iters": iters,
-            "true_rel_residual": st.true_rel_residual, "reliable_updates": st.reliable_updates}
+            "true_rel_residual": st.true_rel_residual, "reliable_updates": st.reliable_updates,
+            "fallbacks": st.fallbacks}
+
+
+def train_network(A, n, coarsen, train_n=512, systems=8, steps=400, seed=1):
+    """The learned preconditioner for an n^2 grid: trained on GPU-built systems at train_n^2
+    (the network is convolutional; the coarse images of train_n / coarsen pixels share its
+    weights with the n / coarsen images), then copied into an n^2 network."""
+    from asd2_amd import learned as LN
+    t0 = time.perf_counter()
+    samples = LN.make_samples(train_n, systems=systems, seed=seed)
+    t1 = time.perf_counter()
+    small = A.UNet(train_n, train_n, max_batch=4, seed=seed + 1, coarsen=coarsen)
+    losses = LN.train(small, samples, steps=steps, batch=4, seed=seed + 2)
+    t2 = time.perf_counter()
+    big = A.UNet(n, n, max_batch=1, coarsen=coarsen)
+    big.set_params(small.get_params())
+    info = {"train_grid": train_n, "coarsen": coarsen, "systems": systems, "samples": samples.count,
+            "steps": steps, "batch": 4, "loss_first": float(losses[0]), "loss_last20": float(np.mean(losses[-20:])),
+            "data_s": t1 - t0, "train_s": t2 - t1}
+    return big, info
 
 
 def profile_kernels(sim, opts, snap, kids):
@@ -213,6 +238,8 @@ def pmc_workload(n):
     for prec in (A.PREC_FP64, A.PREC_MIXED):
         c.poisson_solve(sim.ps, A.Context.opts(tol=1e-30, max_iter=32, precision=prec, mode=A.CG_STANDARD,
                                                check_every=8), raise_on_fail=False)
+    c.poisson_solve(sim.ps, A.Context.opts(tol=1e-30, max_iter=32, precision=A.PREC_MIXED,
+                                           precond=A.PRECOND_MULTILEVEL, check_every=8), raise_on_fail=False)
     c.sync()
     print(json.dumps({"pmc_workload": {"grid": n, "cells": n * n, "copy_doubles": copy_n,
                                        "copy_bytes_per_launch": 16 * copy_n}}))
@@ -227,6 +254,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip training and timing the learned path")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="run the short fixed workload used for hardware-counter passes")
     args = ap.parse_args()
@@ -241,23 +269,37 @@ def main():
     import asd2_amd as A
     sim = Droplet(args.n, A)
     snap = sim.snapshot()
-    o64 = A.Context.opts(tol=args.tol, precision=A.PREC_FP64, mode=A.CG_STANDARD)
-    o32 = A.Context.opts(tol=args.tol, precision=A.PREC_MIXED, mode=A.CG_STANDARD)
+    opts = {
+        "fp64": A.Context.opts(tol=args.tol, precision=A.PREC_FP64, mode=A.CG_STANDARD),
+        "mixed": A.Context.opts(tol=args.tol, precision=A.PREC_MIXED, mode=A.CG_STANDARD),
+        "multilevel": A.Context.opts(tol=args.tol, precision=A.PREC_MIXED, precond=A.PRECOND_MULTILEVEL),
+        "learned": A.Context.opts(tol=args.tol, precision=A.PREC_MIXED, precond=A.PRECOND_LEARNED),
+    }
     print(f"# device: {A.device_name()}  grid {args.n}^2", file=sys.stderr, flush=True)
-    fp64 = run_path(sim, o64, args.steps, args.warmup, snap)
-    print(f"# fp64 PCG: {fp64['ms_per_step']:.1f} ms/step, {fp64['pcg_iters_per_step']:.0f} iters",
-          file=sys.stderr, flush=True)
-    mixed = run_path(sim, o32, args.steps, args.warmup, snap)
-    print(f"# mixed PCG: {mixed['ms_per_step']:.1f} ms/step, {mixed['pcg_iters_per_step']:.0f} iters",
-          file=sys.stderr, flush=True)
-    best_name, best = min((("mixed", mixed), ("fp64", fp64)), key=lambda kv: kv[1]["ms_per_step"])
+    paths, training = {}, None
+    for name in ("fp64", "mixed", "multilevel", "learned"):
+        if name == "learned":
+            if args.no_learned:
+                continue
+            net, training = train_network(A, args.n, coarsen=8)
+            print(f"# learned: trained on {training['samples']} samples in {training['train_s']:.1f} s, "
+                  f"loss {training['loss_first']:.2f} -> {training['loss_last20']:.2f}", file=sys.stderr, flush=True)
+            sim.ctx.set_preconditioner(net)
+        paths[name] = run_path(sim, opts[name], args.steps, args.warmup, snap)
+        print(f"# {name}: {paths[name]['ms_per_step']:.1f} ms/step, {paths[name]['pcg_iters_per_step']:.0f} iters",
+              file=sys.stderr, flush=True)
+    best_name = min((k for k in paths if k != "fp64"), key=lambda k: paths[k]["ms_per_step"])
+    best = paths[best_name]
 
     kernels = {}
     if not args.no_profile:
-        kernels.update(profile_kernels(sim, o32, snap, ["pcg32_spmv", "pcg32_update", "weno_stage",
-                                                           "reinit", "gfm"]))
-        kernels.update(profile_kernels(sim, o64, snap, ["pcg64_spmv", "pcg64_update"]))
-    prefix = "pcg32" if best_name == "mixed" else "pcg64"
+        kernels.update(profile_kernels(sim, opts["mixed"], snap, ["weno_stage", "reinit", "gfm"]))
+        if best_name == "mixed":
+            kernels.update(profile_kernels(sim, opts["mixed"], snap, ["pcg32_spmv", "pcg32_update"]))
+        else:
+            kernels.update(profile_kernels(sim, opts[best_name], snap, ["fcg_begin", "fcg_spmv", "fcg_update"]))
+        kernels.update(profile_kernels(sim, opts["fp64"], snap, ["pcg64_spmv", "pcg64_update"]))
+    prefix = {"mixed": "pcg32", "fp64": "pcg64"}.get(best_name, "fcg")
     dom = max((k for k in kernels if k.startswith(prefix)), key=lambda k: kernels[k]["total_ms"], default=None)
     roofline = None
     if dom:
@@ -265,21 +307,24 @@ def main():
         roofline = {"kernel": dom, "bound": "hbm", "achieved": kd["achieved_GBs"], "peak": PEAK_HBM_GBS,
                     "unit": "GB/s", "frac": kd["achieved_GBs"] / PEAK_HBM_GBS, "traffic": None,
                     "avg_us": kd["avg_us"], "bytes_per_cell": kd["bytes_per_cell"]}
+    fp64 = paths["fp64"]
 
     cpu = None if args.no_cpu else cpu_baseline(args.n, fp64["pcg_iters_per_step"])
     line = {
         "metric": METRIC, "value": best["ms_per_step"], "unit": "ms/step", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": best["ms_per_step"],
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 level set/assembly/outer residual; f32 inner PCG" if best_name == "mixed" else "f64",
+        "dtype": "f64 level set/assembly/outer residual; f32 inner PCG" + ("; bf16 network" if best_name == "learned"
+                                                                           else ""),
         "data": "synthetic",
         "config": {"workload": f"stationary droplet two-phase step at {args.n}x{args.n} (C1 physics: "
                                f"r=0.25, rho 1000:1, mu 100:1, sigma=1; 3 MinRK2 iterations; "
                                f"pressure to rel. FP64 residual {args.tol:g})",
                    "grid": [args.n, args.n], "pressure_path": best_name},
-        "fp64_pcg": fp64, "mixed_pcg": mixed,
-        "speedup_vs_fp64_pcg": fp64["ms_per_step"] / best["ms_per_step"],
-        "pcg_iters": {"fp64": fp64["pcg_iters_per_step"], "mixed": mixed["pcg_iters_per_step"]},
+        "paths": paths,
+        "speedup_vs_fp64_pcg": {k: fp64["ms_per_step"] / v["ms_per_step"] for k, v in paths.items()},
+        "pcg_iters": {k: v["pcg_iters_per_step"] for k, v in paths.items()},
+        "learned_training": training,
         "roofline": roofline, "kernels": kernels, "cpu_baseline": cpu,
     }
     print(json.dumps(line))

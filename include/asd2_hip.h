@@ -282,7 +282,10 @@ int asd2_timer_stop(asd2_ctx* ctx, double* ms);
 #define ASD2_KID_PCG32_SPMV 5     /* FP32 counterparts */
 #define ASD2_KID_PCG32_UPDATE 6
 #define ASD2_KID_GFM 7            /* GFM assembly */
-#define ASD2_KID_UNET 8           /* learned preconditioner application */
+#define ASD2_KID_COARSE 8         /* coarse correction of the flexible PCG (network or V-cycle) */
+#define ASD2_KID_FCG_BEGIN 9      /* flexible PCG: fine pass for the coarse residual and dots */
+#define ASD2_KID_FCG_SPMV 10      /* flexible PCG: z formed on the fly, direction, SpMV, p.q */
+#define ASD2_KID_FCG_UPDATE 11    /* flexible PCG: x/r update + dots */
 int asd2_profile_kernel(asd2_ctx* ctx, int kernel_id);
 int asd2_profile_read(asd2_ctx* ctx, int* launches, double* total_ms);
 
