@@ -1188,12 +1188,11 @@ __global__ void k_set_done(KrylovState* st, int from, int to) {
 // level factor 4 or 2 coarser, down to at most 16 x 16), one Jacobi sweep before and after the
 // coarse correction on every level, and the coarsest system solved exactly with its inverse,
 // formed once per solve.  Everything is linear in g, so only gamma varies between iterations.
-// Level 0 runs on the whole chip (one thread per cell); the levels below it, at most a few
-// thousand cells, run in one workgroup, which keeps the V-cycle at three launches.
+// Levels of at least ML_WIDE cells run on the whole chip (one thread per cell); the small tail
+// of the hierarchy and the coarsest solve run in one workgroup.
 
 constexpr int ML_MAX_LEVELS = 8;
 constexpr int ML_MAX_DENSE = 256;   // unknowns of the coarsest level
-constexpr int ML_LOWER_MAX = 16384; // cells of level 1 handled by the one-workgroup kernel
 constexpr int ML_T = 16;            // level-0 tile of the restriction kernel (ML_T x ML_T cells)
 
 struct Level {
@@ -1233,26 +1232,23 @@ __global__ __launch_bounds__(KB) void k_ml_dense_build(Level L, double* __restri
     }
 }
 
-// in-place Gauss-Jordan inverse of an SPD m x m matrix (no pivoting needed), one block of 1024
-__global__ __launch_bounds__(1024) void k_ml_invert(double* __restrict__ M, int m) {
-    __shared__ double f[ML_MAX_DENSE];
-    for (int k = 0; k < m; k++) {
-        const double piv = 1.0 / M[static_cast<int64_t>(k) * m + k];
-        for (int i = threadIdx.x; i < m; i += 1024) f[i] = M[static_cast<int64_t>(i) * m + k];
-        __syncthreads();
-        for (int j = threadIdx.x; j < m; j += 1024) {
-            double* rk = M + static_cast<int64_t>(k) * m;
-            rk[j] = (j == k) ? piv : rk[j] * piv;
-        }
-        __syncthreads();
-        for (int64_t e = threadIdx.x; e < static_cast<int64_t>(m) * m; e += 1024) {
-            const int i = static_cast<int>(e / m), j = static_cast<int>(e - static_cast<int64_t>(i) * m);
-            if (i == k) continue;
-            const double rkj = M[static_cast<int64_t>(k) * m + j];
-            M[e] = (j == k) ? -f[i] * piv : M[e] - f[i] * rkj;
-        }
-        __syncthreads();
+// One Gauss-Jordan step of the inverse of an SPD m x m matrix (no pivoting needed), pivot k:
+// Mout from Min (two buffers, so no block reads a row another block rewrites); block = row i,
+// thread = column j.  After steps k = 0 .. m-1 the last output holds the inverse.
+__global__ __launch_bounds__(ML_MAX_DENSE) void k_ml_gj_step(const double* __restrict__ Min, double* __restrict__ Mout,
+                                                             int m, int k) {
+    const int i = blockIdx.x, j = threadIdx.x;
+    if (j >= m) return;
+    const double piv = 1.0 / Min[static_cast<int64_t>(k) * m + k];
+    const double rkj = Min[static_cast<int64_t>(k) * m + j];
+    double v;
+    if (i == k) {
+        v = (j == k) ? piv : rkj * piv;
+    } else {
+        const double f = Min[static_cast<int64_t>(i) * m + k];
+        v = (j == k) ? -f * piv : Min[static_cast<int64_t>(i) * m + j] - f * rkj * piv;
     }
+    Mout[static_cast<int64_t>(i) * m + j] = v;
 }
 
 // v of level cell (i, j), 0 outside the level grid
@@ -1265,9 +1261,10 @@ __device__ __forceinline__ double ml_apply(const Level& L, const double* v, int 
            L.acy[k] * ml_at(L, v, i, j - 1) + L.acy[k + 1] * ml_at(L, v, i, j + 1);
 }
 
-// level 0, pre-smoothing and restriction: y1 = g / dc, next.g = block sums of g - A y1.
-// A block owns an ML_T x ML_T tile (whole blocks of the next level: c divides ML_T).
-__global__ __launch_bounds__(KB) void k_ml_down0(Level L, Level next, const KrylovState* st) {
+// pre-smoothing and restriction on a level run by the whole chip: y1 = g / dc, next.g = block
+// sums of g - A y1.  A block owns an ML_T x ML_T tile (whole blocks of the next level: c divides
+// ML_T).
+__global__ __launch_bounds__(KB) void k_ml_down(Level L, Level next, const KrylovState* st) {
     if (st->done != DONE_RUNNING) return;
     __shared__ double sR[ML_T][ML_T];
     const int tiles_j = ceil_div(L.W, ML_T), ntiles = ceil_div(L.H, ML_T) * tiles_j;
@@ -1306,12 +1303,31 @@ __global__ __launch_bounds__(KB) void k_ml_down0(Level L, Level next, const Kryl
     }
 }
 
-// levels 1 .. n-1 in one workgroup: the V-cycle below level 0 (g of level 1 given), y of level 1
-__global__ __launch_bounds__(1024) void k_ml_lower(Hierarchy Hh, const double* __restrict__ Minv,
+// y = A^-1 g on the coarsest level with 4 threads per row (FP64), in a block of 1024
+__device__ __forceinline__ void ml_dense(const Level& L, const double* __restrict__ Minv, double* __restrict__ sred) {
+    const int m = L.H * L.W;
+    const int row = threadIdx.x >> 2, part = threadIdx.x & 3;
+    for (int p0 = 0; p0 < m; p0 += 256) {
+        const int p = p0 + row;
+        double s = 0.0;
+        if (p < m) {
+            const double* rw = Minv + static_cast<int64_t>(p) * m;
+            for (int q = part; q < m; q += 4) s += rw[q] * L.g[q];
+        }
+        s += __shfl_xor(s, 1, 64);
+        s += __shfl_xor(s, 2, 64);
+        if (p < m && part == 0) L.y[p] = s;
+    }
+    (void)sred;
+}
+
+// levels first .. n-1 in one workgroup (the small tail of the V-cycle; g of level `first`
+// given), y of level `first`
+__global__ __launch_bounds__(1024) void k_ml_lower(Hierarchy Hh, int first, const double* __restrict__ Minv,
                                                    const KrylovState* st) {
     if (st->done != DONE_RUNNING) return;
     const int n = Hh.n;
-    for (int l = 1; l + 1 < n; l++) {
+    for (int l = first; l + 1 < n; l++) {
         const Level& L = Hh.L[l];
         const Level& N = Hh.L[l + 1];
         const int P = L.H * L.W;
@@ -1334,18 +1350,9 @@ __global__ __launch_bounds__(1024) void k_ml_lower(Hierarchy Hh, const double* _
         }
         __syncthreads();
     }
-    {
-        const Level& L = Hh.L[n - 1];
-        const int m = L.H * L.W;
-        for (int p = threadIdx.x; p < m; p += 1024) {
-            const double* row = Minv + static_cast<int64_t>(p) * m;
-            double s = 0.0;
-            for (int q = 0; q < m; q++) s += row[q] * L.g[q];
-            L.y[p] = s;
-        }
-        __syncthreads();
-    }
-    for (int l = n - 2; l >= 1; l--) {
+    ml_dense(Hh.L[n - 1], Minv, nullptr);
+    __syncthreads();
+    for (int l = n - 2; l >= first; l--) {
         const Level& L = Hh.L[l];
         const Level& N = Hh.L[l + 1];
         const int P = L.H * L.W;
@@ -1362,14 +1369,15 @@ __global__ __launch_bounds__(1024) void k_ml_lower(Hierarchy Hh, const double* _
     }
 }
 
-// y2 = y1 + P y_next of level-0 cell (i, j), 0 outside the level grid
+// y2 = y1 + P y_next of cell (i, j), 0 outside the level grid
 __device__ __forceinline__ double ml_y2(const Level& L, const Level& next, int i, int j) {
     if (i < 0 || i >= L.H || j < 0 || j >= L.W) return 0.0;
     return L.y1[static_cast<int64_t>(i) * L.W + j] + next.y[static_cast<int64_t>(i / L.c) * next.W + j / L.c];
 }
 
-// level 0, prolongation and post-smoothing: y = y2 + (g - A y2) / dc (FP32 output)
-__global__ __launch_bounds__(KB) void k_ml_up0(Level L, Level next, float* __restrict__ yout, const KrylovState* st) {
+// prolongation and post-smoothing on a level run by the whole chip: y = y2 + (g - A y2) / dc,
+// into L.y, or into yout (FP32) on level 0
+__global__ __launch_bounds__(KB) void k_ml_up(Level L, Level next, float* __restrict__ yout, const KrylovState* st) {
     if (st->done != DONE_RUNNING) return;
     const int64_t P = static_cast<int64_t>(L.H) * L.W;
     const int sc = L.W + 1;
@@ -1378,7 +1386,9 @@ __global__ __launch_bounds__(KB) void k_ml_up0(Level L, Level next, float* __res
         const double y2 = ml_y2(L, next, i, j);
         const double ay = L.dc[k] * y2 + L.acx[k] * ml_y2(L, next, i - 1, j) + L.acx[k + sc] * ml_y2(L, next, i + 1, j) +
                           L.acy[k] * ml_y2(L, next, i, j - 1) + L.acy[k + 1] * ml_y2(L, next, i, j + 1);
-        yout[e] = static_cast<float>(y2 + (L.g[e] - ay) / L.dc[k]);
+        const double y = y2 + (L.g[e] - ay) / L.dc[k];
+        if (yout) yout[e] = static_cast<float>(y);
+        else L.y[e] = y;
     }
 }
 
@@ -1413,11 +1423,17 @@ int ml_plan(int H0, int W0, int* H, int* W, int* c, int* nlev) {
         set_error("multilevel: the coarsest level (%d x %d) exceeds %d unknowns", H[n], W[n], ML_MAX_DENSE);
         return ASD2_ERR_UNSUPPORTED;
     }
-    if (n >= 1 && H[1] * W[1] > ML_LOWER_MAX) {
-        set_error("multilevel: level 1 (%d x %d) exceeds %d cells", H[1], W[1], ML_LOWER_MAX);
-        return ASD2_ERR_UNSUPPORTED;
-    }
     return ASD2_OK;
+}
+
+// levels run by the whole chip (down/up kernels): the first ones down to fewer than
+// ML_WIDE cells; the one-workgroup kernel takes the rest.  Level 0 always runs wide when there
+// is a level below it: its output is the FP32 y of the fine iteration.
+constexpr int ML_WIDE = 2048;
+int ml_first_lower(const Hierarchy& Hh) {
+    int f = Hh.n > 1 ? 1 : 0;
+    while (f + 1 < Hh.n && Hh.L[f].H * Hh.L[f].W >= ML_WIDE) f++;
+    return f;
 }
 
 // Flexible PCG with a coarse correction: learned (the network's linear form) or multilevel
@@ -1465,7 +1481,7 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
     ASD2_TRY(ws(ctx, "lp_qc", Pc, &C.qc));
     // multilevel hierarchy: level 0 shares the first coarsening's arrays
     Hierarchy Hh{};
-    double* Minv = nullptr;
+    double *Minv = nullptr, *Mwork = nullptr;
     float* yml = nullptr;  // the V-cycle's output on level 0 (FP32, read by the SpMV)
     if (!learned) {
         int Hl[ML_MAX_LEVELS], Wl[ML_MAX_LEVELS], cl[ML_MAX_LEVELS];
@@ -1495,13 +1511,14 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
                 ASD2_TRY(ws(ctx, names[l][1], lc, &L.acy));
                 ASD2_TRY(ws(ctx, names[l][2], lc, &L.dc));
                 ASD2_TRY(ws(ctx, names[l][3], lp, &L.g));
-                ASD2_TRY(ws(ctx, names[l][5], lp, &L.r1));
             }
+            ASD2_TRY(ws(ctx, names[l][5], lp, &L.r1));
             ASD2_TRY(ws(ctx, names[l][4], lp, &L.y1));
             ASD2_TRY(ws(ctx, names[l][6], lp, &L.y));
         }
         const int m = Hh.L[Hh.n - 1].H * Hh.L[Hh.n - 1].W;
         ASD2_TRY(ws(ctx, "ml_dense", static_cast<int64_t>(m) * m, &Minv));
+        ASD2_TRY(ws(ctx, "ml_dense_work", static_cast<int64_t>(m) * m, &Mwork));
         ASD2_TRY(ws(ctx, "ml_yout", Pc, &yml));
     }
     double* pb = lp_part;                   // begin: r.D^-1 r, (D^-1 r).q_prev, sum (g / dc)^2
@@ -1534,8 +1551,13 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
             hipLaunchKernelGGL(k_ml_coarsen, dim3(flat_blocks(static_cast<int64_t>(Hh.L[l].H + 1) * (Hh.L[l].W + 1))),
                                dim3(KB), 0, ctx->stream, Hh.L[l - 1], Hh.L[l]);
         const Level& Lc = Hh.L[Hh.n - 1];
-        hipLaunchKernelGGL(k_ml_dense_build, dim3(1), dim3(KB), 0, ctx->stream, Lc, Minv);
-        hipLaunchKernelGGL(k_ml_invert, dim3(1), dim3(1024), 0, ctx->stream, Minv, Lc.H * Lc.W);
+        const int m = Lc.H * Lc.W;
+        hipLaunchKernelGGL(k_ml_dense_build, dim3(1), dim3(KB), 0, ctx->stream, Lc, Mwork);
+        for (int k = 0; k < m; k++)  // ends in Minv: m steps alternate Mwork -> Minv -> Mwork ...
+            hipLaunchKernelGGL(k_ml_gj_step, dim3(m), dim3(ML_MAX_DENSE), 0, ctx->stream, (k & 1) ? Minv : Mwork,
+                               (k & 1) ? Mwork : Minv, m, k);
+        if (m % 2 == 0)
+            ASD2_HIP(hipMemcpyAsync(Minv, Mwork, sizeof(double) * m * m, hipMemcpyDeviceToDevice, ctx->stream));
     }
     hipLaunchKernelGGL(k_fcg_init, dim3(nbf), dim3(KB), 0, ctx->stream, n, S.rhs, x, r, r64, x64, S.dinv32, p0, p1, q,
                        D.pu);
@@ -1544,7 +1566,7 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
     int it = 0, restart_it = 0, fr_it = -1, fallbacks = 0;
     bool use_net = true;  // the coarse correction is in use (false after a fallback to Jacobi)
     const float* ycoarse = learned ? static_cast<const float*>(N->y) : yml;
-    const int ntiles0 = ceil_div(C.Hc, ML_T) * ceil_div(C.Wc, ML_T);
+    const int first = learned ? 0 : ml_first_lower(Hh);
     auto batch = [&](int count) -> int {
         for (int k = 0; k < count; k++, it++) {
             prof_begin(ctx, ASD2_KID_FCG_BEGIN);
@@ -1559,10 +1581,16 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
                 } else if (Hh.n == 1) {
                     hipLaunchKernelGGL(k_ml_dense_only, dim3(1), dim3(KB), 0, ctx->stream, Hh.L[0], Minv, yml, D.st);
                 } else {
-                    hipLaunchKernelGGL(k_ml_down0, dim3(std::min(ntiles0, 4096)), dim3(KB), 0, ctx->stream, Hh.L[0],
-                                       Hh.L[1], D.st);
-                    hipLaunchKernelGGL(k_ml_lower, dim3(1), dim3(1024), 0, ctx->stream, Hh, Minv, D.st);
-                    hipLaunchKernelGGL(k_ml_up0, dim3(nbc), dim3(KB), 0, ctx->stream, Hh.L[0], Hh.L[1], yml, D.st);
+                    for (int l = 0; l < first; l++) {
+                        const int nt = ceil_div(Hh.L[l].H, ML_T) * ceil_div(Hh.L[l].W, ML_T);
+                        hipLaunchKernelGGL(k_ml_down, dim3(std::min(nt, 4096)), dim3(KB), 0, ctx->stream, Hh.L[l],
+                                           Hh.L[l + 1], D.st);
+                    }
+                    hipLaunchKernelGGL(k_ml_lower, dim3(1), dim3(1024), 0, ctx->stream, Hh, first, Minv, D.st);
+                    for (int l = first - 1; l >= 0; l--)
+                        hipLaunchKernelGGL(k_ml_up, dim3(flat_blocks(static_cast<int64_t>(Hh.L[l].H) * Hh.L[l].W)),
+                                           dim3(KB), 0, ctx->stream, Hh.L[l], Hh.L[l + 1],
+                                           l == 0 ? yml : static_cast<float*>(nullptr), D.st);
                 }
                 hipLaunchKernelGGL(k_lp_coarse_dots, dim3(nbc), dim3(KB), 0, ctx->stream, C, ycoarse, D.st, pc);
                 prof_end(ctx, ASD2_KID_COARSE);
