@@ -844,11 +844,42 @@ int solve_mixed(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2_solv
 // coarse data in FP64: a coarse diagonal is the sum of c^2 fine diagonals minus twice the
 // faces inside the block, a cancellation that loses about log2(c) bits per level, and the
 // smooth modes the coarse correction is for sit at the bottom of the spectrum
+constexpr int FCG_M = 4;              // directions kept by the truncated flexible recurrence
+constexpr int FCG_SLOTS = FCG_M + 1;  // ring of directions: iteration k in slot k % FCG_SLOTS
+
 struct Coarse {
     int c, Hc, Wc;
-    double *acx, *acy, *dc;  // coarse operator, (Hc + 1) x (Wc + 1)
-    double *g, *rc, *qc;     // P^T (r - A D^-1 r), P^T r, P^T q_prev, Hc x Wc
+    double *acx, *acy, *dc;   // coarse operator, (Hc + 1) x (Wc + 1)
+    double *g, *rc;           // P^T (r - A D^-1 r), P^T r, Hc x Wc
+    double* qc[FCG_M];        // P^T q of the last FCG_M iterations (j = 1: the latest)
 };
+
+// directions p_k and q_k = A p_k of the last FCG_SLOTS iterations
+struct Ring {
+    float* p[FCG_SLOTS];
+    float* q[FCG_SLOTS];
+};
+
+// the ring seen from iteration k (set on the host, so kernels index it with constants):
+// p[j], q[j] = direction j + 1 iterations back, pnew / qnew = the slots of iteration k
+struct Dirs {
+    const float* p[FCG_M];
+    const float* q[FCG_M];
+    float* pnew;
+    float* qnew;
+};
+
+inline Dirs dirs_for(const Ring& R, int k) {
+    Dirs d{};
+    for (int j = 0; j < FCG_M; j++) {
+        const int s = ((k - 1 - j) % FCG_SLOTS + FCG_SLOTS) % FCG_SLOTS;
+        d.p[j] = R.p[s];
+        d.q[j] = R.q[s];
+    }
+    d.pnew = R.p[k % FCG_SLOTS];
+    d.qnew = R.q[k % FCG_SLOTS];
+    return d;
+}
 
 // fine region of the begin kernel: whole c x c blocks, at most 32 x 64 cells
 __host__ __device__ inline int lp_ri(int c) { return c > 16 ? c : 16; }
@@ -856,9 +887,8 @@ constexpr int LP_RJ = 64;
 
 __global__ __launch_bounds__(KB) void k_fcg_init(int64_t n, const double* __restrict__ b, float* __restrict__ x,
                                                  float* __restrict__ r, double* __restrict__ r64,
-                                                 double* __restrict__ x64, const float* __restrict__ dinv,
-                                                 float* __restrict__ p0, float* __restrict__ p1,
-                                                 float* __restrict__ q, double* partials) {
+                                                 double* __restrict__ x64, const float* __restrict__ dinv, Ring R,
+                                                 double* partials) {
     double rr = 0.0, zz = 0.0, bb = 0.0;
     FLAT_LOOP(k, n) {
         const double bv = b[k];
@@ -868,9 +898,11 @@ __global__ __launch_bounds__(KB) void k_fcg_init(int64_t n, const double* __rest
         r[k] = rv;
         r64[k] = bv;
         x64[k] = 0.0;
-        p0[k] = 0.0f;
-        p1[k] = 0.0f;
-        q[k] = 0.0f;
+#pragma unroll
+        for (int s = 0; s < FCG_SLOTS; s++) {
+            R.p[s][k] = 0.0f;
+            R.q[s][k] = 0.0f;
+        }
         rr += static_cast<double>(rv) * rv;
         zz += static_cast<double>(zv) * zv;
         bb += bv * bv;
@@ -912,18 +944,227 @@ __device__ __forceinline__ double seg_sum(double v, int c) {
     return v;
 }
 
-// Start of iteration `it`: the decision (converged / pause for a reliable update / limit) from
-// the partials (r.r, .) of the kernel that produced r; then one pass over regions of whole
-// blocks: g, P^T r and P^T q_prev into the coarse arrays, partials (r.D^-1 r,
-// (D^-1 r).q_prev, sum (g / dc)^2).
-__global__ __launch_bounds__(KB) void k_lp_begin(Geom G, Coarse C, const float* __restrict__ r,
-                                                 const float* __restrict__ dinv, const float* __restrict__ q,
-                                                 const float* __restrict__ ax, const float* __restrict__ ay,
-                                                 const float* __restrict__ dg, KrylovState* st, const double* pu,
-                                                 int nbu, int it, double* pb) {
+// alpha = rho / p.q of iteration `it`, p.q recorded in its ring slot
+__device__ __forceinline__ bool fcg_alpha(KrylovState* st, const double* ps, int nbs, int it, double& alpha) {
+    double v[1];
+    sum_partials_all(ps, nbs, v);
+    alpha = st->rz[it & 1] / v[0];
+    const bool ok = isfinite(alpha);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->pq = v[0];
+        st->pqv[it % FCG_SLOTS] = v[0];
+        st->alpha = alpha;
+        if (!ok) {
+            st->done = DONE_BREAKDOWN;
+            st->iter = it;
+        }
+    }
+    return ok;
+}
+
+// The fine pass of the coarse correction for iteration `itn`, over regions of whole c x c
+// blocks: g = P^T (r - A D^-1 r), P^T r and P^T q_j of the last nwin directions into the
+// coarse arrays, partials (r.D^-1 r, (D^-1 r).q_1 .. (D^-1 r).q_FCG_M, sum (g / dc)^2).
+// UPD = false: r = r_in (the first iteration after a start or a reliable update).  UPD = true:
+// fused with the update of iteration itn - 1 (alpha = rho / p.q, x += alpha p, r_out = r_in -
+// alpha q, partials (r.r, 0) for the next decision), so the sums are those of the new residual.
+// r alternates between two buffers (neighbouring regions read the old r of their halo while
+// the owner writes the new one).
+template <bool UPD, int CC, int MW>
+__global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* __restrict__ r_in,
+                                                float* __restrict__ r_out, float* __restrict__ x, Dirs Dd,
+                                                const float* __restrict__ dinv, const float* __restrict__ ax,
+                                                const float* __restrict__ ay, const float* __restrict__ dg,
+                                                KrylovState* st, const double* ps, int nbs, double* pu, int itn,
+                                                int nwin, double* pb) {
+    if (st->done != DONE_RUNNING) return;
+    float alpha = 0.f;
+    const int it = itn - 1;  // the iteration updated when UPD
+    if (UPD) {
+        double a64;
+        if (!fcg_alpha(st, ps, nbs, it, a64)) return;
+        alpha = static_cast<float>(a64);
+    }
+    // Dd is the ring seen from itn: Dd.p[0], Dd.q[0] = the direction of the updated iteration
+    const float* __restrict__ pu_dir = Dd.p[0];
+    const float* __restrict__ qu_dir = Dd.q[0];
+    const float* qj[MW];
+#pragma unroll
+    for (int j = 0; j < MW; j++) qj[j] = Dd.q[j];
+    constexpr int c = CC, RI = (CC > 16 ? CC : 16), nbi = RI / c, nbj = LP_RJ / c, nblk = nbi * nbj;
+    constexpr int NS = 2 + MW;  // block sums: w, r, q_1 .. q_MW
+    __shared__ float sZ[RI + 2][LP_RJ + 2];
+    __shared__ float sR[RI][LP_RJ];
+    __shared__ double sB[NS][nblk];
+    const int reg_i = ceil_div(G.nx, RI), nreg = reg_i * ceil_div(G.ny, LP_RJ);
+    const int ilo = G.g, ihi = G.g + G.nx, jlo = G.g, jhi = G.g + G.ny;
+    const int lane = threadIdx.x & 63;
+    int t0, t1;
+    tile_range(nreg, t0, t1);
+    double a_rz = 0.0, a_uu = 0.0, a_rr = 0.0, a_zq[MW] = {};
+    for (int t = t0; t < t1; t++) {
+        const int rj = t / reg_i, ri = t - rj * reg_i;
+        const int i0 = ilo + ri * RI, j0 = jlo + rj * LP_RJ;
+        for (int e = threadIdx.x; e < (RI + 2) * (LP_RJ + 2); e += KB) {
+            const int a = e / (LP_RJ + 2), b = e - a * (LP_RJ + 2);
+            const int i = i0 - 1 + a, j = j0 - 1 + b;
+            float v = 0.0f;
+            if (i >= ilo && i < ihi && j >= jlo && j < jhi) {
+                const int64_t k = G.idx(i, j);
+                float rv = r_in[k];
+                if (UPD) rv = -alpha * qu_dir[k] + rv;
+                v = dinv[k] * rv;
+                if (a >= 1 && a <= RI && b >= 1 && b <= LP_RJ) sR[a - 1][b - 1] = rv;
+            }
+            sZ[a][b] = v;
+        }
+        for (int e = threadIdx.x; e < NS * nblk; e += KB) (&sB[0][0])[e] = 0.0;
+        __syncthreads();
+        for (int e = threadIdx.x; e < RI * LP_RJ; e += KB) {
+            const int a = e / LP_RJ, b = e - a * LP_RJ;  // a wave covers one region row
+            const int i = i0 + a, j = j0 + b;
+            const bool in = i < ihi && j < jhi;
+            double w = 0.0, rv = 0.0, qv[MW] = {};  // FP64 sums of FP32 data (the inputs are exact)
+            if (in) {
+                const int64_t k = G.idx(i, j);
+                const double zc = sZ[a + 1][b + 1];
+                const double az = dg[k] * zc + ax[k] * static_cast<double>(sZ[a][b + 1]) +
+                                  ax[k + G.sy] * static_cast<double>(sZ[a + 2][b + 1]) +
+                                  ay[k] * static_cast<double>(sZ[a + 1][b]) + ay[k + 1] * static_cast<double>(sZ[a + 1][b + 2]);
+                const float rf = sR[a][b];
+                if (UPD) {
+                    r_out[k] = rf;
+                    x[k] = alpha * pu_dir[k] + x[k];
+                    a_rr += static_cast<double>(rf) * rf;
+                }
+                rv = rf;
+                w = rv - az;
+                a_rz += zc * rv;
+#pragma unroll
+                for (int jj = 0; jj < MW; jj++)
+                    if (jj < nwin) {
+                        qv[jj] = qj[jj][k];
+                        a_zq[jj] += zc * qv[jj];
+                    }
+            }
+            w = seg_sum(w, c);
+            rv = seg_sum(rv, c);
+#pragma unroll
+            for (int jj = 0; jj < MW; jj++)
+                if (jj < nwin) qv[jj] = seg_sum(qv[jj], c);
+            if ((lane & (c - 1)) == 0 || c >= 64) {
+                const int blk = (a / c) * nbj + b / c;
+                atomicAdd(&sB[0][blk], w);
+                atomicAdd(&sB[1][blk], rv);
+#pragma unroll
+                for (int jj = 0; jj < MW; jj++)
+                    if (jj < nwin) atomicAdd(&sB[2 + jj][blk], qv[jj]);
+            }
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < nblk; e += KB) {
+            const int bi = e / nbj, bj = e - bi * nbj;
+            const int I = (i0 - ilo) / c + bi, J = (j0 - jlo) / c + bj;
+            if (I < C.Hc && J < C.Wc) {
+                const int64_t pc = static_cast<int64_t>(I) * C.Wc + J;
+                const double gv = sB[0][e];
+                C.g[pc] = gv;
+                C.rc[pc] = sB[1][e];
+#pragma unroll
+                for (int jj = 0; jj < MW; jj++)
+                    if (jj < nwin) C.qc[jj][pc] = sB[2 + jj][e];
+                const double u = gv / C.dc[I * (C.Wc + 1) + J];
+                a_uu += u * u;
+            }
+        }
+        __syncthreads();
+    }
+    double v[2 + FCG_M] = {};  // the partial rows are laid out for FCG_M directions
+    v[0] = a_rz;
+#pragma unroll
+    for (int jj = 0; jj < MW; jj++) v[1 + jj] = a_zq[jj];
+    v[1 + FCG_M] = a_uu;
+    put_partials(v, pb);
+    if (UPD) {
+        double w2[2] = {a_rr, 0.0};
+        put_partials(w2, pu);
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->iter = itn;
+    }
+}
+
+// linear-pass input channel 0 = g / dc / s, s = rms over the coarse cells of g / dc
+__global__ __launch_bounds__(KB) void k_lp_pack(Coarse C, uint16_t* __restrict__ flin, const KrylovState* st,
+                                                const double* pb, int nbb) {
+    if (st->done != DONE_RUNNING) return;
+    double v[1];
+    sum_partials_all(pb + (1 + FCG_M) * nbb, nbb, v);  // row 1 + FCG_M: sum (g / dc)^2
+    const int64_t Pc = static_cast<int64_t>(C.Hc) * C.Wc;
+    double s = sqrt(v[0] / static_cast<double>(Pc));
+    if (!(s > 0.0) || !isfinite(s)) s = 1.0;
+    const float is = static_cast<float>(1.0 / s);
+    FLAT_LOOP(p, Pc) {
+        const int I = static_cast<int>(p / C.Wc), J = static_cast<int>(p - static_cast<int64_t>(I) * C.Wc);
+        flin[p * 4] = f2bf(static_cast<float>(C.g[p] / C.dc[I * (C.Wc + 1) + J]) * is);
+    }
+}
+
+// partials (y.g, y.A_c y, y.P^T r, y.P^T q_1 .. y.P^T q_FCG_M)
+__global__ __launch_bounds__(KB) void k_lp_coarse_dots(Coarse C, const float* __restrict__ y, const KrylovState* st,
+                                                       int nwin, double* pc) {
+    if (st->done != DONE_RUNNING) return;
+    const int64_t Pc = static_cast<int64_t>(C.Hc) * C.Wc;
+    double a[3 + FCG_M] = {};
+    FLAT_LOOP(p, Pc) {
+        const int I = static_cast<int>(p / C.Wc), J = static_cast<int>(p - static_cast<int64_t>(I) * C.Wc);
+        const double yv = y[p];
+        a[0] += yv * C.g[p];
+        a[1] += yv * coarse_apply(C.acx, C.acy, C.dc, y, C.Hc, C.Wc, I, J);
+        a[2] += yv * C.rc[p];
+#pragma unroll
+        for (int j = 0; j < FCG_M; j++)
+            if (j < nwin) a[3 + j] += yv * C.qc[j][p];
+    }
+    put_partials(a, pc);
+}
+
+// launches k_lp_pass<UPD, c, MW> for the run-time coarsening c
+template <bool UPD, int MW>
+void launch_lp_pass(hipStream_t s, int nb, Geom G, Coarse C, const float* r_in, float* r_out, float* x, Dirs R,
+                    const float* dinv, const float* ax, const float* ay, const float* dg, KrylovState* st,
+                    const double* ps, int nbs, double* pu, int itn, int nwin, double* pb) {
+#define ASD2_LP_PASS(CV)                                                                                              \
+    hipLaunchKernelGGL((k_lp_pass<UPD, CV, MW>), dim3(nb), dim3(KB), 0, s, G, C, r_in, r_out, x, R, dinv, ax, ay, dg, \
+                       st, ps, nbs, pu, itn, nwin, pb)
+    switch (C.c) {
+        case 1: ASD2_LP_PASS(1); break;
+        case 2: ASD2_LP_PASS(2); break;
+        case 4: ASD2_LP_PASS(4); break;
+        case 8: ASD2_LP_PASS(8); break;
+        case 16: ASD2_LP_PASS(16); break;
+        default: ASD2_LP_PASS(32); break;
+    }
+#undef ASD2_LP_PASS
+}
+
+// The decision of iteration `it` (converged / pause for a reliable update / limit), then gamma,
+// rho = z.r, the coefficients of the last nwin directions, p = z - sum_j beta_j p_j on a tile
+// with halo (z = D^-1 r + gamma P y formed here), q = A p, partial p.q.  beta_j =
+// (z.q_j) / (p_j.q_j): z made A-orthogonal to the kept directions (Notay's truncated flexible CG;
+// with a preconditioner that is the same in every iteration and nwin >= 1 this is PCG), each
+// dot split into its fine Jacobi part and its coarse part.  gamma: the energy-optimal step along
+// P y, or gamma_fixed when positive (a y that already carries its scale).  rho <= 0 records
+// BREAKDOWN; the host then falls back to Jacobi.  y == nullptr: Jacobi (gamma = 0).
+template <int MW>
+__global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* __restrict__ ax,
+                                                const float* __restrict__ ay, const float* __restrict__ dg,
+                                                const float* __restrict__ r, const float* __restrict__ dinv,
+                                                const float* __restrict__ y, Dirs Dd, KrylovState* st,
+                                                const double* pb, int nbb, const double* pc, int nbc, double* ps,
+                                                int it, int nwin, double gamma_fixed, const double* pu, int nbu) {
     if (st->done != DONE_RUNNING) return;
     {
-        double v[2];
+        // the decision of iteration `it` from the partials (r.r, .) of the kernel that made r
+        double v[1];
         sum_partials_all(pu, nbu, v);
         const double rn = sqrt(v[0]), resid = rn / st->bnorm;
         int done = DONE_RUNNING;
@@ -936,150 +1177,35 @@ __global__ __launch_bounds__(KB) void k_lp_begin(Geom G, Coarse C, const float* 
         }
         if (done != DONE_RUNNING) return;
     }
-    __shared__ float sZ[32 + 2][LP_RJ + 2];
-    __shared__ double sB[3][1024];
-    const int c = C.c, RI = lp_ri(c), nbi = RI / c, nbj = LP_RJ / c, nblk = nbi * nbj;
-    const int reg_i = ceil_div(G.nx, RI), nreg = reg_i * ceil_div(G.ny, LP_RJ);
-    const int ilo = G.g, ihi = G.g + G.nx, jlo = G.g, jhi = G.g + G.ny;
-    const int lane = threadIdx.x & 63;
-    int t0, t1;
-    tile_range(nreg, t0, t1);
-    double a_rz = 0.0, a_zq = 0.0, a_uu = 0.0;
-    for (int t = t0; t < t1; t++) {
-        const int rj = t / reg_i, ri = t - rj * reg_i;
-        const int i0 = ilo + ri * RI, j0 = jlo + rj * LP_RJ;
-        for (int e = threadIdx.x; e < (RI + 2) * (LP_RJ + 2); e += KB) {
-            const int a = e / (LP_RJ + 2), b = e - a * (LP_RJ + 2);
-            const int i = i0 - 1 + a, j = j0 - 1 + b;
-            float v = 0.0f;
-            if (i >= ilo && i < ihi && j >= jlo && j < jhi) {
-                const int64_t k = G.idx(i, j);
-                v = dinv[k] * r[k];
-            }
-            sZ[a][b] = v;
-        }
-        for (int e = threadIdx.x; e < nblk; e += KB) sB[0][e] = sB[1][e] = sB[2][e] = 0.0;
-        __syncthreads();
-        for (int e = threadIdx.x; e < RI * LP_RJ; e += KB) {
-            const int a = e / LP_RJ, b = e - a * LP_RJ;  // a wave covers one region row
-            const int i = i0 + a, j = j0 + b;
-            const bool in = i < ihi && j < jhi;
-            double w = 0.0, rv = 0.0, qv = 0.0;  // FP64 sums of FP32 data (the inputs are exact)
-            if (in) {
-                const int64_t k = G.idx(i, j);
-                const double zc = sZ[a + 1][b + 1];
-                const double az = dg[k] * zc + ax[k] * static_cast<double>(sZ[a][b + 1]) +
-                                  ax[k + G.sy] * static_cast<double>(sZ[a + 2][b + 1]) +
-                                  ay[k] * static_cast<double>(sZ[a + 1][b]) + ay[k + 1] * static_cast<double>(sZ[a + 1][b + 2]);
-                rv = r[k];
-                qv = q[k];
-                w = rv - az;
-                a_rz += zc * rv;
-                a_zq += zc * qv;
-            }
-            w = seg_sum(w, c);
-            rv = seg_sum(rv, c);
-            qv = seg_sum(qv, c);
-            if ((lane & (c - 1)) == 0 || c >= 64) {
-                const int blk = (a / c) * nbj + b / c;
-                atomicAdd(&sB[0][blk], w);
-                atomicAdd(&sB[1][blk], rv);
-                atomicAdd(&sB[2][blk], qv);
-            }
-        }
-        __syncthreads();
-        for (int e = threadIdx.x; e < nblk; e += KB) {
-            const int bi = e / nbj, bj = e - bi * nbj;
-            const int I = (i0 - ilo) / c + bi, J = (j0 - jlo) / c + bj;
-            if (I < C.Hc && J < C.Wc) {
-                const int64_t pc = static_cast<int64_t>(I) * C.Wc + J;
-                const double gv = sB[0][e];
-                C.g[pc] = gv;
-                C.rc[pc] = sB[1][e];
-                C.qc[pc] = sB[2][e];
-                const double u = gv / C.dc[I * (C.Wc + 1) + J];
-                a_uu += u * u;
-            }
-        }
-        __syncthreads();
-    }
-    double v[3] = {a_rz, a_zq, a_uu};
-    put_partials(v, pb);
-}
-
-// linear-pass input channel 0 = g / dc / s, s = rms over the coarse cells of g / dc
-__global__ __launch_bounds__(KB) void k_lp_pack(Coarse C, uint16_t* __restrict__ flin, const KrylovState* st,
-                                                const double* pb, int nbb) {
-    if (st->done != DONE_RUNNING) return;
-    double v[3];
-    sum_partials_all(pb, nbb, v);
-    const int64_t Pc = static_cast<int64_t>(C.Hc) * C.Wc;
-    double s = sqrt(v[2] / static_cast<double>(Pc));
-    if (!(s > 0.0) || !isfinite(s)) s = 1.0;
-    const float is = static_cast<float>(1.0 / s);
-    FLAT_LOOP(p, Pc) {
-        const int I = static_cast<int>(p / C.Wc), J = static_cast<int>(p - static_cast<int64_t>(I) * C.Wc);
-        flin[p * 4] = f2bf(static_cast<float>(C.g[p] / C.dc[I * (C.Wc + 1) + J]) * is);
-    }
-}
-
-// partials (y.g, y.A_c y, y.P^T r, y.P^T q_prev)
-__global__ __launch_bounds__(KB) void k_lp_coarse_dots(Coarse C, const float* __restrict__ y, const KrylovState* st,
-                                                       double* pc) {
-    if (st->done != DONE_RUNNING) return;
-    const int64_t Pc = static_cast<int64_t>(C.Hc) * C.Wc;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    FLAT_LOOP(p, Pc) {
-        const int I = static_cast<int>(p / C.Wc), J = static_cast<int>(p - static_cast<int64_t>(I) * C.Wc);
-        const double yv = y[p];
-        a0 += yv * C.g[p];
-        a1 += yv * coarse_apply(C.acx, C.acy, C.dc, y, C.Hc, C.Wc, I, J);
-        a2 += yv * C.rc[p];
-        a3 += yv * C.qc[p];
-    }
-    double v[4] = {a0, a1, a2, a3};
-    put_partials(v, pc);
-}
-
-// gamma, rho = z.r, beta, then p = z + beta p_old on a tile with halo (z = D^-1 r + gamma P y
-// formed here), q = A p, partial p.q.  gamma: the energy-optimal step along P y, or gamma_fixed
-// when positive (a y that already carries its scale: a scalar that varies with r makes the
-// preconditioner nonlinear, which costs a linear y about half its benefit).  beta: 0 at it == 0 and at restart_it (the preconditioner
-// changed); rho / rho_prev at fr_it (the residual was replaced by a reliable update, so
-// r_k - r_{k-1} = -alpha q no longer holds; the direction is kept); otherwise the flexible form
-// -alpha_prev (z.q_prev) / rho_prev.  rho <= 0 records BREAKDOWN; the host then falls back to
-// Jacobi.  y == nullptr: Jacobi (gamma = 0).
-__global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* __restrict__ ax,
-                                                const float* __restrict__ ay, const float* __restrict__ dg,
-                                                const float* __restrict__ r, const float* __restrict__ dinv,
-                                                const float* __restrict__ y, float* p0, float* p1,
-                                                float* __restrict__ q, KrylovState* st, const double* pb, int nbb,
-                                                const double* pc, int nbc, double* ps, int it, int restart_it,
-                                                int fr_it, double gamma_fixed) {
-    if (st->done != DONE_RUNNING) return;
-    double vb[3];
+    // only the rows this iteration needs (every block re-sums them): r.D^-1 r and the MW
+    // (D^-1 r).q_j of the fine pass; y.g, y.A_c y, y.P^T r and the MW y.P^T q_j
+    double vb[1 + MW];
     sum_partials_all(pb, nbb, vb);
-    double gm = 0.0, yr = 0.0, yq = 0.0;
+    double vc[3 + MW] = {};
+    double gm = 0.0;
     if (y) {
-        double vc[4];
         sum_partials_all(pc, nbc, vc);
         const double t = vc[0] / vc[1];
         if (gamma_fixed > 0.0) gm = gamma_fixed;
         else if (vc[1] > 0.0 && isfinite(t)) gm = t;
-        yr = vc[2];
-        yq = vc[3];
     }
-    const double rho = vb[0] + gm * yr;
-    double beta64 = 0.0;
-    if (it > 0 && it != restart_it) {
-        const double rho_prev = st->rz[(it - 1) & 1];
-        beta64 = (it == fr_it) ? rho / rho_prev : -st->alpha * (vb[1] + gm * yq) / rho_prev;
+    const double rho = vb[0] + gm * vc[2];
+    float coef[MW];
+    bool ok = isfinite(rho) && rho > 0.0;
+#pragma unroll
+    for (int j = 0; j < MW; j++) {
+        double b = 0.0;
+        if (j < nwin) {
+            const double zq = vb[1 + j] + gm * vc[3 + j];
+            b = -zq / st->pqv[(it - 1 - j) % FCG_SLOTS];
+            ok = ok && isfinite(b);
+        }
+        coef[j] = static_cast<float>(b);
     }
-    const bool ok = isfinite(beta64) && isfinite(rho) && rho > 0.0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (ok) {
             st->rz[it & 1] = rho;
-            st->beta = beta64;
+            st->beta = coef[0];
             st->omega = gm;
         } else {
             st->done = DONE_BREAKDOWN;
@@ -1087,10 +1213,12 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
         }
     }
     if (!ok) return;
-    const float beta = static_cast<float>(beta64), gmf = static_cast<float>(gm);
-    const bool odd = (it & 1) != 0;
-    const float* __restrict__ pold = odd ? p1 : p0;
-    float* __restrict__ pnew = odd ? p0 : p1;
+    const float gmf = static_cast<float>(gm);
+    const float* pj[MW];
+#pragma unroll
+    for (int j = 0; j < MW; j++) pj[j] = Dd.p[j];
+    float* __restrict__ pnew = Dd.pnew;
+    float* __restrict__ q = Dd.qnew;
     __shared__ float sP[TI + 2][TJ + 2];
     const int ipad = G.nx + 2 * G.g, jpad = G.ny + 2 * G.g;
     const int ilo = G.g, ihi = G.g + G.nx, jlo = G.g, jhi = G.g + G.ny;
@@ -1106,14 +1234,13 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
             const int rr_ = e / (TJ + 2), cc = e - rr_ * (TJ + 2);
             const int i = i0 - 1 + rr_, j = j0 - 1 + cc;
             float val = 0.0f;
-            if (i < ipad && j < jpad) {
+            if (i < ipad && j < jpad && i >= ilo && i < ihi && j >= jlo && j < jhi) {
                 const int64_t k = G.idx(i, j);
-                float zv = 0.0f;
-                if (i >= ilo && i < ihi && j >= jlo && j < jhi) {
-                    zv = dinv[k] * r[k];
-                    if (y) zv += gmf * y[static_cast<int64_t>((i - ilo) / C.c) * C.Wc + (j - jlo) / C.c];
-                }
-                val = zv + beta * pold[k];
+                val = dinv[k] * r[k];
+                if (y) val += gmf * y[static_cast<int64_t>((i - ilo) / C.c) * C.Wc + (j - jlo) / C.c];
+#pragma unroll
+                for (int jj = 0; jj < MW; jj++)
+                    if (jj < nwin) val += coef[jj] * pj[jj][k];
             }
             sP[rr_][cc] = val;
         }
@@ -1133,31 +1260,6 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
     }
     double v[1] = {acc};
     put_partials(v, ps);
-}
-
-// alpha = rho / p.q, x += alpha p, r -= alpha q, partials (r.r, (D^-1 r)^2)
-__global__ __launch_bounds__(KB) void k_fcg_update(int64_t n, float* __restrict__ x, const float* p0, const float* p1,
-                                                   float* __restrict__ r, const float* __restrict__ q,
-                                                   const float* __restrict__ dinv, KrylovState* st, const double* ps,
-                                                   int nbs, double* pu, int it) {
-    if (st->done != DONE_RUNNING) return;
-    double alpha;
-    if (!cg_alpha(st, ps, nbs, it, alpha)) return;
-    const float a = static_cast<float>(alpha);
-    const float* __restrict__ p = (it & 1) ? p0 : p1;
-    double rr = 0.0, zz = 0.0;
-    FLAT_LOOP(k, n) {
-        const float xv = a * p[k] + x[k];
-        const float rv = -a * q[k] + r[k];
-        const float zv = dinv[k] * rv;
-        x[k] = xv;
-        r[k] = rv;
-        rr += static_cast<double>(rv) * rv;
-        zz += static_cast<double>(zv) * zv;
-    }
-    double v[2] = {rr, zz};
-    put_partials(v, pu);
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->iter = it + 1;
 }
 
 // r32 = r64 after a reliable update, partials (r.r, (D^-1 r)^2)
@@ -1462,23 +1564,32 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
     ASD2_TRY(system_arrays(ctx, &S, false));
     ASD2_TRY(ensure_system32(ctx, &S));
     double *x64, *r64, *lp_part;
-    float *x, *r, *p0, *p1, *q;
+    float *x, *rb[2];  // r alternates: iteration it reads rb[it & 1]
+    Ring R{};
     ASD2_TRY(ws(ctx, "cg_x", n, &x64));
     ASD2_TRY(ws(ctx, "cg_r", n, &r64));
     ASD2_TRY(ws(ctx, "mx_x", n, &x));
-    ASD2_TRY(ws(ctx, "mx_r", n, &r));
-    ASD2_TRY(ws(ctx, "mx_p0", n, &p0));
-    ASD2_TRY(ws(ctx, "mx_p1", n, &p1));
-    ASD2_TRY(ws(ctx, "mx_q", n, &q));
-    ASD2_TRY(ws(ctx, "lp_partials", 7 * MAX_BLOCKS, &lp_part));
-    Coarse C{c0, G.nx / c0, G.ny / c0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    ASD2_TRY(ws(ctx, "mx_r", n, &rb[0]));
+    ASD2_TRY(ws(ctx, "mx_r2", n, &rb[1]));
+    static const char* ring_names[2][FCG_SLOTS] = {{"fcg_p0", "fcg_p1", "fcg_p2", "fcg_p3", "fcg_p4"},
+                                                   {"fcg_q0", "fcg_q1", "fcg_q2", "fcg_q3", "fcg_q4"}};
+    for (int k = 0; k < FCG_SLOTS; k++) {
+        ASD2_TRY(ws(ctx, ring_names[0][k], n, &R.p[k]));
+        ASD2_TRY(ws(ctx, ring_names[1][k], n, &R.q[k]));
+    }
+    ASD2_TRY(ws(ctx, "lp_partials", (5 + 2 * FCG_M) * MAX_BLOCKS, &lp_part));
+    Coarse C{};
+    C.c = c0;
+    C.Hc = G.nx / c0;
+    C.Wc = G.ny / c0;
     const int64_t cs = static_cast<int64_t>(C.Hc + 1) * (C.Wc + 1), Pc = static_cast<int64_t>(C.Hc) * C.Wc;
     ASD2_TRY(ws(ctx, "lp_acx", cs, &C.acx));
     ASD2_TRY(ws(ctx, "lp_acy", cs, &C.acy));
     ASD2_TRY(ws(ctx, "lp_dc", cs, &C.dc));
     ASD2_TRY(ws(ctx, "lp_g", Pc, &C.g));
     ASD2_TRY(ws(ctx, "lp_rc", Pc, &C.rc));
-    ASD2_TRY(ws(ctx, "lp_qc", Pc, &C.qc));
+    static const char* qc_names[FCG_M] = {"lp_qc0", "lp_qc1", "lp_qc2", "lp_qc3"};
+    for (int j = 0; j < FCG_M; j++) ASD2_TRY(ws(ctx, qc_names[j], Pc, &C.qc[j]));
     // multilevel hierarchy: level 0 shares the first coarsening's arrays
     Hierarchy Hh{};
     double *Minv = nullptr, *Mwork = nullptr;
@@ -1521,8 +1632,8 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
         ASD2_TRY(ws(ctx, "ml_dense_work", static_cast<int64_t>(m) * m, &Mwork));
         ASD2_TRY(ws(ctx, "ml_yout", Pc, &yml));
     }
-    double* pb = lp_part;                   // begin: r.D^-1 r, (D^-1 r).q_prev, sum (g / dc)^2
-    double* pc = lp_part + 3 * MAX_BLOCKS;  // coarse dots: y.g, y.A_c y, y.P^T r, y.P^T q_prev
+    double* pb = lp_part;  // fine pass: r.D^-1 r, (D^-1 r).q_j, sum (g / dc)^2
+    double* pc = lp_part + (2 + FCG_M) * MAX_BLOCKS;  // coarse dots: y.g, y.A_c y, y.P^T r, y.P^T q_j
     const int nbs = spmv_blocks(G), nbf = flat_blocks(n), nbi = interior_blocks(G);
     const int RI = lp_ri(C.c);
     const int nbb = std::max(1, std::min<int>(MAX_BLOCKS, ceil_div(G.nx, RI) * ceil_div(G.ny, LP_RJ)));
@@ -1559,20 +1670,30 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
         if (m % 2 == 0)
             ASD2_HIP(hipMemcpyAsync(Minv, Mwork, sizeof(double) * m * m, hipMemcpyDeviceToDevice, ctx->stream));
     }
-    hipLaunchKernelGGL(k_fcg_init, dim3(nbf), dim3(KB), 0, ctx->stream, n, S.rhs, x, r, r64, x64, S.dinv32, p0, p1, q,
+    hipLaunchKernelGGL(k_fcg_init, dim3(nbf), dim3(KB), 0, ctx->stream, n, S.rhs, x, rb[0], r64, x64, S.dinv32, R,
                        D.pu);
     ASD2_LAUNCH_CHECK();
     ASD2_TRY(tail<T_FCG_INIT>(&D, nbf, D.pu));
-    int it = 0, restart_it = 0, fr_it = -1, fallbacks = 0;
+    int it = 0, fallbacks = 0;
     bool use_net = true;  // the coarse correction is in use (false after a fallback to Jacobi)
+    bool fresh = true;    // r was (re)set: the next iteration starts with a standalone fine pass
+    int nbu = nbf;        // blocks of the kernel whose (r.r) partials decide the next iteration
+    // directions kept: the network's coarse step varies with r (gamma), so the learned
+    // preconditioner keeps FCG_M; the V-cycle's is the same linear map in every iteration and
+    // one direction makes the recurrence PCG
+    const int mwin = learned ? FCG_M : 1;
+    int wstart = 0;  // first iteration of the current window of directions
+    auto nwin = [&](int k) { return std::max(0, std::min(mwin, k - wstart)); };
     const float* ycoarse = learned ? static_cast<const float*>(N->y) : yml;
     const int first = learned ? 0 : ml_first_lower(Hh);
     auto batch = [&](int count) -> int {
         for (int k = 0; k < count; k++, it++) {
-            prof_begin(ctx, ASD2_KID_FCG_BEGIN);
-            hipLaunchKernelGGL(k_lp_begin, dim3(nbb), dim3(KB), 0, ctx->stream, G, C, r, S.dinv32, q, S.ax32, S.ay32,
-                               S.diag32, D.st, D.pu, nbf, it, pb);
-            prof_end(ctx, ASD2_KID_FCG_BEGIN);
+            float* r = rb[it & 1];
+            if (fresh) {
+                (learned ? launch_lp_pass<false, FCG_M> : launch_lp_pass<false, 1>)(ctx->stream, nbb, G, C, r, nullptr, x, dirs_for(R, it), S.dinv32, S.ax32, S.ay32, S.diag32,
+                                      D.st, D.ps, nbs, nullptr, it, nwin(it), pb);
+                fresh = false;
+            }
             if (use_net) {
                 prof_begin(ctx, ASD2_KID_COARSE);
                 if (learned) {
@@ -1592,36 +1713,40 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
                                            dim3(KB), 0, ctx->stream, Hh.L[l], Hh.L[l + 1],
                                            l == 0 ? yml : static_cast<float*>(nullptr), D.st);
                 }
-                hipLaunchKernelGGL(k_lp_coarse_dots, dim3(nbc), dim3(KB), 0, ctx->stream, C, ycoarse, D.st, pc);
+                hipLaunchKernelGGL(k_lp_coarse_dots, dim3(nbc), dim3(KB), 0, ctx->stream, C, ycoarse, D.st, nwin(it),
+                                   pc);
                 prof_end(ctx, ASD2_KID_COARSE);
             }
             prof_begin(ctx, ASD2_KID_FCG_SPMV);
-            hipLaunchKernelGGL(k_lp_spmv, dim3(nbs), dim3(KB), 0, ctx->stream, G, C, S.ax32, S.ay32, S.diag32, r,
-                               S.dinv32, use_net ? ycoarse : nullptr, p0, p1, q, D.st, pb, nbb, pc, nbc, D.ps, it,
-                               restart_it, fr_it, learned ? 0.0 : 1.0);
+            hipLaunchKernelGGL(learned ? k_lp_spmv<FCG_M> : k_lp_spmv<1>, dim3(nbs), dim3(KB), 0, ctx->stream, G, C, S.ax32, S.ay32, S.diag32, r,
+                               S.dinv32, use_net ? ycoarse : nullptr, dirs_for(R, it), D.st, pb, nbb, pc, nbc, D.ps, it, nwin(it),
+                               learned ? 0.0 : 1.0, D.pu, nbu);
             prof_end(ctx, ASD2_KID_FCG_SPMV);
             prof_begin(ctx, ASD2_KID_FCG_UPDATE);
-            hipLaunchKernelGGL(k_fcg_update, dim3(nbf), dim3(KB), 0, ctx->stream, n, x, p0, p1, r, q, S.dinv32, D.st,
-                               D.ps, nbs, D.pu, it);
+            (learned ? launch_lp_pass<true, FCG_M> : launch_lp_pass<true, 1>)(ctx->stream, nbb, G, C, r, rb[(it + 1) & 1], x, dirs_for(R, it + 1), S.dinv32, S.ax32, S.ay32, S.diag32,
+                                 D.st, D.ps, nbs, D.pu, it + 1, nwin(it + 1), pb);
             prof_end(ctx, ASD2_KID_FCG_UPDATE);
+            nbu = nbb;
         }
         ASD2_LAUNCH_CHECK();
         return ASD2_OK;
     };
     // reliable update at the paused iteration k: fold x32 into x64, r64 = b - A x64, r32 = r64,
-    // and re-run iteration k keeping the direction (beta = rho_true / rho_prev); after a
-    // switch of preconditioner the recurrence restarts instead (beta = 0)
+    // and re-run iteration k keeping the directions (the A-orthogonalisation of z against them
+    // does not depend on the residual history); after a switch of preconditioner the window
+    // restarts instead
     auto reliable = [&](int k, bool restart) -> int {
         hipLaunchKernelGGL(k_mx_accum, dim3(nbf), dim3(KB), 0, ctx->stream, n, x64, x, D.st, 0);
         hipLaunchKernelGGL(k_true_residual, dim3(nbi), dim3(KB), 0, ctx->stream, G, S.ax, S.ay, S.diag, S.rhs, x64, r64,
                            D.st, D.ps, 0);
         ASD2_TRY(tail<T_TRUE_RES>(&D, nbi, D.ps));
-        hipLaunchKernelGGL(k_fcg_resume, dim3(nbf), dim3(KB), 0, ctx->stream, n, r64, r, S.dinv32, D.st, D.pu);
+        hipLaunchKernelGGL(k_fcg_resume, dim3(nbf), dim3(KB), 0, ctx->stream, n, r64, rb[k & 1], S.dinv32, D.st, D.pu);
         ASD2_TRY(tail<T_MX_RESUME>(&D, 0, D.pu));
         ASD2_LAUNCH_CHECK();
         it = k;
-        if (restart) restart_it = k;
-        else fr_it = k;
+        fresh = true;
+        nbu = nbf;
+        if (restart) wstart = k;
         return ASD2_OK;
     };
     const int check_every = o->check_every > 0 ? o->check_every : 4;
@@ -1664,7 +1789,7 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
             } else if (s.iter - it_best > stall_window) {
                 use_net = false;  // stalled: the remaining iterations use Jacobi, restarting the recurrence
                 fallbacks++;
-                restart_it = it;
+                wstart = it;
             }
         }
         ASD2_TRY(enqueue(slot));

@@ -29,11 +29,12 @@ struct KrylovState {
     double delta;                             // mixed: replacement factor
     double pq, aux0, aux1, aux2;              // scratch reductions
     double rz[2];                             // CG: r.z of iteration k in rz[k & 1]
+    double pqv[8];                            // flexible PCG: p.q of iteration k in pqv[k % slots]
     int iter, maxit, mode, done;              // done: 0 running, 1 converged, 2 maxit, 3 breakdown, 4 paused
     int updates, fallbacks, flags, pad;
 };
 
-// flexible PCG with the learned preconditioner (learned.hip)
+// flexible PCG with a coarse correction, learned or multilevel (krylov.hip)
 int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* opts, double* d_ps, asd2_solve_stats* stats, bool learned);
 
 constexpr int DONE_RUNNING = 0, DONE_CONVERGED = 1, DONE_MAXIT = 2, DONE_BREAKDOWN = 3, DONE_PAUSED = 4;

@@ -15,7 +15,7 @@ import time
 
 import numpy as np
 
-from . import (GRID_P, GRID_U, GRID_V, PREC_MIXED, Context, DeviceArray, UNet, make_fluid)
+from . import GRID_P, GRID_U, GRID_V, PREC_MIXED, Context, DeviceArray, UNet, make_fluid
 
 G = 3
 
@@ -80,35 +80,51 @@ class _View:
         self.ptr = ptr
 
 
-def make_samples(n, systems=6, snapshots=(0, 10, 40, 120), seed=0, rho_h=1000.0, rho_l=(1.0, 10.0),
-                 sigma=(0.3, 1.0), log=None):
-    """`systems` random droplet systems on an n x n grid, one sample per snapshot k (the
-    residual after k mixed-precision Jacobi-PCG iterations; k = 0: the right-hand side)."""
-    rng = np.random.default_rng(seed)
+def system_context(n):
+    """A context for n x n droplet systems (Neumann pressure walls, no-slip velocity walls)."""
     dx = 1.0 / n
-    cols = [[] for _ in range(5)]
     ctx = Context(n, n, dx, dx)
     ctx.set_bc(GRID_P, ("neumann",) * 4)
     ctx.set_bc(GRID_U, ("dirichlet",) * 4)
     ctx.set_bc(GRID_V, ("dirichlet",) * 4)
-    ls, kap, H, div = ctx.array(), ctx.array(), ctx.array(), ctx.array()
-    u, v, ps, y = ctx.array(), ctx.array(), ctx.array(), ctx.array()
+    return ctx
+
+
+def build_system(ctx, rng, rho_h=1000.0, rho_l=(1.0, 10.0), sigma=(0.3, 1.0)):
+    """Assembles the GFM pressure system of a random droplet field in ctx (the path of
+    MACSolver2D::SolvePoisson): circles, curvature, Heaviside, divergence of a smooth random
+    velocity, light-phase density log-uniform in rho_l."""
+    n = ctx.nx
+    dx = 1.0 / n
+    ls, kap, H, div, u, v = (ctx.array() for _ in range(6))
+    ls.upload(droplets(rng, n, dx))
+    ctx.bc_apply(GRID_P, ls)
+    ctx.kappa(ls, kap)
+    ctx.bc_apply(GRID_P, kap)
+    ctx.heaviside(ls, H)
+    uu, vv = smooth_velocity(rng, n, dx)
+    u.upload(uu)
+    v.upload(vv)
+    ctx.bc_apply(GRID_U, u)
+    ctx.bc_apply(GRID_V, v)
+    ctx.divergence(u, v, div)
+    rl = float(np.exp(rng.uniform(np.log(rho_l[0]), np.log(rho_l[1]))))
+    fl = make_fluid(rho_h, rl, 1.0, 0.01, float(rng.uniform(*sigma)), 0.3 * dx)
+    ctx.gfm_assemble(fl, ls, kap, H, div)
+    ctx.sync()
+    return fl
+
+
+def make_samples(n, systems=6, snapshots=(0, 10, 40, 120), seed=0, log=None, **kw):
+    """`systems` random droplet systems on an n x n grid, one sample per snapshot k (the
+    residual after k mixed-precision Jacobi-PCG iterations; k = 0: the right-hand side)."""
+    rng = np.random.default_rng(seed)
+    cols = [[] for _ in range(5)]
+    ctx = system_context(n)
+    ps, y = ctx.array(), ctx.array()
     t0 = time.perf_counter()
     for s in range(systems):
-        ls.upload(droplets(rng, n, dx))
-        ctx.bc_apply(GRID_P, ls)
-        ctx.kappa(ls, kap)
-        ctx.bc_apply(GRID_P, kap)
-        ctx.heaviside(ls, H)
-        uu, vv = smooth_velocity(rng, n, dx)
-        u.upload(uu)
-        v.upload(vv)
-        ctx.bc_apply(GRID_U, u)
-        ctx.bc_apply(GRID_V, v)
-        ctx.divergence(u, v, div)
-        rl = float(np.exp(rng.uniform(np.log(rho_l[0]), np.log(rho_l[1]))))
-        fl = make_fluid(rho_h, rl, 1.0, 0.01, float(rng.uniform(*sigma)), 0.3 * dx)
-        ctx.gfm_assemble(fl, ls, kap, H, div)
+        build_system(ctx, rng, **kw)
         ax, ay, dg, rhs = ctx.gfm_export()
         dinv = np.zeros_like(dg)
         inner = (slice(G, G + n), slice(G, G + n))
