@@ -42,6 +42,14 @@ BYTES_PER_CELL = {
     "gfm": 7 * 8,            # phi, kappa, div -> beta_x, beta_y, rhs, dinv
     "coarse": 0,             # coarse correction (V-cycle or network): no per-cell HBM figure
 }
+# HBM bytes per interior cell and launch measured with rocprofv3 on `--pmc-workload` (FETCH_SIZE x
+# 2.0 + WRITE_SIZE x 1.0: the factors that make both counters read 256 MiB each on the plain copy
+# asd2_stream_copy of 2^25 doubles); the roofline's `traffic` is this figure x cells.
+MEASURED_BYTES_PER_CELL = {
+    "pcg64_spmv": 61.5, "pcg64_update": 64.4, "pcg32_spmv": 31.2, "pcg32_update": 32.2,
+    "fcg_spmv": 36.3, "fcg_update": 47.4, "weno_stage": (80.9 + 83.3 + 82.2) / 3,
+    "reinit": (29.6 + 39.6) / 2, "gfm": 100.6,
+}
 
 
 def cpu_info():
@@ -306,9 +314,11 @@ def main():
     roofline = None
     if dom:
         kd = kernels[dom]
+        mb = MEASURED_BYTES_PER_CELL.get(dom)
         roofline = {"kernel": dom, "bound": "hbm", "achieved": kd["achieved_GBs"], "peak": PEAK_HBM_GBS,
-                    "unit": "GB/s", "frac": kd["achieved_GBs"] / PEAK_HBM_GBS, "traffic": None,
-                    "avg_us": kd["avg_us"], "bytes_per_cell": kd["bytes_per_cell"]}
+                    "unit": "GB/s", "frac": kd["achieved_GBs"] / PEAK_HBM_GBS,
+                    "traffic": mb * args.n * args.n if mb else None,
+                    "traffic_bytes_per_cell": mb, "avg_us": kd["avg_us"], "bytes_per_cell": kd["bytes_per_cell"]}
     fp64 = paths["fp64"]
 
     cpu = None if args.no_cpu else cpu_baseline(args.n, fp64["pcg_iters_per_step"])
