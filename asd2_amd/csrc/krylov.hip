@@ -1154,7 +1154,7 @@ void launch_lp_pass(hipStream_t s, int nb, Geom G, Coarse C, const float* r_in, 
 // dot split into its fine Jacobi part and its coarse part.  gamma: the energy-optimal step along
 // P y, or gamma_fixed when positive (a y that already carries its scale).  rho <= 0 records
 // BREAKDOWN; the host then falls back to Jacobi.  y == nullptr: Jacobi (gamma = 0).
-template <int MW>
+template <int MW, int CC>
 __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* __restrict__ ax,
                                                 const float* __restrict__ ay, const float* __restrict__ dg,
                                                 const float* __restrict__ r, const float* __restrict__ dinv,
@@ -1237,7 +1237,7 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
             if (i < ipad && j < jpad && i >= ilo && i < ihi && j >= jlo && j < jhi) {
                 const int64_t k = G.idx(i, j);
                 val = dinv[k] * r[k];
-                if (y) val += gmf * y[static_cast<int64_t>((i - ilo) / C.c) * C.Wc + (j - jlo) / C.c];
+                if (y) val += gmf * y[static_cast<int64_t>((i - ilo) / CC) * C.Wc + (j - jlo) / CC];
 #pragma unroll
                 for (int jj = 0; jj < MW; jj++)
                     if (jj < nwin) val += coef[jj] * pj[jj][k];
@@ -1260,6 +1260,26 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
     }
     double v[1] = {acc};
     put_partials(v, ps);
+}
+
+// launches k_lp_spmv<MW, c> for the run-time coarsening c
+template <int MW>
+void launch_lp_spmv(hipStream_t s, int nb, Geom G, Coarse C, const float* ax, const float* ay, const float* dg,
+                    const float* r, const float* dinv, const float* y, Dirs Dd, KrylovState* st, const double* pb,
+                    int nbb, const double* pc, int nbc, double* ps, int it, int nwin, double gamma_fixed,
+                    const double* pu, int nbu) {
+#define ASD2_LP_SPMV(CV)                                                                                              \
+    hipLaunchKernelGGL((k_lp_spmv<MW, CV>), dim3(nb), dim3(KB), 0, s, G, C, ax, ay, dg, r, dinv, y, Dd, st, pb, nbb, \
+                       pc, nbc, ps, it, nwin, gamma_fixed, pu, nbu)
+    switch (C.c) {
+        case 1: ASD2_LP_SPMV(1); break;
+        case 2: ASD2_LP_SPMV(2); break;
+        case 4: ASD2_LP_SPMV(4); break;
+        case 8: ASD2_LP_SPMV(8); break;
+        case 16: ASD2_LP_SPMV(16); break;
+        default: ASD2_LP_SPMV(32); break;
+    }
+#undef ASD2_LP_SPMV
 }
 
 // r32 = r64 after a reliable update, partials (r.r, (D^-1 r)^2)
@@ -1478,19 +1498,50 @@ __device__ __forceinline__ double ml_y2(const Level& L, const Level& next, int i
 }
 
 // prolongation and post-smoothing on a level run by the whole chip: y = y2 + (g - A y2) / dc,
-// into L.y, or into yout (FP32) on level 0
-__global__ __launch_bounds__(KB) void k_ml_up(Level L, Level next, float* __restrict__ yout, const KrylovState* st) {
+// into L.y, or into yout (FP32) on level 0, where the block partials (0, 0, y.P^T r, y.P^T q_1)
+// of the fine iteration's scalars go to pc (the step along P y is fixed at 1, so y.g and
+// y.A_c y are not needed)
+__global__ __launch_bounds__(KB) void k_ml_up(Level L, Level next, float* __restrict__ yout, const double* rc,
+                                              const double* qc, const KrylovState* st, double* pc) {
     if (st->done != DONE_RUNNING) return;
     const int64_t P = static_cast<int64_t>(L.H) * L.W;
     const int sc = L.W + 1;
+    double a_r = 0.0, a_q = 0.0;
     FLAT_LOOP(e, P) {
         const int i = static_cast<int>(e / L.W), j = static_cast<int>(e - static_cast<int64_t>(i) * L.W), k = i * sc + j;
         const double y2 = ml_y2(L, next, i, j);
         const double ay = L.dc[k] * y2 + L.acx[k] * ml_y2(L, next, i - 1, j) + L.acx[k + sc] * ml_y2(L, next, i + 1, j) +
                           L.acy[k] * ml_y2(L, next, i, j - 1) + L.acy[k + 1] * ml_y2(L, next, i, j + 1);
         const double y = y2 + (L.g[e] - ay) / L.dc[k];
-        if (yout) yout[e] = static_cast<float>(y);
-        else L.y[e] = y;
+        if (yout) {
+            const float yf = static_cast<float>(y);
+            yout[e] = yf;
+            a_r += static_cast<double>(yf) * rc[e];
+            a_q += static_cast<double>(yf) * qc[e];
+        } else {
+            L.y[e] = y;
+        }
+    }
+    if (pc) {
+        double v[4] = {0.0, 0.0, a_r, a_q};
+        put_partials(v, pc);
+    }
+}
+
+// y = A^-1 g on the coarsest level (FP64 inverse): 16 rows per block, 4 per wave, the lanes
+// splitting each row
+__global__ __launch_bounds__(KB) void k_ml_dense_apply(Level L, const double* __restrict__ Minv,
+                                                       const KrylovState* st) {
+    if (st->done != DONE_RUNNING) return;
+    const int m = L.H * L.W, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int rr = 0; rr < 4; rr++) {
+        const int p = blockIdx.x * 16 + w * 4 + rr;
+        if (p >= m) break;
+        const double* row = Minv + static_cast<int64_t>(p) * m;
+        double s = 0.0;
+        for (int q = lane; q < m; q += 64) s += row[q] * L.g[q];
+        s = wave_sum(s);
+        if (lane == 0) L.y[p] = s;
     }
 }
 
@@ -1707,20 +1758,26 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
                         hipLaunchKernelGGL(k_ml_down, dim3(std::min(nt, 4096)), dim3(KB), 0, ctx->stream, Hh.L[l],
                                            Hh.L[l + 1], D.st);
                     }
-                    hipLaunchKernelGGL(k_ml_lower, dim3(1), dim3(1024), 0, ctx->stream, Hh, first, Minv, D.st);
+                    if (first == Hh.n - 1)
+                        hipLaunchKernelGGL(k_ml_dense_apply, dim3(ceil_div(Hh.L[first].H * Hh.L[first].W, 16)),
+                                           dim3(KB), 0, ctx->stream, Hh.L[first], Minv, D.st);
+                    else
+                        hipLaunchKernelGGL(k_ml_lower, dim3(1), dim3(1024), 0, ctx->stream, Hh, first, Minv, D.st);
                     for (int l = first - 1; l >= 0; l--)
-                        hipLaunchKernelGGL(k_ml_up, dim3(flat_blocks(static_cast<int64_t>(Hh.L[l].H) * Hh.L[l].W)),
+                        hipLaunchKernelGGL(k_ml_up, dim3(l == 0 ? nbc : flat_blocks(static_cast<int64_t>(Hh.L[l].H) * Hh.L[l].W)),
                                            dim3(KB), 0, ctx->stream, Hh.L[l], Hh.L[l + 1],
-                                           l == 0 ? yml : static_cast<float*>(nullptr), D.st);
+                                           l == 0 ? yml : static_cast<float*>(nullptr), C.rc, C.qc[0], D.st,
+                                           l == 0 ? pc : static_cast<double*>(nullptr));
                 }
-                hipLaunchKernelGGL(k_lp_coarse_dots, dim3(nbc), dim3(KB), 0, ctx->stream, C, ycoarse, D.st, nwin(it),
-                                   pc);
+                if (learned || Hh.n == 1)
+                    hipLaunchKernelGGL(k_lp_coarse_dots, dim3(nbc), dim3(KB), 0, ctx->stream, C, ycoarse, D.st,
+                                       nwin(it), pc);
                 prof_end(ctx, ASD2_KID_COARSE);
             }
             prof_begin(ctx, ASD2_KID_FCG_SPMV);
-            hipLaunchKernelGGL(learned ? k_lp_spmv<FCG_M> : k_lp_spmv<1>, dim3(nbs), dim3(KB), 0, ctx->stream, G, C, S.ax32, S.ay32, S.diag32, r,
-                               S.dinv32, use_net ? ycoarse : nullptr, dirs_for(R, it), D.st, pb, nbb, pc, nbc, D.ps, it, nwin(it),
-                               learned ? 0.0 : 1.0, D.pu, nbu);
+            (learned ? launch_lp_spmv<FCG_M> : launch_lp_spmv<1>)(
+                ctx->stream, nbs, G, C, S.ax32, S.ay32, S.diag32, r, S.dinv32, use_net ? ycoarse : nullptr,
+                dirs_for(R, it), D.st, pb, nbb, pc, nbc, D.ps, it, nwin(it), learned ? 0.0 : 1.0, D.pu, nbu);
             prof_end(ctx, ASD2_KID_FCG_SPMV);
             prof_begin(ctx, ASD2_KID_FCG_UPDATE);
             (learned ? launch_lp_pass<true, FCG_M> : launch_lp_pass<true, 1>)(ctx->stream, nbb, G, C, r, rb[(it + 1) & 1], x, dirs_for(R, it + 1), S.dinv32, S.ax32, S.ay32, S.diag32,
