@@ -55,17 +55,42 @@ __global__ __launch_bounds__(WTH) void k_weno(Geom G, GhostRule R, const double*
     const int iend = G.nx + g, jend = G.ny + g;
     const int ipad = G.nx + 2 * g, jpad = G.ny + 2 * g;
 
-    for (int r = ti; r < WI + 6; r += WTH / WJ) {
-        const int i = i0 - 3 + r, j = j0 + tj;
+    // Each thread owns the output rows o = ti + 4k of its column and loads their phi and cell-centred
+    // velocities once: the products go to the flux tiles, the values stay in registers for the
+    // stage update (a second read after the tile phase would miss L2 at these tile counts).
+    constexpr int NR = WI / (WTH / WJ);
+    double phiR[NR], ucR[NR], vcR[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        const int o = ti + (WTH / WJ) * k, i = i0 + o, j = j0 + tj;
+        double ph = 0.0, uc = 0.0, vc = 0.0;
+        if (i < ipad && j < jpad) {
+            ph = phi[G.idx(i, j)];
+            uc = CENTRED ? u[G.idx(i, j)] : uc_face(G, R, u, i, j);
+            vc = CENTRED ? v[G.idx(i, j)] : vc_face(G, R, v, i, j);
+        }
+        sFx[o + 3][tj] = (j < jend && i < ipad) ? ph * uc : 0.0;
+        sFy[o][tj + 3] = (i < iend && j < jpad) ? ph * vc : 0.0;
+        phiR[k] = ph;
+        ucR[k] = uc;
+        vcR[k] = vc;
+    }
+    // the three halo rows above and below (x fluxes) and the three halo columns on each side
+    // (y fluxes)
+    for (int e = threadIdx.x; e < 6 * WJ; e += WTH) {
+        const int h = e / WJ, c = e - h * WJ;
+        const int r = h < 3 ? h : WI + h;
+        const int i = i0 - 3 + r, j = j0 + c;
         double f = 0.0;
         if (j < jend && i < ipad) {
             const double vel = CENTRED ? u[G.idx(i, j)] : uc_face(G, R, u, i, j);
             f = phi[G.idx(i, j)] * vel;
         }
-        sFx[r][tj] = f;
+        sFx[r][c] = f;
     }
-    for (int e = threadIdx.x; e < WI * (WJ + 6); e += WTH) {
-        const int r = e / (WJ + 6), c = e - r * (WJ + 6);
+    for (int e = threadIdx.x; e < WI * 6; e += WTH) {
+        const int r = e / 6, h = e - r * 6;
+        const int c = h < 3 ? h : WJ + h;
         const int i = i0 + r, j = j0 - 3 + c;
         double f = 0.0;
         if (i < iend && j < jpad) {
@@ -78,7 +103,9 @@ __global__ __launch_bounds__(WTH) void k_weno(Geom G, GhostRule R, const double*
 
     const double idx_ = 1.0 / G.dx, idy_ = 1.0 / G.dy;
     const double epsx = 1e-6 * G.dx * G.dx, epsy = 1e-6 * G.dy * G.dy;
-    for (int r = ti; r < WI; r += WTH / WJ) {
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        const int r = ti + (WTH / WJ) * k;
         const int i = i0 + r, j = j0 + tj;
         if (i >= iend || j >= jend) continue;
         double lxp, lxm, lyp, lym;
@@ -93,18 +120,13 @@ __global__ __launch_bounds__(WTH) void k_weno(Geom G, GhostRule R, const double*
             weno5(f1 - f0, f2 - f1, f3 - f2, f4 - f3, f5 - f4, f6 - f5, idy_, epsy, lyp, lym);
         }
         const int64_t c = G.idx(i, j);
-        double ax, ay;
-        if (CENTRED) { ax = u[c]; ay = v[c]; }
-        else {
-            ax = (u[c] + u[G.idx(i + 1, j)]) * 0.5;
-            ay = (v[c] + v[G.idx(i, j + 1)]) * 0.5;
-        }
+        const double ax = ucR[k], ay = vcR[k], phc = phiR[k];
         const double L = 0.5 * (lxp + lxm) + 0.5 * (lyp + lym) - ax * (0.5 * (lxp - lxm)) - ay * (0.5 * (lyp - lym));
         double val;
         if (STAGE == 0) val = L;
-        else if (STAGE == 1) val = phi[c] - dt * L;
-        else if (STAGE == 2) val = 0.75 * phin[c] + 0.25 * phi[c] - 0.25 * dt * L;
-        else val = 1.0 / 3.0 * phin[c] + 2.0 / 3.0 * phi[c] - 2.0 / 3.0 * dt * L;
+        else if (STAGE == 1) val = phc - dt * L;
+        else if (STAGE == 2) val = 0.75 * phin[c] + 0.25 * phc - 0.25 * dt * L;
+        else val = 1.0 / 3.0 * phin[c] + 2.0 / 3.0 * phc - 2.0 / 3.0 * dt * L;
         out[c] = val;
         if (ghosts) write_p_ghosts(G, R, out, i, j, val);
     }
