@@ -99,6 +99,43 @@ __device__ __forceinline__ void sum_partials_all(const double* partials, int nb,
     for (int k = 0; k < NV; k++) v[k] = bc[k];
 }
 
+// sum_partials_all of three partial arrays at once (NA values of na partials from pa, then NB
+// of nb from pb, then NC of nc from pc; pc == nullptr gives zeros): the loads of all rows are in
+// flight together and one block reduction serves them all.  Per value the order is that of
+// sum_partials_all.
+template <int NA, int NB, int NC>
+__device__ __forceinline__ void sum_partials3(const double* pa, int na, const double* pb, int nb, const double* pc,
+                                              int nc, double (&v)[NA + NB + NC]) {
+    __shared__ double bc[NA + NB + NC];
+#pragma unroll
+    for (int k = 0; k < NA; k++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < na; b += KB) s += pa[k * na + b];
+        v[k] = s;
+    }
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nb; b += KB) s += pb[k * nb + b];
+        v[NA + k] = s;
+    }
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        double s = 0.0;
+        if (pc)
+            for (int b = threadIdx.x; b < nc; b += KB) s += pc[k * nc + b];
+        v[NA + NB + k] = s;
+    }
+    block_sum(v);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NA + NB + NC; k++) bc[k] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NA + NB + NC; k++) v[k] = bc[k];
+}
+
 // logical block index: consecutive logical indices run on the same XCD
 __device__ __forceinline__ int logical_block() {
     const int nb = gridDim.x, b = blockIdx.x;
@@ -1162,10 +1199,12 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
                                                 const double* pb, int nbb, const double* pc, int nbc, double* ps,
                                                 int it, int nwin, double gamma_fixed, const double* pu, int nbu) {
     if (st->done != DONE_RUNNING) return;
+    // every block re-sums the rows this iteration needs: r.r of the kernel that made r; r.D^-1 r
+    // and the MW (D^-1 r).q_j of the fine pass; y.g, y.A_c y, y.P^T r and the MW y.P^T q_j
+    double v[1 + (1 + MW) + (3 + MW)];
+    sum_partials3<1, 1 + MW, 3 + MW>(pu, nbu, pb, nbb, y ? pc : nullptr, nbc, v);
     {
-        // the decision of iteration `it` from the partials (r.r, .) of the kernel that made r
-        double v[1];
-        sum_partials_all(pu, nbu, v);
+        // the decision of iteration `it`
         const double rn = sqrt(v[0]), resid = rn / st->bnorm;
         int done = DONE_RUNNING;
         if (!isfinite(resid)) done = DONE_BREAKDOWN;
@@ -1177,14 +1216,10 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
         }
         if (done != DONE_RUNNING) return;
     }
-    // only the rows this iteration needs (every block re-sums them): r.D^-1 r and the MW
-    // (D^-1 r).q_j of the fine pass; y.g, y.A_c y, y.P^T r and the MW y.P^T q_j
-    double vb[1 + MW];
-    sum_partials_all(pb, nbb, vb);
-    double vc[3 + MW] = {};
+    const double* vb = v + 1;
+    const double* vc = v + 2 + MW;
     double gm = 0.0;
     if (y) {
-        sum_partials_all(pc, nbc, vc);
         const double t = vc[0] / vc[1];
         if (gamma_fixed > 0.0) gm = gamma_fixed;
         else if (vc[1] > 0.0 && isfinite(t)) gm = t;
@@ -1258,8 +1293,8 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
         }
         __syncthreads();
     }
-    double v[1] = {acc};
-    put_partials(v, ps);
+    double pq[1] = {acc};
+    put_partials(pq, ps);
 }
 
 // launches k_lp_spmv<MW, c> for the run-time coarsening c
@@ -1315,7 +1350,9 @@ __global__ void k_set_done(KrylovState* st, int from, int to) {
 
 constexpr int ML_MAX_LEVELS = 8;
 constexpr int ML_MAX_DENSE = 256;   // unknowns of the coarsest level
-constexpr int ML_T = 16;            // level-0 tile of the restriction kernel (ML_T x ML_T cells)
+constexpr int ML_T = 16;            // tile of the chip-wide level kernels (ML_T x ML_T cells)
+constexpr int ML_SW = 2;            // Jacobi sweeps before and after the coarse correction
+constexpr int ML_HW = ML_T + 2 * ML_SW;  // tile with the sweeps' halo
 
 struct Level {
     int H, W, c;              // c: factor to the next level (0 on the coarsest)
@@ -1383,30 +1420,65 @@ __device__ __forceinline__ double ml_apply(const Level& L, const double* v, int 
            L.acy[k] * ml_at(L, v, i, j - 1) + L.acy[k + 1] * ml_at(L, v, i, j + 1);
 }
 
-// pre-smoothing and restriction on a level run by the whole chip: y1 = g / dc, next.g = block
-// sums of g - A y1.  A block owns an ML_T x ML_T tile (whole blocks of the next level: c divides
-// ML_T).
+// Jacobi sweep y + (g - A y) / dc at level cell (i, j) from a tile of y with a halo (y[a][b]
+// at the cell's tile position; 0 outside the level)
+template <int HW>
+__device__ __forceinline__ double ml_sweep(const Level& L, const double (*y)[HW], int a, int b, int i, int j) {
+    const int sc = L.W + 1, k = i * sc + j;
+    const double ay = L.dc[k] * y[a][b] + L.acx[k] * y[a - 1][b] + L.acx[k + sc] * y[a + 1][b] +
+                      L.acy[k] * y[a][b - 1] + L.acy[k + 1] * y[a][b + 1];
+    return y[a][b] + (L.g[static_cast<int64_t>(i) * L.W + j] - ay) / L.dc[k];
+}
+
+// Sweeps 2 .. ML_SW of a tile whose state after the first is in buf[0] on the region of halo
+// ML_SW; each sweep shrinks the valid region by one cell, so the result (buf[(ML_SW - 1) & 1])
+// is valid on halo 1.  Cells outside the level stay 0.
+__device__ __forceinline__ void ml_sweeps(const Level& L, double (*buf)[ML_HW][ML_HW], int i0, int j0) {
+#pragma unroll
+    for (int s = 1; s < ML_SW; s++) {
+        const double (*src)[ML_HW] = buf[(s - 1) & 1];
+        double (*dst)[ML_HW] = buf[s & 1];
+        const int lo = s, n = ML_HW - 2 * s;  // region of halo ML_SW - s
+        for (int e = threadIdx.x; e < n * n; e += KB) {
+            const int a = lo + e / n, b = lo + e % n;
+            const int i = i0 - ML_SW + a, j = j0 - ML_SW + b;
+            dst[a][b] = (i >= 0 && i < L.H && j >= 0 && j < L.W) ? ml_sweep<ML_HW>(L, src, a, b, i, j) : 0.0;
+        }
+        __syncthreads();
+    }
+}
+
+// pre-smoothing and restriction on a level run by the whole chip: y1 = ML_SW Jacobi sweeps
+// from 0, next.g = block sums of g - A y1.  A block owns an ML_T x ML_T tile (whole blocks of
+// the next level: c divides ML_T) and sweeps it with a halo of ML_SW cells.
 __global__ __launch_bounds__(KB) void k_ml_down(Level L, Level next, const KrylovState* st) {
     if (st->done != DONE_RUNNING) return;
+    __shared__ double sY[2][ML_HW][ML_HW];
     __shared__ double sR[ML_T][ML_T];
     const int tiles_j = ceil_div(L.W, ML_T), ntiles = ceil_div(L.H, ML_T) * tiles_j;
-    const int c = L.c, nb = ML_T / c;
+    const int c = L.c, nb = ML_T / c, sc = L.W + 1;
+    const int a = threadIdx.x / ML_T, b = threadIdx.x - a * ML_T;  // KB = ML_T * ML_T
+    constexpr int F = (ML_SW - 1) & 1;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int ti = t / tiles_j, tj = t - ti * tiles_j;
-        const int a = threadIdx.x / ML_T, b = threadIdx.x - a * ML_T;  // KB = ML_T * ML_T
-        const int i = ti * ML_T + a, j = tj * ML_T + b;
+        const int i0 = ti * ML_T, j0 = tj * ML_T;
+        for (int e = threadIdx.x; e < ML_HW * ML_HW; e += KB) {  // sweep 1 from y = 0: g / dc
+            const int aa = e / ML_HW, bb = e - aa * ML_HW;
+            const int i = i0 - ML_SW + aa, j = j0 - ML_SW + bb;
+            sY[0][aa][bb] = (i >= 0 && i < L.H && j >= 0 && j < L.W)
+                                ? L.g[static_cast<int64_t>(i) * L.W + j] / L.dc[i * sc + j]
+                                : 0.0;
+        }
+        __syncthreads();
+        ml_sweeps(L, sY, i0, j0);
+        const int i = i0 + a, j = j0 + b;
         double rv = 0.0;
         if (i < L.H && j < L.W) {
-            auto jac = [&](int ii, int jj) {
-                return (ii < 0 || ii >= L.H || jj < 0 || jj >= L.W)
-                           ? 0.0
-                           : L.g[static_cast<int64_t>(ii) * L.W + jj] / L.dc[ii * (L.W + 1) + jj];
-            };
-            const int sc = L.W + 1, k = i * sc + j;
-            const double y1 = jac(i, j);
-            const double ay = L.dc[k] * y1 + L.acx[k] * jac(i - 1, j) + L.acx[k + sc] * jac(i + 1, j) +
-                              L.acy[k] * jac(i, j - 1) + L.acy[k + 1] * jac(i, j + 1);
-            L.y1[static_cast<int64_t>(i) * L.W + j] = y1;
+            const int k = i * sc + j, A = a + ML_SW, B = b + ML_SW;
+            const double (*y)[ML_HW] = sY[F];
+            const double ay = L.dc[k] * y[A][B] + L.acx[k] * y[A - 1][B] + L.acx[k + sc] * y[A + 1][B] +
+                              L.acy[k] * y[A][B - 1] + L.acy[k + 1] * y[A][B + 1];
+            L.y1[static_cast<int64_t>(i) * L.W + j] = y[A][B];
             rv = L.g[static_cast<int64_t>(i) * L.W + j] - ay;
         }
         sR[a][b] = rv;
@@ -1415,10 +1487,10 @@ __global__ __launch_bounds__(KB) void k_ml_down(Level L, Level next, const Krylo
             const int bi = threadIdx.x / nb, bj = threadIdx.x - bi * nb;
             const int I = ti * nb + bi, J = tj * nb + bj;
             if (I < next.H && J < next.W) {
-                double s = 0.0;
+                double sum = 0.0;
                 for (int u = 0; u < c; u++)
-                    for (int v = 0; v < c; v++) s += sR[bi * c + u][bj * c + v];
-                next.g[static_cast<int64_t>(I) * next.W + J] = s;
+                    for (int v = 0; v < c; v++) sum += sR[bi * c + u][bj * c + v];
+                next.g[static_cast<int64_t>(I) * next.W + J] = sum;
             }
         }
         __syncthreads();
@@ -1458,6 +1530,15 @@ __global__ __launch_bounds__(1024) void k_ml_lower(Hierarchy Hh, int first, cons
             L.y1[p] = L.g[p] / L.dc[i * (L.W + 1) + j];
         }
         __syncthreads();
+        for (int s = 1; s < ML_SW; s++) {  // further sweeps through r1
+            for (int p = threadIdx.x; p < P; p += 1024) {
+                const int i = p / L.W, j = p - i * L.W;
+                L.r1[p] = L.y1[p] + (L.g[p] - ml_apply(L, L.y1, i, j)) / L.dc[i * (L.W + 1) + j];
+            }
+            __syncthreads();
+            for (int p = threadIdx.x; p < P; p += 1024) L.y1[p] = L.r1[p];
+            __syncthreads();
+        }
         for (int p = threadIdx.x; p < P; p += 1024) {
             const int i = p / L.W, j = p - i * L.W;
             L.r1[p] = L.g[p] - ml_apply(L, L.y1, i, j);
@@ -1483,44 +1564,60 @@ __global__ __launch_bounds__(1024) void k_ml_lower(Hierarchy Hh, int first, cons
             L.r1[p] = L.y1[p] + N.y[static_cast<int64_t>(i / L.c) * N.W + j / L.c];
         }
         __syncthreads();
-        for (int p = threadIdx.x; p < P; p += 1024) {
-            const int i = p / L.W, j = p - i * L.W;
-            L.y[p] = L.r1[p] + (L.g[p] - ml_apply(L, L.r1, i, j)) / L.dc[i * (L.W + 1) + j];
+        for (int s = 0; s < ML_SW; s++) {
+            for (int p = threadIdx.x; p < P; p += 1024) {
+                const int i = p / L.W, j = p - i * L.W;
+                L.y[p] = L.r1[p] + (L.g[p] - ml_apply(L, L.r1, i, j)) / L.dc[i * (L.W + 1) + j];
+            }
+            __syncthreads();
+            if (s + 1 < ML_SW) {
+                for (int p = threadIdx.x; p < P; p += 1024) L.r1[p] = L.y[p];
+                __syncthreads();
+            }
         }
-        __syncthreads();
     }
 }
 
-// y2 = y1 + P y_next of cell (i, j), 0 outside the level grid
-__device__ __forceinline__ double ml_y2(const Level& L, const Level& next, int i, int j) {
-    if (i < 0 || i >= L.H || j < 0 || j >= L.W) return 0.0;
-    return L.y1[static_cast<int64_t>(i) * L.W + j] + next.y[static_cast<int64_t>(i / L.c) * next.W + j / L.c];
-}
-
-// prolongation and post-smoothing on a level run by the whole chip: y = y2 + (g - A y2) / dc,
-// into L.y, or into yout (FP32) on level 0, where the block partials (0, 0, y.P^T r, y.P^T q_1)
-// of the fine iteration's scalars go to pc (the step along P y is fixed at 1, so y.g and
-// y.A_c y are not needed)
+// prolongation and post-smoothing on a level run by the whole chip: y = ML_SW Jacobi sweeps
+// from y2 = y1 + P y_next (on ML_T x ML_T tiles with the sweeps' halo), into L.y, or into yout
+// (FP32) on level 0, where the block partials (0, 0, y.P^T r, y.P^T q_1) of the fine
+// iteration's scalars go to pc (the step along P y is fixed at 1, so y.g and y.A_c y are not
+// needed)
 __global__ __launch_bounds__(KB) void k_ml_up(Level L, Level next, float* __restrict__ yout, const double* rc,
                                               const double* qc, const KrylovState* st, double* pc) {
     if (st->done != DONE_RUNNING) return;
-    const int64_t P = static_cast<int64_t>(L.H) * L.W;
-    const int sc = L.W + 1;
+    __shared__ double sY[2][ML_HW][ML_HW];
+    const int tiles_j = ceil_div(L.W, ML_T), ntiles = ceil_div(L.H, ML_T) * tiles_j;
+    const int a = threadIdx.x / ML_T, b = threadIdx.x - a * ML_T;  // KB = ML_T * ML_T
+    constexpr int F = (ML_SW - 1) & 1;
     double a_r = 0.0, a_q = 0.0;
-    FLAT_LOOP(e, P) {
-        const int i = static_cast<int>(e / L.W), j = static_cast<int>(e - static_cast<int64_t>(i) * L.W), k = i * sc + j;
-        const double y2 = ml_y2(L, next, i, j);
-        const double ay = L.dc[k] * y2 + L.acx[k] * ml_y2(L, next, i - 1, j) + L.acx[k + sc] * ml_y2(L, next, i + 1, j) +
-                          L.acy[k] * ml_y2(L, next, i, j - 1) + L.acy[k + 1] * ml_y2(L, next, i, j + 1);
-        const double y = y2 + (L.g[e] - ay) / L.dc[k];
-        if (yout) {
-            const float yf = static_cast<float>(y);
-            yout[e] = yf;
-            a_r += static_cast<double>(yf) * rc[e];
-            a_q += static_cast<double>(yf) * qc[e];
-        } else {
-            L.y[e] = y;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int ti = t / tiles_j, tj = t - ti * tiles_j;
+        const int i0 = ti * ML_T, j0 = tj * ML_T;
+        for (int e = threadIdx.x; e < ML_HW * ML_HW; e += KB) {  // y2 = y1 + P y_next
+            const int aa = e / ML_HW, bb = e - aa * ML_HW;
+            const int i = i0 - ML_SW + aa, j = j0 - ML_SW + bb;
+            sY[0][aa][bb] = (i >= 0 && i < L.H && j >= 0 && j < L.W)
+                                ? L.y1[static_cast<int64_t>(i) * L.W + j] +
+                                      next.y[static_cast<int64_t>(i / L.c) * next.W + j / L.c]
+                                : 0.0;
         }
+        __syncthreads();
+        ml_sweeps(L, sY, i0, j0);
+        const int i = i0 + a, j = j0 + b;
+        if (i < L.H && j < L.W) {
+            const int64_t e = static_cast<int64_t>(i) * L.W + j;
+            const double y = ml_sweep<ML_HW>(L, sY[F], a + ML_SW, b + ML_SW, i, j);
+            if (yout) {
+                const float yf = static_cast<float>(y);
+                yout[e] = yf;
+                a_r += static_cast<double>(yf) * rc[e];
+                a_q += static_cast<double>(yf) * qc[e];
+            } else {
+                L.y[e] = y;
+            }
+        }
+        __syncthreads();
     }
     if (pc) {
         double v[4] = {0.0, 0.0, a_r, a_q};

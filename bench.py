@@ -39,7 +39,8 @@ BYTES_PER_CELL = {
     "fcg_update": 10 * 4,    # x, p, r, q, dinv, beta_x, beta_y, diag -> x, r' (+ coarse sums of the new r)
     "weno_stage": 112 / 3,   # 32 + 40 + 40 bytes over the 3 stages of an advection step
     "reinit": 28,            # MinRK2 substage A 24 B, substage B 32 B
-    "gfm": 7 * 8,            # phi, kappa, div -> beta_x, beta_y, rhs, dinv
+    "gfm": 4 * 8 + 5 * 8 + 4 * 4,  # phi, kappa, H, div -> beta_x, beta_y, diag, rhs, dinv (FP64) and
+                                   # the FP32 beta_x, beta_y, diag, dinv of the mixed solver
     "coarse": 0,             # coarse correction (V-cycle or network): no per-cell HBM figure
 }
 # HBM bytes per interior cell and launch measured with rocprofv3 on `--pmc-workload` (FETCH_SIZE x
@@ -47,7 +48,7 @@ BYTES_PER_CELL = {
 # asd2_stream_copy of 2^25 doubles); the roofline's `traffic` is this figure x cells.
 MEASURED_BYTES_PER_CELL = {
     "pcg64_spmv": 61.5, "pcg64_update": 64.4, "pcg32_spmv": 31.2, "pcg32_update": 32.2,
-    "fcg_spmv": 36.3, "fcg_update": 47.4, "weno_stage": (80.9 + 83.3 + 82.2) / 3,
+    "fcg_spmv": 36.3, "fcg_update": 47.4, "weno_stage": (44.7 + 54.9 + 53.7) / 3,
     "reinit": (29.6 + 39.6) / 2, "gfm": 100.6,
 }
 
