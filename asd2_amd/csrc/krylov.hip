@@ -1351,7 +1351,7 @@ __global__ void k_set_done(KrylovState* st, int from, int to) {
 constexpr int ML_MAX_LEVELS = 8;
 constexpr int ML_MAX_DENSE = 256;   // unknowns of the coarsest level
 constexpr int ML_T = 16;            // tile of the chip-wide level kernels (ML_T x ML_T cells)
-constexpr int ML_SW = 2;            // Jacobi sweeps before and after the coarse correction
+constexpr int ML_SW = 3;            // Jacobi sweeps before and after the coarse correction
 constexpr int ML_HW = ML_T + 2 * ML_SW;  // tile with the sweeps' halo
 
 struct Level {
@@ -1391,23 +1391,52 @@ __global__ __launch_bounds__(KB) void k_ml_dense_build(Level L, double* __restri
     }
 }
 
-// One Gauss-Jordan step of the inverse of an SPD m x m matrix (no pivoting needed), pivot k:
-// Mout from Min (two buffers, so no block reads a row another block rewrites); block = row i,
-// thread = column j.  After steps k = 0 .. m-1 the last output holds the inverse.
-__global__ __launch_bounds__(ML_MAX_DENSE) void k_ml_gj_step(const double* __restrict__ Min, double* __restrict__ Mout,
-                                                             int m, int k) {
+// ML_GJ_B consecutive Gauss-Jordan steps (pivots k0 .. k0 + ML_GJ_B - 1, no pivoting: the
+// matrix is SPD) of the inverse of an m x m matrix, Mout from Min (two buffers, so no block
+// reads a row another block rewrites); block = row i, thread = column j.  Every block replays
+// the steps on the launch's pivot rows in registers, so after ceil(m / ML_GJ_B) launches the
+// last output holds the inverse.  Per element each step computes what a one-step-per-launch
+// sweep would: pivot row r_k / p, other rows r - f r_k / p, column k = 1 / p and -f / p.
+constexpr int ML_GJ_B = 16;
+__global__ __launch_bounds__(ML_MAX_DENSE) void k_ml_gj_block(const double* __restrict__ Min,
+                                                              double* __restrict__ Mout, int m, int k0) {
+    __shared__ double col[ML_GJ_B + 1];  // column k of the pivot rows and of the own row
     const int i = blockIdx.x, j = threadIdx.x;
-    if (j >= m) return;
-    const double piv = 1.0 / Min[static_cast<int64_t>(k) * m + k];
-    const double rkj = Min[static_cast<int64_t>(k) * m + j];
-    double v;
-    if (i == k) {
-        v = (j == k) ? piv : rkj * piv;
-    } else {
-        const double f = Min[static_cast<int64_t>(i) * m + k];
-        v = (j == k) ? -f * piv : Min[static_cast<int64_t>(i) * m + j] - f * rkj * piv;
+    const bool act = j < m;
+    const int nb = min(ML_GJ_B, m - k0);
+    double P[ML_GJ_B];
+#pragma unroll
+    for (int t = 0; t < ML_GJ_B; t++)
+        P[t] = (act && t < nb) ? Min[static_cast<int64_t>(k0 + t) * m + j] : 0.0;
+    double x = act ? Min[static_cast<int64_t>(i) * m + j] : 0.0;
+#pragma unroll
+    for (int t = 0; t < ML_GJ_B; t++) {
+        if (t >= nb) break;
+        const int k = k0 + t;
+        if (j == k) {
+#pragma unroll
+            for (int u = 0; u < ML_GJ_B; u++) col[u] = P[u];
+            col[ML_GJ_B] = x;
+        }
+        __syncthreads();
+        const double piv = 1.0 / col[t];
+        const double rkj = P[t];
+#pragma unroll
+        for (int u = 0; u < ML_GJ_B; u++) {
+            if (u == t) continue;
+            const double f = col[u];
+            P[u] = (j == k) ? -f * piv : P[u] - f * rkj * piv;
+        }
+        P[t] = (j == k) ? piv : rkj * piv;
+        const double f = col[ML_GJ_B];
+        x = (j == k) ? -f * piv : x - f * rkj * piv;
+        __syncthreads();
     }
-    Mout[static_cast<int64_t>(i) * m + j] = v;
+    const int own = i - k0;  // the own row is one of this launch's pivot rows
+#pragma unroll
+    for (int t = 0; t < ML_GJ_B; t++)
+        if (t == own) x = P[t];
+    if (act) Mout[static_cast<int64_t>(i) * m + j] = x;
 }
 
 // v of level cell (i, j), 0 outside the level grid
@@ -1811,12 +1840,14 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
                                dim3(KB), 0, ctx->stream, Hh.L[l - 1], Hh.L[l]);
         const Level& Lc = Hh.L[Hh.n - 1];
         const int m = Lc.H * Lc.W;
-        hipLaunchKernelGGL(k_ml_dense_build, dim3(1), dim3(KB), 0, ctx->stream, Lc, Mwork);
-        for (int k = 0; k < m; k++)  // ends in Minv: m steps alternate Mwork -> Minv -> Mwork ...
-            hipLaunchKernelGGL(k_ml_gj_step, dim3(m), dim3(ML_MAX_DENSE), 0, ctx->stream, (k & 1) ? Minv : Mwork,
-                               (k & 1) ? Mwork : Minv, m, k);
-        if (m % 2 == 0)
-            ASD2_HIP(hipMemcpyAsync(Minv, Mwork, sizeof(double) * m * m, hipMemcpyDeviceToDevice, ctx->stream));
+        // nl launches alternate A -> B -> A ...; the matrix is built where the last one ends in Minv
+        const int nl = ceil_div(m, ML_GJ_B);
+        double* A0 = (nl & 1) ? Mwork : Minv;
+        double* B0 = (nl & 1) ? Minv : Mwork;
+        hipLaunchKernelGGL(k_ml_dense_build, dim3(1), dim3(KB), 0, ctx->stream, Lc, A0);
+        for (int q = 0; q < nl; q++)
+            hipLaunchKernelGGL(k_ml_gj_block, dim3(m), dim3(ML_MAX_DENSE), 0, ctx->stream, (q & 1) ? B0 : A0,
+                               (q & 1) ? A0 : B0, m, q * ML_GJ_B);
     }
     hipLaunchKernelGGL(k_fcg_init, dim3(nbf), dim3(KB), 0, ctx->stream, n, S.rhs, x, rb[0], r64, x64, S.dinv32, R,
                        D.pu);
