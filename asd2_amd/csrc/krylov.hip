@@ -668,14 +668,15 @@ int driver_init(asd2_ctx* ctx, Driver* D) {
     return ASD2_OK;
 }
 
-int set_state(Driver* D, const asd2_solve_opts* o) {
+// delta0: the reliable-update factor when the options leave it at 0
+int set_state(Driver* D, const asd2_solve_opts* o, double delta0 = 0.1) {
     ASD2_HIP(hipStreamSynchronize(D->ctx->stream));  // the pinned slot may still be a copy target
     KrylovState& s = D->host[0];
     s = KrylovState{};
     s.tol = o->tol;
     s.maxit = o->max_iter > 0 ? o->max_iter : 1000000;
     s.mode = o->mode;
-    s.delta = o->reliable_delta > 0 ? o->reliable_delta : 0.1;
+    s.delta = o->reliable_delta > 0 ? o->reliable_delta : delta0;
     ASD2_HIP(hipMemcpyAsync(D->st, &s, sizeof(s), hipMemcpyHostToDevice, D->ctx->stream));
     return ASD2_OK;
 }
@@ -1819,7 +1820,9 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
     oo.mode = ASD2_CG_STANDARD;
     Driver D;
     ASD2_TRY(driver_init(ctx, &D));
-    ASD2_TRY(set_state(&D, &oo));
+    // a few hundred iterations at most: the FP32 recurrence stays close to the true residual over
+    // two decades, and every update costs a host round trip, so the default factor is 0.01
+    ASD2_TRY(set_state(&D, &oo, 0.01));
     hipStream_t saved = learned ? N->stream : nullptr;
     if (learned) N->stream = ctx->stream;
     struct Restore {

@@ -115,15 +115,41 @@ __global__ __launch_bounds__(256) void k_update_vel(Geom G, ProjArgs A, double* 
 }
 
 // ------------------------------------------------------------------ CFL
-// max over the interior of |u_{i+1} u_i / 2| and |v_{j+1} v_j / 2| (src/mac2d.cpp:1858-1862)
+// max over the interior of |u_{i+1} u_i / 2| and |v_{j+1} v_j / 2| (src/mac2d.cpp:1858-1862):
+// block maxima into partials, then one block reduces them (max is order-free, so the result
+// does not depend on the split)
 __global__ __launch_bounds__(256) void k_velmax(Geom G, const double* __restrict__ u, const double* __restrict__ v,
-                                                double* __restrict__ partials, unsigned* ticket, double* out) {
+                                                double* __restrict__ partials) {
     double um = 0.0, vm = 0.0;
-    const int64_t cells = static_cast<int64_t>(G.nx) * G.ny;
-    for (int64_t t = blockIdx.x * 256ll + threadIdx.x; t < cells; t += static_cast<int64_t>(gridDim.x) * 256) {
-        const int i = G.g + static_cast<int>(t / G.ny), j = G.g + static_cast<int>(t % G.ny);
-        um = smax(um, fabs((u[G.idx(i + 1, j)] * u[G.idx(i, j)]) * 0.5));
-        vm = smax(vm, fabs((v[G.idx(i, j + 1)] * v[G.idx(i, j)]) * 0.5));
+    for (int i = G.g + blockIdx.y; i < G.nx + G.g; i += gridDim.y)
+        for (int j = G.g + blockIdx.x * 256 + threadIdx.x; j < G.ny + G.g; j += gridDim.x * 256) {
+            um = smax(um, fabs((u[G.idx(i + 1, j)] * u[G.idx(i, j)]) * 0.5));
+            vm = smax(vm, fabs((v[G.idx(i, j + 1)] * v[G.idx(i, j)]) * 0.5));
+        }
+    __shared__ double su[256], sv[256];
+    su[threadIdx.x] = um;
+    sv[threadIdx.x] = vm;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            su[threadIdx.x] = smax(su[threadIdx.x], su[threadIdx.x + s]);
+            sv[threadIdx.x] = smax(sv[threadIdx.x], sv[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int b = blockIdx.y * gridDim.x + blockIdx.x, nb = gridDim.x * gridDim.y;
+        partials[b] = su[0];
+        partials[nb + b] = sv[0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_velmax_final(const double* __restrict__ partials, int nb,
+                                                      double* __restrict__ out) {
+    double um = 0.0, vm = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) {
+        um = smax(um, partials[b]);
+        vm = smax(vm, partials[nb + b]);
     }
     __shared__ double su[256], sv[256];
     su[threadIdx.x] = um;
@@ -136,25 +162,9 @@ __global__ __launch_bounds__(256) void k_velmax(Geom G, const double* __restrict
         }
         __syncthreads();
     }
-    __shared__ int last;
     if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = su[0];
-        partials[2 * blockIdx.x + 1] = sv[0];
-        __threadfence();
-        last = (atomicAdd(ticket, 1u) == gridDim.x - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    if (threadIdx.x == 0) {
-        double a = 0.0, b = 0.0;
-        for (unsigned k = 0; k < gridDim.x; k++) {
-            a = smax(a, partials[2 * k]);
-            b = smax(b, partials[2 * k + 1]);
-        }
-        out[0] = a;
-        out[1] = b;
-        *ticket = 0u;
+        out[0] = su[0];
+        out[1] = sv[0];
     }
 }
 
@@ -214,28 +224,43 @@ __device__ __forceinline__ double rho_arith(double rH, double rL, double a, doub
 }
 
 // -cU + vU + gU at a U face and the RK combination (src/mac2d.cpp:254-279, 308-387, 570-809)
+// Both momentum kernels run on MJ x MI face tiles (j fastest, 4 rows per thread); the WENO
+// fluxes of the convection term are formed once per face into LDS tiles with a 3-face halo.
+constexpr int MJ = 64, MI = 16;
+
 __global__ __launch_bounds__(256) void k_mom_u(Geom G, MomArgs A, const double* __restrict__ u,
                                                const double* __restrict__ v, const double* __restrict__ ls,
                                                const double* __restrict__ Hs, const double* __restrict__ u0,
                                                double* __restrict__ out) {
-    const int j = G.g + blockIdx.x * 64 + (threadIdx.x & 63);
-    const int i = G.g + 1 + blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (i >= G.nx + G.g || j >= G.ny + G.g) return;
+    __shared__ double sFx[MI + 6][MJ];  // u^2 along i
+    __shared__ double sFy[MI][MJ + 6];  // u vbar along j
+    const int tj = threadIdx.x & (MJ - 1), ti = threadIdx.x / MJ;
+    const int j0 = G.g + blockIdx.x * MJ, i0 = G.g + 1 + blockIdx.y * MI;
+    const int ipad = G.nx + 2 * G.g, jpad = G.ny + 2 * G.g;
+    for (int r = ti; r < MI + 6; r += 256 / MJ) {
+        const int i = i0 - 3 + r, j = j0 + tj;
+        double f = 0.0;
+        if (i < ipad && j < jpad) { const double w = u[G.idx(i, j)]; f = w * w; }
+        sFx[r][tj] = f;
+    }
+    for (int e = threadIdx.x; e < MI * (MJ + 6); e += 256) {
+        const int r = e / (MJ + 6), cc = e - r * (MJ + 6);
+        const int i = i0 + r, j = j0 - 3 + cc;
+        sFy[r][cc] = (i < ipad && j < jpad) ? u[G.idx(i, j)] * vbar_u(G, v, i, j) : 0.0;
+    }
+    __syncthreads();
+    for (int r = ti; r < MI; r += 256 / MJ) {
+    const int i = i0 + r, j = j0 + tj;
+    if (i >= G.nx + G.g || j >= G.ny + G.g) continue;
     const int64_t c = G.idx(i, j);
     // convection: WENO of u^2 along i and of u * vbar along j
     double lxp, lxm, lyp, lym;
-    {
-        double f[7];
-        for (int k = 0; k < 7; k++) { const double w = u[G.idx(i - 3 + k, j)]; f[k] = w * w; }
-        weno5(f[1] - f[0], f[2] - f[1], f[3] - f[2], f[4] - f[3], f[5] - f[4], f[6] - f[5], 1.0 / G.dx,
-              1e-6 * G.dx * G.dx, lxp, lxm);
-    }
-    {
-        double f[7];
-        for (int k = 0; k < 7; k++) f[k] = u[G.idx(i, j - 3 + k)] * vbar_u(G, v, i, j - 3 + k);
-        weno5(f[1] - f[0], f[2] - f[1], f[3] - f[2], f[4] - f[3], f[5] - f[4], f[6] - f[5], 1.0 / G.dy,
-              1e-6 * G.dy * G.dy, lyp, lym);
-    }
+    weno5(sFx[r + 1][tj] - sFx[r][tj], sFx[r + 2][tj] - sFx[r + 1][tj], sFx[r + 3][tj] - sFx[r + 2][tj],
+          sFx[r + 4][tj] - sFx[r + 3][tj], sFx[r + 5][tj] - sFx[r + 4][tj], sFx[r + 6][tj] - sFx[r + 5][tj],
+          1.0 / G.dx, 1e-6 * G.dx * G.dx, lxp, lxm);
+    weno5(sFy[r][tj + 1] - sFy[r][tj], sFy[r][tj + 2] - sFy[r][tj + 1], sFy[r][tj + 3] - sFy[r][tj + 2],
+          sFy[r][tj + 4] - sFy[r][tj + 3], sFy[r][tj + 5] - sFy[r][tj + 4], sFy[r][tj + 6] - sFy[r][tj + 5],
+          1.0 / G.dy, 1e-6 * G.dy * G.dy, lyp, lym);
     const double aX = u[c], aY = vbar_u(G, v, i, j);
     const double cU = 0.5 * (lxp + lxm) + 0.5 * (lyp + lym) - aX * (0.5 * (lxp - lxm)) - aY * (0.5 * (lyp - lym));
     double dU = 0.0;
@@ -273,6 +298,7 @@ __global__ __launch_bounds__(256) void k_mom_u(Geom G, MomArgs A, const double* 
     else if (A.mode == 5) o = cU;
     else o = dU;
     out[c] = o;
+    }
 }
 
 // -cV + vV + gV at a V face and the RK combination (src/mac2d.cpp:281-306, 389-463, 811-1041)
@@ -280,23 +306,34 @@ __global__ __launch_bounds__(256) void k_mom_v(Geom G, MomArgs A, const double* 
                                                const double* __restrict__ v, const double* __restrict__ ls,
                                                const double* __restrict__ Hs, const double* __restrict__ v0,
                                                double* __restrict__ out) {
-    const int j = G.g + 1 + blockIdx.x * 64 + (threadIdx.x & 63);
-    const int i = G.g + blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (i >= G.nx + G.g || j >= G.ny + G.g) return;
+    __shared__ double sFx[MI + 6][MJ];  // v ubar along i
+    __shared__ double sFy[MI][MJ + 6];  // v^2 along j
+    const int tj = threadIdx.x & (MJ - 1), ti = threadIdx.x / MJ;
+    const int j0 = G.g + 1 + blockIdx.x * MJ, i0 = G.g + blockIdx.y * MI;
+    const int ipad = G.nx + 2 * G.g, jpad = G.ny + 2 * G.g;
+    for (int r = ti; r < MI + 6; r += 256 / MJ) {
+        const int i = i0 - 3 + r, j = j0 + tj;
+        sFx[r][tj] = (i < ipad && j < jpad) ? v[G.idx(i, j)] * ubar_v(G, u, i, j) : 0.0;
+    }
+    for (int e = threadIdx.x; e < MI * (MJ + 6); e += 256) {
+        const int r = e / (MJ + 6), cc = e - r * (MJ + 6);
+        const int i = i0 + r, j = j0 - 3 + cc;
+        double f = 0.0;
+        if (i < ipad && j < jpad) { const double w = v[G.idx(i, j)]; f = w * w; }
+        sFy[r][cc] = f;
+    }
+    __syncthreads();
+    for (int r = ti; r < MI; r += 256 / MJ) {
+    const int i = i0 + r, j = j0 + tj;
+    if (i >= G.nx + G.g || j >= G.ny + G.g) continue;
     const int64_t c = G.idx(i, j);
     double lxp, lxm, lyp, lym;
-    {
-        double f[7];
-        for (int k = 0; k < 7; k++) f[k] = v[G.idx(i - 3 + k, j)] * ubar_v(G, u, i - 3 + k, j);
-        weno5(f[1] - f[0], f[2] - f[1], f[3] - f[2], f[4] - f[3], f[5] - f[4], f[6] - f[5], 1.0 / G.dx,
-              1e-6 * G.dx * G.dx, lxp, lxm);
-    }
-    {
-        double f[7];
-        for (int k = 0; k < 7; k++) { const double w = v[G.idx(i, j - 3 + k)]; f[k] = w * w; }
-        weno5(f[1] - f[0], f[2] - f[1], f[3] - f[2], f[4] - f[3], f[5] - f[4], f[6] - f[5], 1.0 / G.dy,
-              1e-6 * G.dy * G.dy, lyp, lym);
-    }
+    weno5(sFx[r + 1][tj] - sFx[r][tj], sFx[r + 2][tj] - sFx[r + 1][tj], sFx[r + 3][tj] - sFx[r + 2][tj],
+          sFx[r + 4][tj] - sFx[r + 3][tj], sFx[r + 5][tj] - sFx[r + 4][tj], sFx[r + 6][tj] - sFx[r + 5][tj],
+          1.0 / G.dx, 1e-6 * G.dx * G.dx, lxp, lxm);
+    weno5(sFy[r][tj + 1] - sFy[r][tj], sFy[r][tj + 2] - sFy[r][tj + 1], sFy[r][tj + 3] - sFy[r][tj + 2],
+          sFy[r][tj + 4] - sFy[r][tj + 3], sFy[r][tj + 5] - sFy[r][tj + 4], sFy[r][tj + 6] - sFy[r][tj + 5],
+          1.0 / G.dy, 1e-6 * G.dy * G.dy, lyp, lym);
     const double aX = ubar_v(G, u, i, j), aY = v[c];
     const double cV = 0.5 * (lxp + lxm) + 0.5 * (lyp + lym) - aX * (0.5 * (lxp - lxm)) - aY * (0.5 * (lyp - lym));
     double dV = 0.0;
@@ -334,6 +371,7 @@ __global__ __launch_bounds__(256) void k_mom_v(Geom G, MomArgs A, const double* 
     else if (A.mode == 5) o = cV;
     else o = dV;
     out[c] = o;
+    }
 }
 
 }  // namespace asd2
@@ -393,11 +431,12 @@ int asd2_mac_update_dt(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_u, c
     if (!fl || !d_u || !d_v || !dt_out) { set_error("null argument"); return ASD2_ERR_ARG; }
     const Geom& G = ctx->geo;
     double *partials, *out;
-    unsigned* ticket;
-    ASD2_TRY(ws(ctx, "dt_partials", 2 * 1024, &partials));
+    const dim3 grid(std::max(1, std::min(8, (G.ny + 255) / 256)), std::max(1, std::min(128, G.nx)));
+    const int nb = static_cast<int>(grid.x * grid.y);
+    ASD2_TRY(ws(ctx, "dt_partials", 2 * nb, &partials));
     ASD2_TRY(ws(ctx, "dt_out", 2, &out));
-    ASD2_TRY(ws(ctx, "dt_ticket", 2, &ticket));
-    hipLaunchKernelGGL(k_velmax, dim3(1024), dim3(256), 0, ctx->stream, G, d_u, d_v, partials, ticket, out);
+    hipLaunchKernelGGL(k_velmax, grid, dim3(256), 0, ctx->stream, G, d_u, d_v, partials);
+    hipLaunchKernelGGL(k_velmax_final, dim3(1), dim3(256), 0, ctx->stream, partials, nb, out);
     ASD2_LAUNCH_CHECK();
     double h[2];
     ASD2_HIP(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
@@ -439,7 +478,7 @@ int asd2_mac_intermediate_vel(asd2_ctx* ctx, const asd2_fluid* fl, int time_orde
     ASD2_TRY(ws(ctx, "mom_v2", n, &v2));
     // fresh stage arrays (GetIntermediateVel allocates zero vectors, mac2d.cpp:1146-1147)
     for (double* p : {u1, v1, u2, v2}) ASD2_TRY(zero_async(ctx, p, n * 8));
-    const dim3 gu(cdiv(G.ny, 64), cdiv(G.nx - 1, 4)), gv(cdiv(G.ny - 1, 64), cdiv(G.nx, 4));
+    const dim3 gu(cdiv(G.ny, MJ), cdiv(G.nx - 1, MI)), gv(cdiv(G.ny - 1, MJ), cdiv(G.nx, MI));
     auto stage = [&](const double* us, const double* vs, const double* u0, const double* v0, double* uo, double* vo,
                      int mode) -> int {
         A.mode = mode;
@@ -486,10 +525,10 @@ int asd2_mac_momentum_term(asd2_ctx* ctx, const asd2_fluid* fl, int grid_kind, i
     A.mode = 4 + term;
     ASD2_TRY(zero_async(ctx, d_out, static_cast<size_t>(G.size()) * 8));
     if (grid_kind == ASD2_GRID_U)
-        hipLaunchKernelGGL(k_mom_u, dim3(cdiv(G.ny, 64), cdiv(G.nx - 1, 4)), dim3(256), 0, ctx->stream, G, A, d_u,
+        hipLaunchKernelGGL(k_mom_u, dim3(cdiv(G.ny, MJ), cdiv(G.nx - 1, MI)), dim3(256), 0, ctx->stream, G, A, d_u,
                            d_v, d_ls, d_Hs, d_u, d_out);
     else
-        hipLaunchKernelGGL(k_mom_v, dim3(cdiv(G.ny - 1, 64), cdiv(G.nx, 4)), dim3(256), 0, ctx->stream, G, A, d_u,
+        hipLaunchKernelGGL(k_mom_v, dim3(cdiv(G.ny - 1, MJ), cdiv(G.nx, MI)), dim3(256), 0, ctx->stream, G, A, d_u,
                            d_v, d_ls, d_Hs, d_v, d_out);
     ASD2_LAUNCH_CHECK();
     return ASD2_OK;
