@@ -1655,21 +1655,18 @@ __global__ __launch_bounds__(KB) void k_ml_up(Level L, Level next, float* __rest
     }
 }
 
-// y = A^-1 g on the coarsest level (FP64 inverse): 16 rows per block, 4 per wave, the lanes
-// splitting each row
+// y = A^-1 g on the coarsest level (FP64 inverse): one row per wave, the lanes splitting it
 __global__ __launch_bounds__(KB) void k_ml_dense_apply(Level L, const double* __restrict__ Minv,
                                                        const KrylovState* st) {
     if (st->done != DONE_RUNNING) return;
-    const int m = L.H * L.W, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int rr = 0; rr < 4; rr++) {
-        const int p = blockIdx.x * 16 + w * 4 + rr;
-        if (p >= m) break;
-        const double* row = Minv + static_cast<int64_t>(p) * m;
-        double s = 0.0;
-        for (int q = lane; q < m; q += 64) s += row[q] * L.g[q];
-        s = wave_sum(s);
-        if (lane == 0) L.y[p] = s;
-    }
+    const int m = L.H * L.W, lane = threadIdx.x & 63;
+    const int p = blockIdx.x * (KB / 64) + (threadIdx.x >> 6);  // one row per wave
+    if (p >= m) return;
+    const double* row = Minv + static_cast<int64_t>(p) * m;
+    double s = 0.0;
+    for (int q = lane; q < m; q += 64) s += row[q] * L.g[q];
+    s = wave_sum(s);
+    if (lane == 0) L.y[p] = s;
 }
 
 // a single level (the first coarsening already small): y = A^-1 g, FP32 output; one block
@@ -1890,7 +1887,7 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
                                            Hh.L[l + 1], D.st);
                     }
                     if (first == Hh.n - 1)
-                        hipLaunchKernelGGL(k_ml_dense_apply, dim3(ceil_div(Hh.L[first].H * Hh.L[first].W, 16)),
+                        hipLaunchKernelGGL(k_ml_dense_apply, dim3(ceil_div(Hh.L[first].H * Hh.L[first].W, KB / 64)),
                                            dim3(KB), 0, ctx->stream, Hh.L[first], Minv, D.st);
                     else
                         hipLaunchKernelGGL(k_ml_lower, dim3(1), dim3(1024), 0, ctx->stream, Hh, first, Minv, D.st);
