@@ -151,6 +151,7 @@ def load() -> C.CDLL:
         "asd2_timer_stop": ([_VP, P(C.c_double)], C.c_int),
         "asd2_profile_kernel": ([_VP, C.c_int], C.c_int),
         "asd2_profile_read": ([_VP, P(C.c_int), P(C.c_double)], C.c_int),
+        "asd2_profile_read_each": ([_VP, P(C.c_float), C.c_int, P(C.c_int)], C.c_int),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -493,3 +494,10 @@ class Context:
         ms = C.c_double(0.0)
         check(self.lib.asd2_profile_read(self.h, C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+    def profile_read_each(self, capacity: int = 1 << 16) -> np.ndarray:
+        """Per-launch durations (ms) of the profiled kernel family, in launch order."""
+        buf = np.zeros(capacity, dtype=np.float32)
+        n = C.c_int(0)
+        check(self.lib.asd2_profile_read_each(self.h, buf.ctypes.data_as(C.POINTER(C.c_float)), capacity, C.byref(n)))
+        return buf[:min(n.value, capacity)].copy()

@@ -281,6 +281,21 @@ int asd2_profile_kernel(asd2_ctx* ctx, int kernel_id) {
     return ASD2_OK;
 }
 
+int asd2_profile_read_each(asd2_ctx* ctx, float* durations_ms, int capacity, int* launches) {
+    ASD2_TRY(check_ctx(ctx));
+    if (capacity < 0 || (capacity > 0 && !durations_ms)) { set_error("bad buffer"); return ASD2_ERR_ARG; }
+    ASD2_HIP(hipStreamSynchronize(ctx->stream));
+    int n = 0;
+    for (int k = 0; k + 1 < ctx->prof_used; k += 2, n++) {
+        float f = 0.0f;
+        ASD2_HIP(hipEventElapsedTime(&f, ctx->prof_events[k], ctx->prof_events[k + 1]));
+        if (n < capacity) durations_ms[n] = f;
+    }
+    ctx->prof_used = 0;
+    if (launches) *launches = n;
+    return ASD2_OK;
+}
+
 int asd2_profile_read(asd2_ctx* ctx, int* launches, double* total_ms) {
     ASD2_TRY(check_ctx(ctx));
     ASD2_HIP(hipStreamSynchronize(ctx->stream));

@@ -1016,13 +1016,6 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
                                                 KrylovState* st, const double* ps, int nbs, double* pu, int itn,
                                                 int nwin, double* pb) {
     if (st->done != DONE_RUNNING) return;
-    float alpha = 0.f;
-    const int it = itn - 1;  // the iteration updated when UPD
-    if (UPD) {
-        double a64;
-        if (!fcg_alpha(st, ps, nbs, it, a64)) return;
-        alpha = static_cast<float>(a64);
-    }
     // Dd is the ring seen from itn: Dd.p[0], Dd.q[0] = the direction of the updated iteration
     const float* __restrict__ pu_dir = Dd.p[0];
     const float* __restrict__ qu_dir = Dd.q[0];
@@ -1039,6 +1032,13 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
     const int lane = threadIdx.x & 63;
     int t0, t1;
     tile_range(nreg, t0, t1);
+    float alpha = 0.f;
+    const int it = itn - 1;  // the iteration updated when UPD
+    if (UPD) {
+        double a64;
+        if (!fcg_alpha(st, ps, nbs, it, a64)) return;
+        alpha = static_cast<float>(a64);
+    }
     double a_rz = 0.0, a_uu = 0.0, a_rr = 0.0, a_zq[MW] = {};
     for (int t = t0; t < t1; t++) {
         const int rj = t / reg_i, ri = t - rj * reg_i;
@@ -1200,6 +1200,38 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
                                                 const double* pb, int nbb, const double* pc, int nbc, double* ps,
                                                 int it, int nwin, double gamma_fixed, const double* pu, int nbu) {
     if (st->done != DONE_RUNNING) return;
+    // The tile operands (r, D^-1, y, the kept directions) of a block's first tile are loaded into
+    // registers before the scalars are reduced, so that the reduction's latency overlaps them;
+    // later tiles are loaded while the previous one computes.
+    constexpr int NH = (TI + 2) * (TJ + 2), NE = (NH + KB - 1) / KB;
+    const int ipad = G.nx + 2 * G.g, jpad = G.ny + 2 * G.g;
+    const int ilo = G.g, ihi = G.g + G.nx, jlo = G.g, jhi = G.g + G.ny;
+    const int tiles_i = ceil_div(G.nx, TI), ntiles = tiles_i * ceil_div(G.ny, TJ);
+    int t0, t1;
+    tile_range(ntiles, t0, t1);
+    float ld_r[NE], ld_d[NE], ld_y[NE], ld_p[NE][MW];
+    auto load = [&](int t) {
+        const int bj = t / tiles_i, bi = t - bj * tiles_i;
+        const int j0 = G.g + bj * TJ, i0 = G.g + bi * TI;
+#pragma unroll
+        for (int u = 0; u < NE; u++) {
+            const int e = threadIdx.x + u * KB, rr_ = e / (TJ + 2), cc = e - rr_ * (TJ + 2);
+            const int i = i0 - 1 + rr_, j = j0 - 1 + cc;
+            ld_r[u] = ld_d[u] = ld_y[u] = 0.0f;
+#pragma unroll
+            for (int jj = 0; jj < MW; jj++) ld_p[u][jj] = 0.0f;
+            if (e < NH && i < ipad && j < jpad && i >= ilo && i < ihi && j >= jlo && j < jhi) {
+                const int64_t k = G.idx(i, j);
+                ld_r[u] = r[k];
+                ld_d[u] = dinv[k];
+                if (y) ld_y[u] = y[static_cast<int64_t>((i - ilo) / CC) * C.Wc + (j - jlo) / CC];
+#pragma unroll
+                for (int jj = 0; jj < MW; jj++)
+                    if (jj < nwin) ld_p[u][jj] = Dd.p[jj][k];
+            }
+        }
+    };
+    if (t0 < t1) load(t0);
     // every block re-sums the rows this iteration needs: r.r of the kernel that made r; r.D^-1 r
     // and the MW (D^-1 r).q_j of the fine pass; y.g, y.A_c y, y.P^T r and the MW y.P^T q_j
     double v[1 + (1 + MW) + (3 + MW)];
@@ -1256,30 +1288,27 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
     float* __restrict__ pnew = Dd.pnew;
     float* __restrict__ q = Dd.qnew;
     __shared__ float sP[TI + 2][TJ + 2];
-    const int ipad = G.nx + 2 * G.g, jpad = G.ny + 2 * G.g;
-    const int ilo = G.g, ihi = G.g + G.nx, jlo = G.g, jhi = G.g + G.ny;
-    const int tiles_i = ceil_div(G.nx, TI), ntiles = tiles_i * ceil_div(G.ny, TJ);
-    int t0, t1;
-    tile_range(ntiles, t0, t1);
     const int tj = threadIdx.x & (TJ - 1), ti = threadIdx.x / TJ;
     double acc = 0.0;
     for (int t = t0; t < t1; t++) {
         const int bj = t / tiles_i, bi = t - bj * tiles_i;
         const int j0 = G.g + bj * TJ, i0 = G.g + bi * TI;
-        for (int e = threadIdx.x; e < (TI + 2) * (TJ + 2); e += KB) {
-            const int rr_ = e / (TJ + 2), cc = e - rr_ * (TJ + 2);
+#pragma unroll
+        for (int u = 0; u < NE; u++) {
+            const int e = threadIdx.x + u * KB, rr_ = e / (TJ + 2), cc = e - rr_ * (TJ + 2);
+            if (e >= NH) continue;
             const int i = i0 - 1 + rr_, j = j0 - 1 + cc;
             float val = 0.0f;
             if (i < ipad && j < jpad && i >= ilo && i < ihi && j >= jlo && j < jhi) {
-                const int64_t k = G.idx(i, j);
-                val = dinv[k] * r[k];
-                if (y) val += gmf * y[static_cast<int64_t>((i - ilo) / CC) * C.Wc + (j - jlo) / CC];
+                val = ld_d[u] * ld_r[u];
+                if (y) val += gmf * ld_y[u];
 #pragma unroll
                 for (int jj = 0; jj < MW; jj++)
-                    if (jj < nwin) val += coef[jj] * pj[jj][k];
+                    if (jj < nwin) val += coef[jj] * ld_p[u][jj];
             }
             sP[rr_][cc] = val;
         }
+        if (t + 1 < t1) load(t + 1);
         __syncthreads();
         for (int rr_ = ti; rr_ < TI; rr_ += KB / TJ) {
             const int i = i0 + rr_, j = j0 + tj;

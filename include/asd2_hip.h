@@ -289,6 +289,8 @@ int asd2_timer_stop(asd2_ctx* ctx, double* ms);
 #define ASD2_KID_FCG_UPDATE 11    /* flexible PCG: x/r update + dots */
 int asd2_profile_kernel(asd2_ctx* ctx, int kernel_id);
 int asd2_profile_read(asd2_ctx* ctx, int* launches, double* total_ms);
+/* the same records one by one: the first min(capacity, *launches) durations in launch order */
+int asd2_profile_read_each(asd2_ctx* ctx, float* durations_ms, int capacity, int* launches);
 
 #ifdef __cplusplus
 }
