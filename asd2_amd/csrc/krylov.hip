@@ -1026,6 +1026,7 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
     constexpr int NS = 2 + MW;  // block sums: w, r, q_1 .. q_MW
     __shared__ float sZ[RI + 2][LP_RJ + 2];
     __shared__ float sR[RI][LP_RJ];
+    __shared__ float sQ[UPD ? RI : 1][LP_RJ];  // q of the updated direction (= q_1) on the region
     __shared__ double sB[NS][nblk];
     const int reg_i = ceil_div(G.nx, RI), nreg = reg_i * ceil_div(G.ny, LP_RJ);
     const int ilo = G.g, ihi = G.g + G.nx, jlo = G.g, jhi = G.g + G.ny;
@@ -1050,9 +1051,16 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
             if (i >= ilo && i < ihi && j >= jlo && j < jhi) {
                 const int64_t k = G.idx(i, j);
                 float rv = r_in[k];
-                if (UPD) rv = -alpha * qu_dir[k] + rv;
+                float qv = 0.0f;
+                if (UPD) {
+                    qv = qu_dir[k];
+                    rv = -alpha * qv + rv;
+                }
                 v = dinv[k] * rv;
-                if (a >= 1 && a <= RI && b >= 1 && b <= LP_RJ) sR[a - 1][b - 1] = rv;
+                if (a >= 1 && a <= RI && b >= 1 && b <= LP_RJ) {
+                    sR[a - 1][b - 1] = rv;
+                    if (UPD) sQ[UPD ? a - 1 : 0][b - 1] = qv;
+                }
             }
             sZ[a][b] = v;
         }
@@ -1081,7 +1089,7 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
 #pragma unroll
                 for (int jj = 0; jj < MW; jj++)
                     if (jj < nwin) {
-                        qv[jj] = qj[jj][k];
+                        qv[jj] = (UPD && jj == 0) ? sQ[UPD ? a : 0][b] : qj[jj][k];
                         a_zq[jj] += zc * qv[jj];
                     }
             }
@@ -1846,9 +1854,10 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
     oo.mode = ASD2_CG_STANDARD;
     Driver D;
     ASD2_TRY(driver_init(ctx, &D));
-    // a few hundred iterations at most: the FP32 recurrence stays close to the true residual over
-    // two decades, and every update costs a host round trip, so the default factor is 0.01
-    ASD2_TRY(set_state(&D, &oo, 0.01));
+    // the multilevel solve takes a few hundred iterations of one fixed preconditioner: its FP32
+    // recurrence stays close to the true residual over two decades, and every update costs a
+    // host round trip, so its default factor is 0.01; the learned solve keeps 0.1
+    ASD2_TRY(set_state(&D, &oo, learned ? 0.1 : 0.01));
     hipStream_t saved = learned ? N->stream : nullptr;
     if (learned) N->stream = ctx->stream;
     struct Restore {
