@@ -100,7 +100,9 @@ void prof_begin(asd2_ctx* ctx, int kid) {
     if (ctx->prof_used + 2 > static_cast<int>(ctx->prof_events.size())) {
         for (int k = 0; k < 256; k++) {
             hipEvent_t e;
-            if (hipEventCreate(&e) != hipSuccess) return;
+            // timing only, read after a stream synchronize: no system-scope fence (the default
+            // one writes back the caches at every record and inflates the bracketed kernel)
+            if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return;
             ctx->prof_events.push_back(e);
         }
     }

@@ -48,7 +48,7 @@ BYTES_PER_CELL = {
 # asd2_stream_copy of 2^25 doubles); the roofline's `traffic` is this figure x cells.
 MEASURED_BYTES_PER_CELL = {
     "pcg64_spmv": 61.5, "pcg64_update": 64.4, "pcg32_spmv": 31.2, "pcg32_update": 32.2,
-    "fcg_spmv": 36.3, "fcg_update": 47.4, "weno_stage": (44.7 + 54.9 + 53.7) / 3,
+    "fcg_spmv": 36.1, "fcg_update": 46.4, "weno_stage": (44.8 + 54.9 + 54.0) / 3,
     "reinit": (29.6 + 39.6) / 2, "gfm": 100.6,
 }
 
