@@ -919,8 +919,8 @@ inline Dirs dirs_for(const Ring& R, int k) {
     return d;
 }
 
-// fine region of the begin kernel: whole c x c blocks, at most 32 x 64 cells
-__host__ __device__ inline int lp_ri(int c) { return c > 16 ? c : 16; }
+// fine region of the coarse fine pass: whole c x c blocks, 32 x 64 cells (c x 64 when c > 32)
+__host__ __device__ constexpr int lp_ri(int c) { return c > 32 ? c : 32; }
 constexpr int LP_RJ = 64;
 
 __global__ __launch_bounds__(KB) void k_fcg_init(int64_t n, const double* __restrict__ b, float* __restrict__ x,
@@ -1022,7 +1022,7 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
     const float* qj[MW];
 #pragma unroll
     for (int j = 0; j < MW; j++) qj[j] = Dd.q[j];
-    constexpr int c = CC, RI = (CC > 16 ? CC : 16), nbi = RI / c, nbj = LP_RJ / c, nblk = nbi * nbj;
+    constexpr int c = CC, RI = lp_ri(CC), nbi = RI / c, nbj = LP_RJ / c, nblk = nbi * nbj;
     constexpr int NS = 2 + MW;  // block sums: w, r, q_1 .. q_MW
     __shared__ float sZ[RI + 2][LP_RJ + 2];
     __shared__ float sR[RI][LP_RJ];
