@@ -1208,19 +1208,20 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
                                                 const double* pb, int nbb, const double* pc, int nbc, double* ps,
                                                 int it, int nwin, double gamma_fixed, const double* pu, int nbu) {
     if (st->done != DONE_RUNNING) return;
+    constexpr int FI = 32;  // tile rows (FI x TJ tiles)
     // The tile operands (r, D^-1, y, the kept directions) of a block's first tile are loaded into
     // registers before the scalars are reduced, so that the reduction's latency overlaps them;
     // later tiles are loaded while the previous one computes.
-    constexpr int NH = (TI + 2) * (TJ + 2), NE = (NH + KB - 1) / KB;
+    constexpr int NH = (FI + 2) * (TJ + 2), NE = (NH + KB - 1) / KB;
     const int ipad = G.nx + 2 * G.g, jpad = G.ny + 2 * G.g;
     const int ilo = G.g, ihi = G.g + G.nx, jlo = G.g, jhi = G.g + G.ny;
-    const int tiles_i = ceil_div(G.nx, TI), ntiles = tiles_i * ceil_div(G.ny, TJ);
+    const int tiles_i = ceil_div(G.nx, FI), ntiles = tiles_i * ceil_div(G.ny, TJ);
     int t0, t1;
     tile_range(ntiles, t0, t1);
     float ld_r[NE], ld_d[NE], ld_y[NE], ld_p[NE][MW];
     auto load = [&](int t) {
         const int bj = t / tiles_i, bi = t - bj * tiles_i;
-        const int j0 = G.g + bj * TJ, i0 = G.g + bi * TI;
+        const int j0 = G.g + bj * TJ, i0 = G.g + bi * FI;
 #pragma unroll
         for (int u = 0; u < NE; u++) {
             const int e = threadIdx.x + u * KB, rr_ = e / (TJ + 2), cc = e - rr_ * (TJ + 2);
@@ -1295,12 +1296,12 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
     for (int j = 0; j < MW; j++) pj[j] = Dd.p[j];
     float* __restrict__ pnew = Dd.pnew;
     float* __restrict__ q = Dd.qnew;
-    __shared__ float sP[TI + 2][TJ + 2];
+    __shared__ float sP[FI + 2][TJ + 2];
     const int tj = threadIdx.x & (TJ - 1), ti = threadIdx.x / TJ;
     double acc = 0.0;
     for (int t = t0; t < t1; t++) {
         const int bj = t / tiles_i, bi = t - bj * tiles_i;
-        const int j0 = G.g + bj * TJ, i0 = G.g + bi * TI;
+        const int j0 = G.g + bj * TJ, i0 = G.g + bi * FI;
 #pragma unroll
         for (int u = 0; u < NE; u++) {
             const int e = threadIdx.x + u * KB, rr_ = e / (TJ + 2), cc = e - rr_ * (TJ + 2);
@@ -1318,7 +1319,7 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
         }
         if (t + 1 < t1) load(t + 1);
         __syncthreads();
-        for (int rr_ = ti; rr_ < TI; rr_ += KB / TJ) {
+        for (int rr_ = ti; rr_ < FI; rr_ += KB / TJ) {
             const int i = i0 + rr_, j = j0 + tj;
             if (i >= ihi || j >= jhi) continue;
             const int64_t k = G.idx(i, j);
