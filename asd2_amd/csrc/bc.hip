@@ -1,4 +1,6 @@
 // bc.hip -- ghost-cell fill of the staggered grids (reference src/bc2d.cpp).
+#include <algorithm>
+
 #include "internal.h"
 
 namespace asd2 {
@@ -96,7 +98,9 @@ int bc_apply_p(asd2_ctx* ctx, double* d) {
 // One workgroup walks the sides W, E, S, N in the reference order with a barrier
 // between sides: the U/V rules overlap at corners (bc2d.cpp:599-627,705-733), so the
 // side order is part of the result.  Within a side all writes are independent.
-#define PFOR(count) for (int q = threadIdx.x; q < (count); q += blockDim.x)
+// grid-stride loop over the cells of one ghost-fill statement (one block: block-stride)
+#define PFOR(count) \
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < (count); q += gridDim.x * blockDim.x)
 
 __device__ void u_side(const Geom& G, int side, int k, double c, double* a) {
     const int nx = G.nx, ny = G.ny, g = G.g;
@@ -210,19 +214,35 @@ __device__ void v_side(const Geom& G, int side, int k, double c, double* a) {
 }
 #undef PFOR
 
-__global__ __launch_bounds__(1024) void k_bc_uv(Geom G, BcArgs A, int is_v, double* __restrict__ a) {
-    for (int side = 0; side < 4; side++) {
+// Velocity ghost cells in the reference's side order W, E, S, N, on many blocks.  The W and E
+// statements write disjoint cells and, unless a side is periodic, read only rows they do not
+// write, so they share one launch; likewise S and N, which read what W and E wrote.  A periodic
+// side reads the row (column) the opposite side writes first, so a periodic pair runs one side
+// per launch.  Inside a side no statement reads a cell another one writes.
+__global__ __launch_bounds__(256) void k_bc_uv(Geom G, BcArgs A, int is_v, int side0, int side1,
+                                               double* __restrict__ a) {
+    for (int side = side0; side < side1; side++) {
         if (is_v) v_side(G, side, A.kind[side], A.c[side], a);
         else u_side(G, side, A.kind[side], A.c[side], a);
-        __syncthreads();
     }
 }
 
 int bc_apply_uv(asd2_ctx* ctx, int grid_kind, double* d) {
     BcArgs A;
     for (int s = 0; s < 4; s++) { A.kind[s] = ctx->bc[grid_kind][s]; A.c[s] = ctx->bcc[grid_kind][s]; }
-    hipLaunchKernelGGL(k_bc_uv, dim3(1), dim3(1024), 0, ctx->stream, ctx->geo, A,
-                       grid_kind == ASD2_GRID_V ? 1 : 0, d);
+    const Geom& G = ctx->geo;
+    const int cells = (std::max(G.nx, G.ny) + 2 * G.g) * (G.g + 1);  // largest statement
+    const int nb = std::max(1, std::min(128, (cells + 255) / 256));
+    const int is_v = grid_kind == ASD2_GRID_V ? 1 : 0;
+    const bool tiny = G.nx < 2 * G.g + 2 || G.ny < 2 * G.g + 2;  // mirrored rows could reach the far side
+    for (int pair = 0; pair < 4; pair += 2) {
+        if (tiny || A.kind[pair] == ASD2_BC_PERIODIC || A.kind[pair + 1] == ASD2_BC_PERIODIC) {
+            for (int s = pair; s < pair + 2; s++)
+                hipLaunchKernelGGL(k_bc_uv, dim3(nb), dim3(256), 0, ctx->stream, G, A, is_v, s, s + 1, d);
+        } else {
+            hipLaunchKernelGGL(k_bc_uv, dim3(nb), dim3(256), 0, ctx->stream, G, A, is_v, pair, pair + 2, d);
+        }
+    }
     ASD2_LAUNCH_CHECK();
     return ASD2_OK;
 }

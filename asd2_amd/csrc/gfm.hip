@@ -168,14 +168,15 @@ __global__ void k_dinv(Geom G, const double* dg, double* di) {
     di[G.idx(i, j)] = 1.0 / dg[G.idx(i, j)];
 }
 
+// The FP32 copies of the assembled system: both producers of a system (the assembly kernel and
+// the import) write them together with the FP64 arrays, so only the pointers are fetched here.
 int ensure_system32(asd2_ctx* ctx, System* S) {
     if (S->ax32) return ASD2_OK;
-    ASD2_TRY(system_arrays(ctx, S, true));
-    const int64_t n = ctx->geo.size();
-    hipLaunchKernelGGL(k_sys_to32, dim3(2048), dim3(256), 0, ctx->stream, n, S->ax, S->ay, S->diag, S->dinv,
-                       S->ax32, S->ay32, S->diag32, S->dinv32);
-    ASD2_LAUNCH_CHECK();
-    return ASD2_OK;
+    if (!ctx->system_ready) {
+        set_error("no assembled system");
+        return ASD2_ERR_STATE;
+    }
+    return system_arrays(ctx, S, true);
 }
 
 }  // namespace asd2
