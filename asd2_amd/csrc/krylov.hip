@@ -1014,8 +1014,16 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
                                                 const float* __restrict__ dinv, const float* __restrict__ ax,
                                                 const float* __restrict__ ay, const float* __restrict__ dg,
                                                 KrylovState* st, const double* ps, int nbs, double* pu, int itn,
-                                                int nwin, double* pb) {
-    if (st->done != DONE_RUNNING) return;
+                                                int nwin, double* pb, KrylovState* snap) {
+    // snap (pinned host memory, or nullptr): the state as this kernel leaves it, written by the
+    // thread that owns the state's updates, for the host's poll after the launch
+    auto snapshot = [&]() {
+        if (snap && blockIdx.x == 0 && threadIdx.x == 0) *snap = *st;
+    };
+    if (st->done != DONE_RUNNING) {
+        snapshot();
+        return;
+    }
     // Dd is the ring seen from itn: Dd.p[0], Dd.q[0] = the direction of the updated iteration
     const float* __restrict__ pu_dir = Dd.p[0];
     const float* __restrict__ qu_dir = Dd.q[0];
@@ -1037,7 +1045,10 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
     const int it = itn - 1;  // the iteration updated when UPD
     if (UPD) {
         double a64;
-        if (!fcg_alpha(st, ps, nbs, it, a64)) return;
+        if (!fcg_alpha(st, ps, nbs, it, a64)) {
+            snapshot();
+            return;
+        }
         alpha = static_cast<float>(a64);
     }
     double a_rz = 0.0, a_uu = 0.0, a_rr = 0.0, a_zq[MW] = {};
@@ -1136,6 +1147,7 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
         put_partials(w2, pu);
         if (blockIdx.x == 0 && threadIdx.x == 0) st->iter = itn;
     }
+    snapshot();
 }
 
 // linear-pass input channel 0 = g / dc / s, s = rms over the coarse cells of g / dc
@@ -1177,10 +1189,11 @@ __global__ __launch_bounds__(KB) void k_lp_coarse_dots(Coarse C, const float* __
 template <bool UPD, int MW>
 void launch_lp_pass(hipStream_t s, int nb, Geom G, Coarse C, const float* r_in, float* r_out, float* x, Dirs R,
                     const float* dinv, const float* ax, const float* ay, const float* dg, KrylovState* st,
-                    const double* ps, int nbs, double* pu, int itn, int nwin, double* pb) {
+                    const double* ps, int nbs, double* pu, int itn, int nwin, double* pb,
+                    KrylovState* snap) {
 #define ASD2_LP_PASS(CV)                                                                                              \
     hipLaunchKernelGGL((k_lp_pass<UPD, CV, MW>), dim3(nb), dim3(KB), 0, s, G, C, r_in, r_out, x, R, dinv, ax, ay, dg, \
-                       st, ps, nbs, pu, itn, nwin, pb)
+                       st, ps, nbs, pu, itn, nwin, pb, snap)
     switch (C.c) {
         case 1: ASD2_LP_PASS(1); break;
         case 2: ASD2_LP_PASS(2); break;
@@ -1904,12 +1917,13 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
     auto nwin = [&](int k) { return std::max(0, std::min(mwin, k - wstart)); };
     const float* ycoarse = learned ? static_cast<const float*>(N->y) : yml;
     const int first = learned ? 0 : ml_first_lower(Hh);
-    auto batch = [&](int count) -> int {
+    // count iterations; the last update pass leaves the state in snap for the host's poll
+    auto batch = [&](int count, KrylovState* snap) -> int {
         for (int k = 0; k < count; k++, it++) {
             float* r = rb[it & 1];
             if (fresh) {
                 (learned ? launch_lp_pass<false, FCG_M> : launch_lp_pass<false, 1>)(ctx->stream, nbb, G, C, r, nullptr, x, dirs_for(R, it), S.dinv32, S.ax32, S.ay32, S.diag32,
-                                      D.st, D.ps, nbs, nullptr, it, nwin(it), pb);
+                                      D.st, D.ps, nbs, nullptr, it, nwin(it), pb, nullptr);
                 fresh = false;
             }
             if (use_net) {
@@ -1948,7 +1962,7 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
             prof_end(ctx, ASD2_KID_FCG_SPMV);
             prof_begin(ctx, ASD2_KID_FCG_UPDATE);
             (learned ? launch_lp_pass<true, FCG_M> : launch_lp_pass<true, 1>)(ctx->stream, nbb, G, C, r, rb[(it + 1) & 1], x, dirs_for(R, it + 1), S.dinv32, S.ax32, S.ay32, S.diag32,
-                                 D.st, D.ps, nbs, D.pu, it + 1, nwin(it + 1), pb);
+                                 D.st, D.ps, nbs, D.pu, it + 1, nwin(it + 1), pb, k + 1 == count ? snap : nullptr);
             prof_end(ctx, ASD2_KID_FCG_UPDATE);
             nbu = nbb;
         }
@@ -1982,8 +1996,7 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
     int slot = 0;
     bool stale[2] = {false, false};  // snapshot of a batch enqueued before a host intervention
     auto enqueue = [&](int sl) -> int {
-        ASD2_TRY(batch(check_every));
-        ASD2_HIP(hipMemcpyAsync(&D.host[sl], D.st, sizeof(KrylovState), hipMemcpyDeviceToHost, ctx->stream));
+        ASD2_TRY(batch(check_every, &D.host[sl]));  // the batch's last kernel writes the snapshot
         ASD2_HIP(hipEventRecord(D.ev[sl], ctx->stream));
         return ASD2_OK;
     };
