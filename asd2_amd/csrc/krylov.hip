@@ -1402,9 +1402,9 @@ __global__ void k_set_done(KrylovState* st, int from, int to) {
 
 constexpr int ML_MAX_LEVELS = 8;
 constexpr int ML_MAX_DENSE = 256;   // unknowns of the coarsest level
-constexpr int ML_T = 16;            // tile of the chip-wide level kernels (ML_T x ML_T cells)
+constexpr int ML_T = 16;            // tile of the chip-wide level kernels on large levels
+constexpr int ML_TS = 8;            // ... and on levels with fewer than 128 such tiles
 constexpr int ML_SW = 3;            // Jacobi sweeps before and after the coarse correction
-constexpr int ML_HW = ML_T + 2 * ML_SW;  // tile with the sweeps' halo
 
 struct Level {
     int H, W, c;              // c: factor to the next level (0 on the coarsest)
@@ -1514,36 +1514,41 @@ __device__ __forceinline__ double ml_sweep(const Level& L, const double (*y)[HW]
 // Sweeps 2 .. ML_SW of a tile whose state after the first is in buf[0] on the region of halo
 // ML_SW; each sweep shrinks the valid region by one cell, so the result (buf[(ML_SW - 1) & 1])
 // is valid on halo 1.  Cells outside the level stay 0.
-__device__ __forceinline__ void ml_sweeps(const Level& L, double (*buf)[ML_HW][ML_HW], int i0, int j0) {
+template <int T>
+__device__ __forceinline__ void ml_sweeps(const Level& L, double (*buf)[T + 2 * ML_SW][T + 2 * ML_SW], int i0,
+                                          int j0) {
+    constexpr int HW = T + 2 * ML_SW;
 #pragma unroll
     for (int s = 1; s < ML_SW; s++) {
-        const double (*src)[ML_HW] = buf[(s - 1) & 1];
-        double (*dst)[ML_HW] = buf[s & 1];
-        const int lo = s, n = ML_HW - 2 * s;  // region of halo ML_SW - s
-        for (int e = threadIdx.x; e < n * n; e += KB) {
+        const double (*src)[HW] = buf[(s - 1) & 1];
+        double (*dst)[HW] = buf[s & 1];
+        const int lo = s, n = HW - 2 * s;  // region of halo ML_SW - s
+        for (int e = threadIdx.x; e < n * n; e += T * T) {
             const int a = lo + e / n, b = lo + e % n;
             const int i = i0 - ML_SW + a, j = j0 - ML_SW + b;
-            dst[a][b] = (i >= 0 && i < L.H && j >= 0 && j < L.W) ? ml_sweep<ML_HW>(L, src, a, b, i, j) : 0.0;
+            dst[a][b] = (i >= 0 && i < L.H && j >= 0 && j < L.W) ? ml_sweep<HW>(L, src, a, b, i, j) : 0.0;
         }
         __syncthreads();
     }
 }
 
 // pre-smoothing and restriction on a level run by the whole chip: y1 = ML_SW Jacobi sweeps
-// from 0, next.g = block sums of g - A y1.  A block owns an ML_T x ML_T tile (whole blocks of
-// the next level: c divides ML_T) and sweeps it with a halo of ML_SW cells.
-__global__ __launch_bounds__(KB) void k_ml_down(Level L, Level next, const KrylovState* st) {
+// from 0, next.g = block sums of g - A y1.  A block of T x T threads owns a T x T tile (whole
+// blocks of the next level: c divides T) and sweeps it with a halo of ML_SW cells.
+template <int T>
+__global__ __launch_bounds__(T * T) void k_ml_down(Level L, Level next, const KrylovState* st) {
     if (st->done != DONE_RUNNING) return;
+    constexpr int ML_T = T, ML_HW = T + 2 * ML_SW, NT = T * T;
     __shared__ double sY[2][ML_HW][ML_HW];
     __shared__ double sR[ML_T][ML_T];
     const int tiles_j = ceil_div(L.W, ML_T), ntiles = ceil_div(L.H, ML_T) * tiles_j;
     const int c = L.c, nb = ML_T / c, sc = L.W + 1;
-    const int a = threadIdx.x / ML_T, b = threadIdx.x - a * ML_T;  // KB = ML_T * ML_T
+    const int a = threadIdx.x / ML_T, b = threadIdx.x - a * ML_T;
     constexpr int F = (ML_SW - 1) & 1;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int ti = t / tiles_j, tj = t - ti * tiles_j;
         const int i0 = ti * ML_T, j0 = tj * ML_T;
-        for (int e = threadIdx.x; e < ML_HW * ML_HW; e += KB) {  // sweep 1 from y = 0: g / dc
+        for (int e = threadIdx.x; e < ML_HW * ML_HW; e += NT) {  // sweep 1 from y = 0: g / dc
             const int aa = e / ML_HW, bb = e - aa * ML_HW;
             const int i = i0 - ML_SW + aa, j = j0 - ML_SW + bb;
             sY[0][aa][bb] = (i >= 0 && i < L.H && j >= 0 && j < L.W)
@@ -1551,7 +1556,7 @@ __global__ __launch_bounds__(KB) void k_ml_down(Level L, Level next, const Krylo
                                 : 0.0;
         }
         __syncthreads();
-        ml_sweeps(L, sY, i0, j0);
+        ml_sweeps<T>(L, sY, i0, j0);
         const int i = i0 + a, j = j0 + b;
         double rv = 0.0;
         if (i < L.H && j < L.W) {
@@ -1664,18 +1669,20 @@ __global__ __launch_bounds__(1024) void k_ml_lower(Hierarchy Hh, int first, cons
 // (FP32) on level 0, where the block partials (0, 0, y.P^T r, y.P^T q_1) of the fine
 // iteration's scalars go to pc (the step along P y is fixed at 1, so y.g and y.A_c y are not
 // needed)
-__global__ __launch_bounds__(KB) void k_ml_up(Level L, Level next, float* __restrict__ yout, const double* rc,
-                                              const double* qc, const KrylovState* st, double* pc) {
+template <int T>
+__global__ __launch_bounds__(T * T) void k_ml_up(Level L, Level next, float* __restrict__ yout, const double* rc,
+                                                 const double* qc, const KrylovState* st, double* pc) {
     if (st->done != DONE_RUNNING) return;
+    constexpr int ML_T = T, ML_HW = T + 2 * ML_SW, NT = T * T;
     __shared__ double sY[2][ML_HW][ML_HW];
     const int tiles_j = ceil_div(L.W, ML_T), ntiles = ceil_div(L.H, ML_T) * tiles_j;
-    const int a = threadIdx.x / ML_T, b = threadIdx.x - a * ML_T;  // KB = ML_T * ML_T
+    const int a = threadIdx.x / ML_T, b = threadIdx.x - a * ML_T;
     constexpr int F = (ML_SW - 1) & 1;
     double a_r = 0.0, a_q = 0.0;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int ti = t / tiles_j, tj = t - ti * tiles_j;
         const int i0 = ti * ML_T, j0 = tj * ML_T;
-        for (int e = threadIdx.x; e < ML_HW * ML_HW; e += KB) {  // y2 = y1 + P y_next
+        for (int e = threadIdx.x; e < ML_HW * ML_HW; e += NT) {  // y2 = y1 + P y_next
             const int aa = e / ML_HW, bb = e - aa * ML_HW;
             const int i = i0 - ML_SW + aa, j = j0 - ML_SW + bb;
             sY[0][aa][bb] = (i >= 0 && i < L.H && j >= 0 && j < L.W)
@@ -1684,7 +1691,7 @@ __global__ __launch_bounds__(KB) void k_ml_up(Level L, Level next, float* __rest
                                 : 0.0;
         }
         __syncthreads();
-        ml_sweeps(L, sY, i0, j0);
+        ml_sweeps<T>(L, sY, i0, j0);
         const int i = i0 + a, j = j0 + b;
         if (i < L.H && j < L.W) {
             const int64_t e = static_cast<int64_t>(i) * L.W + j;
@@ -1700,9 +1707,11 @@ __global__ __launch_bounds__(KB) void k_ml_up(Level L, Level next, float* __rest
         }
         __syncthreads();
     }
-    if (pc) {
-        double v[4] = {0.0, 0.0, a_r, a_q};
-        put_partials(v, pc);
+    if constexpr (T * T == KB) {  // the partials' block reduction assumes KB threads (level 0 tiles)
+        if (pc) {
+            double v[4] = {0.0, 0.0, a_r, a_q};
+            put_partials(v, pc);
+        }
     }
 }
 
@@ -1752,6 +1761,12 @@ int ml_plan(int H0, int W0, int* H, int* W, int* c, int* nlev) {
         return ASD2_ERR_UNSUPPORTED;
     }
     return ASD2_OK;
+}
+
+// 8 x 8 tiles on a level below the first where 16 x 16 tiles would give fewer than 128 blocks
+// (level 0 keeps 16 x 16: its up kernel writes the fine iteration's partials, nbc of them)
+bool ml_small_tiles(const Level& L, int l) {
+    return l > 0 && ceil_div(L.H, ML_T) * ceil_div(L.W, ML_T) < 128 && L.c > 0 && ML_TS % L.c == 0;
 }
 
 // levels run by the whole chip (down/up kernels): the first ones down to fewer than
@@ -1935,20 +1950,35 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
                     hipLaunchKernelGGL(k_ml_dense_only, dim3(1), dim3(KB), 0, ctx->stream, Hh.L[0], Minv, yml, D.st);
                 } else {
                     for (int l = 0; l < first; l++) {
-                        const int nt = ceil_div(Hh.L[l].H, ML_T) * ceil_div(Hh.L[l].W, ML_T);
-                        hipLaunchKernelGGL(k_ml_down, dim3(std::min(nt, 4096)), dim3(KB), 0, ctx->stream, Hh.L[l],
-                                           Hh.L[l + 1], D.st);
+                        if (ml_small_tiles(Hh.L[l], l)) {
+                            const int nt = ceil_div(Hh.L[l].H, ML_TS) * ceil_div(Hh.L[l].W, ML_TS);
+                            hipLaunchKernelGGL(k_ml_down<ML_TS>, dim3(std::min(nt, 4096)), dim3(ML_TS * ML_TS), 0,
+                                               ctx->stream, Hh.L[l], Hh.L[l + 1], D.st);
+                        } else {
+                            const int nt = ceil_div(Hh.L[l].H, ML_T) * ceil_div(Hh.L[l].W, ML_T);
+                            hipLaunchKernelGGL(k_ml_down<ML_T>, dim3(std::min(nt, 4096)), dim3(KB), 0, ctx->stream,
+                                               Hh.L[l], Hh.L[l + 1], D.st);
+                        }
                     }
                     if (first == Hh.n - 1)
                         hipLaunchKernelGGL(k_ml_dense_apply, dim3(ceil_div(Hh.L[first].H * Hh.L[first].W, KB / 64)),
                                            dim3(KB), 0, ctx->stream, Hh.L[first], Minv, D.st);
                     else
                         hipLaunchKernelGGL(k_ml_lower, dim3(1), dim3(1024), 0, ctx->stream, Hh, first, Minv, D.st);
-                    for (int l = first - 1; l >= 0; l--)
-                        hipLaunchKernelGGL(k_ml_up, dim3(l == 0 ? nbc : flat_blocks(static_cast<int64_t>(Hh.L[l].H) * Hh.L[l].W)),
-                                           dim3(KB), 0, ctx->stream, Hh.L[l], Hh.L[l + 1],
-                                           l == 0 ? yml : static_cast<float*>(nullptr), C.rc, C.qc[0], D.st,
-                                           l == 0 ? pc : static_cast<double*>(nullptr));
+                    for (int l = first - 1; l >= 0; l--) {
+                        if (ml_small_tiles(Hh.L[l], l)) {
+                            const int nt = ceil_div(Hh.L[l].H, ML_TS) * ceil_div(Hh.L[l].W, ML_TS);
+                            hipLaunchKernelGGL(k_ml_up<ML_TS>, dim3(std::min(nt, 4096)), dim3(ML_TS * ML_TS), 0,
+                                               ctx->stream, Hh.L[l], Hh.L[l + 1], static_cast<float*>(nullptr), C.rc,
+                                               C.qc[0], D.st, static_cast<double*>(nullptr));
+                        } else {
+                            hipLaunchKernelGGL(k_ml_up<ML_T>,
+                                               dim3(l == 0 ? nbc : flat_blocks(static_cast<int64_t>(Hh.L[l].H) * Hh.L[l].W)),
+                                               dim3(KB), 0, ctx->stream, Hh.L[l], Hh.L[l + 1],
+                                               l == 0 ? yml : static_cast<float*>(nullptr), C.rc, C.qc[0], D.st,
+                                               l == 0 ? pc : static_cast<double*>(nullptr));
+                        }
+                    }
                 }
                 if (learned || Hh.n == 1)
                     hipLaunchKernelGGL(k_lp_coarse_dots, dim3(nbc), dim3(KB), 0, ctx->stream, C, ycoarse, D.st,
