@@ -231,7 +231,9 @@ def test_trained_network_saves_iterations(oracle):
     from asd2_amd import learned as LN
     from oracle import make_grid
     n, c = 256, 8
-    samples = LN.make_samples(n, systems=10, seed=11)
+    # 20 training systems: with 10, about one training run in three gave a network no better than
+    # Jacobi on these held-out layouts; with 20 the median gain stayed above 6
+    samples = LN.make_samples(n, systems=20, seed=11)
     net = A.UNet(n, n, max_batch=8, seed=2, coarsen=c)
     loss = LN.train(net, samples, steps=1000, batch=8, seed=3)
     print("training loss", loss[0], np.mean(loss[-50:]))
