@@ -53,6 +53,14 @@ class SolveOpts(C.Structure):
                 ("check_every", C.c_int)]
 
 
+class LsDiag(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ("cells", "mass", "centroid_x", "centroid_y", "length", "circularity",
+                                          "err_linf", "err_l1", "err_l2", "ref_cells", "mass_loss_pct")]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("reliable_updates", C.c_int), ("fallbacks", C.c_int),
                 ("converged", C.c_int), ("rel_residual", C.c_double),
@@ -133,6 +141,7 @@ def load() -> C.CDLL:
         "asd2_ls_smoothed_sign": ([_VP, _VP, _VP], C.c_int),
         "asd2_ls_sussman_constraint": ([_VP, _VP, _VP, _VP, _VP], C.c_int),
         "asd2_ls_unit_weno5": ([_VP, _VP, _VP, _VP, C.c_double, C.c_int, C.c_int], C.c_int),
+        "asd2_ls_diagnostics": ([_VP, _VP, _VP, P(LsDiag)], C.c_int),
         "asd2_mac_momentum_term": ([_VP, P(Fluid), C.c_int, C.c_int, _VP, _VP, _VP, _VP, _VP], C.c_int),
         "asd2_unet_create": ([C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, P(_VP)], C.c_int),
         "asd2_unet_destroy": ([_VP], C.c_int),
@@ -396,6 +405,13 @@ class Context:
         dP, dM = DeviceArray((n + 2 * g,)), DeviceArray((n + 2 * g,))
         check(self.lib.asd2_ls_unit_weno5(self.h, dF.ptr, dP.ptr, dM.ptr, d, n, g))
         return dP.download(), dM.download()
+
+    def ls_diagnostics(self, ls, ref=None) -> dict:
+        """Mass, centroid, interface length, circularity and (with ref) the error norms and mass
+        loss of a level-set field (asd2_ls_diagnostics)."""
+        d = LsDiag()
+        check(self.lib.asd2_ls_diagnostics(self.h, ls.ptr, None if ref is None else ref.ptr, C.byref(d)))
+        return d.as_dict()
 
     # MAC ---------------------------------------------------------------------------
     def kappa(self, ls, kap):

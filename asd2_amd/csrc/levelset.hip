@@ -458,6 +458,85 @@ static Launch tiles(const Geom& G, int tj, int ti) {
     return Launch{dim3(cdiv(G.ny, tj), cdiv(G.nx, ti)), dim3(WTH)};
 }
 
+// ------------------------------------------------------------------ run diagnostics
+// Per-block partials of: positive cells, sum of their centre x and y, interface length (without
+// dx dy), sum |e|, sum e^2, positive cells of the reference, max |e| (e = phi - ref); fixed
+// reduction order, so the result does not depend on timing.
+constexpr int DG_NV = 8;
+
+__device__ __forceinline__ void dg_block_reduce(double (&a)[DG_NV], double* __restrict__ part) {
+    __shared__ double sh[DG_NV][256];
+#pragma unroll
+    for (int k = 0; k < DG_NV; k++) sh[k][threadIdx.x] = a[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) {
+#pragma unroll
+            for (int k = 0; k < DG_NV - 1; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+            sh[DG_NV - 1][threadIdx.x] = fmax(sh[DG_NV - 1][threadIdx.x], sh[DG_NV - 1][threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < DG_NV; k++) part[k * gridDim.x + blockIdx.x] = sh[k][0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ls_diag(Geom G, const double* __restrict__ ls, const double* __restrict__ ref,
+                                                 double bx, double by, double eps, double* __restrict__ part) {
+    double a[DG_NV] = {};
+    const int64_t N = static_cast<int64_t>(G.nx) * G.ny;
+    const double hx = 0.5 / G.dx, hy = 0.5 / G.dy, pe = M_PI / eps, he = 0.5 / eps;
+    for (int64_t t = blockIdx.x * 256LL + threadIdx.x; t < N; t += static_cast<int64_t>(gridDim.x) * 256) {
+        const int i = G.g + static_cast<int>(t / G.ny), j = G.g + static_cast<int>(t % G.ny);
+        const int64_t c = G.idx(i, j);
+        const double p = ls[c];
+        if (p > 0.0) {
+            a[0] += 1.0;
+            a[1] += bx + (static_cast<double>(i - G.g) + 0.5) * G.dx;
+            a[2] += by + (static_cast<double>(j - G.g) + 0.5) * G.dy;
+        }
+        if (fabs(p) < eps) {
+            const double gx = (ls[G.idx(i + 1, j)] - ls[G.idx(i - 1, j)]) * hx;
+            const double gy = (ls[G.idx(i, j + 1)] - ls[G.idx(i, j - 1)]) * hy;
+            a[3] += (1.0 + cos(pe * p)) * he * sqrt(gx * gx + gy * gy);
+        }
+        if (ref) {
+            const double r = ref[c], e = fabs(p - r);
+            a[4] += e;
+            a[5] += e * e;
+            if (r > 0.0) a[6] += 1.0;
+            a[7] = fmax(a[7], e);
+        }
+    }
+    dg_block_reduce(a, part);
+}
+
+// totals of nb partials per value (one block)
+__global__ __launch_bounds__(256) void k_ls_diag_final(const double* __restrict__ part, int nb,
+                                                       double* __restrict__ out) {
+    double a[DG_NV] = {};
+    for (int b = threadIdx.x; b < nb; b += 256) {
+#pragma unroll
+        for (int k = 0; k < DG_NV - 1; k++) a[k] += part[k * nb + b];
+        a[DG_NV - 1] = fmax(a[DG_NV - 1], part[(DG_NV - 1) * nb + b]);
+    }
+    __shared__ double sh[DG_NV][256];
+#pragma unroll
+    for (int k = 0; k < DG_NV; k++) sh[k][threadIdx.x] = a[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) {
+#pragma unroll
+            for (int k = 0; k < DG_NV - 1; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+            sh[DG_NV - 1][threadIdx.x] = fmax(sh[DG_NV - 1][threadIdx.x], sh[DG_NV - 1][threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < DG_NV) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
 }  // namespace asd2
 
 using namespace asd2;
@@ -759,6 +838,40 @@ int asd2_ls_sussman_constraint(asd2_ctx* ctx, const double* d_ls, const double* 
     hipLaunchKernelGGL(k_sussman_constraint, dim3(cdiv(G.ny, SJ), cdiv(G.nx, SI)), dim3(WTH), 0, ctx->stream, G, R,
                        d_ls, d_ls0, d_hd, d_out, 0, 0);
     ASD2_LAUNCH_CHECK();
+    return ASD2_OK;
+}
+
+int asd2_ls_diagnostics(asd2_ctx* ctx, const double* d_ls, const double* d_ref, asd2_ls_diag* out) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!d_ls || !out) { set_error("null argument"); return ASD2_ERR_ARG; }
+    const Geom& G = ctx->geo;
+    const int nb = std::max(1, std::min(1024, static_cast<int>((static_cast<int64_t>(G.nx) * G.ny + 255) / 256)));
+    double *part, *res;
+    ASD2_TRY(ws(ctx, "diag_part", static_cast<int64_t>(DG_NV) * 1024, &part));
+    ASD2_TRY(ws(ctx, "diag_out", DG_NV, &res));
+    const double eps = 1.5 * smin_host(G.dx, G.dy);
+    hipLaunchKernelGGL(k_ls_diag, dim3(nb), dim3(256), 0, ctx->stream, G, d_ls, d_ref, ctx->base_x, ctx->base_y, eps,
+                       part);
+    hipLaunchKernelGGL(k_ls_diag_final, dim3(1), dim3(256), 0, ctx->stream, part, nb, res);
+    ASD2_LAUNCH_CHECK();
+    double h[DG_NV];
+    ASD2_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    ASD2_HIP(hipStreamSynchronize(ctx->stream));
+    asd2_ls_diag d{};
+    d.cells = h[0];
+    d.mass = h[0] * G.dx * G.dy;
+    d.centroid_x = h[0] > 0.0 ? h[1] / h[0] : 0.0;
+    d.centroid_y = h[0] > 0.0 ? h[2] / h[0] : 0.0;
+    d.length = h[3] * G.dx * G.dy;
+    d.circularity = d.length > 0.0 ? 2.0 * std::sqrt(M_PI * d.mass) / d.length : 0.0;
+    if (d_ref) {
+        d.err_linf = h[7];
+        d.err_l1 = h[4];
+        d.err_l2 = std::sqrt(h[5]) / (static_cast<double>(G.nx) * G.ny);
+        d.ref_cells = h[6];
+        d.mass_loss_pct = h[6] > 0.0 ? std::fabs(h[0] - h[6]) / h[6] * 100.0 : 0.0;
+    }
+    *out = d;
     return ASD2_OK;
 }
 

@@ -164,6 +164,20 @@ int asd2_ls_sussman_constraint(asd2_ctx* ctx, const double* d_ls, const double* 
  * (FP, FM written on [g, n + g)) */
 int asd2_ls_unit_weno5(asd2_ctx* ctx, const double* d_F, double* d_FP, double* d_FM, double d, int n, int g);
 
+/* Level-set run diagnostics.  The mass (interior cells with phi > 0, times dx dy) and, against a
+ * reference field, the error norms ||phi - ref||_inf, sum |phi - ref|, sqrt(sum (phi - ref)^2) /
+ * (nx ny) and the mass loss |cells - ref_cells| / ref_cells * 100 are the reference drivers'
+ * figures (tests/LargeTests/test_levelset2d_large.cpp:94-102, 106-127).  Added, with no
+ * reference counterpart: the centroid of the positive cells' centres (base + (i + 0.5) dx), the
+ * interface length sum delta(phi) |grad phi| dx dy (delta = (1 + cos(pi phi / e)) / (2 e) for
+ * |phi| < e, e = 1.5 min(dx, dy); central differences) and the circularity 2 sqrt(pi mass) /
+ * length (1 for a circle).  d_ref may be NULL (then the error fields are 0).  Synchronous. */
+typedef struct asd2_ls_diag {
+    double cells, mass, centroid_x, centroid_y, length, circularity;
+    double err_linf, err_l1, err_l2, ref_cells, mass_loss_pct;
+} asd2_ls_diag;
+int asd2_ls_diagnostics(asd2_ctx* ctx, const double* d_ls, const double* d_ref, asd2_ls_diag* out);
+
 /* ---------------------------------------------------------------- MAC helpers */
 /* MACSolver2D::UpdateKappa, src/mac2d.cpp:221-252 (writes the interior) */
 int asd2_mac_kappa(asd2_ctx* ctx, const double* d_ls, double* d_kappa);

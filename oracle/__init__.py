@@ -123,6 +123,8 @@ class Oracle:
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = None
+        L.or_ls_diagnostics.argtypes = [G, _P, C.c_void_p, C.c_double, C.c_double, _P]
+        L.or_ls_diagnostics.restype = None
         L.or_reinit_iterations.argtypes = [C.c_double, C.c_double]
         L.or_reinit_iterations.restype = C.c_int
         L.or_update_dt.argtypes = [G, F, _P, _P]
@@ -198,6 +200,17 @@ class Oracle:
     def update_kappa(self, gr, ls, kappa=None):
         k = np.zeros_like(ls) if kappa is None else kappa
         self.lib.or_update_kappa(C.byref(gr), _ptr(ls), _ptr(k)); return k
+
+    DIAG_FIELDS = ("cells", "mass", "centroid_x", "centroid_y", "length", "circularity",
+                   "err_linf", "err_l1", "err_l2", "ref_cells", "mass_loss_pct")
+
+    def ls_diagnostics(self, gr, ls, ref=None, base=(0.0, 0.0)):
+        """Mass and error norms of tests/LargeTests/test_levelset2d_large.cpp:94-127, plus centroid,
+        interface length and circularity (no reference counterpart); a dict."""
+        out = np.zeros(11)
+        rp = None if ref is None else ref.ctypes.data_as(C.c_void_p)
+        self.lib.or_ls_diagnostics(C.byref(gr), _ptr(ls), rp, base[0], base[1], _ptr(out))
+        return dict(zip(self.DIAG_FIELDS, out.tolist()))
 
     def heaviside(self, gr, ls):
         H = np.zeros_like(ls); self.lib.or_heaviside(C.byref(gr), _ptr(ls), _ptr(H)); return H

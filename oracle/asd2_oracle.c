@@ -1388,3 +1388,49 @@ int or_bicgstab_stencil(const or_grid* gr, const double* ax, const double* ay, c
     free(r); free(rt); free(p); free(v); free(ph); free(s); free(sh); free(t); free(minv);
     return conv ? 0 : 1;
 }
+
+/* Level-set run diagnostics.  Mass count and error norms as the reference's level-set drivers
+ * compute them (tests/LargeTests/test_levelset2d_large.cpp:94-102 mass = #(phi > 0) dx dy;
+ * :106-127 err_inf = max |phi - ref|, err_l1 = sum |phi - ref|, err_l2 = sqrt(sum (phi - ref)^2)
+ * / (nx ny), mass loss = |#phi - #ref| / #ref * 100).  No reference counterpart: centroid of the
+ * positive cells' centres, interface length sum delta(phi) |grad phi| dx dy with the cosine delta
+ * of half-width 1.5 min(dx, dy) and central differences, circularity 2 sqrt(pi mass) / length. */
+void or_ls_diagnostics(const or_grid* gr, const double* ls, const double* ref, double base_x, double base_y,
+                       double* out) {
+    const int nx = gr->nx, ny = gr->ny, g = gr->g;
+    const double dx = gr->dx, dy = gr->dy, eps = 1.5 * smin(dx, dy);
+    double cells = 0.0, sx = 0.0, sy = 0.0, len = 0.0, l1 = 0.0, l2 = 0.0, rc = 0.0, linf = 0.0;
+    for (int i = g; i < nx + g; i++)
+        for (int j = g; j < ny + g; j++) {
+            const double p = ls[IDX(gr, i, j)];
+            if (p > 0.0) {
+                cells += 1.0;
+                sx += base_x + ((double)(i - g) + 0.5) * dx;
+                sy += base_y + ((double)(j - g) + 0.5) * dy;
+            }
+            if (fabs(p) < eps) {
+                const double gx = (ls[IDX(gr, i + 1, j)] - ls[IDX(gr, i - 1, j)]) / (2.0 * dx);
+                const double gy = (ls[IDX(gr, i, j + 1)] - ls[IDX(gr, i, j - 1)]) / (2.0 * dy);
+                len += (1.0 + cos(M_PI * p / eps)) / (2.0 * eps) * sqrt(gx * gx + gy * gy);
+            }
+            if (ref) {
+                const double r = ref[IDX(gr, i, j)], e = fabs(p - r);
+                l1 += e;
+                l2 += e * e;
+                if (r > 0.0) rc += 1.0;
+                if (e > linf) linf = e;
+            }
+        }
+    const double mass = cells * dx * dy, length = len * dx * dy;
+    out[0] = cells;
+    out[1] = mass;
+    out[2] = cells > 0.0 ? sx / cells : 0.0;
+    out[3] = cells > 0.0 ? sy / cells : 0.0;
+    out[4] = length;
+    out[5] = length > 0.0 ? 2.0 * sqrt(M_PI * mass) / length : 0.0;
+    out[6] = ref ? linf : 0.0;
+    out[7] = ref ? l1 : 0.0;
+    out[8] = ref ? sqrt(l2) / ((double)nx * ny) : 0.0;
+    out[9] = ref ? rc : 0.0;
+    out[10] = (ref && rc > 0.0) ? fabs(cells - rc) / rc * 100.0 : 0.0;
+}

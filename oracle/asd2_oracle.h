@@ -69,6 +69,10 @@ void or_reinit_original(const or_grid* gr, double* ls, int iters);
 void or_reinit_first_time(const or_grid* gr, double* ls, int iters);
 void or_weno_absgrad(const or_grid* gr, const double* ls, const double* ls0, double* out);
 void or_reinit_sussman(const or_grid* gr, double* ls, int iters);
+/* run diagnostics: out = {cells, mass, centroid x, centroid y, length, circularity, err_inf,
+ * err_l1, err_l2, ref cells, mass loss %}; ref may be NULL */
+void or_ls_diagnostics(const or_grid* gr, const double* ls, const double* ref, double base_x, double base_y,
+                       double* out);
 int  or_reinit_iterations(double eps, double max_time);
 
 /* MAC helpers, src/mac2d.cpp */

@@ -253,3 +253,23 @@ def test_periodic_p_refused():
     with pytest.raises(A.Asd2Error) as e:
         ctx.ls_advect(d, ctx.array(), ctx.array(), 0.01)
     assert e.value.code == A.ERR_UNSUPPORTED
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nx,ny", [(96, 80), (256, 256)])
+def test_ls_diagnostics_vs_oracle(oracle, nx, ny):
+    """asd2_ls_diagnostics against the oracle on a perturbed circle with a reference field and
+    offset base coordinates: counts and the maximum exact, sums to rel 1e-12 (block order)."""
+    phi, dx = _level_set(nx, ny)
+    ref = np.ascontiguousarray(phi + 0.3 * dx * S.random_field(7, phi.shape))
+    gr = make_grid(nx, ny, dx, dx)
+    ctx = A.Context(nx, ny, dx, dx, base_x=0.75, base_y=-1.5)
+    got = ctx.ls_diagnostics(ctx.array(phi), ctx.array(ref))
+    exp = oracle.ls_diagnostics(gr, phi, ref=ref, base=(0.75, -1.5))
+    print(got)
+    for k in ("cells", "ref_cells", "err_linf"):
+        assert got[k] == exp[k], k
+    for k in ("mass", "centroid_x", "centroid_y", "length", "circularity", "err_l1", "err_l2", "mass_loss_pct"):
+        assert got[k] == pytest.approx(exp[k], rel=1e-12, abs=1e-300), k
+    alone = ctx.ls_diagnostics(ctx.array(phi))
+    assert alone["err_l1"] == 0.0 and alone["cells"] == exp["cells"]

@@ -336,3 +336,50 @@ def test_momentum_gravity(oracle):
     fl_q = make_fluid(1.0, 1.0, 0.01, 0.01, 0.0, dt, g=9.8, g_axis=1, gravity_fix=0)
     uh, vh = oracle.intermediate_vel(gr, fl_q, 2, phi, u, v, Hs)
     assert np.abs(vh).max() == 0.0  # mac2d.cpp:294: the V equation never sees gravity along y
+
+
+def test_ls_diagnostics_closed_forms(oracle):
+    """Run diagnostics on a circle SDF (radius 0.25 at (0.5, 0.5) on 128^2, heavy inside): the
+    mass count and centroid equal an independent numpy recount over the interior; the area, the
+    interface length (2 pi r) and the circularity (1) match the circle to the grid's accuracy; the
+    centroid is exactly the centre (the cell set is mirror-symmetric and the coordinates are exact
+    binary fractions); against phi - d the error norms are the closed forms d, d N and
+    d / sqrt(N) (the drivers' L2 normalisation sqrt(sum e^2) / (nx ny)) and the mass loss is the
+    relative change of the positive-cell count."""
+    n, r = 128, 0.25
+    dx = 1.0 / n
+    gr = make_grid(n, n, dx, dx)
+    phi = S.circle_sdf(n, n, dx, dx, 0.5, 0.5, r)
+    d = oracle.ls_diagnostics(gr, phi, base=(0.0, 0.0))
+    inner = phi[3:-3, 3:-3]
+    pos = inner > 0.0
+    assert d["cells"] == pos.sum()
+    assert d["mass"] == pos.sum() * dx * dx
+    xc = (np.arange(n) + 0.5) * dx
+    X, Y = np.meshgrid(xc, xc, indexing="ij")
+    assert d["centroid_x"] == pytest.approx(X[pos].mean(), abs=1e-14)
+    assert d["centroid_y"] == pytest.approx(Y[pos].mean(), abs=1e-14)
+    assert d["centroid_x"] == pytest.approx(0.5, abs=1e-13) and d["centroid_y"] == pytest.approx(0.5, abs=1e-13)
+    assert d["mass"] == pytest.approx(np.pi * r * r, rel=1e-2)
+    assert d["length"] == pytest.approx(2.0 * np.pi * r, rel=1e-2)
+    assert d["circularity"] == pytest.approx(1.0, rel=1e-2)
+    # offset base coordinates move the centroid by the offset
+    d_off = oracle.ls_diagnostics(gr, phi, base=(1.25, -0.5))
+    assert d_off["centroid_x"] == pytest.approx(1.75, abs=1e-13) and d_off["centroid_y"] == pytest.approx(0.0, abs=1e-13)
+    # error norms against a uniformly shifted field
+    delta = 0.01
+    ref = phi - delta
+    e = oracle.ls_diagnostics(gr, phi, ref=ref)
+    N = n * n
+    assert e["err_linf"] == pytest.approx(delta, rel=1e-12)
+    assert e["err_l1"] == pytest.approx(delta * N, rel=1e-12)
+    assert e["err_l2"] == pytest.approx(delta / np.sqrt(N), rel=1e-12)
+    ref_pos = (ref[3:-3, 3:-3] > 0.0).sum()
+    assert e["ref_cells"] == ref_pos
+    assert e["mass_loss_pct"] == pytest.approx(abs(pos.sum() - ref_pos) / ref_pos * 100.0, rel=1e-14)
+    # an ellipse (semi-axes 0.3, 0.15) is less circular
+    x = (np.arange(n + 6) - 3 + 0.5) * dx
+    Xp, Yp = np.meshgrid(x, x, indexing="ij")
+    ell = 1.0 - np.sqrt(((Xp - 0.5) / 0.3) ** 2 + ((Yp - 0.5) / 0.15) ** 2)
+    ell *= 0.15  # roughly distance-scaled near the minor axis
+    assert oracle.ls_diagnostics(gr, np.ascontiguousarray(ell))["circularity"] < 0.95
