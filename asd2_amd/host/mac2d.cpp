@@ -440,7 +440,7 @@ int MACSolver2D::SetPLTType(PLTTYPE type) {
 }
 
 // src/mac2d.cpp:1990-2050: Tecplot ASCII zones of cell-centred U, V, LS, PS and of LS, DIV,
-// PS.  The binary .szplt writer needs TECIO, which is not part of this build.
+// PS.  The binary .szplt writer needs TECIO, which is not linked here.
 int MACSolver2D::OutRes(const int iter, const double curTime, const std::string fname_vel_base,
                         const std::string fname_div_base, const std::vector<double>& u, const std::vector<double>& v,
                         const std::vector<double>& ps, const std::vector<double>& ls) {

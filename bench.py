@@ -281,7 +281,7 @@ def main():
                     help="run the short fixed workload used for hardware-counter passes")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
+        print(json.dumps({"error": "asd2_amd runs on one GPU; multi-GPU sharding is design-only "
                                    "(halo exchange + scalar all-reduce per PCG dot)"}))
         sys.exit(2)
 
