@@ -1404,7 +1404,30 @@ constexpr int ML_MAX_LEVELS = 8;
 constexpr int ML_MAX_DENSE = 256;   // unknowns of the coarsest level
 constexpr int ML_T = 16;            // tile of the chip-wide level kernels on large levels
 constexpr int ML_TS = 8;            // ... and on levels with fewer than 128 such tiles
-constexpr int ML_SW = 3;            // Jacobi sweeps before and after the coarse correction
+constexpr int ML_SW = 4;            // smoothing steps before and after the coarse correction
+// The smoother is the Chebyshev polynomial of degree ML_SW in D^-1 A on [LMAX / ALPHA, LMAX]
+// (LMAX = 2 bounds the spectrum of D^-1 A for these diagonally dominant operators): step 0
+// d = D^-1 r / theta, step k d = a_k d + b_k D^-1 r, each followed by y += d, r = g - A y.
+constexpr double ML_LMAX = 2.0, ML_ALPHA = 32.0;
+struct Cheb {
+    double c0;          // 1 / theta
+    double a[ML_SW];    // a_k = rho_k rho_{k-1} (k >= 1)
+    double b[ML_SW];    // b_k = 2 rho_k / delta (k >= 1)
+};
+Cheb ml_cheb() {
+    Cheb c{};
+    const double lo = ML_LMAX / ML_ALPHA, hi = ML_LMAX;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    c.c0 = 1.0 / theta;
+    for (int k = 1; k < ML_SW; k++) {
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        c.a[k] = rn * rho;
+        c.b[k] = 2.0 * rn / delta;
+        rho = rn;
+    }
+    return c;
+}
 
 struct Level {
     int H, W, c;              // c: factor to the next level (0 on the coarsest)
@@ -1503,43 +1526,50 @@ __device__ __forceinline__ double ml_apply(const Level& L, const double* v, int 
 
 // Jacobi sweep y + (g - A y) / dc at level cell (i, j) from a tile of y with a halo (y[a][b]
 // at the cell's tile position; 0 outside the level)
+// Chebyshev step k at level cell (i, j) from a tile of y with a halo (y[a][b] at the cell's tile
+// position; 0 outside the level) and the cell's previous direction d: returns y + d', d' in d
 template <int HW>
-__device__ __forceinline__ double ml_sweep(const Level& L, const double (*y)[HW], int a, int b, int i, int j) {
-    const int sc = L.W + 1, k = i * sc + j;
-    const double ay = L.dc[k] * y[a][b] + L.acx[k] * y[a - 1][b] + L.acx[k + sc] * y[a + 1][b] +
-                      L.acy[k] * y[a][b - 1] + L.acy[k + 1] * y[a][b + 1];
-    return y[a][b] + (L.g[static_cast<int64_t>(i) * L.W + j] - ay) / L.dc[k];
+__device__ __forceinline__ double ml_step(const Level& L, const Cheb& ch, int k, const double (*y)[HW], double& d,
+                                          int a, int b, int i, int j) {
+    const int sc = L.W + 1, kk = i * sc + j;
+    const double ay = L.dc[kk] * y[a][b] + L.acx[kk] * y[a - 1][b] + L.acx[kk + sc] * y[a + 1][b] +
+                      L.acy[kk] * y[a][b - 1] + L.acy[kk + 1] * y[a][b + 1];
+    const double z = (L.g[static_cast<int64_t>(i) * L.W + j] - ay) / L.dc[kk];
+    d = (k == 0) ? ch.c0 * z : ch.a[k] * d + ch.b[k] * z;
+    return y[a][b] + d;
 }
 
-// Sweeps 2 .. ML_SW of a tile whose state after the first is in buf[0] on the region of halo
-// ML_SW; each sweep shrinks the valid region by one cell, so the result (buf[(ML_SW - 1) & 1])
-// is valid on halo 1.  Cells outside the level stay 0.
+// Chebyshev steps k0 .. k1 - 1 of a tile whose state is in buf[0] on the region of halo
+// ML_SW - h0 (h0 = 0: the whole tile) and whose directions are in dir; each step shrinks the
+// valid region by one cell, step k writes buf[(k - k0 + 1) & 1].  Cells outside the level stay 0.
 template <int T>
-__device__ __forceinline__ void ml_sweeps(const Level& L, double (*buf)[T + 2 * ML_SW][T + 2 * ML_SW], int i0,
-                                          int j0) {
+__device__ __forceinline__ void ml_steps(const Level& L, const Cheb& ch, double (*buf)[T + 2 * ML_SW][T + 2 * ML_SW],
+                                         double (*dir)[T + 2 * ML_SW], int k0, int k1, int h0, int i0, int j0) {
     constexpr int HW = T + 2 * ML_SW;
-#pragma unroll
-    for (int s = 1; s < ML_SW; s++) {
-        const double (*src)[HW] = buf[(s - 1) & 1];
-        double (*dst)[HW] = buf[s & 1];
-        const int lo = s, n = HW - 2 * s;  // region of halo ML_SW - s
+    for (int k = k0; k < k1; k++) {
+        const int s = k - k0;
+        const double (*src)[HW] = buf[s & 1];
+        double (*dst)[HW] = buf[(s + 1) & 1];
+        const int lo = h0 + s + 1, n = HW - 2 * lo;  // region of halo ML_SW - lo
         for (int e = threadIdx.x; e < n * n; e += T * T) {
             const int a = lo + e / n, b = lo + e % n;
             const int i = i0 - ML_SW + a, j = j0 - ML_SW + b;
-            dst[a][b] = (i >= 0 && i < L.H && j >= 0 && j < L.W) ? ml_sweep<HW>(L, src, a, b, i, j) : 0.0;
+            dst[a][b] = (i >= 0 && i < L.H && j >= 0 && j < L.W) ? ml_step<HW>(L, ch, k, src, dir[a][b], a, b, i, j)
+                                                                 : 0.0;
         }
         __syncthreads();
     }
 }
 
-// pre-smoothing and restriction on a level run by the whole chip: y1 = ML_SW Jacobi sweeps
-// from 0, next.g = block sums of g - A y1.  A block of T x T threads owns a T x T tile (whole
-// blocks of the next level: c divides T) and sweeps it with a halo of ML_SW cells.
+// pre-smoothing and restriction on a level run by the whole chip: y1 = the ML_SW Chebyshev
+// steps from 0, next.g = block sums of g - A y1.  A block of T x T threads owns a T x T tile
+// (whole blocks of the next level: c divides T) and smooths it with a halo of ML_SW cells.
 template <int T>
-__global__ __launch_bounds__(T * T) void k_ml_down(Level L, Level next, const KrylovState* st) {
+__global__ __launch_bounds__(T * T) void k_ml_down(Level L, Level next, const KrylovState* st, Cheb ch) {
     if (st->done != DONE_RUNNING) return;
     constexpr int ML_T = T, ML_HW = T + 2 * ML_SW, NT = T * T;
     __shared__ double sY[2][ML_HW][ML_HW];
+    __shared__ double sD[ML_HW][ML_HW];
     __shared__ double sR[ML_T][ML_T];
     const int tiles_j = ceil_div(L.W, ML_T), ntiles = ceil_div(L.H, ML_T) * tiles_j;
     const int c = L.c, nb = ML_T / c, sc = L.W + 1;
@@ -1548,15 +1578,17 @@ __global__ __launch_bounds__(T * T) void k_ml_down(Level L, Level next, const Kr
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int ti = t / tiles_j, tj = t - ti * tiles_j;
         const int i0 = ti * ML_T, j0 = tj * ML_T;
-        for (int e = threadIdx.x; e < ML_HW * ML_HW; e += NT) {  // sweep 1 from y = 0: g / dc
+        for (int e = threadIdx.x; e < ML_HW * ML_HW; e += NT) {  // step 0 from y = 0: d = y = c0 g / dc
             const int aa = e / ML_HW, bb = e - aa * ML_HW;
             const int i = i0 - ML_SW + aa, j = j0 - ML_SW + bb;
-            sY[0][aa][bb] = (i >= 0 && i < L.H && j >= 0 && j < L.W)
-                                ? L.g[static_cast<int64_t>(i) * L.W + j] / L.dc[i * sc + j]
-                                : 0.0;
+            const double y0 = (i >= 0 && i < L.H && j >= 0 && j < L.W)
+                                  ? ch.c0 * (L.g[static_cast<int64_t>(i) * L.W + j] / L.dc[i * sc + j])
+                                  : 0.0;
+            sY[0][aa][bb] = y0;
+            sD[aa][bb] = y0;
         }
         __syncthreads();
-        ml_sweeps<T>(L, sY, i0, j0);
+        ml_steps<T>(L, ch, sY, sD, 1, ML_SW, 0, i0, j0);
         const int i = i0 + a, j = j0 + b;
         double rv = 0.0;
         if (i < L.H && j < L.W) {
@@ -1604,22 +1636,27 @@ __device__ __forceinline__ void ml_dense(const Level& L, const double* __restric
 // levels first .. n-1 in one workgroup (the small tail of the V-cycle; g of level `first`
 // given), y of level `first`
 __global__ __launch_bounds__(1024) void k_ml_lower(Hierarchy Hh, int first, const double* __restrict__ Minv,
-                                                   const KrylovState* st) {
+                                                   const KrylovState* st, Cheb ch) {
     if (st->done != DONE_RUNNING) return;
     const int n = Hh.n;
     for (int l = first; l + 1 < n; l++) {
         const Level& L = Hh.L[l];
         const Level& N = Hh.L[l + 1];
         const int P = L.H * L.W;
-        for (int p = threadIdx.x; p < P; p += 1024) {
+        double* d = L.y;  // the Chebyshev directions (L.y is written only by the prolongation)
+        for (int p = threadIdx.x; p < P; p += 1024) {  // step 0 from y = 0
             const int i = p / L.W, j = p - i * L.W;
-            L.y1[p] = L.g[p] / L.dc[i * (L.W + 1) + j];
+            const double y0 = ch.c0 * (L.g[p] / L.dc[i * (L.W + 1) + j]);
+            L.y1[p] = y0;
+            d[p] = y0;
         }
         __syncthreads();
-        for (int s = 1; s < ML_SW; s++) {  // further sweeps through r1
+        for (int k = 1; k < ML_SW; k++) {  // further steps through r1
             for (int p = threadIdx.x; p < P; p += 1024) {
                 const int i = p / L.W, j = p - i * L.W;
-                L.r1[p] = L.y1[p] + (L.g[p] - ml_apply(L, L.y1, i, j)) / L.dc[i * (L.W + 1) + j];
+                const double z = (L.g[p] - ml_apply(L, L.y1, i, j)) / L.dc[i * (L.W + 1) + j];
+                d[p] = ch.a[k] * d[p] + ch.b[k] * z;
+                L.r1[p] = L.y1[p] + d[p];
             }
             __syncthreads();
             for (int p = threadIdx.x; p < P; p += 1024) L.y1[p] = L.r1[p];
@@ -1650,10 +1687,13 @@ __global__ __launch_bounds__(1024) void k_ml_lower(Hierarchy Hh, int first, cons
             L.r1[p] = L.y1[p] + N.y[static_cast<int64_t>(i / L.c) * N.W + j / L.c];
         }
         __syncthreads();
+        double* d = L.y1;  // the Chebyshev directions (y1 is folded into r1)
         for (int s = 0; s < ML_SW; s++) {
             for (int p = threadIdx.x; p < P; p += 1024) {
                 const int i = p / L.W, j = p - i * L.W;
-                L.y[p] = L.r1[p] + (L.g[p] - ml_apply(L, L.r1, i, j)) / L.dc[i * (L.W + 1) + j];
+                const double z = (L.g[p] - ml_apply(L, L.r1, i, j)) / L.dc[i * (L.W + 1) + j];
+                d[p] = (s == 0) ? ch.c0 * z : ch.a[s] * d[p] + ch.b[s] * z;
+                L.y[p] = L.r1[p] + d[p];
             }
             __syncthreads();
             if (s + 1 < ML_SW) {
@@ -1664,17 +1704,18 @@ __global__ __launch_bounds__(1024) void k_ml_lower(Hierarchy Hh, int first, cons
     }
 }
 
-// prolongation and post-smoothing on a level run by the whole chip: y = ML_SW Jacobi sweeps
-// from y2 = y1 + P y_next (on ML_T x ML_T tiles with the sweeps' halo), into L.y, or into yout
+// prolongation and post-smoothing on a level run by the whole chip: y = the ML_SW Chebyshev
+// steps from y2 = y1 + P y_next (on ML_T x ML_T tiles with the sweeps' halo), into L.y, or into yout
 // (FP32) on level 0, where the block partials (0, 0, y.P^T r, y.P^T q_1) of the fine
 // iteration's scalars go to pc (the step along P y is fixed at 1, so y.g and y.A_c y are not
 // needed)
 template <int T>
 __global__ __launch_bounds__(T * T) void k_ml_up(Level L, Level next, float* __restrict__ yout, const double* rc,
-                                                 const double* qc, const KrylovState* st, double* pc) {
+                                                 const double* qc, const KrylovState* st, double* pc, Cheb ch) {
     if (st->done != DONE_RUNNING) return;
     constexpr int ML_T = T, ML_HW = T + 2 * ML_SW, NT = T * T;
     __shared__ double sY[2][ML_HW][ML_HW];
+    __shared__ double sD[ML_HW][ML_HW];
     const int tiles_j = ceil_div(L.W, ML_T), ntiles = ceil_div(L.H, ML_T) * tiles_j;
     const int a = threadIdx.x / ML_T, b = threadIdx.x - a * ML_T;
     constexpr int F = (ML_SW - 1) & 1;
@@ -1691,11 +1732,12 @@ __global__ __launch_bounds__(T * T) void k_ml_up(Level L, Level next, float* __r
                                 : 0.0;
         }
         __syncthreads();
-        ml_sweeps<T>(L, sY, i0, j0);
+        ml_steps<T>(L, ch, sY, sD, 0, ML_SW - 1, 0, i0, j0);
         const int i = i0 + a, j = j0 + b;
         if (i < L.H && j < L.W) {
             const int64_t e = static_cast<int64_t>(i) * L.W + j;
-            const double y = ml_sweep<ML_HW>(L, sY[F], a + ML_SW, b + ML_SW, i, j);
+            const double y = ml_step<ML_HW>(L, ch, ML_SW - 1, sY[F], sD[a + ML_SW][b + ML_SW], a + ML_SW, b + ML_SW,
+                                            i, j);
             if (yout) {
                 const float yf = static_cast<float>(y);
                 yout[e] = yf;
@@ -1930,6 +1972,7 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
     const int mwin = learned ? FCG_M : 1;
     int wstart = 0;  // first iteration of the current window of directions
     auto nwin = [&](int k) { return std::max(0, std::min(mwin, k - wstart)); };
+    const Cheb ch = ml_cheb();  // the V-cycle's smoother
     const float* ycoarse = learned ? static_cast<const float*>(N->y) : yml;
     const int first = learned ? 0 : ml_first_lower(Hh);
     // count iterations; the last update pass leaves the state in snap for the host's poll
@@ -1953,30 +1996,30 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
                         if (ml_small_tiles(Hh.L[l], l)) {
                             const int nt = ceil_div(Hh.L[l].H, ML_TS) * ceil_div(Hh.L[l].W, ML_TS);
                             hipLaunchKernelGGL(k_ml_down<ML_TS>, dim3(std::min(nt, 4096)), dim3(ML_TS * ML_TS), 0,
-                                               ctx->stream, Hh.L[l], Hh.L[l + 1], D.st);
+                                               ctx->stream, Hh.L[l], Hh.L[l + 1], D.st, ch);
                         } else {
                             const int nt = ceil_div(Hh.L[l].H, ML_T) * ceil_div(Hh.L[l].W, ML_T);
                             hipLaunchKernelGGL(k_ml_down<ML_T>, dim3(std::min(nt, 4096)), dim3(KB), 0, ctx->stream,
-                                               Hh.L[l], Hh.L[l + 1], D.st);
+                                               Hh.L[l], Hh.L[l + 1], D.st, ch);
                         }
                     }
                     if (first == Hh.n - 1)
                         hipLaunchKernelGGL(k_ml_dense_apply, dim3(ceil_div(Hh.L[first].H * Hh.L[first].W, KB / 64)),
                                            dim3(KB), 0, ctx->stream, Hh.L[first], Minv, D.st);
                     else
-                        hipLaunchKernelGGL(k_ml_lower, dim3(1), dim3(1024), 0, ctx->stream, Hh, first, Minv, D.st);
+                        hipLaunchKernelGGL(k_ml_lower, dim3(1), dim3(1024), 0, ctx->stream, Hh, first, Minv, D.st, ch);
                     for (int l = first - 1; l >= 0; l--) {
                         if (ml_small_tiles(Hh.L[l], l)) {
                             const int nt = ceil_div(Hh.L[l].H, ML_TS) * ceil_div(Hh.L[l].W, ML_TS);
                             hipLaunchKernelGGL(k_ml_up<ML_TS>, dim3(std::min(nt, 4096)), dim3(ML_TS * ML_TS), 0,
                                                ctx->stream, Hh.L[l], Hh.L[l + 1], static_cast<float*>(nullptr), C.rc,
-                                               C.qc[0], D.st, static_cast<double*>(nullptr));
+                                               C.qc[0], D.st, static_cast<double*>(nullptr), ch);
                         } else {
                             hipLaunchKernelGGL(k_ml_up<ML_T>,
                                                dim3(l == 0 ? nbc : flat_blocks(static_cast<int64_t>(Hh.L[l].H) * Hh.L[l].W)),
                                                dim3(KB), 0, ctx->stream, Hh.L[l], Hh.L[l + 1],
                                                l == 0 ? yml : static_cast<float*>(nullptr), C.rc, C.qc[0], D.st,
-                                               l == 0 ? pc : static_cast<double*>(nullptr));
+                                               l == 0 ? pc : static_cast<double*>(nullptr), ch);
                         }
                     }
                 }
