@@ -13,8 +13,9 @@ namespace asd2 {
 
 // ------------------------------------------------------------------ WENO5
 constexpr int WJ = 64;   // tile width along j (contiguous)
-constexpr int WI = 16;   // tile height along i
+constexpr int WI = 32;   // WENO tile height along i
 constexpr int WTH = 256; // 64 x 4 threads
+constexpr int WTW = 512; // WENO block: 64 x 8 threads, 4 tile rows each
 
 // cell-centred velocity of Solve_LevelSet_2D with the P BC on its ghosts
 // (src/levelset2d.cpp:47-57); CUSTOM sides leave the ghost at 0.
@@ -41,7 +42,7 @@ __device__ __forceinline__ double vc_face(const Geom& G, const GhostRule& R, con
 // STAGE 0: write L (HJWENO5_LS_2D); 1..3: SSP-RK3 stages of Solve_LevelSet_2D.
 // CENTRED: velocities given cell-centred (with ghosts) instead of on faces.
 template <int STAGE, bool CENTRED>
-__global__ __launch_bounds__(WTH) void k_weno(Geom G, GhostRule R, const double* __restrict__ phi,
+__global__ __launch_bounds__(WTW) void k_weno(Geom G, GhostRule R, const double* __restrict__ phi,
                                               const double* phin, const double* __restrict__ u,
                                               const double* __restrict__ v, double* out, double dt,
                                               int ghosts) {
@@ -58,11 +59,11 @@ __global__ __launch_bounds__(WTH) void k_weno(Geom G, GhostRule R, const double*
     // Each thread owns the output rows o = ti + 4k of its column and loads their phi and cell-centred
     // velocities once: the products go to the flux tiles, the values stay in registers for the
     // stage update (a second read after the tile phase would miss L2 at these tile counts).
-    constexpr int NR = WI / (WTH / WJ);
+    constexpr int NR = WI / (WTW / WJ);
     double phiR[NR], ucR[NR], vcR[NR];
 #pragma unroll
     for (int k = 0; k < NR; k++) {
-        const int o = ti + (WTH / WJ) * k, i = i0 + o, j = j0 + tj;
+        const int o = ti + (WTW / WJ) * k, i = i0 + o, j = j0 + tj;
         double ph = 0.0, uc = 0.0, vc = 0.0;
         if (i < ipad && j < jpad) {
             ph = phi[G.idx(i, j)];
@@ -77,7 +78,7 @@ __global__ __launch_bounds__(WTH) void k_weno(Geom G, GhostRule R, const double*
     }
     // the three halo rows above and below (x fluxes) and the three halo columns on each side
     // (y fluxes)
-    for (int e = threadIdx.x; e < 6 * WJ; e += WTH) {
+    for (int e = threadIdx.x; e < 6 * WJ; e += WTW) {
         const int h = e / WJ, c = e - h * WJ;
         const int r = h < 3 ? h : WI + h;
         const int i = i0 - 3 + r, j = j0 + c;
@@ -88,7 +89,7 @@ __global__ __launch_bounds__(WTH) void k_weno(Geom G, GhostRule R, const double*
         }
         sFx[r][c] = f;
     }
-    for (int e = threadIdx.x; e < WI * 6; e += WTH) {
+    for (int e = threadIdx.x; e < WI * 6; e += WTW) {
         const int r = e / 6, h = e - r * 6;
         const int c = h < 3 ? h : WJ + h;
         const int i = i0 + r, j = j0 - 3 + c;
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(WTH) void k_weno(Geom G, GhostRule R, const double*
     const double epsx = 1e-6 * G.dx * G.dx, epsy = 1e-6 * G.dy * G.dy;
 #pragma unroll
     for (int k = 0; k < NR; k++) {
-        const int r = ti + (WTH / WJ) * k;
+        const int r = ti + (WTW / WJ) * k;
         const int i = i0 + r, j = j0 + tj;
         if (i >= iend || j >= jend) continue;
         double lxp, lxm, lyp, lym;
@@ -454,8 +455,8 @@ struct Launch {
     dim3 grid, block;
 };
 
-static Launch tiles(const Geom& G, int tj, int ti) {
-    return Launch{dim3(cdiv(G.ny, tj), cdiv(G.nx, ti)), dim3(WTH)};
+static Launch tiles(const Geom& G, int tj, int ti, int threads = WTH) {
+    return Launch{dim3(cdiv(G.ny, tj), cdiv(G.nx, ti)), dim3(threads)};
 }
 
 // ------------------------------------------------------------------ run diagnostics
@@ -588,7 +589,7 @@ int asd2_ls_advect(asd2_ctx* ctx, double* d_ls, const double* d_u, const double*
         ASD2_TRY(zero_async(ctx, p2, n * 8));
     }
     ASD2_TRY(bc_apply_p(ctx, d_ls));
-    const Launch L = tiles(G, WJ, WI);
+    const Launch L = tiles(G, WJ, WI, WTW);
     const int gh = fused ? 1 : 0;
     if (!fused) {
         // LS-free sides: build tmpU/tmpV with the full P BC, then run the centred variant
@@ -638,7 +639,7 @@ int asd2_ls_weno_rhs(asd2_ctx* ctx, double* d_ls, const double* d_uc, const doub
     const Geom& G = ctx->geo;
     ASD2_TRY(bc_apply_p(ctx, d_ls));
     ASD2_TRY(zero_async(ctx, d_L, static_cast<size_t>(G.size()) * 8));
-    const Launch L = tiles(G, WJ, WI);
+    const Launch L = tiles(G, WJ, WI, WTW);
     hipLaunchKernelGGL((k_weno<0, true>), L.grid, L.block, 0, ctx->stream, G, R, d_ls, d_ls, d_uc, d_vc, d_L, 0.0, 0);
     ASD2_LAUNCH_CHECK();
     return ASD2_OK;
