@@ -1003,6 +1003,8 @@ __device__ __forceinline__ bool fcg_alpha(KrylovState* st, const double* ps, int
 // The fine pass of the coarse correction for iteration `itn`, over regions of whole c x c
 // blocks: g = P^T (r - A D^-1 r), P^T r and P^T q_j of the last nwin directions into the
 // coarse arrays, partials (r.D^-1 r, (D^-1 r).q_1 .. (D^-1 r).q_FCG_M, sum (g / dc)^2).
+// The diagonal term of A D^-1 r is r itself, so r - A D^-1 r = -(off-diagonal terms of A applied
+// to D^-1 r) and the diagonal is never read.
 // UPD = false: r = r_in (the first iteration after a start or a reliable update).  UPD = true:
 // fused with the update of iteration itn - 1 (alpha = rho / p.q, x += alpha p, r_out = r_in -
 // alpha q, partials (r.r, 0) for the next decision), so the sums are those of the new residual.
@@ -1012,8 +1014,8 @@ template <bool UPD, int CC, int MW>
 __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* __restrict__ r_in,
                                                 float* __restrict__ r_out, float* __restrict__ x, Dirs Dd,
                                                 const float* __restrict__ dinv, const float* __restrict__ ax,
-                                                const float* __restrict__ ay, const float* __restrict__ dg,
-                                                KrylovState* st, const double* ps, int nbs, double* pu, int itn,
+                                                const float* __restrict__ ay, KrylovState* st, const double* ps,
+                                                int nbs, double* pu, int itn,
                                                 int nwin, double* pb, KrylovState* snap) {
     // snap (pinned host memory, or nullptr): the state as this kernel leaves it, written by the
     // thread that owns the state's updates, for the host's poll after the launch
@@ -1085,9 +1087,9 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
             if (in) {
                 const int64_t k = G.idx(i, j);
                 const double zc = sZ[a + 1][b + 1];
-                const double az = dg[k] * zc + ax[k] * static_cast<double>(sZ[a][b + 1]) +
-                                  ax[k + G.sy] * static_cast<double>(sZ[a + 2][b + 1]) +
-                                  ay[k] * static_cast<double>(sZ[a + 1][b]) + ay[k + 1] * static_cast<double>(sZ[a + 1][b + 2]);
+                const double off = ax[k] * static_cast<double>(sZ[a][b + 1]) +
+                                   ax[k + G.sy] * static_cast<double>(sZ[a + 2][b + 1]) +
+                                   ay[k] * static_cast<double>(sZ[a + 1][b]) + ay[k + 1] * static_cast<double>(sZ[a + 1][b + 2]);
                 const float rf = sR[a][b];
                 if (UPD) {
                     r_out[k] = rf;
@@ -1095,7 +1097,7 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
                     a_rr += static_cast<double>(rf) * rf;
                 }
                 rv = rf;
-                w = rv - az;
+                w = -off;
                 a_rz += zc * rv;
 #pragma unroll
                 for (int jj = 0; jj < MW; jj++)
@@ -1188,12 +1190,11 @@ __global__ __launch_bounds__(KB) void k_lp_coarse_dots(Coarse C, const float* __
 // launches k_lp_pass<UPD, c, MW> for the run-time coarsening c
 template <bool UPD, int MW>
 void launch_lp_pass(hipStream_t s, int nb, Geom G, Coarse C, const float* r_in, float* r_out, float* x, Dirs R,
-                    const float* dinv, const float* ax, const float* ay, const float* dg, KrylovState* st,
-                    const double* ps, int nbs, double* pu, int itn, int nwin, double* pb,
-                    KrylovState* snap) {
+                    const float* dinv, const float* ax, const float* ay, KrylovState* st, const double* ps,
+                    int nbs, double* pu, int itn, int nwin, double* pb, KrylovState* snap) {
 #define ASD2_LP_PASS(CV)                                                                                              \
-    hipLaunchKernelGGL((k_lp_pass<UPD, CV, MW>), dim3(nb), dim3(KB), 0, s, G, C, r_in, r_out, x, R, dinv, ax, ay, dg, \
-                       st, ps, nbs, pu, itn, nwin, pb, snap)
+    hipLaunchKernelGGL((k_lp_pass<UPD, CV, MW>), dim3(nb), dim3(KB), 0, s, G, C, r_in, r_out, x, R, dinv, ax, ay, st, \
+                       ps, nbs, pu, itn, nwin, pb, snap)
     switch (C.c) {
         case 1: ASD2_LP_PASS(1); break;
         case 2: ASD2_LP_PASS(2); break;
@@ -1212,10 +1213,12 @@ void launch_lp_pass(hipStream_t s, int nb, Geom G, Coarse C, const float* r_in, 
 // with a preconditioner that is the same in every iteration and nwin >= 1 this is PCG), each
 // dot split into its fine Jacobi part and its coarse part.  gamma: the energy-optimal step along
 // P y, or gamma_fixed when positive (a y that already carries its scale).  rho <= 0 records
-// BREAKDOWN; the host then falls back to Jacobi.  y == nullptr: Jacobi (gamma = 0).
+// BREAKDOWN; the host then falls back to Jacobi.  y == nullptr: Jacobi (gamma = 0).  The diagonal
+// of A is taken as 1 / D^-1 from the D^-1 tile that z needs anyway (the FP32 operator of the inner
+// iteration; the FP64 true residual uses the assembled diagonal).
 template <int MW, int CC>
 __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* __restrict__ ax,
-                                                const float* __restrict__ ay, const float* __restrict__ dg,
+                                                const float* __restrict__ ay,
                                                 const float* __restrict__ r, const float* __restrict__ dinv,
                                                 const float* __restrict__ y, Dirs Dd, KrylovState* st,
                                                 const double* pb, int nbb, const double* pc, int nbc, double* ps,
@@ -1310,6 +1313,7 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
     float* __restrict__ pnew = Dd.pnew;
     float* __restrict__ q = Dd.qnew;
     __shared__ float sP[FI + 2][TJ + 2];
+    __shared__ float sDg[FI][TJ];  // the diagonal of A on the tile, 1 / D^-1
     const int tj = threadIdx.x & (TJ - 1), ti = threadIdx.x / TJ;
     double acc = 0.0;
     for (int t = t0; t < t1; t++) {
@@ -1329,6 +1333,7 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
                     if (jj < nwin) val += coef[jj] * ld_p[u][jj];
             }
             sP[rr_][cc] = val;
+            if (rr_ >= 1 && rr_ <= FI && cc >= 1 && cc <= TJ) sDg[rr_ - 1][cc - 1] = __builtin_amdgcn_rcpf(ld_d[u]);
         }
         if (t + 1 < t1) load(t + 1);
         __syncthreads();
@@ -1337,7 +1342,7 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
             if (i >= ihi || j >= jhi) continue;
             const int64_t k = G.idx(i, j);
             const float pcv = sP[rr_ + 1][tj + 1];
-            const float qv = dg[k] * pcv + ax[k] * sP[rr_][tj + 1] + ax[k + G.sy] * sP[rr_ + 2][tj + 1] +
+            const float qv = sDg[rr_][tj] * pcv + ax[k] * sP[rr_][tj + 1] + ax[k + G.sy] * sP[rr_ + 2][tj + 1] +
                              ay[k] * sP[rr_ + 1][tj] + ay[k + 1] * sP[rr_ + 1][tj + 2];
             pnew[k] = pcv;
             q[k] = qv;
@@ -1351,12 +1356,12 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
 
 // launches k_lp_spmv<MW, c> for the run-time coarsening c
 template <int MW>
-void launch_lp_spmv(hipStream_t s, int nb, Geom G, Coarse C, const float* ax, const float* ay, const float* dg,
+void launch_lp_spmv(hipStream_t s, int nb, Geom G, Coarse C, const float* ax, const float* ay,
                     const float* r, const float* dinv, const float* y, Dirs Dd, KrylovState* st, const double* pb,
                     int nbb, const double* pc, int nbc, double* ps, int it, int nwin, double gamma_fixed,
                     const double* pu, int nbu) {
 #define ASD2_LP_SPMV(CV)                                                                                              \
-    hipLaunchKernelGGL((k_lp_spmv<MW, CV>), dim3(nb), dim3(KB), 0, s, G, C, ax, ay, dg, r, dinv, y, Dd, st, pb, nbb, \
+    hipLaunchKernelGGL((k_lp_spmv<MW, CV>), dim3(nb), dim3(KB), 0, s, G, C, ax, ay, r, dinv, y, Dd, st, pb, nbb,     \
                        pc, nbc, ps, it, nwin, gamma_fixed, pu, nbu)
     switch (C.c) {
         case 1: ASD2_LP_SPMV(1); break;
@@ -1980,7 +1985,7 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
         for (int k = 0; k < count; k++, it++) {
             float* r = rb[it & 1];
             if (fresh) {
-                (learned ? launch_lp_pass<false, FCG_M> : launch_lp_pass<false, 1>)(ctx->stream, nbb, G, C, r, nullptr, x, dirs_for(R, it), S.dinv32, S.ax32, S.ay32, S.diag32,
+                (learned ? launch_lp_pass<false, FCG_M> : launch_lp_pass<false, 1>)(ctx->stream, nbb, G, C, r, nullptr, x, dirs_for(R, it), S.dinv32, S.ax32, S.ay32,
                                       D.st, D.ps, nbs, nullptr, it, nwin(it), pb, nullptr);
                 fresh = false;
             }
@@ -2030,11 +2035,11 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
             }
             prof_begin(ctx, ASD2_KID_FCG_SPMV);
             (learned ? launch_lp_spmv<FCG_M> : launch_lp_spmv<1>)(
-                ctx->stream, nbs, G, C, S.ax32, S.ay32, S.diag32, r, S.dinv32, use_net ? ycoarse : nullptr,
+                ctx->stream, nbs, G, C, S.ax32, S.ay32, r, S.dinv32, use_net ? ycoarse : nullptr,
                 dirs_for(R, it), D.st, pb, nbb, pc, nbc, D.ps, it, nwin(it), learned ? 0.0 : 1.0, D.pu, nbu);
             prof_end(ctx, ASD2_KID_FCG_SPMV);
             prof_begin(ctx, ASD2_KID_FCG_UPDATE);
-            (learned ? launch_lp_pass<true, FCG_M> : launch_lp_pass<true, 1>)(ctx->stream, nbb, G, C, r, rb[(it + 1) & 1], x, dirs_for(R, it + 1), S.dinv32, S.ax32, S.ay32, S.diag32,
+            (learned ? launch_lp_pass<true, FCG_M> : launch_lp_pass<true, 1>)(ctx->stream, nbb, G, C, r, rb[(it + 1) & 1], x, dirs_for(R, it + 1), S.dinv32, S.ax32, S.ay32,
                                  D.st, D.ps, nbs, D.pu, it + 1, nwin(it + 1), pb, k + 1 == count ? snap : nullptr);
             prof_end(ctx, ASD2_KID_FCG_UPDATE);
             nbu = nbb;

@@ -35,8 +35,8 @@ BYTES_PER_CELL = {
     "pcg64_update": 8 * 8,   # x, p, r, q, dinv -> x, r, z
     "pcg32_spmv": 6 * 4,
     "pcg32_update": 8 * 4,
-    "fcg_spmv": 8 * 4,       # r, dinv, p_old, beta_x, beta_y, diag -> p_new, q (z formed on the fly)
-    "fcg_update": 10 * 4,    # x, p, r, q, dinv, beta_x, beta_y, diag -> x, r' (+ coarse sums of the new r)
+    "fcg_spmv": 7 * 4,       # r, dinv, p_old, beta_x, beta_y -> p_new, q (z formed on the fly, diag = 1/dinv)
+    "fcg_update": 9 * 4,     # x, p, r, q, dinv, beta_x, beta_y -> x, r' (+ coarse sums of the new r)
     "weno_stage": 112 / 3,   # 32 + 40 + 40 bytes over the 3 stages of an advection step
     "reinit": 28,            # MinRK2 substage A 24 B, substage B 32 B
     "gfm": 4 * 8 + 5 * 8 + 4 * 4,  # phi, kappa, H, div -> beta_x, beta_y, diag, rhs, dinv (FP64) and
@@ -47,8 +47,8 @@ BYTES_PER_CELL = {
 # 2.0 + WRITE_SIZE x 1.0: the factors that make both counters read 256 MiB each on the plain copy
 # asd2_stream_copy of 2^25 doubles); the roofline's `traffic` is this figure x cells.
 MEASURED_BYTES_PER_CELL = {
-    "pcg64_spmv": 61.5, "pcg64_update": 64.4, "pcg32_spmv": 31.2, "pcg32_update": 32.2,
-    "fcg_spmv": 36.1, "fcg_update": 46.4, "weno_stage": (44.8 + 54.9 + 54.0) / 3,
+    "pcg64_spmv": 61.6, "pcg64_update": 64.4, "pcg32_spmv": 31.4, "pcg32_update": 32.2,
+    "fcg_spmv": 31.1, "fcg_update": 41.7, "weno_stage": (41.2 + 51.8 + 51.0) / 3,
     "reinit": (29.6 + 39.6) / 2, "gfm": 100.6,
 }
 
