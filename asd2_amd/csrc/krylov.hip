@@ -1034,10 +1034,16 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
     for (int j = 0; j < MW; j++) qj[j] = Dd.q[j];
     constexpr int c = CC, RI = lp_ri(CC), nbi = RI / c, nbj = LP_RJ / c, nblk = nbi * nbj;
     constexpr int NS = 2 + MW;  // block sums: w, r, q_1 .. q_MW
+    // Wave k handles the region rows k, k + NWV, ...  With c >= 4 each lane sums its column over
+    // the RPW rows of a block that its wave handles, the segment sum adds the c columns, and the
+    // result goes to the wave's own slot (written once); the slots are added in wave order, so
+    // the sums do not depend on timing.  With c <= 2 a block gets at most two row sums, and
+    // x + y = y + x, so one slot with atomics is exact too.
+    constexpr int NWV = KB / 64, NSLOT = CC >= 4 ? NWV : 1, RPW = CC >= 4 ? CC / NWV : 1;
     __shared__ float sZ[RI + 2][LP_RJ + 2];
     __shared__ float sR[RI][LP_RJ];
     __shared__ float sQ[UPD ? RI : 1][LP_RJ];  // q of the updated direction (= q_1) on the region
-    __shared__ double sB[NS][nblk];
+    __shared__ double sB[NSLOT][NS][nblk];
     const int reg_i = ceil_div(G.nx, RI), nreg = reg_i * ceil_div(G.ny, LP_RJ);
     const int ilo = G.g, ihi = G.g + G.nx, jlo = G.g, jhi = G.g + G.ny;
     const int lane = threadIdx.x & 63;
@@ -1077,9 +1083,11 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
             }
             sZ[a][b] = v;
         }
-        for (int e = threadIdx.x; e < NS * nblk; e += KB) (&sB[0][0])[e] = 0.0;
+        if constexpr (NSLOT == 1)
+            for (int e = threadIdx.x; e < NS * nblk; e += KB) (&sB[0][0][0])[e] = 0.0;
         __syncthreads();
-        for (int e = threadIdx.x; e < RI * LP_RJ; e += KB) {
+        double cw = 0.0, cr = 0.0, cq[MW] = {};  // the lane's column sums over its rows of a block
+        for (int e = threadIdx.x, it_r = 0; e < RI * LP_RJ; e += KB, it_r++) {
             const int a = e / LP_RJ, b = e - a * LP_RJ;  // a wave covers one region row
             const int i = i0 + a, j = j0 + b;
             const bool in = i < ihi && j < jhi;
@@ -1106,19 +1114,36 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
                         a_zq[jj] += zc * qv[jj];
                     }
             }
-            w = seg_sum(w, c);
-            rv = seg_sum(rv, c);
+            cw += w;
+            cr += rv;
+#pragma unroll
+            for (int jj = 0; jj < MW; jj++) cq[jj] += qv[jj];
+            if ((it_r + 1) % RPW != 0) continue;  // more rows of this block for this wave
+            cw = seg_sum(cw, c);
+            cr = seg_sum(cr, c);
 #pragma unroll
             for (int jj = 0; jj < MW; jj++)
-                if (jj < nwin) qv[jj] = seg_sum(qv[jj], c);
-            if ((lane & (c - 1)) == 0 || c >= 64) {
+                if (jj < nwin) cq[jj] = seg_sum(cq[jj], c);
+            if ((lane & (c - 1)) == 0) {
                 const int blk = (a / c) * nbj + b / c;
-                atomicAdd(&sB[0][blk], w);
-                atomicAdd(&sB[1][blk], rv);
+                if constexpr (NSLOT > 1) {
+                    double (*sw)[nblk] = sB[threadIdx.x >> 6];  // this wave's slot
+                    sw[0][blk] = cw;
+                    sw[1][blk] = cr;
 #pragma unroll
-                for (int jj = 0; jj < MW; jj++)
-                    if (jj < nwin) atomicAdd(&sB[2 + jj][blk], qv[jj]);
+                    for (int jj = 0; jj < MW; jj++)
+                        if (jj < nwin) sw[2 + jj][blk] = cq[jj];
+                } else {
+                    atomicAdd(&sB[0][0][blk], cw);
+                    atomicAdd(&sB[0][1][blk], cr);
+#pragma unroll
+                    for (int jj = 0; jj < MW; jj++)
+                        if (jj < nwin) atomicAdd(&sB[0][2 + jj][blk], cq[jj]);
+                }
             }
+            cw = cr = 0.0;
+#pragma unroll
+            for (int jj = 0; jj < MW; jj++) cq[jj] = 0.0;
         }
         __syncthreads();
         for (int e = threadIdx.x; e < nblk; e += KB) {
@@ -1126,12 +1151,17 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
             const int I = (i0 - ilo) / c + bi, J = (j0 - jlo) / c + bj;
             if (I < C.Hc && J < C.Wc) {
                 const int64_t pc = static_cast<int64_t>(I) * C.Wc + J;
-                const double gv = sB[0][e];
+                auto slots = [&](int k) {
+                    double t = sB[0][k][e];
+                    for (int sl = 1; sl < NSLOT; sl++) t += sB[sl][k][e];
+                    return t;
+                };
+                const double gv = slots(0);
                 C.g[pc] = gv;
-                C.rc[pc] = sB[1][e];
+                C.rc[pc] = slots(1);
 #pragma unroll
                 for (int jj = 0; jj < MW; jj++)
-                    if (jj < nwin) C.qc[jj][pc] = sB[2 + jj][e];
+                    if (jj < nwin) C.qc[jj][pc] = slots(2 + jj);
                 const double u = gv / C.dc[I * (C.Wc + 1) + J];
                 a_uu += u * u;
             }

@@ -56,3 +56,21 @@ def test_multilevel_rejects_fp64():
     ctx.gfm_assemble(A.make_fluid(1000, 1, 1, 0.01, 0.3, 1e-3), *[ctx.array() for _ in range(4)])
     with pytest.raises(A.Asd2Error):
         ctx.poisson_solve(ctx.array(), ctx.opts(precond=A.PRECOND_MULTILEVEL, precision=A.PREC_FP64))
+
+
+def test_multilevel_deterministic(oracle):
+    """Two solves of the same system give the same bits: every reduction of the solve sums in a
+    fixed order (block partials re-summed in block order, coarse block sums in wave order)."""
+    gr, fl, phi, kap, H, div = gfm_case(oracle, 256, 192, seed=8, rho=(1000.0, 1.0), sigma=0.3)
+    ctx = A.Context(gr.nx, gr.ny, gr.dx, gr.dy)
+    ctx.set_bc(A.GRID_P, ("neumann",) * 4)
+    flg = A.make_fluid(fl.rho_h, fl.rho_l, fl.mu_h, fl.mu_l, fl.sigma, fl.dt)
+    ctx.gfm_assemble(flg, *[ctx.array(a) for a in (phi, kap, H, div)])
+    runs = []
+    for _ in range(3):
+        ps = ctx.array()
+        st = ctx.poisson_solve(ps, ctx.opts(tol=1e-10, precond=A.PRECOND_MULTILEVEL, precision=A.PREC_MIXED))
+        runs.append((st.iterations, st.true_rel_residual, ps.download()))
+    for it, res, x in runs[1:]:
+        assert it == runs[0][0] and res == runs[0][1]
+        np.testing.assert_array_equal(x, runs[0][2])
