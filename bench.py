@@ -47,8 +47,8 @@ BYTES_PER_CELL = {
 # 2.0 + WRITE_SIZE x 1.0: the factors that make both counters read 256 MiB each on the plain copy
 # asd2_stream_copy of 2^25 doubles); the roofline's `traffic` is this figure x cells.
 MEASURED_BYTES_PER_CELL = {
-    "pcg64_spmv": 61.6, "pcg64_update": 64.4, "pcg32_spmv": 31.4, "pcg32_update": 32.2,
-    "fcg_spmv": 31.1, "fcg_update": 41.7, "weno_stage": (41.2 + 51.8 + 51.0) / 3,
+    "pcg64_spmv": 61.6, "pcg64_update": 64.4, "pcg32_spmv": 31.3, "pcg32_update": 32.2,
+    "fcg_spmv": 31.0, "fcg_update": 41.4, "weno_stage": (41.3 + 51.3 + 51.1) / 3,
     "reinit": (29.6 + 39.6) / 2, "gfm": 100.6,
 }
 
