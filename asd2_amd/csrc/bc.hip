@@ -11,19 +11,15 @@ struct BcArgs {
 };
 
 // ------------------------------------------------------------------ P grid
-// One thread per boundary line; each thread fills the g ghosts of its line in the
-// reference's sequential order (needed by the LS-free extrapolation, bc2d.cpp:896-934).
-// W/E lines only cover interior j and S/N lines interior i: corner ghosts are never
-// written (bc2d.cpp:810-832), so the four sides are independent.
-__global__ __launch_bounds__(256) void k_bc_p(Geom G, BcArgs A, double* __restrict__ a) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// One thread per grid line: a j-line (interior j) gets its W ghosts, then its E ghosts; an
+// i-line (interior i) its S ghosts, then its N ghosts, each in the reference's sequential order
+// (needed by the LS-free extrapolation, bc2d.cpp:896-934), so the fill follows the order W, E,
+// S, N of ApplyBC_P_2D (bc2d.cpp:346-353) on every line even where a side could reach the
+// opposite side's ghosts (grids below g cells are refused at context creation); the lines of
+// different threads never overlap (W/E lines cover interior j, S/N lines interior i, corner
+// ghosts are never written, bc2d.cpp:810-832).
+__device__ __forceinline__ void bc_p_side(const Geom& G, const BcArgs& A, double* __restrict__ a, int side, int line) {
     const int nx = G.nx, ny = G.ny, g = G.g;
-    int side, line;
-    if (t < ny) { side = 0; line = t; }
-    else if (t < 2 * ny) { side = 1; line = t - ny; }
-    else if (t < 2 * ny + nx) { side = 2; line = t - 2 * ny; }
-    else if (t < 2 * ny + 2 * nx) { side = 3; line = t - 2 * ny - nx; }
-    else return;
     const int k = A.kind[side];
     const double c = A.c[side];
     const bool neu = (k == ASD2_BC_NEUMANN || k == ASD2_BC_AXISYM || k == ASD2_BC_WALL);
@@ -79,6 +75,17 @@ __global__ __launch_bounds__(256) void k_bc_p(Geom G, BcArgs A, double* __restri
     }
 }
 
+__global__ __launch_bounds__(256) void k_bc_p(Geom G, BcArgs A, double* __restrict__ a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < G.ny) {
+        bc_p_side(G, A, a, 0, t);
+        bc_p_side(G, A, a, 1, t);
+    } else if (t < G.ny + G.nx) {
+        bc_p_side(G, A, a, 2, t - G.ny);
+        bc_p_side(G, A, a, 3, t - G.ny);
+    }
+}
+
 int bc_apply_p(asd2_ctx* ctx, double* d) {
     for (int s = 0; s < 4; s++) {
         if (ctx->bc[ASD2_GRID_P][s] == ASD2_BC_PERIODIC) {
@@ -88,7 +95,7 @@ int bc_apply_p(asd2_ctx* ctx, double* d) {
     }
     BcArgs A;
     for (int s = 0; s < 4; s++) { A.kind[s] = ctx->bc[ASD2_GRID_P][s]; A.c[s] = ctx->bcc[ASD2_GRID_P][s]; }
-    const int lines = 2 * (ctx->geo.nx + ctx->geo.ny);
+    const int lines = ctx->geo.nx + ctx->geo.ny;
     hipLaunchKernelGGL(k_bc_p, dim3(cdiv(lines, 256)), dim3(256), 0, ctx->stream, ctx->geo, A, d);
     ASD2_LAUNCH_CHECK();
     return ASD2_OK;

@@ -557,7 +557,7 @@ int p_rule(asd2_ctx* ctx, GhostRule* R, bool* fused) {
             return ASD2_ERR_UNSUPPORTED;
         }
     }
-    *R = ghost_rule_p(ctx, fused);
+    *R = ghost_rule_p(ctx, fused);  // (grids are at least g cells wide: every mirror is interior)
     return ASD2_OK;
 }
 
