@@ -132,6 +132,7 @@ def load() -> C.CDLL:
         "asd2_poisson_solve": ([_VP, P(SolveOpts), _VP, P(SolveStats)], C.c_int),
         "asd2_poisson_bicgstab": ([_VP, P(SolveOpts), _VP, P(SolveStats)], C.c_int),
         "asd2_csr_pcg": ([_VP, C.c_int, _VP, _VP, _VP, _VP, _VP, _VP, P(SolveOpts), P(SolveStats)], C.c_int),
+        "asd2_csr_gs": ([_VP, C.c_int, _VP, _VP, _VP, _VP, _VP, C.c_int, P(SolveStats)], C.c_int),
         "asd2_csr_bicgstab": ([_VP, C.c_int, _VP, _VP, _VP, _VP, _VP, _VP, P(SolveOpts), P(SolveStats)],
                               C.c_int),
         "asd2_ls_reinit_original": ([_VP, _VP, C.c_int], C.c_int),
@@ -491,6 +492,21 @@ class Context:
         d_x = DeviceArray((n,))
         st = SolveStats()
         check(fn(self.h, n, d_rp.ptr, d_c.ptr, d_v.ptr, d_m.ptr, d_b.ptr, d_x.ptr, C.byref(opts), C.byref(st)))
+        return d_x.download(), Stats(**st.as_dict())
+
+    def csr_gs(self, rowptr, cols, vals, b, x0, max_iter):
+        """PoissonSolver2D::GS_2FUniform_2D (src/poisson2d.cpp:7-88) on a zero-based CSR matrix
+        from the start vector x0: returns (x, stats); stats.iterations = sweeps,
+        stats.rel_residual = the last relative change."""
+        n = len(b)
+        d_rp = DeviceArray.from_host(np.asarray(rowptr, np.int32))
+        d_c = DeviceArray.from_host(np.asarray(cols, np.int32))
+        d_v = DeviceArray.from_host(np.asarray(vals, np.float64))
+        d_b = DeviceArray.from_host(np.asarray(b, np.float64))
+        d_x = DeviceArray.from_host(np.asarray(x0, np.float64))
+        st = SolveStats()
+        check(self.lib.asd2_csr_gs(self.h, n, d_rp.ptr, d_c.ptr, d_v.ptr, d_b.ptr, d_x.ptr, int(max_iter),
+                                   C.byref(st)))
         return d_x.download(), Stats(**st.as_dict())
 
     # timing --------------------------------------------------------------------------

@@ -207,15 +207,47 @@ int PoissonSolver2D::BiCGStab_2FUniform_2D(std::vector<double>& ps, const std::v
     return 0;
 }
 
-// Gauss-Seidel/SOR and the MKL FFT Helmholtz solver are not part of the GPU path.
-int PoissonSolver2D::GS_2FUniform_2D(std::vector<double>&, const std::vector<double>&, std::vector<double>&,
-                                     std::vector<MKL_INT>&, std::vector<MKL_INT>&, const double, const double,
-                                     const double, const double, const std::shared_ptr<BoundaryCondition2D>&,
-                                     const int) {
-    std::cerr << "PoissonSolver2D::GS_2FUniform_2D is not available on the GPU path (use CG or BiCGStab)"
-              << std::endl;
-    return -1;
+// Gauss-Seidel sweeps with relaxation 1.2 to a relative change of 1e-6 (src/poisson2d.cpp:7-88):
+// the unknowns are the nx*ny interior cells, i + j nx, starting from ps; the caller's CSR has a row
+// per unknown.
+int PoissonSolver2D::GS_2FUniform_2D(std::vector<double>& ps, const std::vector<double>& rhs,
+                                     std::vector<double>& AVals, std::vector<MKL_INT>& ACols,
+                                     std::vector<MKL_INT>& ARowIdx, const double, const double, const double,
+                                     const double, const std::shared_ptr<BoundaryCondition2D>&, const int maxIter) {
+    const int n = kNx * kNy;
+    if (static_cast<int>(ARowIdx.size()) - 1 < n) {
+        std::cerr << "GS_2FUniform_2D: the matrix has " << ARowIdx.size() - 1 << " rows, " << n << " needed"
+                  << std::endl;
+        return -1;
+    }
+    std::vector<double> b(static_cast<size_t>(n)), x(static_cast<size_t>(n));
+    for (int i = 0; i < kNx; i++)
+        for (int j = 0; j < kNy; j++) {
+            b[i + static_cast<size_t>(j) * kNx] = rhs[idx(i + kNumBCGrid, j + kNumBCGrid)];
+            x[i + static_cast<size_t>(j) * kNx] = ps[idx(i + kNumBCGrid, j + kNumBCGrid)];
+        }
+    DeviceCSR D;
+    D.n = n;
+    D.rp = upload(ARowIdx.data(), static_cast<size_t>(n) + 1);
+    D.cols = upload(ACols.data(), ACols.size());
+    D.vals = upload(AVals.data(), AVals.size());
+    D.b = upload(b.data(), b.size());
+    D.x = upload(x.data(), x.size());
+    asd2_solve_stats st{};
+    if (asd2_csr_gs(Context(), n, D.rp, D.cols, D.vals, D.b, D.x, maxIter, &st) != ASD2_OK) fatal("GS_2FUniform_2D");
+    check(asd2_memcpy_d2h(x.data(), D.x, static_cast<size_t>(n) * 8), "download");
+    m_lastIters = st.iterations;
+    m_lastRes = st.rel_residual;
+    for (int i = 0; i < kNx; i++)
+        for (int j = 0; j < kNy; j++) ps[idx(i + kNumBCGrid, j + kNumBCGrid)] = x[i + static_cast<size_t>(j) * kNx];
+    for (int i = kNumBCGrid; i < kNx + kNumBCGrid; i++)
+        for (int j = kNumBCGrid; j < kNy + kNumBCGrid; j++)
+            if (std::isnan(ps[idx(i, j)]) || std::isinf(ps[idx(i, j)]))
+                std::cout << "Pseudo-p nan/inf error : " << i << " " << j << " " << ps[idx(i, j)] << std::endl;
+    return 0;
 }
+
+// The MKL FFT Helmholtz solver is not part of the GPU path.
 
 int PoissonSolver2D::MKL_2FUniform_2D(std::vector<double>&, const std::vector<double>&, double, double, double,
                                       double, std::shared_ptr<BoundaryCondition2D>) {

@@ -242,6 +242,16 @@ int asd2_csr_bicgstab(asd2_ctx* ctx, int n, const int* d_rowptr, const int* d_co
                       const double* d_vals, const double* d_mdiag, const double* d_b, double* d_x,
                       const asd2_solve_opts* opts, asd2_solve_stats* stats);
 
+/* PoissonSolver2D::GS_2FUniform_2D on a caller-built zero-based CSR matrix (src/poisson2d.cpp:7-88):
+ * row-order Gauss-Seidel sweeps, each followed by x = x_old + 1.2 (x - x_old), until the relative
+ * change sqrt(sum (x - x_old)^2 / (sum x_old^2 + 1e-100)) <= 1e-6 or max_iter sweeps.  d_x holds
+ * the start vector and receives the result; stats->iterations = sweeps, stats->rel_residual = the
+ * last relative change, converged = 1 when it is <= 1e-6 (true_rel_residual = -1: not evaluated).
+ * The sweeps equal the sequential ones bit for bit (rows run level by level of their row-order
+ * dependencies); a row without a diagonal entry divides by 0. */
+int asd2_csr_gs(asd2_ctx* ctx, int n, const int* d_rowptr, const int* d_cols, const double* d_vals,
+                const double* d_b, double* d_x, int max_iter, asd2_solve_stats* stats);
+
 /* ---------------------------------------------------------------- learned preconditioner
  * No reference counterpart: the reference preconditions its CG with the Jacobi diagonal only
  * (src/poisson2d.cpp:346-347).  A 4-level U-Net (base 16 channels, 3x3, BF16 weights and

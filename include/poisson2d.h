@@ -2,8 +2,9 @@
 //
 // Same CSR-based entry points; the Krylov iterations run on the GPU (asd2_csr_pcg and the
 // CSR BiCGStab of asd2_hip.h).  MKL is not used: MKL_INT is a 32-bit int as in MKL's LP64
-// interface.  The FFT Helmholtz solver (MKL_2FUniform_2D) and Gauss-Seidel
-// (GS_2FUniform_2D) are outside the GPU path and report an error.
+// interface.  GS_2FUniform_2D runs the reference's Gauss-Seidel sweeps on the GPU
+// (asd2_csr_gs); the FFT Helmholtz solver (MKL_2FUniform_2D) is outside the GPU path and
+// reports an error.
 #ifndef ASD2_POISSON2D_H
 #define ASD2_POISSON2D_H
 

@@ -136,6 +136,8 @@ class Oracle:
         L.or_bicgstab_stencil.restype = C.c_int
         L.or_pcg_csr.argtypes = [C.c_int, IP, IP, _P, _P, _P, _P, C.c_double, C.c_int, C.c_int, IP, _P]
         L.or_pcg_csr.restype = C.c_int
+        L.or_gs_csr.argtypes = [C.c_int, IP, IP, _P, _P, _P, C.c_int, IP, _P]
+        L.or_gs_csr.restype = C.c_int
 
     # --- boundary conditions (bc2d.cpp) ---------------------------------------------
     def bc_p(self, gr, a):
@@ -283,6 +285,20 @@ class Oracle:
         st = self.lib.or_pcg_csr(len(b), rp.ctypes.data_as(IP), cl.ctypes.data_as(IP), _ptr(vals),
                                  _ptr(mdiag), _ptr(b), _ptr(x), tol, maxit, mode, C.byref(it), C.byref(rr))
         return x, it.value, rr.value, st
+
+    def gs_csr(self, rowptr, cols, vals, b, x0, maxit):
+        """GS_2FUniform_2D (src/poisson2d.cpp:7-88) on a zero-based CSR matrix from x0:
+        returns (x, sweeps, last relative change)."""
+        IP = C.POINTER(C.c_int)
+        rp = np.ascontiguousarray(rowptr, dtype=np.int32)
+        cl = np.ascontiguousarray(cols, dtype=np.int32)
+        x = np.array(x0, dtype=np.float64, copy=True)
+        it = C.c_int(0); err = C.c_double(0.0)
+        st = self.lib.or_gs_csr(len(b), rp.ctypes.data_as(IP), cl.ctypes.data_as(IP), _ptr(vals), _ptr(b),
+                                _ptr(x), maxit, C.byref(it), C.byref(err))
+        if st != 0:
+            raise MemoryError("or_gs_csr")
+        return x, it.value, err.value
 
 
 class Reference:

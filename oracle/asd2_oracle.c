@@ -19,6 +19,8 @@
  *     symmetry and positive definiteness of the intended operator.
  *   - CG / BiCGStab: scipy sparse direct solve of the same operator, finite
  *     termination on tiny grids.
+ *   - Gauss-Seidel (GS_2FUniform_2D): one sweep of a 2 x 2 system against hand
+ *     arithmetic; fixed point = the direct solve.
  *   - UpdateVel: constant-curvature droplet leaves u* unchanged (balanced jump).
  *   - Momentum predictor (convection, viscosity, gravity, RK): parity unpinned
  *     beyond closed forms (uniform flow has zero convection; linear shear has zero
@@ -1433,4 +1435,43 @@ void or_ls_diagnostics(const or_grid* gr, const double* ls, const double* ref, d
     out[8] = ref ? sqrt(l2) / ((double)nx * ny) : 0.0;
     out[9] = ref ? rc : 0.0;
     out[10] = (ref && rc > 0.0) ? fabs(cells - rc) / rc * 100.0 : 0.0;
+}
+
+/* PoissonSolver2D::GS_2FUniform_2D (src/poisson2d.cpp:7-88): Gauss-Seidel sweeps in row order
+ * (x[c] for c < r already updated in the sweep), then x = x_old + 1.2 (x - x_old) over the whole
+ * vector; stop when sqrt(sum (x - x_old)^2 / (sum x_old^2 + 1e-100)) <= 1e-6 or after maxit
+ * sweeps.  x holds the start vector.  Unlike the reference, a row without a diagonal entry
+ * divides by 0 instead of by the previous row's diagonal. */
+int or_gs_csr(int n, const int* rowptr, const int* cols, const double* vals, const double* b,
+              double* x, int maxit, int* iters, double* rel_change) {
+    double* old = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    if (!old) return -1;
+    for (int i = 0; i < n; i++) old[i] = x[i];
+    const double eps = 1.0e-6, omega = 1.2;
+    double err = 1.0;
+    int it = 0;
+    while (err > eps && it < maxit) {
+        for (int i = 0; i < n; i++) {
+            double t = b[i], diag = 0.0;
+            for (int p = rowptr[i]; p < rowptr[i + 1]; p++) {
+                const int c = cols[p];
+                if (c == i) diag = vals[p];
+                else t -= vals[p] * x[c];
+            }
+            x[i] = t / diag;
+        }
+        for (int i = 0; i < n; i++) x[i] = old[i] + omega * (x[i] - old[i]);
+        double up = 0.0, lo = 0.0;
+        for (int i = 0; i < n; i++) {
+            up += (x[i] - old[i]) * (x[i] - old[i]);
+            lo += old[i] * old[i];
+            old[i] = x[i];
+        }
+        err = sqrt(up / (lo + 1.0e-100));
+        it++;
+    }
+    free(old);
+    *iters = it;
+    *rel_change = err;
+    return 0;
 }

@@ -122,6 +122,10 @@ int or_pcg_stencil(const or_grid* gr, const double* ax, const double* ay, const 
 int or_pcg_csr(int n, const int* rowptr, const int* cols, const double* vals,
                const double* mdiag, const double* b, double* x, double tol, int maxit,
                int mode, int* iters, double* relres);
+/* GS_2FUniform_2D (src/poisson2d.cpp:7-88): Gauss-Seidel sweeps + global relaxation 1.2,
+ * relative change 1e-6; x in: start vector, out: result. */
+int or_gs_csr(int n, const int* rowptr, const int* cols, const double* vals, const double* b,
+              double* x, int maxit, int* iters, double* rel_change);
 int or_bicgstab_stencil(const or_grid* gr, const double* ax, const double* ay, const double* diag,
                         const double* b, double* x, double tol, int maxit,
                         int* iters, double* relres);

@@ -82,3 +82,21 @@ def scatter(gr, x, ids):
     m = ids >= 0
     I[m] = x[ids[m]]
     return out
+
+
+def random_spd_csr(n, seed):
+    """Irregular sparse SPD matrix: a random graph Laplacian plus a positive diagonal shift."""
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for r in range(n):
+        for c in rng.choice(n, size=3, replace=False):
+            if c != r:
+                rows += [r, c]
+                cols += [c, r]
+    W = sp.csr_matrix((rng.uniform(0.5, 2.0, len(rows)), (rows, cols)), shape=(n, n))
+    W = (W + W.T) * 0.5
+    A = sp.diags(np.asarray(W.sum(axis=1)).ravel() + rng.uniform(0.1, 1.0, n)) - W
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    return A
+

@@ -7,7 +7,7 @@ import pytest
 import scipy.sparse.linalg as spla
 
 import asd2_amd as A
-from tests.gfm_common import gfm_case, interior, rhs_vector, scatter, stencil_to_csr
+from tests.gfm_common import gfm_case, interior, random_spd_csr, rhs_vector, scatter, stencil_to_csr
 
 pytestmark = pytest.mark.gpu
 
@@ -172,3 +172,31 @@ def test_solver_errors():
     with pytest.raises(A.Asd2Error) as e:
         ctx.poisson_solve(ctx.array(), ctx.opts())
     assert e.value.code == A.ERR_STATE
+
+
+@pytest.mark.parametrize("case", ["gfm-5pt", "random-spd", "sweep-limit"])
+def test_csr_gauss_seidel_matches_oracle(oracle, case):
+    """GS_2FUniform_2D (src/poisson2d.cpp:7-88) through asd2_csr_gs: the same number of sweeps and
+    the same iterate as the oracle's sequential sweeps (rows run level by level on the GPU)."""
+    rng = np.random.default_rng(7)
+    if case == "gfm-5pt":
+        bc, c_p = ("neumann", "neumann", "dirichlet", "neumann"), (0.0, 0.0, 0.5, 0.0)
+        gr, fl, phi, kap, H, div = gfm_case(oracle, 24, 20, seed=4, bc_p=bc, c_p=c_p, rho=(10.0, 1.0))
+        ax, ay, dg, rhs = oracle.gfm_assemble(gr, fl, phi, kap, H, div)
+        Acsr, ids = stencil_to_csr(gr, ax, ay, dg, pin=False)
+        b = rhs_vector(gr, rhs, ids)
+        maxit = 100000
+    else:
+        Acsr = random_spd_csr(300, 3)
+        b = rng.standard_normal(300)
+        maxit = 100000 if case == "random-spd" else 7
+    x0 = 0.1 * rng.standard_normal(len(b))
+    x_ref, it_ref, err_ref = oracle.gs_csr(Acsr.indptr, Acsr.indices, Acsr.data, b, x0, maxit)
+    ctx = A.Context(16, 16, 0.1, 0.1)
+    x, st = ctx.csr_gs(Acsr.indptr, Acsr.indices, Acsr.data, b, x0, maxit)
+    print(f"{case}: {st.iterations} sweeps (oracle {it_ref}), change {st.rel_residual:.3e} (oracle {err_ref:.3e}), "
+          f"max|dx| {np.abs(x - x_ref).max():.2e}")
+    assert st.iterations == it_ref
+    assert np.abs(x - x_ref).max() <= 1e-14 * np.abs(x_ref).max()
+    assert st.converged == (1 if err_ref <= 1e-6 else 0)
+

@@ -8,7 +8,7 @@ import scipy.sparse.linalg as spla
 
 from oracle import make_fluid, make_grid
 from tests import scenarios as S
-from tests.gfm_common import gfm_case, interior, rhs_vector, scatter, stencil_to_csr
+from tests.gfm_common import gfm_case, interior, random_spd_csr, rhs_vector, scatter, stencil_to_csr
 
 
 # ----------------------------------------------------------------- WENO5 / advection
@@ -383,3 +383,36 @@ def test_ls_diagnostics_closed_forms(oracle):
     ell = 1.0 - np.sqrt(((Xp - 0.5) / 0.3) ** 2 + ((Yp - 0.5) / 0.15) ** 2)
     ell *= 0.15  # roughly distance-scaled near the minor axis
     assert oracle.ls_diagnostics(gr, np.ascontiguousarray(ell))["circularity"] < 0.95
+
+
+def test_gs_one_sweep_by_hand(oracle):
+    """GS_2FUniform_2D (src/poisson2d.cpp:7-88) on [[4, 1], [1, 3]] x = [1, 2] from 0: the sweep
+    gives 1/4 and (2 - 1/4)/3 = 7/12, the relaxation 1.2 times that: (0.3, 0.7)."""
+    x, it, err = oracle.gs_csr([0, 2, 4], [0, 1, 0, 1], np.array([4.0, 1.0, 1.0, 3.0]), np.array([1.0, 2.0]),
+                               np.zeros(2), 1)
+    assert it == 1 and err > 1e6
+    np.testing.assert_allclose(x, [0.3, 0.7], rtol=1e-15)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_gs_sweeps_equal_triangular_solves(oracle, seed):
+    """Each sweep is the forward substitution (D + L) x' = b - U x (a dense triangular solve here),
+    then x = x + 1.2 (x' - x); and a converged run is close to the direct solution."""
+    import scipy.linalg as sl
+    A = random_spd_csr(40, seed)
+    rng = np.random.default_rng(seed + 10)
+    b = rng.standard_normal(40)
+    x0 = rng.standard_normal(40)
+    Ad = A.toarray()
+    L, U = np.tril(Ad), np.triu(Ad, 1)
+    x = x0.copy()
+    for _ in range(5):
+        xs = sl.solve_triangular(L, b - U @ x, lower=True)
+        x = x + 1.2 * (xs - x)
+    got, it, _ = oracle.gs_csr(A.indptr, A.indices, A.data, b, x0, 5)
+    assert it == 5
+    np.testing.assert_allclose(got, x, rtol=1e-12, atol=1e-12 * np.abs(x).max())
+    xc, itc, errc = oracle.gs_csr(A.indptr, A.indices, A.data, b, x0, 100000)
+    assert errc <= 1e-6 and itc < 100000
+    xd = np.linalg.solve(Ad, b)
+    assert np.linalg.norm(xc - xd) / np.linalg.norm(xd) <= 1e-4
