@@ -2362,15 +2362,17 @@ namespace asd2 {
 // reads x[c] of this sweep for c < r and of the previous sweep for c > r.  Rows are taken level
 // by level, level(r) = 1 + max level(c) over c < r in row r, so every c < r a row reads was
 // finished in an earlier level; x_old keeps the previous sweep for c > r.  The sweep therefore
-// gives the sequential sweep's values bit for bit.  One workgroup runs every sweep (levels
+// gives the sequential sweep's values bit for bit.  One workgroup runs the sweeps (levels
 // separated by barriers), the relaxation x = x_old + 1.2 (x - x_old) and the relative change
-// (block sum, fixed order), until the change is <= 1e-6 or max_iter sweeps.  out[0] = sweeps,
+// (block sum, fixed order), until the change is <= 1e-6 or `sweeps` sweeps (the host bounds the
+// work of one launch and launches again while the change is above 1e-6; x equals x_old after
+// every relaxation, so a launch can start where the last one stopped).  out[0] = sweeps run,
 // out[1] = the last relative change.
 __global__ __launch_bounds__(KB) void k_csr_gs(int n, const int* __restrict__ rp, const int* __restrict__ cols,
                                                const double* __restrict__ vals, const double* __restrict__ b,
                                                double* __restrict__ x, double* __restrict__ xo,
                                                const int* __restrict__ perm, const int* __restrict__ lstart,
-                                               int nlev, int max_iter, double* __restrict__ out) {
+                                               int nlev, int sweeps, double* __restrict__ out) {
 #pragma clang fp contract(off)
     const double eps = 1.0e-6, omega = 1.2;
     __shared__ double s_err;
@@ -2378,7 +2380,7 @@ __global__ __launch_bounds__(KB) void k_csr_gs(int n, const int* __restrict__ rp
     __syncthreads();
     double err = 1.0;
     int it = 0;
-    while (err > eps && it < max_iter) {
+    while (err > eps && it < sweeps) {
         for (int l = 0; l < nlev; l++) {
             for (int q = lstart[l] + static_cast<int>(threadIdx.x); q < lstart[l + 1]; q += KB) {
                 const int r = perm[q];
@@ -2473,14 +2475,23 @@ int asd2_csr_gs(asd2_ctx* ctx, int n, const int* d_rowptr, const int* d_cols, co
     ASD2_TRY(ws(ctx, "gs_out", 2, &out));
     ASD2_HIP(hipMemcpyAsync(d_perm, perm.data(), perm.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     ASD2_HIP(hipMemcpyAsync(d_lstart, lstart.data(), lstart.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_csr_gs, dim3(1), dim3(KB), 0, ctx->stream, n, d_rowptr, d_cols, d_vals, d_b, d_x, xo, d_perm,
-                       d_lstart, nlev, max_iter, out);
-    ASD2_LAUNCH_CHECK();
-    double h[2];
-    ASD2_HIP(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
-    ASD2_HIP(hipStreamSynchronize(ctx->stream));
+    // sweeps per launch: about 2^27 row entries and barrier slots of work
+    const double per_sweep = static_cast<double>(nnz) + 256.0 * nlev + n;
+    const int chunk = static_cast<int>(std::max(1.0, std::min(1.0e9, 134217728.0 / per_sweep)));
+    int total = 0;
+    double h[2] = {0.0, 1.0};
+    while (h[1] > 1.0e-6 && total < max_iter) {
+        const int k = std::min(chunk, max_iter - total);
+        hipLaunchKernelGGL(k_csr_gs, dim3(1), dim3(KB), 0, ctx->stream, n, d_rowptr, d_cols, d_vals, d_b, d_x, xo,
+                           d_perm, d_lstart, nlev, k, out);
+        ASD2_LAUNCH_CHECK();
+        ASD2_HIP(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        ASD2_HIP(hipStreamSynchronize(ctx->stream));
+        total += static_cast<int>(h[0]);
+        if (!(h[1] == h[1])) break;  // a NaN change (zero diagonal) ends the run as in the sweep loop
+    }
     if (stats) {
-        stats->iterations = static_cast<int>(h[0]);
+        stats->iterations = total;
         stats->reliable_updates = 0;
         stats->fallbacks = 0;
         stats->converged = h[1] <= 1.0e-6 ? 1 : 0;
