@@ -145,7 +145,13 @@ def test_heavisides(oracle):
     eps = 1.5 * 0.05
     assert np.all(Hs[phi > eps] == 1.0) and np.all(Hs[phi < -eps] == 0.0)
     mid = np.abs(phi) <= eps
-    np.testing.assert_allclose(Hs[mid] + Hs[mid][::-1], 1.0, atol=1e-15) if False else None
+    # the smoothed Heaviside is odd about 1/2: Hs(phi) + Hs(-phi) = 1
+    sym = np.ascontiguousarray(np.concatenate([phi[mid], -phi[mid]]).reshape(1, -1))
+    grs = make_grid(1, sym.shape[1] - 6, 0.05, 0.05)
+    Hsym = oracle.smooth_heaviside(grs, np.ascontiguousarray(np.repeat(sym, 7, axis=0)))[0]
+    half = mid.sum()
+    np.testing.assert_allclose(Hsym[:half] + Hsym[half:], 1.0, rtol=0, atol=2e-16)
+    assert np.any(np.abs(Hsym[:half] - 0.5) > 0.1)
     z = oracle.smooth_heaviside(gr, np.zeros((26, 26)))
     assert np.all(z == 0.5)
 
@@ -168,7 +174,7 @@ def test_gfm_reduces_to_laplacian(oracle):
     ax, ay, dg, rhs = oracle.gfm_assemble(gr, fl, phi, kap, H, div)
     h2 = gr.dx * gr.dx
     AX, AY, D = interior(gr, ax), interior(gr, ay), interior(gr, dg)
-    assert np.allclose(AX[1:, :][:, :], -1 / (rho * h2)) or True
+    np.testing.assert_allclose(AX[1, 1:], -1.0 / (rho * h2), rtol=1e-15)  # row 1 but the pinned cell's face
     np.testing.assert_allclose(AX[2:, :], -1.0 / (rho * h2), rtol=1e-15)
     np.testing.assert_allclose(AX[0, :], 0.0)  # wall faces dropped
     np.testing.assert_allclose(AY[:, 0], 0.0)
