@@ -113,6 +113,7 @@ def load() -> C.CDLL:
         "asd2_set_bc": ([_VP, C.c_int, P(C.c_int), P(C.c_double)], C.c_int),
         "asd2_bc_apply": ([_VP, C.c_int, _VP], C.c_int),
         "asd2_ls_advect": ([_VP, _VP, _VP, _VP, C.c_double], C.c_int),
+        "asd2_ls_advect_noalloc": ([_VP, _VP, _VP, _VP, C.c_double], C.c_int),
         "asd2_ls_weno_rhs": ([_VP, _VP, _VP, _VP, _VP], C.c_int),
         "asd2_ls_reinit_minrk2": ([_VP, _VP, C.c_int], C.c_int),
         "asd2_ls_reinit_sussman": ([_VP, _VP, C.c_int], C.c_int),
@@ -365,6 +366,11 @@ class Context:
     # level set ---------------------------------------------------------------------
     def ls_advect(self, ls, u, v, dt):
         check(self.lib.asd2_ls_advect(self.h, ls.ptr, u.ptr, v.ptr, dt))
+
+    def ls_advect_noalloc(self, ls, u, v, dt):
+        """asd2_ls_advect_noalloc: the same advection with the stage arrays in the library's
+        static per-GPU buffer (no device allocation)"""
+        check(self.lib.asd2_ls_advect_noalloc(self.h, ls.ptr, u.ptr, v.ptr, dt))
 
     def ls_weno_rhs(self, ls, uc, vc, L):
         check(self.lib.asd2_ls_weno_rhs(self.h, ls.ptr, uc.ptr, vc.ptr, L.ptr))

@@ -6,6 +6,8 @@
 // evaluated in the reference's operation order with contraction off, so every branch
 // agrees with the reference; the bulk WENO/ENO algebra is reorganised (one division per
 // WENO side, reciprocal multiplies) and agrees to rounding.
+#include <mutex>
+
 #include "internal.h"
 #include "weno.h"
 
@@ -569,6 +571,59 @@ bool has_custom(const asd2_ctx* ctx) {
     return false;
 }
 
+// The three SSP-RK3 stages of Solve_LevelSet_2D when every P side's ghost rule can be fused into
+// the stage kernels (no LS-free side): d_ls -> p1 -> p2 -> d_ls, each stage writing its own
+// ghosts.  `prof` brackets the launches with the per-launch timing events.
+int advect_stages(asd2_ctx* ctx, const GhostRule& R, double* d_ls, const double* d_u, const double* d_v, double dt,
+                  double* p1, double* p2, bool prof) {
+    const Geom& G = ctx->geo;
+    const Launch L = tiles(G, WJ, WI, WTW);
+    if (prof) prof_begin(ctx, ASD2_KID_WENO_STAGE);
+    hipLaunchKernelGGL((k_weno<1, false>), L.grid, L.block, 0, ctx->stream, G, R, d_ls, d_ls, d_u, d_v, p1, dt, 1);
+    if (prof) prof_end(ctx, ASD2_KID_WENO_STAGE);
+    ASD2_LAUNCH_CHECK();
+    if (prof) prof_begin(ctx, ASD2_KID_WENO_STAGE);
+    hipLaunchKernelGGL((k_weno<2, false>), L.grid, L.block, 0, ctx->stream, G, R, p1, d_ls, d_u, d_v, p2, dt, 1);
+    if (prof) prof_end(ctx, ASD2_KID_WENO_STAGE);
+    ASD2_LAUNCH_CHECK();
+    if (prof) prof_begin(ctx, ASD2_KID_WENO_STAGE);
+    hipLaunchKernelGGL((k_weno<3, false>), L.grid, L.block, 0, ctx->stream, G, R, p2, d_ls, d_u, d_v, d_ls, dt, 1);
+    if (prof) prof_end(ctx, ASD2_KID_WENO_STAGE);
+    ASD2_LAUNCH_CHECK();
+    return ASD2_OK;
+}
+
+// Stage arrays of asd2_ls_advect_noalloc: one copy per GPU, held until the process ends.  Host
+// code must not use this name as an address; it uses the per-device pointer from
+// noalloc_scratch().
+static __device__ __align__(256) double scratch[2 * static_cast<size_t>(ASD2_LS_NOALLOC_MAX_CELLS)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on the current device, looked up once per device
+int noalloc_scratch(double** out) {
+    int dev = 0;
+    ASD2_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("device %d out of range for the static stage buffer", dev);
+        return ASD2_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        ASD2_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            set_error("hipGetSymbolAddress returned a null stage buffer on device %d", dev);
+            return ASD2_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return ASD2_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -590,7 +645,6 @@ int asd2_ls_advect(asd2_ctx* ctx, double* d_ls, const double* d_u, const double*
     }
     ASD2_TRY(bc_apply_p(ctx, d_ls));
     const Launch L = tiles(G, WJ, WI, WTW);
-    const int gh = fused ? 1 : 0;
     if (!fused) {
         // LS-free sides: build tmpU/tmpV with the full P BC, then run the centred variant
         double *uc, *vc;
@@ -613,21 +667,38 @@ int asd2_ls_advect(asd2_ctx* ctx, double* d_ls, const double* d_u, const double*
         ASD2_LAUNCH_CHECK();
         return bc_apply_p(ctx, d_ls);
     }
-    prof_begin(ctx, ASD2_KID_WENO_STAGE);
-    hipLaunchKernelGGL((k_weno<1, false>), L.grid, L.block, 0, ctx->stream, G, R, d_ls, d_ls, d_u, d_v, p1, dt, gh);
-    prof_end(ctx, ASD2_KID_WENO_STAGE);
-    ASD2_LAUNCH_CHECK();
-    ASD2_TRY(finish_ghosts(ctx, fused, p1));
-    prof_begin(ctx, ASD2_KID_WENO_STAGE);
-    hipLaunchKernelGGL((k_weno<2, false>), L.grid, L.block, 0, ctx->stream, G, R, p1, d_ls, d_u, d_v, p2, dt, gh);
-    prof_end(ctx, ASD2_KID_WENO_STAGE);
-    ASD2_LAUNCH_CHECK();
-    ASD2_TRY(finish_ghosts(ctx, fused, p2));
-    prof_begin(ctx, ASD2_KID_WENO_STAGE);
-    hipLaunchKernelGGL((k_weno<3, false>), L.grid, L.block, 0, ctx->stream, G, R, p2, d_ls, d_u, d_v, d_ls, dt, gh);
-    prof_end(ctx, ASD2_KID_WENO_STAGE);
-    ASD2_LAUNCH_CHECK();
-    return finish_ghosts(ctx, fused, d_ls);
+    return advect_stages(ctx, R, d_ls, d_u, d_v, dt, p1, p2, true);
+}
+
+int asd2_ls_advect_noalloc(asd2_ctx* ctx, double* d_ls, const double* d_u, const double* d_v, double dt) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!d_ls || !d_u || !d_v) { set_error("null array"); return ASD2_ERR_ARG; }
+    GhostRule R;
+    bool fused;
+    ASD2_TRY(p_rule(ctx, &R, &fused));
+    if (!fused) {
+        set_error("asd2_ls_advect_noalloc: level-set-free pressure sides need two more work arrays; "
+                  "use asd2_ls_advect");
+        return ASD2_ERR_UNSUPPORTED;
+    }
+    const Geom& G = ctx->geo;
+    const int64_t n = G.size();
+    if (n > ASD2_LS_NOALLOC_MAX_CELLS) {
+        set_error("asd2_ls_advect_noalloc: %lld padded cells exceed the static stage buffer of %d cells per stage",
+                  static_cast<long long>(n), ASD2_LS_NOALLOC_MAX_CELLS);
+        return ASD2_ERR_ARG;
+    }
+    double* base = nullptr;
+    ASD2_TRY(noalloc_scratch(&base));
+    double* p1 = base;
+    double* p2 = base + ASD2_LS_NOALLOC_MAX_CELLS;
+    // the buffer keeps the previous call's values: CUSTOM sides read zero ghosts in the stages
+    if (has_custom(ctx)) {
+        ASD2_TRY(zero_async(ctx, p1, static_cast<size_t>(n) * 8));
+        ASD2_TRY(zero_async(ctx, p2, static_cast<size_t>(n) * 8));
+    }
+    ASD2_TRY(bc_apply_p(ctx, d_ls));
+    return advect_stages(ctx, R, d_ls, d_u, d_v, dt, p1, p2, false);
 }
 
 int asd2_ls_weno_rhs(asd2_ctx* ctx, double* d_ls, const double* d_uc, const double* d_vc, double* d_L) {

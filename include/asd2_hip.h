@@ -132,6 +132,17 @@ int asd2_bc_apply(asd2_ctx* ctx, int grid_kind, double* d_arr);
 /* ---------------------------------------------------------------- level set */
 /* LevelSetSolver2D::Solve_LevelSet_2D, src/levelset2d.cpp:43-86 (in place) */
 int asd2_ls_advect(asd2_ctx* ctx, double* d_ls, const double* d_u, const double* d_v, double dt);
+/* No reference counterpart: asd2_ls_advect for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  It calls no
+ * allocating routine: the two SSP-RK3 stage arrays live in a static device buffer of
+ * 2 x ASD2_LS_NOALLOC_MAX_CELLS doubles (68.2 MB) that the library holds on each GPU it runs on
+ * until the process ends, instead of the context's workspaces.  Grids of more than
+ * ASD2_LS_NOALLOC_MAX_CELLS padded cells, (nx + 6) (ny + 6) (2048 x 2048 fits), return
+ * ASD2_ERR_ARG; level-set-free (LSFREE) pressure sides return ASD2_ERR_UNSUPPORTED.  Every call on
+ * one GPU shares the buffer, which keeps the last call's values: calls that overlap on one GPU
+ * (several streams or host threads at once) race.  Results equal asd2_ls_advect's bit for bit. */
+#define ASD2_LS_NOALLOC_MAX_CELLS 4259840
+int asd2_ls_advect_noalloc(asd2_ctx* ctx, double* d_ls, const double* d_u, const double* d_v, double dt);
 /* LevelSetSolver2D::HJWENO5_LS_2D, src/levelset2d.cpp:88-157 (applies the P BC to d_ls) */
 int asd2_ls_weno_rhs(asd2_ctx* ctx, double* d_ls, const double* d_uc, const double* d_vc, double* d_L);
 /* LevelSetSolver2D::Reinit_MinRK2_2D, src/levelset2d.cpp:433-486; iterations from
