@@ -117,9 +117,22 @@ __global__ __launch_bounds__(256) void k_gfm_assemble(Geom G, GfmArgs A, const d
     const double di = 1.0 / C;
     dinv[c] = di;
     if (ax32) {
+        // the FP32 operator is applied in flux form, sum a (p_nb - p_c) + d p_c, with d the part of
+        // the diagonal no stored coupling accounts for: faces to Dirichlet-family walls, the
+        // couplings to the pinned cell, and the pinned cell's whole row (exact here, no cancellation)
+        double d = 0.0;
+        if (i == G.g && is_dir(A.bc[0])) d += -nW / (dx * dx);
+        if (i == G.g + G.nx - 1 && is_dir(A.bc[1])) d += -nE / (dx * dx);
+        if (j == G.g && is_dir(A.bc[2])) d += -nS / (dy * dy);
+        if (j == G.g + G.ny - 1 && is_dir(A.bc[3])) d += -nN / (dy * dy);
+        if (A.pin) {
+            if (i == G.g + 1 && j == G.g) d += -nW / (dx * dx);
+            else if (i == G.g && j == G.g + 1) d += -nS / (dy * dy);
+            if (i == G.g && j == G.g) d = C;
+        }
         ax32[c] = static_cast<float>(aW);
         ay32[c] = static_cast<float>(aS);
-        dg32[c] = static_cast<float>(C);
+        dg32[c] = static_cast<float>(d);
         dinv32[c] = static_cast<float>(di);
     }
 }
@@ -149,16 +162,22 @@ bool all_neumann_family(const asd2_ctx* ctx) {
     return true;
 }
 
-// FP32 copies of an FP64 system (for an imported system)
-__global__ void k_sys_to32(int64_t n, const double* ax, const double* ay, const double* dg, const double* di,
+// FP32 copies of an FP64 system (for an imported system): couplings, Jacobi inverse and the
+// flux-form remainder d = diag - sum |couplings| of each interior cell, taken as 0 when it is
+// rounding noise of the FP64 diagonal
+__global__ void k_sys_to32(Geom G, const double* ax, const double* ay, const double* dg, const double* di,
                            float* ax32, float* ay32, float* dg32, float* di32) {
-    for (int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; k < n;
-         k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        ax32[k] = static_cast<float>(ax[k]);
-        ay32[k] = static_cast<float>(ay[k]);
-        dg32[k] = static_cast<float>(dg[k]);
-        di32[k] = static_cast<float>(di[k]);
-    }
+    const int j = G.g + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = G.g + blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= G.nx + G.g || j >= G.ny + G.g) return;
+    const int64_t k = G.idx(i, j);
+    const double s = -(ax[k] + ax[k + G.sy] + ay[k] + ay[k + 1]);
+    double d = dg[k] - s;
+    if (fabs(d) <= 1e-12 * fabs(dg[k])) d = 0.0;
+    ax32[k] = static_cast<float>(ax[k]);
+    ay32[k] = static_cast<float>(ay[k]);
+    dg32[k] = static_cast<float>(d);
+    di32[k] = static_cast<float>(di[k]);
 }
 
 __global__ void k_dinv(Geom G, const double* dg, double* di) {
@@ -212,6 +231,7 @@ int asd2_gfm_assemble(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_ls, c
     prof_end(ctx, ASD2_KID_GFM);
     ASD2_LAUNCH_CHECK();
     ctx->system_ready = true;
+    ctx->dres_ring_only = true;
     return ASD2_OK;
 }
 
@@ -242,11 +262,11 @@ int asd2_gfm_import(asd2_ctx* ctx, const double* d_ax, const double* d_ay, const
     const Geom& G = ctx->geo;
     hipLaunchKernelGGL(k_dinv, dim3(cdiv(G.ny, 64), cdiv(G.nx, 4)), dim3(256), 0, ctx->stream, G, S.diag, S.dinv);
     ASD2_LAUNCH_CHECK();
-    const int64_t n = G.size();
-    hipLaunchKernelGGL(k_sys_to32, dim3(2048), dim3(256), 0, ctx->stream, n, S.ax, S.ay, S.diag, S.dinv, S.ax32,
-                       S.ay32, S.diag32, S.dinv32);
+    hipLaunchKernelGGL(k_sys_to32, dim3(cdiv(G.ny, 64), cdiv(G.nx, 4)), dim3(256), 0, ctx->stream, G, S.ax, S.ay,
+                       S.diag, S.dinv, S.ax32, S.ay32, S.diag32, S.dinv32);
     ASD2_LAUNCH_CHECK();
     ctx->system_ready = true;
+    ctx->dres_ring_only = false;
     return ASD2_OK;
 }
 

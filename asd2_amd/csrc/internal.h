@@ -73,6 +73,8 @@ struct asd2_ctx {
 
     // assembled Poisson system state
     bool system_ready = false;
+    // the FP32 flux-form remainder is nonzero only on the boundary ring (assembled systems)
+    bool dres_ring_only = false;
     // network of the learned preconditioner (not owned)
     struct asd2_unet* net = nullptr;
 

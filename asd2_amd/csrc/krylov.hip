@@ -385,8 +385,14 @@ __global__ __launch_bounds__(KB) void k_cg_spmv(Geom G, const T* __restrict__ ax
             if (i >= G.nx + G.g || j >= G.ny + G.g) continue;
             const int64_t k = G.idx(i, j);
             const T pc = sP[r + 1][tj + 1];
-            const T qv = dg[k] * pc + ax[k] * sP[r][tj + 1] + ax[k + G.sy] * sP[r + 2][tj + 1] +
-                         ay[k] * sP[r + 1][tj] + ay[k + 1] * sP[r + 1][tj + 2];
+            T qv;
+            if constexpr (sizeof(T) == 4) {  // FP32: flux form, dg = the remainder d (solver.h)
+                qv = ax[k] * (sP[r][tj + 1] - pc) + ax[k + G.sy] * (sP[r + 2][tj + 1] - pc) +
+                     ay[k] * (sP[r + 1][tj] - pc) + ay[k + 1] * (sP[r + 1][tj + 2] - pc) + dg[k] * pc;
+            } else {
+                qv = dg[k] * pc + ax[k] * sP[r][tj + 1] + ax[k + G.sy] * sP[r + 2][tj + 1] +
+                     ay[k] * sP[r + 1][tj] + ay[k + 1] * sP[r + 1][tj + 2];
+            }
             pnew[k] = pc;
             q[k] = qv;
             acc += static_cast<double>(pc) * static_cast<double>(qv);
@@ -1248,7 +1254,8 @@ void launch_lp_pass(hipStream_t s, int nb, Geom G, Coarse C, const float* r_in, 
 // iteration; the FP64 true residual uses the assembled diagonal).
 template <int MW, int CC>
 __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* __restrict__ ax,
-                                                const float* __restrict__ ay,
+                                                const float* __restrict__ ay, const float* __restrict__ dres,
+                                                int ring_only,
                                                 const float* __restrict__ r, const float* __restrict__ dinv,
                                                 const float* __restrict__ y, Dirs Dd, KrylovState* st,
                                                 const double* pb, int nbb, const double* pc, int nbc, double* ps,
@@ -1343,7 +1350,6 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
     float* __restrict__ pnew = Dd.pnew;
     float* __restrict__ q = Dd.qnew;
     __shared__ float sP[FI + 2][TJ + 2];
-    __shared__ float sDg[FI][TJ];  // the diagonal of A on the tile, 1 / D^-1
     const int tj = threadIdx.x & (TJ - 1), ti = threadIdx.x / TJ;
     double acc = 0.0;
     for (int t = t0; t < t1; t++) {
@@ -1363,7 +1369,6 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
                     if (jj < nwin) val += coef[jj] * ld_p[u][jj];
             }
             sP[rr_][cc] = val;
-            if (rr_ >= 1 && rr_ <= FI && cc >= 1 && cc <= TJ) sDg[rr_ - 1][cc - 1] = __builtin_amdgcn_rcpf(ld_d[u]);
         }
         if (t + 1 < t1) load(t + 1);
         __syncthreads();
@@ -1372,8 +1377,11 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
             if (i >= ihi || j >= jhi) continue;
             const int64_t k = G.idx(i, j);
             const float pcv = sP[rr_ + 1][tj + 1];
-            const float qv = sDg[rr_][tj] * pcv + ax[k] * sP[rr_][tj + 1] + ax[k + G.sy] * sP[rr_ + 2][tj + 1] +
-                             ay[k] * sP[rr_ + 1][tj] + ay[k + 1] * sP[rr_ + 1][tj + 2];
+            // flux form (solver.h); the remainder of an assembled system lives on the boundary ring
+            const bool ring = i == ilo || i == ihi - 1 || j == jlo || j == jhi - 1;
+            const float dv = (!ring_only || ring) ? dres[k] : 0.0f;
+            const float qv = ax[k] * (sP[rr_][tj + 1] - pcv) + ax[k + G.sy] * (sP[rr_ + 2][tj + 1] - pcv) +
+                             ay[k] * (sP[rr_ + 1][tj] - pcv) + ay[k + 1] * (sP[rr_ + 1][tj + 2] - pcv) + dv * pcv;
             pnew[k] = pcv;
             q[k] = qv;
             acc += static_cast<double>(pcv) * qv;
@@ -1386,13 +1394,13 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
 
 // launches k_lp_spmv<MW, c> for the run-time coarsening c
 template <int MW>
-void launch_lp_spmv(hipStream_t s, int nb, Geom G, Coarse C, const float* ax, const float* ay,
-                    const float* r, const float* dinv, const float* y, Dirs Dd, KrylovState* st, const double* pb,
-                    int nbb, const double* pc, int nbc, double* ps, int it, int nwin, double gamma_fixed,
-                    const double* pu, int nbu) {
+void launch_lp_spmv(hipStream_t s, int nb, Geom G, Coarse C, const float* ax, const float* ay, const float* dres,
+                    int ring_only, const float* r, const float* dinv, const float* y, Dirs Dd, KrylovState* st,
+                    const double* pb, int nbb, const double* pc, int nbc, double* ps, int it, int nwin,
+                    double gamma_fixed, const double* pu, int nbu) {
 #define ASD2_LP_SPMV(CV)                                                                                              \
-    hipLaunchKernelGGL((k_lp_spmv<MW, CV>), dim3(nb), dim3(KB), 0, s, G, C, ax, ay, r, dinv, y, Dd, st, pb, nbb,     \
-                       pc, nbc, ps, it, nwin, gamma_fixed, pu, nbu)
+    hipLaunchKernelGGL((k_lp_spmv<MW, CV>), dim3(nb), dim3(KB), 0, s, G, C, ax, ay, dres, ring_only, r, dinv, y, Dd, \
+                       st, pb, nbb, pc, nbc, ps, it, nwin, gamma_fixed, pu, nbu)
     switch (C.c) {
         case 1: ASD2_LP_SPMV(1); break;
         case 2: ASD2_LP_SPMV(2); break;
@@ -2065,7 +2073,8 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
             }
             prof_begin(ctx, ASD2_KID_FCG_SPMV);
             (learned ? launch_lp_spmv<FCG_M> : launch_lp_spmv<1>)(
-                ctx->stream, nbs, G, C, S.ax32, S.ay32, r, S.dinv32, use_net ? ycoarse : nullptr,
+                ctx->stream, nbs, G, C, S.ax32, S.ay32, S.diag32, ctx->dres_ring_only ? 1 : 0, r, S.dinv32,
+                use_net ? ycoarse : nullptr,
                 dirs_for(R, it), D.st, pb, nbb, pc, nbc, D.ps, it, nwin(it), learned ? 0.0 : 1.0, D.pu, nbu);
             prof_end(ctx, ASD2_KID_FCG_SPMV);
             prof_begin(ctx, ASD2_KID_FCG_UPDATE);

@@ -8,6 +8,13 @@ namespace asd2 {
 // Matrix-free 5-point operator on the padded layout (interior cells are unknowns; all
 // ghost entries are 0 so ghost neighbours contribute nothing):
 //   (A p)_c = diag_c p_c + ax_c p_W + ax_E p_E + ay_c p_S + ay_N p_N
+// The FP32 copies apply it in flux form,
+//   (A p)_c = ax_c (p_W - p_c) + ax_E (p_E - p_c) + ay_c (p_S - p_c) + ay_N (p_N - p_c) + d_c p_c,
+// with diag32 holding the remainder d = diag - sum |couplings| (walls with a Dirichlet value, the
+// couplings to a pinned cell; zero in the interior of an assembled system).  For a smooth p the
+// differences are exact in FP32, so A p keeps its relative accuracy where the diagonal form
+// diag p_c - sum |a| p_nb cancels to rounding noise (the pinned all-Neumann operator has a
+// near-null mode at the bottom of its spectrum).
 struct System {
     double *ax = nullptr, *ay = nullptr, *diag = nullptr, *rhs = nullptr, *dinv = nullptr;
     float *ax32 = nullptr, *ay32 = nullptr, *diag32 = nullptr, *dinv32 = nullptr;

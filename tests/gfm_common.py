@@ -100,3 +100,26 @@ def random_spd_csr(n, seed):
     A.sort_indices()
     return A
 
+
+
+def seeded_droplet_system(oracle, n, seed):
+    """The system asd2_amd.learned.build_system assembles on the GPU for this seed, rebuilt with the
+    oracle from the same random draws: droplets (1-20 circles, radius in [0.02, 0.1]), smooth
+    velocity with no-slip walls, density ratio log-uniform in [10, 1000] (rho_h = 1000), no surface
+    tension, dt = 0.3 dx."""
+    import asd2_amd.learned as LN
+    rng = np.random.default_rng(seed)
+    dx = 1.0 / n
+    gr = make_grid(n, n, dx, dx)
+    phi = LN.droplets(rng, n, dx)
+    oracle.bc_p(gr, phi)
+    kap = oracle.update_kappa(gr, phi)
+    oracle.bc_p(gr, kap)
+    H = oracle.heaviside(gr, phi)
+    u, v = LN.smooth_velocity(rng, n, dx)
+    oracle.bc_u(gr, u)
+    oracle.bc_v(gr, v)
+    div = oracle.divergence(gr, u, v)
+    rl = 1000.0 / float(np.exp(rng.uniform(np.log(10.0), np.log(1000.0))))
+    fl = make_fluid(1000.0, rl, 1.0, 0.01, float(rng.uniform(0.0, 0.0)), 0.3 * dx)
+    return gr, oracle.gfm_assemble(gr, fl, phi, kap, H, div)

@@ -74,3 +74,46 @@ def test_multilevel_deterministic(oracle):
     for it, res, x in runs[1:]:
         assert it == runs[0][0] and res == runs[0][1]
         np.testing.assert_array_equal(x, runs[0][2])
+
+
+@pytest.mark.parametrize("seed", [901, 902])
+def test_multilevel_counts_equal_restatement(oracle, seed):
+    """The GPU solve takes the iterations the FP64 restatement (oracle/multilevel.py) takes on the
+    same seeded-droplet system (512^2, exported from the GPU assembly), to within 3."""
+    from asd2_amd import learned as LN
+    from oracle import multilevel as ML
+    n = 512
+    ctx = LN.system_context(n)
+    LN.build_system(ctx, np.random.default_rng(seed))
+    ax, ay, dg, rhs = ctx.gfm_export()
+    st = ctx.poisson_solve(ctx.array(), ctx.opts(tol=1e-8, precond=A.PRECOND_MULTILEVEL, precision=A.PREC_MIXED))
+    Am = ML.stencil_matrix(ax, ay, dg, n, n)
+    b = rhs[3:3 + n, 3:3 + n].ravel().copy()
+    _, it, ok = ML.fcg(Am, b, ML.Multilevel(Am, n, n), tol=1e-8)
+    print(f"seed {seed}: gpu {st.iterations} ({st.reliable_updates} updates), restatement {it}")
+    assert st.converged == 1 and st.fallbacks == 0 and ok
+    assert abs(st.iterations - it) <= 3
+
+
+def test_multilevel_seeded_2048(oracle):
+    """A 2048^2 seeded-droplet system on which an FP32 operator in diagonal form (diag p - sum |a| p)
+    made the solve diverge: the flux form keeps it converging.  The true residual is recomputed on
+    the host with the oracle's stencil, and the solution agrees with FP64 Jacobi-PCG."""
+    from asd2_amd import learned as LN
+    from oracle import make_grid
+    n = 2048
+    ctx = LN.system_context(n)
+    LN.build_system(ctx, np.random.default_rng(901))
+    ps = ctx.array()
+    st = ctx.poisson_solve(ps, ctx.opts(tol=1e-10, precond=A.PRECOND_MULTILEVEL, precision=A.PREC_MIXED))
+    ps_j = ctx.array()
+    st_j = ctx.poisson_solve(ps_j, ctx.opts(tol=1e-10, precision=A.PREC_FP64, mode=A.CG_STANDARD))
+    ax, ay, dg, rhs = ctx.gfm_export()
+    gr = make_grid(n, n, 1.0 / n, 1.0 / n)
+    x = ps.download()
+    r = rhs - oracle.stencil_apply(gr, ax, ay, dg, x)
+    res = float(np.linalg.norm(interior(gr, r)) / np.linalg.norm(interior(gr, rhs)))
+    err = _rel(interior(gr, x), interior(gr, ps_j.download()))
+    print(f"multilevel {st.iterations} iters, jacobi {st_j.iterations}, host residual {res:.2e}, rel L2 {err:.2e}")
+    assert st.converged == 1 and st.fallbacks == 0 and st.iterations <= 400
+    assert res <= 1.01e-10 and err <= 1e-6
