@@ -161,6 +161,69 @@ __global__ __launch_bounds__(256) void k_conv3x3(Src in, const uint16_t* __restr
     }
 }
 
+// The same convolution split along K for small images, where one wave per 64 pixels leaves the
+// chip idle and each wave's K loop is a long chain of dependent loads: a block of KS waves owns one
+// 16-pixel tile and one 16-channel output tile (blockIdx.x, blockIdx.y), wave w takes the K chunks
+// w, w + KS, ..., and the KS partial tiles are added through LDS in wave order (deterministic).
+template <int CIN>
+constexpr int conv_ks() {
+    return ((9 * CIN + 31) / 32) >= 24 ? 8 : (((9 * CIN + 31) / 32) >= 8 ? 4 : 2);
+}
+
+template <int CIN, int COUT, int EPI>
+__global__ __launch_bounds__(64 * conv_ks<CIN>()) void k_conv3x3_sk(Src in, const uint16_t* __restrict__ w,
+                                                                 const float* __restrict__ bias,
+                                                                 uint16_t* __restrict__ out,
+                                                                 const uint16_t* __restrict__ mask, int nimg, int H,
+                                                                 int W) {
+    static_assert(COUT % 16 == 0 && CIN % 4 == 0, "channel counts");
+    constexpr int KTOT = 9 * CIN, KCH = (KTOT + 31) / 32, KS = conv_ks<CIN>();
+    __shared__ f4v red[KS][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = lane >> 4, n = lane & 15;
+    const int64_t P = static_cast<int64_t>(nimg) * H * W;
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * 16 + n;
+    const bool ok = p < P;
+    int img, pi, pj;
+    pixel_of(ok ? p : 0, H, W, img, pi, pj);
+    const int co = 16 * blockIdx.y + n;
+    f4v acc = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+    for (int kc = wv; kc < KCH; kc += KS) {
+        const int k0 = 32 * kc + 8 * g, k1 = k0 + 4;
+        const int t0 = k0 / CIN, c0 = k0 - t0 * CIN;
+        const int t1 = k1 / CIN, c1 = k1 - t1 * CIN;
+        const uint2 alo = (k0 < KTOT) ? *reinterpret_cast<const uint2*>(w + (static_cast<int64_t>(t0) * COUT + co) * CIN + c0)
+                                      : make_uint2(0u, 0u);
+        const uint2 ahi = (k1 < KTOT) ? *reinterpret_cast<const uint2*>(w + (static_cast<int64_t>(t1) * COUT + co) * CIN + c1)
+                                      : make_uint2(0u, 0u);
+        const uint2 blo = (k0 < KTOT && ok) ? load4(in, img, pi + t0 / 3 - 1, pj + t0 % 3 - 1, H, W, c0)
+                                            : make_uint2(0u, 0u);
+        const uint2 bhi = (k1 < KTOT && ok) ? load4(in, img, pi + t1 / 3 - 1, pj + t1 % 3 - 1, H, W, c1)
+                                            : make_uint2(0u, 0u);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(pack8(alo, ahi), pack8(blo, bhi), acc, 0, 0, 0);
+    }
+    red[wv][lane] = acc;
+    __syncthreads();
+    if (wv != 0 || !ok) return;
+#pragma unroll
+    for (int u = 1; u < KS; u++) acc += red[u][lane];
+    const int c4 = 16 * blockIdx.y + 4 * g;
+    float v[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        v[i] = acc[i];
+        if (EPI == EPI_RELU) v[i] = fmaxf(v[i] + bias[c4 + i], 0.0f);
+    }
+    if (EPI == EPI_MASK) {
+        const uint2 mk = *reinterpret_cast<const uint2*>(mask + p * COUT + c4);
+        if (!(lo16(mk.x) > 0.f)) v[0] = 0.f;
+        if (!(hi16(mk.x) > 0.f)) v[1] = 0.f;
+        if (!(lo16(mk.y) > 0.f)) v[2] = 0.f;
+        if (!(hi16(mk.y) > 0.f)) v[3] = 0.f;
+    }
+    *reinterpret_cast<uint2*>(out + p * COUT + c4) = pack4(v[0], v[1], v[2], v[3]);
+}
+
 // dw[tap][co][ci] += sum_p dpre[p][co] in[p + d(tap)][ci] (FP32 atomics); blockIdx.y = tap.
 // A 32-pixel chunk of dpre and of the shifted input is staged transposed in LDS, then each
 // wave runs K = 32 MFMAs for its (co, ci) tiles.
@@ -573,11 +636,17 @@ int blocks_for(int64_t work, int per = 256, int cap = 65535) {
     return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(cap, (work + per - 1) / per)));
 }
 
+// the K-split kernel when the pixel-tiled one would run fewer waves than the chip has SIMDs
 template <int CIN, int COUT, int EPI>
 void launch_conv(hipStream_t s, const Src& in, const uint16_t* w, const float* bias, uint16_t* out,
                  const uint16_t* mask, int nimg, int H, int W) {
     const int64_t tiles = (static_cast<int64_t>(nimg) * H * W + 15) / 16;
     const int64_t waves = (tiles + CONV_NP - 1) / CONV_NP;
+    if (waves < 1024) {
+        hipLaunchKernelGGL((k_conv3x3_sk<CIN, COUT, EPI>), dim3(static_cast<unsigned>(tiles), COUT / 16),
+                           dim3(64 * conv_ks<CIN>()), 0, s, in, w, bias, out, mask, nimg, H, W);
+        return;
+    }
     const int blocks = static_cast<int>((waves + 3) / 4);
     hipLaunchKernelGGL((k_conv3x3<CIN, COUT, EPI>), dim3(blocks), dim3(256), 0, s, in, w, bias, out, mask, nimg, H, W);
 }

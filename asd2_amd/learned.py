@@ -4,10 +4,14 @@ The training systems are the GFM pressure systems of random droplet configuratio
 the library itself: the level set of a union of circles (heavy phase inside, phi = max over
 the circles of r_k - |x - c_k|), its curvature and Heaviside, the divergence of a smooth random
 velocity, assembled by asd2_gfm_assemble (the path of MACSolver2D::SolvePoisson, reference
-src/mac2d.cpp:1249) and exported in FP64.  The residual samples are the right-hand side and the
-residuals b - A x_k after k iterations of the mixed-precision Jacobi-PCG.  numpy only draws the
-random fields and stacks host copies into the FP32 padded layout of asd2_unet_train_step; the
-network's forward and backward passes and the Adam steps are the library's kernels.
+src/mac2d.cpp:1249) and exported in FP64.  The default distribution is the seeded-droplet one of
+the GFM benchmark case: 1-20 non-overlapping circles of radius uniform in [0.02, 0.1], density
+ratio log-uniform in [10, 1000], right-hand side minus the divergence of a smooth random velocity.
+The residual samples are the right-hand side and the residuals b - A x_k after k iterations of a
+solver: the mixed-precision Jacobi-PCG, or the flexible PCG with the multilevel coarse correction,
+whose residuals are the ones a good coarse correction leaves behind.  numpy only draws the random
+fields and stacks host copies into the FP32 padded layout of asd2_unet_train_step; the network's
+forward and backward passes and the Adam steps are the library's kernels.
 """
 from __future__ import annotations
 
@@ -15,7 +19,7 @@ import time
 
 import numpy as np
 
-from . import GRID_P, GRID_U, GRID_V, PREC_MIXED, Context, DeviceArray, UNet, make_fluid
+from . import GRID_P, GRID_U, GRID_V, PREC_MIXED, PRECOND_MULTILEVEL, Context, DeviceArray, UNet, make_fluid
 
 G = 3
 
@@ -25,7 +29,7 @@ def _centres(n, dx, g=G):
     return np.meshgrid(x, x, indexing="ij")
 
 
-def droplets(rng, n, dx, count=(2, 8), radius=(0.05, 0.25)):
+def droplets(rng, n, dx, count=(1, 20), radius=(0.02, 0.1)):
     """Signed distance of `count` non-overlapping circles in the unit box (padded layout)."""
     k = int(rng.integers(count[0], count[1] + 1))
     circles = []
@@ -90,14 +94,16 @@ def system_context(n):
     return ctx
 
 
-def build_system(ctx, rng, rho_h=1000.0, rho_l=(1.0, 10.0), sigma=(0.3, 1.0)):
+def build_system(ctx, rng, rho_h=1000.0, ratio=(10.0, 1000.0), sigma=(0.0, 0.0), count=(1, 20),
+                 radius=(0.02, 0.1)):
     """Assembles the GFM pressure system of a random droplet field in ctx (the path of
     MACSolver2D::SolvePoisson): circles, curvature, Heaviside, divergence of a smooth random
-    velocity, light-phase density log-uniform in rho_l."""
+    velocity, density ratio rho_h / rho_l log-uniform in `ratio`, surface tension uniform in
+    `sigma`."""
     n = ctx.nx
     dx = 1.0 / n
     ls, kap, H, div, u, v = (ctx.array() for _ in range(6))
-    ls.upload(droplets(rng, n, dx))
+    ls.upload(droplets(rng, n, dx, count=count, radius=radius))
     ctx.bc_apply(GRID_P, ls)
     ctx.kappa(ls, kap)
     ctx.bc_apply(GRID_P, kap)
@@ -108,35 +114,45 @@ def build_system(ctx, rng, rho_h=1000.0, rho_l=(1.0, 10.0), sigma=(0.3, 1.0)):
     ctx.bc_apply(GRID_U, u)
     ctx.bc_apply(GRID_V, v)
     ctx.divergence(u, v, div)
-    rl = float(np.exp(rng.uniform(np.log(rho_l[0]), np.log(rho_l[1]))))
+    rl = rho_h / float(np.exp(rng.uniform(np.log(ratio[0]), np.log(ratio[1]))))
     fl = make_fluid(rho_h, rl, 1.0, 0.01, float(rng.uniform(*sigma)), 0.3 * dx)
     ctx.gfm_assemble(fl, ls, kap, H, div)
     ctx.sync()
     return fl
 
 
-def make_samples(n, systems=6, snapshots=(0, 10, 40, 120), seed=0, log=None, **kw):
-    """`systems` random droplet systems on an n x n grid, one sample per snapshot k (the
-    residual after k mixed-precision Jacobi-PCG iterations; k = 0: the right-hand side)."""
+def residual_after(ctx, k, solver, ps, y, rhs):
+    """b - A x_k after k iterations of `solver` ("jacobi": mixed-precision Jacobi-PCG,
+    "multilevel": flexible PCG with the multilevel coarse correction) from x_0 = 0"""
+    if k == 0:
+        return rhs.copy()
+    ps.upload(np.zeros_like(rhs))
+    precond = PRECOND_MULTILEVEL if solver == "multilevel" else 0
+    ctx.poisson_solve(ps, Context.opts(tol=1e-30, max_iter=k, precision=PREC_MIXED, precond=precond),
+                      raise_on_fail=False)
+    ctx.gfm_apply(ps, y)
+    return rhs - y.download()
+
+
+def make_samples(n, systems=6, snapshots=(("jacobi", 0), ("jacobi", 10), ("jacobi", 40), ("jacobi", 120)),
+                 seed=0, log=None, **kw):
+    """`systems` random droplet systems on an n x n grid, one sample per snapshot (solver, k): the
+    residual after k iterations of that solver (k = 0: the right-hand side).  A bare k means
+    ("jacobi", k)."""
     rng = np.random.default_rng(seed)
     cols = [[] for _ in range(5)]
     ctx = system_context(n)
     ps, y = ctx.array(), ctx.array()
     t0 = time.perf_counter()
+    snaps = [("jacobi", s) if isinstance(s, (int, np.integer)) else tuple(s) for s in snapshots]
     for s in range(systems):
         build_system(ctx, rng, **kw)
         ax, ay, dg, rhs = ctx.gfm_export()
         dinv = np.zeros_like(dg)
         inner = (slice(G, G + n), slice(G, G + n))
         dinv[inner] = 1.0 / dg[inner]
-        for k in snapshots:
-            if k == 0:
-                r = rhs.copy()
-            else:
-                ps.upload(np.zeros_like(rhs))
-                ctx.poisson_solve(ps, Context.opts(tol=1e-30, max_iter=k, precision=PREC_MIXED), raise_on_fail=False)
-                ctx.gfm_apply(ps, y)
-                r = rhs - y.download()
+        for solver, k in snaps:
+            r = residual_after(ctx, k, solver, ps, y, rhs)
             rm = np.zeros_like(r)
             rm[inner] = r[inner]
             for col, a in zip(cols, (rm, dinv, ax, ay, dg)):
@@ -161,3 +177,20 @@ def train(net: UNet, samples: Samples, steps=300, batch=4, lr=2e-3, lr_final=5e-
         if log and (step % 50 == 0 or step == steps - 1):
             log(f"step {step}: loss {losses[-1]:.3f}")
     return losses
+
+
+def save_params(net: UNet, path: str, meta: dict | None = None):
+    """FP32 parameter vector (asd2_unet_get_params order) as raw little-endian floats, with a
+    JSON side file of its training record"""
+    import json
+    v = net.get_params()
+    v.astype("<f4").tofile(path)
+    with open(path + ".json", "w") as f:
+        json.dump({"params": int(v.size), **(meta or {})}, f, indent=1)
+
+
+def load_params(net: UNet, path: str):
+    v = np.fromfile(path, dtype="<f4")
+    if v.size != net.nparams:
+        raise ValueError(f"{path} holds {v.size} parameters, the network has {net.nparams}")
+    net.set_params(v)
