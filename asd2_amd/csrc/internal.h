@@ -77,6 +77,11 @@ struct asd2_ctx {
     bool dres_ring_only = false;
     // network of the learned preconditioner (not owned)
     struct asd2_unet* net = nullptr;
+    // recording of the coarse correction's input on one level during the next coarse-corrected
+    // solve (training data): level size, iterations, destination (count x H x W doubles)
+    int rec_h = 0, rec_w = 0;
+    std::vector<int> rec_iters;
+    double* rec_dst = nullptr;
 
     // Krylov polling: pinned host copies of the device scalars and their events
     void* kr_host = nullptr;

@@ -890,6 +890,10 @@ int solve_mixed(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2_solv
 // smooth modes the coarse correction is for sit at the bottom of the spectrum
 constexpr int FCG_M = 4;              // directions kept by the truncated flexible recurrence
 constexpr int FCG_SLOTS = FCG_M + 1;  // ring of directions: iteration k in slot k % FCG_SLOTS
+// directions kept when the network solves a level below the V-cycle's smoothed levels: its
+// output varies with the residual through the scalar step on its level, and the network's map is
+// not symmetric; four directions took 4x fewer iterations than one on 2048^2 droplet systems
+constexpr int HYBRID_M = FCG_M;
 
 struct Coarse {
     int c, Hc, Wc;
@@ -1827,6 +1831,47 @@ __global__ __launch_bounds__(KB) void k_ml_dense_only(Level L, const double* __r
     }
 }
 
+// The learned coarse solve on level L of the hierarchy.  Network input channel 0 = g / dc: the
+// linear pass has no biases, so its output scales with its input, and the step gamma below
+// removes any scale; the other channels stay zero.
+__global__ __launch_bounds__(KB) void k_ml_net_pack(Level L, uint16_t* __restrict__ flin, const KrylovState* st) {
+    if (st->done != DONE_RUNNING) return;
+    const int P = L.H * L.W, sc = L.W + 1;
+    for (int p = blockIdx.x * KB + threadIdx.x; p < P; p += gridDim.x * KB) {
+        const int i = p / L.W, j = p - i * L.W;
+        flin[static_cast<int64_t>(p) * 4] = f2bf(static_cast<float>(L.g[p] / L.dc[i * sc + j]));
+    }
+}
+
+// block partials of y.g and y.A_L y of the network's output y on level L
+__global__ __launch_bounds__(KB) void k_ml_net_dots(Level L, const float* __restrict__ y, const KrylovState* st,
+                                                    double* part) {
+    if (st->done != DONE_RUNNING) return;
+    const int P = L.H * L.W;
+    double v[2] = {0.0, 0.0};
+    for (int p = blockIdx.x * KB + threadIdx.x; p < P; p += gridDim.x * KB) {
+        const int i = p / L.W, j = p - i * L.W;
+        const double yv = y[p];
+        v[0] += yv * L.g[p];
+        v[1] += yv * coarse_apply<double, float>(L.acx, L.acy, L.dc, y, L.H, L.W, i, j);
+    }
+    put_partials(v, part);
+}
+
+// L.y = gamma y, gamma = (y.g) / (y.A_L y): the energy-optimal step along the network's coarse
+// correction on level L (of either sign; 0 when y.A_L y is not positive); every block re-sums the
+// partials
+__global__ __launch_bounds__(KB) void k_ml_net_apply(Level L, const float* __restrict__ y, const KrylovState* st,
+                                                     const double* part, int nbp) {
+    if (st->done != DONE_RUNNING) return;
+    double v[2];
+    sum_partials_all(part, nbp, v);
+    const double t = v[0] / v[1];
+    const double gm = (v[1] > 0.0 && isfinite(t)) ? t : 0.0;  // either sign: the loss does not fix y's
+    const int P = L.H * L.W;
+    for (int p = blockIdx.x * KB + threadIdx.x; p < P; p += gridDim.x * KB) L.y[p] = gm * static_cast<double>(y[p]);
+}
+
 // the hierarchy below a first coarsening of H0 x W0 cells
 int ml_plan(int H0, int W0, int* H, int* W, int* c, int* nlev) {
     int n = 0;
@@ -1881,10 +1926,30 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
                       G.nx, G.ny);
             return ASD2_ERR_UNSUPPORTED;
         }
-        c0 = N->coarsen;
-    } else {
-        while (c0 < 8 && G.nx % (2 * c0) == 0 && G.ny % (2 * c0) == 0) c0 *= 2;
     }
+    while (c0 < 8 && G.nx % (2 * c0) == 0 && G.ny % (2 * c0) == 0) c0 *= 2;
+    // The network replaces the bottom of the V-cycle: it solves on the hierarchy's level NL whose
+    // coarsening equals the network's, the levels above it keep their Chebyshev smoothing.  A
+    // network coarsening the hierarchy does not reach (or c0 itself) makes its own first level.
+    int NL = 0;
+    if (learned && N->coarsen != c0) {
+        int Hl[ML_MAX_LEVELS], Wl[ML_MAX_LEVELS], cl[ML_MAX_LEVELS], nl = 0;
+        bool found = false;
+        if (N->coarsen > c0 && ml_plan(G.nx / c0, G.ny / c0, Hl, Wl, cl, &nl) == ASD2_OK) {
+            int cum = c0;
+            for (int l = 1; l < nl && !found; l++) {
+                cum *= cl[l - 1];
+                if (cum == N->coarsen) {
+                    NL = l;
+                    found = true;
+                }
+            }
+        }
+        if (!found) c0 = N->coarsen;
+    }
+    const bool hybrid = learned && NL > 0;
+    double* net_part = nullptr;  // block partials of the network level's gamma
+    int nbn = 1;
     const int64_t n = G.size();
     System S;
     ASD2_TRY(system_arrays(ctx, &S, false));
@@ -1920,9 +1985,14 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
     Hierarchy Hh{};
     double *Minv = nullptr, *Mwork = nullptr;
     float* yml = nullptr;  // the V-cycle's output on level 0 (FP32, read by the SpMV)
-    if (!learned) {
+    if (!learned || hybrid) {
         int Hl[ML_MAX_LEVELS], Wl[ML_MAX_LEVELS], cl[ML_MAX_LEVELS];
         ASD2_TRY(ml_plan(C.Hc, C.Wc, Hl, Wl, cl, &Hh.n));
+        if (hybrid) {
+            Hh.n = NL + 1;  // the network's level is the last one
+            nbn = std::max(1, std::min(64, ceil_div(static_cast<int64_t>(Hl[NL]) * Wl[NL], KB)));
+            ASD2_TRY(ws(ctx, "ml_net_part", 2 * nbn, &net_part));
+        }
         static const char* names[ML_MAX_LEVELS][7] = {
             {"ml0_acx", "ml0_acy", "ml0_dc", "ml0_g", "ml0_y1", "ml0_r1", "ml0_y"},
             {"ml1_acx", "ml1_acy", "ml1_dc", "ml1_g", "ml1_y1", "ml1_r1", "ml1_y"},
@@ -1953,9 +2023,11 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
             ASD2_TRY(ws(ctx, names[l][4], lp, &L.y1));
             ASD2_TRY(ws(ctx, names[l][6], lp, &L.y));
         }
-        const int m = Hh.L[Hh.n - 1].H * Hh.L[Hh.n - 1].W;
-        ASD2_TRY(ws(ctx, "ml_dense", static_cast<int64_t>(m) * m, &Minv));
-        ASD2_TRY(ws(ctx, "ml_dense_work", static_cast<int64_t>(m) * m, &Mwork));
+        if (!learned) {
+            const int m = Hh.L[Hh.n - 1].H * Hh.L[Hh.n - 1].W;
+            ASD2_TRY(ws(ctx, "ml_dense", static_cast<int64_t>(m) * m, &Minv));
+            ASD2_TRY(ws(ctx, "ml_dense_work", static_cast<int64_t>(m) * m, &Mwork));
+        }
         ASD2_TRY(ws(ctx, "ml_yout", Pc, &yml));
     }
     double* pb = lp_part;  // fine pass: r.D^-1 r, (D^-1 r).q_j, sum (g / dc)^2
@@ -1982,14 +2054,24 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
         }
     } restore{learned ? N : nullptr, saved};
     hipLaunchKernelGGL(k_lp_coarse_op, dim3(flat_blocks(cs)), dim3(KB), 0, ctx->stream, G, S.ax, S.ay, S.diag, C);
+    for (int l = 1; l < Hh.n; l++)
+        hipLaunchKernelGGL(k_ml_coarsen, dim3(flat_blocks(static_cast<int64_t>(Hh.L[l].H + 1) * (Hh.L[l].W + 1))),
+                           dim3(KB), 0, ctx->stream, Hh.L[l - 1], Hh.L[l]);
     if (learned) {
-        hipLaunchKernelGGL(k_lp_static_features, dim3(nbc), dim3(KB), 0, ctx->stream, C, N->feat, N->flin,
-                           static_cast<float>(G.dx * G.dx / C.c), static_cast<float>(G.dy * G.dy / C.c));
+        // the network's level as a coarse-grid view for the static features
+        Coarse CN = C;
+        if (hybrid) {
+            CN.Hc = Hh.L[NL].H;
+            CN.Wc = Hh.L[NL].W;
+            CN.acx = Hh.L[NL].acx;
+            CN.acy = Hh.L[NL].acy;
+            CN.dc = Hh.L[NL].dc;
+        }
+        hipLaunchKernelGGL(k_lp_static_features, dim3(flat_blocks(static_cast<int64_t>(CN.Hc) * CN.Wc)), dim3(KB), 0,
+                           ctx->stream, CN, N->feat, N->flin, static_cast<float>(G.dx * G.dx / N->coarsen),
+                           static_cast<float>(G.dy * G.dy / N->coarsen));
         ASD2_TRY(unet_forward(N, 1));  // the ReLU pattern of this system, fixed for the solve
     } else {
-        for (int l = 1; l < Hh.n; l++)
-            hipLaunchKernelGGL(k_ml_coarsen, dim3(flat_blocks(static_cast<int64_t>(Hh.L[l].H + 1) * (Hh.L[l].W + 1))),
-                               dim3(KB), 0, ctx->stream, Hh.L[l - 1], Hh.L[l]);
         const Level& Lc = Hh.L[Hh.n - 1];
         const int m = Lc.H * Lc.W;
         // nl launches alternate A -> B -> A ...; the matrix is built where the last one ends in Minv
@@ -2012,26 +2094,94 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
     // directions kept: the network's coarse step varies with r (gamma), so the learned
     // preconditioner keeps FCG_M; the V-cycle's is the same linear map in every iteration and
     // one direction makes the recurrence PCG
-    const int mwin = learned ? FCG_M : 1;
+    const int mwin = (learned && !hybrid) ? FCG_M : (hybrid ? HYBRID_M : 1);
+    const bool many = mwin > 1;  // the kernels' direction count: FCG_M or 1
     int wstart = 0;  // first iteration of the current window of directions
     auto nwin = [&](int k) { return std::max(0, std::min(mwin, k - wstart)); };
     const Cheb ch = ml_cheb();  // the V-cycle's smoother
-    const float* ycoarse = learned ? static_cast<const float*>(N->y) : yml;
+    const float* ycoarse = (learned && !hybrid) ? static_cast<const float*>(N->y) : yml;
     const int first = learned ? 0 : ml_first_lower(Hh);
+    // recording of one level's coarse-correction input (training data), armed by
+    // asd2_poisson_record_coarse for this solve
+    int rec_level = -1;
+    if (ctx->rec_dst) {
+        const int top = learned ? (hybrid ? NL : 0) : first;
+        for (int l = 0; l <= top && l < std::max(Hh.n, 1); l++) {
+            const int H = Hh.n > 0 ? Hh.L[l].H : C.Hc, W = Hh.n > 0 ? Hh.L[l].W : C.Wc;
+            if (H == ctx->rec_h && W == ctx->rec_w) rec_level = l;
+        }
+        if (rec_level < 0) {
+            ctx->rec_dst = nullptr;
+            ctx->rec_iters.clear();
+            set_error("no level of %d x %d cells whose input this solve computes", ctx->rec_h, ctx->rec_w);
+            return ASD2_ERR_ARG;
+        }
+    }
+    struct RecordClear {
+        asd2_ctx* c;
+        ~RecordClear() {
+            c->rec_dst = nullptr;
+            c->rec_iters.clear();
+        }
+    } record_clear{ctx};
+    auto record = [&](int level, int iter) -> int {
+        if (level != rec_level) return ASD2_OK;
+        const double* src = (Hh.n > 0) ? Hh.L[level].g : C.g;
+        const int64_t cells = static_cast<int64_t>(ctx->rec_h) * ctx->rec_w;
+        for (size_t k = 0; k < ctx->rec_iters.size(); k++)
+            if (ctx->rec_iters[k] == iter)
+                ASD2_HIP(hipMemcpyAsync(ctx->rec_dst + static_cast<int64_t>(k) * cells, src, cells * 8,
+                                        hipMemcpyDeviceToDevice, ctx->stream));
+        return ASD2_OK;
+    };
     // count iterations; the last update pass leaves the state in snap for the host's poll
     auto batch = [&](int count, KrylovState* snap) -> int {
         for (int k = 0; k < count; k++, it++) {
             float* r = rb[it & 1];
             if (fresh) {
-                (learned ? launch_lp_pass<false, FCG_M> : launch_lp_pass<false, 1>)(ctx->stream, nbb, G, C, r, nullptr, x, dirs_for(R, it), S.dinv32, S.ax32, S.ay32,
+                (many ? launch_lp_pass<false, FCG_M> : launch_lp_pass<false, 1>)(ctx->stream, nbb, G, C, r, nullptr, x, dirs_for(R, it), S.dinv32, S.ax32, S.ay32,
                                       D.st, D.ps, nbs, nullptr, it, nwin(it), pb, nullptr);
                 fresh = false;
             }
             if (use_net) {
                 prof_begin(ctx, ASD2_KID_COARSE);
-                if (learned) {
+                ASD2_TRY(record(0, it));
+                if (hybrid) {
+                    for (int l = 0; l < NL; l++) {
+                        if (ml_small_tiles(Hh.L[l], l)) {
+                            const int nt = ceil_div(Hh.L[l].H, ML_TS) * ceil_div(Hh.L[l].W, ML_TS);
+                            hipLaunchKernelGGL(k_ml_down<ML_TS>, dim3(std::min(nt, 4096)), dim3(ML_TS * ML_TS), 0,
+                                               ctx->stream, Hh.L[l], Hh.L[l + 1], D.st, ch);
+                        } else {
+                            const int nt = ceil_div(Hh.L[l].H, ML_T) * ceil_div(Hh.L[l].W, ML_T);
+                            hipLaunchKernelGGL(k_ml_down<ML_T>, dim3(std::min(nt, 4096)), dim3(KB), 0, ctx->stream,
+                                               Hh.L[l], Hh.L[l + 1], D.st, ch);
+                        }
+                        ASD2_TRY(record(l + 1, it));
+                    }
+                    hipLaunchKernelGGL(k_ml_net_pack, dim3(nbn), dim3(KB), 0, ctx->stream, Hh.L[NL], N->flin, D.st);
+                    ASD2_TRY(unet_forward_linear(N, 1, false));
+                    hipLaunchKernelGGL(k_ml_net_dots, dim3(nbn), dim3(KB), 0, ctx->stream, Hh.L[NL], N->y, D.st,
+                                       net_part);
+                    hipLaunchKernelGGL(k_ml_net_apply, dim3(nbn), dim3(KB), 0, ctx->stream, Hh.L[NL], N->y, D.st,
+                                       net_part, nbn);
+                    for (int l = NL - 1; l >= 0; l--) {
+                        if (ml_small_tiles(Hh.L[l], l)) {
+                            const int nt = ceil_div(Hh.L[l].H, ML_TS) * ceil_div(Hh.L[l].W, ML_TS);
+                            hipLaunchKernelGGL(k_ml_up<ML_TS>, dim3(std::min(nt, 4096)), dim3(ML_TS * ML_TS), 0,
+                                               ctx->stream, Hh.L[l], Hh.L[l + 1], static_cast<float*>(nullptr), C.rc,
+                                               C.qc[0], D.st, static_cast<double*>(nullptr), ch);
+                        } else {
+                            hipLaunchKernelGGL(k_ml_up<ML_T>,
+                                               dim3(l == 0 ? nbc : flat_blocks(static_cast<int64_t>(Hh.L[l].H) * Hh.L[l].W)),
+                                               dim3(KB), 0, ctx->stream, Hh.L[l], Hh.L[l + 1],
+                                               l == 0 ? yml : static_cast<float*>(nullptr), C.rc, C.qc[0], D.st,
+                                               l == 0 ? pc : static_cast<double*>(nullptr), ch);
+                        }
+                    }
+                } else if (learned) {
                     hipLaunchKernelGGL(k_lp_pack, dim3(nbc), dim3(KB), 0, ctx->stream, C, N->flin, D.st, pb, nbb);
-                    ASD2_TRY(unet_forward_linear(N, 1));
+                    ASD2_TRY(unet_forward_linear(N, 1, false));
                 } else if (Hh.n == 1) {
                     hipLaunchKernelGGL(k_ml_dense_only, dim3(1), dim3(KB), 0, ctx->stream, Hh.L[0], Minv, yml, D.st);
                 } else {
@@ -2045,6 +2195,7 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
                             hipLaunchKernelGGL(k_ml_down<ML_T>, dim3(std::min(nt, 4096)), dim3(KB), 0, ctx->stream,
                                                Hh.L[l], Hh.L[l + 1], D.st, ch);
                         }
+                        ASD2_TRY(record(l + 1, it));
                     }
                     if (first == Hh.n - 1)
                         hipLaunchKernelGGL(k_ml_dense_apply, dim3(ceil_div(Hh.L[first].H * Hh.L[first].W, KB / 64)),
@@ -2072,13 +2223,14 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
                 prof_end(ctx, ASD2_KID_COARSE);
             }
             prof_begin(ctx, ASD2_KID_FCG_SPMV);
-            (learned ? launch_lp_spmv<FCG_M> : launch_lp_spmv<1>)(
+            (many ? launch_lp_spmv<FCG_M> : launch_lp_spmv<1>)(
                 ctx->stream, nbs, G, C, S.ax32, S.ay32, S.diag32, ctx->dres_ring_only ? 1 : 0, r, S.dinv32,
                 use_net ? ycoarse : nullptr,
-                dirs_for(R, it), D.st, pb, nbb, pc, nbc, D.ps, it, nwin(it), learned ? 0.0 : 1.0, D.pu, nbu);
+                dirs_for(R, it), D.st, pb, nbb, pc, nbc, D.ps, it, nwin(it), (learned && !hybrid) ? 0.0 : 1.0, D.pu,
+                nbu);
             prof_end(ctx, ASD2_KID_FCG_SPMV);
             prof_begin(ctx, ASD2_KID_FCG_UPDATE);
-            (learned ? launch_lp_pass<true, FCG_M> : launch_lp_pass<true, 1>)(ctx->stream, nbb, G, C, r, rb[(it + 1) & 1], x, dirs_for(R, it + 1), S.dinv32, S.ax32, S.ay32,
+            (many ? launch_lp_pass<true, FCG_M> : launch_lp_pass<true, 1>)(ctx->stream, nbb, G, C, r, rb[(it + 1) & 1], x, dirs_for(R, it + 1), S.dinv32, S.ax32, S.ay32,
                                  D.st, D.ps, nbs, D.pu, it + 1, nwin(it + 1), pb, k + 1 == count ? snap : nullptr);
             prof_end(ctx, ASD2_KID_FCG_UPDATE);
             nbu = nbb;
@@ -2202,6 +2354,19 @@ int asd2_poisson_solve(asd2_ctx* ctx, const asd2_solve_opts* opts, double* d_ps,
 int asd2_set_preconditioner(asd2_ctx* ctx, asd2_unet* net) {
     ASD2_TRY(check_ctx(ctx));
     ctx->net = net;
+    return ASD2_OK;
+}
+
+int asd2_poisson_record_coarse(asd2_ctx* ctx, int level_h, int level_w, int count, const int* iters, double* d_out) {
+    ASD2_TRY(check_ctx(ctx));
+    if (count < 0 || (count > 0 && (!iters || !d_out)) || level_h <= 0 || level_w <= 0) {
+        set_error("bad argument");
+        return ASD2_ERR_ARG;
+    }
+    ctx->rec_h = level_h;
+    ctx->rec_w = level_w;
+    ctx->rec_iters.assign(iters, iters + count);
+    ctx->rec_dst = count > 0 ? d_out : nullptr;
     return ASD2_OK;
 }
 

@@ -114,7 +114,9 @@ __device__ __forceinline__ T coarse_apply(const T* __restrict__ acx, const T* __
 int unet_forward(::asd2_unet* net, int nimg);
 // linear pass from net->flin (BF16 [nimg][H][W][4], channels 1..3 zero) to net->y: every conv
 // output multiplied by (mask > 0) with the masks of the last unet_forward, no biases
-int unet_forward_linear(::asd2_unet* net, int nimg);
+// keep_pooled = false forms the pooled inputs inside the convolutions' loads (same values) and
+// stores none; the backward pass needs them stored
+int unet_forward_linear(::asd2_unet* net, int nimg, bool keep_pooled = true);
 
 }  // namespace asd2
 

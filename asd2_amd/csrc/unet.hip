@@ -52,17 +52,33 @@ __device__ __forceinline__ float hi16(uint32_t u) { return bf2f(static_cast<uint
 
 // ------------------------------------------------------------------ tensors
 // Input of a convolution: [A (H x W, ca channels), nearest-up(B) (H/2 x W/2, cb channels)].
+// pool = 1: A is read from the 2H x 2W tensor it is the 2 x 2 mean of, computed on the fly with the
+// arithmetic of k_pool2 (same sums, same rounding), so no pooled copy is needed.
 struct Src {
     const uint16_t* a;
     int ca;
     const uint16_t* b;
     int cb;
+    int pool = 0;
 };
 
 // channels [c, c + 4) at pixel (img, i, j) of an H x W image stack; zeros outside the image
 __device__ __forceinline__ uint2 load4(const Src& s, int img, int i, int j, int H, int W, int c) {
     if (i < 0 || i >= H || j < 0 || j >= W) return make_uint2(0u, 0u);
     if (c < s.ca) {
+        if (s.pool) {
+            float t[4] = {0.f, 0.f, 0.f, 0.f};
+            for (int a = 0; a < 2; a++)
+                for (int b = 0; b < 2; b++) {
+                    const int64_t p = (static_cast<int64_t>(img) * (2 * H) + 2 * i + a) * (2 * W) + 2 * j + b;
+                    const uint2 v = *reinterpret_cast<const uint2*>(s.a + p * s.ca + c);
+                    t[0] += lo16(v.x);
+                    t[1] += hi16(v.x);
+                    t[2] += lo16(v.y);
+                    t[3] += hi16(v.y);
+                }
+            return pack4(0.25f * t[0], 0.25f * t[1], 0.25f * t[2], 0.25f * t[3]);
+        }
         const int64_t p = (static_cast<int64_t>(img) * H + i) * W + j;
         return *reinterpret_cast<const uint2*>(s.a + p * s.ca + c);
     }
@@ -614,6 +630,22 @@ __global__ __launch_bounds__(256) void k_tr_dy(const float* __restrict__ y, cons
     }
 }
 
+// coarse training samples: sums 0 = g.D_c^-1 g (the loss normaliser), 1 = 0, 2 = sum (g / dc)^2
+__global__ __launch_bounds__(256) void k_tr_gnorm(const float* __restrict__ g, const float* __restrict__ dc, int Hc,
+                                                  int Wc, float* __restrict__ sums) {
+    const int b = blockIdx.y;
+    const int64_t Pc = static_cast<int64_t>(Hc) * Wc, cs = static_cast<int64_t>(Hc + 1) * (Wc + 1);
+    float a0 = 0.f, a2 = 0.f;
+    for (int64_t p = blockIdx.x * 256LL + threadIdx.x; p < Pc; p += static_cast<int64_t>(gridDim.x) * 256) {
+        const int I = static_cast<int>(p / Wc), J = static_cast<int>(p - static_cast<int64_t>(I) * Wc);
+        const float gv = g[b * Pc + p], d = dc[b * cs + I * (Wc + 1) + J];
+        a0 += gv * gv / d;
+        a2 += (gv / d) * (gv / d);
+    }
+    wave_atomic(a0, &sums[TS * b + 0]);
+    wave_atomic(a2, &sums[TS * b + 2]);
+}
+
 // FP32 features [P][4] -> feat (channel 0 zeroed) and flin (channel 0 only), BF16
 __global__ void k_split_features(const float* __restrict__ in, uint16_t* __restrict__ f, uint16_t* __restrict__ fl,
                                  int64_t P) {
@@ -832,24 +864,26 @@ int unet_forward(asd2_unet* N, int nimg) {
     return ASD2_OK;
 }
 
-int unet_forward_linear(asd2_unet* N, int nimg) {
+int unet_forward_linear(asd2_unet* N, int nimg, bool keep_pooled) {
     const int H = N->H, W = N->W;
     const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
     hipStream_t s = N->stream;
     uint16_t** a = N->lin;
+    // the pooled inputs of convs 2, 4, 6: stored (the backward pass reads them) or formed in the
+    // convolution's loads
+    auto pooled = [&](int k, const uint16_t* fine, int c, int Hf, int Wf) -> Src {
+        if (!keep_pooled) return Src{fine, c, nullptr, 0, 1};
+        hipLaunchKernelGGL(k_pool2, dim3(blocks_for(static_cast<int64_t>(nimg) * (Hf / 2) * (Wf / 2) * (c / 4))),
+                           dim3(256), 0, s, fine, N->linpooled[k], nimg, Hf, Wf, c);
+        return Src{N->linpooled[k], c, nullptr, 0};
+    };
     conv_forward_lin(N, 0, Src{N->flin, 4, nullptr, 0}, nimg, H, W);
     conv_forward_lin(N, 1, Src{a[0], 16, nullptr, 0}, nimg, H, W);
-    hipLaunchKernelGGL(k_pool2, dim3(blocks_for(static_cast<int64_t>(nimg) * H1 * W1 * 4)), dim3(256), 0, s, a[1],
-                       N->linpooled[0], nimg, H, W, 16);
-    conv_forward_lin(N, 2, Src{N->linpooled[0], 16, nullptr, 0}, nimg, H1, W1);
+    conv_forward_lin(N, 2, pooled(0, a[1], 16, H, W), nimg, H1, W1);
     conv_forward_lin(N, 3, Src{a[2], 32, nullptr, 0}, nimg, H1, W1);
-    hipLaunchKernelGGL(k_pool2, dim3(blocks_for(static_cast<int64_t>(nimg) * H2 * W2 * 8)), dim3(256), 0, s, a[3],
-                       N->linpooled[1], nimg, H1, W1, 32);
-    conv_forward_lin(N, 4, Src{N->linpooled[1], 32, nullptr, 0}, nimg, H2, W2);
+    conv_forward_lin(N, 4, pooled(1, a[3], 32, H1, W1), nimg, H2, W2);
     conv_forward_lin(N, 5, Src{a[4], 64, nullptr, 0}, nimg, H2, W2);
-    hipLaunchKernelGGL(k_pool2, dim3(blocks_for(static_cast<int64_t>(nimg) * H3 * W3 * 16)), dim3(256), 0, s, a[5],
-                       N->linpooled[2], nimg, H2, W2, 64);
-    conv_forward_lin(N, 6, Src{N->linpooled[2], 64, nullptr, 0}, nimg, H3, W3);
+    conv_forward_lin(N, 6, pooled(2, a[5], 64, H2, W2), nimg, H3, W3);
     conv_forward_lin(N, 7, Src{a[6], 128, nullptr, 0}, nimg, H3, W3);
     conv_forward_lin(N, 8, Src{a[5], 64, a[7], 128}, nimg, H2, W2);
     conv_forward_lin(N, 9, Src{a[8], 64, nullptr, 0}, nimg, H2, W2);
@@ -1136,6 +1170,52 @@ int asd2_unet_backward(asd2_unet* N, int batch, const float* d_dy) {
 
 // One Adam step on the energy loss of `batch` systems, each in the solver's FP32 padded
 // layout ((nx + 6) (ny + 6) values per array, concatenated, nx = H c): r, dinv, ax, ay, diag.
+int asd2_unet_train_step_coarse(asd2_unet* N, int batch, const float* d_g, const float* d_acx, const float* d_acy,
+                                const float* d_dc, double h2x, double h2y, float lr, float* loss_out) {
+    ASD2_TRY(check_net(N));
+    if (batch <= 0 || batch > N->max_batch || !d_g || !d_acx || !d_acy || !d_dc || !(lr > 0.f) || !(h2x > 0.0) ||
+        !(h2y > 0.0)) {
+        set_error("bad argument");
+        return ASD2_ERR_ARG;
+    }
+    const int Hc = N->H, Wc = N->W;
+    const int64_t Pc = static_cast<int64_t>(Hc) * Wc;
+    hipStream_t s = N->stream;
+    float* sums = N->scratch;  // [TS B]
+    ASD2_HIP(hipMemsetAsync(sums, 0, TS * batch * sizeof(float), s));
+    ASD2_HIP(hipMemsetAsync(N->grad, 0, N->L.total * sizeof(float), s));
+    const dim3 gc(blocks_for(Pc, 256, 1024), batch);
+    hipLaunchKernelGGL(k_tr_gnorm, gc, dim3(256), 0, s, d_g, d_dc, Hc, Wc, sums);
+    hipLaunchKernelGGL(k_tr_features, gc, dim3(256), 0, s, d_g, d_acx, d_acy, d_dc, Hc, Wc, static_cast<float>(h2x),
+                       static_cast<float>(h2y), sums, N->feat, N->flin);
+    ASD2_TRY(unet_forward(N, batch));
+    ASD2_TRY(unet_forward_linear(N, batch));
+    hipLaunchKernelGGL(k_tr_gamma, gc, dim3(256), 0, s, N->y, d_g, d_acx, d_acy, d_dc, Hc, Wc, sums);
+    hipLaunchKernelGGL(k_tr_dy, gc, dim3(256), 0, s, N->y, d_g, d_acx, d_acy, d_dc, Hc, Wc, sums, N->dy, batch);
+    ASD2_TRY(unet_backward(N, batch, true));
+    N->adam_t += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - std::pow(b1, static_cast<float>(N->adam_t)), c2 = 1.f - std::pow(b2, static_cast<float>(N->adam_t));
+    hipLaunchKernelGGL(k_adam, dim3(blocks_for(N->L.total, 256, 4096)), dim3(256), 0, s, N->w32, N->grad, N->adam_m,
+                       N->adam_v, N->L.total, lr, b1, b2, 1e-8f, c1, c2, 1.0f);
+    ASD2_TRY(unet_refresh(N));
+    std::vector<float> h(static_cast<size_t>(TS * batch));
+    ASD2_HIP(hipMemcpyAsync(h.data(), sums, h.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+    ASD2_HIP(hipStreamSynchronize(s));
+    if (loss_out) {
+        // per sample -(y.g)^2 / (y.A_c y) / g.D_c^-1 g: the energy of the best step along the network's
+        // correction, relative to the Jacobi scale of the sample
+        double L = 0.0;
+        for (int b = 0; b < batch; b++) {
+            const double nrm = h[TS * b], yg = h[TS * b + 3], yay = h[TS * b + 4];
+            if (nrm > 0.0 && yay > 0.0) L -= yg * yg / yay / nrm;
+        }
+        *loss_out = static_cast<float>(L / batch);
+    }
+    ASD2_LAUNCH_CHECK();
+    return ASD2_OK;
+}
+
 int asd2_unet_train_step(asd2_unet* N, int batch, const float* d_r, const float* d_dinv, const float* d_ax,
                          const float* d_ay, const float* d_dg, double dx, double dy, float lr, float* loss_out) {
     ASD2_TRY(check_net(N));

@@ -19,7 +19,8 @@ import time
 
 import numpy as np
 
-from . import GRID_P, GRID_U, GRID_V, PREC_MIXED, PRECOND_MULTILEVEL, Context, DeviceArray, UNet, make_fluid
+from . import (GRID_P, GRID_U, GRID_V, PREC_MIXED, PRECOND_LEARNED, PRECOND_MULTILEVEL, Context, DeviceArray,
+               UNet, make_fluid)
 
 G = 3
 
@@ -194,3 +195,147 @@ def load_params(net: UNet, path: str):
     if v.size != net.nparams:
         raise ValueError(f"{path} holds {v.size} parameters, the network has {net.nparams}")
     net.set_params(v)
+
+
+# ---------------------------------------------------------------------------- coarse-level training
+def coarse_operator(ax, ay, dg, c, g=G):
+    """P^T A P of c x c blocks from padded FP64 fine arrays (ax / ay = W / S face coefficient of each
+    cell), in the layout of the solver's level arrays: (H + 1) x (W + 1), acx = W-face coefficient,
+    acy = S-face coefficient, dc = diagonal (the block's fine diagonals plus twice its inner faces)"""
+    nx, ny = ax.shape[0] - 2 * g, ax.shape[1] - 2 * g
+    Hc, Wc = nx // c, ny // c
+    AX, AY, D = ax[g:g + nx, g:g + ny], ay[g:g + nx, g:g + ny], dg[g:g + nx, g:g + ny]
+    acx = np.zeros((Hc + 1, Wc + 1))
+    acy = np.zeros((Hc + 1, Wc + 1))
+    dc = np.zeros((Hc + 1, Wc + 1))
+    acx[:Hc, :Wc] = AX[::c, :].reshape(Hc, Wc, c).sum(axis=2)
+    acy[:Hc, :Wc] = AY[:, ::c].reshape(Hc, c, Wc).sum(axis=1)
+    inner_i = (np.arange(nx) % c != 0)[:, None]
+    inner_j = (np.arange(ny) % c != 0)[None, :]
+    Dn = D + 2.0 * AX * inner_i + 2.0 * AY * inner_j
+    dc[:Hc, :Wc] = Dn.reshape(Hc, c, Wc, c).sum(axis=(1, 3))
+    return acx, acy, dc
+
+
+class CoarseSamples:
+    """Coarse training systems on the device (FP32): g [S][H][W], acx / acy / dc [S][(H + 1)(W + 1)]"""
+
+    def __init__(self, g, acx, acy, dc, h2):
+        self.count = g.shape[0]
+        self.H, self.W = g.shape[1], g.shape[2]
+        self.h2 = h2
+        self.dev = [DeviceArray.from_host(np.ascontiguousarray(a, np.float32)) for a in (g, acx, acy, dc)]
+        self.strides = [self.H * self.W * 4] + [(self.H + 1) * (self.W + 1) * 4] * 3
+
+    def batch(self, start):
+        return [_View(d.ptr + start * st) for d, st in zip(self.dev, self.strides)]
+
+
+def make_coarse_samples(n, coarsen, systems=16, iters=(0, 2, 5, 10, 20, 40, 70, 100), seed=0, net=None, log=None,
+                        **kw):
+    """Coarse-correction inputs g the solver itself produces on the level of coarsening `coarsen`
+    (recorded by asd2_poisson_record_coarse at the given iterations of a multilevel solve, or of
+    the learned solve with `net` attached), with that level's Galerkin operator, for `systems`
+    random droplet systems on an n x n grid."""
+    rng = np.random.default_rng(seed)
+    ctx = system_context(n)
+    H = W = n // coarsen
+    rec = DeviceArray((len(iters), H, W))
+    gs, ops = [], []
+    t0 = time.perf_counter()
+    for s in range(systems):
+        build_system(ctx, rng, **kw)
+        rec.upload(np.zeros((len(iters), H, W)))
+        ctx.record_coarse(H, W, list(iters), rec)
+        if net is not None:
+            ctx.set_preconditioner(net)
+        ctx.poisson_solve(ctx.array(), Context.opts(tol=1e-10, max_iter=max(iters) + 1, precision=PREC_MIXED,
+                                                    precond=PRECOND_LEARNED if net is not None else PRECOND_MULTILEVEL),
+                          raise_on_fail=False)
+        g = rec.download()
+        ax, ay, dg, _ = ctx.gfm_export()
+        op = coarse_operator(ax, ay, dg, coarsen)
+        for k in range(len(iters)):
+            if np.any(g[k] != 0.0):
+                gs.append(g[k])
+                ops.append(op)
+        if log:
+            log(f"system {s + 1}/{systems}: {len(gs)} samples ({time.perf_counter() - t0:.1f} s)")
+    order = rng.permutation(len(gs))
+    g = np.stack([gs[i] for i in order])
+    acx, acy, dc = (np.stack([ops[i][k] for i in order]) for k in range(3))
+    return CoarseSamples(g, acx, acy, dc, (1.0 / n) ** 2 / coarsen)
+
+
+def train_coarse(net: UNet, samples: CoarseSamples, steps=2000, batch=8, lr=2e-3, lr_final=5e-4, seed=0, log=None):
+    """Adam on the coarse energy loss over random contiguous windows of the samples; returns the
+    per-step losses"""
+    rng = np.random.default_rng(seed)
+    batch = min(batch, samples.count, net.max_batch)
+    losses = []
+    for step in range(steps):
+        start = int(rng.integers(0, samples.count - batch + 1))
+        rate = lr if step < 0.75 * steps else lr_final
+        losses.append(net.train_step_coarse(batch, *samples.batch(start), samples.h2, samples.h2, rate))
+        if log and (step % 200 == 0 or step == steps - 1):
+            log(f"step {step}: loss {losses[-1]:.4f}")
+    return losses
+
+
+# droplet families of the training systems (keyword arguments of build_system): the seeded-droplet
+# distribution, larger droplets with surface tension, and one to three large droplets at high
+# density ratio (the stationary-droplet step's kind of system)
+FAMILIES = {
+    "seeded": {},
+    "large": dict(radius=(0.02, 0.3), sigma=(0.0, 1.0)),
+    "few_large": dict(count=(1, 3), radius=(0.15, 0.3), ratio=(100.0, 1000.0), sigma=(0.5, 1.5)),
+}
+
+
+def train_hybrid(n=2048, coarsen=32, systems=(32, 16, 16), steps=3000, rounds=2, seed=0, log=None):
+    """The network for the bottom of the n x n hierarchy (the level of coarsening `coarsen`): round 1
+    trains on coarse-correction inputs recorded from multilevel solves, every further round adds
+    inputs recorded from the learned solve with the network of the round before (its own input
+    distribution).  Returns the network and a record of the training."""
+    net = UNet(n, n, max_batch=8, seed=seed + 7, coarsen=coarsen)
+    record = {"grid": n, "coarsen": coarsen, "image": [n // coarsen, n // coarsen], "rounds": []}
+    pools = []
+    for rnd in range(rounds):
+        t0 = time.perf_counter()
+        new = []
+        for k, (fam, count) in enumerate(zip(FAMILIES, systems)):
+            new.append(make_coarse_samples(n, coarsen, systems=count, seed=seed + 1000 * rnd + 100 * k,
+                                           net=net if rnd > 0 else None, **FAMILIES[fam]))
+        pools.extend(new)
+        t1 = time.perf_counter()
+        losses = []
+        schedule = [(p, steps if i == len(pools) - len(new) else steps // 2) for i, p in enumerate(pools)]
+        for i, (p, k) in enumerate(schedule):
+            lr = 2e-3 if rnd == 0 else 1e-3
+            losses += train_coarse(net, p, steps=k, batch=8, seed=seed + 17 * rnd + i, lr=lr, lr_final=lr / 4)
+        t2 = time.perf_counter()
+        rec = {"samples": [p.count for p in new], "data_s": t1 - t0, "train_s": t2 - t1, "steps": len(losses),
+               "loss_first": float(losses[0]), "loss_last100": float(np.mean(losses[-100:]))}
+        record["rounds"].append(rec)
+        if log:
+            log(f"round {rnd + 1}: {rec}")
+    return net, record
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="train the learned coarse solve on the GPU and save its parameters")
+    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--coarsen", type=int, default=32)
+    ap.add_argument("--systems", default="32,16,16", help="training systems per family and round")
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--rounds", type=int, default=2)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", required=True)
+    a = ap.parse_args()
+    net, rec = train_hybrid(a.n, a.coarsen, tuple(int(s) for s in a.systems.split(",")), a.steps, a.rounds, a.seed,
+                            log=lambda m: print(m, flush=True))
+    rec["families"] = {k: {kk: list(vv) for kk, vv in v.items()} for k, v in FAMILIES.items()}
+    save_params(net, a.out, rec)
+    print(json.dumps(rec))

@@ -302,8 +302,28 @@ int asd2_unet_backward_linear(asd2_unet* net, int batch, const float* d_dy);
  * step */
 int asd2_unet_train_step(asd2_unet* net, int batch, const float* d_r, const float* d_dinv, const float* d_ax,
                          const float* d_ay, const float* d_dg, double dx, double dy, float lr, float* loss_out);
-/* network used by asd2_poisson_solve with precond = ASD2_PRECOND_LEARNED (not owned; NULL detaches) */
+/* one Adam step on coarse systems given directly (no reference counterpart): per sample the coarse
+ * residual g (FP32 [batch][H][W]) and the coarse operator in the layout of the level arrays (FP32
+ * [batch][(H + 1) (W + 1)] each: W-face and S-face coefficients and diagonal, stride W + 1), H x W the
+ * network's image size; h2x, h2y = dx^2 / c, dy^2 / c for the coefficient features (c the level's
+ * coarsening of the fine grid).  Loss: the mean of -(y.g)^2 / (y.A_c y) / (g.D_c^-1 g), the energy
+ * of the best step along the network's coarse correction; loss_out = loss before the step */
+int asd2_unet_train_step_coarse(asd2_unet* net, int batch, const float* d_g, const float* d_acx, const float* d_acy,
+                                const float* d_dc, double h2x, double h2y, float lr, float* loss_out);
+/* network used by asd2_poisson_solve with precond = ASD2_PRECOND_LEARNED (not owned; NULL detaches).
+ * The network solves on the level of the multilevel hierarchy (first coarsening c0 = 8 x 8 blocks,
+ * then 4 x 4) whose coarsening equals the network's, with the levels above it smoothed as in
+ * ASD2_PRECOND_MULTILEVEL, its output scaled by the energy-optimal step on that level; a network
+ * coarsening c0 or one the hierarchy does not reach is the coarse correction of the fine grid
+ * itself (z = D^-1 r + gamma P y). */
 int asd2_set_preconditioner(asd2_ctx* ctx, asd2_unet* net);
+/* Training data from the solver (no reference counterpart): the next asd2_poisson_solve with
+ * ASD2_PRECOND_MULTILEVEL or ASD2_PRECOND_LEARNED copies the input g of its coarse correction on the
+ * hierarchy level of level_h x level_w cells at iterations iters[0 .. count) into
+ * d_out[k * level_h * level_w] (FP64, row-major); slots of iterations the solve does not reach keep
+ * their contents.  The request is cleared by that solve; it fails (ASD2_ERR_ARG) when the solve
+ * computes no input of that size. */
+int asd2_poisson_record_coarse(asd2_ctx* ctx, int level_h, int level_w, int count, const int* iters, double* d_out);
 
 /* ---------------------------------------------------------------- timing */
 /* records the start/stop of a region on the context's stream (hipEvents) */
