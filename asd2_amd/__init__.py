@@ -158,8 +158,8 @@ def load() -> C.CDLL:
         "asd2_unet_train_step": ([_VP, C.c_int, _VP, _VP, _VP, _VP, _VP, C.c_double, C.c_double, C.c_float,
                                   P(C.c_float)], C.c_int),
         "asd2_set_preconditioner": ([_VP, _VP], C.c_int),
-        "asd2_unet_train_step_coarse": ([_VP, C.c_int, _VP, _VP, _VP, _VP, C.c_double, C.c_double, C.c_float,
-                                         C.POINTER(C.c_float)], C.c_int),
+        "asd2_unet_train_step_coarse": ([_VP, C.c_int, _VP, _VP, _VP, _VP, _VP, C.c_double, C.c_double,
+                                         C.c_float, C.POINTER(C.c_float)], C.c_int),
         "asd2_poisson_record_coarse": ([_VP, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), _VP], C.c_int),
         "asd2_timer_start": ([_VP], C.c_int),
         "asd2_timer_stop": ([_VP, P(C.c_double)], C.c_int),
@@ -316,12 +316,13 @@ class UNet:
         d_dy = DeviceArray.from_host(dy)
         check(self.lib.asd2_unet_backward_linear(self.h, dy.shape[0], d_dy.ptr))
 
-    def train_step_coarse(self, batch, d_g, d_acx, d_acy, d_dc, h2x, h2y, lr):
+    def train_step_coarse(self, batch, d_g, d_acx, d_acy, d_dc, h2x, h2y, lr, d_energy=None):
         """one Adam step on `batch` coarse systems (device FP32: g [batch][H][W], acx / acy / dc
-        [batch][(H + 1) (W + 1)]); returns the loss before the step"""
+        [batch][(H + 1) (W + 1)], energy g.A^-1 g [batch] or None); returns the loss before the step"""
         loss = C.c_float(0.0)
-        check(self.lib.asd2_unet_train_step_coarse(self.h, batch, d_g.ptr, d_acx.ptr, d_acy.ptr, d_dc.ptr, h2x, h2y,
-                                                   lr, C.byref(loss)))
+        check(self.lib.asd2_unet_train_step_coarse(self.h, batch, d_g.ptr, d_acx.ptr, d_acy.ptr, d_dc.ptr,
+                                                   d_energy.ptr if d_energy is not None else None, h2x, h2y, lr,
+                                                   C.byref(loss)))
         return loss.value
 
     def train_step(self, batch, d_r, d_dinv, d_ax, d_ay, d_dg, dx, dy, lr):

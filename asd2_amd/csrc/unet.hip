@@ -630,10 +630,12 @@ __global__ __launch_bounds__(256) void k_tr_dy(const float* __restrict__ y, cons
     }
 }
 
-// coarse training samples: sums 0 = g.D_c^-1 g (the loss normaliser), 1 = 0, 2 = sum (g / dc)^2
+// coarse training samples: sums 0 = the loss normaliser (energy[b] = g.A_c^-1 g when given, else
+// g.D_c^-1 g), 1 = 0, 2 = sum (g / dc)^2
 __global__ __launch_bounds__(256) void k_tr_gnorm(const float* __restrict__ g, const float* __restrict__ dc, int Hc,
-                                                  int Wc, float* __restrict__ sums) {
+                                                  int Wc, const float* __restrict__ energy, float* __restrict__ sums) {
     const int b = blockIdx.y;
+    if (energy && blockIdx.x == 0 && threadIdx.x == 0) sums[TS * b + 0] = energy[b];
     const int64_t Pc = static_cast<int64_t>(Hc) * Wc, cs = static_cast<int64_t>(Hc + 1) * (Wc + 1);
     float a0 = 0.f, a2 = 0.f;
     for (int64_t p = blockIdx.x * 256LL + threadIdx.x; p < Pc; p += static_cast<int64_t>(gridDim.x) * 256) {
@@ -642,7 +644,7 @@ __global__ __launch_bounds__(256) void k_tr_gnorm(const float* __restrict__ g, c
         a0 += gv * gv / d;
         a2 += (gv / d) * (gv / d);
     }
-    wave_atomic(a0, &sums[TS * b + 0]);
+    if (!energy) wave_atomic(a0, &sums[TS * b + 0]);
     wave_atomic(a2, &sums[TS * b + 2]);
 }
 
@@ -1168,10 +1170,11 @@ int asd2_unet_backward(asd2_unet* N, int batch, const float* d_dy) {
     return ASD2_OK;
 }
 
-// One Adam step on the energy loss of `batch` systems, each in the solver's FP32 padded
-// layout ((nx + 6) (ny + 6) values per array, concatenated, nx = H c): r, dinv, ax, ay, diag.
+// One Adam step on the energy loss of `batch` coarse systems (residual g, the level's operator, and
+// optionally the energy g.A_c^-1 g of each sample).
 int asd2_unet_train_step_coarse(asd2_unet* N, int batch, const float* d_g, const float* d_acx, const float* d_acy,
-                                const float* d_dc, double h2x, double h2y, float lr, float* loss_out) {
+                                const float* d_dc, const float* d_energy, double h2x, double h2y, float lr,
+                                float* loss_out) {
     ASD2_TRY(check_net(N));
     if (batch <= 0 || batch > N->max_batch || !d_g || !d_acx || !d_acy || !d_dc || !(lr > 0.f) || !(h2x > 0.0) ||
         !(h2y > 0.0)) {
@@ -1185,7 +1188,7 @@ int asd2_unet_train_step_coarse(asd2_unet* N, int batch, const float* d_g, const
     ASD2_HIP(hipMemsetAsync(sums, 0, TS * batch * sizeof(float), s));
     ASD2_HIP(hipMemsetAsync(N->grad, 0, N->L.total * sizeof(float), s));
     const dim3 gc(blocks_for(Pc, 256, 1024), batch);
-    hipLaunchKernelGGL(k_tr_gnorm, gc, dim3(256), 0, s, d_g, d_dc, Hc, Wc, sums);
+    hipLaunchKernelGGL(k_tr_gnorm, gc, dim3(256), 0, s, d_g, d_dc, Hc, Wc, d_energy, sums);
     hipLaunchKernelGGL(k_tr_features, gc, dim3(256), 0, s, d_g, d_acx, d_acy, d_dc, Hc, Wc, static_cast<float>(h2x),
                        static_cast<float>(h2y), sums, N->feat, N->flin);
     ASD2_TRY(unet_forward(N, batch));
@@ -1203,9 +1206,10 @@ int asd2_unet_train_step_coarse(asd2_unet* N, int batch, const float* d_g, const
     ASD2_HIP(hipMemcpyAsync(h.data(), sums, h.size() * sizeof(float), hipMemcpyDeviceToHost, s));
     ASD2_HIP(hipStreamSynchronize(s));
     if (loss_out) {
-        // per sample -(y.g)^2 / (y.A_c y) / g.D_c^-1 g: the energy of the best step along the network's
-        // correction, relative to the Jacobi scale of the sample
-        double L = 0.0;
+        // per sample -(y.g)^2 / (y.A_c y) / nrm: the energy of the best step along the network's
+        // correction relative to the sample's energy (1 + that: the A_c-norm error left, squared and
+        // relative) or to its Jacobi scale
+        double L = d_energy ? static_cast<double>(batch) : 0.0;
         for (int b = 0; b < batch; b++) {
             const double nrm = h[TS * b], yg = h[TS * b + 3], yay = h[TS * b + 4];
             if (nrm > 0.0 && yay > 0.0) L -= yg * yg / yay / nrm;

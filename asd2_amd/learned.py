@@ -6,12 +6,22 @@ the circles of r_k - |x - c_k|), its curvature and Heaviside, the divergence of 
 velocity, assembled by asd2_gfm_assemble (the path of MACSolver2D::SolvePoisson, reference
 src/mac2d.cpp:1249) and exported in FP64.  The default distribution is the seeded-droplet one of
 the GFM benchmark case: 1-20 non-overlapping circles of radius uniform in [0.02, 0.1], density
-ratio log-uniform in [10, 1000], right-hand side minus the divergence of a smooth random velocity.
-The residual samples are the right-hand side and the residuals b - A x_k after k iterations of a
-solver: the mixed-precision Jacobi-PCG, or the flexible PCG with the multilevel coarse correction,
-whose residuals are the ones a good coarse correction leaves behind.  numpy only draws the random
-fields and stacks host copies into the FP32 padded layout of asd2_unet_train_step; the network's
-forward and backward passes and the Adam steps are the library's kernels.
+ratio log-uniform in [10, 1000], right-hand side minus the divergence of a smooth random velocity;
+FAMILIES adds larger droplets with surface tension and one to three large droplets at high
+density ratio.
+
+The network serves as the bottom of the multilevel hierarchy (the level whose coarsening equals
+the network's, 32 x 32 blocks: a 64^2 image at 2048^2).  Its training inputs are the coarse
+corrections g that the solver itself hands to that level (asd2_poisson_record_coarse, recorded
+during multilevel solves and then during learned solves with the network being trained), with
+that level's Galerkin operator P^T A P and the energy g.A^-1 g of each input (a sparse LU on the
+host).  The loss is the relative A-norm error left by the best step along the network's output
+(asd2_unet_train_step_coarse); the network's passes and the Adam steps are the library's kernels.
+train_hybrid is the production recipe, and WEIGHTS names the parameters it produced, which
+`pretrained` loads.
+
+The fine-level samples (make_samples / train: residuals after k iterations of a solver, with the
+network's own block restriction) serve networks used as the whole coarse correction.
 """
 from __future__ import annotations
 
@@ -217,15 +227,41 @@ def coarse_operator(ax, ay, dg, c, g=G):
     return acx, acy, dc
 
 
-class CoarseSamples:
-    """Coarse training systems on the device (FP32): g [S][H][W], acx / acy / dc [S][(H + 1)(W + 1)]"""
+def coarse_matrix(acx, acy, dc):
+    """the level operator (layout of coarse_operator) as a scipy CSR matrix over the H x W cells
+    (unknown I * W + J)"""
+    import scipy.sparse as sp
+    Hc, Wc = acx.shape[0] - 1, acx.shape[1] - 1
+    idx = np.arange(Hc * Wc).reshape(Hc, Wc)
+    rows, cols, vals = [idx.ravel()], [idx.ravel()], [dc[:Hc, :Wc].ravel()]
+    for w, a, b in ((acx[1:Hc, :Wc], idx[1:, :], idx[:-1, :]), (acy[:Hc, 1:Wc], idx[:, 1:], idx[:, :-1])):
+        rows += [a.ravel(), b.ravel()]
+        cols += [b.ravel(), a.ravel()]
+        vals += [w.ravel(), w.ravel()]
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(Hc * Wc, Hc * Wc))
 
-    def __init__(self, g, acx, acy, dc, h2):
+
+class CoarseSamples:
+    """Coarse training systems on the device (FP32): g [S][H][W], acx / acy / dc [S][(H + 1)(W + 1)],
+    energy [S] = g.A^-1 g"""
+
+    def __init__(self, g, acx, acy, dc, energy, h2):
         self.count = g.shape[0]
         self.H, self.W = g.shape[1], g.shape[2]
         self.h2 = h2
-        self.dev = [DeviceArray.from_host(np.ascontiguousarray(a, np.float32)) for a in (g, acx, acy, dc)]
-        self.strides = [self.H * self.W * 4] + [(self.H + 1) * (self.W + 1) * 4] * 3
+        self.host = [np.ascontiguousarray(a, np.float32) for a in (g, acx, acy, dc, energy)]
+        self.dev = [DeviceArray.from_host(a) for a in self.host]
+        self.strides = [self.H * self.W * 4] + [(self.H + 1) * (self.W + 1) * 4] * 3 + [4]
+
+    @classmethod
+    def union(cls, pools, seed=0):
+        """the samples of several pools (same grid and coarsening) in one shuffled pool"""
+        if any(p.h2 != pools[0].h2 or (p.H, p.W) != (pools[0].H, pools[0].W) for p in pools):
+            raise ValueError("pools of different levels")
+        order = np.random.default_rng(seed).permutation(sum(p.count for p in pools))
+        cols = [np.concatenate([p.host[k] for p in pools])[order] for k in range(5)]
+        return cls(*cols, pools[0].h2)
 
     def batch(self, start):
         return [_View(d.ptr + start * st) for d, st in zip(self.dev, self.strides)]
@@ -235,13 +271,14 @@ def make_coarse_samples(n, coarsen, systems=16, iters=(0, 2, 5, 10, 20, 40, 70, 
                         **kw):
     """Coarse-correction inputs g the solver itself produces on the level of coarsening `coarsen`
     (recorded by asd2_poisson_record_coarse at the given iterations of a multilevel solve, or of
-    the learned solve with `net` attached), with that level's Galerkin operator, for `systems`
-    random droplet systems on an n x n grid."""
+    the learned solve with `net` attached), with that level's Galerkin operator and the energy g.A^-1 g of each
+    input (a sparse LU of the level operator), for `systems` random droplet systems on an n x n grid."""
+    import scipy.sparse.linalg as spla
     rng = np.random.default_rng(seed)
     ctx = system_context(n)
     H = W = n // coarsen
     rec = DeviceArray((len(iters), H, W))
-    gs, ops = [], []
+    gs, ops, es = [], [], []
     t0 = time.perf_counter()
     for s in range(systems):
         build_system(ctx, rng, **kw)
@@ -255,28 +292,35 @@ def make_coarse_samples(n, coarsen, systems=16, iters=(0, 2, 5, 10, 20, 40, 70, 
         g = rec.download()
         ax, ay, dg, _ = ctx.gfm_export()
         op = coarse_operator(ax, ay, dg, coarsen)
+        lu = spla.splu(coarse_matrix(*op).tocsc())
         for k in range(len(iters)):
             if np.any(g[k] != 0.0):
+                gv = g[k].ravel()
                 gs.append(g[k])
                 ops.append(op)
+                es.append(float(gv @ lu.solve(gv)))
         if log:
             log(f"system {s + 1}/{systems}: {len(gs)} samples ({time.perf_counter() - t0:.1f} s)")
     order = rng.permutation(len(gs))
     g = np.stack([gs[i] for i in order])
     acx, acy, dc = (np.stack([ops[i][k] for i in order]) for k in range(3))
-    return CoarseSamples(g, acx, acy, dc, (1.0 / n) ** 2 / coarsen)
+    return CoarseSamples(g, acx, acy, dc, np.array(es)[order], (1.0 / n) ** 2 / coarsen)
 
 
-def train_coarse(net: UNet, samples: CoarseSamples, steps=2000, batch=8, lr=2e-3, lr_final=5e-4, seed=0, log=None):
-    """Adam on the coarse energy loss over random contiguous windows of the samples; returns the
-    per-step losses"""
+def train_coarse(net: UNet, samples: CoarseSamples, steps=2000, batch=8, lr=2e-3, lr_final=5e-4, seed=0, log=None,
+                 relative=True):
+    """Adam on the coarse energy loss over random contiguous windows of the samples (relative: the
+    A-norm error left relative to the sample's energy; else relative to its Jacobi scale); returns
+    the per-step losses"""
     rng = np.random.default_rng(seed)
     batch = min(batch, samples.count, net.max_batch)
     losses = []
     for step in range(steps):
         start = int(rng.integers(0, samples.count - batch + 1))
         rate = lr if step < 0.75 * steps else lr_final
-        losses.append(net.train_step_coarse(batch, *samples.batch(start), samples.h2, samples.h2, rate))
+        g, acx, acy, dc, e = samples.batch(start)
+        losses.append(net.train_step_coarse(batch, g, acx, acy, dc, samples.h2, samples.h2, rate,
+                                            e if relative else None))
         if log and (step % 200 == 0 or step == steps - 1):
             log(f"step {step}: loss {losses[-1]:.4f}")
     return losses
@@ -292,34 +336,90 @@ FAMILIES = {
 }
 
 
-def train_hybrid(n=2048, coarsen=32, systems=(32, 16, 16), steps=3000, rounds=2, seed=0, log=None):
-    """The network for the bottom of the n x n hierarchy (the level of coarsening `coarsen`): round 1
-    trains on coarse-correction inputs recorded from multilevel solves, every further round adds
-    inputs recorded from the learned solve with the network of the round before (its own input
-    distribution).  Returns the network and a record of the training."""
-    net = UNet(n, n, max_batch=8, seed=seed + 7, coarsen=coarsen)
-    record = {"grid": n, "coarsen": coarsen, "image": [n // coarsen, n // coarsen], "rounds": []}
-    pools = []
+def learned_iterations(net, contexts, tol=1e-8):
+    """iterations of the learned solve (mixed precision, `net` attached) on each context's system"""
+    its = []
+    for ctx in contexts:
+        ctx.set_preconditioner(net)
+        st = ctx.poisson_solve(ctx.array(), Context.opts(tol=tol, precision=PREC_MIXED, precond=PRECOND_LEARNED),
+                               raise_on_fail=False)
+        ctx.set_preconditioner(None)
+        its.append(int(st.iterations) if st.converged == 1 else 10 ** 6)
+    return its
+
+
+def train_hybrid(n=2048, coarsen=32, systems=(32, 16, 16), steps=6000, rounds=3, candidates=2, validation=3, seed=0,
+                 log=None, after_round=None):
+    """The network for the bottom of the n x n hierarchy (the level of coarsening `coarsen`).
+
+    Round 1 records coarse-correction inputs from multilevel solves of the training systems (the
+    FAMILIES, `systems` of each) and trains `candidates` networks from different initial weights on
+    them; the one whose learned solve takes the fewest iterations on `validation` systems of each
+    family (seeds disjoint from the training ones) goes on.  Every further round records inputs from
+    the learned solve with the current network (its own input distribution), adds them to the pool
+    and continues training on all of it.  Loss: the relative A-norm error of the best step along the
+    network's correction.  Returns the network and a record of the training."""
+    vals = []
+    for k, fam in enumerate(FAMILIES):
+        for v in range(validation):
+            ctx = system_context(n)
+            build_system(ctx, np.random.default_rng(seed + 700 + 10 * k + v), **FAMILIES[fam])
+            vals.append(ctx)
+    record = {"grid": n, "coarsen": coarsen, "image": [n // coarsen, n // coarsen], "steps": steps,
+              "families": {f: {kk: list(vv) for kk, vv in kw.items()} for f, kw in FAMILIES.items()},
+              "systems": list(systems), "rounds": []}
+    pools, net = [], None
     for rnd in range(rounds):
         t0 = time.perf_counter()
-        new = []
         for k, (fam, count) in enumerate(zip(FAMILIES, systems)):
-            new.append(make_coarse_samples(n, coarsen, systems=count, seed=seed + 1000 * rnd + 100 * k,
-                                           net=net if rnd > 0 else None, **FAMILIES[fam]))
-        pools.extend(new)
+            pools.append(make_coarse_samples(n, coarsen, systems=count, seed=seed + 1000 * rnd + 100 * k, net=net,
+                                             **FAMILIES[fam]))
+            if log:
+                log(f"round {rnd + 1} {fam}: {pools[-1].count} samples ({time.perf_counter() - t0:.1f} s)")
+        union = CoarseSamples.union(pools, seed=seed + rnd)
         t1 = time.perf_counter()
-        losses = []
-        schedule = [(p, steps if i == len(pools) - len(new) else steps // 2) for i, p in enumerate(pools)]
-        for i, (p, k) in enumerate(schedule):
-            lr = 2e-3 if rnd == 0 else 1e-3
-            losses += train_coarse(net, p, steps=k, batch=8, seed=seed + 17 * rnd + i, lr=lr, lr_final=lr / 4)
-        t2 = time.perf_counter()
-        rec = {"samples": [p.count for p in new], "data_s": t1 - t0, "train_s": t2 - t1, "steps": len(losses),
-               "loss_first": float(losses[0]), "loss_last100": float(np.mean(losses[-100:]))}
+        rec = {"samples": union.count, "data_s": t1 - t0}
+        if rnd == 0:
+            best = None
+            rec["candidates"] = []
+            for c in range(candidates):
+                cand = UNet(n, n, max_batch=8, seed=seed + 7 + 4 * c, coarsen=coarsen)
+                losses = train_coarse(cand, union, steps=steps, batch=8, seed=seed + 9 + 4 * c, lr=2e-3, lr_final=5e-4)
+                its = learned_iterations(cand, vals)
+                rec["candidates"].append({"loss_last200": float(np.mean(losses[-200:])), "validation_iters": its})
+                if log:
+                    log(f"round 1 candidate {c + 1}: loss {np.mean(losses[-200:]):.4f}, validation iterations {its}")
+                if best is None or sum(its) < best[0]:
+                    best = (sum(its), cand, c)
+            net = best[1]
+            rec["chosen"] = best[2]
+        else:
+            losses = train_coarse(net, union, steps=steps, batch=8, seed=seed + 31 * rnd, lr=1e-3, lr_final=2.5e-4)
+            rec["loss_last200"] = float(np.mean(losses[-200:]))
+            rec["validation_iters"] = learned_iterations(net, vals)
+        rec["train_s"] = time.perf_counter() - t1
         record["rounds"].append(rec)
         if log:
             log(f"round {rnd + 1}: {rec}")
+        if after_round:
+            after_round(rnd, net)
     return net, record
+
+
+WEIGHTS = {32: "droplet_2048_c32.f32"}  # coarsening -> parameter file
+
+
+def pretrained(n, coarsen=32, max_batch=1):
+    """the committed network for the bottom of the n x n hierarchy (trained by train_hybrid on the
+    droplet families at 2048^2: the weights are convolutional and serve any grid whose level of
+    coarsening `coarsen` is a whole image), or None when there is none for this coarsening"""
+    import os
+    name = WEIGHTS.get(coarsen)
+    if name is None or n % coarsen:
+        return None
+    net = UNet(n, n, max_batch=max_batch, coarsen=coarsen)
+    load_params(net, os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", name))
+    return net
 
 
 if __name__ == "__main__":
@@ -329,13 +429,13 @@ if __name__ == "__main__":
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--coarsen", type=int, default=32)
     ap.add_argument("--systems", default="32,16,16", help="training systems per family and round")
-    ap.add_argument("--steps", type=int, default=3000)
-    ap.add_argument("--rounds", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--rounds", type=int, default=3)
+    ap.add_argument("--candidates", type=int, default=2)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", required=True)
     a = ap.parse_args()
-    net, rec = train_hybrid(a.n, a.coarsen, tuple(int(s) for s in a.systems.split(",")), a.steps, a.rounds, a.seed,
-                            log=lambda m: print(m, flush=True))
-    rec["families"] = {k: {kk: list(vv) for kk, vv in v.items()} for k, v in FAMILIES.items()}
+    net, rec = train_hybrid(a.n, a.coarsen, tuple(int(s) for s in a.systems.split(",")), a.steps, a.rounds,
+                            a.candidates, seed=a.seed, log=lambda m: print(m, flush=True))
     save_params(net, a.out, rec)
     print(json.dumps(rec))

@@ -5,11 +5,13 @@ One step = the reference's stationary-droplet loop (tests/LargeTests/test_mac2d_
 Heaviside, RK3 momentum predictor, divergence, GFM assembly, pressure solve to a relative
 FP64 residual of 1e-8, projection, CFL time step.  The step is timed from the same state with
 four pressure paths: the plain FP64 Jacobi-PCG, the FP32 PCG with FP64 reliable updates, and
-that flexible FP32 PCG with a coarse correction from the learned network (trained first, on
-the GPU, on systems the library builds from random droplet fields) or from the multilevel
-V-cycle.  `value` is the ms/step of the fastest path (inputs resident in HBM).
+that flexible FP32 PCG with a coarse correction from the multilevel V-cycle, or from the V-cycle
+whose bottom (the 64^2 level of 32 x 32 blocks and below) is the learned network (the committed
+weights, trained on the GPU on droplet systems the library builds; `--train` retrains them first).
+The learned path is also timed with a zero network, the ablation that shows what the network
+gains.  `value` is the ms/step of the fastest path (inputs resident in HBM).
 
-Usage: python bench.py [--steps K] [--warmup W] [--n 2048] [--no-cpu] [--no-learned] [--pmc-workload]
+Usage: python bench.py [--steps K] [--warmup W] [--n 2048] [--no-cpu] [--no-learned] [--train] [--pmc-workload]
 """
 from __future__ import annotations
 
@@ -149,23 +151,25 @@ def run_path(sim, opts, steps, warmup, snap):
             "fallbacks": st.fallbacks}
 
 
-def train_network(A, n, coarsen, train_n=512, systems=8, steps=400, seed=1):
-    """The learned preconditioner for an n^2 grid: trained on GPU-built systems at train_n^2
-    (the network is convolutional; the coarse images of train_n / coarsen pixels share its
-    weights with the n / coarsen images), then copied into an n^2 network."""
+def load_network(A, n, train=False):
+    """The learned coarse solve for an n^2 grid: the network for the level of 32 x 32 blocks (64^2
+    at 2048^2) committed with the package (trained by asd2_amd.learned.train_hybrid on droplet
+    systems that the library builds on the GPU; the bench's own system is not among them), or
+    trained here first with `train`."""
     from asd2_amd import learned as LN
-    t0 = time.perf_counter()
-    samples = LN.make_samples(train_n, systems=systems, seed=seed)
-    t1 = time.perf_counter()
-    small = A.UNet(train_n, train_n, max_batch=4, seed=seed + 1, coarsen=coarsen)
-    losses = LN.train(small, samples, steps=steps, batch=4, seed=seed + 2)
-    t2 = time.perf_counter()
-    big = A.UNet(n, n, max_batch=1, coarsen=coarsen)
-    big.set_params(small.get_params())
-    info = {"train_grid": train_n, "coarsen": coarsen, "systems": systems, "samples": samples.count,
-            "steps": steps, "batch": 4, "loss_first": float(losses[0]), "loss_last20": float(np.mean(losses[-20:])),
-            "data_s": t1 - t0, "train_s": t2 - t1}
-    return big, info
+    if train:
+        t0 = time.perf_counter()
+        net, rec = LN.train_hybrid(n, 32, log=lambda m: print(f"# {m}", file=sys.stderr, flush=True))
+        big = A.UNet(n, n, max_batch=1, coarsen=32)
+        big.set_params(net.get_params())
+        return big, {"source": "trained in this run", "train_s": time.perf_counter() - t0, **rec}
+    net = LN.pretrained(n, 32)
+    if net is None:
+        raise SystemExit(f"no committed network for a {n}^2 grid at coarsening 32")
+    with open(os.path.join(ROOT, "asd2_amd", "weights", LN.WEIGHTS[32] + ".json")) as f:
+        rec = json.load(f)
+    return net, {"source": "asd2_amd/weights/" + LN.WEIGHTS[32], "coarsen": 32,
+                 "validation_iters_per_round": [r.get("validation_iters", r.get("candidates")) for r in rec["rounds"]]}
 
 
 # Krylov kernels keep being launched for the iterations already enqueued when the solver stops
@@ -274,7 +278,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
-    ap.add_argument("--no-learned", action="store_true", help="skip training and timing the learned path")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned path")
+    ap.add_argument("--train", action="store_true",
+                    help="train the learned path's network first (about 6 minutes) instead of loading the committed one")
     ap.add_argument("--paths", default="fp64,mixed,multilevel,learned",
                     help="pressure paths to time, a comma list of fp64, mixed, multilevel, learned")
     ap.add_argument("--pmc-workload", action="store_true",
@@ -298,7 +304,7 @@ def main():
         "learned": A.Context.opts(tol=args.tol, precision=A.PREC_MIXED, precond=A.PRECOND_LEARNED),
     }
     print(f"# device: {A.device_name()}  grid {args.n}^2", file=sys.stderr, flush=True)
-    paths, training = {}, None
+    paths, training, ablation = {}, None, None
     wanted = {p.strip() for p in args.paths.split(",") if p.strip()}
     for name in ("fp64", "mixed", "multilevel", "learned"):
         if name not in wanted:
@@ -306,9 +312,15 @@ def main():
         if name == "learned":
             if args.no_learned:
                 continue
-            net, training = train_network(A, args.n, coarsen=8)
-            print(f"# learned: trained on {training['samples']} samples in {training['train_s']:.1f} s, "
-                  f"loss {training['loss_first']:.2f} -> {training['loss_last20']:.2f}", file=sys.stderr, flush=True)
+            net, training = load_network(A, args.n, args.train)
+            # ablation: the same solve with a zero network (the hierarchy above the network's level
+            # without its bottom), what the trained network is measured against
+            zero = A.UNet(args.n, args.n, max_batch=1, coarsen=32)
+            zero.set_params(np.zeros(zero.nparams, np.float32))
+            sim.ctx.set_preconditioner(zero)
+            ablation = run_path(sim, opts[name], args.steps, args.warmup, snap)
+            print(f"# learned, zero network: {ablation['ms_per_step']:.1f} ms/step, "
+                  f"{ablation['pcg_iters_per_step']:.0f} iters", file=sys.stderr, flush=True)
             sim.ctx.set_preconditioner(net)
         paths[name] = run_path(sim, opts[name], args.steps, args.warmup, snap)
         print(f"# {name}: {paths[name]['ms_per_step']:.1f} ms/step, {paths[name]['pcg_iters_per_step']:.0f} iters",
@@ -355,7 +367,8 @@ def main():
         "paths": paths,
         "speedup_vs_fp64_pcg": {k: fp64["ms_per_step"] / v["ms_per_step"] for k, v in paths.items()} if fp64 else None,
         "pcg_iters": {k: v["pcg_iters_per_step"] for k, v in paths.items()},
-        "learned_training": training,
+        "learned_network": training,
+        "learned_zero_network": ablation,
         "roofline": roofline, "kernels": kernels, "cpu_baseline": cpu,
     }
     print(json.dumps(line))

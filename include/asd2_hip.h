@@ -306,10 +306,13 @@ int asd2_unet_train_step(asd2_unet* net, int batch, const float* d_r, const floa
  * residual g (FP32 [batch][H][W]) and the coarse operator in the layout of the level arrays (FP32
  * [batch][(H + 1) (W + 1)] each: W-face and S-face coefficients and diagonal, stride W + 1), H x W the
  * network's image size; h2x, h2y = dx^2 / c, dy^2 / c for the coefficient features (c the level's
- * coarsening of the fine grid).  Loss: the mean of -(y.g)^2 / (y.A_c y) / (g.D_c^-1 g), the energy
- * of the best step along the network's coarse correction; loss_out = loss before the step */
+ * coarsening of the fine grid).  d_energy (FP32 [batch], device, or NULL): g.A_c^-1 g of each sample.
+ * Loss with d_energy: the mean of 1 - (y.g)^2 / ((y.A_c y)(g.A_c^-1 g)), the squared A_c-norm error
+ * left by the best step along the network's correction y, relative to that of no correction; without
+ * it, the mean of -(y.g)^2 / (y.A_c y) / (g.D_c^-1 g).  loss_out = loss before the step */
 int asd2_unet_train_step_coarse(asd2_unet* net, int batch, const float* d_g, const float* d_acx, const float* d_acy,
-                                const float* d_dc, double h2x, double h2y, float lr, float* loss_out);
+                                const float* d_dc, const float* d_energy, double h2x, double h2y, float lr,
+                                float* loss_out);
 /* network used by asd2_poisson_solve with precond = ASD2_PRECOND_LEARNED (not owned; NULL detaches).
  * The network solves on the level of the multilevel hierarchy (first coarsening c0 = 8 x 8 blocks,
  * then 4 x 4) whose coarsening equals the network's, with the levels above it smoothed as in
