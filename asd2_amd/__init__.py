@@ -161,6 +161,7 @@ def load() -> C.CDLL:
         "asd2_unet_train_step_coarse": ([_VP, C.c_int, _VP, _VP, _VP, _VP, _VP, C.c_double, C.c_double,
                                          C.c_float, C.POINTER(C.c_float)], C.c_int),
         "asd2_poisson_record_coarse": ([_VP, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), _VP], C.c_int),
+        "asd2_fast_poisson": ([_VP, _VP, _VP, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double], C.c_int),
         "asd2_timer_start": ([_VP], C.c_int),
         "asd2_timer_stop": ([_VP, P(C.c_double)], C.c_int),
         "asd2_profile_kernel": ([_VP, C.c_int], C.c_int),
@@ -484,6 +485,12 @@ class Context:
 
     def gfm_apply(self, x, y):
         check(self.lib.asd2_gfm_apply(self.h, x.ptr, y.ptr))
+
+    def fast_poisson(self, rhs, ps, len_x, len_y, dx, dy, q=0.0):
+        """the constant-coefficient fast solve of PoissonSolver2D::MKL_2FUniform_2D(ps, rhs, lenX,
+        lenY, dx, dy, PBC) with the context's pressure sides: ps (device, padded) receives the
+        solution at every node of the mesh of cell centres, ghosts included"""
+        check(self.lib.asd2_fast_poisson(self.h, rhs.ptr, ps.ptr, len_x, len_y, dx, dy, q))
 
     @staticmethod
     def opts(tol=1e-8, max_iter=100000, mode=CG_STANDARD, precision=PREC_FP64,

@@ -223,6 +223,7 @@ int asd2_create(const asd2_grid* grid, asd2_ctx** out) {
 int asd2_destroy(asd2_ctx* ctx) {
     if (!ctx) return ASD2_OK;
     (void)hipStreamSynchronize(ctx->stream);
+    fast_poisson_release(ctx);
     for (auto& kv : ctx->bufs) (void)hipFree(kv.second.ptr);
     for (auto e : ctx->prof_events) (void)hipEventDestroy(e);
     if (ctx->kr_host) (void)hipHostFree(ctx->kr_host);

@@ -83,6 +83,10 @@ struct asd2_ctx {
     std::vector<int> rec_iters;
     double* rec_dst = nullptr;
 
+    // fast Poisson solver: rocBLAS handle (rocblas_handle) and the key of the cached eigenbasis
+    void* blas = nullptr;
+    std::string fp_key;
+
     // Krylov polling: pinned host copies of the device scalars and their events
     void* kr_host = nullptr;
     hipEvent_t kr_ev[2] = {nullptr, nullptr};
@@ -121,6 +125,8 @@ int check_ctx(const asd2_ctx* ctx);
 // internal launchers shared across translation units
 int bc_apply_p(asd2_ctx* ctx, double* d);
 int bc_apply_uv(asd2_ctx* ctx, int grid_kind, double* d);
+// frees the fast Poisson solver's library handle (fastpoisson.hip)
+void fast_poisson_release(asd2_ctx* ctx);
 
 inline unsigned cdiv(int64_t a, int64_t b) { return static_cast<unsigned>((a + b - 1) / b); }
 

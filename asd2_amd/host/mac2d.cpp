@@ -312,9 +312,8 @@ int MACSolver2D::SolvePoisson(std::vector<double>& ps, const std::vector<double>
                               const std::vector<double>& H, const int maxIter) {
     if (!m_Poisson)
         perror("Solver method for Poisson equations are not set. Please add SetPoissonSolver Method to running code");
-    if (m_PoissonSolverType == POISSONTYPE::MKL || m_PoissonSolverType == POISSONTYPE::GS ||
-        m_PoissonSolverType == POISSONTYPE::ICPCG) {
-        std::cerr << "MACSolver2D::SolvePoisson: only POISSONTYPE::CG and POISSONTYPE::BICGSTAB run on the GPU path"
+    if (m_PoissonSolverType == POISSONTYPE::GS || m_PoissonSolverType == POISSONTYPE::ICPCG) {
+        std::cerr << "MACSolver2D::SolvePoisson: POISSONTYPE::GS and POISSONTYPE::ICPCG have no GPU path"
                   << std::endl;
         std::exit(1);
     }
@@ -326,6 +325,29 @@ int MACSolver2D::SolvePoisson(std::vector<double>& ps, const std::vector<double>
         check(asd2_mac_kappa(D.ctx, d_ls, d_kappa), "UpdateKappa");
         check(asd2_bc_apply(D.ctx, ASD2_GRID_P, d_kappa), "ApplyBC_P_2D(kappa)");
         D.take("kappa", m_kappa);
+    }
+    if (m_PoissonSolverType == POISSONTYPE::MKL) {
+        // src/mac2d.cpp:1658-1689: the constant-coefficient solve with rhs = -div on the interior
+        // (ghosts 0) and ps zeroed; the assembled ghost-fluid system is not used on this path
+        std::vector<double> rhs(static_cast<size_t>(kArrSize), 0.0);
+        for (int j = kNumBCGrid; j < kNy + kNumBCGrid; j++)
+            for (int i = kNumBCGrid; i < kNx + kNumBCGrid; i++) rhs[idx(i, j)] = -div[idx(i, j)];
+        ps.assign(static_cast<size_t>(kArrSize), 0.0);
+        double* d_ps = D.put("ps", ps);
+        if (asd2_fast_poisson(D.ctx, D.put("rhs", rhs), d_ps, kLenX, kLenY, kDx, kDy, 0.0) != ASD2_OK) {
+            std::cerr << "MKL Error:d_Helmholtz_2D: " << asd2_last_error() << std::endl;
+            std::exit(1);
+        }
+        check(asd2_bc_apply(D.ctx, ASD2_GRID_P, d_ps), "ApplyBC_P_2D(ps)");
+        D.take("ps", ps);  // the ghost update leaves the interior the reference checks unchanged
+        for (int j = kNumBCGrid; j < kNy + kNumBCGrid; j++)
+            for (int i = kNumBCGrid; i < kNx + kNumBCGrid; i++)
+                if (std::isnan(ps[idx(i, j)]) || std::isinf(ps[idx(i, j)]))
+                    std::cout << "Pseudo-p nan/inf error : " << i << " " << j << " " << ps[idx(i, j)] << std::endl;
+        if (m_p.size() != static_cast<size_t>(kArrSize)) m_p.assign(static_cast<size_t>(kArrSize), 0.0);
+        for (int i = kNumBCGrid; i < kNx + kNumBCGrid; i++)
+            for (int j = kNumBCGrid; j < kNy + kNumBCGrid; j++) m_p[idx(i, j)] = ps[idx(i, j)] / m_dt;
+        return 0;
     }
     asd2_fluid f;
     FluidParams(&f);

@@ -247,13 +247,24 @@ int PoissonSolver2D::GS_2FUniform_2D(std::vector<double>& ps, const std::vector<
     return 0;
 }
 
-// The MKL FFT Helmholtz solver is not part of the GPU path.
-
-int PoissonSolver2D::MKL_2FUniform_2D(std::vector<double>&, const std::vector<double>&, double, double, double,
-                                      double, std::shared_ptr<BoundaryCondition2D>) {
-    std::cerr << "PoissonSolver2D::MKL_2FUniform_2D (MKL FFT Helmholtz) is not available on the GPU path"
-              << std::endl;
-    return -1;
+// src/poisson2d.cpp:90-228: the constant-coefficient Poisson solve the reference hands to the MKL
+// Poisson Library, here asd2_fast_poisson (an eigenbasis transform along y and tridiagonal
+// solves along x on the GPU).  The reference maps only Dirichlet, Neumann and periodic pressure
+// sides to MKL's side types; periodic and the other kinds stop with the reference's "MKL Error"
+// convention (perror + exit(1)).
+int PoissonSolver2D::MKL_2FUniform_2D(std::vector<double>& ps, const std::vector<double>& rhs, double lenX,
+                                      double lenY, double dx, double dy, std::shared_ptr<BoundaryCondition2D> PBC) {
+    const size_t n = static_cast<size_t>(kNx + 2 * kNumBCGrid) * static_cast<size_t>(kNy + 2 * kNumBCGrid);
+    if (ps.size() != n || rhs.size() != n) fatal("PoissonSolver2D::MKL_2FUniform_2D: array size mismatch");
+    asd2_ctx* ctx = Context();
+    PBC->Configure(ctx);
+    DevVec d_ps(ps), d_rhs(rhs);
+    if (asd2_fast_poisson(ctx, d_rhs.p(), d_ps.p(), lenX, lenY, dx, dy, 0.0) != ASD2_OK) {
+        std::cerr << "MKL Error:d_Helmholtz_2D: " << asd2_last_error() << std::endl;
+        std::exit(1);
+    }
+    d_ps.get(ps);
+    return 0;
 }
 
 // Declared by the reference header without a definition: true when A equals its transpose.

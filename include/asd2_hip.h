@@ -327,6 +327,20 @@ int asd2_set_preconditioner(asd2_ctx* ctx, asd2_unet* net);
  * their contents.  The request is cleared by that solve; it fails (ASD2_ERR_ARG) when the solve
  * computes no input of that size. */
 int asd2_poisson_record_coarse(asd2_ctx* ctx, int level_h, int level_w, int count, const int* iters, double* d_out);
+/* Constant-coefficient fast Poisson / Helmholtz solve; replaces the MKL Poisson Library call of
+ * PoissonSolver2D::MKL_2FUniform_2D (reference src/poisson2d.cpp:90-228).  Solves
+ * -u_xx - u_yy + q u = rhs with the five-point difference on the node mesh of the padded cell
+ * centres from the first ghost to the last (node (i, j) = padded cell (i + g - 1, j + g - 1),
+ * i = 0..nx+1, spacing (len_x + dx) / (nx + 1); len, d as the reference's lenX, lenY, dx, dy
+ * arguments), with the context's pressure sides: Neumann
+ * (the side's nodes are unknowns, zero normal derivative by the central difference) or Dirichlet
+ * (node values 2c - ps read from d_ps as the reference reads them: ps(g, j+g), ps(nx+g, j+g),
+ * ps(i+g, g), ps(i+g, ny+g)).  rhs is read at every node, ghost cells included; the solution is
+ * written to d_ps at every node.  q = 0 with four Neumann sides: the compatible projection,
+ * solution with zero weighted mean (weights 1/2 on boundary nodes).  Other side kinds:
+ * ASD2_ERR_UNSUPPORTED.  Needs g >= 2; q >= 0. */
+int asd2_fast_poisson(asd2_ctx* ctx, const double* d_rhs, double* d_ps, double len_x, double len_y, double dx,
+                      double dy, double q);
 
 /* ---------------------------------------------------------------- timing */
 /* records the start/stop of a region on the context's stream (hipEvents) */
@@ -345,6 +359,9 @@ int asd2_timer_stop(asd2_ctx* ctx, double* ms);
 #define ASD2_KID_FCG_BEGIN 9      /* flexible PCG: fine pass for the coarse residual and dots */
 #define ASD2_KID_FCG_SPMV 10      /* flexible PCG: z formed on the fly, direction, SpMV, p.q */
 #define ASD2_KID_FCG_UPDATE 11    /* flexible PCG: x/r update + dots */
+#define ASD2_KID_FAST_PASS 12     /* fast Poisson: boundary values + gather, scatter */
+#define ASD2_KID_FAST_GEMM 13     /* fast Poisson: the two eigenbasis transforms (rocBLAS DGEMM) */
+#define ASD2_KID_FAST_TRIDIAG 14  /* fast Poisson: tridiagonal solves along x, one per y mode */
 int asd2_profile_kernel(asd2_ctx* ctx, int kernel_id);
 int asd2_profile_read(asd2_ctx* ctx, int* launches, double* total_ms);
 /* the same records one by one: the first min(capacity, *launches) durations in launch order */

@@ -12,6 +12,12 @@
 //   dropin_driver ls <dir> <nx> <ny>        LevelSetSolver2D methods on <dir>/phi.bin, u.bin, v.bin
 //   dropin_driver csr <dir> <nx> <ny> <all_neumann 0|1>
 //                                           PoissonSolver2D CG + BiCGStab on a CSR system
+//   dropin_driver fast <dir> <nx> <ny> <sides>
+//                                           PoissonSolver2D::MKL_2FUniform_2D on <dir>/rhs.bin,
+//                                           ps.bin, consts.bin (W E S N) with pressure sides
+//                                           <sides> (4 letters N/D), lenX = 1, lenY = ny / nx;
+//                                           then MACSolver2D::SolvePoisson with POISSONTYPE::MKL
+//                                           on <dir>/div.bin (all sides Neumann)
 //
 // Arrays are raw little-endian float64 (int32 for CSR indices) in the padded layout.
 #include <cstdint>
@@ -216,6 +222,40 @@ int run_csr(const std::string& dir, int nx, int ny, bool all_neumann) {
     return 0;
 }
 
+int run_fast(const std::string& dir, int nx, int ny, const std::string& sides) {
+    const double lenX = 1.0, lenY = static_cast<double>(ny) / nx, dx = lenX / nx, dy = lenY / ny;
+    const auto rhs = load<double>(dir + "/rhs.bin");
+    auto ps = load<double>(dir + "/ps.bin");
+    const auto c = load<double>(dir + "/consts.bin");
+    auto word = [](char k) { return std::string(k == 'D' ? "dirichlet" : "neumann"); };
+    auto bc = std::make_shared<BoundaryCondition2D>(nx, ny, 3);
+    bc->SetBC_P_2D(word(sides[0]), word(sides[1]), word(sides[2]), word(sides[3]));
+    bc->SetBCConstantPW(c[0]);
+    bc->SetBCConstantPE(c[1]);
+    bc->SetBCConstantPS(c[2]);
+    bc->SetBCConstantPN(c[3]);
+    PoissonSolver2D P(nx, ny, 3);
+    P.MKL_2FUniform_2D(ps, rhs, lenX, lenY, dx, dy, bc);
+    save(dir + "/ps_fast.bin", ps);
+
+    // the MAC path: rhs = -div, ps zeroed, P BCs, m_p = ps / dt
+    auto M = std::make_unique<MACSolver2D>(1000.0, 1.0, 1.0, 0.01, 0.0, GAXISENUM2D::Y, 1.0, 1.0, 0.0, nx, ny, 0.0,
+                                           0.0, lenX, lenY, TIMEORDERENUM::RK3, 0.3, 5.0, 20, 1, 3, false);
+    M->SetBC_U_2D("dirichlet", "dirichlet", "dirichlet", "dirichlet");
+    M->SetBC_V_2D("dirichlet", "dirichlet", "dirichlet", "dirichlet");
+    M->SetBC_P_2D("neumann", "neumann", "neumann", "neumann");
+    M->SetPLTType(PLTTYPE::NONE);
+    M->SetPoissonSolver(POISSONTYPE::MKL);
+    M->AllocateVariables();
+    M->m_dt = 0.01;
+    const auto div = load<double>(dir + "/div.bin");
+    const std::vector<double> zero(div.size(), 0.0);
+    M->SolvePoisson(M->m_ps, div, zero, zero, zero, zero, zero, 100);
+    save(dir + "/mac_ps.bin", M->m_ps);
+    save(dir + "/mac_p.bin", M->m_p);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -227,6 +267,8 @@ int main(int argc, char** argv) {
     if (cmd == "step" && argc == 6) return run_step(dir, std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]));
     if (cmd == "ls" && argc == 5) return run_ls(dir, std::atoi(argv[3]), std::atoi(argv[4]));
     if (cmd == "csr" && argc == 6) return run_csr(dir, std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]) != 0);
+    if (cmd == "fast" && argc == 6 && std::string(argv[5]).size() == 4)
+        return run_fast(dir, std::atoi(argv[3]), std::atoi(argv[4]), argv[5]);
     std::cerr << "bad arguments" << std::endl;
     return 2;
 }

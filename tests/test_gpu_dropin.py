@@ -181,3 +181,36 @@ def test_poisson_class_csr(oracle, tmp_path, all_neumann):
         assert err <= 1e-7, name
     np.testing.assert_array_equal(_load(tmp_path, "minv.bin"), 1.0 / A.diagonal())
     assert _load(tmp_path, "sym.bin", dtype=np.int32)[0] == 1
+
+
+@pytest.mark.parametrize("sides", ["NNNN", "DDNN", "NDDN"])
+def test_poisson_class_fast(tmp_path, sides):
+    """PoissonSolver2D::MKL_2FUniform_2D and MACSolver2D::SolvePoisson with POISSONTYPE::MKL
+    (reference src/poisson2d.cpp:90-228, src/mac2d.cpp:1658-1689) through the drop-in classes,
+    against the oracle's restatement of the same discrete problem (oracle/fastpoisson.py)"""
+    from oracle import fastpoisson as FP
+    nx, ny = 40, 28
+    rng = np.random.default_rng(17)
+    shp = (nx + 6, ny + 6)
+    rhs = rng.standard_normal(shp)
+    ps = rng.standard_normal(shp)
+    c = rng.uniform(-1, 1, 4)
+    div = np.zeros(shp)
+    div[3:nx + 3, 3:ny + 3] = rng.standard_normal((nx, ny))
+    for name, a in (("rhs", rhs), ("ps", ps), ("consts", c), ("div", div)):
+        a.astype(np.float64).tofile(tmp_path / f"{name}.bin")
+    _run("fast", tmp_path, nx, ny, sides)
+    lx, ly = 1.0, ny / nx
+    want = FP.mkl_2funiform(ps, rhs, lx, ly, lx / nx, ly / ny, sides, c)
+    got = _load(tmp_path, "ps_fast.bin", shp)
+    assert _rel(got, want) <= 1e-11
+    # MAC path: rhs = -div on the interior (ghosts 0), ps zeroed, then Neumann ghosts, p = ps / dt
+    r = np.zeros(shp)
+    r[3:nx + 3, 3:ny + 3] = -div[3:nx + 3, 3:ny + 3]
+    want = FP.mkl_2funiform(np.zeros(shp), r, lx, ly, lx / nx, ly / ny, "NNNN", [0.0] * 4)
+    got = _load(tmp_path, "mac_ps.bin", shp)
+    inner = (slice(3, nx + 3), slice(3, ny + 3))
+    assert _rel(got[inner], want[inner]) <= 1e-11
+    np.testing.assert_array_equal(got[2, 3:ny + 3], got[3, 3:ny + 3])  # Neumann ghost of the W side
+    p = _load(tmp_path, "mac_p.bin", shp)
+    np.testing.assert_allclose(p[inner], got[inner] / 0.01, rtol=1e-15, atol=0)
