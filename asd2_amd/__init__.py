@@ -127,6 +127,11 @@ def load() -> C.CDLL:
         "asd2_mac_update_dt": ([_VP, P(Fluid), _VP, _VP, P(C.c_double)], C.c_int),
         "asd2_mac_intermediate_vel": ([_VP, P(Fluid), C.c_int, _VP, _VP, _VP, _VP, _VP, _VP], C.c_int),
         "asd2_gfm_assemble": ([_VP, P(Fluid), _VP, _VP, _VP, _VP], C.c_int),
+        "asd2_gfm_assemble_axisym": ([_VP, P(Fluid), _VP, _VP, _VP, _VP], C.c_int),
+        "asd2_mac_divergence_axisym": ([_VP, _VP, _VP, _VP, C.c_int], C.c_int),
+        "asd2_mac_kappa_axisym": ([_VP, _VP, _VP], C.c_int),
+        "asd2_mac_update_dt_axisym": ([_VP, P(Fluid), _VP, _VP, _VP, P(C.c_double)], C.c_int),
+        "asd2_tdma": ([_VP, C.c_int, C.c_int, _VP, _VP, _VP, _VP], C.c_int),
         "asd2_gfm_export": ([_VP, _VP, _VP, _VP, _VP], C.c_int),
         "asd2_gfm_import": ([_VP, _VP, _VP, _VP, _VP], C.c_int),
         "asd2_gfm_apply": ([_VP, _VP, _VP], C.c_int),
@@ -459,6 +464,27 @@ class Context:
     def momentum_term(self, fl, grid_kind, term, ls, u, v, Hs, out):
         check(self.lib.asd2_mac_momentum_term(self.h, C.byref(fl), grid_kind, term, ls.ptr, u.ptr, v.ptr,
                                               Hs.ptr, out.ptr))
+
+    # axisymmetric (x = r from the grid's base x, y = z) ----------------------------
+    def kappa_axisym(self, ls, kap):
+        check(self.lib.asd2_mac_kappa_axisym(self.h, ls.ptr, kap.ptr))
+
+    def divergence_axisym(self, u, v, div, for_poisson):
+        check(self.lib.asd2_mac_divergence_axisym(self.h, u.ptr, v.ptr, div.ptr, 1 if for_poisson else 0))
+
+    def update_dt_axisym(self, fl, u, v, kap) -> float:
+        out = C.c_double(0.0)
+        check(self.lib.asd2_mac_update_dt_axisym(self.h, C.byref(fl), u.ptr, v.ptr, kap.ptr, C.byref(out)))
+        return out.value
+
+    def gfm_assemble_axisym(self, fl, ls, kap, H, div4p):
+        check(self.lib.asd2_gfm_assemble_axisym(self.h, C.byref(fl), ls.ptr, kap.ptr,
+                                                H.ptr if H is not None else None, div4p.ptr))
+
+    def tdma(self, a, b, c, d):
+        """count tridiagonal systems of n unknowns ([n][count] device arrays); the solution into d"""
+        n, count = d.shape
+        check(self.lib.asd2_tdma(self.h, n, count, a.ptr, b.ptr, c.ptr, d.ptr))
 
     # GFM / Poisson -----------------------------------------------------------------
     def gfm_assemble(self, fl, ls, kap, H, div):

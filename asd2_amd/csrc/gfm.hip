@@ -33,13 +33,29 @@ struct GfmArgs {
     int bc[4];
     double c[4];
     int pin;
+    double base_r;  // axisymmetric: radius of the west wall
 };
 
+// the side kinds the assembly treats as Dirichlet (drop the neighbour, move the wall value to the
+// right-hand side): WALL is Dirichlet-like in the planar assembly (src/mac2d.cpp:1521-1528) and
+// Neumann-like in the axisymmetric one (src/mac2dAxisym.cpp:1790-1862)
+template <bool AXI>
 __device__ __forceinline__ bool is_dir(int k) {
     return k == ASD2_BC_DIRICHLET || k == ASD2_BC_INLET || k == ASD2_BC_OUTLET || k == ASD2_BC_PRESSURE ||
-           k == ASD2_BC_WALL;
+           (!AXI && k == ASD2_BC_WALL);
+}
+template <bool AXI>
+__device__ __forceinline__ bool is_neu(int k) {
+    return k == ASD2_BC_NEUMANN || (AXI && (k == ASD2_BC_AXISYM || k == ASD2_BC_WALL));
 }
 
+// Planar (AXI = false): face coefficients beta = 1 / rho_eff.  Axisymmetric (AXI = true,
+// MACSolver2DAxisym::SolvePoisson, src/mac2dAxisym.cpp:1488-1950, x = r, y = z): r_eff / rho_eff,
+// r_eff the W / E face radius where the face lies in one phase, else the radius of the interface
+// point r_M theta + r_nb (1 - theta) (clamped at 0 on the W face); r of the cell centre on the S / N
+// faces.  Dirichlet walls contribute twice the wall constant there (the reference's
+// -ps + 2c with ps zeroed).
+template <bool AXI>
 __global__ __launch_bounds__(256) void k_gfm_assemble(Geom G, GfmArgs A, const double* __restrict__ ls,
                                                       const double* __restrict__ kap, const double* __restrict__ Hs,
                                                       const double* __restrict__ div, double* __restrict__ ax,
@@ -62,24 +78,42 @@ __global__ __launch_bounds__(256) void k_gfm_assemble(Geom G, GfmArgs A, const d
     }
     const double kM = kap[c];
     const double dx = G.dx, dy = G.dy, dt = A.dt, sigma = A.sigma;
+    // radii of the axisymmetric faces (src/mac2dAxisym.cpp:1577-1581)
+    const double rPW = A.base_r + (i - 0.5 - G.g) * dx, rPM = A.base_r + (i + 0.5 - G.g) * dx;
+    const double rPE = A.base_r + (i + 1.5 - G.g) * dx;
+    const double rPWHalf = A.base_r + (i - G.g) * dx, rPEHalf = A.base_r + (i + 1.0 - G.g) * dx;
     double thH, th;
     face_theta(lsW, lsM, true, thH, th);
-    const double iW = 1.0 / (A.rho_h * thH + A.rho_l * (1.0 - thH));
+    double iW;
+    if (AXI) {
+        double rEff = rPWHalf;
+        if ((lsW >= 0.0) != (lsM >= 0.0)) rEff = smax(rPM * th + rPW * (1.0 - th), 0.0);
+        iW = rEff / (A.rho_h * thH + A.rho_l * (1.0 - thH));
+    } else {
+        iW = 1.0 / (A.rho_h * thH + A.rho_l * (1.0 - thH));
+    }
     double kE = kM * th + kap[cw] * (1.0 - th);
     double FW = -dt * (-sigma * kE) * (HM - HW);
     FW *= iW / (dx * dx);
     face_theta(lsM, lsE, false, thH, th);
-    const double iE = 1.0 / (A.rho_h * thH + A.rho_l * (1.0 - thH));
+    double iE;
+    if (AXI) {
+        double rEff = rPEHalf;
+        if ((lsM >= 0.0) != (lsE >= 0.0)) rEff = rPM * th + rPE * (1.0 - th);
+        iE = rEff / (A.rho_h * thH + A.rho_l * (1.0 - thH));
+    } else {
+        iE = 1.0 / (A.rho_h * thH + A.rho_l * (1.0 - thH));
+    }
     kE = kM * th + kap[ce] * (1.0 - th);
     double FE = dt * (-sigma * kE) * (HE - HM);
     FE *= iE / (dx * dx);
     face_theta(lsS, lsM, true, thH, th);
-    const double iS = 1.0 / (A.rho_h * thH + A.rho_l * (1.0 - thH));
+    const double iS = (AXI ? rPM : 1.0) / (A.rho_h * thH + A.rho_l * (1.0 - thH));
     kE = kM * th + kap[cs] * (1.0 - th);
     double FS = -dt * (-sigma * kE) * (HM - HS);
     FS *= iS / (dy * dy);
     face_theta(lsM, lsN, false, thH, th);
-    const double iN = 1.0 / (A.rho_h * thH + A.rho_l * (1.0 - thH));
+    const double iN = (AXI ? rPM : 1.0) / (A.rho_h * thH + A.rho_l * (1.0 - thH));
     kE = kM * th + kap[cn] * (1.0 - th);
     double FN = dt * (-sigma * kE) * (HN - HM);
     FN *= iN / (dy * dy);
@@ -89,21 +123,23 @@ __global__ __launch_bounds__(256) void k_gfm_assemble(Geom G, GfmArgs A, const d
     const double nS = -iS, nW = -iW, nE = -iE, nN = -iN;
     double C = -(nW + nE) / (dx * dx) - (nS + nN) / (dy * dy);
     double aW = nW / (dx * dx), aS = nS / (dy * dy);
+    // wall value: the constant (planar), -ps + 2c with ps = 0 (axisymmetric)
+    auto wall = [&](int s) { return AXI ? 2.0 * A.c[s] : A.c[s]; };
     if (i == G.g) {
-        if (A.bc[0] == ASD2_BC_NEUMANN) { aW = 0.0; C += nW / (dx * dx); }
-        else if (is_dir(A.bc[0])) { aW = 0.0; r -= nW / (dx * dx) * A.c[0]; }
+        if (is_neu<AXI>(A.bc[0])) { aW = 0.0; C += nW / (dx * dx); }
+        else if (is_dir<AXI>(A.bc[0])) { aW = 0.0; r -= nW / (dx * dx) * wall(0); }
     }
     if (i == G.g + G.nx - 1) {
-        if (A.bc[1] == ASD2_BC_NEUMANN) C += nE / (dx * dx);
-        else if (is_dir(A.bc[1])) r -= nE / (dx * dx) * A.c[1];
+        if (is_neu<AXI>(A.bc[1])) C += nE / (dx * dx);
+        else if (is_dir<AXI>(A.bc[1])) r -= nE / (dx * dx) * wall(1);
     }
     if (j == G.g) {
-        if (A.bc[2] == ASD2_BC_NEUMANN) { aS = 0.0; C += nS / (dy * dy); }
-        else if (is_dir(A.bc[2])) { aS = 0.0; r -= nS / (dy * dy) * A.c[2]; }
+        if (is_neu<AXI>(A.bc[2])) { aS = 0.0; C += nS / (dy * dy); }
+        else if (is_dir<AXI>(A.bc[2])) { aS = 0.0; r -= nS / (dy * dy) * wall(2); }
     }
     if (j == G.g + G.ny - 1) {
-        if (A.bc[3] == ASD2_BC_NEUMANN) C += nN / (dy * dy);
-        else if (is_dir(A.bc[3])) r -= nN / (dy * dy) * A.c[3];
+        if (is_neu<AXI>(A.bc[3])) C += nN / (dy * dy);
+        else if (is_dir<AXI>(A.bc[3])) r -= nN / (dy * dy) * wall(3);
     }
     if (A.pin) {
         if (i == G.g + 1 && j == G.g) aW = 0.0;
@@ -121,10 +157,10 @@ __global__ __launch_bounds__(256) void k_gfm_assemble(Geom G, GfmArgs A, const d
         // the diagonal no stored coupling accounts for: faces to Dirichlet-family walls, the
         // couplings to the pinned cell, and the pinned cell's whole row (exact here, no cancellation)
         double d = 0.0;
-        if (i == G.g && is_dir(A.bc[0])) d += -nW / (dx * dx);
-        if (i == G.g + G.nx - 1 && is_dir(A.bc[1])) d += -nE / (dx * dx);
-        if (j == G.g && is_dir(A.bc[2])) d += -nS / (dy * dy);
-        if (j == G.g + G.ny - 1 && is_dir(A.bc[3])) d += -nN / (dy * dy);
+        if (i == G.g && is_dir<AXI>(A.bc[0])) d += -nW / (dx * dx);
+        if (i == G.g + G.nx - 1 && is_dir<AXI>(A.bc[1])) d += -nE / (dx * dx);
+        if (j == G.g && is_dir<AXI>(A.bc[2])) d += -nS / (dy * dy);
+        if (j == G.g + G.ny - 1 && is_dir<AXI>(A.bc[3])) d += -nN / (dy * dy);
         if (A.pin) {
             if (i == G.g + 1 && j == G.g) d += -nW / (dx * dx);
             else if (i == G.g && j == G.g + 1) d += -nS / (dy * dy);
@@ -204,16 +240,17 @@ using namespace asd2;
 
 extern "C" {
 
-int asd2_gfm_assemble(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_ls, const double* d_kappa,
-                      const double* d_H, const double* d_div) {
+namespace {
+int gfm_assemble(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_ls, const double* d_kappa, const double* d_H,
+                 const double* d_div, bool axi) {
     ASD2_TRY(check_ctx(ctx));
     if (!fl || !d_ls || !d_kappa || !d_div) { set_error("null argument"); return ASD2_ERR_ARG; }
     for (int s = 0; s < 4; s++) {
         const int k = ctx->bc[ASD2_GRID_P][s];
-        if (k == ASD2_BC_PERIODIC || k == ASD2_BC_AXISYM) {
+        if (k == ASD2_BC_PERIODIC || (!axi && k == ASD2_BC_AXISYM)) {
             // mac2d.cpp:1529-1531 (periodic) and the AXISYM rows of :1516-1583 couple the
             // wrong unknowns; the GPU path refuses them.
-            set_error("periodic/axisym pressure boundaries are not supported by the GFM assembly");
+            set_error("periodic%s pressure boundaries are not supported by the GFM assembly", axi ? "" : "/axisym");
             return ASD2_ERR_UNSUPPORTED;
         }
     }
@@ -223,16 +260,32 @@ int asd2_gfm_assemble(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_ls, c
     A.rho_h = fl->rho_h; A.rho_l = fl->rho_l; A.sigma = fl->sigma; A.dt = fl->dt;
     for (int s = 0; s < 4; s++) { A.bc[s] = ctx->bc[ASD2_GRID_P][s]; A.c[s] = ctx->bcc[ASD2_GRID_P][s]; }
     A.pin = all_neumann_family(ctx) ? 1 : 0;
+    A.base_r = ctx->base_x;
     const Geom& G = ctx->geo;
+    const dim3 grid(cdiv(G.ny, 64), cdiv(G.nx, 4));
     prof_begin(ctx, ASD2_KID_GFM);
-    hipLaunchKernelGGL(k_gfm_assemble, dim3(cdiv(G.ny, 64), cdiv(G.nx, 4)), dim3(256), 0, ctx->stream, G, A,
-                       d_ls, d_kappa, d_H, d_div, S.ax, S.ay, S.diag, S.rhs, S.dinv, S.ax32, S.ay32, S.diag32,
-                       S.dinv32);
+    if (axi)
+        hipLaunchKernelGGL(k_gfm_assemble<true>, grid, dim3(256), 0, ctx->stream, G, A, d_ls, d_kappa, d_H, d_div,
+                           S.ax, S.ay, S.diag, S.rhs, S.dinv, S.ax32, S.ay32, S.diag32, S.dinv32);
+    else
+        hipLaunchKernelGGL(k_gfm_assemble<false>, grid, dim3(256), 0, ctx->stream, G, A, d_ls, d_kappa, d_H, d_div,
+                           S.ax, S.ay, S.diag, S.rhs, S.dinv, S.ax32, S.ay32, S.diag32, S.dinv32);
     prof_end(ctx, ASD2_KID_GFM);
     ASD2_LAUNCH_CHECK();
     ctx->system_ready = true;
     ctx->dres_ring_only = true;
     return ASD2_OK;
+}
+}  // namespace
+
+int asd2_gfm_assemble(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_ls, const double* d_kappa,
+                      const double* d_H, const double* d_div) {
+    return gfm_assemble(ctx, fl, d_ls, d_kappa, d_H, d_div, false);
+}
+
+int asd2_gfm_assemble_axisym(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_ls, const double* d_kappa,
+                             const double* d_H, const double* d_div4p) {
+    return gfm_assemble(ctx, fl, d_ls, d_kappa, d_H, d_div4p, true);
 }
 
 int asd2_gfm_export(asd2_ctx* ctx, double* d_ax, double* d_ay, double* d_diag, double* d_rhs) {

@@ -118,22 +118,26 @@ __global__ __launch_bounds__(256) void k_update_vel(Geom G, ProjArgs A, double* 
 // max over the interior of |u_{i+1} u_i / 2| and |v_{j+1} v_j / 2| (src/mac2d.cpp:1858-1862):
 // block maxima into partials, then one block reduces them (max is order-free, so the result
 // does not depend on the split)
+// (kap given: also the max of |kappa|, for the axisymmetric time step, src/mac2dAxisym.cpp:2103)
 __global__ __launch_bounds__(256) void k_velmax(Geom G, const double* __restrict__ u, const double* __restrict__ v,
-                                                double* __restrict__ partials) {
-    double um = 0.0, vm = 0.0;
+                                                const double* __restrict__ kap, double* __restrict__ partials) {
+    double um = 0.0, vm = 0.0, km = 0.0;
     for (int i = G.g + blockIdx.y; i < G.nx + G.g; i += gridDim.y)
         for (int j = G.g + blockIdx.x * 256 + threadIdx.x; j < G.ny + G.g; j += gridDim.x * 256) {
             um = smax(um, fabs((u[G.idx(i + 1, j)] * u[G.idx(i, j)]) * 0.5));
             vm = smax(vm, fabs((v[G.idx(i, j + 1)] * v[G.idx(i, j)]) * 0.5));
+            if (kap) km = smax(km, fabs(kap[G.idx(i, j)]));
         }
-    __shared__ double su[256], sv[256];
+    __shared__ double su[256], sv[256], sk[256];
     su[threadIdx.x] = um;
     sv[threadIdx.x] = vm;
+    sk[threadIdx.x] = km;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
             su[threadIdx.x] = smax(su[threadIdx.x], su[threadIdx.x + s]);
             sv[threadIdx.x] = smax(sv[threadIdx.x], sv[threadIdx.x + s]);
+            sk[threadIdx.x] = smax(sk[threadIdx.x], sk[threadIdx.x + s]);
         }
         __syncthreads();
     }
@@ -141,30 +145,35 @@ __global__ __launch_bounds__(256) void k_velmax(Geom G, const double* __restrict
         const int b = blockIdx.y * gridDim.x + blockIdx.x, nb = gridDim.x * gridDim.y;
         partials[b] = su[0];
         partials[nb + b] = sv[0];
+        partials[2 * nb + b] = sk[0];
     }
 }
 
 __global__ __launch_bounds__(256) void k_velmax_final(const double* __restrict__ partials, int nb,
                                                       double* __restrict__ out) {
-    double um = 0.0, vm = 0.0;
+    double um = 0.0, vm = 0.0, km = 0.0;
     for (int b = threadIdx.x; b < nb; b += 256) {
         um = smax(um, partials[b]);
         vm = smax(vm, partials[nb + b]);
+        km = smax(km, partials[2 * nb + b]);
     }
-    __shared__ double su[256], sv[256];
+    __shared__ double su[256], sv[256], sk[256];
     su[threadIdx.x] = um;
     sv[threadIdx.x] = vm;
+    sk[threadIdx.x] = km;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
             su[threadIdx.x] = smax(su[threadIdx.x], su[threadIdx.x + s]);
             sv[threadIdx.x] = smax(sv[threadIdx.x], sv[threadIdx.x + s]);
+            sk[threadIdx.x] = smax(sk[threadIdx.x], sk[threadIdx.x + s]);
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
         out[0] = su[0];
         out[1] = sv[0];
+        out[2] = sk[0];
     }
 }
 
@@ -378,6 +387,26 @@ __global__ __launch_bounds__(256) void k_mom_v(Geom G, MomArgs A, const double* 
 
 using namespace asd2;
 
+namespace asd2 {
+namespace {
+// h = {max |u_{i+1} u_i / 2|, max |v_{j+1} v_j / 2|, max |kappa| (0 without kappa)} over the interior
+int velocity_maxima(asd2_ctx* ctx, const double* d_u, const double* d_v, const double* d_kappa, double h[3]) {
+    const Geom& G = ctx->geo;
+    double *partials, *out;
+    const dim3 grid(std::max(1, std::min(8, (G.ny + 255) / 256)), std::max(1, std::min(128, G.nx)));
+    const int nb = static_cast<int>(grid.x * grid.y);
+    ASD2_TRY(ws(ctx, "dt_partials", 3 * nb, &partials));
+    ASD2_TRY(ws(ctx, "dt_out", 3, &out));
+    hipLaunchKernelGGL(k_velmax, grid, dim3(256), 0, ctx->stream, G, d_u, d_v, d_kappa, partials);
+    hipLaunchKernelGGL(k_velmax_final, dim3(1), dim3(256), 0, ctx->stream, partials, nb, out);
+    ASD2_LAUNCH_CHECK();
+    ASD2_HIP(hipMemcpyAsync(h, out, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    ASD2_HIP(hipStreamSynchronize(ctx->stream));
+    return ASD2_OK;
+}
+}  // namespace
+}  // namespace asd2
+
 extern "C" {
 
 int asd2_mac_kappa(asd2_ctx* ctx, const double* d_ls, double* d_kappa) {
@@ -426,21 +455,33 @@ int asd2_mac_update_vel(asd2_ctx* ctx, const asd2_fluid* fl, double* d_u, double
     return ASD2_OK;
 }
 
+
+// MACSolver2DAxisym::UpdateDt (src/mac2dAxisym.cpp:2094-2128): the capillary limit from max |kappa|,
+// no viscous term in the bound
+int asd2_mac_update_dt_axisym(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_u, const double* d_v,
+                              const double* d_kappa, double* dt_out) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!fl || !d_u || !d_v || !d_kappa || !dt_out) { set_error("null argument"); return ASD2_ERR_ARG; }
+    double h[3];
+    ASD2_TRY(velocity_maxima(ctx, d_u, d_v, d_kappa, h));
+    const double dr = ctx->geo.dx, dz = ctx->geo.dy;
+    const double C = h[0] / dr + h[1] / dz;
+    const double Gf = std::max(std::sqrt(std::fabs(fl->g) / dz), std::sqrt(std::fabs(fl->g) / dr));
+    const double S = std::sqrt((fl->sigma * h[2]) / (std::min(fl->rho_h, fl->rho_l) * std::pow(std::min(dr, dz), 2.0)));
+    double dt = std::numeric_limits<double>::max();
+    dt = std::min(dt, fl->cfl / (0.5 * (C + std::sqrt(std::pow(C, 2.0) + 4.0 * std::pow(Gf, 2.0) +
+                                                     4.0 * std::pow(S, 2.0)))));
+    *dt_out = dt;
+    if (!std::isfinite(dt)) { set_error("dt nan/inf"); return ASD2_ERR_NONFINITE; }
+    return ASD2_OK;
+}
+
 int asd2_mac_update_dt(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_u, const double* d_v, double* dt_out) {
     ASD2_TRY(check_ctx(ctx));
     if (!fl || !d_u || !d_v || !dt_out) { set_error("null argument"); return ASD2_ERR_ARG; }
     const Geom& G = ctx->geo;
-    double *partials, *out;
-    const dim3 grid(std::max(1, std::min(8, (G.ny + 255) / 256)), std::max(1, std::min(128, G.nx)));
-    const int nb = static_cast<int>(grid.x * grid.y);
-    ASD2_TRY(ws(ctx, "dt_partials", 2 * nb, &partials));
-    ASD2_TRY(ws(ctx, "dt_out", 2, &out));
-    hipLaunchKernelGGL(k_velmax, grid, dim3(256), 0, ctx->stream, G, d_u, d_v, partials);
-    hipLaunchKernelGGL(k_velmax_final, dim3(1), dim3(256), 0, ctx->stream, partials, nb, out);
-    ASD2_LAUNCH_CHECK();
-    double h[2];
-    ASD2_HIP(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
-    ASD2_HIP(hipStreamSynchronize(ctx->stream));
+    double h[3];
+    ASD2_TRY(velocity_maxima(ctx, d_u, d_v, nullptr, h));
     const double dx = G.dx, dy = G.dy;
     const double C = h[0] / dx + h[1] / dy;
     const double V = std::max(fl->mu_h, fl->mu_l) * (2.0 / (dx * dx + dy * dy));

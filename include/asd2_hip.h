@@ -197,11 +197,30 @@ int asd2_mac_kappa(asd2_ctx* ctx, const double* d_ls, double* d_kappa);
 int asd2_mac_heaviside(asd2_ctx* ctx, const double* d_ls, double* d_H, int smooth);
 /* MACSolver2D::GetDivergence, src/mac2d.cpp:1729-1738 */
 int asd2_mac_divergence(asd2_ctx* ctx, const double* d_u, const double* d_v, double* d_div);
+/* MACSolver2DAxisym::GetDivergence4Poisson (for_poisson = 1, src/mac2dAxisym.cpp:1974-1989):
+ * (r_{i+1/2} u_{i+1} - r_{i-1/2} u_i) / dr + r_i (v_{j+1} - v_j) / dz; GetDivergence (for_poisson =
+ * 0, :1991-2008): (u_i + u_{i+1}) / (2 r_i) + (u_{i+1} - u_i) / dr + (v_{j+1} - v_j) / dz.  x = r
+ * from the grid's base x, y = z; zero outside the interior. */
+int asd2_mac_divergence_axisym(asd2_ctx* ctx, const double* d_u, const double* d_v, double* d_div, int for_poisson);
+/* MACSolver2DAxisym::UpdateKappa (src/mac2dAxisym.cpp:222-262): the planar curvature plus the hoop
+ * term phi_r |grad phi|^2 / r, phi_r one-sided on the axis row, clipped to 1 / min(dr, dz);
+ * interior cells written */
+int asd2_mac_kappa_axisym(asd2_ctx* ctx, const double* d_ls, double* d_kappa);
+/* MACSolver2DAxisym::SolveTDMA (src/mac2dAxisym.cpp:2255-2304), batched: count tridiagonal systems
+ * of n >= 2 unknowns (sub-diagonal a, diagonal b, super-diagonal c, right-hand side d), element k of
+ * system s at index k * count + s of each device array; the reference's recurrence, solution in d,
+ * c overwritten with the eliminated super-diagonal (as the reference leaves them).  The reference's
+ * n = 1 reads d[-1]; here n = 1 is refused. */
+int asd2_tdma(asd2_ctx* ctx, int n, int count, const double* d_a, const double* d_b, double* d_c, double* d_d);
 /* MACSolver2D::UpdateVel, src/mac2d.cpp:1740-1849 (then U/V BCs) */
 int asd2_mac_update_vel(asd2_ctx* ctx, const asd2_fluid* fl, double* d_u, double* d_v,
                         const double* d_us, const double* d_vs, const double* d_ps,
                         const double* d_ls, const double* d_kappa, const double* d_H);
 /* MACSolver2D::UpdateDt, src/mac2d.cpp:1851-1880 */
+/* MACSolver2DAxisym::UpdateDt (src/mac2dAxisym.cpp:2094-2128): the planar bound with the capillary
+ * term sqrt(sigma max|kappa| / (min rho min(dr, dz)^2)) and no viscous term; x = r, y = z */
+int asd2_mac_update_dt_axisym(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_u, const double* d_v,
+                              const double* d_kappa, double* dt_out);
 int asd2_mac_update_dt(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_u, const double* d_v,
                        double* dt_out);
 /* MACSolver2D::GetIntermediateVel, src/mac2d.cpp:1081-1239 (time_order 0/1/2 =
@@ -226,6 +245,17 @@ int asd2_mac_momentum_term(asd2_ctx* ctx, const asd2_fluid* fl, int grid_kind, i
  * intended operator (the reference leaves the centre column unshifted, :1601-1604). */
 int asd2_gfm_assemble(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_ls,
                       const double* d_kappa, const double* d_H, const double* d_div);
+/* Assembly half of MACSolver2DAxisym::SolvePoisson, src/mac2dAxisym.cpp:1488-1950, on an (r, z)
+ * grid (x = r, base radius = the grid's base x; y = z): the planar assembly with the face
+ * coefficients r_eff / rho_eff (r_eff the W / E face radius in one phase, the interface-point
+ * radius across the interface, clamped at 0 on W; the cell radius on S / N), WALL and AXISYM
+ * sides Neumann-like, Dirichlet-family sides adding twice the wall constant (the reference's
+ * -ps + 2c with ps zeroed).  d_div4p: the r-weighted divergence (asd2_mac_divergence_axisym with
+ * for_poisson = 1).  The stored operator is the symmetric one (the W value of a cell also serves
+ * as the E value of its west neighbour; the reference's own E value differs from it by rounding
+ * across the interface).  Solve with asd2_poisson_solve as for the planar system. */
+int asd2_gfm_assemble_axisym(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_ls, const double* d_kappa,
+                             const double* d_H, const double* d_div4p);
 /* copies the assembled system out (any pointer may be NULL): ax/ay = CSR value of the
  * west/south neighbour of each cell, diag = CSR centre value, rhs = right-hand side */
 int asd2_gfm_export(asd2_ctx* ctx, double* d_ax, double* d_ay, double* d_diag, double* d_rhs);

@@ -118,6 +118,10 @@ class Oracle:
             "or_intermediate_vel": [G, F, C.c_int, _P, _P, _P, _P, _P, _P],
             "or_gfm_assemble_stencil": [G, F, _P, _P, _P, _P, _P, _P, _P, _P],
             "or_stencil_apply": [G, _P, _P, _P, _P, _P],
+            "or_gfm_assemble_axisym": [G, F, C.c_double, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+            "or_divergence_axisym": [G, C.c_double, _P, _P, _P, C.c_int],
+            "or_tdma": [_P, _P, _P, _P, C.c_int],
+            "or_update_kappa_axisym": [G, C.c_double, _P, _P],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -129,6 +133,8 @@ class Oracle:
         L.or_reinit_iterations.restype = C.c_int
         L.or_update_dt.argtypes = [G, F, _P, _P]
         L.or_update_dt.restype = C.c_double
+        L.or_update_dt_axisym.argtypes = [G, F, _P, _P, _P]
+        L.or_update_dt_axisym.restype = C.c_double
         IP = C.POINTER(C.c_int)
         L.or_pcg_stencil.argtypes = [G, _P, _P, _P, _P, _P, C.c_double, C.c_int, C.c_int, IP, _P]
         L.or_pcg_stencil.restype = C.c_int
@@ -259,6 +265,32 @@ class Oracle:
         self.lib.or_gfm_assemble_stencil(C.byref(gr), C.byref(fl), _ptr(ls), _ptr(kappa), _ptr(H),
                                          _ptr(div), _ptr(ax), _ptr(ay), _ptr(dg), _ptr(rhs))
         return ax, ay, dg, rhs
+
+    def gfm_assemble_axisym(self, gr, fl, base_r, ls, kappa, H, div):
+        """(ax, ay, ae, diag, rhs) of MACSolver2DAxisym::SolvePoisson's system"""
+        out = [np.zeros_like(ls) for _ in range(5)]
+        self.lib.or_gfm_assemble_axisym(C.byref(gr), C.byref(fl), base_r, _ptr(ls), _ptr(kappa), _ptr(H),
+                                        _ptr(div), *[_ptr(a) for a in out])
+        return tuple(out)
+
+    def divergence_axisym(self, gr, base_r, u, v, for_poisson):
+        d = np.zeros_like(u)
+        self.lib.or_divergence_axisym(C.byref(gr), base_r, _ptr(u), _ptr(v), _ptr(d), int(for_poisson))
+        return d
+
+    def update_kappa_axisym(self, gr, base_r, ls):
+        k = np.zeros_like(ls)
+        self.lib.or_update_kappa_axisym(C.byref(gr), base_r, _ptr(ls), _ptr(k))
+        return k
+
+    def update_dt_axisym(self, gr, fl, u, v, kappa):
+        return self.lib.or_update_dt_axisym(C.byref(gr), C.byref(fl), _ptr(u), _ptr(v), _ptr(kappa))
+
+    def tdma(self, a, b, c, d):
+        """x of the tridiagonal system (copies in, the reference's recurrence)"""
+        a, b, c, d = (np.array(v, dtype=np.float64) for v in (a, b, c, d))
+        self.lib.or_tdma(_ptr(a), _ptr(b), _ptr(c), _ptr(d), len(d))
+        return d
 
     def stencil_apply(self, gr, ax, ay, dg, p):
         q = np.zeros_like(p)

@@ -1475,3 +1475,197 @@ int or_gs_csr(int n, const int* rowptr, const int* cols, const double* vals, con
     *rel_change = err;
     return 0;
 }
+
+/* ------------------------------------------------------------------ axisymmetric (r, z) */
+/* MACSolver2DAxisym::SolvePoisson assembly, src/mac2dAxisym.cpp:1564-1870 (x = r, y = z, r of
+ * cell i = base_r + (i + 0.5 - g) dr).  Stored like or_gfm_assemble_stencil: ax / ay = CSR value
+ * of the W / S neighbour; ae = the row's own E value (the reference's, for checking that it
+ * equals the east neighbour's ax up to rounding). */
+void or_gfm_assemble_axisym(const or_grid* gr, const or_fluid* fl, double base_r, const double* ls,
+                            const double* kappa, const double* H, const double* div, double* ax, double* ay,
+                            double* ae, double* diag, double* rhs) {
+    const int nr = gr->nx, nz = gr->ny, g = gr->g;
+    const double dr = gr->dx, dz = gr->dy, dt = fl->dt, sigma = fl->sigma;
+    for (int64_t k = 0; k < or_size(gr); k++) { ax[k] = 0.0; ay[k] = 0.0; ae[k] = 0.0; diag[k] = 0.0; rhs[k] = 0.0; }
+    int pin = 1;
+    for (int s = 0; s < 4; s++) {
+        const int k = gr->bc_p[s];
+        if (!(k == OR_BC_NEUMANN || k == OR_BC_AXISYM || k == OR_BC_WALL)) pin = 0;
+    }
+    for (int j = g; j < nz + g; j++)
+        for (int i = g; i < nr + g; i++) {
+            const int64_t c = IDX(gr, i, j);
+            const double lsW = ls[IDX(gr, i - 1, j)], lsE = ls[IDX(gr, i + 1, j)], lsM = ls[c];
+            const double lsS = ls[IDX(gr, i, j - 1)], lsN = ls[IDX(gr, i, j + 1)];
+            const double rPW = base_r + (i - 0.5 - g) * dr, rPM = base_r + (i + 0.5 - g) * dr;
+            const double rPE = base_r + (i + 1.5 - g) * dr;
+            const double rPWHalf = base_r + (i - g) * dr, rPEHalf = base_r + (i + 1.0 - g) * dr;
+            double thetaH = 0.0, theta = 0.0, rEff = 0.0, rhoEff, kappaEff;
+            /* W face */
+            if (lsW >= 0.0 && lsM >= 0.0) { thetaH = 1.0; theta = 0.0; rEff = rPWHalf; }
+            else if (lsW < 0.0 && lsM < 0.0) { thetaH = 0.0; theta = 0.0; rEff = rPWHalf; }
+            else if (lsW >= 0.0 && lsM < 0.0) {
+                thetaH = fabs(lsW) / (fabs(lsW) + fabs(lsM));
+                theta = fabs(lsW) / (fabs(lsW) + fabs(lsM));
+                rEff = smax(rPM * theta + rPW * (1.0 - theta), 0.0);
+            } else {
+                thetaH = fabs(lsM) / (fabs(lsW) + fabs(lsM));
+                theta = fabs(lsW) / (fabs(lsW) + fabs(lsM));
+                rEff = smax(rPM * theta + rPW * (1.0 - theta), 0.0);
+            }
+            rhoEff = fl->rho_h * thetaH + fl->rho_l * (1.0 - thetaH);
+            kappaEff = kappa[c] * theta + kappa[IDX(gr, i - 1, j)] * (1.0 - theta);
+            double pW = rEff / rhoEff;
+            double FW = -dt * (-sigma * kappaEff) * (H[c] - H[IDX(gr, i - 1, j)]);
+            FW *= pW / (dr * dr);
+            /* E face */
+            if (lsM >= 0.0 && lsE >= 0.0) { thetaH = 1.0; theta = 0.0; rEff = rPEHalf; }
+            else if (lsM < 0.0 && lsE < 0.0) { thetaH = 0.0; theta = 0.0; rEff = rPEHalf; }
+            else if (lsM >= 0.0 && lsE < 0.0) {
+                thetaH = fabs(lsM) / (fabs(lsM) + fabs(lsE));
+                theta = fabs(lsE) / (fabs(lsM) + fabs(lsE));
+                rEff = rPM * theta + rPE * (1.0 - theta);
+            } else {
+                thetaH = fabs(lsE) / (fabs(lsM) + fabs(lsE));
+                theta = fabs(lsE) / (fabs(lsM) + fabs(lsE));
+                rEff = rPM * theta + rPE * (1.0 - theta);
+            }
+            rhoEff = fl->rho_h * thetaH + fl->rho_l * (1.0 - thetaH);
+            kappaEff = kappa[c] * theta + kappa[IDX(gr, i + 1, j)] * (1.0 - theta);
+            double pE = rEff / rhoEff;
+            double FE = dt * (-sigma * kappaEff) * (H[IDX(gr, i + 1, j)] - H[c]);
+            FE *= pE / (dr * dr);
+            /* S face */
+            face_theta(lsS, lsM, 1, &thetaH, &theta);
+            rhoEff = fl->rho_h * thetaH + fl->rho_l * (1.0 - thetaH);
+            kappaEff = kappa[c] * theta + kappa[IDX(gr, i, j - 1)] * (1.0 - theta);
+            double pS = rPM / rhoEff;
+            double FS = -dt * (-sigma * kappaEff) * (H[c] - H[IDX(gr, i, j - 1)]);
+            FS *= pS / (dz * dz);
+            /* N face */
+            face_theta(lsM, lsN, 0, &thetaH, &theta);
+            rhoEff = fl->rho_h * thetaH + fl->rho_l * (1.0 - thetaH);
+            kappaEff = kappa[c] * theta + kappa[IDX(gr, i, j + 1)] * (1.0 - theta);
+            double pN = rPM / rhoEff;
+            double FN = dt * (-sigma * kappaEff) * (H[IDX(gr, i, j + 1)] - H[c]);
+            FN *= pN / (dz * dz);
+            rhs[c] -= FW + FE + FS + FN;
+            rhs[c] -= div[c];
+            /* matrix row, :1756-1870 (ps is zeroed before the assembly) */
+            pS *= -1.0; pW *= -1.0; pE *= -1.0; pN *= -1.0;
+            double aS = pS / (dz * dz), aW = pW / (dr * dr), aE = pE / (dr * dr);
+            double C = -(pW + pE) / (dr * dr) - (pS + pN) / (dz * dz);
+            for (int s = 0; s < 4; s++) {
+                const int k = gr->bc_p[s];
+                const int at = (s == 0 && i == g) || (s == 1 && i == g + nr - 1) || (s == 2 && j == g) ||
+                               (s == 3 && j == g + nz - 1);
+                if (!at) continue;
+                const double p = s == 0 ? pW : s == 1 ? pE : s == 2 ? pS : pN;
+                const double h2 = s < 2 ? dr * dr : dz * dz;
+                if (k == OR_BC_NEUMANN || k == OR_BC_AXISYM || k == OR_BC_WALL) {
+                    if (s == 0) aW = 0.0;
+                    if (s == 1) aE = 0.0;
+                    if (s == 2) aS = 0.0;
+                    C += p / h2;
+                } else if (k == OR_BC_DIRICHLET || k == OR_BC_INLET || k == OR_BC_OUTLET || k == OR_BC_PRESSURE) {
+                    if (s == 0) aW = 0.0;
+                    if (s == 1) aE = 0.0;
+                    if (s == 2) aS = 0.0;
+                    rhs[c] -= p / h2 * (-0.0 + 2.0 * gr->c_p[s]);
+                }
+            }
+            if (pin) {
+                if (i == g + 1 && j == g) aW = 0.0;
+                else if (i == g && j == g + 1) aS = 0.0;
+                if (i == g && j == g) rhs[c] = 0.0;
+            }
+            ax[c] = aW;
+            ay[c] = aS;
+            ae[c] = aE;
+            diag[c] = C;
+        }
+}
+
+/* GetDivergence4Poisson (for_poisson = 1, src/mac2dAxisym.cpp:1974-1989) and GetDivergence
+ * (:1991-2008) */
+void or_divergence_axisym(const or_grid* gr, double base_r, const double* u, const double* v, double* div,
+                          int for_poisson) {
+    const int nr = gr->nx, nz = gr->ny, g = gr->g;
+    const double dr = gr->dx, dz = gr->dy;
+    for (int64_t k = 0; k < or_size(gr); k++) div[k] = 0.0;
+    for (int i = g; i < nr + g; i++)
+        for (int j = g; j < nz + g; j++) {
+            const double rPM = base_r + (i + 0.5 - g) * dr;
+            const double rPEHalf = base_r + (i + 1.0 - g) * dr, rPWHalf = base_r + (i - g) * dr;
+            if (for_poisson)
+                div[IDX(gr, i, j)] = (rPEHalf * u[IDX(gr, i + 1, j)] - rPWHalf * u[IDX(gr, i, j)]) / dr
+                                   + rPM * (v[IDX(gr, i, j + 1)] - v[IDX(gr, i, j)]) / dz;
+            else
+                div[IDX(gr, i, j)] = 0.5 * (u[IDX(gr, i, j)] + u[IDX(gr, i + 1, j)]) / rPM
+                                   + (u[IDX(gr, i + 1, j)] - u[IDX(gr, i, j)]) / dr
+                                   + (v[IDX(gr, i, j + 1)] - v[IDX(gr, i, j)]) / dz;
+        }
+}
+
+/* MACSolver2DAxisym::UpdateDt, src/mac2dAxisym.cpp:2094-2128 */
+double or_update_dt_axisym(const or_grid* gr, const or_fluid* fl, const double* u, const double* v,
+                           const double* kappa) {
+    const int nr = gr->nx, nz = gr->ny, g = gr->g;
+    const double dr = gr->dx, dz = gr->dy;
+    double uA = 0.0, vA = 0.0, kA = 0.0;
+    for (int i = g; i < nr + g; i++)
+        for (int j = g; j < nz + g; j++) {
+            uA = smax(uA, fabs((u[IDX(gr, i + 1, j)] * u[IDX(gr, i, j)]) * 0.5));
+            vA = smax(vA, fabs((v[IDX(gr, i, j + 1)] * v[IDX(gr, i, j)]) * 0.5));
+            kA = smax(kA, fabs(kappa[IDX(gr, i, j)]));
+        }
+    const double C = uA / dr + vA / dz;
+    const double G = smax(sqrt(fabs(fl->g) / dz), sqrt(fabs(fl->g) / dr));
+    const double S = sqrt((fl->sigma * kA) / (smin(fl->rho_h, fl->rho_l) * pow(smin(dr, dz), 2.0)));
+    double dt = 1.7976931348623157e308;
+    dt = smin(dt, fl->cfl / (0.5 * (C + sqrt(pow(C, 2.0) + 4.0 * pow(G, 2.0) + 4.0 * pow(S, 2.0)))));
+    return dt;
+}
+
+/* MACSolver2DAxisym::SolveTDMA, src/mac2dAxisym.cpp:2255-2304 (n >= 2 unknowns; d <- x) */
+void or_tdma(double* a, double* b, double* c, double* d, int n) {
+    n--;
+    c[0] /= b[0];
+    d[0] /= b[0];
+    for (int i = 1; i < n; i++) {
+        c[i] /= b[i] - a[i] * c[i - 1];
+        d[i] = (d[i] - a[i] * d[i - 1]) / (b[i] - a[i] * c[i - 1]);
+    }
+    d[n] = (d[n] - a[n] * d[n - 1]) / (b[n] - a[n] * c[n - 1]);
+    for (int i = n; i-- > 0;) d[i] -= c[i] * d[i + 1];
+}
+
+/* MACSolver2DAxisym::UpdateKappa, src/mac2dAxisym.cpp:222-262 (interior cells written) */
+void or_update_kappa_axisym(const or_grid* gr, double base_r, const double* ls, double* kappa) {
+    const int nr = gr->nx, nz = gr->ny, g = gr->g;
+    const double dr = gr->dx, dz = gr->dy, eps = 1.0e-100;
+    double* dR = zalloc(gr);
+    double* dZ = zalloc(gr);
+    double* S = zalloc(gr);
+    for (int i = g; i < nr + g; i++)
+        for (int j = g; j < nz + g; j++) {
+            const int64_t c = IDX(gr, i, j);
+            dR[c] = (ls[IDX(gr, i + 1, j)] - ls[IDX(gr, i - 1, j)]) / (2.0 * dr);
+            dZ[c] = (ls[IDX(gr, i, j + 1)] - ls[IDX(gr, i, j - 1)]) / (2.0 * dz);
+            if (i == g) dR[c] = (ls[IDX(gr, i + 1, j)] - ls[c]) / dr;
+            S[c] = sqrt(dR[c] * dR[c] + dZ[c] * dZ[c] + eps);
+        }
+    for (int i = g; i < nr + g; i++)
+        for (int j = g; j < nz + g; j++) {
+            const int64_t c = IDX(gr, i, j);
+            const double rPM = base_r + (i + 0.5 - g) * dr;
+            double k = ((ls[IDX(gr, i - 1, j)] - 2.0 * ls[c] + ls[IDX(gr, i + 1, j)]) / (dr * dr) * dZ[c] * dZ[c]
+                        - 2.0 * (dZ[IDX(gr, i + 1, j)] - dZ[IDX(gr, i - 1, j)]) / (2.0 * dr) * dR[c] * dZ[c]
+                        + (ls[IDX(gr, i, j - 1)] - 2.0 * ls[c] + ls[IDX(gr, i, j + 1)]) / (dz * dz) * dR[c] * dR[c]
+                        + dR[c] * pow(S[c], 2.0) / rPM) / pow(S[c], 3.0);
+            kappa[c] = ((k > 0) - (k < 0)) * fabs(smin(fabs(k), 1.0 / smin(dr, dz)));
+        }
+    free(dR);
+    free(dZ);
+    free(S);
+}

@@ -114,6 +114,19 @@ void or_gfm_assemble_stencil(const or_grid* gr, const or_fluid* fl, const double
 void or_stencil_apply(const or_grid* gr, const double* ax, const double* ay, const double* diag,
                       const double* p, double* q);
 
+/* axisymmetric (r, z): x = r with the cell radius base_r + (i + 0.5 - g) dr, y = z.
+ * src/mac2dAxisym.cpp:1564-1870 (assembly; ae = each row's own E value), :1974-2008
+ * (divergences), :2094-2128 (time step), :2255-2304 (tridiagonal solve, n >= 2) */
+void or_gfm_assemble_axisym(const or_grid* gr, const or_fluid* fl, double base_r, const double* ls,
+                            const double* kappa, const double* H, const double* div, double* ax, double* ay,
+                            double* ae, double* diag, double* rhs);
+void or_divergence_axisym(const or_grid* gr, double base_r, const double* u, const double* v, double* div,
+                          int for_poisson);
+double or_update_dt_axisym(const or_grid* gr, const or_fluid* fl, const double* u, const double* v,
+                           const double* kappa);
+void or_tdma(double* a, double* b, double* c, double* d, int n);
+void or_update_kappa_axisym(const or_grid* gr, double base_r, const double* ls, double* kappa);
+
 /* Krylov solvers.  mode 0 = reference CG iteration (poisson2d.cpp:345-396: iteration-0
  * p = rho*r + z, iteration-1 p = z, alpha denominator + 1e-12), mode 1 = textbook PCG. */
 int or_pcg_stencil(const or_grid* gr, const double* ax, const double* ay, const double* diag,
