@@ -1,7 +1,7 @@
 """The rising-bubble run (bubble r = 0.25 at (0.5, 0.5) in the [0, 1] x [0, 2] box, density
 1000/100, viscosity 10/1, sigma = 24.5, g = 0.98, no-slip walls, Neumann pressure) at 256 x 512
-for 20 steps, once with the mixed-precision multilevel pressure path and once with the FP64
-Jacobi-PCG path (the reference's method, src/poisson2d.cpp:285-401), both to a relative FP64
+for 20 steps and at the full 1024 x 2048 for 5 steps, once with the mixed-precision multilevel
+pressure path and once with the FP64 Jacobi-PCG path (the reference's method, src/poisson2d.cpp:285-401), both to a relative FP64
 residual of 1e-8.  The step is the driver loop of the reference (tests/LargeTests/
 test_mac2d_large.cpp:376-540: advection, 3 MinRK2 reinitialisation iterations, curvature,
 Heaviside, momentum predictor, GFM assembly, pressure, projection, CFL step), gravity on v
@@ -17,13 +17,11 @@ from tests import scenarios as S
 
 pytestmark = pytest.mark.gpu
 
-NX, NY, STEPS = 256, 512, 20
-
 
 class Bubble:
     RHO_H, RHO_L, MU_H, MU_L, SIGMA, G, CFL = 1000.0, 100.0, 10.0, 1.0, 24.5, 0.98, 0.3
 
-    def __init__(self):
+    def __init__(self, NX, NY):
         self.dx = 1.0 / NX
         c = A.Context(NX, NY, self.dx, self.dx)
         c.set_bc(A.GRID_P, ("neumann",) * 4)
@@ -68,9 +66,9 @@ class Bubble:
         return self.c.ls_diagnostics(neg)
 
 
-def _run(opts):
+def _run(opts, NX, NY, STEPS):
     import time
-    b = Bubble()
+    b = Bubble(NX, NY)
     d0 = b.light_phase()
     iters = []
     t0 = time.perf_counter()
@@ -81,11 +79,13 @@ def _run(opts):
     return d0, b.light_phase(), iters, ms, b.ls.download(), b.v.download()
 
 
-def test_rising_bubble_mixed_matches_fp64():
+@pytest.mark.parametrize("NX,NY,STEPS", [(256, 512, 20), (1024, 2048, 5)])
+def test_rising_bubble_mixed_matches_fp64(NX, NY, STEPS):
     tol = 1e-8
     d0, dm, it_m, ms_m, phi_m, v_m = _run(A.Context.opts(tol=tol, precision=A.PREC_MIXED,
-                                                          precond=A.PRECOND_MULTILEVEL))
-    _, df, it_f, ms_f, phi_f, v_f = _run(A.Context.opts(tol=tol, precision=A.PREC_FP64, mode=A.CG_STANDARD))
+                                                          precond=A.PRECOND_MULTILEVEL), NX, NY, STEPS)
+    _, df, it_f, ms_f, phi_f, v_f = _run(A.Context.opts(tol=tol, precision=A.PREC_FP64, mode=A.CG_STANDARD),
+                                         NX, NY, STEPS)
     g = 3
     inner = (slice(g, g + NX), slice(g, g + NY))
     e_phi = float(np.abs(phi_m[inner] - phi_f[inner]).max() / np.abs(phi_f[inner]).max())
