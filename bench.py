@@ -11,7 +11,8 @@ weights, trained on the GPU on droplet systems the library builds; `--train` ret
 The learned path is also timed with a zero network, the ablation that shows what the network
 gains.  `value` is the ms/step of the fastest path (inputs resident in HBM).
 
-Usage: python bench.py [--steps K] [--warmup W] [--n 2048] [--no-cpu] [--no-learned] [--train] [--pmc-workload]
+Usage: python bench.py [--steps K] [--warmup W] [--n 2048] [--no-cpu] [--no-learned] [--train] [--no-configs]
+       [--pmc-workload]
 """
 from __future__ import annotations
 
@@ -30,6 +31,7 @@ sys.path.insert(0, ROOT)
 METRIC = "2048² two-phase step ms + PCG iters: learned/mixed vs FP64 PCG @1e-8 res"
 
 PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6  # dense FP64 matrix peak of MI355X
 # Algorithmic (compulsory) bytes per interior cell and launch: every array the unit of work
 # must read once and write once; halos, ghosts and re-reads are free.
 BYTES_PER_CELL = {
@@ -201,41 +203,151 @@ def profile_kernels(sim, opts, snap, kids):
     return out
 
 
-def cpu_baseline(n, gpu_iters_fp64, budget_s=20.0):
-    """The FP64 oracle (single thread) on a bounded sample of the same step: one
-    advection + reinit + assembly at n^2 and a few PCG iterations, extrapolated to the
-    FP64 GPU iteration count."""
+def cpu_baseline(n, gpu_iters_fp64, repeats=5):
+    """The FP64 oracle (single thread) on a bounded sample of the same step, each part timed
+    `repeats` times and the median taken: one advection + 3 MinRK2 iterations, curvature +
+    Heaviside + GFM assembly at n^2, and 10 PCG iterations, extrapolated to the FP64 GPU
+    iteration count."""
     from oracle import Oracle, make_fluid, make_grid
     from tests import scenarios as S
     O = Oracle()
     dx = 1.0 / n
     gr = make_grid(n, n, dx, dx)
-    phi = S.circle_sdf(n, n, dx, dx, 0.5, 0.5, 0.25)
-    u = np.zeros_like(phi)
-    v = np.zeros_like(phi)
-    t = time.perf_counter()
-    O.solve_levelset(gr, phi, u, v, 0.3 * dx)
-    O.reinit_minrk2(gr, phi, 3)
-    t_ls = time.perf_counter() - t
-    t = time.perf_counter()
-    kap = O.update_kappa(gr, phi)
-    O.bc_p(gr, kap)
-    H = O.heaviside(gr, phi)
-    div = np.zeros_like(phi)
+    phi0 = S.circle_sdf(n, n, dx, dx, 0.5, 0.5, 0.25)
+    u = np.zeros_like(phi0)
+    v = np.zeros_like(phi0)
     fl = make_fluid(1000.0, 1.0, 1e-2, 1e-4, 1.0, 1e-5)
-    ax, ay, dg, rhs = O.gfm_assemble(gr, fl, phi, kap, H, div)
-    t_asm = time.perf_counter() - t
     k = 10
-    t = time.perf_counter()
-    O.pcg(gr, ax, ay, dg, rhs, 1e-30, k, 1)
-    t_it = (time.perf_counter() - t) / k
-    ms = (t_ls + t_asm + t_it * gpu_iters_fp64) * 1e3
+    t_ls, t_asm, t_it = [], [], []
+    for _ in range(repeats):
+        phi = phi0.copy()
+        t = time.perf_counter()
+        phi = O.solve_levelset(gr, phi, u, v, 0.3 * dx)
+        O.reinit_minrk2(gr, phi, 3)
+        t_ls.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        kap = np.zeros_like(phi)
+        O.update_kappa(gr, phi, kap)
+        O.bc_p(gr, kap)
+        H = O.heaviside(gr, phi)
+        ax, ay, dg, rhs = O.gfm_assemble(gr, fl, phi, kap, H, np.zeros_like(phi))
+        t_asm.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        O.pcg(gr, ax, ay, dg, rhs, 1e-30, k, 1)
+        t_it.append((time.perf_counter() - t) / k)
+    m_ls, m_asm, m_it = (float(np.median(x)) for x in (t_ls, t_asm, t_it))
+    ms = (m_ls + m_asm + m_it * gpu_iters_fp64) * 1e3
     return {"value": ms, "unit": "ms/step", "cores": 1, "kind": "port",
-            "sample": f"oracle (FP64, 1 thread) at {n}^2: 1 advection + 3 MinRK2 iterations "
-                      f"({t_ls:.2f} s), kappa+H+GFM assembly ({t_asm:.2f} s), {k} PCG iterations "
-                      f"({t_it * 1e3:.1f} ms each) extrapolated to {gpu_iters_fp64:.0f} iterations; "
+            "sample": f"oracle (FP64, 1 thread) at {n}^2, median of {repeats}: 1 advection + 3 MinRK2 iterations "
+                      f"({m_ls:.2f} s), kappa+H+GFM assembly ({m_asm:.2f} s), {k} PCG iterations "
+                      f"({m_it * 1e3:.1f} ms each) extrapolated to {gpu_iters_fp64:.0f} iterations; "
                       f"the level-set part is bit-identical to the reference's own code",
+            "parts_s": {"levelset": t_ls, "assembly": t_asm, "pcg_iteration": t_it},
             "cpu": cpu_info()[0], "nproc": cpu_info()[1]}
+
+
+def config_c2(A, n=4096, steps=10, reinit=5):
+    """Single-vortex level-set transport at n^2: per step one HJ-WENO5 / SSP-RK3 advection and
+    `reinit` MinRK2 iterations (the level-set part of the reference driver
+    tests/LargeTests/test_levelset2d_large.cpp:84-104).  The velocity is the vortex at t = 0 for
+    every timed step (the per-step cos(pi t / T) factor changes values, not the work)."""
+    from tests import scenarios as S
+    dx = 1.0 / n
+    c = A.Context(n, n, dx, dx)
+    c.set_bc(A.GRID_P, ("neumann",) * 4)
+    phi = np.sqrt((S.centres(n, n, dx, dx)[0] - 0.5) ** 2 + (S.centres(n, n, dx, dx)[1] - 0.75) ** 2) - 0.15
+    ls = c.array(np.ascontiguousarray(phi))
+    u, v = (c.array(a) for a in S.single_vortex(n, n, dx, dx))
+    dt = 0.5 * dx
+    c.ls_advect(ls, u, v, dt)
+    c.ls_reinit_minrk2(ls, reinit)
+    c.sync()
+    t0 = time.perf_counter()
+    c.timer_start()
+    for _ in range(steps):
+        c.ls_advect(ls, u, v, dt)
+        c.ls_reinit_minrk2(ls, reinit)
+    ev = c.timer_stop() / steps
+    c.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    bytes_cell = BYTES_PER_CELL["weno_stage"] * 3 + BYTES_PER_CELL["reinit"] * 2 * reinit
+    return {"workload": f"single vortex level set at {n}x{n}: 1 advection + {reinit} MinRK2 per step",
+            "steps": steps, "ms_per_step": ms, "event_ms_per_step": ev,
+            "cell_updates_per_s": n * n / (ms * 1e-3),
+            "algorithmic_GBs": bytes_cell * n * n / (ms * 1e-3) / 1e9, "bytes_per_cell_step": bytes_cell}
+
+
+def config_c3(A, n=2048, seed=901):
+    """One held-out seeded-droplet pressure system (1-20 circles, radius U[0.02, 0.1], density ratio
+    log-uniform in [10, 1000]) solved to a relative FP64 residual of 1e-8 by each pressure path"""
+    from asd2_amd import learned as LN
+    ctx = LN.system_context(n)
+    LN.build_system(ctx, np.random.default_rng(seed))
+    net = LN.pretrained(n, 32)
+    out = {"workload": f"seeded-droplet GFM pressure system {n}x{n} (seed {seed}), tol 1e-8"}
+    paths = {"fp64": dict(precision=A.PREC_FP64, mode=A.CG_STANDARD),
+             "mixed": dict(precision=A.PREC_MIXED, mode=A.CG_STANDARD, max_iter=60000),
+             "multilevel": dict(precision=A.PREC_MIXED, precond=A.PRECOND_MULTILEVEL),
+             "learned": dict(precision=A.PREC_MIXED, precond=A.PRECOND_LEARNED)}
+    ctx.set_preconditioner(net)
+    for name, kw in paths.items():
+        ps = ctx.array()
+        ctx.sync()
+        t = time.perf_counter()
+        st = ctx.poisson_solve(ps, ctx.opts(tol=1e-8, **kw), raise_on_fail=False)
+        ctx.sync()
+        out[name] = {"ms": (time.perf_counter() - t) * 1e3, "iterations": st.iterations,
+                     "converged": bool(st.converged == 1), "true_rel_residual": st.true_rel_residual,
+                     "fallbacks": st.fallbacks}
+    ctx.set_preconditioner(None)
+    return out
+
+
+def config_c5(A, nx=1024, ny=2048, steps=3):
+    """Rising bubble on the full grid (bubble r = 0.25 at (0.5, 0.5) in [0, 1] x [0, 2], density
+    1000 / 100, viscosity 10 / 1, sigma = 24.5, g = 0.98): ms per step of the mixed multilevel
+    path and of the FP64 Jacobi-PCG path, pressure to 1e-8"""
+    from tests.test_gpu_rising_bubble import Bubble
+    out = {"workload": f"rising bubble {nx}x{ny}, {steps} steps per path"}
+    for name, opts in (("multilevel", A.Context.opts(tol=1e-8, precision=A.PREC_MIXED,
+                                                     precond=A.PRECOND_MULTILEVEL)),
+                       ("fp64", A.Context.opts(tol=1e-8, precision=A.PREC_FP64, mode=A.CG_STANDARD))):
+        b = Bubble(nx, ny)
+        b.step(opts)
+        b.c.sync()
+        t = time.perf_counter()
+        its = [b.step(opts).iterations for _ in range(steps)]
+        b.c.sync()
+        out[name] = {"ms_per_step": (time.perf_counter() - t) * 1e3 / steps, "pcg_iters": its}
+    return out
+
+
+def config_fast_poisson(A, n=2048, repeats=5):
+    """The constant-coefficient fast solve (PoissonSolver2D::MKL_2FUniform_2D) at n^2, Neumann
+    walls: wall time per solve and the two eigenbasis DGEMMs against the FP64 matrix peak"""
+    dx = 1.0 / n
+    c = A.Context(n, n, dx, dx)
+    c.set_bc(A.GRID_P, ("neumann",) * 4)
+    rhs = c.array(np.random.default_rng(0).standard_normal((n + 6, n + 6)))
+    ps = c.array()
+    c.fast_poisson(rhs, ps, 1.0, 1.0, dx, dx)
+    c.sync()
+    t = time.perf_counter()
+    for _ in range(repeats):
+        c.fast_poisson(rhs, ps, 1.0, 1.0, dx, dx)
+    c.sync()
+    ms = (time.perf_counter() - t) * 1e3 / repeats
+    c.profile_kernel(A.KID["fast_gemm"])
+    c.fast_poisson(rhs, ps, 1.0, 1.0, dx, dx)
+    d = c.profile_read_each()
+    c.profile_kernel(0)
+    m = n + 2
+    flops = 2.0 * m * m * m  # one DGEMM, m x m times m x m
+    gemm_ms = float(np.mean(d)) if len(d) else float("nan")
+    return {"workload": f"fast Poisson {n}x{n}, Neumann walls ({m}x{m} node mesh)", "ms_per_solve": ms,
+            "dgemm": {"avg_us": gemm_ms * 1e3, "launches_per_solve": int(len(d)), "flops": flops,
+                      "achieved_TFLOPs": flops / (gemm_ms * 1e-3) / 1e12,
+                      "frac_of_fp64_peak": flops / (gemm_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS}}
 
 
 def pmc_workload(n):
@@ -283,6 +395,8 @@ def main():
                     help="train the learned path's network first (about 6 minutes) instead of loading the committed one")
     ap.add_argument("--paths", default="fp64,mixed,multilevel,learned",
                     help="pressure paths to time, a comma list of fp64, mixed, multilevel, learned")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other configurations (4096^2 level set, seeded droplets, rising bubble, fast Poisson)")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="run the short fixed workload used for hardware-counter passes")
     args = ap.parse_args()
@@ -341,7 +455,12 @@ def main():
         if "fp64" in paths:
             kernels.update(profile_kernels(sim, opts["fp64"], snap, ["pcg64_spmv", "pcg64_update"]))
     prefix = {"mixed": "pcg32", "fp64": "pcg64"}.get(best_name, "fcg")
-    dom = max((k for k in kernels if k.startswith(prefix)), key=lambda k: kernels[k]["total_ms"], default=None)
+    # the solve's kernel families by total time, the coarse correction included; the roofline is
+    # that of the largest family with an HBM traffic model (the coarse correction is a chain of
+    # small latency-bound launches on the 256^2 and coarser levels, with no per-cell HBM figure)
+    family = [k for k in kernels if k.startswith(prefix) or (prefix == "fcg" and k == "coarse")]
+    breakdown = {k: kernels[k]["total_ms"] for k in sorted(family, key=lambda k: -kernels[k]["total_ms"])}
+    dom = max((k for k in family if BYTES_PER_CELL.get(k, 0) > 0), key=lambda k: kernels[k]["total_ms"], default=None)
     roofline = None
     if dom:
         kd = kernels[dom]
@@ -349,9 +468,18 @@ def main():
         roofline = {"kernel": dom, "bound": "hbm", "achieved": kd["achieved_GBs"], "peak": PEAK_HBM_GBS,
                     "unit": "GB/s", "frac": kd["achieved_GBs"] / PEAK_HBM_GBS,
                     "traffic": mb * args.n * args.n if mb else None,
-                    "traffic_bytes_per_cell": mb, "avg_us": kd["avg_us"], "bytes_per_cell": kd["bytes_per_cell"]}
+                    "traffic_bytes_per_cell": mb, "avg_us": kd["avg_us"], "bytes_per_cell": kd["bytes_per_cell"],
+                    "solve_breakdown_ms": breakdown,
+                    "largest_family": next(iter(breakdown), None)}
     fp64 = paths.get("fp64")
 
+    configs = {}
+    if not args.no_configs:
+        for name, fn in (("c2_levelset_4096", config_c2), ("c3_seeded_2048", config_c3),
+                         ("c5_rising_bubble_1024x2048", config_c5), ("fast_poisson_2048", config_fast_poisson)):
+            print(f"# config {name} ...", file=sys.stderr, flush=True)
+            configs[name] = fn(A)
+            print(f"# {name}: {json.dumps(configs[name])[:300]}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu or not fp64 else cpu_baseline(args.n, fp64["pcg_iters_per_step"])
     line = {
         "metric": METRIC, "value": best["ms_per_step"], "unit": "ms/step", "n_gpus": 1,
@@ -369,7 +497,7 @@ def main():
         "pcg_iters": {k: v["pcg_iters_per_step"] for k, v in paths.items()},
         "learned_network": training,
         "learned_zero_network": ablation,
-        "roofline": roofline, "kernels": kernels, "cpu_baseline": cpu,
+        "roofline": roofline, "kernels": kernels, "cpu_baseline": cpu, "configs": configs,
     }
     print(json.dumps(line))
 
