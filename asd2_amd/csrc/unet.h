@@ -152,4 +152,8 @@ struct asd2_unet {
     float* tacy = nullptr;
     float* tdc = nullptr;
     double* partials = nullptr;
+    // per-block partial sums (weight, bias and output-layer gradients, training sums), added in
+    // block order by a second kernel so that every gradient is reproducible bit for bit
+    float* gpart = nullptr;
+    int64_t gpart_n = 0;
 };

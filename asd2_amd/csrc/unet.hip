@@ -305,12 +305,22 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(Src in, const uint16_t* __re
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int co = 16 * mt + 4 * g + i;
-            atomicAdd(&dw[(static_cast<int64_t>(tap) * COUT + co) * CIN + ci], acc[s][i]);
+            dw[((static_cast<int64_t>(blockIdx.x) * 9 + tap) * COUT + co) * CIN + ci] = acc[s][i];
         }
     }
 }
 
-// db[co] += sum_p dpre[p][co]
+// dst[k] += sum over blocks b (in order) of part[b stride + k], k < count
+__global__ __launch_bounds__(256) void k_sum_blocks(const float* __restrict__ part, int nblk, int64_t count,
+                                                    int64_t stride, float* __restrict__ dst) {
+    for (int64_t k = blockIdx.x * 256LL + threadIdx.x; k < count; k += static_cast<int64_t>(gridDim.x) * 256) {
+        float s = 0.f;
+        for (int b = 0; b < nblk; b++) s += part[static_cast<int64_t>(b) * stride + k];
+        dst[k] += s;
+    }
+}
+
+// part[block][co] = sum over the block's pixels of dpre[p][co]
 template <int COUT>
 __global__ __launch_bounds__(256) void k_bias_grad(const uint16_t* __restrict__ dpre, float* __restrict__ db, int64_t P) {
     constexpr int R = 256 / COUT;  // pixel lanes per channel
@@ -324,7 +334,7 @@ __global__ __launch_bounds__(256) void k_bias_grad(const uint16_t* __restrict__ 
     if (r == 0) {
         float t = 0.f;
         for (int k = 0; k < R; k++) t += sh[k * COUT + c];
-        atomicAdd(&db[c], t);
+        db[static_cast<int64_t>(blockIdx.x) * COUT + c] = t;
     }
 }
 
@@ -427,7 +437,7 @@ __global__ __launch_bounds__(256) void k_out_fwd(const uint16_t* __restrict__ d0
     }
 }
 
-// dd0[p][c] = dy[p] w[c] (m0 > 0); gw[c] += sum_p dy d0[p][c]; gb += sum_p dy (gb != nullptr);
+// dd0[p][c] = dy[p] w[c] (m0 > 0); part[block] = (sum_p dy d0[p][c] for c < 16, sum_p dy);
 // m0 = d0 for the ReLU pass, the ReLU pass's activation for the linear pass
 __global__ __launch_bounds__(256) void k_out_bwd(const uint16_t* __restrict__ d0, const uint16_t* __restrict__ m0,
                                                  const float* __restrict__ w, const float* __restrict__ dy,
@@ -464,10 +474,11 @@ __global__ __launch_bounds__(256) void k_out_bwd(const uint16_t* __restrict__ d0
         if (lane == 0) sh[k][wv] = v;
     }
     __syncthreads();
-    if (threadIdx.x < 16 || (threadIdx.x == 16 && gb)) {
+    if (threadIdx.x < 17) {
         const float t = sh[threadIdx.x][0] + sh[threadIdx.x][1] + sh[threadIdx.x][2] + sh[threadIdx.x][3];
-        atomicAdd(threadIdx.x < 16 ? &gw[threadIdx.x] : gb, t);
+        gw[static_cast<int64_t>(blockIdx.x) * 17 + threadIdx.x] = t;
     }
+    (void)gb;
 }
 
 // ------------------------------------------------------------------ weights and Adam
@@ -504,9 +515,36 @@ __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float
 // 2: sum (g / dc)^2, 3: y.g, 4: y.A_c y.
 constexpr int TS = 6;
 
-__device__ __forceinline__ void wave_atomic(float v, float* dst) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(dst, v);
+// the block's sums of NV per-thread values (wave shuffles, then the four waves in order) into
+// sp[((b nblk + block) TS + s0 + k)]; k_tr_finish adds the blocks in order
+template <int NV>
+__device__ __forceinline__ void block_sums(const float (&v)[NV], float* __restrict__ sp, int b, int s0) {
+    __shared__ float sh[NV][4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        float x = v[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if (lane == 0) sh[k][wv] = x;
+    }
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < NV) {
+        const int k = threadIdx.x;
+        sp[(static_cast<int64_t>(b) * gridDim.x + blockIdx.x) * TS + s0 + k] = sh[k][0] + sh[k][1] + sh[k][2] + sh[k][3];
+    }
+}
+
+// sums[TS b + s] = sum over the nblk blocks (in order) of the partials of slots s0 .. s0 + nv - 1;
+// slot 0 = energy[b] instead when energy is given
+__global__ __launch_bounds__(64) void k_tr_finish(const float* __restrict__ sp, int nblk, int batch, int s0, int nv,
+                                                  const float* __restrict__ energy, float* __restrict__ sums) {
+    for (int t = threadIdx.x; t < batch * nv; t += 64) {
+        const int b = t / nv, s = s0 + t % nv;
+        float v = 0.f;
+        for (int k = 0; k < nblk; k++) v += sp[(static_cast<int64_t>(b) * nblk + k) * TS + s];
+        if (energy && s == 0) v = energy[b];
+        sums[TS * b + s] = v;
+    }
 }
 
 // coarse operators of the batch (blockIdx.y = sample)
@@ -523,51 +561,57 @@ __global__ __launch_bounds__(256) void k_tr_coarse_op(const float* __restrict__ 
     }
 }
 
-// g[b] += P^T (r - A D^-1 r) (g zeroed before), sums 0 and 1
+// g[b] = P^T (r - A D^-1 r), one thread per coarse cell summing its c x c fine cells; block
+// partials of sums 0 (r.D^-1 r) and 1 ((D^-1 r).(r - A D^-1 r))
 __global__ __launch_bounds__(256) void k_tr_restrict(const float* __restrict__ r, const float* __restrict__ dinv,
                                                      const float* __restrict__ ax, const float* __restrict__ ay,
                                                      const float* __restrict__ dg, int64_t stride, int nx, int ny,
-                                                     int c, float* __restrict__ g, float* __restrict__ sums) {
+                                                     int c, float* __restrict__ g, float* __restrict__ sp) {
     const int b = blockIdx.y, sy = ny + 6, Wc = ny / c;
-    const int64_t P = static_cast<int64_t>(nx) * ny;
+    const int64_t Pc = static_cast<int64_t>(nx / c) * Wc;
     const float* rb = r + b * stride;
     const float* db = dinv + b * stride;
     const float* axb = ax + b * stride;
     const float* ayb = ay + b * stride;
     const float* dgb = dg + b * stride;
-    float* gb = g + b * static_cast<int64_t>(nx / c) * Wc;
-    float a0 = 0.f, a1 = 0.f;
-    for (int64_t p = blockIdx.x * 256LL + threadIdx.x; p < P; p += static_cast<int64_t>(gridDim.x) * 256) {
-        const int i = static_cast<int>(p / ny), j = static_cast<int>(p - static_cast<int64_t>(i) * ny);
-        const int64_t k = static_cast<int64_t>(i + 3) * sy + j + 3;
-        auto zj = [&](int64_t m) { return db[m] * rb[m]; };
-        const float z0 = zj(k);
-        float az = dgb[k] * z0;
-        if (i > 0) az += axb[k] * zj(k - sy);
-        if (i + 1 < nx) az += axb[k + sy] * zj(k + sy);
-        if (j > 0) az += ayb[k] * zj(k - 1);
-        if (j + 1 < ny) az += ayb[k + 1] * zj(k + 1);
-        const float w = rb[k] - az;
-        atomicAdd(&gb[static_cast<int64_t>(i / c) * Wc + j / c], w);
-        a0 += z0 * rb[k];
-        a1 += z0 * w;
+    float* gb = g + b * Pc;
+    float a[2] = {0.f, 0.f};
+    for (int64_t p = blockIdx.x * 256LL + threadIdx.x; p < Pc; p += static_cast<int64_t>(gridDim.x) * 256) {
+        const int I = static_cast<int>(p / Wc), J = static_cast<int>(p - static_cast<int64_t>(I) * Wc);
+        float gs = 0.f;
+        for (int u = 0; u < c; u++)
+            for (int v = 0; v < c; v++) {
+                const int i = I * c + u, j = J * c + v;
+                const int64_t k = static_cast<int64_t>(i + 3) * sy + j + 3;
+                auto zj = [&](int64_t m) { return db[m] * rb[m]; };
+                const float z0 = zj(k);
+                float az = dgb[k] * z0;
+                if (i > 0) az += axb[k] * zj(k - sy);
+                if (i + 1 < nx) az += axb[k + sy] * zj(k + sy);
+                if (j > 0) az += ayb[k] * zj(k - 1);
+                if (j + 1 < ny) az += ayb[k + 1] * zj(k + 1);
+                const float w = rb[k] - az;
+                gs += w;
+                a[0] += z0 * rb[k];
+                a[1] += z0 * w;
+            }
+        gb[p] = gs;
     }
-    wave_atomic(a0, &sums[TS * b + 0]);
-    wave_atomic(a1, &sums[TS * b + 1]);
+    block_sums<2>(a, sp, b, 0);
 }
 
 // sums 2 = sum over the coarse cells of (g / dc)^2
 __global__ __launch_bounds__(256) void k_tr_usum(const float* __restrict__ g, const float* __restrict__ dc, int Hc,
-                                                 int Wc, float* __restrict__ sums) {
+                                                 int Wc, float* __restrict__ sp) {
     const int b = blockIdx.y;
     const int64_t Pc = static_cast<int64_t>(Hc) * Wc, cs = static_cast<int64_t>(Hc + 1) * (Wc + 1);
-    float a = 0.f;
+    float a[1] = {0.f};
     for (int64_t p = blockIdx.x * 256LL + threadIdx.x; p < Pc; p += static_cast<int64_t>(gridDim.x) * 256) {
         const int I = static_cast<int>(p / Wc), J = static_cast<int>(p - static_cast<int64_t>(I) * Wc);
         const float u = g[b * Pc + p] / dc[b * cs + I * (Wc + 1) + J];
-        a += u * u;
+        a[0] += u * u;
     }
-    wave_atomic(a, &sums[TS * b + 2]);
+    block_sums<1>(a, sp, b, 2);
 }
 
 // f[b][p] = (0, log coarse W-face, S-face coefficient, log coarse diagonal): the ReLU pass;
@@ -594,18 +638,17 @@ __global__ __launch_bounds__(256) void k_tr_features(const float* __restrict__ g
 __global__ __launch_bounds__(256) void k_tr_gamma(const float* __restrict__ y, const float* __restrict__ g,
                                                   const float* __restrict__ acx, const float* __restrict__ acy,
                                                   const float* __restrict__ dc, int Hc, int Wc,
-                                                  float* __restrict__ sums) {
+                                                  float* __restrict__ sp) {
     const int b = blockIdx.y;
     const int64_t Pc = static_cast<int64_t>(Hc) * Wc, cs = static_cast<int64_t>(Hc + 1) * (Wc + 1);
-    float a0 = 0.f, a1 = 0.f;
+    float a[2] = {0.f, 0.f};
     for (int64_t p = blockIdx.x * 256LL + threadIdx.x; p < Pc; p += static_cast<int64_t>(gridDim.x) * 256) {
         const int I = static_cast<int>(p / Wc), J = static_cast<int>(p - static_cast<int64_t>(I) * Wc);
         const float yv = y[b * Pc + p];
-        a0 += yv * g[b * Pc + p];
-        a1 += yv * coarse_apply(acx + b * cs, acy + b * cs, dc + b * cs, y + b * Pc, Hc, Wc, I, J);
+        a[0] += yv * g[b * Pc + p];
+        a[1] += yv * coarse_apply(acx + b * cs, acy + b * cs, dc + b * cs, y + b * Pc, Hc, Wc, I, J);
     }
-    wave_atomic(a0, &sums[TS * b + 3]);
-    wave_atomic(a1, &sums[TS * b + 4]);
+    block_sums<2>(a, sp, b, 3);
 }
 
 __device__ __forceinline__ float gamma_from(float yg, float yay) {
@@ -630,22 +673,20 @@ __global__ __launch_bounds__(256) void k_tr_dy(const float* __restrict__ y, cons
     }
 }
 
-// coarse training samples: sums 0 = the loss normaliser (energy[b] = g.A_c^-1 g when given, else
-// g.D_c^-1 g), 1 = 0, 2 = sum (g / dc)^2
+// coarse training samples: block partials of sums 0 = g.D_c^-1 g (k_tr_finish replaces it by
+// energy[b] = g.A_c^-1 g when given: the loss normaliser), 1 = 0, 2 = sum (g / dc)^2
 __global__ __launch_bounds__(256) void k_tr_gnorm(const float* __restrict__ g, const float* __restrict__ dc, int Hc,
-                                                  int Wc, const float* __restrict__ energy, float* __restrict__ sums) {
+                                                  int Wc, float* __restrict__ sp) {
     const int b = blockIdx.y;
-    if (energy && blockIdx.x == 0 && threadIdx.x == 0) sums[TS * b + 0] = energy[b];
     const int64_t Pc = static_cast<int64_t>(Hc) * Wc, cs = static_cast<int64_t>(Hc + 1) * (Wc + 1);
-    float a0 = 0.f, a2 = 0.f;
+    float a[3] = {0.f, 0.f, 0.f};
     for (int64_t p = blockIdx.x * 256LL + threadIdx.x; p < Pc; p += static_cast<int64_t>(gridDim.x) * 256) {
         const int I = static_cast<int>(p / Wc), J = static_cast<int>(p - static_cast<int64_t>(I) * Wc);
         const float gv = g[b * Pc + p], d = dc[b * cs + I * (Wc + 1) + J];
-        a0 += gv * gv / d;
-        a2 += (gv / d) * (gv / d);
+        a[0] += gv * gv / d;
+        a[2] += (gv / d) * (gv / d);
     }
-    if (!energy) wave_atomic(a0, &sums[TS * b + 0]);
-    wave_atomic(a2, &sums[TS * b + 2]);
+    block_sums<3>(a, sp, b, 0);
 }
 
 // FP32 features [P][4] -> feat (channel 0 zeroed) and flin (channel 0 only), BF16
@@ -685,19 +726,30 @@ void launch_conv(hipStream_t s, const Src& in, const uint16_t* w, const float* b
     hipLaunchKernelGGL((k_conv3x3<CIN, COUT, EPI>), dim3(blocks), dim3(256), 0, s, in, w, bias, out, mask, nimg, H, W);
 }
 
+// at most WG_BLOCKS pixel ranges per tap: the partial slab holds WG_BLOCKS x 9 x COUT x CIN sums
+constexpr int WG_BLOCKS = 32, BG_BLOCKS = 2048;
+
+void sum_blocks(hipStream_t s, const float* part, int nblk, int64_t count, float* dst, int64_t stride = -1) {
+    hipLaunchKernelGGL(k_sum_blocks, dim3(blocks_for(count, 256, 4096)), dim3(256), 0, s, part, nblk, count,
+                       stride < 0 ? count : stride, dst);
+}
+
 template <int CIN, int COUT>
-void launch_wgrad(hipStream_t s, const Src& in, const uint16_t* dpre, float* dw, int nimg, int H, int W) {
+void launch_wgrad(hipStream_t s, float* part, const Src& in, const uint16_t* dpre, float* dw, int nimg, int H, int W) {
     const int64_t P = static_cast<int64_t>(nimg) * H * W;
-    int64_t per = std::max<int64_t>(256, ((P + 255) / 256 + 31) / 32 * 32);
+    const int64_t per = std::max<int64_t>(256, ((P + WG_BLOCKS - 1) / WG_BLOCKS + 31) / 32 * 32);
     const int blocks = static_cast<int>((P + per - 1) / per);
-    hipLaunchKernelGGL((k_conv_wgrad<CIN, COUT>), dim3(blocks, 9), dim3(256), 0, s, in, dpre, dw, nimg, H, W,
+    hipLaunchKernelGGL((k_conv_wgrad<CIN, COUT>), dim3(blocks, 9), dim3(256), 0, s, in, dpre, part, nimg, H, W,
                        static_cast<int>(per));
+    sum_blocks(s, part, blocks, 9LL * COUT * CIN, dw);
 }
 
 template <int COUT>
-void launch_bgrad(hipStream_t s, const uint16_t* dpre, float* db, int64_t P) {
+void launch_bgrad(hipStream_t s, float* part, const uint16_t* dpre, float* db, int64_t P) {
     if (P == 0) return;
-    hipLaunchKernelGGL(k_bias_grad<COUT>, dim3(blocks_for(P, 256 / COUT * 64, 2048)), dim3(256), 0, s, dpre, db, P);
+    const int blocks = blocks_for(P, 256 / COUT * 64, BG_BLOCKS);
+    hipLaunchKernelGGL(k_bias_grad<COUT>, dim3(blocks), dim3(256), 0, s, dpre, part, P);
+    sum_blocks(s, part, blocks, COUT, db);
 }
 
 // forward conv l (template dispatch by layer)
@@ -777,25 +829,26 @@ void conv_dgrad(asd2_unet* N, int l, const uint16_t* dpre, uint16_t* out, const 
 }
 
 void conv_wgrad(asd2_unet* N, int l, const Src& in, const uint16_t* dpre, int nimg, int H, int W, bool bias) {
+    float* part = N->gpart;
     float* dw = N->grad + N->L.w[l];
     float* db = N->grad + N->L.b[l];
     hipStream_t s = N->stream;
     const int64_t P = bias ? static_cast<int64_t>(nimg) * H * W : 0;  // P = 0: no bias gradient
     switch (l) {
-        case 0: launch_wgrad<4, 16>(s, in, dpre, dw, nimg, H, W); launch_bgrad<16>(s, dpre, db, P); break;
-        case 1: launch_wgrad<16, 16>(s, in, dpre, dw, nimg, H, W); launch_bgrad<16>(s, dpre, db, P); break;
-        case 2: launch_wgrad<16, 32>(s, in, dpre, dw, nimg, H, W); launch_bgrad<32>(s, dpre, db, P); break;
-        case 3: launch_wgrad<32, 32>(s, in, dpre, dw, nimg, H, W); launch_bgrad<32>(s, dpre, db, P); break;
-        case 4: launch_wgrad<32, 64>(s, in, dpre, dw, nimg, H, W); launch_bgrad<64>(s, dpre, db, P); break;
-        case 5: launch_wgrad<64, 64>(s, in, dpre, dw, nimg, H, W); launch_bgrad<64>(s, dpre, db, P); break;
-        case 6: launch_wgrad<64, 128>(s, in, dpre, dw, nimg, H, W); launch_bgrad<128>(s, dpre, db, P); break;
-        case 7: launch_wgrad<128, 128>(s, in, dpre, dw, nimg, H, W); launch_bgrad<128>(s, dpre, db, P); break;
-        case 8: launch_wgrad<192, 64>(s, in, dpre, dw, nimg, H, W); launch_bgrad<64>(s, dpre, db, P); break;
-        case 9: launch_wgrad<64, 64>(s, in, dpre, dw, nimg, H, W); launch_bgrad<64>(s, dpre, db, P); break;
-        case 10: launch_wgrad<96, 32>(s, in, dpre, dw, nimg, H, W); launch_bgrad<32>(s, dpre, db, P); break;
-        case 11: launch_wgrad<32, 32>(s, in, dpre, dw, nimg, H, W); launch_bgrad<32>(s, dpre, db, P); break;
-        case 12: launch_wgrad<48, 16>(s, in, dpre, dw, nimg, H, W); launch_bgrad<16>(s, dpre, db, P); break;
-        case 13: launch_wgrad<16, 16>(s, in, dpre, dw, nimg, H, W); launch_bgrad<16>(s, dpre, db, P); break;
+        case 0: launch_wgrad<4, 16>(s, part, in, dpre, dw, nimg, H, W); launch_bgrad<16>(s, part, dpre, db, P); break;
+        case 1: launch_wgrad<16, 16>(s, part, in, dpre, dw, nimg, H, W); launch_bgrad<16>(s, part, dpre, db, P); break;
+        case 2: launch_wgrad<16, 32>(s, part, in, dpre, dw, nimg, H, W); launch_bgrad<32>(s, part, dpre, db, P); break;
+        case 3: launch_wgrad<32, 32>(s, part, in, dpre, dw, nimg, H, W); launch_bgrad<32>(s, part, dpre, db, P); break;
+        case 4: launch_wgrad<32, 64>(s, part, in, dpre, dw, nimg, H, W); launch_bgrad<64>(s, part, dpre, db, P); break;
+        case 5: launch_wgrad<64, 64>(s, part, in, dpre, dw, nimg, H, W); launch_bgrad<64>(s, part, dpre, db, P); break;
+        case 6: launch_wgrad<64, 128>(s, part, in, dpre, dw, nimg, H, W); launch_bgrad<128>(s, part, dpre, db, P); break;
+        case 7: launch_wgrad<128, 128>(s, part, in, dpre, dw, nimg, H, W); launch_bgrad<128>(s, part, dpre, db, P); break;
+        case 8: launch_wgrad<192, 64>(s, part, in, dpre, dw, nimg, H, W); launch_bgrad<64>(s, part, dpre, db, P); break;
+        case 9: launch_wgrad<64, 64>(s, part, in, dpre, dw, nimg, H, W); launch_bgrad<64>(s, part, dpre, db, P); break;
+        case 10: launch_wgrad<96, 32>(s, part, in, dpre, dw, nimg, H, W); launch_bgrad<32>(s, part, dpre, db, P); break;
+        case 11: launch_wgrad<32, 32>(s, part, in, dpre, dw, nimg, H, W); launch_bgrad<32>(s, part, dpre, db, P); break;
+        case 12: launch_wgrad<48, 16>(s, part, in, dpre, dw, nimg, H, W); launch_bgrad<16>(s, part, dpre, db, P); break;
+        case 13: launch_wgrad<16, 16>(s, part, in, dpre, dw, nimg, H, W); launch_bgrad<16>(s, part, dpre, db, P); break;
         default: break;
     }
 }
@@ -806,6 +859,16 @@ int check_net(const asd2_unet* N) {
         return ASD2_ERR_ARG;
     }
     return ASD2_OK;
+}
+
+// floats of the partial slab: the largest weight gradient (WG_BLOCKS x 9 x COUT x CIN), the bias
+// and output-layer gradients (BG_BLOCKS x 128, 2048 x 17), the training sums (batch x 1024 x TS)
+int64_t gpart_size(int max_batch) {
+    int64_t n = static_cast<int64_t>(BG_BLOCKS) * 128;
+    for (int l = 0; l < UNET_NCONV; l++)
+        n = std::max<int64_t>(n, static_cast<int64_t>(WG_BLOCKS) * 9 * UNET_CONVS[l].cout *
+                                     (UNET_CONVS[l].cin_a + UNET_CONVS[l].cin_b));
+    return std::max<int64_t>(n, static_cast<int64_t>(max_batch) * 1024 * TS);
 }
 
 template <typename T>
@@ -935,8 +998,12 @@ int unet_backward(asd2_unet* N, int nimg, bool linear) {
     uint16_t* d1 = N->g0 + P0 * 32;        // [P0][16]
     uint16_t* d0 = N->g0 + P0 * 48;        // [P0][16]
 
-    hipLaunchKernelGGL(k_out_bwd, dim3(blocks_for(P0, 256, 2048)), dim3(256), 0, s, a[13], m[13], N->w32 + N->L.out_w,
-                       N->dy, d13, N->grad + N->L.out_w, bias ? N->grad + N->L.out_b : static_cast<float*>(nullptr), P0);
+    {  // output layer: 16 weights and the bias (stored after them) from block partials
+        const int nb = blocks_for(P0, 256, 2048);
+        hipLaunchKernelGGL(k_out_bwd, dim3(nb), dim3(256), 0, s, a[13], m[13], N->w32 + N->L.out_w, N->dy, d13,
+                           N->gpart, static_cast<float*>(nullptr), P0);
+        sum_blocks(s, N->gpart, nb, bias ? 17 : 16, N->grad + N->L.out_w, 17);
+    }
     // dec0
     conv_wgrad(N, 13, Src{a[12], 16, nullptr, 0}, d13, nimg, H, W, bias);
     conv_dgrad(N, 13, d13, d12, m[12], nimg, H, W);
@@ -1021,6 +1088,7 @@ int asd2_unet_create(int nx, int ny, int coarsen, int max_batch, unsigned long l
         (st = dalloc(&N->adam_v, T)) || (st = dalloc(&N->wf, T)) || (st = dalloc(&N->wb, T)) ||
         (st = dalloc(&N->feat, P0 * 4)) || (st = dalloc(&N->y, P0)) || (st = dalloc(&N->dy, P0)) ||
         (st = dalloc(&N->scratch, 8LL * max_batch + 64)) || (st = dalloc(&N->partials, 4096)) ||
+        (st = dalloc(&N->gpart, N->gpart_n = gpart_size(max_batch))) ||
         (st = dalloc(&N->tg, P0)) || (st = dalloc(&N->tacx, CS)) || (st = dalloc(&N->tacy, CS)) ||
         (st = dalloc(&N->tdc, CS)))
         return fail(st);
@@ -1074,7 +1142,7 @@ int asd2_unet_destroy(asd2_unet* N) {
     if (N->own_stream) (void)hipStreamSynchronize(N->own_stream);
     void* bufs[] = {N->w32, N->grad, N->adam_m, N->adam_v, N->wf, N->wb, N->feat, N->y, N->dy, N->scratch,
                     N->partials, N->pooled[0], N->pooled[1], N->pooled[2], N->g0, N->g1, N->g2, N->tg, N->tacx,
-                    N->tacy, N->tdc, N->flin, N->linpooled[0], N->linpooled[1], N->linpooled[2]};
+                    N->tacy, N->tdc, N->flin, N->linpooled[0], N->linpooled[1], N->linpooled[2], N->gpart};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     for (auto* p : N->act)
@@ -1185,15 +1253,20 @@ int asd2_unet_train_step_coarse(asd2_unet* N, int batch, const float* d_g, const
     const int64_t Pc = static_cast<int64_t>(Hc) * Wc;
     hipStream_t s = N->stream;
     float* sums = N->scratch;  // [TS B]
+    float* sp = N->gpart;      // block partials of the sums
     ASD2_HIP(hipMemsetAsync(sums, 0, TS * batch * sizeof(float), s));
     ASD2_HIP(hipMemsetAsync(N->grad, 0, N->L.total * sizeof(float), s));
     const dim3 gc(blocks_for(Pc, 256, 1024), batch);
-    hipLaunchKernelGGL(k_tr_gnorm, gc, dim3(256), 0, s, d_g, d_dc, Hc, Wc, d_energy, sums);
+    const int nbc = static_cast<int>(gc.x);
+    hipLaunchKernelGGL(k_tr_gnorm, gc, dim3(256), 0, s, d_g, d_dc, Hc, Wc, sp);
+    hipLaunchKernelGGL(k_tr_finish, dim3(1), dim3(64), 0, s, sp, nbc, batch, 0, 3, d_energy, sums);
     hipLaunchKernelGGL(k_tr_features, gc, dim3(256), 0, s, d_g, d_acx, d_acy, d_dc, Hc, Wc, static_cast<float>(h2x),
                        static_cast<float>(h2y), sums, N->feat, N->flin);
     ASD2_TRY(unet_forward(N, batch));
     ASD2_TRY(unet_forward_linear(N, batch));
-    hipLaunchKernelGGL(k_tr_gamma, gc, dim3(256), 0, s, N->y, d_g, d_acx, d_acy, d_dc, Hc, Wc, sums);
+    hipLaunchKernelGGL(k_tr_gamma, gc, dim3(256), 0, s, N->y, d_g, d_acx, d_acy, d_dc, Hc, Wc, sp);
+    hipLaunchKernelGGL(k_tr_finish, dim3(1), dim3(64), 0, s, sp, nbc, batch, 3, 2, static_cast<const float*>(nullptr),
+                       sums);
     hipLaunchKernelGGL(k_tr_dy, gc, dim3(256), 0, s, N->y, d_g, d_acx, d_acy, d_dc, Hc, Wc, sums, N->dy, batch);
     ASD2_TRY(unet_backward(N, batch, true));
     N->adam_t += 1;
@@ -1233,20 +1306,26 @@ int asd2_unet_train_step(asd2_unet* N, int batch, const float* d_r, const float*
     const int64_t cs = static_cast<int64_t>(Hc + 1) * (Wc + 1);
     hipStream_t s = N->stream;
     float* sums = N->scratch;  // [TS B]
-    ASD2_HIP(hipMemsetAsync(N->tg, 0, batch * Pc * sizeof(float), s));
+    float* sp = N->gpart;      // block partials of the sums
     ASD2_HIP(hipMemsetAsync(sums, 0, TS * batch * sizeof(float), s));
     ASD2_HIP(hipMemsetAsync(N->grad, 0, N->L.total * sizeof(float), s));
     hipLaunchKernelGGL(k_tr_coarse_op, dim3(blocks_for(cs, 256, 1024), batch), dim3(256), 0, s, d_ax, d_ay, d_dg,
                        stride, ny + 6, c, Hc, Wc, N->tacx, N->tacy, N->tdc);
-    hipLaunchKernelGGL(k_tr_restrict, dim3(blocks_for(Pf, 256, 2048), batch), dim3(256), 0, s, d_r, d_dinv, d_ax,
-                       d_ay, d_dg, stride, nx, ny, c, N->tg, sums);
     const dim3 gc(blocks_for(Pc, 256, 1024), batch);
-    hipLaunchKernelGGL(k_tr_usum, gc, dim3(256), 0, s, N->tg, N->tdc, Hc, Wc, sums);
+    const int nbc = static_cast<int>(gc.x);
+    hipLaunchKernelGGL(k_tr_restrict, gc, dim3(256), 0, s, d_r, d_dinv, d_ax, d_ay, d_dg, stride, nx, ny, c, N->tg, sp);
+    hipLaunchKernelGGL(k_tr_finish, dim3(1), dim3(64), 0, s, sp, nbc, batch, 0, 2, static_cast<const float*>(nullptr),
+                       sums);
+    hipLaunchKernelGGL(k_tr_usum, gc, dim3(256), 0, s, N->tg, N->tdc, Hc, Wc, sp);
+    hipLaunchKernelGGL(k_tr_finish, dim3(1), dim3(64), 0, s, sp, nbc, batch, 2, 1, static_cast<const float*>(nullptr),
+                       sums);
     hipLaunchKernelGGL(k_tr_features, gc, dim3(256), 0, s, N->tg, N->tacx, N->tacy, N->tdc, Hc, Wc,
                        static_cast<float>(dx * dx / c), static_cast<float>(dy * dy / c), sums, N->feat, N->flin);
     ASD2_TRY(unet_forward(N, batch));
     ASD2_TRY(unet_forward_linear(N, batch));
-    hipLaunchKernelGGL(k_tr_gamma, gc, dim3(256), 0, s, N->y, N->tg, N->tacx, N->tacy, N->tdc, Hc, Wc, sums);
+    hipLaunchKernelGGL(k_tr_gamma, gc, dim3(256), 0, s, N->y, N->tg, N->tacx, N->tacy, N->tdc, Hc, Wc, sp);
+    hipLaunchKernelGGL(k_tr_finish, dim3(1), dim3(64), 0, s, sp, nbc, batch, 3, 2, static_cast<const float*>(nullptr),
+                       sums);
     hipLaunchKernelGGL(k_tr_dy, gc, dim3(256), 0, s, N->y, N->tg, N->tacx, N->tacy, N->tdc, Hc, Wc, sums, N->dy,
                        batch);
     ASD2_TRY(unet_backward(N, batch, true));

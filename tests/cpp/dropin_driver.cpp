@@ -12,6 +12,9 @@
 //   dropin_driver ls <dir> <nx> <ny>        LevelSetSolver2D methods on <dir>/phi.bin, u.bin, v.bin
 //   dropin_driver csr <dir> <nx> <ny> <all_neumann 0|1>
 //                                           PoissonSolver2D CG + BiCGStab on a CSR system
+//   dropin_driver outres <dir> <nx> <ny>     MACSolver2D::OutRes (Tecplot ASCII) of <dir>/u.bin, v.bin,
+//                                           ps.bin, ls.bin at iterations 0 and 1 into <dir>/vel_ASCII.plt
+//                                           and <dir>/div_ASCII.plt (grid of 1 x 1.5, base (0.25, -0.5))
 //   dropin_driver fast <dir> <nx> <ny> <sides>
 //                                           PoissonSolver2D::MKL_2FUniform_2D on <dir>/rhs.bin,
 //                                           ps.bin, consts.bin (W E S N) with pressure sides
@@ -222,6 +225,23 @@ int run_csr(const std::string& dir, int nx, int ny, bool all_neumann) {
     return 0;
 }
 
+int run_outres(const std::string& dir, int nx, int ny) {
+    auto M = std::make_unique<MACSolver2D>(1000.0, 1.0, 1.0, 0.01, 0.0, GAXISENUM2D::Y, 1.0, 1.0, 0.0, nx, ny, 0.25,
+                                           -0.5, 1.0, 1.5, TIMEORDERENUM::RK3, 0.3, 5.0, 20, 1, 3, false);
+    M->SetBC_U_2D("dirichlet", "dirichlet", "dirichlet", "dirichlet");
+    M->SetBC_V_2D("dirichlet", "dirichlet", "dirichlet", "dirichlet");
+    M->SetBC_P_2D("neumann", "neumann", "neumann", "neumann");
+    M->SetPLTType(PLTTYPE::ASCII);
+    M->AllocateVariables();
+    const auto u = load<double>(dir + "/u.bin"), v = load<double>(dir + "/v.bin");
+    const auto ps = load<double>(dir + "/ps.bin"), ls = load<double>(dir + "/ls.bin");
+    M->m_iter = 0;
+    M->OutRes(0, 0.0, dir + "/vel", dir + "/div", u, v, ps, ls);
+    M->m_iter = 1;
+    M->OutRes(7, 0.125, dir + "/vel", dir + "/div", u, v, ps, ls);
+    return 0;
+}
+
 int run_fast(const std::string& dir, int nx, int ny, const std::string& sides) {
     const double lenX = 1.0, lenY = static_cast<double>(ny) / nx, dx = lenX / nx, dy = lenY / ny;
     const auto rhs = load<double>(dir + "/rhs.bin");
@@ -267,6 +287,7 @@ int main(int argc, char** argv) {
     if (cmd == "step" && argc == 6) return run_step(dir, std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]));
     if (cmd == "ls" && argc == 5) return run_ls(dir, std::atoi(argv[3]), std::atoi(argv[4]));
     if (cmd == "csr" && argc == 6) return run_csr(dir, std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]) != 0);
+    if (cmd == "outres" && argc == 5) return run_outres(dir, std::atoi(argv[3]), std::atoi(argv[4]));
     if (cmd == "fast" && argc == 6 && std::string(argv[5]).size() == 4)
         return run_fast(dir, std::atoi(argv[3]), std::atoi(argv[4]), argv[5]);
     std::cerr << "bad arguments" << std::endl;

@@ -214,3 +214,41 @@ def test_poisson_class_fast(tmp_path, sides):
     np.testing.assert_array_equal(got[2, 3:ny + 3], got[3, 3:ny + 3])  # Neumann ghost of the W side
     p = _load(tmp_path, "mac_p.bin", shp)
     np.testing.assert_allclose(p[inner], got[inner] / 0.01, rtol=1e-15, atol=0)
+
+
+def test_outres_tecplot_ascii(oracle, tmp_path):
+    """MACSolver2D::OutRes (reference src/mac2d.cpp:1990-2061) with PLTTYPE::ASCII: the title and
+    variable lines written at iteration 0, then one zone per call (cell-centre coordinates, face
+    velocities averaged to the centre, level set and pressure; for the second file the level set,
+    the divergence and the pressure), x fastest, values as C++ streams print a double by default
+    (6 significant digits); compared line by line with the text expected from the oracle's fields"""
+    nx, ny, g = 12, 9, 3
+    rng = np.random.default_rng(33)
+    shp = (nx + 2 * g, ny + 2 * g)
+    u, v, ps, ls = (rng.standard_normal(shp) * s for s in (0.1, 0.1, 1e-3, 0.3))
+    for name, a in (("u", u), ("v", v), ("ps", ps), ("ls", ls)):
+        a.tofile(tmp_path / f"{name}.bin")
+    _run("outres", tmp_path, nx, ny)
+    dx, dy, bx, by = 1.0 / nx, 1.5 / ny, 0.25, -0.5
+    gr = make_grid(nx, ny, dx, dy)
+    div = oracle.divergence(gr, u, v)
+
+    def num(x):
+        return f"{x:g}"
+
+    def zone(it, t):
+        return f'ZONE T="{it}", I={nx}, J={ny}, SOLUTIONTIME={num(t)}, STRANDID={it + 1}'
+
+    vel = ["TITLE = VEL", 'VARIABLES = "X", "Y", "U", "V", "LS", "PS" ']
+    dv = ["TITLE = DIV", 'VARIABLES = "X", "Y", "LS", "DIV", "PS" ']
+    for it, t in ((0, 0.0), (7, 0.125)):
+        vel.append(zone(it, t))
+        dv.append(zone(it, t))
+        for j in range(g, ny + g):
+            for i in range(g, nx + g):
+                x, y = bx + (i + 0.5 - g) * dx, by + (j + 0.5 - g) * dy
+                uc, vc = (u[i, j] + u[i + 1, j]) * 0.5, (v[i, j] + v[i, j + 1]) * 0.5
+                vel.append(",".join(num(a) for a in (x, y, uc, vc, ls[i, j], ps[i, j])))
+                dv.append(",".join(num(a) for a in (x, y, ls[i, j], div[i, j], ps[i, j])))
+    assert (tmp_path / "vel_ASCII.plt").read_text().splitlines() == vel
+    assert (tmp_path / "div_ASCII.plt").read_text().splitlines() == dv

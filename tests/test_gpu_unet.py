@@ -1,13 +1,16 @@
 """GPU network of the learned preconditioner (asd2_amd/csrc/unet.hip, krylov.hip) against the
 FP64 CPU oracle (oracle/unet.py) run with the same BF16 rounding points.
 
-Bars: forward output relative L2 <= 2e-2 and per-layer weight gradients relative L2 <= 5e-2
-(BF16 activations: an accumulation-order difference can flip a stored value by one BF16 ulp,
-0.4 %, and such flips propagate); the training step's energy loss within 1e-2 and its gradients
-within 1e-1 of the oracle's (the loss gradient adds FP32 sums of the coarse step to the BF16
-effects); training lowers the loss; the flexible PCG reaches an FP64 true residual of 1e-8 with
-an untrained network at every coarsening, equals Jacobi-PCG when the network's output is zero,
-and the trained network saves iterations on held-out systems (the ablation)."""
+Bars, about ten times the errors observed on these seeded inputs (the GPU sums every gradient in
+a fixed order, so the observed errors repeat): forward output relative L2 <= 1e-2 (observed 9e-8
+on one image and 1.3e-3 on the other, where one BF16 activation rounds the other way than in the
+oracle and the difference propagates), per-layer weight gradients <= 3e-6 (2.5e-7); the masked linear pass 1e-6 and its gradients
+1e-3 (one layer shows 9e-5 where a BF16 value rounds the other way than in the oracle; additivity
+holds to BF16 accuracy, 5e-3); the training step's energy loss within 1e-5 and its gradients within
+1e-4 (6.5e-6) of the oracle's; training lowers the loss and is bit-reproducible; the flexible PCG
+reaches an FP64 true residual of 1e-8 with an untrained network at every coarsening, equals
+Jacobi-PCG when the network's output is zero, and the trained network saves at least 5x the
+iterations on every held-out system at a fixed seed (the ablation; observed 9.2-15.8x)."""
 import numpy as np
 import pytest
 
@@ -34,7 +37,7 @@ def test_forward_matches_oracle():
         yr, _ = U.forward(p, f[b].astype(np.float64), bf=True)
         err = _rel(y[b], yr)
         print(f"image {b}: forward rel L2 {err:.2e}")
-        assert err <= 2e-2
+        assert err <= 1e-2
 
 
 def test_backward_matches_oracle():
@@ -56,7 +59,7 @@ def test_backward_matches_oracle():
     for k in sorted(ref):
         err = _rel(g[k], ref[k])
         print(f"{k:8s} rel L2 {err:.2e}")
-        assert err <= 5e-2, k
+        assert err <= 3e-6, k
 
 
 def test_linear_form_matches_oracle_and_is_linear():
@@ -74,7 +77,7 @@ def test_linear_form_matches_oracle_and_is_linear():
         yr, _ = U.network_output(p, f[b].astype(np.float64), bf=True)
         err = _rel(y[b], yr)
         print(f"image {b}: linear forward rel L2 {err:.2e}")
-        assert err <= 2e-2
+        assert err <= 1e-6
     f2 = f.copy()
     f2[:, :, :, 0] = rng.standard_normal((B, H, W))
     f12 = f.copy()
@@ -97,7 +100,7 @@ def test_linear_form_matches_oracle_and_is_linear():
             continue
         err = _rel(g[k], ref[k])
         print(f"{k:8s} linear grad rel L2 {err:.2e}")
-        assert err <= 5e-2, k
+        assert err <= 1e-3, k
 
 
 def _systems(oracle, n, count, seed0, snapshots=(0, 5, 20, 60)):
@@ -143,13 +146,13 @@ def test_training_loss_and_gradients_match_oracle(oracle):
         gref = {k: v / len(raw) for k, v in gg.items()} if gref is None else {k: gref[k] + gg[k] / len(raw)
                                                                                 for k in gg}
     print(f"first-step loss gpu {loss0:.6f} oracle {ref:.6f}")
-    assert abs(loss0 - ref) <= 1e-2 * abs(ref)
+    assert abs(loss0 - ref) <= 1e-5 * abs(ref)
     worst = 0.0
     for k in sorted(gref):
         err = _rel(g_gpu[k], gref[k])
         worst = max(worst, err)
         print(f"{k:8s} grad rel L2 {err:.2e}")
-    assert worst <= 1e-1
+    assert worst <= 1e-4
     losses = [loss0] + [net.train_step(B, *d, dx, dx, 1e-3) for _ in range(60)]
     print("loss", losses[0], losses[10], losses[-1])
     assert np.mean(losses[-5:]) < losses[0]
@@ -231,8 +234,7 @@ def test_trained_network_saves_iterations(oracle):
     from asd2_amd import learned as LN
     from oracle import make_grid
     n, c = 256, 8
-    # 20 training systems: with 10, about one training run in three gave a network no better than
-    # Jacobi on these held-out layouts; with 20 the median gain stayed above 6
+    # 20 training systems at a fixed seed; training is bit-reproducible, so the gains below repeat
     samples = LN.make_samples(n, systems=20, seed=11)
     net = A.UNet(n, n, max_batch=8, seed=2, coarsen=c)
     loss = LN.train(net, samples, steps=1000, batch=8, seed=3)
@@ -255,6 +257,46 @@ def test_trained_network_saves_iterations(oracle):
         assert st.converged == 1 and st.true_rel_residual <= 1e-8 and err <= 1e-6
         gains.append(st_j.iterations / max(st.iterations, 1))
     print("gains", gains)
-    # training is not bit-reproducible (FP32 atomics), and a single layout can land near 1, so
-    # the gain is judged on the median and mean over the held-out systems
-    assert np.median(gains) > 1.3 and np.mean(gains) > 1.5, gains
+    assert min(gains) >= 5.0, gains
+
+
+def _coarse_batch(rng, B, H, W):
+    """random coarse systems in the level-array layout: negative couplings, diagonally dominant"""
+    acx = np.zeros((B, H + 1, W + 1), np.float32)
+    acy = np.zeros_like(acx)
+    acx[:, 1:H, :W] = -rng.uniform(0.1, 5.0, (B, H - 1, W))
+    acy[:, :H, 1:W] = -rng.uniform(0.1, 5.0, (B, H, W - 1))
+    dc = np.zeros_like(acx)
+    dc[:, :H, :W] = -(acx[:, :H, :W] + acx[:, 1:H + 1, :W] + acy[:, :H, :W] + acy[:, :H, 1:W + 1]) + 0.01
+    g = rng.standard_normal((B, H, W)).astype(np.float32)
+    e = rng.uniform(0.5, 2.0, B).astype(np.float32)
+    return [A.DeviceArray.from_host(np.ascontiguousarray(a)) for a in (g, acx, acy, dc, e)]
+
+
+@pytest.mark.parametrize("energy", [False, True])
+def test_training_is_deterministic(energy):
+    """gradients are summed per block and the blocks added in a fixed order (no atomics), so two
+    networks with the same seed trained on the same batches end with the same bits, and a backward
+    pass repeated on the same input gives the same gradients"""
+    rng = np.random.default_rng(5)
+    B = 4
+    nets = [A.UNet(512, 512, max_batch=B, seed=3, coarsen=32) for _ in range(2)]
+    batches = [_coarse_batch(rng, B, 16, 16) for _ in range(5)]
+    h2 = (1.0 / 512) ** 2 / 32
+    losses = []
+    for net in nets:
+        ls = []
+        for g, acx, acy, dc, e in batches:
+            ls.append(net.train_step_coarse(B, g, acx, acy, dc, h2, h2, 1e-3, e if energy else None))
+        losses.append(ls)
+    assert losses[0] == losses[1]
+    np.testing.assert_array_equal(nets[0].get_params(), nets[1].get_params())
+    assert not np.array_equal(nets[0].get_params(), A.UNet(512, 512, max_batch=B, seed=3, coarsen=32).get_params())
+    f = rng.standard_normal((2, 16, 16, U.CIN)).astype(np.float32)
+    dy = rng.standard_normal((2, 16, 16)).astype(np.float32)
+    grads = []
+    for _ in range(2):
+        nets[0].forward(f)
+        nets[0].backward(dy)
+        grads.append(nets[0].get_grads())
+    np.testing.assert_array_equal(grads[0], grads[1])
