@@ -243,6 +243,8 @@ __global__ __launch_bounds__(256) void k_mom_u(Geom G, MomArgs A, const double* 
                                                double* __restrict__ out) {
     __shared__ double sFx[MI + 6][MJ];  // u^2 along i
     __shared__ double sFy[MI][MJ + 6];  // u vbar along j
+    __shared__ double sMu[MI + 1][MJ + 2];  // mu at cells (i0 - 1 + a, j0 - 1 + b)
+    __shared__ double sLs[MI + 1][MJ + 2];  // level set at the same cells
     const int tj = threadIdx.x & (MJ - 1), ti = threadIdx.x / MJ;
     const int j0 = G.g + blockIdx.x * MJ, i0 = G.g + 1 + blockIdx.y * MI;
     const int ipad = G.nx + 2 * G.g, jpad = G.ny + 2 * G.g;
@@ -251,6 +253,15 @@ __global__ __launch_bounds__(256) void k_mom_u(Geom G, MomArgs A, const double* 
         double f = 0.0;
         if (i < ipad && j < jpad) { const double w = u[G.idx(i, j)]; f = w * w; }
         sFx[r][tj] = f;
+    }
+    if (A.viscous) {
+        for (int e = threadIdx.x; e < (MI + 1) * (MJ + 2); e += 256) {
+            const int a = e / (MJ + 2), b = e - a * (MJ + 2);
+            const int i = i0 - 1 + a, j = j0 - 1 + b;
+            const bool in = i < ipad && j < jpad;
+            sMu[a][b] = in ? mu_at(G, Hs, A.mu_h, A.mu_l, i, j) : 0.0;
+            sLs[a][b] = in ? ls[G.idx(i, j)] : 0.0;
+        }
     }
     for (int e = threadIdx.x; e < MI * (MJ + 6); e += 256) {
         const int r = e / (MJ + 6), cc = e - r * (MJ + 6);
@@ -279,12 +290,16 @@ __global__ __launch_bounds__(256) void k_mom_u(Geom G, MomArgs A, const double* 
         const double uM = u[c], uW = u[G.idx(i - 1, j)], uE = u[G.idx(i + 1, j)];
         const double uS = u[G.idx(i, j - 1)], uN = u[G.idx(i, j + 1)];
         const double vW = v[G.idx(i - 1, j)], vW_N = v[G.idx(i - 1, j + 1)], vM = v[c], vN = v[G.idx(i, j + 1)];
-        const double lsW = ls[G.idx(i - 1, j)], lsM = ls[c];
-        const double lsUNHalf = 0.25 * (ls[c] + ls[G.idx(i - 1, j)] + ls[G.idx(i - 1, j + 1)] + ls[G.idx(i, j + 1)]);
-        const double lsUSHalf = 0.25 * (ls[c] + ls[G.idx(i - 1, j)] + ls[G.idx(i - 1, j - 1)] + ls[G.idx(i, j - 1)]);
+        // tile cell (i + di, j + dj) of the staged mu / level set
+        const int a0 = r + 1, b0 = tj + 1;
+#define LS(di, dj) sLs[a0 + (di)][b0 + (dj)]
+        const double lsW = LS(-1, 0), lsM = LS(0, 0);
+        const double lsUNHalf = 0.25 * (LS(0, 0) + LS(-1, 0) + LS(-1, 1) + LS(0, 1));
+        const double lsUSHalf = 0.25 * (LS(0, 0) + LS(-1, 0) + LS(-1, -1) + LS(0, -1));
+#undef LS
         const double iWE = irho_harmonic(A.rho_h, A.rho_l, lsW, lsM);
         const double iSN = irho_harmonic(A.rho_h, A.rho_l, lsUSHalf, lsUNHalf);
-#define MU(a, b) mu_at(G, Hs, A.mu_h, A.mu_l, a, b)
+#define MU(a, b) sMu[a0 + (a) - i][b0 + (b) - j]
         const double muU_X_W = MU(i - 1, j) * (uM - uW) / dx;
         const double muU_X_E = MU(i, j) * (uE - uM) / dx;
         const double muU_Y_S = 0.25 * (MU(i, j) + MU(i - 1, j) + MU(i - 1, j - 1) + MU(i, j - 1)) * (uM - uS) / dy;
@@ -317,9 +332,20 @@ __global__ __launch_bounds__(256) void k_mom_v(Geom G, MomArgs A, const double* 
                                                double* __restrict__ out) {
     __shared__ double sFx[MI + 6][MJ];  // v ubar along i
     __shared__ double sFy[MI][MJ + 6];  // v^2 along j
+    __shared__ double sMu[MI + 2][MJ + 1];  // mu at cells (i0 - 1 + a, j0 - 1 + b)
+    __shared__ double sLs[MI + 2][MJ + 1];  // level set at the same cells
     const int tj = threadIdx.x & (MJ - 1), ti = threadIdx.x / MJ;
     const int j0 = G.g + 1 + blockIdx.x * MJ, i0 = G.g + blockIdx.y * MI;
     const int ipad = G.nx + 2 * G.g, jpad = G.ny + 2 * G.g;
+    if (A.viscous) {
+        for (int e = threadIdx.x; e < (MI + 2) * (MJ + 1); e += 256) {
+            const int a = e / (MJ + 1), b = e - a * (MJ + 1);
+            const int i = i0 - 1 + a, j = j0 - 1 + b;
+            const bool in = i < ipad && j < jpad;
+            sMu[a][b] = in ? mu_at(G, Hs, A.mu_h, A.mu_l, i, j) : 0.0;
+            sLs[a][b] = in ? ls[G.idx(i, j)] : 0.0;
+        }
+    }
     for (int r = ti; r < MI + 6; r += 256 / MJ) {
         const int i = i0 - 3 + r, j = j0 + tj;
         sFx[r][tj] = (i < ipad && j < jpad) ? v[G.idx(i, j)] * ubar_v(G, u, i, j) : 0.0;
@@ -352,11 +378,15 @@ __global__ __launch_bounds__(256) void k_mom_v(Geom G, MomArgs A, const double* 
         const double vW = v[G.idx(i - 1, j)], vE = v[G.idx(i + 1, j)], vS = v[G.idx(i, j - 1)];
         const double vN = v[G.idx(i, j + 1)], vM = v[c];
         const double uM = u[c], uE = u[G.idx(i + 1, j)], uS = u[G.idx(i, j - 1)], uE_S = u[G.idx(i + 1, j - 1)];
-        const double lsVWHalf = 0.25 * (ls[c] + ls[G.idx(i - 1, j)] + ls[G.idx(i - 1, j - 1)] + ls[G.idx(i, j - 1)]);
-        const double lsVEHalf = 0.25 * (ls[c] + ls[G.idx(i + 1, j)] + ls[G.idx(i + 1, j - 1)] + ls[G.idx(i, j - 1)]);
+        // tile cell (i + di, j + dj) of the staged mu / level set
+        const int a0 = r + 1, b0 = tj + 1;
+#define LS(di, dj) sLs[a0 + (di)][b0 + (dj)]
+        const double lsVWHalf = 0.25 * (LS(0, 0) + LS(-1, 0) + LS(-1, -1) + LS(0, -1));
+        const double lsVEHalf = 0.25 * (LS(0, 0) + LS(1, 0) + LS(1, -1) + LS(0, -1));
         const double iWE = 1.0 / rho_arith(A.rho_h, A.rho_l, lsVWHalf, lsVEHalf);
-        const double iSN = 1.0 / rho_arith(A.rho_h, A.rho_l, ls[G.idx(i, j - 1)], ls[c]);
-#define MU(a, b) mu_at(G, Hs, A.mu_h, A.mu_l, a, b)
+        const double iSN = 1.0 / rho_arith(A.rho_h, A.rho_l, LS(0, -1), LS(0, 0));
+#undef LS
+#define MU(a, b) sMu[a0 + (a) - i][b0 + (b) - j]
         const double muU_Y_W = 0.25 * (MU(i, j) + MU(i - 1, j) + MU(i - 1, j - 1) + MU(i, j - 1)) * (uM - uS) / dy;
         const double muU_Y_E = 0.25 * (MU(i, j) + MU(i + 1, j) + MU(i + 1, j - 1) + MU(i, j - 1)) * (uE - uE_S) / dy;
         const double muV_X_W = 0.25 * (MU(i, j) + MU(i - 1, j) + MU(i - 1, j - 1) + MU(i, j - 1)) * (vM - vW) / dx;
