@@ -109,6 +109,8 @@ def load() -> C.CDLL:
         "asd2_memcpy_d2d": ([_VP, _VP, C.c_size_t], C.c_int),
         "asd2_device_synchronize": ([], C.c_int),
         "asd2_stream_copy": ([_VP, _VP, _VP, C.c_size_t], C.c_int),
+        "asd2_fill": ([_VP, _VP, C.c_size_t, C.c_double], C.c_int),
+        "asd2_mac_pressure": ([_VP, _VP, _VP, C.c_double], C.c_int),
         "asd2_create": ([P(Grid), P(_VP)], C.c_int), "asd2_destroy": ([_VP], C.c_int),
         "asd2_synchronize": ([_VP], C.c_int),
         "asd2_set_bc": ([_VP, C.c_int, P(C.c_int), P(C.c_double)], C.c_int),
@@ -448,6 +450,13 @@ class Context:
 
     def divergence(self, u, v, div):
         check(self.lib.asd2_mac_divergence(self.h, u.ptr, v.ptr, div.ptr))
+
+    def fill(self, arr, value):
+        check(self.lib.asd2_fill(self.h, arr.ptr, arr.nbytes // 8, C.c_double(value)))
+
+    def pressure(self, ps, p, dt):
+        """p = ps / dt on the interior (MACSolver2D::SolvePoisson's last loop)"""
+        check(self.lib.asd2_mac_pressure(self.h, ps.ptr, p.ptr, C.c_double(dt)))
 
     def update_vel(self, fl, u, v, us, vs, ps, ls, kap, H):
         check(self.lib.asd2_mac_update_vel(self.h, C.byref(fl), u.ptr, v.ptr, us.ptr, vs.ptr, ps.ptr,

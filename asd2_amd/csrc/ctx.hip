@@ -60,6 +60,11 @@ __global__ __launch_bounds__(256) void k_stream_copy(double* __restrict__ dst, c
         dst[i] = src[i];
 }
 
+__global__ __launch_bounds__(256) void k_fill(double* __restrict__ dst, int64_t count, double value) {
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < count; i += static_cast<int64_t>(gridDim.x) * 256)
+        dst[i] = value;
+}
+
 int zero_async(asd2_ctx* ctx, void* p, size_t bytes) {
     if (bytes == 0) return ASD2_OK;
     const size_t n64 = bytes / 8;
@@ -176,6 +181,16 @@ int asd2_stream_copy(asd2_ctx* ctx, double* d_dst, const double* d_src, size_t c
     const int blocks = static_cast<int>(std::min<size_t>(4096, (count + 255) / 256));
     hipLaunchKernelGGL(k_stream_copy, dim3(std::max(blocks, 1)), dim3(256), 0, ctx->stream, d_dst, d_src,
                        static_cast<int64_t>(count));
+    ASD2_LAUNCH_CHECK();
+    return ASD2_OK;
+}
+
+int asd2_fill(asd2_ctx* ctx, double* d_dst, size_t count, double value) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!d_dst && count) { set_error("null array"); return ASD2_ERR_ARG; }
+    if (count == 0) return ASD2_OK;
+    const int blocks = static_cast<int>(std::min<size_t>(4096, (count + 255) / 256));
+    hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, ctx->stream, d_dst, static_cast<int64_t>(count), value);
     ASD2_LAUNCH_CHECK();
     return ASD2_OK;
 }

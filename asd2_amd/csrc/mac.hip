@@ -66,6 +66,15 @@ __global__ __launch_bounds__(256) void k_divergence(Geom G, const double* __rest
     div[G.idx(i, j)] = d;
 }
 
+// p = ps / dt on the interior, the ghosts of p kept (src/mac2d.cpp:1722-1724)
+__global__ __launch_bounds__(256) void k_pressure(Geom G, const double* __restrict__ ps, double* __restrict__ p,
+                                                  double dt) {
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i < G.g || i >= G.nx + G.g || j < G.g || j >= G.ny + G.g) return;
+    p[G.idx(i, j)] = ps[G.idx(i, j)] / dt;
+}
+
 // ------------------------------------------------------------------ projection
 // face between a (west/south cell) and b (the cell itself), src/mac2d.cpp:1775-1788
 __device__ __forceinline__ void face_theta_w(double a, double b, double& thH, double& th) {
@@ -463,6 +472,16 @@ int asd2_mac_divergence(asd2_ctx* ctx, const double* d_u, const double* d_v, dou
     if (!d_u || !d_v || !d_div) { set_error("null array"); return ASD2_ERR_ARG; }
     const Geom& G = ctx->geo;
     hipLaunchKernelGGL(k_divergence, rows_grid(G), dim3(256), 0, ctx->stream, G, d_u, d_v, d_div);
+    ASD2_LAUNCH_CHECK();
+    return ASD2_OK;
+}
+
+int asd2_mac_pressure(asd2_ctx* ctx, const double* d_ps, double* d_p, double dt) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!d_ps || !d_p) { set_error("null array"); return ASD2_ERR_ARG; }
+    if (!(dt != 0.0)) { set_error("dt must be nonzero"); return ASD2_ERR_ARG; }
+    const Geom& G = ctx->geo;
+    hipLaunchKernelGGL(k_pressure, rows_grid(G), dim3(256), 0, ctx->stream, G, d_ps, d_p, dt);
     ASD2_LAUNCH_CHECK();
     return ASD2_OK;
 }

@@ -21,8 +21,25 @@ BoundaryCondition2D::BoundaryCondition2D(int nx, int ny, double dx, double dy, i
       m_BC_DirichletConstantPE(0), m_BC_DirichletConstantPS(0), m_BC_DirichletConstantPN(0) {}
 
 BoundaryCondition2D::~BoundaryCondition2D() {
-    if (m_ctx) asd2_destroy(m_ctx);
+    if (m_ctx) {
+        mirror::release(m_ctx);
+        asd2_destroy(m_ctx);
+    }
 }
+
+namespace {
+// the ghost fill of arr on ctx: through arr's mirror in resident mode, else upload / download
+void fill_ghosts(asd2_ctx* ctx, int grid_kind, std::vector<double>& arr, const char* where) {
+    if (mirror::resident()) {
+        check(asd2_bc_apply(ctx, grid_kind, mirror::acquire(ctx, arr, true)), where);
+        mirror::written(ctx, arr);
+        return;
+    }
+    DevVec d(arr);
+    check(asd2_bc_apply(ctx, grid_kind, d.p()), where);
+    d.get(arr);
+}
+}  // namespace
 
 int BoundaryCondition2D::SetBC_U_2D(std::string W, std::string E, std::string S, std::string N) {
     for (auto* s : {&W, &E, &S, &N})
@@ -85,9 +102,7 @@ void BoundaryCondition2D::Configure(asd2_ctx* ctx) const {
 void BoundaryCondition2D::ApplyAll(int grid_kind, std::vector<double>& arr) {
     if (!m_ctx) m_ctx = make_ctx(kNx, kNy, kNumBCGrid, 0.0, 0.0, kDx, kDy);
     Configure(m_ctx);
-    DevVec d(arr);
-    check(asd2_bc_apply(m_ctx, grid_kind, d.p()), "ApplyBC");
-    d.get(arr);
+    fill_ghosts(m_ctx, grid_kind, arr, "ApplyBC");
 }
 
 void BoundaryCondition2D::ApplySide(int grid_kind, int side, int kind, std::vector<double>& arr) {
@@ -102,9 +117,7 @@ void BoundaryCondition2D::ApplySide(int grid_kind, int side, int kind, std::vect
     double consts[4];
     for (int s = 0; s < 4; s++) consts[s] = *all[grid_kind][s];
     check(asd2_set_bc(m_ctx, grid_kind, kinds, consts), "SetBC side");
-    DevVec d(arr);
-    check(asd2_bc_apply(m_ctx, grid_kind, d.p()), "BC side");
-    d.get(arr);
+    fill_ghosts(m_ctx, grid_kind, arr, "BC side");
 }
 
 int BoundaryCondition2D::ApplyBC_U_2D(std::vector<double>& arr) { ApplyAll(ASD2_GRID_U, arr); return 0; }

@@ -1,8 +1,9 @@
 // levelset2d.cpp -- drop-in LevelSetSolver2D on the gfx950 level-set kernels.
-// Reference: src/levelset2d.cpp.  Every std::vector method uploads its inputs into the
-// solver's device buffers, runs the asd2_ls_* entry point and downloads the result; the
-// pseudo-time loops of the reinitialisations become an iteration count computed with the
-// reference's own floating-point loop (asd2_ls_reinit_iterations).
+// Reference: src/levelset2d.cpp.  By default every std::vector method uploads its inputs into the
+// solver's device buffers, runs the asd2_ls_* entry point and downloads the result; in resident
+// mode (asd2_resident.h) the level set stays on the device between calls.  The pseudo-time loops
+// of the reinitialisations become an iteration count computed with the reference's own
+// floating-point loop (asd2_ls_reinit_iterations).
 #include "levelset2d.h"
 
 #include <limits>
@@ -11,29 +12,7 @@
 
 using namespace asd2host;
 
-struct LevelSetSolver2D::Device {
-    asd2_ctx* ctx = nullptr;
-    std::unique_ptr<DevVec> buf[5];
-    size_t n = 0;
-    ~Device() {
-        for (auto& b : buf) b.reset();
-        if (ctx) asd2_destroy(ctx);
-    }
-    double* get(int k) {
-        if (!buf[k]) buf[k].reset(new DevVec(n));
-        return buf[k]->p();
-    }
-    double* put(int k, const std::vector<double>& h) {
-        if (h.size() != n) {
-            std::cerr << "LevelSetSolver2D: array of " << h.size() << " values, expected " << n << std::endl;
-            std::exit(1);
-        }
-        double* p = get(k);
-        buf[k]->put(h);
-        return p;
-    }
-    void take(int k, std::vector<double>& h) { buf[k]->get(h); }
-};
+struct LevelSetSolver2D::Device : SolverDevice {};
 
 // src/levelset2d.cpp:3-13: eps = min(dx, dy), thickness 1.5 eps, dtau = eps / 2,
 // default end time 1.5 * thickness
@@ -54,6 +33,7 @@ LevelSetSolver2D::Device& LevelSetSolver2D::Dev() {
         m_dev.reset(new Device);
         m_dev->ctx = make_ctx(kNx, kNy, kNumBCGrid, kBaseX, kBaseY, kDx, kDy);
         m_dev->n = static_cast<size_t>(kArrSize);
+        m_dev->who = "LevelSetSolver2D";
     }
     EnsureBC();
     m_BC->Configure(m_dev->ctx);
@@ -74,9 +54,10 @@ int LevelSetSolver2D::ReinitIterations() const { return asd2_ls_reinit_iteration
 int LevelSetSolver2D::Solve_LevelSet_2D(std::vector<double>& ls, const std::vector<double>& u,
                                         const std::vector<double>& v, double dt) {
     Device& D = Dev();
-    double* d_ls = D.put(0, ls);
-    check(asd2_ls_advect(D.ctx, d_ls, D.put(1, u), D.put(2, v), dt), "Solve_LevelSet_2D");
-    D.take(0, ls);
+    double* d_u = D.in("u", u);
+    double* d_v = D.in("v", v);
+    check(asd2_ls_advect(D.ctx, D.inout("ls", ls), d_u, d_v, dt), "Solve_LevelSet_2D");
+    D.wrote("ls", ls);
     return 0;
 }
 
@@ -88,12 +69,11 @@ int LevelSetSolver2D::Solve_LevelSet_2D_Device(double* d_ls, const double* d_u, 
 std::vector<double> LevelSetSolver2D::HJWENO5_LS_2D(std::vector<double>& ls, const std::vector<double>& u,
                                                     const std::vector<double>& v) {
     Device& D = Dev();
-    double* d_ls = D.put(0, ls);
-    check(asd2_ls_weno_rhs(D.ctx, d_ls, D.put(1, u), D.put(2, v), D.get(3)), "HJWENO5_LS_2D");
-    D.take(0, ls);  // the BC fill of ls is visible to the caller, as in the reference (:99)
-    std::vector<double> L;
-    D.take(3, L);
-    return L;
+    double* d_u = D.in("u", u);
+    double* d_v = D.in("v", v);
+    check(asd2_ls_weno_rhs(D.ctx, D.inout("ls", ls), d_u, d_v, D.scratch("out")), "HJWENO5_LS_2D");
+    D.wrote("ls", ls);  // the BC fill of ls is visible to the caller, as in the reference (:99)
+    return D.result("out");
 }
 
 int LevelSetSolver2D::UnitHJWENO5(const std::vector<double>& F, std::vector<double>& FP, std::vector<double>& FM,
@@ -116,20 +96,29 @@ int LevelSetSolver2D::UnitHJWENO5(const std::vector<double>& F, std::vector<doub
 }
 
 // ------------------------------------------------------------------ reinitialisation
-int LevelSetSolver2D::Reinit_Original_2D(std::vector<double>& ls) {
+// method 0 = Reinit_Original_2D, 1 = Reinit_Sussman_2D, 2 = Reinit_MinRK2_2D, 3 = first-time Sussman
+int LevelSetSolver2D::Reinit(int method, std::vector<double>& ls, int iterations, const char* where) {
     Device& D = Dev();
-    double* d = D.put(0, ls);
-    check(asd2_ls_reinit_original(D.ctx, d, ReinitIterations()), "Reinit_Original_2D");
-    D.take(0, ls);
-    m_atime = ReinitIterations() * kAdt;
+    double* d = D.inout("ls", ls);
+    int rc = ASD2_OK;
+    switch (method) {
+        case 0: rc = asd2_ls_reinit_original(D.ctx, d, iterations); break;
+        case 1: rc = asd2_ls_reinit_sussman(D.ctx, d, iterations); break;
+        case 2: rc = asd2_ls_reinit_minrk2(D.ctx, d, iterations); break;
+        default: rc = asd2_ls_reinit_first_time(D.ctx, d, iterations); break;
+    }
+    check(rc, where);
+    D.wrote("ls", ls);
+    m_atime = iterations * kAdt;
     return 0;
 }
 
+int LevelSetSolver2D::Reinit_Original_2D(std::vector<double>& ls) {
+    return Reinit(0, ls, ReinitIterations(), "Reinit_Original_2D");
+}
+
 int LevelSetSolver2D::Reinit_Sussman_2D(std::vector<double>& ls) {
-    Device& D = Dev();
-    check(Reinit_Sussman_2D_Device(D.put(0, ls)), "Reinit_Sussman_2D");
-    D.take(0, ls);
-    return 0;
+    return Reinit(1, ls, ReinitIterations(), "Reinit_Sussman_2D");
 }
 
 int LevelSetSolver2D::Reinit_Sussman_2D_Device(double* d_ls) {
@@ -139,10 +128,7 @@ int LevelSetSolver2D::Reinit_Sussman_2D_Device(double* d_ls) {
 }
 
 int LevelSetSolver2D::Reinit_MinRK2_2D(std::vector<double>& ls) {
-    Device& D = Dev();
-    check(Reinit_MinRK2_2D_Device(D.put(0, ls)), "Reinit_MinRK2_2D");
-    D.take(0, ls);
-    return 0;
+    return Reinit(2, ls, ReinitIterations(), "Reinit_MinRK2_2D");
 }
 
 int LevelSetSolver2D::Reinit_MinRK2_2D_Device(double* d_ls) {
@@ -153,13 +139,8 @@ int LevelSetSolver2D::Reinit_MinRK2_2D_Device(double* d_ls) {
 
 // src/levelset2d.cpp:377-431: the loop runs until the pseudo time reaches max(dx nx, dy ny)
 int LevelSetSolver2D::FirstTimeOnlyReinit_Sussman_2D(std::vector<double>& ls) {
-    Device& D = Dev();
-    const int iters = asd2_ls_reinit_iterations(kEps, std::max(kDx * kNx, kDy * kNy));
-    double* d = D.put(0, ls);
-    check(asd2_ls_reinit_first_time(D.ctx, d, iters), "FirstTimeOnlyReinit_Sussman_2D");
-    D.take(0, ls);
-    m_atime = iters * kAdt;
-    return 0;
+    return Reinit(3, ls, asd2_ls_reinit_iterations(kEps, std::max(kDx * kNx, kDy * kNy)),
+                  "FirstTimeOnlyReinit_Sussman_2D");
 }
 
 // Declared by the reference header (include/levelset2d.h:30) without a definition; here it
@@ -170,55 +151,54 @@ std::vector<double> LevelSetSolver2D::GetSussmanReinitConstraint(const std::vect
                                                                  const std::vector<double>& lsInit,
                                                                  const std::vector<double>& heavisideDeriv) {
     Device& D = Dev();
-    check(asd2_ls_sussman_constraint(D.ctx, D.put(0, ls), D.put(1, lsInit), D.put(2, heavisideDeriv), D.get(3)),
-          "GetSussmanReinitConstraint");
-    std::vector<double> out;
-    D.take(3, out);
-    return out;
+    double* d_ls = D.in("ls", ls);
+    double* d_ls0 = D.in("ls0", lsInit);
+    double* d_hd = D.in("hd", heavisideDeriv);
+    check(asd2_ls_sussman_constraint(D.ctx, d_ls, d_ls0, d_hd, D.scratch("out")), "GetSussmanReinitConstraint");
+    return D.result("out");
 }
 
 // ------------------------------------------------------------------ |grad phi| and pointwise helpers
+// op 0 = HJWENO5 |grad phi|, 1 = ENO, 2 = subcell ENO, 3 = Heaviside derivative, 4 = smoothed sign
+std::vector<double> LevelSetSolver2D::Pointwise(int op, const std::vector<double>& ls,
+                                                const std::vector<double>* ls0, const char* where) {
+    Device& D = Dev();
+    double* d_ls = D.in("ls", ls);
+    double* d_ls0 = ls0 ? D.in("ls0", *ls0) : nullptr;
+    double* out = D.scratch("out");
+    int rc = ASD2_OK;
+    switch (op) {
+        case 0: rc = asd2_ls_weno_absgrad(D.ctx, d_ls, d_ls0, out); break;
+        case 1: rc = asd2_ls_eno_absgrad(D.ctx, d_ls, d_ls0, out); break;
+        case 2: rc = asd2_ls_subcell_absgrad(D.ctx, d_ls, d_ls0, out); break;
+        case 3: rc = asd2_ls_heaviside_deriv(D.ctx, d_ls, out); break;
+        default: rc = asd2_ls_smoothed_sign(D.ctx, d_ls, out); break;
+    }
+    check(rc, where);
+    return D.result("out");
+}
+
 std::vector<double> LevelSetSolver2D::HJWENO5_DerivAbsLS_2D(const std::vector<double>& ls,
                                                             const std::vector<double>& lsInit) {
-    Device& D = Dev();
-    check(asd2_ls_weno_absgrad(D.ctx, D.put(0, ls), D.put(1, lsInit), D.get(3)), "HJWENO5_DerivAbsLS_2D");
-    std::vector<double> out;
-    D.take(3, out);
-    return out;
+    return Pointwise(0, ls, &lsInit, "HJWENO5_DerivAbsLS_2D");
 }
 
 std::vector<double> LevelSetSolver2D::ENO_DerivAbsLS_2D(const std::vector<double>& ls,
                                                         const std::vector<double>& lsInit) {
-    Device& D = Dev();
-    check(asd2_ls_eno_absgrad(D.ctx, D.put(0, ls), D.put(1, lsInit), D.get(3)), "ENO_DerivAbsLS_2D");
-    std::vector<double> out;
-    D.take(3, out);
-    return out;
+    return Pointwise(1, ls, &lsInit, "ENO_DerivAbsLS_2D");
 }
 
 std::vector<double> LevelSetSolver2D::SubcellENO_Min_DerivAbsLS_2D(const std::vector<double>& ls,
                                                                    const std::vector<double>& lsInit) {
-    Device& D = Dev();
-    check(asd2_ls_subcell_absgrad(D.ctx, D.put(0, ls), D.put(1, lsInit), D.get(3)), "SubcellENO_Min_DerivAbsLS_2D");
-    std::vector<double> out;
-    D.take(3, out);
-    return out;
+    return Pointwise(2, ls, &lsInit, "SubcellENO_Min_DerivAbsLS_2D");
 }
 
 std::vector<double> LevelSetSolver2D::UpdateHeavisideFuncDeriv(const std::vector<double>& ls) {
-    Device& D = Dev();
-    check(asd2_ls_heaviside_deriv(D.ctx, D.put(0, ls), D.get(3)), "UpdateHeavisideFuncDeriv");
-    std::vector<double> out;
-    D.take(3, out);
-    return out;
+    return Pointwise(3, ls, nullptr, "UpdateHeavisideFuncDeriv");
 }
 
 std::vector<double> LevelSetSolver2D::GetSmoothedSignFunc(const std::vector<double>& ls) {
-    Device& D = Dev();
-    check(asd2_ls_smoothed_sign(D.ctx, D.put(0, ls), D.get(3)), "GetSmoothedSignFunc");
-    std::vector<double> out;
-    D.take(3, out);
-    return out;
+    return Pointwise(4, ls, nullptr, "GetSmoothedSignFunc");
 }
 
 // src/levelset2d.cpp:895-912 (integer result, host only: it is not used by any solver path)
@@ -262,9 +242,17 @@ int LevelSetSolver2D::SetBC_P_2D(std::string W, std::string E, std::string S, st
     EnsureBC();
     return m_BC->SetBC_P_2D(W, E, S, N);
 }
-int LevelSetSolver2D::ApplyBC_U_2D(std::vector<double>& arr) { EnsureBC(); return m_BC->ApplyBC_U_2D(arr); }
-int LevelSetSolver2D::ApplyBC_V_2D(std::vector<double>& arr) { EnsureBC(); return m_BC->ApplyBC_V_2D(arr); }
-int LevelSetSolver2D::ApplyBC_P_2D(std::vector<double>& arr) { EnsureBC(); return m_BC->ApplyBC_P_2D(arr); }
+// the ghost fills run on the solver's own context, configured with m_BC's rules and the same
+// grid spacing (src/levelset2d.cpp:925-953 build m_BC with dx, dy)
+int LevelSetSolver2D::ApplyBC(int grid_kind, std::vector<double>& arr, const char* where) {
+    Device& D = Dev();
+    check(asd2_bc_apply(D.ctx, grid_kind, D.inout("bc", arr)), where);
+    D.wrote("bc", arr);
+    return 0;
+}
+int LevelSetSolver2D::ApplyBC_U_2D(std::vector<double>& arr) { return ApplyBC(ASD2_GRID_U, arr, "ApplyBC_U_2D"); }
+int LevelSetSolver2D::ApplyBC_V_2D(std::vector<double>& arr) { return ApplyBC(ASD2_GRID_V, arr, "ApplyBC_V_2D"); }
+int LevelSetSolver2D::ApplyBC_P_2D(std::vector<double>& arr) { return ApplyBC(ASD2_GRID_P, arr, "ApplyBC_P_2D"); }
 void LevelSetSolver2D::SetBCConstantUW(double c) { EnsureBC(); m_BC->SetBCConstantUW(c); }
 void LevelSetSolver2D::SetBCConstantUE(double c) { EnsureBC(); m_BC->SetBCConstantUE(c); }
 void LevelSetSolver2D::SetBCConstantUS(double c) { EnsureBC(); m_BC->SetBCConstantUS(c); }

@@ -2,46 +2,19 @@
 // asd2_poisson_*).  Reference: src/mac2d.cpp.
 //
 // The public fields stay host std::vectors because the reference's drivers read and write
-// them directly (tests/LargeTests/test_mac2d_large.cpp:451-457,488,534-535).  Each method
-// uploads what it reads into the solver's named device buffers, runs the kernels, and
-// downloads what it writes.  SolvePoisson never builds the CSR matrix: the ghost-fluid
-// coefficients are assembled matrix-free and solved on the GPU; PoissonSolver2D keeps its
-// CSR entry points for callers that build their own matrices.
+// them directly (tests/LargeTests/test_mac2d_large.cpp:451-457,488,534-535).  By default each
+// method uploads what it reads into the solver's named device buffers, runs the kernels, and
+// downloads what it writes; in resident mode (asd2_resident.h) the vectors it writes keep their
+// newest contents on the device between calls.  SolvePoisson never builds the CSR matrix: the
+// ghost-fluid coefficients are assembled matrix-free and solved on the GPU; PoissonSolver2D
+// keeps its CSR entry points for callers that build their own matrices.
 #include "mac2d.h"
-
-#include <map>
 
 #include "host_util.h"
 
 using namespace asd2host;
 
-struct MACSolver2D::Device {
-    asd2_ctx* ctx = nullptr;
-    size_t n = 0;
-    std::map<std::string, std::unique_ptr<DevVec>> bufs;
-    ~Device() {
-        bufs.clear();
-        if (ctx) asd2_destroy(ctx);
-    }
-    double* get(const std::string& name) {
-        auto& b = bufs[name];
-        if (!b) b.reset(new DevVec(n));
-        return b->p();
-    }
-    double* put(const std::string& name, const std::vector<double>& h) {
-        if (h.size() != n) {
-            std::cerr << "MACSolver2D: array '" << name << "' has " << h.size() << " values, expected " << n
-                      << std::endl;
-            std::exit(1);
-        }
-        double* p = get(name);
-        bufs[name]->put(h);
-        return p;
-    }
-    void zero(const std::string& name) { bufs[name]->put(std::vector<double>(n, 0.0)); }
-    std::vector<double> take(const std::string& name) { return bufs[name]->get(); }
-    void take(const std::string& name, std::vector<double>& h) { bufs[name]->get(h); }
-};
+struct MACSolver2D::Device : SolverDevice {};
 
 // src/mac2d.cpp:3-26 (dimensionless groups given; g = Fr L / U^2 as written there)
 MACSolver2D::MACSolver2D(double Re, double We, double Fr, GAXISENUM2D GAxis, double L, double U, double sigma,
@@ -81,13 +54,21 @@ MACSolver2D::MACSolver2D(double rhoH, double rhoL, double muH, double muL, doubl
     m_dt = std::min(0.005, kCFL * std::min(lenX / static_cast<double>(nx), lenY / static_cast<double>(ny)) / U);
 }
 
-MACSolver2D::~MACSolver2D() = default;
+// the members' device mirrors (resident mode) go with the members
+void MACSolver2D::ForgetMembers() {
+    for (auto* a : {&m_u, &m_v, &m_rho, &m_kappa, &m_mu, &m_ps, &m_p, &m_J11, &m_J12, &m_J21, &m_J22, &m_nx, &m_ny,
+                    &m_t1x, &m_t1y, &m_t1z, &m_t2x, &m_t2y, &m_t2z})
+        mirror::forget(*a);
+}
+
+MACSolver2D::~MACSolver2D() { ForgetMembers(); }
 
 MACSolver2D::Device& MACSolver2D::Dev() {
     if (!m_dev) {
         m_dev.reset(new Device);
         m_dev->ctx = make_ctx(kNx, kNy, kNumBCGrid, kBaseX, kBaseY, kDx, kDy);
         m_dev->n = static_cast<size_t>(kArrSize);
+        m_dev->who = "MACSolver2D";
     }
     // the reference creates m_BC in SetBC_*_2D (src/mac2d.cpp:1883-1911); without one all
     // sides are CUSTOM
@@ -124,6 +105,7 @@ void MACSolver2D::SetPoissonOptions(double tol, int mode, int precision, int pre
 
 // src/mac2d.cpp:59-85
 int MACSolver2D::AllocateVariables() {
+    ForgetMembers();
     for (auto* a : {&m_u, &m_v, &m_rho, &m_kappa, &m_mu, &m_ps, &m_p, &m_J11, &m_J12, &m_J21, &m_J22, &m_nx, &m_ny,
                     &m_t1x, &m_t1y, &m_t1z, &m_t2x, &m_t2y, &m_t2z})
         *a = std::vector<double>(static_cast<size_t>(kArrSize), 0.0);
@@ -133,23 +115,37 @@ int MACSolver2D::AllocateVariables() {
 // ------------------------------------------------------------------ interface fields
 std::vector<double> MACSolver2D::UpdateHeavisideFunc(const std::vector<double>& ls) {
     Device& D = Dev();
-    check(asd2_mac_heaviside(D.ctx, D.put("ls", ls), D.get("H"), 0), "UpdateHeavisideFunc");
-    return D.take("H");
+    check(asd2_mac_heaviside(D.ctx, D.in("ls", ls), D.scratch("H"), 0), "UpdateHeavisideFunc");
+    return D.result("H");
 }
 
 std::vector<double> MACSolver2D::UpdateSmoothHeavisideFunc(const std::vector<double>& ls) {
     Device& D = Dev();
-    check(asd2_mac_heaviside(D.ctx, D.put("ls", ls), D.get("Hs"), 1), "UpdateSmoothHeavisideFunc");
-    return D.take("Hs");
+    check(asd2_mac_heaviside(D.ctx, D.in("ls", ls), D.scratch("Hs"), 1), "UpdateSmoothHeavisideFunc");
+    return D.result("Hs");
+}
+
+void MACSolver2D::UpdateHeavisideFunc(const std::vector<double>& ls, std::vector<double>& H) {
+    Device& D = Dev();
+    double* d_ls = D.in("ls", ls);
+    check(asd2_mac_heaviside(D.ctx, d_ls, D.out("H", H), 0), "UpdateHeavisideFunc");
+    D.wrote("H", H);
+}
+
+void MACSolver2D::UpdateSmoothHeavisideFunc(const std::vector<double>& ls, std::vector<double>& H) {
+    Device& D = Dev();
+    double* d_ls = D.in("ls", ls);
+    check(asd2_mac_heaviside(D.ctx, d_ls, D.out("Hs", H), 1), "UpdateSmoothHeavisideFunc");
+    D.wrote("Hs", H);
 }
 
 // src/mac2d.cpp:221-252: m_kappa's interior is rewritten, its ghosts are kept
 int MACSolver2D::UpdateKappa(const std::vector<double>& ls) {
     Device& D = Dev();
     if (m_kappa.size() != static_cast<size_t>(kArrSize)) m_kappa.assign(static_cast<size_t>(kArrSize), 0.0);
-    double* k = D.put("kappa", m_kappa);
-    check(asd2_mac_kappa(D.ctx, D.put("ls", ls), k), "UpdateKappa");
-    D.take("kappa", m_kappa);
+    double* d_ls = D.in("ls", ls);
+    check(asd2_mac_kappa(D.ctx, d_ls, D.inout("kappa", m_kappa)), "UpdateKappa");
+    D.wrote("kappa", m_kappa);
     return 0;
 }
 
@@ -164,84 +160,68 @@ int MACSolver2D::UpdateNTK(const std::shared_ptr<LevelSetSolver2D>& LSolver, con
                            std::vector<double>&) {
     const double eta = 0.0001, eps = 1.0e-100;
     const std::vector<double> absd = LSolver->ENO_DerivAbsLS_2D(ls, ls);
+    SolverDevice::host_read(ls);
+    std::vector<double>& nX = std::get<0>(normalVec);
+    std::vector<double>& nY = std::get<1>(normalVec);
+    SolverDevice::host_read(nX);  // cells that are not rewritten keep their values
+    SolverDevice::host_read(nY);
     for (int i = kNumBCGrid; i < kNx + kNumBCGrid; i++)
         for (int j = kNumBCGrid; j < kNy + kNumBCGrid; j++) {
             if (std::fabs(1.0 - absd[idx(i, j)]) > eta) continue;
             const double dX = (ls[idx(i + 1, j)] - ls[idx(i - 1, j)]) / (2.0 * kDx);
             const double dY = (ls[idx(i, j + 1)] - ls[idx(i, j - 1)]) / (2.0 * kDy);
-            std::get<0>(normalVec)[idx(i, j)] = 1.0 / std::sqrt(dX * dX + dY * dY + eps) * dX;
-            std::get<1>(normalVec)[idx(i, j)] = 1.0 / std::sqrt(dX * dX + dY * dY + eps) * dY;
+            nX[idx(i, j)] = 1.0 / std::sqrt(dX * dX + dY * dY + eps) * dX;
+            nY[idx(i, j)] = 1.0 / std::sqrt(dX * dX + dY * dY + eps) * dY;
         }
+    asd2::ModifiedHost(nX);
+    asd2::ModifiedHost(nY);
     return 0;
 }
 
 // ------------------------------------------------------------------ momentum predictor
-std::vector<double> MACSolver2D::UpdateFU(const std::shared_ptr<LevelSetSolver2D>&, const std::vector<double>& ls,
-                                          const std::vector<double>& u, const std::vector<double>& v,
-                                          const std::vector<double>& H) {
+std::vector<double> MACSolver2D::MomentumTerm(int grid_kind, int term, const std::vector<double>* ls,
+                                              const std::vector<double>& u, const std::vector<double>& v,
+                                              const std::vector<double>* H, const char* where) {
     Device& D = Dev();
     asd2_fluid f;
     FluidParams(&f);
-    check(asd2_mac_momentum_term(D.ctx, &f, ASD2_GRID_U, ASD2_MOM_RHS, D.put("ls", ls), D.put("u", u), D.put("v", v),
-                                 D.put("Hs", H), D.get("out")),
-          "UpdateFU");
-    return D.take("out");
+    // the convection term reads neither ls nor H
+    double* d_ls = ls ? D.in("ls", *ls) : D.scratch("ls");
+    double* d_u = D.in("u", u);
+    double* d_v = D.in("v", v);
+    double* d_H = H ? D.in("Hs", *H) : D.scratch("Hs");
+    check(asd2_mac_momentum_term(D.ctx, &f, grid_kind, term, d_ls, d_u, d_v, d_H, D.scratch("out")), where);
+    return D.result("out");
+}
+
+std::vector<double> MACSolver2D::UpdateFU(const std::shared_ptr<LevelSetSolver2D>&, const std::vector<double>& ls,
+                                          const std::vector<double>& u, const std::vector<double>& v,
+                                          const std::vector<double>& H) {
+    return MomentumTerm(ASD2_GRID_U, ASD2_MOM_RHS, &ls, u, v, &H, "UpdateFU");
 }
 
 std::vector<double> MACSolver2D::UpdateFV(const std::shared_ptr<LevelSetSolver2D>&, const std::vector<double>& ls,
                                           const std::vector<double>& u, const std::vector<double>& v,
                                           const std::vector<double>& H) {
-    Device& D = Dev();
-    asd2_fluid f;
-    FluidParams(&f);
-    check(asd2_mac_momentum_term(D.ctx, &f, ASD2_GRID_V, ASD2_MOM_RHS, D.put("ls", ls), D.put("u", u), D.put("v", v),
-                                 D.put("Hs", H), D.get("out")),
-          "UpdateFV");
-    return D.take("out");
+    return MomentumTerm(ASD2_GRID_V, ASD2_MOM_RHS, &ls, u, v, &H, "UpdateFV");
 }
 
 std::vector<double> MACSolver2D::AddConvectionFU(const std::vector<double>& u, const std::vector<double>& v) {
-    Device& D = Dev();
-    asd2_fluid f;
-    FluidParams(&f);
-    double* ls = D.get("ls");  // not read by the convection term
-    check(asd2_mac_momentum_term(D.ctx, &f, ASD2_GRID_U, ASD2_MOM_CONVECTION, ls, D.put("u", u), D.put("v", v),
-                                 D.get("Hs"), D.get("out")),
-          "AddConvectionFU");
-    return D.take("out");
+    return MomentumTerm(ASD2_GRID_U, ASD2_MOM_CONVECTION, nullptr, u, v, nullptr, "AddConvectionFU");
 }
 
 std::vector<double> MACSolver2D::AddConvectionFV(const std::vector<double>& u, const std::vector<double>& v) {
-    Device& D = Dev();
-    asd2_fluid f;
-    FluidParams(&f);
-    double* ls = D.get("ls");
-    check(asd2_mac_momentum_term(D.ctx, &f, ASD2_GRID_V, ASD2_MOM_CONVECTION, ls, D.put("u", u), D.put("v", v),
-                                 D.get("Hs"), D.get("out")),
-          "AddConvectionFV");
-    return D.take("out");
+    return MomentumTerm(ASD2_GRID_V, ASD2_MOM_CONVECTION, nullptr, u, v, nullptr, "AddConvectionFV");
 }
 
 std::vector<double> MACSolver2D::AddViscosityFU(const std::vector<double>& u, const std::vector<double>& v,
                                                 const std::vector<double>& ls, const std::vector<double>& H) {
-    Device& D = Dev();
-    asd2_fluid f;
-    FluidParams(&f);
-    check(asd2_mac_momentum_term(D.ctx, &f, ASD2_GRID_U, ASD2_MOM_VISCOSITY, D.put("ls", ls), D.put("u", u),
-                                 D.put("v", v), D.put("Hs", H), D.get("out")),
-          "AddViscosityFU");
-    return D.take("out");
+    return MomentumTerm(ASD2_GRID_U, ASD2_MOM_VISCOSITY, &ls, u, v, &H, "AddViscosityFU");
 }
 
 std::vector<double> MACSolver2D::AddViscosityFV(const std::vector<double>& u, const std::vector<double>& v,
                                                 const std::vector<double>& ls, const std::vector<double>& H) {
-    Device& D = Dev();
-    asd2_fluid f;
-    FluidParams(&f);
-    check(asd2_mac_momentum_term(D.ctx, &f, ASD2_GRID_V, ASD2_MOM_VISCOSITY, D.put("ls", ls), D.put("u", u),
-                                 D.put("v", v), D.put("Hs", H), D.get("out")),
-          "AddViscosityFV");
-    return D.take("out");
+    return MomentumTerm(ASD2_GRID_V, ASD2_MOM_VISCOSITY, &ls, u, v, &H, "AddViscosityFV");
 }
 
 int MACSolver2D::UnitHJWENO5(const std::vector<double>& F, std::vector<double>& FP, std::vector<double>& FM,
@@ -287,13 +267,15 @@ int MACSolver2D::GetIntermediateVel(const std::shared_ptr<LevelSetSolver2D>&, co
     asd2_fluid f;
     FluidParams(&f);
     const int order = (kTimeOrder == TIMEORDERENUM::EULER) ? 0 : (kTimeOrder == TIMEORDERENUM::RK2 ? 1 : 2);
-    double* du = D.put("uhat", uhat);
-    double* dv = D.put("vhat", vhat);
-    check(asd2_mac_intermediate_vel(D.ctx, &f, order, D.put("ls", ls), D.put("u", u), D.put("v", v), du, dv,
-                                    D.put("Hs", H)),
-          "GetIntermediateVel");
-    D.take("uhat", uhat);
-    D.take("vhat", vhat);
+    double* d_ls = D.in("ls", ls);
+    double* d_u = D.in("u", u);
+    double* d_v = D.in("v", v);
+    double* d_H = D.in("Hs", H);
+    double* du = D.inout("uhat", uhat);
+    double* dv = D.inout("vhat", vhat);
+    check(asd2_mac_intermediate_vel(D.ctx, &f, order, d_ls, d_u, d_v, du, dv, d_H), "GetIntermediateVel");
+    D.wrote("uhat", uhat);
+    D.wrote("vhat", vhat);
     return 0;
 }
 
@@ -302,6 +284,14 @@ int MACSolver2D::SetPoissonSolver(POISSONTYPE type) {
     m_PoissonSolverType = type;
     if (!m_Poisson) m_Poisson = std::make_shared<PoissonSolver2D>(kNx, kNy, kNumBCGrid);
     return 0;
+}
+
+// m_p = ps / dt on the interior (src/mac2d.cpp:1722-1724), on the device
+void MACSolver2D::PressureFromPs(double* d_ps) {
+    Device& D = Dev();
+    if (m_p.size() != static_cast<size_t>(kArrSize)) m_p.assign(static_cast<size_t>(kArrSize), 0.0);
+    check(asd2_mac_pressure(D.ctx, d_ps, D.inout("p", m_p), m_dt), "SolvePoisson (p = ps / dt)");
+    D.wrote("p", m_p);
 }
 
 // src/mac2d.cpp:1249-1727.  ps is zeroed, kappa refreshed from ls when the Weber number is
@@ -319,41 +309,42 @@ int MACSolver2D::SolvePoisson(std::vector<double>& ps, const std::vector<double>
     }
     Device& D = Dev();
     if (m_kappa.size() != static_cast<size_t>(kArrSize)) m_kappa.assign(static_cast<size_t>(kArrSize), 0.0);
-    double* d_ls = D.put("ls", ls);
-    double* d_kappa = D.put("kappa", m_kappa);
+    double* d_ls = D.in("ls", ls);
+    double* d_kappa = D.inout("kappa", m_kappa);
     if (kWe != 0.0 && !std::isnan(kWe) && !std::isinf(kWe)) {
         check(asd2_mac_kappa(D.ctx, d_ls, d_kappa), "UpdateKappa");
         check(asd2_bc_apply(D.ctx, ASD2_GRID_P, d_kappa), "ApplyBC_P_2D(kappa)");
-        D.take("kappa", m_kappa);
+        D.wrote("kappa", m_kappa);
     }
+    const size_t n = static_cast<size_t>(kArrSize);
     if (m_PoissonSolverType == POISSONTYPE::MKL) {
         // src/mac2d.cpp:1658-1689: the constant-coefficient solve with rhs = -div on the interior
         // (ghosts 0) and ps zeroed; the assembled ghost-fluid system is not used on this path
-        std::vector<double> rhs(static_cast<size_t>(kArrSize), 0.0);
+        SolverDevice::host_read(div);
+        std::vector<double> rhs(n, 0.0);
         for (int j = kNumBCGrid; j < kNy + kNumBCGrid; j++)
             for (int i = kNumBCGrid; i < kNx + kNumBCGrid; i++) rhs[idx(i, j)] = -div[idx(i, j)];
-        ps.assign(static_cast<size_t>(kArrSize), 0.0);
-        double* d_ps = D.put("ps", ps);
-        if (asd2_fast_poisson(D.ctx, D.put("rhs", rhs), d_ps, kLenX, kLenY, kDx, kDy, 0.0) != ASD2_OK) {
+        double* d_ps = D.out("ps", ps);
+        check(asd2_fill(D.ctx, d_ps, n, 0.0), "SolvePoisson (ps = 0)");
+        if (asd2_fast_poisson(D.ctx, D.in("rhs", rhs), d_ps, kLenX, kLenY, kDx, kDy, 0.0) != ASD2_OK) {
             std::cerr << "MKL Error:d_Helmholtz_2D: " << asd2_last_error() << std::endl;
             std::exit(1);
         }
         check(asd2_bc_apply(D.ctx, ASD2_GRID_P, d_ps), "ApplyBC_P_2D(ps)");
-        D.take("ps", ps);  // the ghost update leaves the interior the reference checks unchanged
+        D.wrote("ps", ps);
+        SolverDevice::host_read(ps);  // the ghost update leaves the interior the reference checks unchanged
         for (int j = kNumBCGrid; j < kNy + kNumBCGrid; j++)
             for (int i = kNumBCGrid; i < kNx + kNumBCGrid; i++)
                 if (std::isnan(ps[idx(i, j)]) || std::isinf(ps[idx(i, j)]))
                     std::cout << "Pseudo-p nan/inf error : " << i << " " << j << " " << ps[idx(i, j)] << std::endl;
-        if (m_p.size() != static_cast<size_t>(kArrSize)) m_p.assign(static_cast<size_t>(kArrSize), 0.0);
-        for (int i = kNumBCGrid; i < kNx + kNumBCGrid; i++)
-            for (int j = kNumBCGrid; j < kNy + kNumBCGrid; j++) m_p[idx(i, j)] = ps[idx(i, j)] / m_dt;
+        PressureFromPs(d_ps);
         return 0;
     }
     asd2_fluid f;
     FluidParams(&f);
-    check(asd2_gfm_assemble(D.ctx, &f, d_ls, d_kappa, D.put("H", H), D.put("div", div)), "SolvePoisson assembly");
-    ps.assign(static_cast<size_t>(kArrSize), 0.0);
-    double* d_ps = D.put("ps", ps);
+    check(asd2_gfm_assemble(D.ctx, &f, d_ls, d_kappa, D.in("H", H), D.in("div", div)), "SolvePoisson assembly");
+    double* d_ps = D.out("ps", ps);
+    check(asd2_fill(D.ctx, d_ps, n, 0.0), "SolvePoisson (ps = 0)");
     asd2_solve_opts o{};
     o.tol = m_tol;
     o.max_iter = maxIter;
@@ -372,17 +363,24 @@ int MACSolver2D::SolvePoisson(std::vector<double>& ps, const std::vector<double>
     std::cout << (m_PoissonSolverType == POISSONTYPE::BICGSTAB ? "BiCG : " : "CG : ") << st.iterations << " "
               << maxIter << " Err : " << st.rel_residual << std::endl;
     check(asd2_bc_apply(D.ctx, ASD2_GRID_P, d_ps), "ApplyBC_P_2D(ps)");
-    D.take("ps", ps);
-    if (m_p.size() != static_cast<size_t>(kArrSize)) m_p.assign(static_cast<size_t>(kArrSize), 0.0);
-    for (int i = kNumBCGrid; i < kNx + kNumBCGrid; i++)
-        for (int j = kNumBCGrid; j < kNy + kNumBCGrid; j++) m_p[idx(i, j)] = ps[idx(i, j)] / m_dt;
+    D.wrote("ps", ps);
+    PressureFromPs(d_ps);
     return 0;
 }
 
 std::vector<double> MACSolver2D::GetDivergence(const std::vector<double>& u, const std::vector<double>& v) {
     Device& D = Dev();
-    check(asd2_mac_divergence(D.ctx, D.put("u", u), D.put("v", v), D.get("div")), "GetDivergence");
-    return D.take("div");
+    double* d_u = D.in("u", u);
+    check(asd2_mac_divergence(D.ctx, d_u, D.in("v", v), D.scratch("div")), "GetDivergence");
+    return D.result("div");
+}
+
+void MACSolver2D::GetDivergence(const std::vector<double>& u, const std::vector<double>& v, std::vector<double>& div) {
+    Device& D = Dev();
+    double* d_u = D.in("u", u);
+    double* d_v = D.in("v", v);
+    check(asd2_mac_divergence(D.ctx, d_u, d_v, D.out("div", div)), "GetDivergence");
+    D.wrote("div", div);
 }
 
 int MACSolver2D::UpdateVel(std::vector<double>& u, std::vector<double>& v, const std::vector<double>& us,
@@ -392,13 +390,17 @@ int MACSolver2D::UpdateVel(std::vector<double>& u, std::vector<double>& v, const
     asd2_fluid f;
     FluidParams(&f);
     if (m_kappa.size() != static_cast<size_t>(kArrSize)) m_kappa.assign(static_cast<size_t>(kArrSize), 0.0);
-    double* du = D.put("u", u);
-    double* dv = D.put("v", v);
-    check(asd2_mac_update_vel(D.ctx, &f, du, dv, D.put("us", us), D.put("vs", vs), D.put("ps", ps), D.put("ls", ls),
-                              D.put("kappa", m_kappa), D.put("H", H)),
-          "UpdateVel");
-    D.take("u", u);
-    D.take("v", v);
+    double* d_us = D.in("us", us);
+    double* d_vs = D.in("vs", vs);
+    double* d_ps = D.in("ps", ps);
+    double* d_ls = D.in("ls", ls);
+    double* d_k = D.in("kappa", m_kappa);
+    double* d_H = D.in("H", H);
+    double* du = D.inout("u", u);
+    double* dv = D.inout("v", v);
+    check(asd2_mac_update_vel(D.ctx, &f, du, dv, d_us, d_vs, d_ps, d_ls, d_k, d_H), "UpdateVel");
+    D.wrote("u", u);
+    D.wrote("v", v);
     return 0;
 }
 
@@ -407,12 +409,28 @@ double MACSolver2D::UpdateDt(const std::vector<double>& u, const std::vector<dou
     asd2_fluid f;
     FluidParams(&f);
     double dt = 0.0;
-    check(asd2_mac_update_dt(D.ctx, &f, D.put("u", u), D.put("v", v), &dt), "UpdateDt");
+    double* d_u = D.in("u", u);
+    check(asd2_mac_update_dt(D.ctx, &f, d_u, D.in("v", v), &dt), "UpdateDt");
     if (std::isnan(dt) || std::isinf(dt)) std::cout << "dt nan/inf error" << std::endl;
     return dt;
 }
 
 // ------------------------------------------------------------------ boundary conditions
+// The ghost fills run on the solver's own context (configured with m_BC's rules), so a resident
+// vector stays on this context's stream.  An LS-free pressure side extrapolates with the BC
+// object's grid spacing, which the reference's MACSolver2D never sets (src/mac2d.cpp:1884-1904
+// build m_BC without dx, dy): those fills go through m_BC, whose rounding follows the reference.
+int MACSolver2D::ApplyBC(int grid_kind, std::vector<double>& arr, const char* where) {
+    Device& D = Dev();
+    if (grid_kind == ASD2_GRID_P) {
+        for (BC2D k : {m_BC->m_BC_PW, m_BC->m_BC_PE, m_BC->m_BC_PS, m_BC->m_BC_PN})
+            if (k == BC2D::LSFREE) return m_BC->ApplyBC_P_2D(arr);
+    }
+    check(asd2_bc_apply(D.ctx, grid_kind, D.inout("bc", arr)), where);
+    D.wrote("bc", arr);
+    return 0;
+}
+
 int MACSolver2D::SetBC_U_2D(std::string W, std::string E, std::string S, std::string N) {
     if (!m_BC) m_BC = std::make_shared<BoundaryCondition2D>(kNx, kNy, kNumBCGrid);
     return m_BC->SetBC_U_2D(W, E, S, N);
@@ -425,9 +443,9 @@ int MACSolver2D::SetBC_P_2D(std::string W, std::string E, std::string S, std::st
     if (!m_BC) m_BC = std::make_shared<BoundaryCondition2D>(kNx, kNy, kNumBCGrid);
     return m_BC->SetBC_P_2D(W, E, S, N);
 }
-int MACSolver2D::ApplyBC_U_2D(std::vector<double>& arr) { Dev(); return m_BC->ApplyBC_U_2D(arr); }
-int MACSolver2D::ApplyBC_V_2D(std::vector<double>& arr) { Dev(); return m_BC->ApplyBC_V_2D(arr); }
-int MACSolver2D::ApplyBC_P_2D(std::vector<double>& arr) { Dev(); return m_BC->ApplyBC_P_2D(arr); }
+int MACSolver2D::ApplyBC_U_2D(std::vector<double>& arr) { return ApplyBC(ASD2_GRID_U, arr, "ApplyBC_U_2D"); }
+int MACSolver2D::ApplyBC_V_2D(std::vector<double>& arr) { return ApplyBC(ASD2_GRID_V, arr, "ApplyBC_V_2D"); }
+int MACSolver2D::ApplyBC_P_2D(std::vector<double>& arr) { return ApplyBC(ASD2_GRID_P, arr, "ApplyBC_P_2D"); }
 void MACSolver2D::SetBCConstantUW(double c) { Dev(); m_BC->SetBCConstantUW(c); }
 void MACSolver2D::SetBCConstantUE(double c) { Dev(); m_BC->SetBCConstantUE(c); }
 void MACSolver2D::SetBCConstantUS(double c) { Dev(); m_BC->SetBCConstantUS(c); }
@@ -486,6 +504,7 @@ int MACSolver2D::OutRes(const int iter, const double curTime, const std::string 
         outF.close();
     }
     const std::vector<double> div = GetDivergence(u, v);
+    for (const std::vector<double>* a : {&u, &v, &ps, &ls}) SolverDevice::host_read(*a);
     auto zone = [&](std::ofstream& o) {
         o << "ZONE T=\"" << iter << "\", I=" << kNx << ", J=" << kNy << ", SOLUTIONTIME=" << curTime
           << ", STRANDID=" << iter + 1 << std::endl;

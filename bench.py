@@ -350,6 +350,28 @@ def config_fast_poisson(A, n=2048, repeats=5):
                       "frac_of_fp64_peak": flops / (gemm_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS}}
 
 
+def config_dropin(A, n=2048, steps=10):
+    """The benchmark's droplet step through the drop-in C++ classes (tests/cpp/dropin_driver.cpp
+    bench mode: the reference driver's loop, tests/LargeTests/test_mac2d_large.cpp:496-535, with the
+    same physics and multilevel pressure path as the C-ABI step), once with the arrays kept on the
+    device between calls (include/asd2_resident.h) and once with the reference's host-vector
+    semantics (every call uploads its inputs and downloads its outputs)."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "cpp", "dropin_driver")
+    if not os.path.exists(exe):
+        return {"error": f"{exe} is not built"}
+    out = {"workload": f"droplet step {n}x{n} through LevelSetSolver2D / MACSolver2D (reference loop order)"}
+    for name, resident in (("resident", 1), ("host_vectors", 0)):
+        res = subprocess.run([exe, "bench", str(n), str(steps), str(resident)], capture_output=True, text=True,
+                             timeout=600)
+        lines = [l for l in res.stdout.splitlines() if l.startswith("BENCH ")]
+        if res.returncode != 0 or not lines:
+            out[name] = {"error": f"rc {res.returncode}: {res.stderr[-300:]}"}
+            continue
+        out[name] = json.loads(lines[-1][len("BENCH "):])
+    return out
+
+
 def pmc_workload(n):
     """A short fixed workload for hardware-counter runs (rocprofv3 --pmc): three plain copies
     of 2^25 doubles (the calibration kernel), the stencil part of one step and 32 iterations
@@ -476,10 +498,15 @@ def main():
     configs = {}
     if not args.no_configs:
         for name, fn in (("c2_levelset_4096", config_c2), ("c3_seeded_2048", config_c3),
-                         ("c5_rising_bubble_1024x2048", config_c5), ("fast_poisson_2048", config_fast_poisson)):
+                         ("c5_rising_bubble_1024x2048", config_c5), ("fast_poisson_2048", config_fast_poisson),
+                         ("dropin_2048", config_dropin)):
             print(f"# config {name} ...", file=sys.stderr, flush=True)
             configs[name] = fn(A)
             print(f"# {name}: {json.dumps(configs[name])[:300]}", file=sys.stderr, flush=True)
+        dr, ml = configs.get("dropin_2048", {}).get("resident"), paths.get("multilevel")
+        if isinstance(dr, dict) and "ms_per_step" in dr and ml and args.n == dr.get("n"):
+            # the same step and pressure path driven through the C ABI from Python (this run)
+            configs["dropin_2048"]["resident_over_c_abi"] = dr["ms_per_step"] / ml["ms_per_step"]
     cpu = None if args.no_cpu or not fp64 else cpu_baseline(args.n, fp64["pcg_iters_per_step"])
     line = {
         "metric": METRIC, "value": best["ms_per_step"], "unit": "ms/step", "n_gpus": 1,

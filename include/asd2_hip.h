@@ -113,6 +113,10 @@ int asd2_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes);
 /* d_dst[0:count] = d_src[0:count] by a plain streaming kernel on the context's stream.  No
  * reference counterpart: it is the known-size copy that HBM byte counters are calibrated on. */
 int asd2_stream_copy(asd2_ctx* ctx, double* d_dst, const double* d_src, size_t count);
+/* d_dst[0:count] = value on the context's stream (a kernel, no memset node).  Stands for the
+ * whole-array host loops of the reference's classes, e.g. the zero initial guess of the pressure
+ * CG (src/poisson2d.cpp:299-300), when the drop-in classes keep the arrays on the device. */
+int asd2_fill(asd2_ctx* ctx, double* d_dst, size_t count, double value);
 int asd2_device_synchronize(void);
 
 /* ---------------------------------------------------------------- context */
@@ -216,13 +220,16 @@ int asd2_tdma(asd2_ctx* ctx, int n, int count, const double* d_a, const double* 
 int asd2_mac_update_vel(asd2_ctx* ctx, const asd2_fluid* fl, double* d_u, double* d_v,
                         const double* d_us, const double* d_vs, const double* d_ps,
                         const double* d_ls, const double* d_kappa, const double* d_H);
+/* the pressure of MACSolver2D::SolvePoisson's last loop, src/mac2d.cpp:1722-1724: p = ps / dt on the
+ * interior, the ghosts of p kept */
+int asd2_mac_pressure(asd2_ctx* ctx, const double* d_ps, double* d_p, double dt);
 /* MACSolver2D::UpdateDt, src/mac2d.cpp:1851-1880 */
+int asd2_mac_update_dt(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_u, const double* d_v,
+                       double* dt_out);
 /* MACSolver2DAxisym::UpdateDt (src/mac2dAxisym.cpp:2094-2128): the planar bound with the capillary
  * term sqrt(sigma max|kappa| / (min rho min(dr, dz)^2)) and no viscous term; x = r, y = z */
 int asd2_mac_update_dt_axisym(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_u, const double* d_v,
                               const double* d_kappa, double* dt_out);
-int asd2_mac_update_dt(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_u, const double* d_v,
-                       double* dt_out);
 /* MACSolver2D::GetIntermediateVel, src/mac2d.cpp:1081-1239 (time_order 0/1/2 =
  * EULER/RK2/RK3; d_Hs is the smoothed Heaviside) */
 int asd2_mac_intermediate_vel(asd2_ctx* ctx, const asd2_fluid* fl, int time_order,

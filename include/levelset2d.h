@@ -15,6 +15,7 @@
 #include <tuple>
 #include <vector>
 
+#include "asd2_resident.h"
 #include "bc2d.h"
 #include "common.h"
 
@@ -103,6 +104,10 @@ private:
     struct Device;  // device context, workspace buffers
     void EnsureBC();
     Device& Dev();
+    std::vector<double> Pointwise(int op, const std::vector<double>& ls, const std::vector<double>* ls0,
+                                  const char* where);
+    int Reinit(int method, std::vector<double>& ls, int iterations, const char* where);
+    int ApplyBC(int grid_kind, std::vector<double>& arr, const char* where);
     std::unique_ptr<Device> m_dev;
 };
 

@@ -162,11 +162,23 @@ public:
     void SetGravityFix(bool fix) { m_gravityFix = fix; }
     int LastPoissonIterations() const { return m_lastIters; }
     asd2_ctx* Context();
+    // in-place forms of the by-value UpdateHeavisideFunc / UpdateSmoothHeavisideFunc /
+    // GetDivergence: the result goes into H / div (sized to the grid if it is not), so in
+    // resident mode (asd2_resident.h) it stays on the device and no host vector is allocated
+    void UpdateHeavisideFunc(const std::vector<double>& ls, std::vector<double>& H);
+    void UpdateSmoothHeavisideFunc(const std::vector<double>& ls, std::vector<double>& H);
+    void GetDivergence(const std::vector<double>& u, const std::vector<double>& v, std::vector<double>& div);
 
 private:
     struct Device;  // device context and named workspace buffers
     Device& Dev();
     void FluidParams(void* out) const;
+    void ForgetMembers();
+    std::vector<double> MomentumTerm(int grid_kind, int term, const std::vector<double>* ls,
+                                     const std::vector<double>& u, const std::vector<double>& v,
+                                     const std::vector<double>* H, const char* where);
+    void PressureFromPs(double* d_ps);
+    int ApplyBC(int grid_kind, std::vector<double>& arr, const char* where);
     std::unique_ptr<Device> m_dev;
     double m_tol = 1.0e-6;
     int m_mode = 0, m_precision = 0, m_precond = 0;

@@ -3,12 +3,22 @@
 // headers and libasd2_hip.so only.  The Python tests feed it inputs and compare what it
 // writes with the oracle.
 //
-//   dropin_driver step <dir> <n> <steps> <mode>
+//   dropin_driver step <dir> <n> <steps> <mode> [resident]
 //                                           stationary bubble (tests/LargeTests/
 //                                           test_mac2d_large.cpp:376-540) on an n x n grid;
 //                                           mode 0 keeps the reference CG (recurrence and
 //                                           1e-6 tolerance), mode 1 solves to 1e-10 with
-//                                           the textbook recurrence
+//                                           the textbook recurrence; resident 1 runs the same
+//                                           loop with the arrays kept on the device
+//                                           (asd2_resident.h) and the in-place overloads
+//   dropin_driver bench <n> <steps> <resident>
+//                                           the droplet step of the benchmark (unit box, drop
+//                                           r = 0.25 heavy inside, density 1000 : 1, viscosity
+//                                           1e-2 : 1e-4, sigma = 1, 3 MinRK2 iterations,
+//                                           mixed-precision multilevel PCG to 1e-8) through the
+//                                           classes in the reference loop's order; one warmup
+//                                           step, then a line "BENCH {json}" with ms/step and
+//                                           the host <-> device bytes per step
 //   dropin_driver ls <dir> <nx> <ny>        LevelSetSolver2D methods on <dir>/phi.bin, u.bin, v.bin
 //   dropin_driver csr <dir> <nx> <ny> <all_neumann 0|1>
 //                                           PoissonSolver2D CG + BiCGStab on a CSR system
@@ -23,6 +33,7 @@
 //                                           on <dir>/div.bin (all sides Neumann)
 //
 // Arrays are raw little-endian float64 (int32 for CSR indices) in the padded layout.
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <fstream>
@@ -59,7 +70,8 @@ void save(const std::string& path, const std::vector<T>& v) {
     f.write(reinterpret_cast<const char*>(v.data()), static_cast<std::streamsize>(v.size() * sizeof(T)));
 }
 
-int run_step(const std::string& dir, int n, int steps, int mode) {
+int run_step(const std::string& dir, int n, int steps, int mode, bool resident) {
+    asd2::SetDeviceResident(resident);
     const double rhoL = 1.226, muL = 1.78e-5, rhoH = 1000, muH = 1.137e-3, sigma = 0.0728, gConstant = 0.0;
     const double L = 1.0, U = 1.0, baseX = 0.0, baseY = 0.0, lenX = 0.04, lenY = 0.04, cfl = 0.3, radius = 0.01;
     const int nx = n, ny = n, num_bc_grid = 3, poissonMaxIter = 20000;
@@ -102,6 +114,7 @@ int run_step(const std::string& dir, int n, int steps, int mode) {
     LSolver->ApplyBC_P_2D(ls);
     LSolver->Reinit_MinRK2_2D(ls);
     LSolver->ApplyBC_P_2D(ls);
+    asd2::SyncHost(ls);  // resident: the host copy is refreshed only on request
     save(dir + "/ls0.bin", ls);
     MSolver->m_dt = cfl * std::min(dx, dy) / U;
     std::vector<double> uhat(N), vhat(N), div(N);
@@ -115,12 +128,20 @@ int run_step(const std::string& dir, int n, int steps, int mode) {
         LSolver->Reinit_MinRK2_2D(ls);
         LSolver->ApplyBC_P_2D(ls);
         MSolver->UpdateKappa(ls);
-        H = MSolver->UpdateHeavisideFunc(ls);
-        HSmooth = MSolver->UpdateSmoothHeavisideFunc(ls);
+        if (resident) {
+            MSolver->UpdateHeavisideFunc(ls, H);
+            MSolver->UpdateSmoothHeavisideFunc(ls, HSmooth);
+        } else {
+            H = MSolver->UpdateHeavisideFunc(ls);
+            HSmooth = MSolver->UpdateSmoothHeavisideFunc(ls);
+        }
         MSolver->GetIntermediateVel(LSolver, ls, MSolver->m_u, MSolver->m_v, uhat, vhat, HSmooth);
         MSolver->ApplyBC_U_2D(uhat);
         MSolver->ApplyBC_V_2D(vhat);
-        div = MSolver->GetDivergence(uhat, vhat);
+        if (resident)
+            MSolver->GetDivergence(uhat, vhat, div);
+        else
+            div = MSolver->GetDivergence(uhat, vhat);
         MSolver->ApplyBC_P_2D(MSolver->m_ps);
         LSolver->ApplyBC_P_2D(ls);
         MSolver->SolvePoisson(MSolver->m_ps, div, ls, lsB, MSolver->m_u, MSolver->m_v, H, poissonMaxIter);
@@ -134,7 +155,10 @@ int run_step(const std::string& dir, int n, int steps, int mode) {
         MSolver->m_curTime += MSolver->m_dt;
         std::fill(uhat.begin(), uhat.end(), 0.0);
         std::fill(vhat.begin(), vhat.end(), 0.0);
+        asd2::ModifiedHost(uhat);  // resident: the next call uploads the zeros
+        asd2::ModifiedHost(vhat);
     }
+    asd2::SyncHostAll();
     dts.push_back(MSolver->m_dt);
     save(dir + "/ls.bin", ls);
     save(dir + "/u.bin", MSolver->m_u);
@@ -144,6 +168,98 @@ int run_step(const std::string& dir, int n, int steps, int mode) {
     save(dir + "/kappa.bin", MSolver->m_kappa);
     save(dir + "/dt.bin", dts);
     save(dir + "/iters.bin", iters);
+    return 0;
+}
+
+int run_bench(int n, int steps, bool resident) {
+    asd2::SetDeviceResident(resident);
+    const double rhoH = 1000.0, rhoL = 1.0, muH = 1.0e-2, muL = 1.0e-4, sigma = 1.0, cfl = 0.3, radius = 0.25;
+    const int nx = n, ny = n, g = 3, poissonMaxIter = 100000;
+    const double lenX = 1.0, lenY = 1.0, dx = lenX / nx, dy = lenY / ny;
+    auto MSolver = std::make_unique<MACSolver2D>(rhoH, rhoL, muH, muL, 0.0, GAXISENUM2D::Y, 1.0, 1.0, sigma, nx, ny,
+                                                 0.0, 0.0, lenX, lenY, TIMEORDERENUM::RK3, cfl, 1.0e9, 1 << 30, 1, g,
+                                                 false);
+    MSolver->SetBC_U_2D("dirichlet", "dirichlet", "dirichlet", "dirichlet");
+    MSolver->SetBC_V_2D("dirichlet", "dirichlet", "dirichlet", "dirichlet");
+    MSolver->SetBC_P_2D("neumann", "neumann", "neumann", "neumann");
+    MSolver->SetPLTType(PLTTYPE::NONE);
+    MSolver->SetPoissonSolver(POISSONTYPE::CG);
+    MSolver->SetPoissonOptions(1e-8, ASD2_CG_STANDARD, ASD2_PREC_MIXED, ASD2_PRECOND_MULTILEVEL);
+    // 3 MinRK2 iterations: a pseudo end time of 2.5 pseudo steps (eps / 2 each)
+    const double eps = std::min(dx, dy);
+    auto LSolver = std::make_shared<LevelSetSolver2D>(nx, ny, g, 0.0, 0.0, dx, dy, 1.25 * eps);
+    if (LSolver->ReinitIterations() != 3) {
+        std::cerr << "reinit iterations " << LSolver->ReinitIterations() << ", expected 3" << std::endl;
+        return 1;
+    }
+    LSolver->SetBC_P_2D("neumann", "neumann", "neumann", "neumann");
+    const size_t N = static_cast<size_t>((nx + 2 * g) * (ny + 2 * g));
+    std::vector<double> lsB(N, 0.0), ls(N, 0.0), H(N, 0.0), HSmooth(N, 0.0), uhat(N, 0.0), vhat(N, 0.0), div(N, 0.0);
+    MSolver->AllocateVariables();
+    for (int i = 0; i < nx + 2 * g; i++)
+        for (int j = 0; j < ny + 2 * g; j++) {
+            const double x = (i + 0.5 - g) * dx, y = (j + 0.5 - g) * dy;
+            ls[LSolver->idx(i, j)] = radius - std::sqrt((x - 0.5) * (x - 0.5) + (y - 0.5) * (y - 0.5));
+        }
+    LSolver->ApplyBC_P_2D(ls);
+    LSolver->Reinit_MinRK2_2D(ls);
+    LSolver->ApplyBC_P_2D(ls);
+    MSolver->m_dt = cfl * std::min(dx, dy);
+    std::vector<int> iters;
+    auto step = [&]() {
+        MSolver->m_iter++;
+        if (!resident) lsB = ls;  // read by neither class (SolvePoisson and UpdateVel ignore lsB)
+        LSolver->Solve_LevelSet_2D(ls, MSolver->m_u, MSolver->m_v, MSolver->m_dt);
+        LSolver->ApplyBC_P_2D(ls);
+        LSolver->Reinit_MinRK2_2D(ls);
+        LSolver->ApplyBC_P_2D(ls);
+        MSolver->UpdateKappa(ls);
+        if (resident) {
+            MSolver->UpdateHeavisideFunc(ls, H);
+            MSolver->UpdateSmoothHeavisideFunc(ls, HSmooth);
+        } else {
+            H = MSolver->UpdateHeavisideFunc(ls);
+            HSmooth = MSolver->UpdateSmoothHeavisideFunc(ls);
+        }
+        MSolver->GetIntermediateVel(LSolver, ls, MSolver->m_u, MSolver->m_v, uhat, vhat, HSmooth);
+        MSolver->ApplyBC_U_2D(uhat);
+        MSolver->ApplyBC_V_2D(vhat);
+        if (resident)
+            MSolver->GetDivergence(uhat, vhat, div);
+        else
+            div = MSolver->GetDivergence(uhat, vhat);
+        MSolver->ApplyBC_P_2D(MSolver->m_ps);
+        LSolver->ApplyBC_P_2D(ls);
+        MSolver->SolvePoisson(MSolver->m_ps, div, ls, lsB, MSolver->m_u, MSolver->m_v, H, poissonMaxIter);
+        MSolver->ApplyBC_P_2D(MSolver->m_ps);
+        MSolver->UpdateVel(MSolver->m_u, MSolver->m_v, uhat, vhat, MSolver->m_ps, ls, lsB, H);
+        MSolver->ApplyBC_U_2D(MSolver->m_u);
+        MSolver->ApplyBC_V_2D(MSolver->m_v);
+        MSolver->m_dt = MSolver->UpdateDt(MSolver->m_u, MSolver->m_v);
+        MSolver->m_curTime += MSolver->m_dt;
+        iters.push_back(MSolver->LastPoissonIterations());
+    };
+    step();  // warmup
+    const asd2::TransferCounts c0 = asd2::Transfers();
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int s = 0; s < steps; s++) step();
+    const auto t1 = std::chrono::steady_clock::now();
+    const asd2::TransferCounts c1 = asd2::Transfers();
+    const double ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / steps;
+    double it = 0.0;
+    for (int s = 1; s <= steps; s++) it += iters[s];
+    asd2::SyncHostAll();
+    double mass = 0.0;
+    for (int i = g; i < nx + g; i++)
+        for (int j = g; j < ny + g; j++) mass += ls[LSolver->idx(i, j)] >= 0.0 ? dx * dy : 0.0;
+    std::printf("BENCH {\"n\": %d, \"steps\": %d, \"resident\": %d, \"ms_per_step\": %.4f, \"pcg_iters_per_step\": %.2f, "
+                "\"upload_MB_per_step\": %.3f, \"download_MB_per_step\": %.3f, \"uploads_per_step\": %.2f, "
+                "\"downloads_per_step\": %.2f, \"mirrors\": %lld, \"mass\": %.12f, \"dt\": %.12e}\n",
+                n, steps, resident ? 1 : 0, ms, it / steps, (c1.upload_bytes - c0.upload_bytes) / 1e6 / steps,
+                (c1.download_bytes - c0.download_bytes) / 1e6 / steps,
+                static_cast<double>(c1.uploads - c0.uploads) / steps,
+                static_cast<double>(c1.downloads - c0.downloads) / steps, c1.mirrors, mass, MSolver->m_dt);
+    std::fflush(stdout);
     return 0;
 }
 
@@ -284,7 +400,9 @@ int main(int argc, char** argv) {
         return 2;
     }
     const std::string cmd = argv[1], dir = argv[2];
-    if (cmd == "step" && argc == 6) return run_step(dir, std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]));
+    if (cmd == "step" && (argc == 6 || argc == 7))
+        return run_step(dir, std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]), argc == 7 && std::atoi(argv[6]));
+    if (cmd == "bench" && argc == 5) return run_bench(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]) != 0);
     if (cmd == "ls" && argc == 5) return run_ls(dir, std::atoi(argv[3]), std::atoi(argv[4]));
     if (cmd == "csr" && argc == 6) return run_csr(dir, std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]) != 0);
     if (cmd == "outres" && argc == 5) return run_outres(dir, std::atoi(argv[3]), std::atoi(argv[4]));

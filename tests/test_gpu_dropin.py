@@ -252,3 +252,68 @@ def test_outres_tecplot_ascii(oracle, tmp_path):
                 dv.append(",".join(num(a) for a in (x, y, ls[i, j], div[i, j], ps[i, j])))
     assert (tmp_path / "vel_ASCII.plt").read_text().splitlines() == vel
     assert (tmp_path / "div_ASCII.plt").read_text().splitlines() == dv
+
+
+def _arrays(d, n):
+    shp = (n + 6, n + 6)
+    out = {k: _load(d, f"{k}.bin", shp) for k in ("ls0", "ls", "u", "v", "ps", "p", "kappa")}
+    out["dt"] = _load(d, "dt.bin")
+    out["iters"] = _load(d, "iters.bin", dtype=np.int32)
+    return out
+
+
+def test_resident_step_matches_host_step(tmp_path):
+    """The stationary-bubble loop with the arrays kept on the device between calls (resident mode,
+    include/asd2_resident.h: the in-place H / HSmooth / div overloads, ModifiedHost after the
+    driver zeroes uhat / vhat on the host, SyncHostAll before the fields are saved) runs the same
+    kernels on the same data as the default mode, so every saved array is bit-identical."""
+    n, steps = 64, 3
+    host, dev = tmp_path / "host", tmp_path / "resident"
+    host.mkdir()
+    dev.mkdir()
+    _run("step", host, n, steps, 1, 0)
+    _run("step", dev, n, steps, 1, 1)
+    a, b = _arrays(host, n), _arrays(dev, n)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def _bench(n, steps, resident):
+    out = _run("bench", n, steps, resident)
+    line = [l for l in out.splitlines() if l.startswith("BENCH ")][-1]
+    import json
+    return json.loads(line[len("BENCH "):])
+
+
+def test_resident_bench_moves_no_fields():
+    """The benchmark's droplet step through the classes (tests/cpp/dropin_driver.cpp bench mode):
+    resident mode moves no array between host and device after the warmup step, and it computes
+    what the default mode computes (same mass, same next dt, same iteration counts)."""
+    n, steps = 128, 3
+    host = _bench(n, steps, 0)
+    dev = _bench(n, steps, 1)
+    print("default", host)
+    print("resident", dev)
+    assert dev["upload_MB_per_step"] == 0.0 and dev["download_MB_per_step"] == 0.0
+    assert host["upload_MB_per_step"] > 0.0  # the default mode uploads every input it reads
+    assert dev["mass"] == host["mass"] and dev["dt"] == host["dt"]
+    assert dev["pcg_iters_per_step"] == host["pcg_iters_per_step"]
+
+
+def test_fill_and_pressure():
+    """asd2_fill (the classes' whole-array host loops on the device) and asd2_mac_pressure
+    (src/mac2d.cpp:1722-1724: p = ps / dt on the interior, ghosts of p kept) against numpy."""
+    import asd2_amd as A
+    nx, ny, g = 37, 29, 3
+    ctx = A.Context(nx, ny, 1.0 / nx, 1.0 / nx)
+    rng = np.random.default_rng(4)
+    shp = (nx + 2 * g, ny + 2 * g)
+    ps, p0 = rng.standard_normal(shp), rng.standard_normal(shp)
+    d_ps, d_p = ctx.array(ps), ctx.array(p0)
+    dt = 3.7e-3
+    ctx.pressure(d_ps, d_p, dt)
+    want = p0.copy()
+    want[g:g + nx, g:g + ny] = ps[g:g + nx, g:g + ny] / dt
+    assert np.array_equal(d_p.download(), want)
+    ctx.fill(d_ps, -2.5)
+    assert np.all(d_ps.download() == -2.5)
