@@ -1263,7 +1263,8 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
                                                 const float* __restrict__ r, const float* __restrict__ dinv,
                                                 const float* __restrict__ y, Dirs Dd, KrylovState* st,
                                                 const double* pb, int nbb, const double* pc, int nbc, double* ps,
-                                                int it, int nwin, double gamma_fixed, const double* pu, int nbu) {
+                                                int it, int nwin, double gamma_fixed, const double* pu, int nbu,
+                                                const double* fine_totals) {
     if (st->done != DONE_RUNNING) return;
     constexpr int FI = 32;  // tile rows (FI x TJ tiles)
     // The tile operands (r, D^-1, y, the kept directions) of a block's first tile are loaded into
@@ -1299,9 +1300,20 @@ __global__ __launch_bounds__(KB) void k_lp_spmv(Geom G, Coarse C, const float* _
     };
     if (t0 < t1) load(t0);
     // every block re-sums the rows this iteration needs: r.r of the kernel that made r; r.D^-1 r
-    // and the MW (D^-1 r).q_j of the fine pass; y.g, y.A_c y, y.P^T r and the MW y.P^T q_j
+    // and the MW (D^-1 r).q_j of the fine pass; y.g, y.A_c y, y.P^T r and the MW y.P^T q_j.  The
+    // fine rows may come summed already (fine_totals, by a kernel between the pass and this one,
+    // in the same order), which leaves the coarse rows.
     double v[1 + (1 + MW) + (3 + MW)];
-    sum_partials3<1, 1 + MW, 3 + MW>(pu, nbu, pb, nbb, y ? pc : nullptr, nbc, v);
+    if (fine_totals) {
+        double w[3 + MW];
+        sum_partials3<0, 0, 3 + MW>(nullptr, 0, nullptr, 0, y ? pc : nullptr, nbc, w);
+#pragma unroll
+        for (int k = 0; k < 2 + MW; k++) v[k] = fine_totals[k];
+#pragma unroll
+        for (int k = 0; k < 3 + MW; k++) v[2 + MW + k] = w[k];
+    } else {
+        sum_partials3<1, 1 + MW, 3 + MW>(pu, nbu, pb, nbb, y ? pc : nullptr, nbc, v);
+    }
     {
         // the decision of iteration `it`
         const double rn = sqrt(v[0]), resid = rn / st->bnorm;
@@ -1401,10 +1413,10 @@ template <int MW>
 void launch_lp_spmv(hipStream_t s, int nb, Geom G, Coarse C, const float* ax, const float* ay, const float* dres,
                     int ring_only, const float* r, const float* dinv, const float* y, Dirs Dd, KrylovState* st,
                     const double* pb, int nbb, const double* pc, int nbc, double* ps, int it, int nwin,
-                    double gamma_fixed, const double* pu, int nbu) {
+                    double gamma_fixed, const double* pu, int nbu, const double* fine_totals) {
 #define ASD2_LP_SPMV(CV)                                                                                              \
     hipLaunchKernelGGL((k_lp_spmv<MW, CV>), dim3(nb), dim3(KB), 0, s, G, C, ax, ay, dres, ring_only, r, dinv, y, Dd, \
-                       st, pb, nbb, pc, nbc, ps, it, nwin, gamma_fixed, pu, nbu)
+                       st, pb, nbb, pc, nbc, ps, it, nwin, gamma_fixed, pu, nbu, fine_totals)
     switch (C.c) {
         case 1: ASD2_LP_SPMV(1); break;
         case 2: ASD2_LP_SPMV(2); break;
@@ -1804,10 +1816,28 @@ __global__ __launch_bounds__(T * T) void k_ml_up(Level L, Level next, float* __r
     }
 }
 
-// y = A^-1 g on the coarsest level (FP64 inverse): one row per wave, the lanes splitting it
+// the fine pass's scalar rows for the SpMV (r.r, r.D^-1 r, (D^-1 r).q_1), summed once in the order
+// every SpMV block would use
+struct FineRows {
+    const double* pu;
+    int nbu;
+    const double* pb;
+    int nbb;
+    double* out;  // nullptr: not summed here
+};
+
+// y = A^-1 g on the coarsest level (FP64 inverse): one row per wave, the lanes splitting it; with
+// F.out set, one more block sums the fine rows (the pass that wrote them has finished)
 __global__ __launch_bounds__(KB) void k_ml_dense_apply(Level L, const double* __restrict__ Minv,
-                                                       const KrylovState* st) {
+                                                       const KrylovState* st, FineRows F) {
     if (st->done != DONE_RUNNING) return;
+    if (F.out && blockIdx.x == gridDim.x - 1) {
+        double v[3];
+        sum_partials3<1, 2, 0>(F.pu, F.nbu, F.pb, F.nbb, nullptr, 0, v);
+        if (threadIdx.x == 0)
+            for (int k = 0; k < 3; k++) F.out[k] = v[k];
+        return;
+    }
     const int m = L.H * L.W, lane = threadIdx.x & 63;
     const int p = blockIdx.x * (KB / 64) + (threadIdx.x >> 6);  // one row per wave
     if (p >= m) return;
@@ -2030,6 +2060,8 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
         }
         ASD2_TRY(ws(ctx, "ml_yout", Pc, &yml));
     }
+    double* fine_totals = nullptr;  // the fine rows of the SpMV, summed by the dense-apply launch
+    ASD2_TRY(ws(ctx, "lp_fine_totals", 4, &fine_totals));
     double* pb = lp_part;  // fine pass: r.D^-1 r, (D^-1 r).q_j, sum (g / dc)^2
     double* pc = lp_part + (2 + FCG_M) * MAX_BLOCKS;  // coarse dots: y.g, y.A_c y, y.P^T r, y.P^T q_j
     const int nbs = spmv_blocks(G), nbf = flat_blocks(n), nbi = interior_blocks(G);
@@ -2198,8 +2230,10 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
                         ASD2_TRY(record(l + 1, it));
                     }
                     if (first == Hh.n - 1)
-                        hipLaunchKernelGGL(k_ml_dense_apply, dim3(ceil_div(Hh.L[first].H * Hh.L[first].W, KB / 64)),
-                                           dim3(KB), 0, ctx->stream, Hh.L[first], Minv, D.st);
+                        hipLaunchKernelGGL(k_ml_dense_apply,
+                                           dim3(ceil_div(Hh.L[first].H * Hh.L[first].W, KB / 64) + 1), dim3(KB), 0,
+                                           ctx->stream, Hh.L[first], Minv, D.st,
+                                           FineRows{D.pu, nbu, pb, nbb, many ? nullptr : fine_totals});
                     else
                         hipLaunchKernelGGL(k_ml_lower, dim3(1), dim3(1024), 0, ctx->stream, Hh, first, Minv, D.st, ch);
                     for (int l = first - 1; l >= 0; l--) {
@@ -2227,7 +2261,7 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
                 ctx->stream, nbs, G, C, S.ax32, S.ay32, S.diag32, ctx->dres_ring_only ? 1 : 0, r, S.dinv32,
                 use_net ? ycoarse : nullptr,
                 dirs_for(R, it), D.st, pb, nbb, pc, nbc, D.ps, it, nwin(it), (learned && !hybrid) ? 0.0 : 1.0, D.pu,
-                nbu);
+                nbu, (use_net && !learned && Hh.n > 1 && first == Hh.n - 1 && !many) ? fine_totals : nullptr);
             prof_end(ctx, ASD2_KID_FCG_SPMV);
             prof_begin(ctx, ASD2_KID_FCG_UPDATE);
             (many ? launch_lp_pass<true, FCG_M> : launch_lp_pass<true, 1>)(ctx->stream, nbb, G, C, r, rb[(it + 1) & 1], x, dirs_for(R, it + 1), S.dinv32, S.ax32, S.ay32,
