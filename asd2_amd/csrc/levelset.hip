@@ -19,31 +19,34 @@ constexpr int WI = 32;   // WENO tile height along i
 constexpr int WTH = 256; // 64 x 4 threads
 constexpr int WTW = 512; // WENO block: 64 x 8 threads, 4 tile rows each
 
-// cell-centred velocity of Solve_LevelSet_2D with the P BC on its ghosts
-// (src/levelset2d.cpp:47-57); CUSTOM sides leave the ghost at 0.
-__device__ __forceinline__ double uc_face(const Geom& G, const GhostRule& R, const double* u, int i, int j) {
-    const int g = G.g;
-    if (i >= g && i < G.nx + g) return (u[G.idx(i, j)] + u[G.idx(i + 1, j)]) * 0.5;
-    const int s = (i < g) ? 0 : 1;
-    const int m = (i < g) ? (2 * g - 1 - i) : (2 * G.nx + 2 * g - 1 - i);
-    if (R.kind[s] == 2) return 0.0;
-    const double x = (u[G.idx(m, j)] + u[G.idx(m + 1, j)]) * 0.5;
-    return R.kind[s] == 0 ? x : (-x + R.two_c[s]);
+// Cell-centred velocity of Solve_LevelSet_2D with the P BC on its ghosts (src/levelset2d.cpp:47-57),
+// split into its loads and its arithmetic so that a kernel can issue every load of a tile before it
+// waits for any.  Along one axis of n interior cells: the source cell of position i (itself inside,
+// its mirror on a ghost), and the value from the two face velocities at the source and the next
+// face; CUSTOM sides leave the ghost at 0.
+__device__ __forceinline__ int centre_src(int i, int g, int n) {
+    return i < g ? 2 * g - 1 - i : (i >= n + g ? 2 * n + 2 * g - 1 - i : i);
 }
-
-__device__ __forceinline__ double vc_face(const Geom& G, const GhostRule& R, const double* v, int i, int j) {
-    const int g = G.g;
-    if (j >= g && j < G.ny + g) return (v[G.idx(i, j)] + v[G.idx(i, j + 1)]) * 0.5;
-    const int s = (j < g) ? 2 : 3;
-    const int m = (j < g) ? (2 * g - 1 - j) : (2 * G.ny + 2 * g - 1 - j);
-    if (R.kind[s] == 2) return 0.0;
-    const double x = (v[G.idx(i, m)] + v[G.idx(i, m + 1)]) * 0.5;
-    return R.kind[s] == 0 ? x : (-x + R.two_c[s]);
+template <int SLO>
+__device__ __forceinline__ double centre_vel(const GhostRule& R, int i, int g, int n, double f0, double f1) {
+    const double x = (f0 + f1) * 0.5;
+    if (i >= g && i < n + g) return x;
+    const bool lo = i < g;  // the side's rule, read with constant indices (kernel-argument registers)
+    const int kind = lo ? R.kind[SLO] : R.kind[SLO + 1];
+    const double tc = lo ? R.two_c[SLO] : R.two_c[SLO + 1];
+    if (kind == 2) return 0.0;
+    return kind == 0 ? x : (-x + tc);
 }
 
 // STAGE 0: write L (HJWENO5_LS_2D); 1..3: SSP-RK3 stages of Solve_LevelSet_2D.
-// CENTRED: velocities given cell-centred (with ghosts) instead of on faces.
-template <int STAGE, bool CENTRED>
+// CENTRED: velocities given cell-centred (with ghosts) instead of on faces.  POW2: dx and dy are
+// powers of two.  The WENO derivatives, L and the stage update follow the reference's operation
+// order without contraction, so a stage reproduces the reference bit for bit.
+// A block owns a WI x WJ tile; the x fluxes phi u_c need 3 halo rows above and below it, the y fluxes
+// phi v_c 3 halo columns on each side.  Every global load of a thread (its NR tile cells: phi, the
+// velocity sources, phi^n; one x-halo and one y-halo cell) is issued at addresses clamped into the
+// array before the first value is used.
+template <int STAGE, bool CENTRED, bool POW2>
 __global__ __launch_bounds__(WTW) void k_weno(Geom G, GhostRule R, const double* __restrict__ phi,
                                               const double* phin, const double* __restrict__ u,
                                               const double* __restrict__ v, double* out, double dt,
@@ -58,80 +61,111 @@ __global__ __launch_bounds__(WTW) void k_weno(Geom G, GhostRule R, const double*
     const int iend = G.nx + g, jend = G.ny + g;
     const int ipad = G.nx + 2 * g, jpad = G.ny + 2 * g;
 
-    // Each thread owns the output rows o = ti + 4k of its column and loads their phi and cell-centred
-    // velocities once: the products go to the flux tiles, the values stay in registers for the
-    // stage update (a second read after the tile phase would miss L2 at these tile counts).
+    // Each thread owns the output rows o = ti + 8k of its column; phi and the cell-centred velocities
+    // stay in registers for the stage update after the flux phase.
     constexpr int NR = WI / (WTW / WJ);
-    double phiR[NR], ucR[NR], vcR[NR];
+    double phiR[NR], ua[NR], ub[NR], va[NR], vb[NR], pn[NR];
+    int okR = 0;
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        const int i = i0 + ti + (WTW / WJ) * k, j = j0 + tj;
+        const bool ok = i < ipad && j < jpad;
+        okR |= ok ? (1 << k) : 0;
+        const int ic = ok ? i : g, jc = ok ? j : g;
+        phiR[k] = phi[G.idx(ic, jc)];
+        if (CENTRED) {
+            ua[k] = u[G.idx(ic, jc)];
+            va[k] = v[G.idx(ic, jc)];
+            ub[k] = vb[k] = 0.0;
+        } else {
+            const int mi = centre_src(ic, g, G.nx), mj = centre_src(jc, g, G.ny);
+            ua[k] = u[G.idx(mi, jc)];
+            ub[k] = u[G.idx(mi + 1, jc)];
+            va[k] = v[G.idx(ic, mj)];
+            vb[k] = v[G.idx(ic, mj + 1)];
+        }
+        pn[k] = (STAGE >= 2) ? phin[G.idx(ic, jc)] : 0.0;
+    }
+    // x-flux halo: rows i0 - 3 .. i0 - 1 and i0 + WI .. i0 + WI + 2 (one cell per thread < 6 WJ)
+    const int ex = threadIdx.x;
+    const int hxr = (ex / WJ) < 3 ? (ex / WJ) : WI + (ex / WJ);
+    const int hxi = i0 - 3 + hxr, hxj = j0 + (ex & (WJ - 1));
+    const bool hxok = ex < 6 * WJ && hxj < jend && hxi < ipad;
+    const int hxic = hxok ? hxi : g, hxjc = hxok ? hxj : g;
+    const double hxp = phi[G.idx(hxic, hxjc)];
+    const int hxm = CENTRED ? hxic : centre_src(hxic, g, G.nx);
+    const double hx0 = u[G.idx(hxm, hxjc)];
+    const double hx1 = CENTRED ? 0.0 : u[G.idx(hxm + 1, hxjc)];
+    // y-flux halo: columns j0 - 3 .. j0 - 1 and j0 + WJ .. j0 + WJ + 2 (one cell per thread < 6 WI)
+    const int ey = threadIdx.x;
+    const int hyr = ey / 6, hyh = ey - hyr * 6;
+    const int hyi = i0 + hyr, hyj = j0 - 3 + (hyh < 3 ? hyh : WJ + hyh);
+    const bool hyok = ey < 6 * WI && hyi < iend && hyj < jpad;
+    const int hyic = hyok ? hyi : g, hyjc = hyok ? hyj : g;
+    const double hyp = phi[G.idx(hyic, hyjc)];
+    const int hym = CENTRED ? hyjc : centre_src(hyjc, g, G.ny);
+    const double hy0 = v[G.idx(hyic, hym)];
+    const double hy1 = CENTRED ? 0.0 : v[G.idx(hyic, hym + 1)];
+
+    double ucR[NR], vcR[NR];
 #pragma unroll
     for (int k = 0; k < NR; k++) {
         const int o = ti + (WTW / WJ) * k, i = i0 + o, j = j0 + tj;
-        double ph = 0.0, uc = 0.0, vc = 0.0;
-        if (i < ipad && j < jpad) {
-            ph = phi[G.idx(i, j)];
-            uc = CENTRED ? u[G.idx(i, j)] : uc_face(G, R, u, i, j);
-            vc = CENTRED ? v[G.idx(i, j)] : vc_face(G, R, v, i, j);
-        }
+        const bool ok = (okR >> k) & 1;
+        const double ph = ok ? phiR[k] : 0.0;
+        const double uc = !ok ? 0.0 : CENTRED ? ua[k] : centre_vel<0>(R, i, g, G.nx, ua[k], ub[k]);
+        const double vc = !ok ? 0.0 : CENTRED ? va[k] : centre_vel<2>(R, j, g, G.ny, va[k], vb[k]);
         sFx[o + 3][tj] = (j < jend && i < ipad) ? ph * uc : 0.0;
         sFy[o][tj + 3] = (i < iend && j < jpad) ? ph * vc : 0.0;
         phiR[k] = ph;
         ucR[k] = uc;
         vcR[k] = vc;
     }
-    // the three halo rows above and below (x fluxes) and the three halo columns on each side
-    // (y fluxes)
-    for (int e = threadIdx.x; e < 6 * WJ; e += WTW) {
-        const int h = e / WJ, c = e - h * WJ;
-        const int r = h < 3 ? h : WI + h;
-        const int i = i0 - 3 + r, j = j0 + c;
-        double f = 0.0;
-        if (j < jend && i < ipad) {
-            const double vel = CENTRED ? u[G.idx(i, j)] : uc_face(G, R, u, i, j);
-            f = phi[G.idx(i, j)] * vel;
-        }
-        sFx[r][c] = f;
+    if (ex < 6 * WJ) {
+        const double vel = CENTRED ? hx0 : centre_vel<0>(R, hxi, g, G.nx, hx0, hx1);
+        sFx[hxr][ex & (WJ - 1)] = hxok ? hxp * vel : 0.0;
     }
-    for (int e = threadIdx.x; e < WI * 6; e += WTW) {
-        const int r = e / 6, h = e - r * 6;
-        const int c = h < 3 ? h : WJ + h;
-        const int i = i0 + r, j = j0 - 3 + c;
-        double f = 0.0;
-        if (i < iend && j < jpad) {
-            const double vel = CENTRED ? v[G.idx(i, j)] : vc_face(G, R, v, i, j);
-            f = phi[G.idx(i, j)] * vel;
-        }
-        sFy[r][c] = f;
+    if (ey < 6 * WI) {
+        const double vel = CENTRED ? hy0 : centre_vel<2>(R, hyj, g, G.ny, hy0, hy1);
+        sFy[hyr][hyh < 3 ? hyh : WJ + hyh] = hyok ? hyp * vel : 0.0;
     }
     __syncthreads();
 
+    // one row at a time (interleaving the rows' division chains costs more registers than it saves)
     const double idx_ = 1.0 / G.dx, idy_ = 1.0 / G.dy;
-    const double epsx = 1e-6 * G.dx * G.dx, epsy = 1e-6 * G.dy * G.dy;
+    double val[NR];
 #pragma unroll
     for (int k = 0; k < NR; k++) {
         const int r = ti + (WTW / WJ) * k;
-        const int i = i0 + r, j = j0 + tj;
-        if (i >= iend || j >= jend) continue;
+        if (i0 + r >= iend || j0 + tj >= jend) continue;
         double lxp, lxm, lyp, lym;
         {
             const double f0 = sFx[r][tj], f1 = sFx[r + 1][tj], f2 = sFx[r + 2][tj], f3 = sFx[r + 3][tj];
             const double f4 = sFx[r + 4][tj], f5 = sFx[r + 5][tj], f6 = sFx[r + 6][tj];
-            weno5(f1 - f0, f2 - f1, f3 - f2, f4 - f3, f5 - f4, f6 - f5, idx_, epsx, lxp, lxm);
+            weno5_ref<POW2>(f1 - f0, f2 - f1, f3 - f2, f4 - f3, f5 - f4, f6 - f5, G.dx, idx_, lxp, lxm);
         }
         {
             const double f0 = sFy[r][tj], f1 = sFy[r][tj + 1], f2 = sFy[r][tj + 2], f3 = sFy[r][tj + 3];
             const double f4 = sFy[r][tj + 4], f5 = sFy[r][tj + 5], f6 = sFy[r][tj + 6];
-            weno5(f1 - f0, f2 - f1, f3 - f2, f4 - f3, f5 - f4, f6 - f5, idy_, epsy, lyp, lym);
+            weno5_ref<POW2>(f1 - f0, f2 - f1, f3 - f2, f4 - f3, f5 - f4, f6 - f5, G.dy, idy_, lyp, lym);
         }
-        const int64_t c = G.idx(i, j);
         const double ax = ucR[k], ay = vcR[k], phc = phiR[k];
-        const double L = 0.5 * (lxp + lxm) + 0.5 * (lyp + lym) - ax * (0.5 * (lxp - lxm)) - ay * (0.5 * (lyp - lym));
-        double val;
-        if (STAGE == 0) val = L;
-        else if (STAGE == 1) val = phc - dt * L;
-        else if (STAGE == 2) val = 0.75 * phin[c] + 0.25 * phc - 0.25 * dt * L;
-        else val = 1.0 / 3.0 * phin[c] + 2.0 / 3.0 * phc - 2.0 / 3.0 * dt * L;
-        out[c] = val;
-        if (ghosts) write_p_ghosts(G, R, out, i, j, val);
+        {
+#pragma clang fp contract(off)
+            const double L =
+                0.5 * (lxp + lxm) + 0.5 * (lyp + lym) - ax * (0.5 * (lxp - lxm)) - ay * (0.5 * (lyp - lym));
+            if (STAGE == 0) val[k] = L;
+            else if (STAGE == 1) val[k] = phc - dt * L;
+            else if (STAGE == 2) val[k] = 0.75 * pn[k] + 0.25 * phc - 0.25 * dt * L;
+            else val[k] = 1.0 / 3.0 * pn[k] + 2.0 / 3.0 * phc - 2.0 / 3.0 * dt * L;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        const int i = i0 + ti + (WTW / WJ) * k, j = j0 + tj;
+        if (i >= iend || j >= jend) continue;
+        out[G.idx(i, j)] = val[k];
+        if (ghosts) write_p_ghosts(G, R, out, i, j, val[k]);
     }
 }
 
@@ -152,13 +186,35 @@ __global__ __launch_bounds__(256) void k_centre_velocity(Geom G, const double* _
 constexpr int RJ = 64, RI = 16, RH = 2;  // reinit tile and halo
 constexpr int RW = RJ + 2 * RH;
 
-__device__ __forceinline__ void load_tile2(const Geom& G, const double* __restrict__ src,
-                                           double (*s)[RW], int i0, int j0) {
+// Loads the TH x TW tiles of a and b whose top-left cell is (ib, jb) into sa and sb, 0 outside the
+// padded array.  Every load of a thread is issued before the first LDS write (clamped addresses,
+// no branch around a load), so a block waits for one memory latency instead of one per pass of a
+// guarded load loop.
+template <int TH, int TW, int NT>
+__device__ __forceinline__ void tile_load2(const Geom& G, const double* __restrict__ a, const double* __restrict__ b,
+                                           double (*sa)[TW], double (*sb)[TW], int ib, int jb) {
+    constexpr int NE = TH * TW, NL = (NE + NT - 1) / NT;
     const int ipad = G.nx + 2 * G.g, jpad = G.ny + 2 * G.g;
-    for (int e = threadIdx.x; e < (RI + 2 * RH) * RW; e += WTH) {
-        const int r = e / RW, c = e - r * RW;
-        const int i = i0 - RH + r, j = j0 - RH + c;
-        s[r][c] = (i < ipad && j < jpad) ? src[G.idx(i, j)] : 0.0;
+    double va[NL], vb[NL];
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        const int e = threadIdx.x + k * NT;
+        const int r = e / TW, c = e - r * TW;
+        const int i = ib + r, j = jb + c;
+        const bool in = e < NE && i >= 0 && i < ipad && j >= 0 && j < jpad;
+        const int64_t o = in ? G.idx(i, j) : 0;
+        const double x = a[o], y = b[o];
+        va[k] = in ? x : 0.0;
+        vb[k] = in ? y : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        const int e = threadIdx.x + k * NT;
+        if (NE % NT == 0 || e < NE) {
+            const int r = e / TW, c = e - r * TW;
+            sa[r][c] = va[k];
+            sb[r][c] = vb[k];
+        }
     }
 }
 
@@ -254,8 +310,7 @@ __global__ __launch_bounds__(WTH) void k_absgrad(Geom G, const double* __restric
     __shared__ double sL[RI + 2 * RH][RW];
     __shared__ double sI[RI + 2 * RH][RW];
     const int j0 = G.g + blockIdx.x * RJ, i0 = G.g + blockIdx.y * RI;
-    load_tile2(G, ls, sL, i0, j0);
-    load_tile2(G, ls0, sI, i0, j0);
+    tile_load2<RI + 2 * RH, RW, WTH>(G, ls, ls0, sL, sI, i0 - RH, j0 - RH);
     __syncthreads();
     const int tj = threadIdx.x & (RJ - 1), ti = threadIdx.x / RJ;
     const double idx_ = 1.0 / G.dx, idy_ = 1.0 / G.dy;
@@ -279,8 +334,7 @@ __global__ __launch_bounds__(WTH) void k_reinit(Geom G, GhostRule R, const doubl
     __shared__ double sL[RI + 2 * RH][RW];
     __shared__ double sI[RI + 2 * RH][RW];
     const int j0 = G.g + blockIdx.x * RJ, i0 = G.g + blockIdx.y * RI;
-    load_tile2(G, src, sL, i0, j0);
-    load_tile2(G, ls0, sI, i0, j0);
+    tile_load2<RI + 2 * RH, RW, WTH>(G, src, ls0, sL, sI, i0 - RH, j0 - RH);
     __syncthreads();
     const int tj = threadIdx.x & (RJ - 1), ti = threadIdx.x / RJ;
     const double idx_ = 1.0 / G.dx, idy_ = 1.0 / G.dy;
@@ -301,6 +355,112 @@ __global__ __launch_bounds__(WTH) void k_reinit(Geom G, GhostRule R, const doubl
         }
         out[G.idx(i, j)] = val;
         if (ghosts) write_p_ghosts(G, R, out, i, j, val);
+    }
+}
+
+// |grad phi| with the subcell fix from a phi tile and a phi0 tile of different halos: (RL, CL) and
+// (RI_, CI) are the cell's positions in the two tiles.  The same arithmetic as absgrad_at<true>.
+template <int LW, int IW>
+__device__ __forceinline__ double absgrad_sub2(const double (*sL)[LW], int RL, int CL, const double (*sI)[IW], int RI_,
+                                               int CI, double dx, double dy, double idx_, double idy_) {
+    double dPX, dMX, dPY, dMY;
+    const double l0 = sL[RL][CL];
+    eno2(sL[RL - 2][CL], sL[RL - 1][CL], l0, sL[RL + 1][CL], sL[RL + 2][CL], idx_, dPX, dMX);
+    eno2(sL[RL][CL - 2], sL[RL][CL - 1], l0, sL[RL][CL + 1], sL[RL][CL + 2], idy_, dPY, dMY);
+    const double I0 = sI[RI_][CI];
+    const double Ixm2 = sI[RI_ - 2][CI], Ixm1 = sI[RI_ - 1][CI], Ixp1 = sI[RI_ + 1][CI], Ixp2 = sI[RI_ + 2][CI];
+    const double Iym2 = sI[RI_][CI - 2], Iym1 = sI[RI_][CI - 1], Iyp1 = sI[RI_][CI + 1], Iyp2 = sI[RI_][CI + 2];
+    if (I0 * Ixp1 < 0) dPX = subcell_plus(Ixm1, I0, Ixp1, Ixp2, l0, dx);
+    if (I0 * Ixm1 < 0) dMX = subcell_minus(Ixm2, Ixm1, I0, Ixp1, l0, dx);
+    if (I0 * Iyp1 < 0) dPY = subcell_plus(Iym1, I0, Iyp1, Iyp2, l0, dy);
+    if (I0 * Iym1 < 0) dMY = subcell_minus(Iym2, Iym1, I0, Iyp1, l0, dy);
+    return godunov(I0, dPX, dMX, dPY, dMY);
+}
+
+// One whole MinRK2 iteration (src/levelset2d.cpp:451-481) per launch, for grids whose P ghost
+// rules the kernels can evaluate (no LS-free side).  A block owns an FI x FJ tile: it loads phi
+// and phi0 with a 4-cell halo, runs substage A (LS1 = phi - dtau S (|grad phi|_sub - 1)) on the
+// tile and on the 2-cell cross-shaped ring the ENO/subcell stencils of substage B read, fills
+// the ring's ghost cells from LS1's interior values by the P ghost rule (what ApplyBC_P_2D does to
+// LS1 between the substages), then runs substage B and phi' = (phi + LS2) / 2 on the tile and
+// writes phi' with its ghosts.  phi' must not alias phi: neighbouring blocks read phi's halo.
+// (CUSTOM sides keep the caller's ghosts of phi, which a ping-ponged phi' would not carry: those
+// grids take the two-launch path.)
+// Per cell and iteration: phi and phi0 read, phi' written (24 B), against 56 B for the two
+// substage launches; the ring costs (FI + 4) FJ + 4 FI substage-A evaluations per FI FJ cells.
+constexpr int FJ = 64, FH = 4;
+constexpr int RFI = 32, RFT = 512;  // tile height and threads of the launches
+
+template <int FI, int NT>
+__global__ __launch_bounds__(NT) void k_reinit_minrk2(Geom G, GhostRule R, const double* __restrict__ phi,
+                                                      const double* __restrict__ ls0, double* __restrict__ out) {
+    constexpr int PW = FJ + 2 * FH, PH = FI + 2 * FH;  // phi / phi0 tiles: halo 4
+    constexpr int LW = FJ + 4, LH = FI + 4;            // LS1 tile: halo 2
+    __shared__ double sP[PH][PW];
+    __shared__ double sI[PH][PW];
+    __shared__ double sL[LH][LW];
+    const int g = G.g;
+    const int j0 = g + blockIdx.x * FJ, i0 = g + blockIdx.y * FI;
+    const int iend = min(i0 + FI, G.nx + g), jend = min(j0 + FJ, G.ny + g);
+    tile_load2<PH, PW, NT>(G, phi, ls0, sP, sI, i0 - FH, j0 - FH);
+    __syncthreads();
+    const double idx_ = 1.0 / G.dx, idy_ = 1.0 / G.dy;
+    const double eps = smin(G.dx, G.dy), adt = eps * 0.5;
+    // substage A on the interior cells of the cross-shaped ring + tile
+    for (int e = threadIdx.x; e < LH * LW; e += NT) {
+        const int r = e / LW, c = e - r * LW;
+        const int i = i0 - 2 + r, j = j0 - 2 + c;
+        const bool need = (i >= i0 - 2 && i < iend + 2 && j >= j0 && j < jend) ||
+                          (i >= i0 && i < iend && j >= j0 - 2 && j < jend + 2);
+        if (!need || i < g || i >= G.nx + g || j < g || j >= G.ny + g) continue;
+        const double a = absgrad_sub2<PW, PW>(sP, r + 2, c + 2, sI, r + 2, c + 2, G.dx, G.dy, idx_, idy_);
+        const double I0 = sI[r + 2][c + 2];
+        double x;
+        {
+#pragma clang fp contract(off)
+            const double S = smoothed_sign(I0, eps);
+            x = sP[r + 2][c + 2] - adt * S * (a - 1.0);
+        }
+        sL[r][c] = x;
+    }
+    __syncthreads();
+    // LS1's ghost cells on the ring (one coordinate outside the interior; the mirror is interior
+    // and was computed above)
+    for (int e = threadIdx.x; e < LH * LW; e += NT) {
+        const int r = e / LW, c = e - r * LW;
+        const int i = i0 - 2 + r, j = j0 - 2 + c;
+        const bool need = (i >= i0 - 2 && i < iend + 2 && j >= j0 && j < jend) ||
+                          (i >= i0 && i < iend && j >= j0 - 2 && j < jend + 2);
+        if (!need) continue;
+        int side = -1, mi = i, mj = j;
+        if (i < g) { side = 0; mi = 2 * g - 1 - i; }
+        else if (i >= G.nx + g) { side = 1; mi = 2 * G.nx + 2 * g - 1 - i; }
+        else if (j < g) { side = 2; mj = 2 * g - 1 - j; }
+        else if (j >= G.ny + g) { side = 3; mj = 2 * G.ny + 2 * g - 1 - j; }
+        if (side < 0) continue;
+        const double m = sL[mi - (i0 - 2)][mj - (j0 - 2)];
+        // the side's rule with constant indices (kernel-argument registers, no memory load)
+        const int kind = side == 0 ? R.kind[0] : side == 1 ? R.kind[1] : side == 2 ? R.kind[2] : R.kind[3];
+        const double tc = side == 0 ? R.two_c[0] : side == 1 ? R.two_c[1] : side == 2 ? R.two_c[2] : R.two_c[3];
+        sL[r][c] = kind == 0 ? m : kind == 1 ? (-m + tc) : 0.0;
+    }
+    __syncthreads();
+    // substage B and the average on the tile
+    for (int e = threadIdx.x; e < FI * FJ; e += NT) {
+        const int r = e / FJ, c = e - r * FJ;
+        const int i = i0 + r, j = j0 + c;
+        if (i >= iend || j >= jend) continue;
+        const double a = absgrad_sub2<LW, PW>(sL, r + 2, c + 2, sI, r + FH, c + FH, G.dx, G.dy, idx_, idy_);
+        const double I0 = sI[r + FH][c + FH];
+        double val;
+        {
+#pragma clang fp contract(off)
+            const double S = smoothed_sign(I0, eps);
+            const double x = sL[r + 2][c + 2] - adt * S * (a - 1.0);
+            val = 0.5 * (sP[r + FH][c + FH] + x);
+        }
+        out[G.idx(i, j)] = val;
+        write_p_ghosts(G, R, out, i, j, val);
     }
 }
 
@@ -430,7 +590,7 @@ __global__ __launch_bounds__(256) void k_unit_weno5(const double* __restrict__ F
     double f[7];
     for (int k = 0; k < 7; k++) f[k] = F[i - 3 + k];
     double fp, fm;
-    weno5(f[1] - f[0], f[2] - f[1], f[3] - f[2], f[4] - f[3], f[5] - f[4], f[6] - f[5], 1.0 / d, 1e-6 * d * d, fp, fm);
+    weno5_ref<false>(f[1] - f[0], f[2] - f[1], f[3] - f[2], f[4] - f[3], f[5] - f[4], f[6] - f[5], d, 1.0 / d, fp, fm);
     FP[i] = fp;
     FM[i] = fm;
 }
@@ -445,11 +605,11 @@ __global__ __launch_bounds__(256) void k_weno_absgrad(Geom G, const double* __re
     if (i >= G.nx + G.g || j >= G.ny + G.g) return;
     double f[7], dPX, dMX, dPY, dMY;
     for (int k = 0; k < 7; k++) f[k] = ls[G.idx(i - 3 + k, j)];
-    weno5(f[1] - f[0], f[2] - f[1], f[3] - f[2], f[4] - f[3], f[5] - f[4], f[6] - f[5], 1.0 / G.dx,
-          1e-6 * G.dx * G.dx, dPX, dMX);
+    weno5_ref<false>(f[1] - f[0], f[2] - f[1], f[3] - f[2], f[4] - f[3], f[5] - f[4], f[6] - f[5], G.dx, 1.0 / G.dx,
+                     dPX, dMX);
     for (int k = 0; k < 7; k++) f[k] = ls[G.idx(i, j - 3 + k)];
-    weno5(f[1] - f[0], f[2] - f[1], f[3] - f[2], f[4] - f[3], f[5] - f[4], f[6] - f[5], 1.0 / G.dy,
-          1e-6 * G.dy * G.dy, dPY, dMY);
+    weno5_ref<false>(f[1] - f[0], f[2] - f[1], f[3] - f[2], f[4] - f[3], f[5] - f[4], f[6] - f[5], G.dy, 1.0 / G.dy,
+                     dPY, dMY);
     out[G.idx(i, j)] = godunov(ls0[G.idx(i, j)], dPX, dMX, dPY, dMY);
 }
 
@@ -459,6 +619,20 @@ struct Launch {
 
 static Launch tiles(const Geom& G, int tj, int ti, int threads = WTH) {
     return Launch{dim3(cdiv(G.ny, tj), cdiv(G.nx, ti)), dim3(threads)};
+}
+
+// one WENO stage launch; the power-of-two variant when both spacings are powers of two
+template <int STAGE, bool CENTRED>
+void launch_weno(asd2_ctx* ctx, const GhostRule& R, const double* phi, const double* phin, const double* u,
+                 const double* v, double* out, double dt, int ghosts) {
+    const Geom& G = ctx->geo;
+    const Launch L = tiles(G, WJ, WI, WTW);
+    if (is_pow2(G.dx) && is_pow2(G.dy))
+        hipLaunchKernelGGL((k_weno<STAGE, CENTRED, true>), L.grid, L.block, 0, ctx->stream, G, R, phi, phin, u, v, out,
+                           dt, ghosts);
+    else
+        hipLaunchKernelGGL((k_weno<STAGE, CENTRED, false>), L.grid, L.block, 0, ctx->stream, G, R, phi, phin, u, v,
+                           out, dt, ghosts);
 }
 
 // ------------------------------------------------------------------ run diagnostics
@@ -577,17 +751,16 @@ bool has_custom(const asd2_ctx* ctx) {
 int advect_stages(asd2_ctx* ctx, const GhostRule& R, double* d_ls, const double* d_u, const double* d_v, double dt,
                   double* p1, double* p2, bool prof) {
     const Geom& G = ctx->geo;
-    const Launch L = tiles(G, WJ, WI, WTW);
     if (prof) prof_begin(ctx, ASD2_KID_WENO_STAGE);
-    hipLaunchKernelGGL((k_weno<1, false>), L.grid, L.block, 0, ctx->stream, G, R, d_ls, d_ls, d_u, d_v, p1, dt, 1);
+    launch_weno<1, false>(ctx, R, d_ls, d_ls, d_u, d_v, p1, dt, 1);
     if (prof) prof_end(ctx, ASD2_KID_WENO_STAGE);
     ASD2_LAUNCH_CHECK();
     if (prof) prof_begin(ctx, ASD2_KID_WENO_STAGE);
-    hipLaunchKernelGGL((k_weno<2, false>), L.grid, L.block, 0, ctx->stream, G, R, p1, d_ls, d_u, d_v, p2, dt, 1);
+    launch_weno<2, false>(ctx, R, p1, d_ls, d_u, d_v, p2, dt, 1);
     if (prof) prof_end(ctx, ASD2_KID_WENO_STAGE);
     ASD2_LAUNCH_CHECK();
     if (prof) prof_begin(ctx, ASD2_KID_WENO_STAGE);
-    hipLaunchKernelGGL((k_weno<3, false>), L.grid, L.block, 0, ctx->stream, G, R, p2, d_ls, d_u, d_v, d_ls, dt, 1);
+    launch_weno<3, false>(ctx, R, p2, d_ls, d_u, d_v, d_ls, dt, 1);
     if (prof) prof_end(ctx, ASD2_KID_WENO_STAGE);
     ASD2_LAUNCH_CHECK();
     return ASD2_OK;
@@ -644,7 +817,6 @@ int asd2_ls_advect(asd2_ctx* ctx, double* d_ls, const double* d_u, const double*
         ASD2_TRY(zero_async(ctx, p2, n * 8));
     }
     ASD2_TRY(bc_apply_p(ctx, d_ls));
-    const Launch L = tiles(G, WJ, WI, WTW);
     if (!fused) {
         // LS-free sides: build tmpU/tmpV with the full P BC, then run the centred variant
         double *uc, *vc;
@@ -657,13 +829,13 @@ int asd2_ls_advect(asd2_ctx* ctx, double* d_ls, const double* d_u, const double*
         ASD2_LAUNCH_CHECK();
         ASD2_TRY(bc_apply_p(ctx, uc));
         ASD2_TRY(bc_apply_p(ctx, vc));
-        hipLaunchKernelGGL((k_weno<1, true>), L.grid, L.block, 0, ctx->stream, G, R, d_ls, d_ls, uc, vc, p1, dt, 0);
+        launch_weno<1, true>(ctx, R, d_ls, d_ls, uc, vc, p1, dt, 0);
         ASD2_LAUNCH_CHECK();
         ASD2_TRY(bc_apply_p(ctx, p1));
-        hipLaunchKernelGGL((k_weno<2, true>), L.grid, L.block, 0, ctx->stream, G, R, p1, d_ls, uc, vc, p2, dt, 0);
+        launch_weno<2, true>(ctx, R, p1, d_ls, uc, vc, p2, dt, 0);
         ASD2_LAUNCH_CHECK();
         ASD2_TRY(bc_apply_p(ctx, p2));
-        hipLaunchKernelGGL((k_weno<3, true>), L.grid, L.block, 0, ctx->stream, G, R, p2, d_ls, uc, vc, d_ls, dt, 0);
+        launch_weno<3, true>(ctx, R, p2, d_ls, uc, vc, d_ls, dt, 0);
         ASD2_LAUNCH_CHECK();
         return bc_apply_p(ctx, d_ls);
     }
@@ -710,8 +882,7 @@ int asd2_ls_weno_rhs(asd2_ctx* ctx, double* d_ls, const double* d_uc, const doub
     const Geom& G = ctx->geo;
     ASD2_TRY(bc_apply_p(ctx, d_ls));
     ASD2_TRY(zero_async(ctx, d_L, static_cast<size_t>(G.size()) * 8));
-    const Launch L = tiles(G, WJ, WI, WTW);
-    hipLaunchKernelGGL((k_weno<0, true>), L.grid, L.block, 0, ctx->stream, G, R, d_ls, d_ls, d_uc, d_vc, d_L, 0.0, 0);
+    launch_weno<0, true>(ctx, R, d_ls, d_ls, d_uc, d_vc, d_L, 0.0, 0);
     ASD2_LAUNCH_CHECK();
     return ASD2_OK;
 }
@@ -761,6 +932,22 @@ int asd2_ls_reinit_minrk2(asd2_ctx* ctx, double* d_ls, int iterations) {
     if (has_custom(ctx)) ASD2_TRY(zero_async(ctx, ls1, n * 8));
     ASD2_TRY(bc_apply_p(ctx, d_ls));
     ASD2_HIP(hipMemcpyAsync(ls0, d_ls, n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    if (fused && !has_custom(ctx)) {
+        // one launch per iteration, phi ping-ponged between d_ls and ls1 so that the last
+        // iteration writes d_ls; with an odd count the first iteration reads phi from ls0 (equal
+        // to d_ls after the BC)
+        const Launch F = tiles(G, FJ, RFI, RFT);
+        const double* src = (iterations & 1) ? ls0 : d_ls;
+        for (int it = 0; it < iterations; it++) {
+            double* dst = ((iterations - it) & 1) ? d_ls : ls1;
+            prof_begin(ctx, ASD2_KID_REINIT);
+            hipLaunchKernelGGL((k_reinit_minrk2<RFI, RFT>), F.grid, F.block, 0, ctx->stream, G, R, src, ls0, dst);
+            prof_end(ctx, ASD2_KID_REINIT);
+            ASD2_LAUNCH_CHECK();
+            src = dst;
+        }
+        return ASD2_OK;
+    }
     const Launch L = tiles(G, RJ, RI);
     const int gh = fused ? 1 : 0;
     for (int it = 0; it < iterations; it++) {

@@ -73,7 +73,10 @@ def test_levelset_4096_windows_match_oracle(oracle):
         print(f"window at ({cx:.3f}, {cy:.3f}) interface={crosses}: advection max|dphi| = {e_adv:.3e}, "
               f"reinit of the same input {e_re:.3e}, step {err:.3e}")
         worst = max(worst, err, e_adv, e_re)
-    assert worst <= 1e-12
+    # h = 1/4096 is a power of two: advection and reinitialisation evaluate the reference's operation
+    # order exactly, so the step is bit-exact (the subcell fix amplifies last-bit differences of the
+    # advected field by up to 1e6 near the interface, so nothing short of exact would be safe here)
+    assert worst == 0.0
 
 
 def _relL2(a, b):
