@@ -27,7 +27,7 @@ PRECOND_JACOBI, PRECOND_LEARNED, PRECOND_MULTILEVEL = 0, 1, 2
 MOM_RHS, MOM_CONVECTION, MOM_VISCOSITY = 0, 1, 2
 KID = {"weno_stage": 1, "reinit": 2, "pcg64_spmv": 3, "pcg64_update": 4, "pcg32_spmv": 5,
        "pcg32_update": 6, "gfm": 7, "coarse": 8, "fcg_begin": 9, "fcg_spmv": 10, "fcg_update": 11,
-       "fast_pass": 12, "fast_gemm": 13, "fast_tridiag": 14}
+       "fast_pass": 12, "fast_gemm": 13, "fast_tridiag": 14, "momentum": 15}
 
 
 class Asd2Error(RuntimeError):

@@ -147,6 +147,43 @@ __device__ __forceinline__ double minmod(double v1, double v2) {
     return (p <= 0) ? 0.0 : __builtin_nan("");
 }
 
+// A TH x TW tile of a padded array held in registers between its loads and its LDS stores.
+// tile_fetch issues the loads (addresses clamped into the array, no branch around a load; cells
+// outside the padded array read as 0) and tile_store writes the LDS tile, so a kernel that fetches
+// all of its tiles before storing any waits for one memory latency instead of one per guarded load.
+template <int TH, int TW, int NT>
+struct TileRegs {
+    static constexpr int NE = TH * TW, NL = (NE + NT - 1) / NT;
+    double v[NL];
+};
+
+template <int TH, int TW, int NT>
+__device__ __forceinline__ void tile_fetch(const Geom& G, const double* __restrict__ a, TileRegs<TH, TW, NT>& t,
+                                           int ib, int jb) {
+    const int ipad = G.nx + 2 * G.g, jpad = G.ny + 2 * G.g;
+#pragma unroll
+    for (int k = 0; k < TileRegs<TH, TW, NT>::NL; k++) {
+        const int e = static_cast<int>(threadIdx.x) + k * NT;
+        const int r = e / TW, c = e - r * TW;
+        const int i = ib + r, j = jb + c;
+        const bool in = e < TH * TW && i >= 0 && i < ipad && j >= 0 && j < jpad;
+        const double x = a[in ? G.idx(i, j) : 0];
+        t.v[k] = in ? x : 0.0;
+    }
+}
+
+template <int TH, int TW, int NT>
+__device__ __forceinline__ void tile_store(const TileRegs<TH, TW, NT>& t, double (*s)[TW]) {
+#pragma unroll
+    for (int k = 0; k < TileRegs<TH, TW, NT>::NL; k++) {
+        const int e = static_cast<int>(threadIdx.x) + k * NT;
+        if ((TH * TW) % NT == 0 || e < TH * TW) {
+            const int r = e / TW, c = e - r * TW;
+            s[r][c] = t.v[k];
+        }
+    }
+}
+
 // write the ghost images of an interior value at (i, j) for the P grid
 __device__ __forceinline__ void write_p_ghosts(const Geom& G, const GhostRule& R, double* out,
                                                int i, int j, double val) {

@@ -196,6 +196,7 @@ __device__ __forceinline__ double mu_at(const Geom& G, const double* Hs, double 
 
 struct MomArgs {
     double rho_h, rho_l, mu_h, mu_l, dt;
+    double irh, irl, irlrh;  // 1 / rho_h, 1 / rho_l, 1 / (rho_l rho_h), rounded once on the host
     double gu, gv;    // gravity source on U faces / V faces
     int gv_from_i;    // V gravity only where i >= g + 1 (the U-grid array of mac2d.cpp:294)
     int viscous;      // kRe > 0
@@ -203,102 +204,153 @@ struct MomArgs {
                       // 4: F; 5: convection term; 6: viscous term
 };
 
-// tmpV of AddConvectionFU at a U face (src/mac2d.cpp:314-316), 0 elsewhere
-__device__ __forceinline__ double vbar_u(const Geom& G, const double* v, int i, int j) {
-    if (j < G.g || j >= G.ny + G.g || i < G.g + 1 || i >= G.nx + G.g) return 0.0;
-    return (v[G.idx(i, j)] + v[G.idx(i - 1, j)] + v[G.idx(i - 1, j + 1)] + v[G.idx(i, j + 1)]) * 0.25;
-}
-
-// tmpU of AddConvectionFV at a V face (src/mac2d.cpp:395-397), 0 elsewhere
-__device__ __forceinline__ double ubar_v(const Geom& G, const double* u, int i, int j) {
-    if (j < G.g + 1 || j >= G.ny + G.g || i < G.g || i >= G.nx + G.g) return 0.0;
-    return (u[G.idx(i, j)] + u[G.idx(i, j - 1)] + u[G.idx(i + 1, j - 1)] + u[G.idx(i + 1, j)]) * 0.25;
-}
-
-__device__ __forceinline__ double irho_harmonic(double rH, double rL, double a, double b) {
+// 1 / rho at a U face (harmonic mean across the interface, src/mac2d.cpp:570-809); the reciprocals
+// of the densities are the host's correctly rounded 1 / rho_h, 1 / rho_l and 1 / (rho_l rho_h), the
+// values the reference's expressions produce
+__device__ __forceinline__ double irho_harmonic(const MomArgs& A, double a, double b) {
 #pragma clang fp contract(off)
-    if (a >= 0 && b >= 0) return 1.0 / rH;
-    else if (a <= 0 && b <= 0) return 1.0 / rL;
+    if (a >= 0 && b >= 0) return A.irh;
+    else if (a <= 0 && b <= 0) return A.irl;
     else if (a > 0 && b <= 0) {
         const double th = fabs(a) / (fabs(a) + fabs(b));
-        return 1.0 / (rL * rH) / (1.0 / rL * th + 1.0 / rH * (1.0 - th));
+        return A.irlrh / (A.irl * th + A.irh * (1.0 - th));
     } else {
         const double th = fabs(a) / (fabs(a) + fabs(b));
-        return 1.0 / (rH * rL) / (1.0 / rH * th + 1.0 / rL * (1.0 - th));
+        return A.irlrh / (A.irh * th + A.irl * (1.0 - th));
     }
 }
 
-__device__ __forceinline__ double rho_arith(double rH, double rL, double a, double b) {
+// 1 / rho at a V face (arithmetic mean across the interface, the FV override of
+// src/mac2d.cpp:1000-1001)
+__device__ __forceinline__ double irho_arith(const MomArgs& A, double a, double b) {
 #pragma clang fp contract(off)
-    if (a >= 0 && b >= 0) return rH;
-    else if (a <= 0 && b <= 0) return rL;
+    const double rH = A.rho_h, rL = A.rho_l;
+    if (a >= 0 && b >= 0) return A.irh;
+    else if (a <= 0 && b <= 0) return A.irl;
     else if (a > 0 && b <= 0) {
         const double th = fabs(a) / (fabs(a) + fabs(b));
-        return rH * th + rL * (1.0 - th);
+        return 1.0 / (rH * th + rL * (1.0 - th));
     } else {
         const double th = fabs(a) / (fabs(a) + fabs(b));
-        return rL * th + rH * (1.0 - th);
+        return 1.0 / (rL * th + rH * (1.0 - th));
     }
 }
 
 // -cU + vU + gU at a U face and the RK combination (src/mac2d.cpp:254-279, 308-387, 570-809)
-// Both momentum kernels run on MJ x MI face tiles (j fastest, 4 rows per thread); the WENO
-// fluxes of the convection term are formed once per face into LDS tiles with a 3-face halo.
+// Both momentum kernels run on MJ x MI face tiles (j fastest, 4 rows per thread).  A block stages
+// its tile of u and v with the halos the stencils read, and mu and the level set at the cells
+// around its faces, in LDS, fetching every one of them before the first LDS store; the WENO fluxes
+// are then formed from the staged values (the product flux once per face into an LDS tile with a
+// 3-face halo, the squared flux where the WENO stencil reads it).
 constexpr int MJ = 64, MI = 16;
 
+// mu at the staged cells: mu_at over Hs values already in registers
+template <int TH, int TW, int NT>
+__device__ __forceinline__ void mu_store(const Geom& G, const TileRegs<TH, TW, NT>& t, double (*s)[TW], int ib, int jb,
+                                         double muH, double muL) {
+#pragma unroll
+    for (int k = 0; k < TileRegs<TH, TW, NT>::NL; k++) {
+        const int e = static_cast<int>(threadIdx.x) + k * NT;
+        if ((TH * TW) % NT == 0 || e < TH * TW) {
+            const int r = e / TW, c = e - r * TW;
+            const int i = ib + r, j = jb + c;
+            double m;
+            {
+#pragma clang fp contract(off)
+                m = (i < G.g || i >= G.nx + G.g || j < G.g || j >= G.ny + G.g) ? 0.0 : muL + (muH - muL) * t.v[k];
+            }
+            s[r][c] = m;
+        }
+    }
+}
+
+template <bool POW2>
 __global__ __launch_bounds__(256) void k_mom_u(Geom G, MomArgs A, const double* __restrict__ u,
                                                const double* __restrict__ v, const double* __restrict__ ls,
                                                const double* __restrict__ Hs, const double* __restrict__ u0,
                                                double* __restrict__ out) {
-    __shared__ double sFx[MI + 6][MJ];  // u^2 along i
+    constexpr int UH = MI + 6, UW = MJ + 6;  // u at (i0 - 3 + a, j0 - 3 + b)
+    constexpr int VH = MI + 1, VW = MJ + 7;  // v at (i0 - 1 + a, j0 - 3 + b)
+    constexpr int CH = MI + 1, CW = MJ + 2;  // mu, level set at cells (i0 - 1 + a, j0 - 1 + b)
+    __shared__ double sU[UH][UW];
+    __shared__ double sV[VH][VW];
     __shared__ double sFy[MI][MJ + 6];  // u vbar along j
-    __shared__ double sMu[MI + 1][MJ + 2];  // mu at cells (i0 - 1 + a, j0 - 1 + b)
-    __shared__ double sLs[MI + 1][MJ + 2];  // level set at the same cells
+    __shared__ double sMu[CH][CW];
+    __shared__ double sLs[CH][CW];
     const int tj = threadIdx.x & (MJ - 1), ti = threadIdx.x / MJ;
     const int j0 = G.g + blockIdx.x * MJ, i0 = G.g + 1 + blockIdx.y * MI;
     const int ipad = G.nx + 2 * G.g, jpad = G.ny + 2 * G.g;
-    for (int r = ti; r < MI + 6; r += 256 / MJ) {
-        const int i = i0 - 3 + r, j = j0 + tj;
-        double f = 0.0;
-        if (i < ipad && j < jpad) { const double w = u[G.idx(i, j)]; f = w * w; }
-        sFx[r][tj] = f;
-    }
+    const int iend = G.nx + G.g, jend = G.ny + G.g;
+    TileRegs<UH, UW, 256> tu;
+    TileRegs<VH, VW, 256> tv;
+    TileRegs<CH, CW, 256> th, tl;
+    tile_fetch(G, u, tu, i0 - 3, j0 - 3);
+    tile_fetch(G, v, tv, i0 - 1, j0 - 3);
     if (A.viscous) {
-        for (int e = threadIdx.x; e < (MI + 1) * (MJ + 2); e += 256) {
-            const int a = e / (MJ + 2), b = e - a * (MJ + 2);
-            const int i = i0 - 1 + a, j = j0 - 1 + b;
-            const bool in = i < ipad && j < jpad;
-            sMu[a][b] = in ? mu_at(G, Hs, A.mu_h, A.mu_l, i, j) : 0.0;
-            sLs[a][b] = in ? ls[G.idx(i, j)] : 0.0;
-        }
+        tile_fetch(G, Hs, th, i0 - 1, j0 - 1);
+        tile_fetch(G, ls, tl, i0 - 1, j0 - 1);
     }
+    constexpr int NR = MI / (256 / MJ);
+    double u0R[NR];
+    const bool rk = A.mode >= 1 && A.mode <= 3;
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        const int i = i0 + ti + (256 / MJ) * k, j = j0 + tj;
+        const bool ok = rk && i < iend && j < jend;
+        const double x = u0[ok ? G.idx(i, j) : 0];
+        u0R[k] = ok ? x : 0.0;
+    }
+    tile_store(tu, sU);
+    tile_store(tv, sV);
+    if (A.viscous) {
+        mu_store(G, th, sMu, i0 - 1, j0 - 1, A.mu_h, A.mu_l);
+        tile_store(tl, sLs);
+    }
+    __syncthreads();
+    // tmpV of AddConvectionFU at a U face (src/mac2d.cpp:314-316) from the staged v, 0 elsewhere
+    auto vbar = [&](int i, int j) -> double {
+        if (j < G.g || j >= G.ny + G.g || i < G.g + 1 || i >= G.nx + G.g) return 0.0;
+        const int a = i - (i0 - 1), b = j - (j0 - 3);
+        return (sV[a][b] + sV[a - 1][b] + sV[a - 1][b + 1] + sV[a][b + 1]) * 0.25;
+    };
     for (int e = threadIdx.x; e < MI * (MJ + 6); e += 256) {
         const int r = e / (MJ + 6), cc = e - r * (MJ + 6);
         const int i = i0 + r, j = j0 - 3 + cc;
-        sFy[r][cc] = (i < ipad && j < jpad) ? u[G.idx(i, j)] * vbar_u(G, v, i, j) : 0.0;
+        sFy[r][cc] = (i < ipad && j < jpad) ? sU[r + 3][cc] * vbar(i, j) : 0.0;
     }
     __syncthreads();
-    for (int r = ti; r < MI; r += 256 / MJ) {
+    const double idx = 1.0 / G.dx, idy = 1.0 / G.dy, epsx = 1e-6 * G.dx * G.dx, epsy = 1e-6 * G.dy * G.dy;
+    for (int k = 0; k < NR; k++) {
+    const int r = ti + (256 / MJ) * k;
     const int i = i0 + r, j = j0 + tj;
-    if (i >= G.nx + G.g || j >= G.ny + G.g) continue;
+    if (i >= iend || j >= jend) continue;
     const int64_t c = G.idx(i, j);
+    const int R = r + 3, C = tj + 3;  // the face in sU
     // convection: WENO of u^2 along i and of u * vbar along j
     double lxp, lxm, lyp, lym;
-    weno5(sFx[r + 1][tj] - sFx[r][tj], sFx[r + 2][tj] - sFx[r + 1][tj], sFx[r + 3][tj] - sFx[r + 2][tj],
-          sFx[r + 4][tj] - sFx[r + 3][tj], sFx[r + 5][tj] - sFx[r + 4][tj], sFx[r + 6][tj] - sFx[r + 5][tj],
-          1.0 / G.dx, 1e-6 * G.dx * G.dx, lxp, lxm);
+    {
+        double f[7];
+#pragma unroll
+        for (int q = 0; q < 7; q++) {
+            const double w = sU[R - 3 + q][C];
+            f[q] = w * w;
+        }
+        weno5(f[1] - f[0], f[2] - f[1], f[3] - f[2], f[4] - f[3], f[5] - f[4], f[6] - f[5], idx, epsx, lxp,
+              lxm);
+    }
     weno5(sFy[r][tj + 1] - sFy[r][tj], sFy[r][tj + 2] - sFy[r][tj + 1], sFy[r][tj + 3] - sFy[r][tj + 2],
           sFy[r][tj + 4] - sFy[r][tj + 3], sFy[r][tj + 5] - sFy[r][tj + 4], sFy[r][tj + 6] - sFy[r][tj + 5],
-          1.0 / G.dy, 1e-6 * G.dy * G.dy, lyp, lym);
-    const double aX = u[c], aY = vbar_u(G, v, i, j);
+          idy, epsy, lyp, lym);
+    const double aX = sU[R][C], aY = vbar(i, j);
     const double cU = 0.5 * (lxp + lxm) + 0.5 * (lyp + lym) - aX * (0.5 * (lxp - lxm)) - aY * (0.5 * (lyp - lym));
     double dU = 0.0;
     if (A.viscous) {
 #pragma clang fp contract(off)
         const double dx = G.dx, dy = G.dy;
-        const double uM = u[c], uW = u[G.idx(i - 1, j)], uE = u[G.idx(i + 1, j)];
-        const double uS = u[G.idx(i, j - 1)], uN = u[G.idx(i, j + 1)];
-        const double vW = v[G.idx(i - 1, j)], vW_N = v[G.idx(i - 1, j + 1)], vM = v[c], vN = v[G.idx(i, j + 1)];
+        const double uM = sU[R][C], uW = sU[R - 1][C], uE = sU[R + 1][C];
+        const double uS = sU[R][C - 1], uN = sU[R][C + 1];
+        const int a = i - (i0 - 1), b = j - (j0 - 3);  // the face's own cell in sV
+        const double vW = sV[a - 1][b], vW_N = sV[a - 1][b + 1], vM = sV[a][b], vN = sV[a][b + 1];
         // tile cell (i + di, j + dj) of the staged mu / level set
         const int a0 = r + 1, b0 = tj + 1;
 #define LS(di, dj) sLs[a0 + (di)][b0 + (dj)]
@@ -306,27 +358,31 @@ __global__ __launch_bounds__(256) void k_mom_u(Geom G, MomArgs A, const double* 
         const double lsUNHalf = 0.25 * (LS(0, 0) + LS(-1, 0) + LS(-1, 1) + LS(0, 1));
         const double lsUSHalf = 0.25 * (LS(0, 0) + LS(-1, 0) + LS(-1, -1) + LS(0, -1));
 #undef LS
-        const double iWE = irho_harmonic(A.rho_h, A.rho_l, lsW, lsM);
-        const double iSN = irho_harmonic(A.rho_h, A.rho_l, lsUSHalf, lsUNHalf);
+        const double iWE = irho_harmonic(A, lsW, lsM);
+        const double iSN = irho_harmonic(A, lsUSHalf, lsUNHalf);
 #define MU(a, b) sMu[a0 + (a) - i][b0 + (b) - j]
-        const double muU_X_W = MU(i - 1, j) * (uM - uW) / dx;
-        const double muU_X_E = MU(i, j) * (uE - uM) / dx;
-        const double muU_Y_S = 0.25 * (MU(i, j) + MU(i - 1, j) + MU(i - 1, j - 1) + MU(i, j - 1)) * (uM - uS) / dy;
-        const double muU_Y_N = 0.25 * (MU(i, j) + MU(i - 1, j) + MU(i - 1, j + 1) + MU(i, j + 1)) * (uN - uM) / dy;
-        const double muV_X_S = 0.25 * (MU(i, j) + MU(i - 1, j) + MU(i - 1, j - 1) + MU(i, j - 1)) * (vM - vW) / dx;
-        const double muV_X_N = 0.25 * (MU(i, j) + MU(i - 1, j) + MU(i - 1, j + 1) + MU(i, j + 1)) * (vN - vW_N) / dx;
+        const double muU_X_W = wdiv<POW2>(MU(i - 1, j) * (uM - uW), dx, idx);
+        const double muU_X_E = wdiv<POW2>(MU(i, j) * (uE - uM), dx, idx);
+        const double muU_Y_S =
+            wdiv<POW2>(0.25 * (MU(i, j) + MU(i - 1, j) + MU(i - 1, j - 1) + MU(i, j - 1)) * (uM - uS), dy, idy);
+        const double muU_Y_N =
+            wdiv<POW2>(0.25 * (MU(i, j) + MU(i - 1, j) + MU(i - 1, j + 1) + MU(i, j + 1)) * (uN - uM), dy, idy);
+        const double muV_X_S =
+            wdiv<POW2>(0.25 * (MU(i, j) + MU(i - 1, j) + MU(i - 1, j - 1) + MU(i, j - 1)) * (vM - vW), dx, idx);
+        const double muV_X_N =
+            wdiv<POW2>(0.25 * (MU(i, j) + MU(i - 1, j) + MU(i - 1, j + 1) + MU(i, j + 1)) * (vN - vW_N), dx, idx);
 #undef MU
-        const double visX = 2.0 * (muU_X_E - muU_X_W) / dx;
-        const double visY = (muU_Y_N - muU_Y_S) / dy + (muV_X_N - muV_X_S) / dy;
+        const double visX = wdiv<POW2>(2.0 * (muU_X_E - muU_X_W), dx, idx);
+        const double visY = wdiv<POW2>(muU_Y_N - muU_Y_S, dy, idy) + wdiv<POW2>(muV_X_N - muV_X_S, dy, idy);
         dU = visX * iWE + visY * iSN;
     }
     const double F = -cU + dU + A.gu;
-    const double t = u[c] + A.dt * F;
+    const double t = sU[R][C] + A.dt * F;
     double o;
     if (A.mode == 0) o = t;
-    else if (A.mode == 1) o = 0.75 * u0[c] + 0.25 * t;
-    else if (A.mode == 2) o = 1.0 / 3.0 * u0[c] + 2.0 / 3.0 * t;
-    else if (A.mode == 3) o = 0.5 * u0[c] + 0.5 * t;
+    else if (A.mode == 1) o = 0.75 * u0R[k] + 0.25 * t;
+    else if (A.mode == 2) o = 1.0 / 3.0 * u0R[k] + 2.0 / 3.0 * t;
+    else if (A.mode == 3) o = 0.5 * u0R[k] + 0.5 * t;
     else if (A.mode == 4) o = F;
     else if (A.mode == 5) o = cU;
     else o = dU;
@@ -335,86 +391,123 @@ __global__ __launch_bounds__(256) void k_mom_u(Geom G, MomArgs A, const double* 
 }
 
 // -cV + vV + gV at a V face and the RK combination (src/mac2d.cpp:281-306, 389-463, 811-1041)
+template <bool POW2>
 __global__ __launch_bounds__(256) void k_mom_v(Geom G, MomArgs A, const double* __restrict__ u,
                                                const double* __restrict__ v, const double* __restrict__ ls,
                                                const double* __restrict__ Hs, const double* __restrict__ v0,
                                                double* __restrict__ out) {
+    constexpr int VH = MI + 6, VW = MJ + 6;  // v at (i0 - 3 + a, j0 - 3 + b)
+    constexpr int UH = MI + 7, UW = MJ + 1;  // u at (i0 - 3 + a, j0 - 1 + b)
+    constexpr int CH = MI + 2, CW = MJ + 1;  // mu, level set at cells (i0 - 1 + a, j0 - 1 + b)
+    __shared__ double sV[VH][VW];
+    __shared__ double sU[UH][UW];
     __shared__ double sFx[MI + 6][MJ];  // v ubar along i
-    __shared__ double sFy[MI][MJ + 6];  // v^2 along j
-    __shared__ double sMu[MI + 2][MJ + 1];  // mu at cells (i0 - 1 + a, j0 - 1 + b)
-    __shared__ double sLs[MI + 2][MJ + 1];  // level set at the same cells
+    __shared__ double sMu[CH][CW];
+    __shared__ double sLs[CH][CW];
     const int tj = threadIdx.x & (MJ - 1), ti = threadIdx.x / MJ;
     const int j0 = G.g + 1 + blockIdx.x * MJ, i0 = G.g + blockIdx.y * MI;
     const int ipad = G.nx + 2 * G.g, jpad = G.ny + 2 * G.g;
+    const int iend = G.nx + G.g, jend = G.ny + G.g;
+    TileRegs<VH, VW, 256> tv;
+    TileRegs<UH, UW, 256> tu;
+    TileRegs<CH, CW, 256> th, tl;
+    tile_fetch(G, v, tv, i0 - 3, j0 - 3);
+    tile_fetch(G, u, tu, i0 - 3, j0 - 1);
     if (A.viscous) {
-        for (int e = threadIdx.x; e < (MI + 2) * (MJ + 1); e += 256) {
-            const int a = e / (MJ + 1), b = e - a * (MJ + 1);
-            const int i = i0 - 1 + a, j = j0 - 1 + b;
-            const bool in = i < ipad && j < jpad;
-            sMu[a][b] = in ? mu_at(G, Hs, A.mu_h, A.mu_l, i, j) : 0.0;
-            sLs[a][b] = in ? ls[G.idx(i, j)] : 0.0;
-        }
+        tile_fetch(G, Hs, th, i0 - 1, j0 - 1);
+        tile_fetch(G, ls, tl, i0 - 1, j0 - 1);
     }
-    for (int r = ti; r < MI + 6; r += 256 / MJ) {
-        const int i = i0 - 3 + r, j = j0 + tj;
-        sFx[r][tj] = (i < ipad && j < jpad) ? v[G.idx(i, j)] * ubar_v(G, u, i, j) : 0.0;
+    constexpr int NR = MI / (256 / MJ);
+    double v0R[NR];
+    const bool rk = A.mode >= 1 && A.mode <= 3;
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        const int i = i0 + ti + (256 / MJ) * k, j = j0 + tj;
+        const bool ok = rk && i < iend && j < jend;
+        const double x = v0[ok ? G.idx(i, j) : 0];
+        v0R[k] = ok ? x : 0.0;
     }
-    for (int e = threadIdx.x; e < MI * (MJ + 6); e += 256) {
-        const int r = e / (MJ + 6), cc = e - r * (MJ + 6);
-        const int i = i0 + r, j = j0 - 3 + cc;
-        double f = 0.0;
-        if (i < ipad && j < jpad) { const double w = v[G.idx(i, j)]; f = w * w; }
-        sFy[r][cc] = f;
+    tile_store(tv, sV);
+    tile_store(tu, sU);
+    if (A.viscous) {
+        mu_store(G, th, sMu, i0 - 1, j0 - 1, A.mu_h, A.mu_l);
+        tile_store(tl, sLs);
     }
     __syncthreads();
-    for (int r = ti; r < MI; r += 256 / MJ) {
+    // tmpU of AddConvectionFV at a V face (src/mac2d.cpp:395-397) from the staged u, 0 elsewhere
+    auto ubar = [&](int i, int j) -> double {
+        if (j < G.g + 1 || j >= G.ny + G.g || i < G.g || i >= G.nx + G.g) return 0.0;
+        const int a = i - (i0 - 3), b = j - (j0 - 1);
+        return (sU[a][b] + sU[a][b - 1] + sU[a + 1][b - 1] + sU[a + 1][b]) * 0.25;
+    };
+    for (int r = ti; r < MI + 6; r += 256 / MJ) {
+        const int i = i0 - 3 + r, j = j0 + tj;
+        sFx[r][tj] = (i < ipad && j < jpad) ? sV[r][tj + 3] * ubar(i, j) : 0.0;
+    }
+    __syncthreads();
+    const double idx = 1.0 / G.dx, idy = 1.0 / G.dy, epsx = 1e-6 * G.dx * G.dx, epsy = 1e-6 * G.dy * G.dy;
+    for (int k = 0; k < NR; k++) {
+    const int r = ti + (256 / MJ) * k;
     const int i = i0 + r, j = j0 + tj;
-    if (i >= G.nx + G.g || j >= G.ny + G.g) continue;
+    if (i >= iend || j >= jend) continue;
     const int64_t c = G.idx(i, j);
+    const int R = r + 3, C = tj + 3;  // the face in sV
     double lxp, lxm, lyp, lym;
     weno5(sFx[r + 1][tj] - sFx[r][tj], sFx[r + 2][tj] - sFx[r + 1][tj], sFx[r + 3][tj] - sFx[r + 2][tj],
           sFx[r + 4][tj] - sFx[r + 3][tj], sFx[r + 5][tj] - sFx[r + 4][tj], sFx[r + 6][tj] - sFx[r + 5][tj],
-          1.0 / G.dx, 1e-6 * G.dx * G.dx, lxp, lxm);
-    weno5(sFy[r][tj + 1] - sFy[r][tj], sFy[r][tj + 2] - sFy[r][tj + 1], sFy[r][tj + 3] - sFy[r][tj + 2],
-          sFy[r][tj + 4] - sFy[r][tj + 3], sFy[r][tj + 5] - sFy[r][tj + 4], sFy[r][tj + 6] - sFy[r][tj + 5],
-          1.0 / G.dy, 1e-6 * G.dy * G.dy, lyp, lym);
-    const double aX = ubar_v(G, u, i, j), aY = v[c];
+          idx, epsx, lxp, lxm);
+    {
+        double f[7];
+#pragma unroll
+        for (int q = 0; q < 7; q++) {
+            const double w = sV[R][C - 3 + q];
+            f[q] = w * w;
+        }
+        weno5(f[1] - f[0], f[2] - f[1], f[3] - f[2], f[4] - f[3], f[5] - f[4], f[6] - f[5], idy, epsy, lyp,
+              lym);
+    }
+    const double aX = ubar(i, j), aY = sV[R][C];
     const double cV = 0.5 * (lxp + lxm) + 0.5 * (lyp + lym) - aX * (0.5 * (lxp - lxm)) - aY * (0.5 * (lyp - lym));
     double dV = 0.0;
     if (A.viscous) {
 #pragma clang fp contract(off)
         const double dx = G.dx, dy = G.dy;
-        const double vW = v[G.idx(i - 1, j)], vE = v[G.idx(i + 1, j)], vS = v[G.idx(i, j - 1)];
-        const double vN = v[G.idx(i, j + 1)], vM = v[c];
-        const double uM = u[c], uE = u[G.idx(i + 1, j)], uS = u[G.idx(i, j - 1)], uE_S = u[G.idx(i + 1, j - 1)];
+        const double vW = sV[R - 1][C], vE = sV[R + 1][C], vS = sV[R][C - 1];
+        const double vN = sV[R][C + 1], vM = sV[R][C];
+        const int a = i - (i0 - 3), b = j - (j0 - 1);  // the face's own cell in sU
+        const double uM = sU[a][b], uE = sU[a + 1][b], uS = sU[a][b - 1], uE_S = sU[a + 1][b - 1];
         // tile cell (i + di, j + dj) of the staged mu / level set
         const int a0 = r + 1, b0 = tj + 1;
 #define LS(di, dj) sLs[a0 + (di)][b0 + (dj)]
         const double lsVWHalf = 0.25 * (LS(0, 0) + LS(-1, 0) + LS(-1, -1) + LS(0, -1));
         const double lsVEHalf = 0.25 * (LS(0, 0) + LS(1, 0) + LS(1, -1) + LS(0, -1));
-        const double iWE = 1.0 / rho_arith(A.rho_h, A.rho_l, lsVWHalf, lsVEHalf);
-        const double iSN = 1.0 / rho_arith(A.rho_h, A.rho_l, LS(0, -1), LS(0, 0));
+        const double iWE = irho_arith(A, lsVWHalf, lsVEHalf);
+        const double iSN = irho_arith(A, LS(0, -1), LS(0, 0));
 #undef LS
 #define MU(a, b) sMu[a0 + (a) - i][b0 + (b) - j]
-        const double muU_Y_W = 0.25 * (MU(i, j) + MU(i - 1, j) + MU(i - 1, j - 1) + MU(i, j - 1)) * (uM - uS) / dy;
-        const double muU_Y_E = 0.25 * (MU(i, j) + MU(i + 1, j) + MU(i + 1, j - 1) + MU(i, j - 1)) * (uE - uE_S) / dy;
-        const double muV_X_W = 0.25 * (MU(i, j) + MU(i - 1, j) + MU(i - 1, j - 1) + MU(i, j - 1)) * (vM - vW) / dx;
-        const double muV_X_E = 0.25 * (MU(i, j) + MU(i + 1, j) + MU(i + 1, j - 1) + MU(i, j - 1)) * (vE - vM) / dx;
-        const double muV_Y_S = MU(i, j - 1) * (vM - vS) / dy;
-        const double muV_Y_N = MU(i, j) * (vN - vM) / dy;
+        const double muU_Y_W =
+            wdiv<POW2>(0.25 * (MU(i, j) + MU(i - 1, j) + MU(i - 1, j - 1) + MU(i, j - 1)) * (uM - uS), dy, idy);
+        const double muU_Y_E =
+            wdiv<POW2>(0.25 * (MU(i, j) + MU(i + 1, j) + MU(i + 1, j - 1) + MU(i, j - 1)) * (uE - uE_S), dy, idy);
+        const double muV_X_W =
+            wdiv<POW2>(0.25 * (MU(i, j) + MU(i - 1, j) + MU(i - 1, j - 1) + MU(i, j - 1)) * (vM - vW), dx, idx);
+        const double muV_X_E =
+            wdiv<POW2>(0.25 * (MU(i, j) + MU(i + 1, j) + MU(i + 1, j - 1) + MU(i, j - 1)) * (vE - vM), dx, idx);
+        const double muV_Y_S = wdiv<POW2>(MU(i, j - 1) * (vM - vS), dy, idy);
+        const double muV_Y_N = wdiv<POW2>(MU(i, j) * (vN - vM), dy, idy);
 #undef MU
-        const double visX = (muU_Y_E - muU_Y_W) / dx + (muV_X_E - muV_X_W) / dx;
-        const double visY = 2.0 * (muV_Y_N - muV_Y_S) / dy;
+        const double visX = wdiv<POW2>(muU_Y_E - muU_Y_W, dx, idx) + wdiv<POW2>(muV_X_E - muV_X_W, dx, idx);
+        const double visY = wdiv<POW2>(2.0 * (muV_Y_N - muV_Y_S), dy, idy);
         dV = visX * iWE + visY * iSN;
     }
     const double gv = (A.gv_from_i && i < G.g + 1) ? 0.0 : A.gv;
     const double F = -cV + dV + gv;
-    const double t = v[c] + A.dt * F;
+    const double t = sV[R][C] + A.dt * F;
     double o;
     if (A.mode == 0) o = t;
-    else if (A.mode == 1) o = 0.75 * v0[c] + 0.25 * t;
-    else if (A.mode == 2) o = 1.0 / 3.0 * v0[c] + 2.0 / 3.0 * t;
-    else if (A.mode == 3) o = 0.5 * v0[c] + 0.5 * t;
+    else if (A.mode == 1) o = 0.75 * v0R[k] + 0.25 * t;
+    else if (A.mode == 2) o = 1.0 / 3.0 * v0R[k] + 2.0 / 3.0 * t;
+    else if (A.mode == 3) o = 0.5 * v0R[k] + 0.5 * t;
     else if (A.mode == 4) o = F;
     else if (A.mode == 5) o = cV;
     else o = dV;
@@ -556,6 +649,7 @@ int asd2_mac_intermediate_vel(asd2_ctx* ctx, const asd2_fluid* fl, int time_orde
     const size_t n = static_cast<size_t>(G.size());
     MomArgs A{};
     A.rho_h = fl->rho_h; A.rho_l = fl->rho_l; A.mu_h = fl->mu_h; A.mu_l = fl->mu_l; A.dt = fl->dt;
+    A.irh = 1.0 / fl->rho_h; A.irl = 1.0 / fl->rho_l; A.irlrh = 1.0 / (fl->rho_l * fl->rho_h);
     const bool fr_ok = (fl->fr != 0 || fl->g != 0.0) && !std::isnan(fl->fr) && !std::isinf(fl->fr);
     A.gu = (fr_ok && fl->g_axis == 0) ? -fl->g : 0.0;
     if (fl->gravity_fix) { A.gv = (fr_ok && fl->g_axis == 1) ? -fl->g : 0.0; A.gv_from_i = 0; }
@@ -569,11 +663,18 @@ int asd2_mac_intermediate_vel(asd2_ctx* ctx, const asd2_fluid* fl, int time_orde
     // fresh stage arrays (GetIntermediateVel allocates zero vectors, mac2d.cpp:1146-1147)
     for (double* p : {u1, v1, u2, v2}) ASD2_TRY(zero_async(ctx, p, n * 8));
     const dim3 gu(cdiv(G.ny, MJ), cdiv(G.nx - 1, MI)), gv(cdiv(G.ny - 1, MJ), cdiv(G.nx, MI));
+    const bool pow2 = is_pow2(G.dx) && is_pow2(G.dy);  // divisions by dx, dy are exact products
     auto stage = [&](const double* us, const double* vs, const double* u0, const double* v0, double* uo, double* vo,
                      int mode) -> int {
         A.mode = mode;
-        hipLaunchKernelGGL(k_mom_u, gu, dim3(256), 0, ctx->stream, G, A, us, vs, d_ls, d_Hs, u0, uo);
-        hipLaunchKernelGGL(k_mom_v, gv, dim3(256), 0, ctx->stream, G, A, us, vs, d_ls, d_Hs, v0, vo);
+        prof_begin(ctx, ASD2_KID_MOMENTUM);
+        if (pow2) hipLaunchKernelGGL(k_mom_u<true>, gu, dim3(256), 0, ctx->stream, G, A, us, vs, d_ls, d_Hs, u0, uo);
+        else hipLaunchKernelGGL(k_mom_u<false>, gu, dim3(256), 0, ctx->stream, G, A, us, vs, d_ls, d_Hs, u0, uo);
+        prof_end(ctx, ASD2_KID_MOMENTUM);
+        prof_begin(ctx, ASD2_KID_MOMENTUM);
+        if (pow2) hipLaunchKernelGGL(k_mom_v<true>, gv, dim3(256), 0, ctx->stream, G, A, us, vs, d_ls, d_Hs, v0, vo);
+        else hipLaunchKernelGGL(k_mom_v<false>, gv, dim3(256), 0, ctx->stream, G, A, us, vs, d_ls, d_Hs, v0, vo);
+        prof_end(ctx, ASD2_KID_MOMENTUM);
         ASD2_LAUNCH_CHECK();
         return ASD2_OK;
     };
@@ -607,6 +708,7 @@ int asd2_mac_momentum_term(asd2_ctx* ctx, const asd2_fluid* fl, int grid_kind, i
     const Geom& G = ctx->geo;
     MomArgs A{};
     A.rho_h = fl->rho_h; A.rho_l = fl->rho_l; A.mu_h = fl->mu_h; A.mu_l = fl->mu_l; A.dt = fl->dt;
+    A.irh = 1.0 / fl->rho_h; A.irl = 1.0 / fl->rho_l; A.irlrh = 1.0 / (fl->rho_l * fl->rho_h);
     const bool fr_ok = (fl->fr != 0 || fl->g != 0.0) && !std::isnan(fl->fr) && !std::isinf(fl->fr);
     A.gu = (fr_ok && fl->g_axis == 0) ? -fl->g : 0.0;
     if (fl->gravity_fix) { A.gv = (fr_ok && fl->g_axis == 1) ? -fl->g : 0.0; A.gv_from_i = 0; }
@@ -614,12 +716,15 @@ int asd2_mac_momentum_term(asd2_ctx* ctx, const asd2_fluid* fl, int grid_kind, i
     A.viscous = fl->re > 0.0 ? 1 : 0;
     A.mode = 4 + term;
     ASD2_TRY(zero_async(ctx, d_out, static_cast<size_t>(G.size()) * 8));
-    if (grid_kind == ASD2_GRID_U)
-        hipLaunchKernelGGL(k_mom_u, dim3(cdiv(G.ny, MJ), cdiv(G.nx - 1, MI)), dim3(256), 0, ctx->stream, G, A, d_u,
-                           d_v, d_ls, d_Hs, d_u, d_out);
-    else
-        hipLaunchKernelGGL(k_mom_v, dim3(cdiv(G.ny - 1, MJ), cdiv(G.nx, MI)), dim3(256), 0, ctx->stream, G, A, d_u,
-                           d_v, d_ls, d_Hs, d_v, d_out);
+    const bool pow2 = is_pow2(G.dx) && is_pow2(G.dy);
+    const dim3 gu(cdiv(G.ny, MJ), cdiv(G.nx - 1, MI)), gv(cdiv(G.ny - 1, MJ), cdiv(G.nx, MI));
+    if (grid_kind == ASD2_GRID_U) {
+        if (pow2) hipLaunchKernelGGL(k_mom_u<true>, gu, dim3(256), 0, ctx->stream, G, A, d_u, d_v, d_ls, d_Hs, d_u, d_out);
+        else hipLaunchKernelGGL(k_mom_u<false>, gu, dim3(256), 0, ctx->stream, G, A, d_u, d_v, d_ls, d_Hs, d_u, d_out);
+    } else {
+        if (pow2) hipLaunchKernelGGL(k_mom_v<true>, gv, dim3(256), 0, ctx->stream, G, A, d_u, d_v, d_ls, d_Hs, d_v, d_out);
+        else hipLaunchKernelGGL(k_mom_v<false>, gv, dim3(256), 0, ctx->stream, G, A, d_u, d_v, d_ls, d_Hs, d_v, d_out);
+    }
     ASD2_LAUNCH_CHECK();
     return ASD2_OK;
 }

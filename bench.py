@@ -45,6 +45,8 @@ BYTES_PER_CELL = {
     "reinit": 56,            # one MinRK2 iteration per launch: substage A 24 B + substage B 32 B
     "gfm": 4 * 8 + 5 * 8 + 4 * 4,  # phi, kappa, H, div -> beta_x, beta_y, diag, rhs, dinv (FP64) and
                                    # the FP32 beta_x, beta_y, diag, dinv of the mixed solver
+    "momentum": (40 + 48 + 48) / 3,  # one face grid of an RK3 stage: u, v, phi, Hs read, u0 read in
+                                     # stages 2 and 3, the face array written
     "coarse": 0,             # coarse correction (V-cycle or network): no per-cell HBM figure
 }
 # HBM bytes per interior cell and launch measured with rocprofv3 on `--pmc-workload` (FETCH_SIZE x
@@ -469,7 +471,7 @@ def main():
 
     kernels = {}
     if not args.no_profile:
-        kernels.update(profile_kernels(sim, opts["mixed"], snap, ["weno_stage", "reinit", "gfm"]))
+        kernels.update(profile_kernels(sim, opts["mixed"], snap, ["weno_stage", "reinit", "gfm", "momentum"]))
         if best_name == "mixed":
             kernels.update(profile_kernels(sim, opts["mixed"], snap, ["pcg32_spmv", "pcg32_update"]))
         elif best_name != "fp64":

@@ -386,7 +386,7 @@ int asd2_timer_stop(asd2_ctx* ctx, double* ms);
 /* per-launch timing of one kernel family (hipEvents around each of its launches):
  * kernel_id is one of ASD2_KID_*, 0 = off; reading resets the counters */
 #define ASD2_KID_WENO_STAGE 1     /* fused HJ-WENO5 + SSP-RK3 stage */
-#define ASD2_KID_REINIT 2         /* MinRK2 substages */
+#define ASD2_KID_REINIT 2         /* MinRK2 iterations (one launch each where the P sides allow) */
 #define ASD2_KID_PCG64_SPMV 3     /* FP64 stencil SpMV + direction update + dot */
 #define ASD2_KID_PCG64_UPDATE 4   /* FP64 x/r/z update + dots */
 #define ASD2_KID_PCG32_SPMV 5     /* FP32 counterparts */
@@ -399,6 +399,7 @@ int asd2_timer_stop(asd2_ctx* ctx, double* ms);
 #define ASD2_KID_FAST_PASS 12     /* fast Poisson: boundary values + gather, scatter */
 #define ASD2_KID_FAST_GEMM 13     /* fast Poisson: the two eigenbasis transforms (rocBLAS DGEMM) */
 #define ASD2_KID_FAST_TRIDIAG 14  /* fast Poisson: tridiagonal solves along x, one per y mode */
+#define ASD2_KID_MOMENTUM 15      /* momentum predictor: one U-face or V-face launch of an RK stage */
 int asd2_profile_kernel(asd2_ctx* ctx, int kernel_id);
 int asd2_profile_read(asd2_ctx* ctx, int* launches, double* total_ms);
 /* the same records one by one: the first min(capacity, *launches) durations in launch order */
