@@ -1583,27 +1583,71 @@ __device__ __forceinline__ double ml_apply(const Level& L, const double* v, int 
            L.acy[k] * ml_at(L, v, i, j - 1) + L.acy[k + 1] * ml_at(L, v, i, j + 1);
 }
 
-// Jacobi sweep y + (g - A y) / dc at level cell (i, j) from a tile of y with a halo (y[a][b]
-// at the cell's tile position; 0 outside the level)
-// Chebyshev step k at level cell (i, j) from a tile of y with a halo (y[a][b] at the cell's tile
-// position; 0 outside the level) and the cell's previous direction d: returns y + d', d' in d
+// A block's TH x TW window of a level array (row-major, leading dimension ld, nrows x ncols), top-left
+// (ib, jb), 0 outside: every load issued before any value is used.
+template <int TH, int TW, int NT>
+__device__ __forceinline__ void lvl_fetch(const double* __restrict__ a, int ld, int nrows, int ncols,
+                                          TileRegs<TH, TW, NT>& t, int ib, int jb) {
+#pragma unroll
+    for (int k = 0; k < TileRegs<TH, TW, NT>::NL; k++) {
+        const int e = static_cast<int>(threadIdx.x) + k * NT;
+        const int r = e / TW, c = e - r * TW;
+        const int i = ib + r, j = jb + c;
+        const bool in = e < TH * TW && i >= 0 && i < nrows && j >= 0 && j < ncols;
+        const double x = a[in ? static_cast<int64_t>(i) * ld + j : 0];
+        t.v[k] = in ? x : 0.0;
+    }
+}
+
+// The operator and right-hand side of a level on a tile with the smoothing halo, staged in LDS:
+// ax[a][b] = acx at region cell (a, b) (its W face; ax[a + 1][b] its E face), ay likewise along j,
+// dc, g; the region's top-left is level cell (i0 - ML_SW, j0 - ML_SW).
 template <int HW>
-__device__ __forceinline__ double ml_step(const Level& L, const Cheb& ch, int k, const double (*y)[HW], double& d,
-                                          int a, int b, int i, int j) {
-    const int sc = L.W + 1, kk = i * sc + j;
-    const double ay = L.dc[kk] * y[a][b] + L.acx[kk] * y[a - 1][b] + L.acx[kk + sc] * y[a + 1][b] +
-                      L.acy[kk] * y[a][b - 1] + L.acy[kk + 1] * y[a][b + 1];
-    const double z = (L.g[static_cast<int64_t>(i) * L.W + j] - ay) / L.dc[kk];
+struct LevelTile {
+    double ax[HW + 1][HW + 1], ay[HW + 1][HW + 1], dc[HW][HW], g[HW][HW];
+};
+
+template <int HW, int NT>
+struct LevelTileRegs {
+    TileRegs<HW + 1, HW + 1, NT> ax, ay;
+    TileRegs<HW, HW, NT> dc, g;
+    __device__ __forceinline__ void fetch(const Level& L, int i0, int j0) {
+        const int sc = L.W + 1;
+        lvl_fetch(L.acx, sc, L.H + 1, L.W + 1, ax, i0 - ML_SW, j0 - ML_SW);
+        lvl_fetch(L.acy, sc, L.H + 1, L.W + 1, ay, i0 - ML_SW, j0 - ML_SW);
+        lvl_fetch(L.dc, sc, L.H + 1, L.W + 1, dc, i0 - ML_SW, j0 - ML_SW);
+        lvl_fetch(L.g, L.W, L.H, L.W, g, i0 - ML_SW, j0 - ML_SW);
+    }
+    __device__ __forceinline__ void store(LevelTile<HW>& s) const {
+        tile_store(ax, s.ax);
+        tile_store(ay, s.ay);
+        tile_store(dc, s.dc);
+        tile_store(g, s.g);
+    }
+};
+
+// A y at region cell (a, b) from the staged operator (the expression of ml_apply / ml_step)
+template <int HW>
+__device__ __forceinline__ double ml_apply_s(const LevelTile<HW>& O, const double (*y)[HW], int a, int b) {
+    return O.dc[a][b] * y[a][b] + O.ax[a][b] * y[a - 1][b] + O.ax[a + 1][b] * y[a + 1][b] + O.ay[a][b] * y[a][b - 1] +
+           O.ay[a][b + 1] * y[a][b + 1];
+}
+
+// ml_step from the staged operator
+template <int HW>
+__device__ __forceinline__ double ml_step_s(const LevelTile<HW>& O, const Cheb& ch, int k, const double (*y)[HW],
+                                            double& d, int a, int b) {
+    const double ay = ml_apply_s<HW>(O, y, a, b);
+    const double z = (O.g[a][b] - ay) / O.dc[a][b];
     d = (k == 0) ? ch.c0 * z : ch.a[k] * d + ch.b[k] * z;
     return y[a][b] + d;
 }
 
-// Chebyshev steps k0 .. k1 - 1 of a tile whose state is in buf[0] on the region of halo
-// ML_SW - h0 (h0 = 0: the whole tile) and whose directions are in dir; each step shrinks the
-// valid region by one cell, step k writes buf[(k - k0 + 1) & 1].  Cells outside the level stay 0.
+// ml_steps from the staged operator
 template <int T>
-__device__ __forceinline__ void ml_steps(const Level& L, const Cheb& ch, double (*buf)[T + 2 * ML_SW][T + 2 * ML_SW],
-                                         double (*dir)[T + 2 * ML_SW], int k0, int k1, int h0, int i0, int j0) {
+__device__ __forceinline__ void ml_steps_s(const Level& L, const LevelTile<T + 2 * ML_SW>& O, const Cheb& ch,
+                                           double (*buf)[T + 2 * ML_SW][T + 2 * ML_SW],
+                                           double (*dir)[T + 2 * ML_SW], int k0, int k1, int h0, int i0, int j0) {
     constexpr int HW = T + 2 * ML_SW;
     for (int k = k0; k < k1; k++) {
         const int s = k - k0;
@@ -1613,8 +1657,7 @@ __device__ __forceinline__ void ml_steps(const Level& L, const Cheb& ch, double 
         for (int e = threadIdx.x; e < n * n; e += T * T) {
             const int a = lo + e / n, b = lo + e % n;
             const int i = i0 - ML_SW + a, j = j0 - ML_SW + b;
-            dst[a][b] = (i >= 0 && i < L.H && j >= 0 && j < L.W) ? ml_step<HW>(L, ch, k, src, dir[a][b], a, b, i, j)
-                                                                 : 0.0;
+            dst[a][b] = (i >= 0 && i < L.H && j >= 0 && j < L.W) ? ml_step_s<HW>(O, ch, k, src, dir[a][b], a, b) : 0.0;
         }
         __syncthreads();
     }
@@ -1622,41 +1665,47 @@ __device__ __forceinline__ void ml_steps(const Level& L, const Cheb& ch, double 
 
 // pre-smoothing and restriction on a level run by the whole chip: y1 = the ML_SW Chebyshev
 // steps from 0, next.g = block sums of g - A y1.  A block of T x T threads owns a T x T tile
-// (whole blocks of the next level: c divides T) and smooths it with a halo of ML_SW cells.
+// (whole blocks of the next level: c divides T) and smooths it with a halo of ML_SW cells.  The
+// tile's operator and g are staged in LDS in one batch of loads issued with the read of the solver
+// state, so a launch waits for memory about once.
 template <int T>
 __global__ __launch_bounds__(T * T) void k_ml_down(Level L, Level next, const KrylovState* st, Cheb ch) {
-    if (st->done != DONE_RUNNING) return;
-    constexpr int ML_T = T, ML_HW = T + 2 * ML_SW, NT = T * T;
-    __shared__ double sY[2][ML_HW][ML_HW];
-    __shared__ double sD[ML_HW][ML_HW];
-    __shared__ double sR[ML_T][ML_T];
-    const int tiles_j = ceil_div(L.W, ML_T), ntiles = ceil_div(L.H, ML_T) * tiles_j;
-    const int c = L.c, nb = ML_T / c, sc = L.W + 1;
-    const int a = threadIdx.x / ML_T, b = threadIdx.x - a * ML_T;
+    constexpr int HW = T + 2 * ML_SW, NT = T * T;
+    __shared__ double sY[2][HW][HW];
+    __shared__ double sD[HW][HW];
+    __shared__ double sR[T][T];
+    __shared__ LevelTile<HW> sO;
+    const int done = st->done;
+    const int tiles_j = ceil_div(L.W, T), ntiles = ceil_div(L.H, T) * tiles_j;
+    const int c = L.c, nb = T / c;
+    const int a = threadIdx.x / T, b = threadIdx.x - a * T;
     constexpr int F = (ML_SW - 1) & 1;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int ti = t / tiles_j, tj = t - ti * tiles_j;
-        const int i0 = ti * ML_T, j0 = tj * ML_T;
-        for (int e = threadIdx.x; e < ML_HW * ML_HW; e += NT) {  // step 0 from y = 0: d = y = c0 g / dc
-            const int aa = e / ML_HW, bb = e - aa * ML_HW;
+        const int i0 = ti * T, j0 = tj * T;
+        LevelTileRegs<HW, NT> tr;
+        tr.fetch(L, i0, j0);
+        if (done != DONE_RUNNING) return;
+        tr.store(sO);
+        __syncthreads();
+        for (int e = threadIdx.x; e < HW * HW; e += NT) {  // step 0 from y = 0: d = y = c0 g / dc
+            const int aa = e / HW, bb = e - aa * HW;
             const int i = i0 - ML_SW + aa, j = j0 - ML_SW + bb;
-            const double y0 = (i >= 0 && i < L.H && j >= 0 && j < L.W)
-                                  ? ch.c0 * (L.g[static_cast<int64_t>(i) * L.W + j] / L.dc[i * sc + j])
-                                  : 0.0;
+            const double y0 =
+                (i >= 0 && i < L.H && j >= 0 && j < L.W) ? ch.c0 * (sO.g[aa][bb] / sO.dc[aa][bb]) : 0.0;
             sY[0][aa][bb] = y0;
             sD[aa][bb] = y0;
         }
         __syncthreads();
-        ml_steps<T>(L, ch, sY, sD, 1, ML_SW, 0, i0, j0);
+        ml_steps_s<T>(L, sO, ch, sY, sD, 1, ML_SW, 0, i0, j0);
         const int i = i0 + a, j = j0 + b;
         double rv = 0.0;
         if (i < L.H && j < L.W) {
-            const int k = i * sc + j, A = a + ML_SW, B = b + ML_SW;
-            const double (*y)[ML_HW] = sY[F];
-            const double ay = L.dc[k] * y[A][B] + L.acx[k] * y[A - 1][B] + L.acx[k + sc] * y[A + 1][B] +
-                              L.acy[k] * y[A][B - 1] + L.acy[k + 1] * y[A][B + 1];
+            const int A = a + ML_SW, B = b + ML_SW;
+            const double (*y)[HW] = sY[F];
+            const double ay = ml_apply_s<HW>(sO, y, A, B);
             L.y1[static_cast<int64_t>(i) * L.W + j] = y[A][B];
-            rv = L.g[static_cast<int64_t>(i) * L.W + j] - ay;
+            rv = sO.g[A][B] - ay;
         }
         sR[a][b] = rv;
         __syncthreads();
@@ -1764,46 +1813,64 @@ __global__ __launch_bounds__(1024) void k_ml_lower(Hierarchy Hh, int first, cons
 }
 
 // prolongation and post-smoothing on a level run by the whole chip: y = the ML_SW Chebyshev
-// steps from y2 = y1 + P y_next (on ML_T x ML_T tiles with the sweeps' halo), into L.y, or into yout
+// steps from y2 = y1 + P y_next (on T x T tiles with the sweeps' halo), into L.y, or into yout
 // (FP32) on level 0, where the block partials (0, 0, y.P^T r, y.P^T q_1) of the fine
 // iteration's scalars go to pc (the step along P y is fixed at 1, so y.g and y.A_c y are not
-// needed)
+// needed).  The tile's operator, y1, the coarse values under it and (level 0) the coarse dots'
+// operands are loaded in one batch with the read of the solver state.
 template <int T>
 __global__ __launch_bounds__(T * T) void k_ml_up(Level L, Level next, float* __restrict__ yout, const double* rc,
                                                  const double* qc, const KrylovState* st, double* pc, Cheb ch) {
-    if (st->done != DONE_RUNNING) return;
-    constexpr int ML_T = T, ML_HW = T + 2 * ML_SW, NT = T * T;
-    __shared__ double sY[2][ML_HW][ML_HW];
-    __shared__ double sD[ML_HW][ML_HW];
-    const int tiles_j = ceil_div(L.W, ML_T), ntiles = ceil_div(L.H, ML_T) * tiles_j;
-    const int a = threadIdx.x / ML_T, b = threadIdx.x - a * ML_T;
+    constexpr int HW = T + 2 * ML_SW, NT = T * T;
+    __shared__ double sY[2][HW][HW];
+    __shared__ double sD[HW][HW];
+    __shared__ LevelTile<HW> sO;
+    const int done = st->done;
+    const int tiles_j = ceil_div(L.W, T), ntiles = ceil_div(L.H, T) * tiles_j;
+    const int a = threadIdx.x / T, b = threadIdx.x - a * T;
     constexpr int F = (ML_SW - 1) & 1;
     double a_r = 0.0, a_q = 0.0;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int ti = t / tiles_j, tj = t - ti * tiles_j;
-        const int i0 = ti * ML_T, j0 = tj * ML_T;
-        for (int e = threadIdx.x; e < ML_HW * ML_HW; e += NT) {  // y2 = y1 + P y_next
-            const int aa = e / ML_HW, bb = e - aa * ML_HW;
+        const int i0 = ti * T, j0 = tj * T;
+        LevelTileRegs<HW, NT> tr;
+        tr.fetch(L, i0, j0);
+        TileRegs<HW, HW, NT> ty1, tyn;
+        lvl_fetch(L.y1, L.W, L.H, L.W, ty1, i0 - ML_SW, j0 - ML_SW);
+#pragma unroll
+        for (int k = 0; k < TileRegs<HW, HW, NT>::NL; k++) {  // P y_next at the region's cells
+            const int e = static_cast<int>(threadIdx.x) + k * NT;
+            const int aa = e / HW, bb = e - aa * HW;
             const int i = i0 - ML_SW + aa, j = j0 - ML_SW + bb;
-            sY[0][aa][bb] = (i >= 0 && i < L.H && j >= 0 && j < L.W)
-                                ? L.y1[static_cast<int64_t>(i) * L.W + j] +
-                                      next.y[static_cast<int64_t>(i / L.c) * next.W + j / L.c]
-                                : 0.0;
+            const bool in = e < HW * HW && i >= 0 && i < L.H && j >= 0 && j < L.W;
+            const double x = next.y[in ? static_cast<int64_t>(i / L.c) * next.W + j / L.c : 0];
+            tyn.v[k] = in ? x : 0.0;
+        }
+        const int i = i0 + a, j = j0 + b;
+        const bool mine = i < L.H && j < L.W;
+        const int64_t e0 = mine ? static_cast<int64_t>(i) * L.W + j : 0;
+        const double rcv = yout ? rc[e0] : 0.0, qcv = yout ? qc[e0] : 0.0;
+        if (done != DONE_RUNNING) return;
+        tr.store(sO);
+#pragma unroll
+        for (int k = 0; k < TileRegs<HW, HW, NT>::NL; k++) {  // y2 = y1 + P y_next (0 outside the level)
+            const int e = static_cast<int>(threadIdx.x) + k * NT;
+            if ((HW * HW) % NT == 0 || e < HW * HW) {
+                const int aa = e / HW, bb = e - aa * HW;
+                sY[0][aa][bb] = ty1.v[k] + tyn.v[k];
+            }
         }
         __syncthreads();
-        ml_steps<T>(L, ch, sY, sD, 0, ML_SW - 1, 0, i0, j0);
-        const int i = i0 + a, j = j0 + b;
-        if (i < L.H && j < L.W) {
-            const int64_t e = static_cast<int64_t>(i) * L.W + j;
-            const double y = ml_step<ML_HW>(L, ch, ML_SW - 1, sY[F], sD[a + ML_SW][b + ML_SW], a + ML_SW, b + ML_SW,
-                                            i, j);
+        ml_steps_s<T>(L, sO, ch, sY, sD, 0, ML_SW - 1, 0, i0, j0);
+        if (mine) {
+            const double y = ml_step_s<HW>(sO, ch, ML_SW - 1, sY[F], sD[a + ML_SW][b + ML_SW], a + ML_SW, b + ML_SW);
             if (yout) {
                 const float yf = static_cast<float>(y);
-                yout[e] = yf;
-                a_r += static_cast<double>(yf) * rc[e];
-                a_q += static_cast<double>(yf) * qc[e];
+                yout[e0] = yf;
+                a_r += static_cast<double>(yf) * rcv;
+                a_q += static_cast<double>(yf) * qcv;
             } else {
-                L.y[e] = y;
+                L.y[e0] = y;
             }
         }
         __syncthreads();
