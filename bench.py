@@ -53,9 +53,9 @@ BYTES_PER_CELL = {
 # 2.0 + WRITE_SIZE x 1.0: the factors that make both counters read 256 MiB each on the plain copy
 # asd2_stream_copy of 2^25 doubles); the roofline's `traffic` is this figure x cells.
 MEASURED_BYTES_PER_CELL = {
-    "pcg64_spmv": 61.6, "pcg64_update": 64.4, "pcg32_spmv": 31.3, "pcg32_update": 32.2,
-    "fcg_spmv": 31.0, "fcg_update": 41.4, "weno_stage": (41.3 + 51.3 + 51.1) / 3,
-    "gfm": 100.6,
+    "pcg64_spmv": 61.5, "pcg64_update": 64.4, "pcg32_spmv": 31.2, "pcg32_update": 32.2,
+    "fcg_spmv": 31.7, "fcg_update": 41.4, "weno_stage": (40.1 + 50.3 + 50.2) / 3,
+    "reinit": 30.6, "gfm": 100.6, "momentum": (59.9 + 58.9) / 2,
 }
 
 
