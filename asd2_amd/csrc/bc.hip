@@ -17,7 +17,11 @@ struct BcArgs {
 // S, N of ApplyBC_P_2D (bc2d.cpp:346-353) on every line even where a side could reach the
 // opposite side's ghosts (grids below g cells are refused at context creation); the lines of
 // different threads never overlap (W/E lines cover interior j, S/N lines interior i, corner
-// ghosts are never written, bc2d.cpp:810-832).
+// ghosts are never written, bc2d.cpp:810-832).  Periodic sides copy the opposite interior rows
+// (bc2d.cpp:786-808); the reference's east side writes rows nx .. nx + g - 1, inside the domain,
+// from rows g .. 2g - 1, and leaves the east ghosts as they were: reproduced as written.  Those
+// interior rows are what the S/N lines read, so a grid with a periodic side fills its W/E lines
+// and its S/N lines in two launches.
 __device__ __forceinline__ void bc_p_side(const Geom& G, const BcArgs& A, double* __restrict__ a, int side, int line) {
     const int nx = G.nx, ny = G.ny, g = G.g;
     const int k = A.kind[side];
@@ -26,9 +30,12 @@ __device__ __forceinline__ void bc_p_side(const Geom& G, const BcArgs& A, double
     const bool dir = (k == ASD2_BC_DIRICHLET || k == ASD2_BC_INLET || k == ASD2_BC_OUTLET ||
                       k == ASD2_BC_PRESSURE);
     const bool lsf = (k == ASD2_BC_LSFREE);
+    const bool per = (k == ASD2_BC_PERIODIC);
     if (side == 0) {
         const int j = g + line;
-        if (lsf) {
+        if (per) {
+            for (int i = 0; i < g; i++) a[G.idx(i, j)] = a[G.idx(nx + i, j)];
+        } else if (lsf) {
             for (int i = g - 1; i >= 0; i--)
                 a[G.idx(i, j)] = a[G.idx(i + 1, j)] + G.dx * (a[G.idx(i + 1, j)] - a[G.idx(i + 2, j)]) / G.dx;
         } else if (neu || dir) {
@@ -39,7 +46,9 @@ __device__ __forceinline__ void bc_p_side(const Geom& G, const BcArgs& A, double
         }
     } else if (side == 1) {
         const int j = g + line;
-        if (lsf) {
+        if (per) {
+            for (int i = 0; i < g; i++) a[G.idx(nx + i, j)] = a[G.idx(i + g, j)];
+        } else if (lsf) {
             for (int i = 0; i < g; i++)
                 a[G.idx(nx + g + i, j)] = a[G.idx(nx + g + i - 1, j)]
                     + G.dx * (a[G.idx(nx + g + i - 1, j)] - a[G.idx(nx + g + i - 2, j)]) / G.dx;
@@ -51,7 +60,9 @@ __device__ __forceinline__ void bc_p_side(const Geom& G, const BcArgs& A, double
         }
     } else if (side == 2) {
         const int i = g + line;
-        if (lsf) {
+        if (per) {
+            for (int j = 0; j < g; j++) a[G.idx(i, j)] = a[G.idx(i, ny + j)];
+        } else if (lsf) {
             for (int j = g - 1; j >= 0; j--)
                 a[G.idx(i, j)] = a[G.idx(i, j + 1)] + G.dy * (a[G.idx(i, j + 1)] - a[G.idx(i, j + 2)]) / G.dy;
         } else if (neu || dir) {
@@ -62,7 +73,9 @@ __device__ __forceinline__ void bc_p_side(const Geom& G, const BcArgs& A, double
         }
     } else {
         const int i = g + line;
-        if (lsf) {
+        if (per) {
+            for (int j = 0; j < g; j++) a[G.idx(i, ny + j + g)] = a[G.idx(i, j + g)];
+        } else if (lsf) {
             for (int j = 0; j < g; j++)
                 a[G.idx(i, ny + g + j)] = a[G.idx(i, ny + g + j - 1)]
                     + G.dy * (a[G.idx(i, ny + g + j - 1)] - a[G.idx(i, ny + g + j - 2)]) / G.dy;
@@ -75,8 +88,10 @@ __device__ __forceinline__ void bc_p_side(const Geom& G, const BcArgs& A, double
     }
 }
 
-__global__ __launch_bounds__(256) void k_bc_p(Geom G, BcArgs A, double* __restrict__ a) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// lines first .. last - 1 of the W/E (t < ny) and S/N (t >= ny) line numbering
+__global__ __launch_bounds__(256) void k_bc_p(Geom G, BcArgs A, double* __restrict__ a, int first, int last) {
+    const int t = first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= last) return;
     if (t < G.ny) {
         bc_p_side(G, A, a, 0, t);
         bc_p_side(G, A, a, 1, t);
@@ -87,16 +102,20 @@ __global__ __launch_bounds__(256) void k_bc_p(Geom G, BcArgs A, double* __restri
 }
 
 int bc_apply_p(asd2_ctx* ctx, double* d) {
-    for (int s = 0; s < 4; s++) {
-        if (ctx->bc[ASD2_GRID_P][s] == ASD2_BC_PERIODIC) {
-            set_error("periodic P boundaries are not supported on the GPU path");
-            return ASD2_ERR_UNSUPPORTED;
-        }
-    }
     BcArgs A;
-    for (int s = 0; s < 4; s++) { A.kind[s] = ctx->bc[ASD2_GRID_P][s]; A.c[s] = ctx->bcc[ASD2_GRID_P][s]; }
-    const int lines = ctx->geo.nx + ctx->geo.ny;
-    hipLaunchKernelGGL(k_bc_p, dim3(cdiv(lines, 256)), dim3(256), 0, ctx->stream, ctx->geo, A, d);
+    bool periodic = false;
+    for (int s = 0; s < 4; s++) {
+        A.kind[s] = ctx->bc[ASD2_GRID_P][s];
+        A.c[s] = ctx->bcc[ASD2_GRID_P][s];
+        periodic = periodic || A.kind[s] == ASD2_BC_PERIODIC;
+    }
+    const int nx = ctx->geo.nx, ny = ctx->geo.ny;
+    if (periodic) {  // W/E lines, then S/N lines (see bc_p_side)
+        hipLaunchKernelGGL(k_bc_p, dim3(cdiv(ny, 256)), dim3(256), 0, ctx->stream, ctx->geo, A, d, 0, ny);
+        hipLaunchKernelGGL(k_bc_p, dim3(cdiv(nx, 256)), dim3(256), 0, ctx->stream, ctx->geo, A, d, ny, ny + nx);
+    } else {
+        hipLaunchKernelGGL(k_bc_p, dim3(cdiv(nx + ny, 256)), dim3(256), 0, ctx->stream, ctx->geo, A, d, 0, nx + ny);
+    }
     ASD2_LAUNCH_CHECK();
     return ASD2_OK;
 }

@@ -326,7 +326,8 @@ __global__ __launch_bounds__(WTH) void k_absgrad(Geom G, const double* __restric
 // Sussman (SUBCELL = false, sign(phi0)): A writes LS1 without ghosts (:344-348),
 // B writes tmp = (phi + LS2)/2 with ghosts (:351-363).
 // SUB 2 (first-time Sussman, :405-421): LS2 = phi - dtau sign(phi0) (|grad LS1| - 1),
-// phi = (phi + LS2)/2, i.e. the second stage restarts from phi instead of LS1.
+// phi = (phi + LS2)/2, i.e. the second stage restarts from phi instead of LS1.  SUB 3: that LS2
+// alone (the average left to k_ls_average, for grids whose P BC must run on LS2 first).
 template <bool SUBCELL, int SUB>
 __global__ __launch_bounds__(WTH) void k_reinit(Geom G, GhostRule R, const double* __restrict__ src,
                                                 const double* __restrict__ ls0, const double* phi_c,
@@ -348,14 +349,24 @@ __global__ __launch_bounds__(WTH) void k_reinit(Geom G, GhostRule R, const doubl
         {
 #pragma clang fp contract(off)
             const double S = SUBCELL ? smoothed_sign(I0, eps) : static_cast<double>(isign(I0));
-            const double base = (SUB == 2) ? phi_c[G.idx(i, j)] : sL[r + RH][tj + RH];
+            const double base = (SUB == 2 || SUB == 3) ? phi_c[G.idx(i, j)] : sL[r + RH][tj + RH];
             const double x = base - adt * S * (a - 1.0);
-            if (SUB == 0) val = x;
+            if (SUB == 0 || SUB == 3) val = x;
             else val = 0.5 * (phi_c[G.idx(i, j)] + x);
         }
         out[G.idx(i, j)] = val;
         if (ghosts) write_p_ghosts(G, R, out, i, j, val);
     }
+}
+
+// phi = (phi + LS2) / 2 on the interior cells (the reinitialisations' average, taken after the P BC
+// of LS2 where a periodic east side makes that BC change interior rows)
+__global__ __launch_bounds__(256) void k_ls_average(Geom G, double* __restrict__ phi, const double* __restrict__ ls2) {
+    const int j = G.g + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = G.g + blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= G.nx + G.g || j >= G.ny + G.g) return;
+    const int64_t c = G.idx(i, j);
+    phi[c] = 0.5 * (phi[c] + ls2[c]);
 }
 
 // |grad phi| with the subcell fix from a phi tile and a phi0 tile of different halos: (RL, CL) and
@@ -726,13 +737,10 @@ int finish_ghosts(asd2_ctx* ctx, bool fused, double* d) {
     return bc_apply_p(ctx, d);
 }
 
+// The ghost rule the stage kernels evaluate themselves; `fused` is false when a side's rule needs
+// the whole array (LS-free extrapolation, periodic copies: the kernels then leave the ghosts and the
+// P BC runs after them, as ApplyBC_P_2D does in the reference)
 int p_rule(asd2_ctx* ctx, GhostRule* R, bool* fused) {
-    for (int s = 0; s < 4; s++) {
-        if (ctx->bc[ASD2_GRID_P][s] == ASD2_BC_PERIODIC) {
-            set_error("periodic P boundaries are not supported on the GPU level-set path");
-            return ASD2_ERR_UNSUPPORTED;
-        }
-    }
     *R = ghost_rule_p(ctx, fused);  // (grids are at least g cells wide: every mirror is interior)
     return ASD2_OK;
 }
@@ -743,6 +751,32 @@ bool has_custom(const asd2_ctx* ctx) {
     for (int s = 0; s < 4; s++)
         if (ctx->bc[ASD2_GRID_P][s] == ASD2_BC_CUSTOM) return true;
     return false;
+}
+
+bool has_periodic(const asd2_ctx* ctx) {
+    for (int s = 0; s < 4; s++)
+        if (ctx->bc[ASD2_GRID_P][s] == ASD2_BC_PERIODIC) return true;
+    return false;
+}
+
+// Substage B and the average of one reinitialisation iteration with the P BC of LS2 in between
+// (Reinit_MinRK2_2D, Reinit_Original_2D, FirstTimeOnlyReinit_Sussman_2D: src/levelset2d.cpp:474,
+// :299, :419), for grids with a periodic side.  The reference's periodic east fill writes interior
+// rows (bc2d.cpp:792-796), so there a BC changes interior values and two BCs differ from one: these
+// grids replay the reference's sequence of BCs exactly.  LS2 into ls2 (SUB 0 from LS1, SUB 3 from
+// phi), BC(LS2), phi = (phi + LS2) / 2, and with bc_avg the BC of the reference's average buffer
+// (MinRK2 and Original; its interior is what they copy back).
+template <bool SUBCELL, int SUB>
+int reinit_b_periodic(asd2_ctx* ctx, const GhostRule& R, const double* ls1, const double* ls0, double* d_ls,
+                      double* ls2, bool bc_avg) {
+    const Geom& G = ctx->geo;
+    const Launch L = tiles(G, RJ, RI);
+    hipLaunchKernelGGL((k_reinit<SUBCELL, SUB>), L.grid, L.block, 0, ctx->stream, G, R, ls1, ls0, d_ls, ls2, 0);
+    ASD2_LAUNCH_CHECK();
+    ASD2_TRY(bc_apply_p(ctx, ls2));
+    hipLaunchKernelGGL(k_ls_average, dim3(cdiv(G.ny, 64), cdiv(G.nx, 4)), dim3(256), 0, ctx->stream, G, d_ls, ls2);
+    ASD2_LAUNCH_CHECK();
+    return bc_avg ? bc_apply_p(ctx, d_ls) : ASD2_OK;
 }
 
 // The three SSP-RK3 stages of Solve_LevelSet_2D when every P side's ghost rule can be fused into
@@ -812,8 +846,9 @@ int asd2_ls_advect(asd2_ctx* ctx, double* d_ls, const double* d_u, const double*
     double *p1, *p2;
     ASD2_TRY(ws(ctx, "ls_stage1", n, &p1));
     ASD2_TRY(ws(ctx, "ls_stage2", n, &p2));
-    if (has_custom(ctx)) {  // fresh stage arrays keep zero ghosts on CUSTOM sides
-        ASD2_TRY(zero_async(ctx, p1, n * 8));
+    const bool periodic = has_periodic(ctx);
+    if (has_custom(ctx) || periodic) {  // fresh stage arrays: zero ghosts on CUSTOM sides and in the
+        ASD2_TRY(zero_async(ctx, p1, n * 8));  // east ghosts a periodic east side never writes
         ASD2_TRY(zero_async(ctx, p2, n * 8));
     }
     ASD2_TRY(bc_apply_p(ctx, d_ls));
@@ -831,10 +866,14 @@ int asd2_ls_advect(asd2_ctx* ctx, double* d_ls, const double* d_u, const double*
         ASD2_TRY(bc_apply_p(ctx, vc));
         launch_weno<1, true>(ctx, R, d_ls, d_ls, uc, vc, p1, dt, 0);
         ASD2_LAUNCH_CHECK();
+        // LS1 and LS2 get the P BC twice in the reference (after the stage and inside the next
+        // HJWENO5_LS_2D); with a periodic side the second one changes them again
         ASD2_TRY(bc_apply_p(ctx, p1));
+        if (periodic) ASD2_TRY(bc_apply_p(ctx, p1));
         launch_weno<2, true>(ctx, R, p1, d_ls, uc, vc, p2, dt, 0);
         ASD2_LAUNCH_CHECK();
         ASD2_TRY(bc_apply_p(ctx, p2));
+        if (periodic) ASD2_TRY(bc_apply_p(ctx, p2));
         launch_weno<3, true>(ctx, R, p2, d_ls, uc, vc, d_ls, dt, 0);
         ASD2_LAUNCH_CHECK();
         return bc_apply_p(ctx, d_ls);
@@ -849,8 +888,8 @@ int asd2_ls_advect_noalloc(asd2_ctx* ctx, double* d_ls, const double* d_u, const
     bool fused;
     ASD2_TRY(p_rule(ctx, &R, &fused));
     if (!fused) {
-        set_error("asd2_ls_advect_noalloc: level-set-free pressure sides need two more work arrays; "
-                  "use asd2_ls_advect");
+        set_error("asd2_ls_advect_noalloc: level-set-free and periodic pressure sides need two more work "
+                  "arrays; use asd2_ls_advect");
         return ASD2_ERR_UNSUPPORTED;
     }
     const Geom& G = ctx->geo;
@@ -950,16 +989,32 @@ int asd2_ls_reinit_minrk2(asd2_ctx* ctx, double* d_ls, int iterations) {
     }
     const Launch L = tiles(G, RJ, RI);
     const int gh = fused ? 1 : 0;
+    // A periodic side replays the reference's BCs one for one (reinit_b_periodic): the BC of phi
+    // at the start of every iteration (here the first one; later ones end the previous iteration),
+    // BC(LS1), BC(LS2), the BC of the average; LS1 and LS2 start from zero as the reference's
+    // fresh vectors do (a periodic east side never writes their east ghosts).
+    const bool periodic = has_periodic(ctx);
+    double* ls2 = nullptr;
+    if (periodic) {
+        ASD2_TRY(ws(ctx, "ri_ls2", n, &ls2));
+        ASD2_TRY(zero_async(ctx, ls1, n * 8));
+        ASD2_TRY(zero_async(ctx, ls2, n * 8));
+        ASD2_TRY(bc_apply_p(ctx, d_ls));
+    }
     for (int it = 0; it < iterations; it++) {
         prof_begin(ctx, ASD2_KID_REINIT);
         hipLaunchKernelGGL((k_reinit<true, 0>), L.grid, L.block, 0, ctx->stream, G, R, d_ls, ls0, d_ls, ls1, gh);
         prof_end(ctx, ASD2_KID_REINIT);
         ASD2_LAUNCH_CHECK();
         ASD2_TRY(finish_ghosts(ctx, fused, ls1));
-        prof_begin(ctx, ASD2_KID_REINIT);
-        hipLaunchKernelGGL((k_reinit<true, 1>), L.grid, L.block, 0, ctx->stream, G, R, ls1, ls0, d_ls, d_ls, gh);
-        prof_end(ctx, ASD2_KID_REINIT);
-        ASD2_LAUNCH_CHECK();
+        if (periodic) {
+            ASD2_TRY((reinit_b_periodic<true, 0>(ctx, R, ls1, ls0, d_ls, ls2, true)));
+        } else {
+            prof_begin(ctx, ASD2_KID_REINIT);
+            hipLaunchKernelGGL((k_reinit<true, 1>), L.grid, L.block, 0, ctx->stream, G, R, ls1, ls0, d_ls, d_ls, gh);
+            prof_end(ctx, ASD2_KID_REINIT);
+            ASD2_LAUNCH_CHECK();
+        }
         ASD2_TRY(finish_ghosts(ctx, fused, d_ls));
     }
     return ASD2_OK;
@@ -1016,10 +1071,21 @@ int asd2_ls_reinit_original(asd2_ctx* ctx, double* d_ls, int iterations) {
     ASD2_TRY(bc_apply_p(ctx, d_ls));
     const Launch L = tiles(G, RJ, RI);
     const int gh = fused ? 1 : 0;
+    const bool periodic = has_periodic(ctx);  // BCs replayed one for one (reinit_b_periodic)
+    double* ls2 = nullptr;
+    if (periodic) {
+        ASD2_TRY(ws(ctx, "ri_ls2", n, &ls2));
+        ASD2_TRY(zero_async(ctx, ls2, n * 8));
+    }
     for (int it = 0; it < iterations; it++) {
         hipLaunchKernelGGL((k_reinit<false, 0>), L.grid, L.block, 0, ctx->stream, G, R, d_ls, ls0, d_ls, ls1, gh);
         ASD2_LAUNCH_CHECK();
         ASD2_TRY(finish_ghosts(ctx, fused, ls1));
+        if (periodic) {
+            ASD2_TRY((reinit_b_periodic<false, 0>(ctx, R, ls1, ls0, d_ls, ls2, true)));
+            ASD2_TRY(finish_ghosts(ctx, fused, d_ls));
+            continue;
+        }
         hipLaunchKernelGGL((k_reinit<false, 1>), L.grid, L.block, 0, ctx->stream, G, R, ls1, ls0, d_ls, d_ls, gh);
         ASD2_LAUNCH_CHECK();
         ASD2_TRY(finish_ghosts(ctx, fused, d_ls));
@@ -1045,10 +1111,25 @@ int asd2_ls_reinit_first_time(asd2_ctx* ctx, double* d_ls, int iterations) {
     ASD2_TRY(bc_apply_p(ctx, d_ls));
     const Launch L = tiles(G, RJ, RI);
     const int gh = fused ? 1 : 0;
+    // A periodic side replays the reference's BCs one for one: BC of the normalised phi0 (the
+    // normalisation keeps every sign, and only signs are read), BC(LS2) before the average
+    // (reinit_b_periodic); LS2 starts from zero as the reference's fresh vector does.
+    const bool periodic = has_periodic(ctx);
+    double* ls2 = nullptr;
+    if (periodic) {
+        ASD2_TRY(ws(ctx, "ri_ls2", n, &ls2));
+        ASD2_TRY(zero_async(ctx, ls2, n * 8));
+        ASD2_TRY(bc_apply_p(ctx, ls0));
+    }
     for (int it = 0; it < iterations; it++) {
         hipLaunchKernelGGL((k_reinit<false, 0>), L.grid, L.block, 0, ctx->stream, G, R, d_ls, ls0, d_ls, ls1, gh);
         ASD2_LAUNCH_CHECK();
         ASD2_TRY(finish_ghosts(ctx, fused, ls1));
+        if (periodic) {
+            ASD2_TRY((reinit_b_periodic<false, 3>(ctx, R, ls1, ls0, d_ls, ls2, false)));
+            ASD2_TRY(finish_ghosts(ctx, fused, d_ls));
+            continue;
+        }
         hipLaunchKernelGGL((k_reinit<false, 2>), L.grid, L.block, 0, ctx->stream, G, R, ls1, ls0, d_ls, d_ls, gh);
         ASD2_LAUNCH_CHECK();
         ASD2_TRY(finish_ghosts(ctx, fused, d_ls));

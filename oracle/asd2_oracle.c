@@ -634,7 +634,7 @@ void or_reinit_original(const or_grid* gr, double* ls, int iters) {
     const double eps = smin(gr->dx, gr->dy), adt = eps * 0.5;
     const int64_t n = or_size(gr);
     double* ls0 = zalloc(gr); double* a1 = zalloc(gr); double* a2 = zalloc(gr);
-    double* LS1 = zalloc(gr); double* LS2 = zalloc(gr);
+    double* LS1 = zalloc(gr); double* LS2 = zalloc(gr); double* tmp = zalloc(gr);
     memcpy(ls0, ls, sizeof(double) * (size_t)n);
     for (int it = 0; it < iters; it++) {
         or_bc_p(gr, ls);
@@ -651,14 +651,20 @@ void or_reinit_original(const or_grid* gr, double* ls, int iters) {
                 const int64_t c = IDX(gr, i, j);
                 LS2[c] = LS1[c] - adt * isign(ls0[c]) * (a2[c] - 1.0);
             }
+        /* the BCs of LS2 and tmpLS (levelset2d.cpp:299, 305): they change interior rows where the
+         * east P side is periodic (bc2d.cpp:792-796 writes inside the domain) */
+        or_bc_p(gr, LS2);
         for (int i = g; i < nx + g; i++)
             for (int j = g; j < ny + g; j++) {
                 const int64_t c = IDX(gr, i, j);
-                ls[c] = 0.5 * (ls[c] + LS2[c]);
+                tmp[c] = 0.5 * (ls[c] + LS2[c]);
             }
+        or_bc_p(gr, tmp);
+        for (int i = g; i < nx + g; i++)
+            for (int j = g; j < ny + g; j++) ls[IDX(gr, i, j)] = tmp[IDX(gr, i, j)];
     }
     or_bc_p(gr, ls);
-    free(ls0); free(a1); free(a2); free(LS1); free(LS2);
+    free(ls0); free(a1); free(a2); free(LS1); free(LS2); free(tmp);
 }
 
 /* src/levelset2d.cpp:377-431: phi0 normalised by its ENO2 |grad phi0| where that is
@@ -693,6 +699,7 @@ void or_reinit_first_time(const or_grid* gr, double* ls, int iters) {
                 const int64_t c = IDX(gr, i, j);
                 LS2[c] = ls[c] - adt * isign(ls0[c]) * (a2[c] - 1.0);
             }
+        or_bc_p(gr, LS2);  /* levelset2d.cpp:419: changes interior rows where the east P side is periodic */
         for (int i = g; i < nx + g; i++)
             for (int j = g; j < ny + g; j++) {
                 const int64_t c = IDX(gr, i, j);

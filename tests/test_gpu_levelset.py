@@ -273,11 +273,50 @@ def test_reinit_iterations_match_time_loop(oracle):
             assert lib.asd2_ls_reinit_iterations(eps, mt) == oracle.reinit_iterations(eps, mt)
 
 
-def test_periodic_p_refused():
-    ctx = _ctx(16, 16, 0.1, 0.1, ("periodic",) * 4)
-    d = ctx.array(np.zeros(ctx.shape))
+PERIODIC_BCS = [("periodic",) * 4, ("periodic", "periodic", "neumann", "dirichlet"),
+                ("neumann", "dirichlet", "periodic", "periodic")]
+
+
+@pytest.mark.parametrize("bc", PERIODIC_BCS, ids=lambda b: "-".join(b))
+def test_periodic_p_level_set(oracle, bc):
+    """Periodic P sides (src/bc2d.cpp:786-808, reproduced with the east fill that writes interior
+    rows) through the ghost fill, one advection and every reinitialisation, against the oracle,
+    which the CPU tests compare with the reference's own code on the same sides."""
+    nx, ny = 45, 38
+    c_p = (0.02, -0.01, 0.03, 0.0)
+    phi, dx = _level_set(nx, ny, 31)
+    gr = make_grid(nx, ny, dx, dx, bc_p=bc, c_p=c_p)
+    ctx = _ctx(nx, ny, dx, dx, bc, c_p)
+    a = S.random_field(5, (nx + 6, ny + 6))
+    d = ctx.array(a)
+    ctx.bc_apply(A.GRID_P, d)
+    np.testing.assert_array_equal(d.download(), oracle.bc_p(gr, a.copy()))
+    u, v = S.single_vortex(nx, ny, dx, dx)
+    d = ctx.array(phi)
+    ctx.ls_advect(d, ctx.array(u), ctx.array(v), 0.4 * dx)
+    ref = oracle.solve_levelset(gr, phi.copy(), u, v, 0.4 * dx)
+    assert np.abs(d.download() - ref).max() == 0.0
+    wavy = np.ascontiguousarray(phi * (1.0 + 0.3 * np.sin(5 * phi / dx)))
+    for name, k in (("minrk2", 3), ("original", 4), ("sussman", 3), ("first_time", 6)):
+        d = ctx.array(wavy)
+        getattr(ctx, "ls_reinit_" + name)(d, k)
+        ref = getattr(oracle, "reinit_" + name)(gr, wavy.copy(), k)
+        err = np.abs(d.download() - ref).max()
+        print(f"{name} {bc}: max|dphi| = {err:.3e}")
+        assert err <= TOL_PHI
+
+
+def test_periodic_p_refused_by_gfm():
+    """The ghost-fluid assembly keeps refusing periodic P sides (the reference's periodic rows,
+    src/mac2d.cpp:1529-1531, store a density reciprocal as the coupling without the 1/h^2 factor and
+    keep the wrapped column index)."""
+    n = 16
+    dx = 1.0 / n
+    ctx = _ctx(n, n, dx, dx, ("periodic",) * 4)
+    z = ctx.array()
+    fl = A.make_fluid(1000.0, 1.0, 1e-2, 1e-4, 1.0, 1e-4)
     with pytest.raises(A.Asd2Error) as e:
-        ctx.ls_advect(d, ctx.array(), ctx.array(), 0.01)
+        ctx.gfm_assemble(fl, z, z, z, z)
     assert e.value.code == A.ERR_UNSUPPORTED
 
 

@@ -122,7 +122,8 @@ def test_droplet_step(oracle):
 
 # ----------------------------------------------------------------- live, against oracle/_ref
 @pytest.mark.parametrize("seed", [1, 2, 3])
-@pytest.mark.parametrize("kinds", [("neumann",) * 4, ("dirichlet", "neumann", "outlet", "wall")])
+@pytest.mark.parametrize("kinds", [("neumann",) * 4, ("dirichlet", "neumann", "outlet", "wall"), ("periodic",) * 4,
+                                   ("periodic", "periodic", "neumann", "dirichlet")])
 def test_live_levelset_path(oracle, reference, seed, kinds):
     nx, ny = 33 + seed, 29 + 2 * seed
     dx = 1.0 / nx
