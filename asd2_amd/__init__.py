@@ -135,6 +135,12 @@ def load() -> C.CDLL:
         "asd2_mac_kappa_axisym": ([_VP, _VP, _VP], C.c_int),
         "asd2_mac_update_dt_axisym": ([_VP, P(Fluid), _VP, _VP, _VP, P(C.c_double)], C.c_int),
         "asd2_tdma": ([_VP, C.c_int, C.c_int, _VP, _VP, _VP, _VP], C.c_int),
+        "asd2_mac_rhs_u_axisym": ([_VP, P(Fluid), _VP, _VP, _VP, _VP, _VP], C.c_int),
+        "asd2_mac_rhs_v_axisym": ([_VP, P(Fluid), _VP, _VP, _VP, _VP, _VP, _VP], C.c_int),
+        "asd2_mac_momentum_term_axisym": ([_VP, P(Fluid), C.c_int, C.c_int, _VP, _VP, _VP, _VP, _VP, _VP],
+                                          C.c_int),
+        "asd2_mac_uhat_axisym": ([_VP, P(Fluid), _VP, _VP, _VP, _VP], C.c_int),
+        "asd2_mac_vhat_axisym": ([_VP, P(Fluid), _VP, _VP, _VP, _VP], C.c_int),
         "asd2_gfm_export": ([_VP, _VP, _VP, _VP, _VP], C.c_int),
         "asd2_gfm_import": ([_VP, _VP, _VP, _VP, _VP], C.c_int),
         "asd2_gfm_apply": ([_VP, _VP, _VP], C.c_int),
@@ -490,6 +496,23 @@ class Context:
     def gfm_assemble_axisym(self, fl, ls, kap, H, div4p):
         check(self.lib.asd2_gfm_assemble_axisym(self.h, C.byref(fl), ls.ptr, kap.ptr,
                                                 H.ptr if H is not None else None, div4p.ptr))
+
+    def rhs_u_axisym(self, fl, ls, u, v, H, rhs):
+        check(self.lib.asd2_mac_rhs_u_axisym(self.h, C.byref(fl), ls.ptr, u.ptr, v.ptr, H.ptr, rhs.ptr))
+
+    def rhs_v_axisym(self, fl, ls, u, uhat, v, H, rhs):
+        check(self.lib.asd2_mac_rhs_v_axisym(self.h, C.byref(fl), ls.ptr, u.ptr, uhat.ptr, v.ptr, H.ptr, rhs.ptr))
+
+    def momentum_term_axisym(self, fl, grid_kind, term, ls, u, uhat, v, H, out):
+        check(self.lib.asd2_mac_momentum_term_axisym(self.h, C.byref(fl), grid_kind, term, ls.ptr, u.ptr,
+                                                     uhat.ptr if uhat is not None else None, v.ptr, H.ptr,
+                                                     out.ptr))
+
+    def uhat_axisym(self, fl, ls, rhs, H, uhat):
+        check(self.lib.asd2_mac_uhat_axisym(self.h, C.byref(fl), ls.ptr, rhs.ptr, H.ptr, uhat.ptr))
+
+    def vhat_axisym(self, fl, ls, rhs, H, vhat):
+        check(self.lib.asd2_mac_vhat_axisym(self.h, C.byref(fl), ls.ptr, rhs.ptr, H.ptr, vhat.ptr))
 
     def tdma(self, a, b, c, d):
         """count tridiagonal systems of n unknowns ([n][count] device arrays); the solution into d"""

@@ -147,6 +147,19 @@ __device__ __forceinline__ double minmod(double v1, double v2) {
     return (p <= 0) ? 0.0 : __builtin_nan("");
 }
 
+// density of the axisymmetric solver's viscous terms and line sweeps at a segment between the
+// level-set samples a and b: rho_H thetaH + rho_L (1 - thetaH), thetaH the heavy share of the
+// segment (1 or 0 when both samples lie in one phase), src/mac2dAxisym.cpp:686-748, 1024-1049
+__device__ __forceinline__ double rho_eff_axi(double rho_h, double rho_l, double a, double b) {
+#pragma clang fp contract(off)
+    double thH = 0.0;
+    if (a >= 0.0 && b >= 0.0) thH = 1.0;
+    else if (a <= 0.0 && b <= 0.0) thH = 0.0;
+    else if (a >= 0.0 && b < 0.0) thH = fabs(a) / (fabs(a) + fabs(b));
+    else if (a < 0.0 && b >= 0.0) thH = fabs(b) / (fabs(a) + fabs(b));
+    return rho_h * thH + rho_l * (1.0 - thH);
+}
+
 // A TH x TW tile of a padded array held in registers between its loads and its LDS stores.
 // tile_fetch issues the loads (addresses clamped into the array, no branch around a load; cells
 // outside the padded array read as 0) and tile_store writes the LDS tile, so a kernel that fetches

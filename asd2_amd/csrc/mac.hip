@@ -244,8 +244,9 @@ __device__ __forceinline__ double irho_arith(const MomArgs& A, double a, double 
 // 3-face halo, the squared flux where the WENO stencil reads it).
 constexpr int MJ = 64, MI = 16;
 
-// mu at the staged cells: mu_at over Hs values already in registers
-template <int TH, int TW, int NT>
+// mu at the staged cells: mu_at over Hs values already in registers (FULL: over the whole padded
+// array, as the axisymmetric solver computes it, src/mac2dAxisym.cpp:607-609)
+template <bool FULL, int TH, int TW, int NT>
 __device__ __forceinline__ void mu_store(const Geom& G, const TileRegs<TH, TW, NT>& t, double (*s)[TW], int ib, int jb,
                                          double muH, double muL) {
 #pragma unroll
@@ -257,14 +258,17 @@ __device__ __forceinline__ void mu_store(const Geom& G, const TileRegs<TH, TW, N
             double m;
             {
 #pragma clang fp contract(off)
-                m = (i < G.g || i >= G.nx + G.g || j < G.g || j >= G.ny + G.g) ? 0.0 : muL + (muH - muL) * t.v[k];
+                m = (!FULL && (i < G.g || i >= G.nx + G.g || j < G.g || j >= G.ny + G.g)) ? 0.0
+                                                                                      : muL + (muH - muL) * t.v[k];
             }
             s[r][c] = m;
         }
     }
 }
 
-template <bool POW2>
+// AXI: the axisymmetric solver's U right-hand side (GetRHSU, src/mac2dAxisym.cpp:264-292): the same
+// convection, the viscous term of AddExternalViscosityFU (:590-753) and u + dt (-cU + gU + vU)
+template <bool POW2, bool AXI>
 __global__ __launch_bounds__(256) void k_mom_u(Geom G, MomArgs A, const double* __restrict__ u,
                                                const double* __restrict__ v, const double* __restrict__ ls,
                                                const double* __restrict__ Hs, const double* __restrict__ u0,
@@ -303,7 +307,7 @@ __global__ __launch_bounds__(256) void k_mom_u(Geom G, MomArgs A, const double* 
     tile_store(tu, sU);
     tile_store(tv, sV);
     if (A.viscous) {
-        mu_store(G, th, sMu, i0 - 1, j0 - 1, A.mu_h, A.mu_l);
+        mu_store<AXI>(G, th, sMu, i0 - 1, j0 - 1, A.mu_h, A.mu_l);
         tile_store(tl, sLs);
     }
     __syncthreads();
@@ -344,7 +348,25 @@ __global__ __launch_bounds__(256) void k_mom_u(Geom G, MomArgs A, const double* 
     const double aX = sU[R][C], aY = vbar(i, j);
     const double cU = 0.5 * (lxp + lxm) + 0.5 * (lyp + lym) - aX * (0.5 * (lxp - lxm)) - aY * (0.5 * (lyp - lym));
     double dU = 0.0;
-    if (A.viscous) {
+    if (AXI && A.viscous) {
+#pragma clang fp contract(off)
+        // d/dz (mu dv/dr) over the arithmetic-mean density of the S-N half points
+        const int a = i - (i0 - 1), b = j - (j0 - 3);
+        const double vW = sV[a - 1][b], vW_N = sV[a - 1][b + 1], vM = sV[a][b], vN = sV[a][b + 1];
+        const int a0 = r + 1, b0 = tj + 1;
+#define LS(di, dj) sLs[a0 + (di)][b0 + (dj)]
+#define MU(di, dj) sMu[a0 + (di)][b0 + (dj)]
+        const double lsUNHalf = 0.25 * (LS(0, 0) + LS(-1, 0) + LS(-1, 1) + LS(0, 1));
+        const double lsUSHalf = 0.25 * (LS(0, 0) + LS(-1, 0) + LS(-1, -1) + LS(0, -1));
+        const double muUNHalf = 0.25 * (MU(0, 0) + MU(-1, 0) + MU(-1, 1) + MU(0, 1));
+        const double muUSHalf = 0.25 * (MU(0, 0) + MU(-1, 0) + MU(-1, -1) + MU(0, -1));
+#undef LS
+#undef MU
+        const double rhoSN = rho_eff_axi(A.rho_h, A.rho_l, lsUSHalf, lsUNHalf);
+        const double visZ = wdiv<POW2>(wdiv<POW2>(muUNHalf * (vN - vW_N), G.dx, idx)
+                                           - wdiv<POW2>(muUSHalf * (vM - vW), G.dx, idx), G.dy, idy);
+        dU = visZ / rhoSN;
+    } else if (!AXI && A.viscous) {
 #pragma clang fp contract(off)
         const double dx = G.dx, dy = G.dy;
         const double uM = sU[R][C], uW = sU[R - 1][C], uE = sU[R + 1][C];
@@ -376,7 +398,7 @@ __global__ __launch_bounds__(256) void k_mom_u(Geom G, MomArgs A, const double* 
         const double visY = wdiv<POW2>(muU_Y_N - muU_Y_S, dy, idy) + wdiv<POW2>(muV_X_N - muV_X_S, dy, idy);
         dU = visX * iWE + visY * iSN;
     }
-    const double F = -cU + dU + A.gu;
+    const double F = AXI ? (-cU + A.gu) + dU : -cU + dU + A.gu;
     const double t = sU[R][C] + A.dt * F;
     double o;
     if (A.mode == 0) o = t;
@@ -391,11 +413,14 @@ __global__ __launch_bounds__(256) void k_mom_u(Geom G, MomArgs A, const double* 
 }
 
 // -cV + vV + gV at a V face and the RK combination (src/mac2d.cpp:281-306, 389-463, 811-1041)
-template <bool POW2>
+// AXI: the axisymmetric solver's V right-hand side (GetRHSV, src/mac2dAxisym.cpp:294-322): the same
+// convection, the viscous term of AddExternalViscosityFV (:755-910, with the new U-face velocity
+// uh) and v + dt (-cV + gV + vV)
+template <bool POW2, bool AXI>
 __global__ __launch_bounds__(256) void k_mom_v(Geom G, MomArgs A, const double* __restrict__ u,
                                                const double* __restrict__ v, const double* __restrict__ ls,
                                                const double* __restrict__ Hs, const double* __restrict__ v0,
-                                               double* __restrict__ out) {
+                                               const double* __restrict__ uh, double* __restrict__ out) {
     constexpr int VH = MI + 6, VW = MJ + 6;  // v at (i0 - 3 + a, j0 - 3 + b)
     constexpr int UH = MI + 7, UW = MJ + 1;  // u at (i0 - 3 + a, j0 - 1 + b)
     constexpr int CH = MI + 2, CW = MJ + 1;  // mu, level set at cells (i0 - 1 + a, j0 - 1 + b)
@@ -404,6 +429,8 @@ __global__ __launch_bounds__(256) void k_mom_v(Geom G, MomArgs A, const double* 
     __shared__ double sFx[MI + 6][MJ];  // v ubar along i
     __shared__ double sMu[CH][CW];
     __shared__ double sLs[CH][CW];
+    constexpr int HH = AXI ? MI + 1 : 1, HW = AXI ? MJ + 1 : 1;  // uh at (i0 + a, j0 - 1 + b)
+    __shared__ double sUh[HH][HW];
     const int tj = threadIdx.x & (MJ - 1), ti = threadIdx.x / MJ;
     const int j0 = G.g + 1 + blockIdx.x * MJ, i0 = G.g + blockIdx.y * MI;
     const int ipad = G.nx + 2 * G.g, jpad = G.ny + 2 * G.g;
@@ -411,11 +438,13 @@ __global__ __launch_bounds__(256) void k_mom_v(Geom G, MomArgs A, const double* 
     TileRegs<VH, VW, 256> tv;
     TileRegs<UH, UW, 256> tu;
     TileRegs<CH, CW, 256> th, tl;
+    TileRegs<HH, HW, 256> tuh;
     tile_fetch(G, v, tv, i0 - 3, j0 - 3);
     tile_fetch(G, u, tu, i0 - 3, j0 - 1);
     if (A.viscous) {
         tile_fetch(G, Hs, th, i0 - 1, j0 - 1);
         tile_fetch(G, ls, tl, i0 - 1, j0 - 1);
+        if (AXI) tile_fetch(G, uh, tuh, i0, j0 - 1);
     }
     constexpr int NR = MI / (256 / MJ);
     double v0R[NR];
@@ -430,8 +459,9 @@ __global__ __launch_bounds__(256) void k_mom_v(Geom G, MomArgs A, const double* 
     tile_store(tv, sV);
     tile_store(tu, sU);
     if (A.viscous) {
-        mu_store(G, th, sMu, i0 - 1, j0 - 1, A.mu_h, A.mu_l);
+        mu_store<AXI>(G, th, sMu, i0 - 1, j0 - 1, A.mu_h, A.mu_l);
         tile_store(tl, sLs);
+        if (AXI) tile_store(tuh, sUh);
     }
     __syncthreads();
     // tmpU of AddConvectionFV at a V face (src/mac2d.cpp:395-397) from the staged u, 0 elsewhere
@@ -469,7 +499,32 @@ __global__ __launch_bounds__(256) void k_mom_v(Geom G, MomArgs A, const double* 
     const double aX = ubar(i, j), aY = sV[R][C];
     const double cV = 0.5 * (lxp + lxm) + 0.5 * (lyp + lym) - aX * (0.5 * (lxp - lxm)) - aY * (0.5 * (lyp - lym));
     double dV = 0.0;
-    if (A.viscous) {
+    if (AXI && A.viscous) {
+#pragma clang fp contract(off)
+        // d/dr (mu du/dz) over the W-E half points' density, plus mu / r d(u*)/dz over the S-N
+        // cells' density; r without the base radius, as the reference computes it
+        const int a = i - (i0 - 3), b = j - (j0 - 1);
+        const double uM = sU[a][b], uE = sU[a + 1][b], uS = sU[a][b - 1], uS_E = sU[a + 1][b - 1];
+        const int a0 = r + 1, b0 = tj + 1;
+#define LS(di, dj) sLs[a0 + (di)][b0 + (dj)]
+#define MU(di, dj) sMu[a0 + (di)][b0 + (dj)]
+        const double lsVWHalf = 0.25 * (LS(0, 0) + LS(-1, 0) + LS(-1, -1) + LS(0, -1));
+        const double lsVEHalf = 0.25 * (LS(0, 0) + LS(1, 0) + LS(1, -1) + LS(0, -1));
+        const double muVWHalf = 0.25 * (MU(0, 0) + MU(-1, 0) + MU(-1, -1) + MU(0, -1));
+        const double muVEHalf = 0.25 * (MU(0, 0) + MU(1, 0) + MU(1, -1) + MU(0, -1));
+        const double rhoWE = rho_eff_axi(A.rho_h, A.rho_l, lsVWHalf, lsVEHalf);
+        const double rhoSN = rho_eff_axi(A.rho_h, A.rho_l, LS(0, -1), LS(0, 0));
+        const double muSN = 0.5 * (MU(0, 0) + MU(0, -1));
+#undef LS
+#undef MU
+        const double rPM = (i + 0.5 - G.g) * G.dx;
+        const double visR = wdiv<POW2>(wdiv<POW2>(muVEHalf * (uE - uS_E), G.dy, idy)
+                                           - wdiv<POW2>(muVWHalf * (uM - uS), G.dy, idy), G.dx, idx);
+        const int ha = r, hb = tj + 1;  // the face's own U face (i, j) in sUh
+        const double duh = 0.5 * (sUh[ha][hb] + sUh[ha + 1][hb]) - 0.5 * (sUh[ha][hb - 1] + sUh[ha + 1][hb - 1]);
+        const double visZ = wdiv<POW2>(muSN / rPM * duh, G.dy, idy);
+        dV = visR / rhoWE + visZ / rhoSN;
+    } else if (!AXI && A.viscous) {
 #pragma clang fp contract(off)
         const double dx = G.dx, dy = G.dy;
         const double vW = sV[R - 1][C], vE = sV[R + 1][C], vS = sV[R][C - 1];
@@ -501,7 +556,7 @@ __global__ __launch_bounds__(256) void k_mom_v(Geom G, MomArgs A, const double* 
         dV = visX * iWE + visY * iSN;
     }
     const double gv = (A.gv_from_i && i < G.g + 1) ? 0.0 : A.gv;
-    const double F = -cV + dV + gv;
+    const double F = AXI ? (-cV + gv) + dV : -cV + dV + gv;
     const double t = sV[R][C] + A.dt * F;
     double o;
     if (A.mode == 0) o = t;
@@ -536,6 +591,51 @@ int velocity_maxima(asd2_ctx* ctx, const double* d_u, const double* d_v, const d
     ASD2_HIP(hipStreamSynchronize(ctx->stream));
     return ASD2_OK;
 }
+// momentum arguments of a fluid: densities, their reciprocals rounded once on the host, gravity on
+// the U and V faces (the V equation takes the U-grid array unless gravity_fix, src/mac2d.cpp:294,
+// src/mac2dAxisym.cpp:312), kRe > 0 for the viscous terms
+MomArgs mom_args(const asd2_fluid* fl, int mode) {
+    MomArgs A{};
+    A.rho_h = fl->rho_h; A.rho_l = fl->rho_l; A.mu_h = fl->mu_h; A.mu_l = fl->mu_l; A.dt = fl->dt;
+    A.irh = 1.0 / fl->rho_h; A.irl = 1.0 / fl->rho_l; A.irlrh = 1.0 / (fl->rho_l * fl->rho_h);
+    const bool fr_ok = (fl->fr != 0 || fl->g != 0.0) && !std::isnan(fl->fr) && !std::isinf(fl->fr);
+    A.gu = (fr_ok && fl->g_axis == 0) ? -fl->g : 0.0;
+    if (fl->gravity_fix) { A.gv = (fr_ok && fl->g_axis == 1) ? -fl->g : 0.0; A.gv_from_i = 0; }
+    else { A.gv = A.gu; A.gv_from_i = 1; }
+    A.viscous = fl->re > 0.0 ? 1 : 0;
+    A.mode = mode;
+    return A;
+}
+
+// one momentum launch on the interior U (grid_kind = ASD2_GRID_U) or V faces; s0 is the RK stage's
+// u0 / v0, uh the new U-face velocity of the axisymmetric V term
+int launch_mom(asd2_ctx* ctx, const MomArgs& A, int grid_kind, bool axi, const double* us, const double* vs,
+               const double* ls, const double* Hs, const double* s0, const double* uh, double* out) {
+    const Geom& G = ctx->geo;
+    const bool pow2 = is_pow2(G.dx) && is_pow2(G.dy);  // divisions by dx, dy are exact products
+    hipStream_t st = ctx->stream;
+    prof_begin(ctx, ASD2_KID_MOMENTUM);
+    if (grid_kind == ASD2_GRID_U) {
+        const dim3 gr(cdiv(G.ny, MJ), cdiv(G.nx - 1, MI));
+        if (axi && pow2) hipLaunchKernelGGL((k_mom_u<true, true>), gr, dim3(256), 0, st, G, A, us, vs, ls, Hs, s0, out);
+        else if (axi) hipLaunchKernelGGL((k_mom_u<false, true>), gr, dim3(256), 0, st, G, A, us, vs, ls, Hs, s0, out);
+        else if (pow2) hipLaunchKernelGGL((k_mom_u<true, false>), gr, dim3(256), 0, st, G, A, us, vs, ls, Hs, s0, out);
+        else hipLaunchKernelGGL((k_mom_u<false, false>), gr, dim3(256), 0, st, G, A, us, vs, ls, Hs, s0, out);
+    } else {
+        const dim3 gr(cdiv(G.ny - 1, MJ), cdiv(G.nx, MI));
+        if (axi && pow2)
+            hipLaunchKernelGGL((k_mom_v<true, true>), gr, dim3(256), 0, st, G, A, us, vs, ls, Hs, s0, uh, out);
+        else if (axi)
+            hipLaunchKernelGGL((k_mom_v<false, true>), gr, dim3(256), 0, st, G, A, us, vs, ls, Hs, s0, uh, out);
+        else if (pow2)
+            hipLaunchKernelGGL((k_mom_v<true, false>), gr, dim3(256), 0, st, G, A, us, vs, ls, Hs, s0, uh, out);
+        else hipLaunchKernelGGL((k_mom_v<false, false>), gr, dim3(256), 0, st, G, A, us, vs, ls, Hs, s0, uh, out);
+    }
+    prof_end(ctx, ASD2_KID_MOMENTUM);
+    ASD2_LAUNCH_CHECK();
+    return ASD2_OK;
+}
+
 }  // namespace
 }  // namespace asd2
 
@@ -647,14 +747,7 @@ int asd2_mac_intermediate_vel(asd2_ctx* ctx, const asd2_fluid* fl, int time_orde
     }
     const Geom& G = ctx->geo;
     const size_t n = static_cast<size_t>(G.size());
-    MomArgs A{};
-    A.rho_h = fl->rho_h; A.rho_l = fl->rho_l; A.mu_h = fl->mu_h; A.mu_l = fl->mu_l; A.dt = fl->dt;
-    A.irh = 1.0 / fl->rho_h; A.irl = 1.0 / fl->rho_l; A.irlrh = 1.0 / (fl->rho_l * fl->rho_h);
-    const bool fr_ok = (fl->fr != 0 || fl->g != 0.0) && !std::isnan(fl->fr) && !std::isinf(fl->fr);
-    A.gu = (fr_ok && fl->g_axis == 0) ? -fl->g : 0.0;
-    if (fl->gravity_fix) { A.gv = (fr_ok && fl->g_axis == 1) ? -fl->g : 0.0; A.gv_from_i = 0; }
-    else { A.gv = A.gu; A.gv_from_i = 1; }  // mac2d.cpp:294 adds AddGravityFU() to the V equation
-    A.viscous = fl->re > 0.0 ? 1 : 0;
+    MomArgs A = mom_args(fl, 0);
     double *u1, *v1, *u2, *v2;
     ASD2_TRY(ws(ctx, "mom_u1", n, &u1));
     ASD2_TRY(ws(ctx, "mom_v1", n, &v1));
@@ -662,21 +755,11 @@ int asd2_mac_intermediate_vel(asd2_ctx* ctx, const asd2_fluid* fl, int time_orde
     ASD2_TRY(ws(ctx, "mom_v2", n, &v2));
     // fresh stage arrays (GetIntermediateVel allocates zero vectors, mac2d.cpp:1146-1147)
     for (double* p : {u1, v1, u2, v2}) ASD2_TRY(zero_async(ctx, p, n * 8));
-    const dim3 gu(cdiv(G.ny, MJ), cdiv(G.nx - 1, MI)), gv(cdiv(G.ny - 1, MJ), cdiv(G.nx, MI));
-    const bool pow2 = is_pow2(G.dx) && is_pow2(G.dy);  // divisions by dx, dy are exact products
     auto stage = [&](const double* us, const double* vs, const double* u0, const double* v0, double* uo, double* vo,
                      int mode) -> int {
         A.mode = mode;
-        prof_begin(ctx, ASD2_KID_MOMENTUM);
-        if (pow2) hipLaunchKernelGGL(k_mom_u<true>, gu, dim3(256), 0, ctx->stream, G, A, us, vs, d_ls, d_Hs, u0, uo);
-        else hipLaunchKernelGGL(k_mom_u<false>, gu, dim3(256), 0, ctx->stream, G, A, us, vs, d_ls, d_Hs, u0, uo);
-        prof_end(ctx, ASD2_KID_MOMENTUM);
-        prof_begin(ctx, ASD2_KID_MOMENTUM);
-        if (pow2) hipLaunchKernelGGL(k_mom_v<true>, gv, dim3(256), 0, ctx->stream, G, A, us, vs, d_ls, d_Hs, v0, vo);
-        else hipLaunchKernelGGL(k_mom_v<false>, gv, dim3(256), 0, ctx->stream, G, A, us, vs, d_ls, d_Hs, v0, vo);
-        prof_end(ctx, ASD2_KID_MOMENTUM);
-        ASD2_LAUNCH_CHECK();
-        return ASD2_OK;
+        ASD2_TRY(launch_mom(ctx, A, ASD2_GRID_U, false, us, vs, d_ls, d_Hs, u0, nullptr, uo));
+        return launch_mom(ctx, A, ASD2_GRID_V, false, us, vs, d_ls, d_Hs, v0, nullptr, vo);
     };
     if (time_order == 0) {
         ASD2_TRY(stage(d_u, d_v, d_u, d_v, d_uhat, d_vhat, 0));
@@ -705,28 +788,48 @@ int asd2_mac_momentum_term(asd2_ctx* ctx, const asd2_fluid* fl, int grid_kind, i
         set_error("bad argument");
         return ASD2_ERR_ARG;
     }
-    const Geom& G = ctx->geo;
-    MomArgs A{};
-    A.rho_h = fl->rho_h; A.rho_l = fl->rho_l; A.mu_h = fl->mu_h; A.mu_l = fl->mu_l; A.dt = fl->dt;
-    A.irh = 1.0 / fl->rho_h; A.irl = 1.0 / fl->rho_l; A.irlrh = 1.0 / (fl->rho_l * fl->rho_h);
-    const bool fr_ok = (fl->fr != 0 || fl->g != 0.0) && !std::isnan(fl->fr) && !std::isinf(fl->fr);
-    A.gu = (fr_ok && fl->g_axis == 0) ? -fl->g : 0.0;
-    if (fl->gravity_fix) { A.gv = (fr_ok && fl->g_axis == 1) ? -fl->g : 0.0; A.gv_from_i = 0; }
-    else { A.gv = A.gu; A.gv_from_i = 1; }
-    A.viscous = fl->re > 0.0 ? 1 : 0;
-    A.mode = 4 + term;
-    ASD2_TRY(zero_async(ctx, d_out, static_cast<size_t>(G.size()) * 8));
-    const bool pow2 = is_pow2(G.dx) && is_pow2(G.dy);
-    const dim3 gu(cdiv(G.ny, MJ), cdiv(G.nx - 1, MI)), gv(cdiv(G.ny - 1, MJ), cdiv(G.nx, MI));
-    if (grid_kind == ASD2_GRID_U) {
-        if (pow2) hipLaunchKernelGGL(k_mom_u<true>, gu, dim3(256), 0, ctx->stream, G, A, d_u, d_v, d_ls, d_Hs, d_u, d_out);
-        else hipLaunchKernelGGL(k_mom_u<false>, gu, dim3(256), 0, ctx->stream, G, A, d_u, d_v, d_ls, d_Hs, d_u, d_out);
-    } else {
-        if (pow2) hipLaunchKernelGGL(k_mom_v<true>, gv, dim3(256), 0, ctx->stream, G, A, d_u, d_v, d_ls, d_Hs, d_v, d_out);
-        else hipLaunchKernelGGL(k_mom_v<false>, gv, dim3(256), 0, ctx->stream, G, A, d_u, d_v, d_ls, d_Hs, d_v, d_out);
+    ASD2_TRY(zero_async(ctx, d_out, static_cast<size_t>(ctx->geo.size()) * 8));
+    return launch_mom(ctx, mom_args(fl, 4 + term), grid_kind, false, d_u, d_v, d_ls, d_Hs,
+                      grid_kind == ASD2_GRID_U ? d_u : d_v, nullptr, d_out);
+}
+
+// the axisymmetric grids need nr, nz >= 3 (the line sweeps solve nr - 1 and nz - 1 unknowns)
+static int axi_grid_ok(const asd2_ctx* ctx) {
+    if (ctx->geo.nx < 3 || ctx->geo.ny < 3) {
+        set_error("axisymmetric momentum: needs nr >= 3 and nz >= 3");
+        return ASD2_ERR_ARG;
     }
-    ASD2_LAUNCH_CHECK();
     return ASD2_OK;
+}
+
+int asd2_mac_momentum_term_axisym(asd2_ctx* ctx, const asd2_fluid* fl, int grid_kind, int term, const double* d_ls,
+                                  const double* d_u, const double* d_uhat, const double* d_v, const double* d_H,
+                                  double* d_out) {
+    ASD2_TRY(check_ctx(ctx));
+    if (!fl || !d_ls || !d_u || !d_v || !d_H || !d_out || term < 0 || term > 2 ||
+        (grid_kind != ASD2_GRID_U && grid_kind != ASD2_GRID_V) || (grid_kind == ASD2_GRID_V && !d_uhat)) {
+        set_error("bad argument");
+        return ASD2_ERR_ARG;
+    }
+    ASD2_TRY(axi_grid_ok(ctx));
+    if (d_out == d_u || d_out == d_v || d_out == d_uhat) { set_error("d_out aliases an input"); return ASD2_ERR_ARG; }
+    ASD2_TRY(zero_async(ctx, d_out, static_cast<size_t>(ctx->geo.size()) * 8));
+    const int mode = term == ASD2_MOM_RHS ? 0 : 4 + term;
+    // s0 is not read outside the RK modes, but the kernels' clamped loads need a valid address
+    ASD2_TRY(launch_mom(ctx, mom_args(fl, mode), grid_kind, true, d_u, d_v, d_ls, d_H,
+                        grid_kind == ASD2_GRID_U ? d_u : d_v, d_uhat, d_out));
+    if (term == ASD2_MOM_RHS) ASD2_TRY(bc_apply_uv(ctx, grid_kind, d_out));
+    return ASD2_OK;
+}
+
+int asd2_mac_rhs_u_axisym(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_ls, const double* d_u,
+                          const double* d_v, const double* d_H, double* d_rhs) {
+    return asd2_mac_momentum_term_axisym(ctx, fl, ASD2_GRID_U, ASD2_MOM_RHS, d_ls, d_u, nullptr, d_v, d_H, d_rhs);
+}
+
+int asd2_mac_rhs_v_axisym(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_ls, const double* d_u,
+                          const double* d_uhat, const double* d_v, const double* d_H, double* d_rhs) {
+    return asd2_mac_momentum_term_axisym(ctx, fl, ASD2_GRID_V, ASD2_MOM_RHS, d_ls, d_u, d_uhat, d_v, d_H, d_rhs);
 }
 
 }  // extern "C"

@@ -216,6 +216,42 @@ int asd2_mac_kappa_axisym(asd2_ctx* ctx, const double* d_ls, double* d_kappa);
  * c overwritten with the eliminated super-diagonal (as the reference leaves them).  The reference's
  * n = 1 reads d[-1]; here n = 1 is refused. */
 int asd2_tdma(asd2_ctx* ctx, int n, int count, const double* d_a, const double* d_b, double* d_c, double* d_d);
+/* MACSolver2DAxisym::GetRHSU (src/mac2dAxisym.cpp:264-292): u + dt (-convection + gravity +
+ * viscosity) on the interior U faces, 0 on the rest, then the U ghost fill.  The convection is the
+ * planar AddConvectionFU (x = r, y = z), the viscous term AddExternalViscosityFU (:590-753, zero
+ * when re <= 0), the gravity AddGravityFU (:912-929, g_axis 0 = R, 1 = Z).  mu = mu_L + (mu_H -
+ * mu_L) H over the whole padded array.  d_H: the smoothed Heaviside the reference driver passes.
+ * d_rhs must not alias an input. */
+int asd2_mac_rhs_u_axisym(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_ls, const double* d_u,
+                          const double* d_v, const double* d_H, double* d_rhs);
+/* MACSolver2DAxisym::GetRHSV (src/mac2dAxisym.cpp:294-322): v + dt (-convection + gravity +
+ * viscosity) on the interior V faces, then the V ghost fill.  The viscous term
+ * AddExternalViscosityFV (:755-910) reads the new U-face velocity d_uhat (GetUHat's result after the
+ * U ghost fill); the hoop term's radius omits the base radius, as the reference computes it.  The
+ * gravity is AddGravityFU's array as the reference adds it (gravity_fix = 1: AddGravityFV's). */
+int asd2_mac_rhs_v_axisym(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_ls, const double* d_u,
+                          const double* d_uhat, const double* d_v, const double* d_H, double* d_rhs);
+/* one term of the two right-hand sides above (term as for asd2_mac_momentum_term: ASD2_MOM_RHS =
+ * the whole right-hand side with its ghost fill, ASD2_MOM_CONVECTION, ASD2_MOM_VISCOSITY = the
+ * external viscosity term; interior faces, 0 elsewhere); d_uhat is read for V only */
+int asd2_mac_momentum_term_axisym(asd2_ctx* ctx, const asd2_fluid* fl, int grid_kind, int term, const double* d_ls,
+                                  const double* d_u, const double* d_uhat, const double* d_v, const double* d_H,
+                                  double* d_out);
+/* MACSolver2DAxisym::GetUHat (src/mac2dAxisym.cpp:950-1229): the semi-implicit viscous update of
+ * the U faces from GetRHSU's d_rhs: a batched tridiagonal line sweep along r (weights 2 r mu at the
+ * cell radii over the face radius), one along z, then the division by 1 + dt / rho (mu_W + mu_M) /
+ * r^2; rho the arithmetic mean across the interface, radii without the base radius.  W / E:
+ * Neumann, Outlet fold; Dirichlet, Axisym, Inlet, Wall add coupling x constant; S / N: Neumann,
+ * Axisym, Outlet fold, Dirichlet, Inlet, Wall add twice the constant; Periodic takes the far end's
+ * right-hand side.  Interior U faces written, 0 elsewhere (the caller applies the U ghost fill). */
+int asd2_mac_uhat_axisym(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_ls, const double* d_rhs,
+                         const double* d_H, double* d_uhat);
+/* MACSolver2DAxisym::GetVHat (src/mac2dAxisym.cpp:1231-1478): the same for the V faces, weights r mu
+ * at the half radii over the cell radius along r and 2 mu along z; W / E: Neumann, Axisym, Outlet
+ * fold, Dirichlet, Inlet, Wall add twice the constant; S / N: Neumann, Outlet fold, Dirichlet,
+ * Axisym, Inlet, Wall add the constant.  Interior V faces written, 0 elsewhere. */
+int asd2_mac_vhat_axisym(asd2_ctx* ctx, const asd2_fluid* fl, const double* d_ls, const double* d_rhs,
+                         const double* d_H, double* d_vhat);
 /* MACSolver2D::UpdateVel, src/mac2d.cpp:1740-1849 (then U/V BCs) */
 int asd2_mac_update_vel(asd2_ctx* ctx, const asd2_fluid* fl, double* d_u, double* d_v,
                         const double* d_us, const double* d_vs, const double* d_ps,

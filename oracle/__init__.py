@@ -122,6 +122,12 @@ class Oracle:
             "or_divergence_axisym": [G, C.c_double, _P, _P, _P, C.c_int],
             "or_tdma": [_P, _P, _P, _P, C.c_int],
             "or_update_kappa_axisym": [G, C.c_double, _P, _P],
+            "or_viscosity_u_axisym": [G, F, _P, _P, _P, _P],
+            "or_viscosity_v_axisym": [G, F, _P, _P, _P, _P, _P],
+            "or_rhs_u_axisym": [G, F, _P, _P, _P, _P, _P],
+            "or_rhs_v_axisym": [G, F, _P, _P, _P, _P, _P, _P],
+            "or_uhat_axisym": [G, F, _P, _P, _P, _P],
+            "or_vhat_axisym": [G, F, _P, _P, _P, _P],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -285,6 +291,37 @@ class Oracle:
 
     def update_dt_axisym(self, gr, fl, u, v, kappa):
         return self.lib.or_update_dt_axisym(C.byref(gr), C.byref(fl), _ptr(u), _ptr(v), _ptr(kappa))
+
+    def viscosity_u_axisym(self, gr, fl, v, ls, H):
+        o = np.zeros_like(v)
+        self.lib.or_viscosity_u_axisym(C.byref(gr), C.byref(fl), _ptr(v), _ptr(ls), _ptr(H), _ptr(o))
+        return o
+
+    def viscosity_v_axisym(self, gr, fl, u, uhat, ls, H):
+        o = np.zeros_like(u)
+        self.lib.or_viscosity_v_axisym(C.byref(gr), C.byref(fl), _ptr(u), _ptr(uhat), _ptr(ls), _ptr(H), _ptr(o))
+        return o
+
+    def rhs_u_axisym(self, gr, fl, ls, u, v, H):
+        o = np.zeros_like(u)
+        self.lib.or_rhs_u_axisym(C.byref(gr), C.byref(fl), _ptr(ls), _ptr(u), _ptr(v), _ptr(H), _ptr(o))
+        return o
+
+    def rhs_v_axisym(self, gr, fl, ls, u, uhat, v, H):
+        o = np.zeros_like(v)
+        self.lib.or_rhs_v_axisym(C.byref(gr), C.byref(fl), _ptr(ls), _ptr(u), _ptr(uhat), _ptr(v), _ptr(H),
+                                 _ptr(o))
+        return o
+
+    def uhat_axisym(self, gr, fl, ls, rhs, H):
+        o = np.zeros_like(rhs)
+        self.lib.or_uhat_axisym(C.byref(gr), C.byref(fl), _ptr(ls), _ptr(rhs), _ptr(H), _ptr(o))
+        return o
+
+    def vhat_axisym(self, gr, fl, ls, rhs, H):
+        o = np.zeros_like(rhs)
+        self.lib.or_vhat_axisym(C.byref(gr), C.byref(fl), _ptr(ls), _ptr(rhs), _ptr(H), _ptr(o))
+        return o
 
     def tdma(self, a, b, c, d):
         """x of the tridiagonal system (copies in, the reference's recurrence)"""

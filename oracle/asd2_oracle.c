@@ -1676,3 +1676,299 @@ void or_update_kappa_axisym(const or_grid* gr, double base_r, const double* ls, 
     free(dZ);
     free(S);
 }
+
+/* ------------------------------------------------------------------ axisymmetric momentum */
+/* rho_eff of the axisymmetric viscous terms and line sweeps: rho_H thetaH + rho_L (1 - thetaH),
+ * thetaH the heavy share of the segment between the level-set samples a and b (1 or 0 when both
+ * lie in one phase); the branch order of src/mac2dAxisym.cpp:686-748, 1024-1049 */
+static double rho_eff_axi(const or_fluid* fl, double a, double b) {
+    double thH = 0.0;
+    if (a >= 0.0 && b >= 0.0) thH = 1.0;
+    else if (a <= 0.0 && b <= 0.0) thH = 0.0;
+    else if (a >= 0.0 && b < 0.0) thH = fabs(a) / (fabs(a) + fabs(b));
+    else if (a < 0.0 && b >= 0.0) thH = fabs(b) / (fabs(a) + fabs(b));
+    return fl->rho_h * thH + fl->rho_l * (1.0 - thH);
+}
+
+/* mu = mu_L + (mu_H - mu_L) H over the whole padded array (the axisymmetric solver's loops include
+ * the ghosts, src/mac2dAxisym.cpp:607-609) */
+static double* mu_full_axi(const or_grid* gr, const or_fluid* fl, const double* H) {
+    double* mu = zalloc(gr);
+    for (int64_t k = 0; k < or_size(gr); k++) mu[k] = fl->mu_l + (fl->mu_h - fl->mu_l) * H[k];
+    return mu;
+}
+
+/* MACSolver2DAxisym::AddExternalViscosityFU, src/mac2dAxisym.cpp:590-753: the cross term
+ * d/dz (mu dv/dr) only, over the arithmetic-mean density of the S-N segment */
+void or_viscosity_u_axisym(const or_grid* gr, const or_fluid* fl, const double* v, const double* ls,
+                           const double* H, double* dU) {
+    const int nr = gr->nx, nz = gr->ny, g = gr->g;
+    const double dr = gr->dx, dz = gr->dy;
+    for (int64_t k = 0; k < or_size(gr); k++) dU[k] = 0.0;
+    if (fl->re <= 0.0) return;
+    double* mu = mu_full_axi(gr, fl, H);
+#define A_(a, i, j) (a)[IDX(gr, i, j)]
+    for (int i = g + 1; i < nr + g; i++)
+        for (int j = g; j < nz + g; j++) {
+            const double vW = A_(v, i - 1, j), vW_N = A_(v, i - 1, j + 1), vM = A_(v, i, j), vN = A_(v, i, j + 1);
+            const double lsUNHalf = 0.25 * (A_(ls, i, j) + A_(ls, i - 1, j) + A_(ls, i - 1, j + 1) + A_(ls, i, j + 1));
+            const double lsUSHalf = 0.25 * (A_(ls, i, j) + A_(ls, i - 1, j) + A_(ls, i - 1, j - 1) + A_(ls, i, j - 1));
+            const double muUNHalf = 0.25 * (A_(mu, i, j) + A_(mu, i - 1, j) + A_(mu, i - 1, j + 1) + A_(mu, i, j + 1));
+            const double muUSHalf = 0.25 * (A_(mu, i, j) + A_(mu, i - 1, j) + A_(mu, i - 1, j - 1) + A_(mu, i, j - 1));
+            const double rhoSN = rho_eff_axi(fl, lsUSHalf, lsUNHalf);
+            const double visZ = (muUNHalf * (vN - vW_N) / dr - muUSHalf * (vM - vW) / dr) / dz;
+            A_(dU, i, j) = visZ / rhoSN;
+        }
+    free(mu);
+}
+
+/* MACSolver2DAxisym::AddExternalViscosityFV, src/mac2dAxisym.cpp:755-910: d/dr (mu du/dz) over the
+ * W-E segment's density plus mu / r d(u*)/dz from the new U-face velocity (r without the base
+ * radius, as the reference computes it) over the S-N density */
+void or_viscosity_v_axisym(const or_grid* gr, const or_fluid* fl, const double* u, const double* uhat,
+                           const double* ls, const double* H, double* dV) {
+    const int nr = gr->nx, nz = gr->ny, g = gr->g;
+    const double dr = gr->dx, dz = gr->dy;
+    for (int64_t k = 0; k < or_size(gr); k++) dV[k] = 0.0;
+    if (fl->re <= 0.0) return;
+    double* mu = mu_full_axi(gr, fl, H);
+    for (int i = g; i < nr + g; i++)
+        for (int j = g + 1; j < nz + g; j++) {
+            const double rPM = (i + 0.5 - g) * dr;
+            const double uM = A_(u, i, j), uE = A_(u, i + 1, j), uS = A_(u, i, j - 1), uS_E = A_(u, i + 1, j - 1);
+            const double lsVWHalf = 0.25 * (A_(ls, i, j) + A_(ls, i - 1, j) + A_(ls, i - 1, j - 1) + A_(ls, i, j - 1));
+            const double lsVEHalf = 0.25 * (A_(ls, i, j) + A_(ls, i + 1, j) + A_(ls, i + 1, j - 1) + A_(ls, i, j - 1));
+            const double lsM = A_(ls, i, j), lsS = A_(ls, i, j - 1);
+            const double muVWHalf = 0.25 * (A_(mu, i, j) + A_(mu, i - 1, j) + A_(mu, i - 1, j - 1) + A_(mu, i, j - 1));
+            const double muVEHalf = 0.25 * (A_(mu, i, j) + A_(mu, i + 1, j) + A_(mu, i + 1, j - 1) + A_(mu, i, j - 1));
+            const double rhoWE = rho_eff_axi(fl, lsVWHalf, lsVEHalf);
+            const double rhoSN = rho_eff_axi(fl, lsS, lsM);
+            const double visR = (muVEHalf * (uE - uS_E) / dz - muVWHalf * (uM - uS) / dz) / dr;
+            const double visZ = 0.5 * (A_(mu, i, j) + A_(mu, i, j - 1)) / rPM
+                * (0.5 * (A_(uhat, i, j) + A_(uhat, i + 1, j)) - 0.5 * (A_(uhat, i, j - 1) + A_(uhat, i + 1, j - 1))) / dz;
+            A_(dV, i, j) = visR / rhoWE + visZ / rhoSN;
+        }
+    free(mu);
+}
+
+/* MACSolver2DAxisym::GetRHSU (src/mac2dAxisym.cpp:264-292): u + dt (-convection + gravity +
+ * viscosity) on the interior U faces, 0 elsewhere, then the U ghost fill.  The convection is the
+ * planar AddConvectionFU (:324-405 = src/mac2d.cpp:308-387) with x = r, y = z; the gravity is
+ * AddGravityFU (:912-929), -g on the interior U faces when it acts along r. */
+void or_rhs_u_axisym(const or_grid* gr, const or_fluid* fl, const double* ls, const double* u, const double* v,
+                     const double* H, double* rhs) {
+    const int nr = gr->nx, nz = gr->ny, g = gr->g;
+    double* cU = zalloc(gr);
+    double* vU = zalloc(gr);
+    or_convection_u(gr, u, v, cU);
+    or_viscosity_u_axisym(gr, fl, v, ls, H, vU);
+    const double gU = gravity_active(fl, 0) ? -fl->g : 0.0;
+    for (int64_t k = 0; k < or_size(gr); k++) rhs[k] = 0.0;
+    for (int i = g + 1; i < nr + g; i++)
+        for (int j = g; j < nz + g; j++) {
+            const int64_t c = IDX(gr, i, j);
+            rhs[c] = u[c] + fl->dt * (-cU[c] + gU + vU[c]);
+        }
+    or_bc_u(gr, rhs);
+    free(cU);
+    free(vU);
+}
+
+/* MACSolver2DAxisym::GetRHSV (src/mac2dAxisym.cpp:294-322): v + dt (-convection + gravity +
+ * viscosity) on the interior V faces, then the V ghost fill.  The gravity array is AddGravityFU's
+ * (the U grid: -g where it acts along r and i >= g + 1), as the reference calls it; gravity_fix
+ * selects AddGravityFV (-g where it acts along z) instead. */
+void or_rhs_v_axisym(const or_grid* gr, const or_fluid* fl, const double* ls, const double* u, const double* uhat,
+                     const double* v, const double* H, double* rhs) {
+    const int nr = gr->nx, nz = gr->ny, g = gr->g;
+    double* cV = zalloc(gr);
+    double* vV = zalloc(gr);
+    or_convection_v(gr, u, v, cV);
+    or_viscosity_v_axisym(gr, fl, u, uhat, ls, H, vV);
+    for (int64_t k = 0; k < or_size(gr); k++) rhs[k] = 0.0;
+    for (int i = g; i < nr + g; i++)
+        for (int j = g + 1; j < nz + g; j++) {
+            const int64_t c = IDX(gr, i, j);
+            double gv;
+            if (fl->gravity_fix) gv = gravity_active(fl, 1) ? -fl->g : 0.0;
+            else gv = (gravity_active(fl, 0) && i >= g + 1) ? -fl->g : 0.0;
+            rhs[c] = v[c] + fl->dt * (-cV[c] + gv + vV[c]);
+        }
+    or_bc_v(gr, rhs);
+    free(cV);
+    free(vV);
+}
+
+/* the W / E rows of the U line sweep along r: Neumann, Outlet fold the dropped coupling into the
+ * diagonal; Dirichlet, Axisym, Inlet, Wall move coupling x constant to the right-hand side; Periodic
+ * moves coupling x the other end's right-hand side value (src/mac2dAxisym.cpp:1057-1091) */
+static int sweep_fold(int k, int axis_is_neumann) {
+    if (k == OR_BC_NEUMANN || k == OR_BC_OUTLET) return 1;
+    if (k == OR_BC_AXISYM) return axis_is_neumann ? 1 : 2;
+    if (k == OR_BC_DIRICHLET || k == OR_BC_INLET || k == OR_BC_WALL) return 2;
+    if (k == OR_BC_PERIODIC) return 3;
+    return 0;
+}
+
+/* MACSolver2DAxisym::GetUHat, src/mac2dAxisym.cpp:950-1229: the semi-implicit viscous update of
+ * the U faces by a line sweep along r (2 r mu weights over r_{i+1/2} at the face radius), a line
+ * sweep along z, and the division by 1 + dt / rho (mu_W + mu_M) / r^2.  Radii without the base
+ * radius, as the reference computes them.  Interior U faces written, 0 elsewhere. */
+void or_uhat_axisym(const or_grid* gr, const or_fluid* fl, const double* ls, const double* rhs, const double* H,
+                    double* uhat) {
+    const int nr = gr->nx, nz = gr->ny, g = gr->g;
+    const double dr = gr->dx, dz = gr->dy, dt = fl->dt;
+    const int nu = nr - 1;
+    double* mu = mu_full_axi(gr, fl, H);
+    double* us3 = (double*)calloc((size_t)nu * nz, sizeof(double));
+    double* us2 = (double*)calloc((size_t)nu * nz, sizeof(double));
+    double* us1 = (double*)calloc((size_t)nu * nz, sizeof(double));
+    const int n = nu > nz ? nu : nz;
+    double* a = (double*)calloc((size_t)n, sizeof(double));
+    double* b = (double*)calloc((size_t)n, sizeof(double));
+    double* c = (double*)calloc((size_t)n, sizeof(double));
+    double* d = (double*)calloc((size_t)n, sizeof(double));
+    for (int i = g + 1; i < nr + g; i++)
+        for (int j = g; j < nz + g; j++) us3[(j - g) + nz * (i - 1 - g)] = A_(rhs, i, j);
+    const int fW = sweep_fold(gr->bc_u[0], 0), fE = sweep_fold(gr->bc_u[1], 0);
+    const int fS = sweep_fold(gr->bc_u[2], 1), fN = sweep_fold(gr->bc_u[3], 1);
+    for (int j = 0; j < nz; j++) {
+        for (int i = 0; i < nu; i++) d[i] = us3[j + nz * i];
+        for (int i = 0; i < nu; i++) {
+            const double rUM = (i + 1.0) * dr, rPW = (i + 0.5) * dr, rPM = (i + 1.5) * dr;
+            const double lsW = A_(ls, i + g, j + g), lsM = A_(ls, i + g + 1, j + g);
+            const double muW = A_(mu, i + g, j + g), muM = A_(mu, i + g + 1, j + g);
+            const double rho = rho_eff_axi(fl, lsW, lsM);
+            a[i] = -dt / (rho * rUM) * (2.0 * rPW * muW) / (dr * dr);
+            b[i] = 1.0 + dt / (rho * rUM) * 2.0 * (rPW * muW + rPM * muM) / (dr * dr);
+            c[i] = -dt / (rho * rUM) * (2.0 * rPM * muM) / (dr * dr);
+            if (i == 0) {
+                if (fW == 1) { b[i] += a[i]; a[i] = 0.0; }
+                else if (fW == 2) { d[i] -= a[i] * gr->c_u[0]; a[i] = 0.0; }
+                else if (fW == 3) { d[i] -= a[i] * us3[j + nz * (nu - 1)]; a[i] = 0.0; }
+            }
+            if (i == nu - 1) {
+                if (fE == 1) { b[i] += c[i]; c[i] = 0.0; }
+                else if (fE == 2) { d[i] -= c[i] * gr->c_u[1]; c[i] = 0.0; }
+                else if (fE == 3) { d[i] -= c[i] * us3[j + nz * 0]; c[i] = 0.0; }
+            }
+        }
+        or_tdma(a, b, c, d, nu);
+        for (int i = 0; i < nu; i++) us2[j + nz * i] = d[i];
+    }
+    for (int i = 0; i < nu; i++) {
+        for (int j = 0; j < nz; j++) d[j] = us2[j + nz * i];
+        for (int j = 0; j < nz; j++) {
+            const int I = i + g + 1, J = j + g;  /* the U face */
+            const double lsUSHalf = 0.25 * (A_(ls, I, J) + A_(ls, I - 1, J) + A_(ls, I - 1, J - 1) + A_(ls, I, J - 1));
+            const double lsUNHalf = 0.25 * (A_(ls, I, J) + A_(ls, I - 1, J) + A_(ls, I - 1, J + 1) + A_(ls, I, J + 1));
+            const double muUSHalf = 0.25 * (A_(mu, I, J) + A_(mu, I - 1, J) + A_(mu, I - 1, J - 1) + A_(mu, I, J - 1));
+            const double muUNHalf = 0.25 * (A_(mu, I, J) + A_(mu, I - 1, J) + A_(mu, I - 1, J + 1) + A_(mu, I, J + 1));
+            const double rho = rho_eff_axi(fl, lsUSHalf, lsUNHalf);
+            a[j] = -dt / rho * muUSHalf / (dz * dz);
+            b[j] = 1.0 + dt / rho * (muUSHalf + muUNHalf) / (dz * dz);
+            c[j] = -dt / rho * muUNHalf / (dz * dz);
+            if (j == 0) {
+                if (fS == 1) { b[j] += a[j]; a[j] = 0.0; }
+                else if (fS == 2) { d[j] -= a[j] * 2.0 * gr->c_u[2]; a[j] = 0.0; }
+                else if (fS == 3) { d[j] -= a[j] * us2[(nz - 1) + nz * i]; a[j] = 0.0; }
+            }
+            if (j == nz - 1) {
+                if (fN == 1) { b[j] += c[j]; c[j] = 0.0; }
+                else if (fN == 2) { d[j] -= c[j] * 2.0 * gr->c_u[3]; c[j] = 0.0; }
+                else if (fN == 3) { d[j] -= c[j] * us2[0 + nz * i]; c[j] = 0.0; }
+            }
+        }
+        or_tdma(a, b, c, d, nz);
+        for (int j = 0; j < nz; j++) us1[j + nz * i] = d[j];
+    }
+    for (int64_t k = 0; k < or_size(gr); k++) uhat[k] = 0.0;
+    for (int i = g + 1; i < nr + g; i++)
+        for (int j = g; j < nz + g; j++) {
+            const double rUM = (i - g) * dr;
+            const double rho = rho_eff_axi(fl, A_(ls, i - 1, j), A_(ls, i, j));
+            A_(uhat, i, j) = us1[(j - g) + nz * (i - g - 1)]
+                / (1.0 + dt / rho * (A_(mu, i - 1, j) + A_(mu, i, j)) / (rUM * rUM));
+        }
+    free(mu); free(us3); free(us2); free(us1); free(a); free(b); free(c); free(d);
+}
+
+/* MACSolver2DAxisym::GetVHat, src/mac2dAxisym.cpp:1231-1478: the semi-implicit viscous update of
+ * the V faces by a line sweep along r (r mu weights at the half radii over the cell radius) and a
+ * line sweep along z (2 mu weights).  Here Axisym folds like Neumann on W / E and acts like
+ * Dirichlet on S / N; Dirichlet-family sides add twice the constant along r, once along z.
+ * Interior V faces written, 0 elsewhere. */
+void or_vhat_axisym(const or_grid* gr, const or_fluid* fl, const double* ls, const double* rhs, const double* H,
+                    double* vhat) {
+    const int nr = gr->nx, nz = gr->ny, g = gr->g;
+    const double dr = gr->dx, dz = gr->dy, dt = fl->dt;
+    const int nv = nz - 1;
+    double* mu = mu_full_axi(gr, fl, H);
+    double* vs2 = (double*)calloc((size_t)nr * nv, sizeof(double));
+    double* vs1 = (double*)calloc((size_t)nr * nv, sizeof(double));
+    double* vs0 = (double*)calloc((size_t)nr * nv, sizeof(double));
+    const int n = nr > nz ? nr : nz;
+    double* a = (double*)calloc((size_t)n, sizeof(double));
+    double* b = (double*)calloc((size_t)n, sizeof(double));
+    double* c = (double*)calloc((size_t)n, sizeof(double));
+    double* d = (double*)calloc((size_t)n, sizeof(double));
+    for (int i = g; i < nr + g; i++)
+        for (int j = g + 1; j < nz + g; j++) vs2[(j - g - 1) + nv * (i - g)] = A_(rhs, i, j);
+    const int fW = sweep_fold(gr->bc_v[0], 1), fE = sweep_fold(gr->bc_v[1], 1);
+    const int fS = sweep_fold(gr->bc_v[2], 0), fN = sweep_fold(gr->bc_v[3], 0);
+    for (int j = 0; j < nv; j++) {
+        for (int i = 0; i < nr; i++) d[i] = vs2[j + nv * i];
+        for (int i = 0; i < nr; i++) {
+            const double rVM = (i + 0.5) * dr, rVWHalf = i * dr, rVEHalf = (i + 1.0) * dr;
+            const int I = i + g, J = j + g + 1;  /* the V face */
+            const double lsVWHalf = 0.25 * (A_(ls, I, J) + A_(ls, I - 1, J) + A_(ls, I, J - 1) + A_(ls, I - 1, J - 1));
+            const double lsVEHalf = 0.25 * (A_(ls, I, J) + A_(ls, I + 1, J) + A_(ls, I, J - 1) + A_(ls, I + 1, J - 1));
+            const double muVWHalf = 0.25 * (A_(mu, I, J) + A_(mu, I - 1, J) + A_(mu, I, J - 1) + A_(mu, I - 1, J - 1));
+            const double muVEHalf = 0.25 * (A_(mu, I, J) + A_(mu, I + 1, J) + A_(mu, I, J - 1) + A_(mu, I + 1, J - 1));
+            const double rho = rho_eff_axi(fl, lsVWHalf, lsVEHalf);
+            a[i] = -dt / (rho * rVM) * rVWHalf * muVWHalf / (dr * dr);
+            b[i] = 1.0 + dt / (rho * rVM) * (rVWHalf * muVWHalf + rVEHalf * muVEHalf) / (dr * dr);
+            c[i] = -dt / (rho * rVM) * rVEHalf * muVEHalf / (dr * dr);
+            if (i == 0) {
+                if (fW == 1) { b[i] += a[i]; a[i] = 0.0; }
+                else if (fW == 2) { d[i] -= a[i] * 2.0 * gr->c_v[0]; a[i] = 0.0; }
+                else if (fW == 3) { d[i] -= a[i] * vs2[j + nv * (nr - 1)]; a[i] = 0.0; }
+            }
+            if (i == nr - 1) {
+                if (fE == 1) { b[i] += c[i]; c[i] = 0.0; }
+                else if (fE == 2) { d[i] -= c[i] * 2.0 * gr->c_v[1]; c[i] = 0.0; }
+                else if (fE == 3) { d[i] -= c[i] * vs2[j + nv * 0]; c[i] = 0.0; }
+            }
+        }
+        or_tdma(a, b, c, d, nr);
+        for (int i = 0; i < nr; i++) vs1[j + nv * i] = d[i];
+    }
+    for (int i = 0; i < nr; i++) {
+        for (int j = 0; j < nv; j++) d[j] = vs1[j + nv * i];
+        for (int j = 0; j < nv; j++) {
+            const double lsS = A_(ls, i + g, j + g), lsM = A_(ls, i + g, j + g + 1);
+            const double muS = A_(mu, i + g, j + g), muM = A_(mu, i + g, j + g + 1);
+            const double rho = rho_eff_axi(fl, lsS, lsM);
+            a[j] = -dt / rho * (2.0 * muS) / (dz * dz);
+            b[j] = 1.0 + dt / rho * 2.0 * (muS + muM) / (dz * dz);
+            c[j] = -dt / rho * (2.0 * muM) / (dz * dz);
+            if (j == 0) {
+                if (fS == 1) { b[j] += a[j]; a[j] = 0.0; }
+                else if (fS == 2) { d[j] -= a[j] * gr->c_v[2]; a[j] = 0.0; }
+                else if (fS == 3) { d[j] -= a[j] * vs1[(nv - 1) + nv * i]; a[j] = 0.0; }
+            }
+            if (j == nv - 1) {
+                if (fN == 1) { b[j] += c[j]; c[j] = 0.0; }
+                else if (fN == 2) { d[j] -= c[j] * gr->c_v[3]; c[j] = 0.0; }
+                else if (fN == 3) { d[j] -= c[j] * vs1[0 + nv * i]; c[j] = 0.0; }
+            }
+        }
+        or_tdma(a, b, c, d, nv);
+        for (int j = 0; j < nv; j++) vs0[j + nv * i] = d[j];
+    }
+    for (int64_t k = 0; k < or_size(gr); k++) vhat[k] = 0.0;
+    for (int i = g; i < nr + g; i++)
+        for (int j = g + 1; j < nz + g; j++) A_(vhat, i, j) = vs0[(j - g - 1) + nv * (i - g)];
+#undef A_
+    free(mu); free(vs2); free(vs1); free(vs0); free(a); free(b); free(c); free(d);
+}

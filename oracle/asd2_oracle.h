@@ -126,6 +126,19 @@ double or_update_dt_axisym(const or_grid* gr, const or_fluid* fl, const double* 
                            const double* kappa);
 void or_tdma(double* a, double* b, double* c, double* d, int n);
 void or_update_kappa_axisym(const or_grid* gr, double base_r, const double* ls, double* kappa);
+/* axisymmetric momentum (src/mac2dAxisym.cpp:264-322, 590-910, 950-1478); x = r, y = z */
+void or_viscosity_u_axisym(const or_grid* gr, const or_fluid* fl, const double* v, const double* ls,
+                           const double* H, double* dU);
+void or_viscosity_v_axisym(const or_grid* gr, const or_fluid* fl, const double* u, const double* uhat,
+                           const double* ls, const double* H, double* dV);
+void or_rhs_u_axisym(const or_grid* gr, const or_fluid* fl, const double* ls, const double* u, const double* v,
+                     const double* H, double* rhs);
+void or_rhs_v_axisym(const or_grid* gr, const or_fluid* fl, const double* ls, const double* u, const double* uhat,
+                     const double* v, const double* H, double* rhs);
+void or_uhat_axisym(const or_grid* gr, const or_fluid* fl, const double* ls, const double* rhs, const double* H,
+                    double* uhat);
+void or_vhat_axisym(const or_grid* gr, const or_fluid* fl, const double* ls, const double* rhs, const double* H,
+                    double* vhat);
 
 /* Krylov solvers.  mode 0 = reference CG iteration (poisson2d.cpp:345-396: iteration-0
  * p = rho*r + z, iteration-1 p = z, alpha denominator + 1e-12), mode 1 = textbook PCG. */

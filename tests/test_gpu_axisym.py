@@ -1,8 +1,9 @@
 """The axisymmetric (r, z) path (MACSolver2DAxisym, reference src/mac2dAxisym.cpp) against the
-oracle's restatement of it (oracle/asd2_oracle.c, pinned by tests/test_oracle_axisym.py): the
-curvature with its hoop term, the two divergences, the r-weighted ghost-fluid assembly, the
-pressure solve on that system with the library's solvers, the time step and the batched
-tridiagonal solve.  A drop sits on the axis (a sphere in 3D), heavy inside; the W side is the
+oracle's restatement of it (oracle/asd2_oracle.c, pinned by tests/test_oracle_axisym.py and
+tests/test_oracle_axisym_momentum.py): the curvature with its hoop term, the two divergences, the
+r-weighted ghost-fluid assembly, the pressure solve on that system with the library's solvers,
+the time step, the batched tridiagonal solve, and the momentum step (right-hand sides, viscous
+terms, semi-implicit line sweeps).  A drop sits on the axis (a sphere in 3D), heavy inside; the W side is the
 axis.  Bars: curvature, divergences, assembly, time step 1e-13 relative (one pow may round
 differently on the device); pressure rel L2 1e-8 at a true relative residual of 1e-10."""
 import numpy as np
@@ -40,6 +41,13 @@ BCS = [(("axisym", "neumann", "neumann", "neumann"), (0.0,) * 4),
 
 def _rel_max(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def _close(out, ref, rel, what):
+    scale = max(np.abs(ref).max(), 1e-300)
+    err = float(np.abs(out - ref).max() / scale)
+    print(f"{what}: max rel err {err:.3e} (scale {scale:.3e})")
+    assert err <= rel, (what, err)
 
 
 @pytest.mark.parametrize("bc", range(len(BCS)))
@@ -149,3 +157,92 @@ def test_tdma_rejects_single_unknown():
     x = A.DeviceArray.from_host(np.ones((1, 4)))
     with pytest.raises(A.Asd2Error):
         ctx.tdma(x, x, x, x)
+
+
+# axisymmetric momentum: U / V ghost rules (kinds, constants) of each case
+MOM_BCS = [
+    (("axisym", "wall", "neumann", "outlet"), (0.0, 0.0, 0.0, 0.0),
+     ("axisym", "wall", "wall", "outlet"), (0.0, 0.0, 0.0, 0.0)),
+    (("dirichlet", "neumann", "dirichlet", "inlet"), (0.0, 0.0, 0.1, -0.2),
+     ("neumann", "dirichlet", "axisym", "dirichlet"), (0.0, 0.05, 0.0, 0.3)),
+    (("periodic",) * 4, (0.0,) * 4, ("periodic",) * 4, (0.0,) * 4),
+]
+
+
+@pytest.mark.parametrize("bc", range(len(MOM_BCS)))
+@pytest.mark.parametrize("nr,nz,gaxis,fix", [(64, 64, 1, 1), (37, 52, 0, 0), (48, 29, 0, 1)])
+def test_axisym_momentum(oracle, nr, nz, gaxis, fix, bc):
+    """GetRHSU -> GetUHat -> U ghosts -> GetRHSV -> GetVHat -> V ghosts (the reference driver's
+    order, tests/LargeTests/test_mac2d_large.cpp:1282-1290) against the oracle, each piece on the
+    oracle's own inputs and then the chain on the device.  Bars: viscous terms and both line sweeps
+    bit for bit (the same operations in the same order; dr = 1/64 takes the exact-product path);
+    the right-hand sides, whose convection is the momentum kernels' WENO5, relative 1e-11 of their
+    scale as for the planar momentum terms; the chained velocities relative 1e-10."""
+    import tests.scenarios as S
+    bu, cu, bv, cv = MOM_BCS[bc]
+    dr, dz = 1.0 / nr, 1.0 / nr
+    gr = make_grid(nr, nz, dr, dz, bc_p=("axisym", "neumann", "neumann", "neumann"),
+                   bc_u=bu, c_u=cu, bc_v=bv, c_v=cv)
+    rng = np.random.default_rng(nr + nz + bc)
+    rc = (np.arange(nr + 2 * G) + 0.5 - G) * dr
+    zc = (np.arange(nz + 2 * G) + 0.5 - G) * dz
+    Rr, Zz = np.meshgrid(rc, zc, indexing="ij")
+    R0, z0 = rng.uniform(0.2, 0.3), rng.uniform(0.4, 0.6) * nz * dz
+    ls = np.ascontiguousarray(R0 - np.sqrt(Rr ** 2 + (Zz - z0) ** 2))
+    H = oracle.smooth_heaviside(gr, ls)
+    u, v = S.smooth_velocity(nr + nz, nr, nz, dr, dz)
+    u, v = u * 0.5, v * 0.5
+    oracle.bc_u(gr, u)
+    oracle.bc_v(gr, v)
+    fl = make_fluid(1000.0, 1.2, 1e-2, 1.8e-3, 0.0, 2e-3, g=9.8, g_axis=gaxis, gravity_fix=fix)
+    flg = A.make_fluid(1000.0, 1.2, 1e-2, 1.8e-3, 0.0, 2e-3, g=9.8, g_axis=gaxis, gravity_fix=fix)
+    ctx = A.Context(nr, nz, dr, dz)
+    ctx.set_bc(A.GRID_U, bu, cu)
+    ctx.set_bc(A.GRID_V, bv, cv)
+    d_ls, d_H, d_u, d_v = ctx.array(ls), ctx.array(H), ctx.array(u), ctx.array(v)
+    tag = f"{nr}x{nz} bc {bc} g_axis {gaxis} fix {fix}"
+    print(tag, flush=True)
+    # U: viscous term, right-hand side, line sweeps
+    out = ctx.array()
+    ctx.momentum_term_axisym(flg, A.GRID_U, A.MOM_VISCOSITY, d_ls, d_u, None, d_v, d_H, out)
+    np.testing.assert_array_equal(out.download(), oracle.viscosity_u_axisym(gr, fl, v, ls, H))
+    rhs_u = oracle.rhs_u_axisym(gr, fl, ls, u, v, H)
+    ctx.rhs_u_axisym(flg, d_ls, d_u, d_v, d_H, out)
+    _close(out.download(), rhs_u, 1e-11, f"{tag}: U right-hand side")
+    uh_ref = oracle.uhat_axisym(gr, fl, ls, rhs_u, H)
+    d_uh = ctx.array()
+    ctx.uhat_axisym(flg, d_ls, ctx.array(rhs_u), d_H, d_uh)
+    np.testing.assert_array_equal(d_uh.download(), uh_ref)
+    oracle.bc_u(gr, uh_ref)
+    # V: viscous term with u*, right-hand side, line sweeps
+    d_uhr = ctx.array(uh_ref)
+    ctx.momentum_term_axisym(flg, A.GRID_V, A.MOM_VISCOSITY, d_ls, d_u, d_uhr, d_v, d_H, out)
+    np.testing.assert_array_equal(out.download(), oracle.viscosity_v_axisym(gr, fl, u, uh_ref, ls, H))
+    rhs_v = oracle.rhs_v_axisym(gr, fl, ls, u, uh_ref, v, H)
+    ctx.rhs_v_axisym(flg, d_ls, d_u, d_uhr, d_v, d_H, out)
+    _close(out.download(), rhs_v, 1e-11, f"{tag}: V right-hand side")
+    vh_ref = oracle.vhat_axisym(gr, fl, ls, rhs_v, H)
+    d_vh = ctx.array()
+    ctx.vhat_axisym(flg, d_ls, ctx.array(rhs_v), d_H, d_vh)
+    np.testing.assert_array_equal(d_vh.download(), vh_ref)
+    oracle.bc_v(gr, vh_ref)
+    # the chain on the device
+    d_ru, d_rv, d_us, d_vs = ctx.array(), ctx.array(), ctx.array(), ctx.array()
+    ctx.rhs_u_axisym(flg, d_ls, d_u, d_v, d_H, d_ru)
+    ctx.uhat_axisym(flg, d_ls, d_ru, d_H, d_us)
+    ctx.bc_apply(A.GRID_U, d_us)
+    ctx.rhs_v_axisym(flg, d_ls, d_u, d_us, d_v, d_H, d_rv)
+    ctx.vhat_axisym(flg, d_ls, d_rv, d_H, d_vs)
+    ctx.bc_apply(A.GRID_V, d_vs)
+    _close(d_us.download(), uh_ref, 1e-10, f"{tag}: chained u*")
+    _close(d_vs.download(), vh_ref, 1e-10, f"{tag}: chained v*")
+
+
+def test_axisym_momentum_refusals():
+    fl = A.make_fluid(1.0, 1.0, 1.0, 1.0, 0.0, 1e-3)
+    ctx = A.Context(8, 8, 0.125, 0.125)
+    x, y = ctx.array(), ctx.array()
+    with pytest.raises(A.Asd2Error):
+        ctx.vhat_axisym(fl, x, y, x, y)  # output aliases the right-hand side
+    with pytest.raises(A.Asd2Error):
+        ctx.momentum_term_axisym(fl, A.GRID_V, A.MOM_VISCOSITY, x, x, None, x, x, y)  # V needs u*
