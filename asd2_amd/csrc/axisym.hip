@@ -103,7 +103,7 @@ struct SweepArgs {
 
 __device__ __forceinline__ double mu_of(const SweepArgs& S, double h) {
 #pragma clang fp contract(off)
-    return S.mu_l + (S.mu_h - S.mu_l) * h;  // over the whole padded array (src/mac2dAxisym.cpp:1014-1019)
+    return S.mu_l + (S.mu_h - S.mu_l) * h;  // over the whole padded array (src/mac2dAxisym.cpp:996-999)
 }
 
 // the first / last row of a system: fold, constant or periodic value
@@ -261,7 +261,7 @@ namespace {
 // sweep end rules of the U (grid 1) or V (grid 2) faces.  U: W / E take Axisym as a wall (u = 0 on
 // the axis) and add the constant once, S / N fold Axisym and add twice the constant; V: W / E fold
 // Axisym and add twice the constant, S / N take Axisym as a wall and add the constant once
-// (src/mac2dAxisym.cpp:1057-1091, 1153-1187, 1305-1339, 1401-1435)
+// (src/mac2dAxisym.cpp:1052-1078, 1141-1169, 1336-1364, 1423-1449)
 SweepArgs sweep_args(const asd2_ctx* ctx, const asd2_fluid* fl, int grid) {
     SweepArgs S{};
     S.rho_h = fl->rho_h; S.rho_l = fl->rho_l; S.mu_h = fl->mu_h; S.mu_l = fl->mu_l; S.dt = fl->dt;

@@ -149,7 +149,7 @@ __device__ __forceinline__ double minmod(double v1, double v2) {
 
 // density of the axisymmetric solver's viscous terms and line sweeps at a segment between the
 // level-set samples a and b: rho_H thetaH + rho_L (1 - thetaH), thetaH the heavy share of the
-// segment (1 or 0 when both samples lie in one phase), src/mac2dAxisym.cpp:686-748, 1024-1049
+// segment (1 or 0 when both samples lie in one phase), src/mac2dAxisym.cpp:687-737, 1024-1045
 __device__ __forceinline__ double rho_eff_axi(double rho_h, double rho_l, double a, double b) {
 #pragma clang fp contract(off)
     double thH = 0.0;

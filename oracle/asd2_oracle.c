@@ -22,10 +22,14 @@
  *   - Gauss-Seidel (GS_2FUniform_2D): one sweep of a 2 x 2 system against hand
  *     arithmetic; fixed point = the direct solve.
  *   - UpdateVel: constant-curvature droplet leaves u* unchanged (balanced jump).
- *   - Momentum predictor (convection, viscosity, gravity, RK): parity unpinned
- *     beyond closed forms (uniform flow has zero convection; linear shear has zero
- *     viscous force in one phase); src/mac2d.cpp needs TECIO headers and cannot be
- *     compiled here.
+ *   - Momentum predictor (convection, viscosity, gravity, RK): closed forms
+ *     (WENO5 exact on cubic fluxes, quadratic velocities with constant mu, planar
+ *     interface face densities, RK stage weights on a linear operator) with planted
+ *     mistakes (tests/test_oracle_pins.py); src/mac2d.cpp needs TECIO headers and
+ *     cannot be compiled here.
+ *   - Axisymmetric pieces (src/mac2dAxisym.cpp, not compilable here either): closed
+ *     forms with planted mistakes (tests/test_oracle_axisym.py,
+ *     tests/test_oracle_axisym_momentum.py).
  */
 #include "asd2_oracle.h"
 
@@ -1680,7 +1684,7 @@ void or_update_kappa_axisym(const or_grid* gr, double base_r, const double* ls, 
 /* ------------------------------------------------------------------ axisymmetric momentum */
 /* rho_eff of the axisymmetric viscous terms and line sweeps: rho_H thetaH + rho_L (1 - thetaH),
  * thetaH the heavy share of the segment between the level-set samples a and b (1 or 0 when both
- * lie in one phase); the branch order of src/mac2dAxisym.cpp:686-748, 1024-1049 */
+ * lie in one phase); the branch order of src/mac2dAxisym.cpp:687-737, 1024-1045 */
 static double rho_eff_axi(const or_fluid* fl, double a, double b) {
     double thH = 0.0;
     if (a >= 0.0 && b >= 0.0) thH = 1.0;
@@ -1801,7 +1805,8 @@ void or_rhs_v_axisym(const or_grid* gr, const or_fluid* fl, const double* ls, co
 
 /* the W / E rows of the U line sweep along r: Neumann, Outlet fold the dropped coupling into the
  * diagonal; Dirichlet, Axisym, Inlet, Wall move coupling x constant to the right-hand side; Periodic
- * moves coupling x the other end's right-hand side value (src/mac2dAxisym.cpp:1057-1091) */
+ * moves coupling x the other end's right-hand side value (src/mac2dAxisym.cpp:1052-1078, 1141-1169, 1336-1364,
+ * 1423-1449) */
 static int sweep_fold(int k, int axis_is_neumann) {
     if (k == OR_BC_NEUMANN || k == OR_BC_OUTLET) return 1;
     if (k == OR_BC_AXISYM) return axis_is_neumann ? 1 : 2;
