@@ -1669,58 +1669,77 @@ __device__ __forceinline__ void ml_steps_s(const Level& L, const LevelTile<T + 2
 // tile's operator and g are staged in LDS in one batch of loads issued with the read of the solver
 // state, so a launch waits for memory about once.
 template <int T>
-__global__ __launch_bounds__(T * T) void k_ml_down(Level L, Level next, const KrylovState* st, Cheb ch) {
+struct MlTileLds {
+    static constexpr int HW = T + 2 * ML_SW;
+    double y[2][HW][HW];
+    double d[HW][HW];
+    double r[T][T];
+    LevelTile<HW> o;
+};
+
+// one tile of k_ml_down: false when the solve is not running (checked after the tile's loads are
+// issued, so their latency overlaps the read of the state)
+template <int T>
+__device__ __forceinline__ bool ml_down_tile(const Level& L, const Level& next, const Cheb& ch, int done, int t,
+                                             MlTileLds<T>& sm) {
     constexpr int HW = T + 2 * ML_SW, NT = T * T;
-    __shared__ double sY[2][HW][HW];
-    __shared__ double sD[HW][HW];
-    __shared__ double sR[T][T];
-    __shared__ LevelTile<HW> sO;
-    const int done = st->done;
-    const int tiles_j = ceil_div(L.W, T), ntiles = ceil_div(L.H, T) * tiles_j;
+    const int tiles_j = ceil_div(L.W, T);
     const int c = L.c, nb = T / c;
     const int a = threadIdx.x / T, b = threadIdx.x - a * T;
     constexpr int F = (ML_SW - 1) & 1;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int ti = t / tiles_j, tj = t - ti * tiles_j;
-        const int i0 = ti * T, j0 = tj * T;
-        LevelTileRegs<HW, NT> tr;
-        tr.fetch(L, i0, j0);
-        if (done != DONE_RUNNING) return;
-        tr.store(sO);
-        __syncthreads();
-        for (int e = threadIdx.x; e < HW * HW; e += NT) {  // step 0 from y = 0: d = y = c0 g / dc
-            const int aa = e / HW, bb = e - aa * HW;
-            const int i = i0 - ML_SW + aa, j = j0 - ML_SW + bb;
-            const double y0 =
-                (i >= 0 && i < L.H && j >= 0 && j < L.W) ? ch.c0 * (sO.g[aa][bb] / sO.dc[aa][bb]) : 0.0;
-            sY[0][aa][bb] = y0;
-            sD[aa][bb] = y0;
-        }
-        __syncthreads();
-        ml_steps_s<T>(L, sO, ch, sY, sD, 1, ML_SW, 0, i0, j0);
-        const int i = i0 + a, j = j0 + b;
-        double rv = 0.0;
-        if (i < L.H && j < L.W) {
-            const int A = a + ML_SW, B = b + ML_SW;
-            const double (*y)[HW] = sY[F];
-            const double ay = ml_apply_s<HW>(sO, y, A, B);
-            L.y1[static_cast<int64_t>(i) * L.W + j] = y[A][B];
-            rv = sO.g[A][B] - ay;
-        }
-        sR[a][b] = rv;
-        __syncthreads();
-        if (static_cast<int>(threadIdx.x) < nb * nb) {
-            const int bi = threadIdx.x / nb, bj = threadIdx.x - bi * nb;
-            const int I = ti * nb + bi, J = tj * nb + bj;
-            if (I < next.H && J < next.W) {
-                double sum = 0.0;
-                for (int u = 0; u < c; u++)
-                    for (int v = 0; v < c; v++) sum += sR[bi * c + u][bj * c + v];
-                next.g[static_cast<int64_t>(I) * next.W + J] = sum;
-            }
-        }
-        __syncthreads();
+    const int ti = t / tiles_j, tj = t - ti * tiles_j;
+    const int i0 = ti * T, j0 = tj * T;
+    LevelTileRegs<HW, NT> tr;
+    tr.fetch(L, i0, j0);
+    if (done != DONE_RUNNING) return false;
+    tr.store(sm.o);
+    __syncthreads();
+    for (int e = threadIdx.x; e < HW * HW; e += NT) {  // step 0 from y = 0: d = y = c0 g / dc
+        const int aa = e / HW, bb = e - aa * HW;
+        const int i = i0 - ML_SW + aa, j = j0 - ML_SW + bb;
+        const double y0 = (i >= 0 && i < L.H && j >= 0 && j < L.W) ? ch.c0 * (sm.o.g[aa][bb] / sm.o.dc[aa][bb]) : 0.0;
+        sm.y[0][aa][bb] = y0;
+        sm.d[aa][bb] = y0;
     }
+    __syncthreads();
+    ml_steps_s<T>(L, sm.o, ch, sm.y, sm.d, 1, ML_SW, 0, i0, j0);
+    const int i = i0 + a, j = j0 + b;
+    double rv = 0.0;
+    if (i < L.H && j < L.W) {
+        const int A = a + ML_SW, B = b + ML_SW;
+        const double (*y)[HW] = sm.y[F];
+        const double ay = ml_apply_s<HW>(sm.o, y, A, B);
+        L.y1[static_cast<int64_t>(i) * L.W + j] = y[A][B];
+        rv = sm.o.g[A][B] - ay;
+    }
+    sm.r[a][b] = rv;
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < nb * nb) {
+        const int bi = threadIdx.x / nb, bj = threadIdx.x - bi * nb;
+        const int I = ti * nb + bi, J = tj * nb + bj;
+        if (I < next.H && J < next.W) {
+            double sum = 0.0;
+            for (int u = 0; u < c; u++)
+                for (int v = 0; v < c; v++) sum += sm.r[bi * c + u][bj * c + v];
+            next.g[static_cast<int64_t>(I) * next.W + J] = sum;
+        }
+    }
+    __syncthreads();
+    return true;
+}
+
+// pre-smoothing and restriction on a level run by the whole chip: y1 = the ML_SW Chebyshev
+// steps from 0, next.g = block sums of g - A y1.  A block of T x T threads owns a T x T tile
+// (whole blocks of the next level: c divides T) and smooths it with a halo of ML_SW cells.  The
+// tile's operator and g are staged in LDS in one batch of loads issued with the read of the solver
+// state, so a launch waits for memory about once.
+template <int T>
+__global__ __launch_bounds__(T * T) void k_ml_down(Level L, Level next, const KrylovState* st, Cheb ch) {
+    __shared__ MlTileLds<T> sm;
+    const int done = st->done;
+    const int ntiles = ceil_div(L.H, T) * ceil_div(L.W, T);
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x)
+        if (!ml_down_tile<T>(L, next, ch, done, t, sm)) return;
 }
 
 // y = A^-1 g on the coarsest level with 4 threads per row (FP64), in a block of 1024
@@ -1818,63 +1837,76 @@ __global__ __launch_bounds__(1024) void k_ml_lower(Hierarchy Hh, int first, cons
 // iteration's scalars go to pc (the step along P y is fixed at 1, so y.g and y.A_c y are not
 // needed).  The tile's operator, y1, the coarse values under it and (level 0) the coarse dots'
 // operands are loaded in one batch with the read of the solver state.
+// one tile of k_ml_up (yout, rc, qc: level 0 only; a_r, a_q accumulate the partials' operands)
+template <int T>
+__device__ __forceinline__ bool ml_up_tile(const Level& L, const Level& next, float* __restrict__ yout,
+                                           const double* rc, const double* qc, const Cheb& ch, int done, int t,
+                                           MlTileLds<T>& sm, double& a_r, double& a_q) {
+    constexpr int HW = T + 2 * ML_SW, NT = T * T;
+    const int tiles_j = ceil_div(L.W, T);
+    const int a = threadIdx.x / T, b = threadIdx.x - a * T;
+    constexpr int F = (ML_SW - 1) & 1;
+    const int ti = t / tiles_j, tj = t - ti * tiles_j;
+    const int i0 = ti * T, j0 = tj * T;
+    LevelTileRegs<HW, NT> tr;
+    tr.fetch(L, i0, j0);
+    TileRegs<HW, HW, NT> ty1, tyn;
+    lvl_fetch(L.y1, L.W, L.H, L.W, ty1, i0 - ML_SW, j0 - ML_SW);
+#pragma unroll
+    for (int k = 0; k < TileRegs<HW, HW, NT>::NL; k++) {  // P y_next at the region's cells
+        const int e = static_cast<int>(threadIdx.x) + k * NT;
+        const int aa = e / HW, bb = e - aa * HW;
+        const int i = i0 - ML_SW + aa, j = j0 - ML_SW + bb;
+        const bool in = e < HW * HW && i >= 0 && i < L.H && j >= 0 && j < L.W;
+        const double x = next.y[in ? static_cast<int64_t>(i / L.c) * next.W + j / L.c : 0];
+        tyn.v[k] = in ? x : 0.0;
+    }
+    const int i = i0 + a, j = j0 + b;
+    const bool mine = i < L.H && j < L.W;
+    const int64_t e0 = mine ? static_cast<int64_t>(i) * L.W + j : 0;
+    const double rcv = yout ? rc[e0] : 0.0, qcv = yout ? qc[e0] : 0.0;
+    if (done != DONE_RUNNING) return false;
+    tr.store(sm.o);
+#pragma unroll
+    for (int k = 0; k < TileRegs<HW, HW, NT>::NL; k++) {  // y2 = y1 + P y_next (0 outside the level)
+        const int e = static_cast<int>(threadIdx.x) + k * NT;
+        if ((HW * HW) % NT == 0 || e < HW * HW) {
+            const int aa = e / HW, bb = e - aa * HW;
+            sm.y[0][aa][bb] = ty1.v[k] + tyn.v[k];
+        }
+    }
+    __syncthreads();
+    ml_steps_s<T>(L, sm.o, ch, sm.y, sm.d, 0, ML_SW - 1, 0, i0, j0);
+    if (mine) {
+        const double y = ml_step_s<HW>(sm.o, ch, ML_SW - 1, sm.y[F], sm.d[a + ML_SW][b + ML_SW], a + ML_SW, b + ML_SW);
+        if (yout) {
+            const float yf = static_cast<float>(y);
+            yout[e0] = yf;
+            a_r += static_cast<double>(yf) * rcv;
+            a_q += static_cast<double>(yf) * qcv;
+        } else {
+            L.y[e0] = y;
+        }
+    }
+    __syncthreads();
+    return true;
+}
+
+// prolongation and post-smoothing on a level run by the whole chip: y = the ML_SW Chebyshev
+// steps from y2 = y1 + P y_next (on T x T tiles with the sweeps' halo), into L.y, or into yout
+// (FP32) on level 0, where the block partials (0, 0, y.P^T r, y.P^T q_1) of the fine
+// iteration's scalars go to pc (the step along P y is fixed at 1, so y.g and y.A_c y are not
+// needed).  The tile's operator, y1, the coarse values under it and (level 0) the coarse dots'
+// operands are loaded in one batch with the read of the solver state.
 template <int T>
 __global__ __launch_bounds__(T * T) void k_ml_up(Level L, Level next, float* __restrict__ yout, const double* rc,
                                                  const double* qc, const KrylovState* st, double* pc, Cheb ch) {
-    constexpr int HW = T + 2 * ML_SW, NT = T * T;
-    __shared__ double sY[2][HW][HW];
-    __shared__ double sD[HW][HW];
-    __shared__ LevelTile<HW> sO;
+    __shared__ MlTileLds<T> sm;
     const int done = st->done;
-    const int tiles_j = ceil_div(L.W, T), ntiles = ceil_div(L.H, T) * tiles_j;
-    const int a = threadIdx.x / T, b = threadIdx.x - a * T;
-    constexpr int F = (ML_SW - 1) & 1;
+    const int ntiles = ceil_div(L.H, T) * ceil_div(L.W, T);
     double a_r = 0.0, a_q = 0.0;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int ti = t / tiles_j, tj = t - ti * tiles_j;
-        const int i0 = ti * T, j0 = tj * T;
-        LevelTileRegs<HW, NT> tr;
-        tr.fetch(L, i0, j0);
-        TileRegs<HW, HW, NT> ty1, tyn;
-        lvl_fetch(L.y1, L.W, L.H, L.W, ty1, i0 - ML_SW, j0 - ML_SW);
-#pragma unroll
-        for (int k = 0; k < TileRegs<HW, HW, NT>::NL; k++) {  // P y_next at the region's cells
-            const int e = static_cast<int>(threadIdx.x) + k * NT;
-            const int aa = e / HW, bb = e - aa * HW;
-            const int i = i0 - ML_SW + aa, j = j0 - ML_SW + bb;
-            const bool in = e < HW * HW && i >= 0 && i < L.H && j >= 0 && j < L.W;
-            const double x = next.y[in ? static_cast<int64_t>(i / L.c) * next.W + j / L.c : 0];
-            tyn.v[k] = in ? x : 0.0;
-        }
-        const int i = i0 + a, j = j0 + b;
-        const bool mine = i < L.H && j < L.W;
-        const int64_t e0 = mine ? static_cast<int64_t>(i) * L.W + j : 0;
-        const double rcv = yout ? rc[e0] : 0.0, qcv = yout ? qc[e0] : 0.0;
-        if (done != DONE_RUNNING) return;
-        tr.store(sO);
-#pragma unroll
-        for (int k = 0; k < TileRegs<HW, HW, NT>::NL; k++) {  // y2 = y1 + P y_next (0 outside the level)
-            const int e = static_cast<int>(threadIdx.x) + k * NT;
-            if ((HW * HW) % NT == 0 || e < HW * HW) {
-                const int aa = e / HW, bb = e - aa * HW;
-                sY[0][aa][bb] = ty1.v[k] + tyn.v[k];
-            }
-        }
-        __syncthreads();
-        ml_steps_s<T>(L, sO, ch, sY, sD, 0, ML_SW - 1, 0, i0, j0);
-        if (mine) {
-            const double y = ml_step_s<HW>(sO, ch, ML_SW - 1, sY[F], sD[a + ML_SW][b + ML_SW], a + ML_SW, b + ML_SW);
-            if (yout) {
-                const float yf = static_cast<float>(y);
-                yout[e0] = yf;
-                a_r += static_cast<double>(yf) * rcv;
-                a_q += static_cast<double>(yf) * qcv;
-            } else {
-                L.y[e0] = y;
-            }
-        }
-        __syncthreads();
-    }
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x)
+        if (!ml_up_tile<T>(L, next, yout, rc, qc, ch, done, t, sm, a_r, a_q)) return;
     if constexpr (T * T == KB) {  // the partials' block reduction assumes KB threads (level 0 tiles)
         if (pc) {
             double v[4] = {0.0, 0.0, a_r, a_q};
@@ -2296,13 +2328,14 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
                         }
                         ASD2_TRY(record(l + 1, it));
                     }
-                    if (first == Hh.n - 1)
+                    if (first == Hh.n - 1) {
                         hipLaunchKernelGGL(k_ml_dense_apply,
                                            dim3(ceil_div(Hh.L[first].H * Hh.L[first].W, KB / 64) + 1), dim3(KB), 0,
                                            ctx->stream, Hh.L[first], Minv, D.st,
                                            FineRows{D.pu, nbu, pb, nbb, many ? nullptr : fine_totals});
-                    else
+                    } else {
                         hipLaunchKernelGGL(k_ml_lower, dim3(1), dim3(1024), 0, ctx->stream, Hh, first, Minv, D.st, ch);
+                    }
                     for (int l = first - 1; l >= 0; l--) {
                         if (ml_small_tiles(Hh.L[l], l)) {
                             const int nt = ceil_div(Hh.L[l].H, ML_TS) * ceil_div(Hh.L[l].W, ML_TS);
