@@ -900,6 +900,7 @@ struct Coarse {
     double *acx, *acy, *dc;   // coarse operator, (Hc + 1) x (Wc + 1)
     double *g, *rc;           // P^T (r - A D^-1 r), P^T r, Hc x Wc
     double* qc[FCG_M];        // P^T q of the last FCG_M iterations (j = 1: the latest)
+    double* aq;               // P^T A D^-1 q_1 (the multiplicative fine level's coarse dot operand)
 };
 
 // directions p_k and q_k = A p_k of the last FCG_SLOTS iterations
@@ -1020,13 +1021,19 @@ __device__ __forceinline__ bool fcg_alpha(KrylovState* st, const double* ps, int
 // alpha q, partials (r.r, 0) for the next decision), so the sums are those of the new residual.
 // r alternates between two buffers (neighbouring regions read the old r of their halo while
 // the owner writes the new one).
-template <bool UPD, int CC, int MW>
+// MULT (the multiplicative fine level, one direction): z0 = omega D^-1 r before the coarse
+// correction, so g = P^T (r - omega A D^-1 r); with A D^-1 r = r + off, also the rows
+// (A D^-1 r).D^-1 r and (A D^-1 r).D^-1 q_1 into pm and the coarse vector P^T A D^-1 q_1 into C.aq
+// (the fine parts of the smoothed z's dots, see k_lp_spmv_mult).
+template <bool UPD, int CC, int MW, bool MULT = false>
 __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* __restrict__ r_in,
                                                 float* __restrict__ r_out, float* __restrict__ x, Dirs Dd,
                                                 const float* __restrict__ dinv, const float* __restrict__ ax,
                                                 const float* __restrict__ ay, KrylovState* st, const double* ps,
                                                 int nbs, double* pu, int itn,
-                                                int nwin, double* pb, KrylovState* snap) {
+                                                int nwin, double* pb, KrylovState* snap, double omega = 0.0,
+                                                double* pm = nullptr) {
+    static_assert(!MULT || MW == 1, "the multiplicative fine level keeps one direction");
     // snap (pinned host memory, or nullptr): the state as this kernel leaves it, written by the
     // thread that owns the state's updates, for the host's poll after the launch
     auto snapshot = [&]() {
@@ -1043,7 +1050,7 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
 #pragma unroll
     for (int j = 0; j < MW; j++) qj[j] = Dd.q[j];
     constexpr int c = CC, RI = lp_ri(CC), nbi = RI / c, nbj = LP_RJ / c, nblk = nbi * nbj;
-    constexpr int NS = 2 + MW;  // block sums: w, r, q_1 .. q_MW
+    constexpr int NS = 2 + MW + (MULT ? 1 : 0);  // block sums: w, r, q_1 .. q_MW (, A D^-1 q_1)
     // Wave k handles the region rows k, k + NWV, ...  With c >= 4 each lane sums its column over
     // the RPW rows of a block that its wave handles, the segment sum adds the c columns, and the
     // result goes to the wave's own slot (written once); the slots are added in wave order, so
@@ -1053,7 +1060,9 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
     __shared__ float sZ[RI + 2][LP_RJ + 2];
     __shared__ float sR[RI][LP_RJ];
     __shared__ float sQ[UPD ? RI : 1][LP_RJ];  // q of the updated direction (= q_1) on the region
+    __shared__ float sDQ[MULT ? RI + 2 : 1][MULT ? LP_RJ + 2 : 1];  // D^-1 q_1 on the region and halo
     __shared__ double sB[NSLOT][NS][nblk];
+    const bool mq = MULT && nwin >= 1;  // the q_1 terms of the multiplicative level
     const int reg_i = ceil_div(G.nx, RI), nreg = reg_i * ceil_div(G.ny, LP_RJ);
     const int ilo = G.g, ihi = G.g + G.nx, jlo = G.g, jhi = G.g + G.ny;
     const int lane = threadIdx.x & 63;
@@ -1069,14 +1078,14 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
         }
         alpha = static_cast<float>(a64);
     }
-    double a_rz = 0.0, a_uu = 0.0, a_rr = 0.0, a_zq[MW] = {};
+    double a_rz = 0.0, a_uu = 0.0, a_rr = 0.0, a_zq[MW] = {}, a_ss = 0.0, a_sq = 0.0;
     for (int t = t0; t < t1; t++) {
         const int rj = t / reg_i, ri = t - rj * reg_i;
         const int i0 = ilo + ri * RI, j0 = jlo + rj * LP_RJ;
         for (int e = threadIdx.x; e < (RI + 2) * (LP_RJ + 2); e += KB) {
             const int a = e / (LP_RJ + 2), b = e - a * (LP_RJ + 2);
             const int i = i0 - 1 + a, j = j0 - 1 + b;
-            float v = 0.0f;
+            float v = 0.0f, dq = 0.0f;
             if (i >= ilo && i < ihi && j >= jlo && j < jhi) {
                 const int64_t k = G.idx(i, j);
                 float rv = r_in[k];
@@ -1084,30 +1093,35 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
                 if (UPD) {
                     qv = qu_dir[k];
                     rv = -alpha * qv + rv;
+                } else if (mq) {
+                    qv = qj[0][k];
                 }
-                v = dinv[k] * rv;
+                const float dk = dinv[k];
+                v = dk * rv;
+                if (MULT) dq = dk * qv;
                 if (a >= 1 && a <= RI && b >= 1 && b <= LP_RJ) {
                     sR[a - 1][b - 1] = rv;
                     if (UPD) sQ[UPD ? a - 1 : 0][b - 1] = qv;
                 }
             }
             sZ[a][b] = v;
+            if (MULT) sDQ[MULT ? a : 0][MULT ? b : 0] = dq;
         }
         if constexpr (NSLOT == 1)
             for (int e = threadIdx.x; e < NS * nblk; e += KB) (&sB[0][0][0])[e] = 0.0;
         __syncthreads();
-        double cw = 0.0, cr = 0.0, cq[MW] = {};  // the lane's column sums over its rows of a block
+        double cw = 0.0, cr = 0.0, cq[MW] = {}, caq = 0.0;  // the lane's column sums over its rows of a block
         for (int e = threadIdx.x, it_r = 0; e < RI * LP_RJ; e += KB, it_r++) {
             const int a = e / LP_RJ, b = e - a * LP_RJ;  // a wave covers one region row
             const int i = i0 + a, j = j0 + b;
             const bool in = i < ihi && j < jhi;
-            double w = 0.0, rv = 0.0, qv[MW] = {};  // FP64 sums of FP32 data (the inputs are exact)
+            double w = 0.0, rv = 0.0, qv[MW] = {}, aqv = 0.0;  // FP64 sums of FP32 data (the inputs are exact)
             if (in) {
                 const int64_t k = G.idx(i, j);
                 const double zc = sZ[a + 1][b + 1];
-                const double off = ax[k] * static_cast<double>(sZ[a][b + 1]) +
-                                   ax[k + G.sy] * static_cast<double>(sZ[a + 2][b + 1]) +
-                                   ay[k] * static_cast<double>(sZ[a + 1][b]) + ay[k + 1] * static_cast<double>(sZ[a + 1][b + 2]);
+                const float axw = ax[k], axe = ax[k + G.sy], ays = ay[k], ayn = ay[k + 1];
+                const double off = axw * static_cast<double>(sZ[a][b + 1]) + axe * static_cast<double>(sZ[a + 2][b + 1]) +
+                                   ays * static_cast<double>(sZ[a + 1][b]) + ayn * static_cast<double>(sZ[a + 1][b + 2]);
                 const float rf = sR[a][b];
                 if (UPD) {
                     r_out[k] = rf;
@@ -1115,7 +1129,7 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
                     a_rr += static_cast<double>(rf) * rf;
                 }
                 rv = rf;
-                w = -off;
+                w = MULT ? (1.0 - omega) * rv - omega * off : -off;
                 a_rz += zc * rv;
 #pragma unroll
                 for (int jj = 0; jj < MW; jj++)
@@ -1123,17 +1137,30 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
                         qv[jj] = (UPD && jj == 0) ? sQ[UPD ? a : 0][b] : qj[jj][k];
                         a_zq[jj] += zc * qv[jj];
                     }
+                if (MULT) {
+                    const double as = rv + off;  // A D^-1 r (its diagonal term is r)
+                    a_ss += as * zc;
+                    if (mq) {
+                        a_sq += as * static_cast<double>(sDQ[MULT ? a + 1 : 0][MULT ? b + 1 : 0]);
+                        aqv = qv[0] + (axw * static_cast<double>(sDQ[MULT ? a : 0][MULT ? b + 1 : 0]) +
+                                       axe * static_cast<double>(sDQ[MULT ? a + 2 : 0][MULT ? b + 1 : 0]) +
+                                       ays * static_cast<double>(sDQ[MULT ? a + 1 : 0][MULT ? b : 0]) +
+                                       ayn * static_cast<double>(sDQ[MULT ? a + 1 : 0][MULT ? b + 2 : 0]));
+                    }
+                }
             }
             cw += w;
             cr += rv;
 #pragma unroll
             for (int jj = 0; jj < MW; jj++) cq[jj] += qv[jj];
+            if (MULT) caq += aqv;
             if ((it_r + 1) % RPW != 0) continue;  // more rows of this block for this wave
             cw = seg_sum(cw, c);
             cr = seg_sum(cr, c);
 #pragma unroll
             for (int jj = 0; jj < MW; jj++)
                 if (jj < nwin) cq[jj] = seg_sum(cq[jj], c);
+            if (mq) caq = seg_sum(caq, c);
             if ((lane & (c - 1)) == 0) {
                 const int blk = (a / c) * nbj + b / c;
                 if constexpr (NSLOT > 1) {
@@ -1143,15 +1170,17 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
 #pragma unroll
                     for (int jj = 0; jj < MW; jj++)
                         if (jj < nwin) sw[2 + jj][blk] = cq[jj];
+                    if (mq) sw[NS - 1][blk] = caq;
                 } else {
                     atomicAdd(&sB[0][0][blk], cw);
                     atomicAdd(&sB[0][1][blk], cr);
 #pragma unroll
                     for (int jj = 0; jj < MW; jj++)
                         if (jj < nwin) atomicAdd(&sB[0][2 + jj][blk], cq[jj]);
+                    if (mq) atomicAdd(&sB[0][NS - 1][blk], caq);
                 }
             }
-            cw = cr = 0.0;
+            cw = cr = caq = 0.0;
 #pragma unroll
             for (int jj = 0; jj < MW; jj++) cq[jj] = 0.0;
         }
@@ -1172,6 +1201,7 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
 #pragma unroll
                 for (int jj = 0; jj < MW; jj++)
                     if (jj < nwin) C.qc[jj][pc] = slots(2 + jj);
+                if (mq) C.aq[pc] = slots(NS - 1);
                 const double u = gv / C.dc[I * (C.Wc + 1) + J];
                 a_uu += u * u;
             }
@@ -1184,6 +1214,10 @@ __global__ __launch_bounds__(KB) void k_lp_pass(Geom G, Coarse C, const float* _
     for (int jj = 0; jj < MW; jj++) v[1 + jj] = a_zq[jj];
     v[1 + FCG_M] = a_uu;
     put_partials(v, pb);
+    if (MULT) {
+        double m2[2] = {a_ss, a_sq};
+        put_partials(m2, pm);
+    }
     if (UPD) {
         double w2[2] = {a_rr, 0.0};
         put_partials(w2, pu);
@@ -1209,11 +1243,12 @@ __global__ __launch_bounds__(KB) void k_lp_pack(Coarse C, uint16_t* __restrict__
 }
 
 // partials (y.g, y.A_c y, y.P^T r, y.P^T q_1 .. y.P^T q_FCG_M)
+// (and, with pcm, y.P^T A D^-1 q_1 of the multiplicative fine level)
 __global__ __launch_bounds__(KB) void k_lp_coarse_dots(Coarse C, const float* __restrict__ y, const KrylovState* st,
-                                                       int nwin, double* pc) {
+                                                       int nwin, double* pc, double* pcm = nullptr) {
     if (st->done != DONE_RUNNING) return;
     const int64_t Pc = static_cast<int64_t>(C.Hc) * C.Wc;
-    double a[3 + FCG_M] = {};
+    double a[3 + FCG_M] = {}, am[1] = {};
     FLAT_LOOP(p, Pc) {
         const int I = static_cast<int>(p / C.Wc), J = static_cast<int>(p - static_cast<int64_t>(I) * C.Wc);
         const double yv = y[p];
@@ -1223,18 +1258,21 @@ __global__ __launch_bounds__(KB) void k_lp_coarse_dots(Coarse C, const float* __
 #pragma unroll
         for (int j = 0; j < FCG_M; j++)
             if (j < nwin) a[3 + j] += yv * C.qc[j][p];
+        if (pcm && nwin >= 1) am[0] += yv * C.aq[p];
     }
     put_partials(a, pc);
+    if (pcm) put_partials(am, pcm);
 }
 
-// launches k_lp_pass<UPD, c, MW> for the run-time coarsening c
-template <bool UPD, int MW>
+// launches k_lp_pass<UPD, c, MW, MULT> for the run-time coarsening c
+template <bool UPD, int MW, bool MULT = false>
 void launch_lp_pass(hipStream_t s, int nb, Geom G, Coarse C, const float* r_in, float* r_out, float* x, Dirs R,
                     const float* dinv, const float* ax, const float* ay, KrylovState* st, const double* ps,
-                    int nbs, double* pu, int itn, int nwin, double* pb, KrylovState* snap) {
+                    int nbs, double* pu, int itn, int nwin, double* pb, KrylovState* snap, double omega = 0.0,
+                    double* pm = nullptr) {
 #define ASD2_LP_PASS(CV)                                                                                              \
-    hipLaunchKernelGGL((k_lp_pass<UPD, CV, MW>), dim3(nb), dim3(KB), 0, s, G, C, r_in, r_out, x, R, dinv, ax, ay, st, \
-                       ps, nbs, pu, itn, nwin, pb, snap)
+    hipLaunchKernelGGL((k_lp_pass<UPD, CV, MW, MULT>), dim3(nb), dim3(KB), 0, s, G, C, r_in, r_out, x, R, dinv, ax,   \
+                       ay, st, ps, nbs, pu, itn, nwin, pb, snap, omega, pm)
     switch (C.c) {
         case 1: ASD2_LP_PASS(1); break;
         case 2: ASD2_LP_PASS(2); break;
@@ -1428,6 +1466,194 @@ void launch_lp_spmv(hipStream_t s, int nb, Geom G, Coarse C, const float* ax, co
 #undef ASD2_LP_SPMV
 }
 
+// The multiplicative fine level's direction kernel (one kept direction, the multilevel solve).
+// z is the fine level's damped Jacobi step on either side of the coarse correction:
+//   z1 = omega D^-1 r + P y,   z = z1 + omega D^-1 (r - A z1),
+// y the V-cycle of g = P^T (r - omega A D^-1 r) (k_lp_pass<.., MULT>).  z is formed on a tile with a
+// halo of one cell (A z1 from a halo of two), then p = z + b p_1 and q = A p on the tile.  With
+// s = D^-1 r, z = 2 omega s - omega^2 D^-1 A s + P y - omega D^-1 A P y, so its dots split into
+//   z.r   = 2 omega s.r - omega^2 (A s).s + y.g       (y.P^T A s = y.(P^T r - g) / omega)
+//   z.q_1 = 2 omega s.q_1 - omega^2 (A s).D^-1 q_1 + y.P^T q_1 - omega y.P^T A D^-1 q_1,
+// fine rows from the fine pass (pb: s.r, s.q_1; pm: (A s).s, (A s).D^-1 q_1), coarse rows from the
+// level-0 up kernel (pc: y.g, y.A_c y, y.P^T r, y.P^T q_1; pcm: y.P^T A D^-1 q_1), so b =
+// -(z.q_1) / (p_1.q_1) is known before p is formed.  The decision and the recorded scalars are
+// those of k_lp_spmv with gamma = 1.
+template <int CC>
+__global__ __launch_bounds__(KB) void k_lp_spmv_mult(Geom G, Coarse C, const float* __restrict__ ax,
+                                                     const float* __restrict__ ay, const float* __restrict__ dres,
+                                                     int ring_only, const float* __restrict__ r,
+                                                     const float* __restrict__ dinv, const float* __restrict__ y,
+                                                     Dirs Dd, KrylovState* st, const double* pb, const double* pm,
+                                                     int nbb, const double* pc, const double* pcm, int nbc, double* ps,
+                                                     int it, int nwin, double omega, const double* pu, int nbu,
+                                                     const double* fine_totals) {
+    if (st->done != DONE_RUNNING) return;
+    // A tile of FI x FJ cells sits in an operand region of RH x RW = 32 x 64 cells (a halo of two):
+    // lane = region column, wave w = region rows w, w + 4, ..., so every index is a lane or a row.
+    constexpr int RH = 32, RW = 64, FI = RH - 4, FJ = RW - 4;
+    constexpr int NWV = KB / 64, NR = RH / NWV;
+    static_assert(RW == 64, "one region row per wave");
+    const int ilo = G.g, ihi = G.g + G.nx, jlo = G.g, jhi = G.g + G.ny;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int tiles_i = ceil_div(G.nx, FI), ntiles = tiles_i * ceil_div(G.ny, FJ);
+    int t0, t1;
+    tile_range(ntiles, t0, t1);
+    const float* __restrict__ p1 = Dd.p[0];
+    // A tile's operands in one batch of loads: r, D^-1, y, p_1 (0 outside the interior), the W-face
+    // couplings (rows up to the ghost row below the interior) and the S-face couplings (columns up
+    // to the ghost column), which hold the faces the tile and its ring use.
+    float ld_r[NR], ld_d[NR], ld_y[NR], ld_p[NR], ld_ax[NR], ld_ay[NR];
+    auto load = [&](int t) {
+        const int bj = t / tiles_i, bi = t - bj * tiles_i;
+        const int i0 = ilo + bi * FI - 2, j = jlo + bj * FJ - 2 + lane;  // the region's first row, this lane's column
+        const bool jin = j >= jlo && j < jhi, jiny = j >= jlo && j <= jhi;
+        const int jc = (j - jlo) / CC;
+#pragma unroll
+        for (int u = 0; u < NR; u++) {
+            const int i = i0 + wv + NWV * u;
+            const bool in = jin && i >= ilo && i < ihi;
+            const bool inx = jin && i >= ilo && i <= ihi;
+            const bool iny = jiny && i >= ilo && i < ihi;
+            // 32-bit offsets (a padded array holds fewer than 2^31 cells)
+            const unsigned k = static_cast<unsigned>(j + G.sy * i);
+            const unsigned kc = in ? k : 0u;
+            const float vr = r[kc], vd = dinv[kc];
+            const float vy = y[in ? static_cast<unsigned>(((i - ilo) / CC) * C.Wc + jc) : 0u];
+            const float vp = p1[(in && nwin >= 1) ? k : 0u];
+            const float vx = ax[inx ? k : 0u], vya = ay[iny ? k : 0u];
+            ld_r[u] = in ? vr : 0.0f;
+            ld_d[u] = in ? vd : 0.0f;
+            ld_y[u] = in ? vy : 0.0f;
+            ld_p[u] = (in && nwin >= 1) ? vp : 0.0f;
+            ld_ax[u] = inx ? vx : 0.0f;
+            ld_ay[u] = iny ? vya : 0.0f;
+        }
+    };
+    if (t0 < t1) load(t0);
+    // f: r.r, s.r, s.q_1, (A s).s, (A s).D^-1 q_1; cv: y.g, y.A_c y, y.P^T r, y.P^T q_1, y.P^T A D^-1 q_1
+    double f[5], cv[5];
+    if (fine_totals) {
+#pragma unroll
+        for (int k = 0; k < 5; k++) f[k] = fine_totals[k];
+    } else {
+        sum_partials3<1, 2, 2>(pu, nbu, pb, nbb, pm, nbb, f);
+    }
+    sum_partials3<0, 4, 1>(nullptr, 0, pc, nbc, pcm, nbc, cv);
+    {
+        // the decision of iteration `it`
+        const double rn = sqrt(f[0]), resid = rn / st->bnorm;
+        int done = DONE_RUNNING;
+        if (!isfinite(resid)) done = DONE_BREAKDOWN;
+        else if (resid <= st->tol || rn < st->delta * st->rmax || it >= st->maxit) done = DONE_PAUSED;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->rr = f[0];
+            st->iter = it;
+            if (done != DONE_RUNNING) st->done = done;
+        }
+        if (done != DONE_RUNNING) return;
+    }
+    const double w2 = omega * omega;
+    const double rho = 2.0 * omega * f[1] - w2 * f[3] + cv[0];
+    double b = 0.0;
+    bool ok = isfinite(rho) && rho > 0.0;
+    if (nwin >= 1) {
+        const double zq = 2.0 * omega * f[2] - w2 * f[4] + cv[3] - omega * cv[4];
+        b = -zq / st->pqv[(it - 1) % FCG_SLOTS];
+        ok = ok && isfinite(b);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (ok) {
+            st->rz[it & 1] = rho;
+            st->beta = b;
+            st->omega = 1.0;
+        } else {
+            st->done = DONE_BREAKDOWN;
+            st->iter = it;
+        }
+    }
+    if (!ok) return;
+    const float bf = static_cast<float>(b), omf = static_cast<float>(omega);
+    float* __restrict__ pnew = Dd.pnew;
+    float* __restrict__ q = Dd.qnew;
+    __shared__ float sZ[RH][RW];   // z1 on the region
+    __shared__ float sAX[RH][RW];  // W-face couplings
+    __shared__ float sAY[RH][RW];  // S-face couplings
+    __shared__ float sP[RH][RW];   // p on the tile and its ring
+    double acc = 0.0;
+    for (int t = t0; t < t1; t++) {
+        const int bj = t / tiles_i, bi = t - bj * tiles_i;
+        const int i0 = ilo + bi * FI - 2, j = jlo + bj * FJ - 2 + lane;
+        const bool jin = j >= jlo && j < jhi;
+#pragma unroll
+        for (int u = 0; u < NR; u++) {
+            const int rr_ = wv + NWV * u;
+            sZ[rr_][lane] = omf * ld_d[u] * ld_r[u] + ld_y[u];  // 0 outside the interior (every operand 0)
+            sAX[rr_][lane] = ld_ax[u];
+            sAY[rr_][lane] = ld_ay[u];
+        }
+        __syncthreads();
+        // z = z1 + omega D^-1 (r - A z1) and p = z + b p_1 on the tile and its ring (r, D^-1, p_1
+        // from the registers)
+#pragma unroll
+        for (int u = 0; u < NR; u++) {
+            const int rr_ = wv + NWV * u, i = i0 + rr_;
+            if (rr_ < 1 || rr_ > RH - 2 || lane < 1 || lane > RW - 2) continue;
+            float val = 0.0f;
+            if (jin && i >= ilo && i < ihi) {
+                const float zc = sZ[rr_][lane];
+                const bool ring = i == ilo || i == ihi - 1 || j == jlo || j == jhi - 1;
+                const float dv = (!ring_only || ring) ? dres[G.idx(i, j)] : 0.0f;
+                const float az = sAX[rr_][lane] * (sZ[rr_ - 1][lane] - zc) + sAX[rr_ + 1][lane] * (sZ[rr_ + 1][lane] - zc) +
+                                 sAY[rr_][lane] * (sZ[rr_][lane - 1] - zc) + sAY[rr_][lane + 1] * (sZ[rr_][lane + 1] - zc) +
+                                 dv * zc;
+                val = zc + omf * ld_d[u] * (ld_r[u] - az);
+                if (nwin >= 1) val += bf * ld_p[u];
+            }
+            sP[rr_][lane] = val;
+        }
+        __syncthreads();
+        if (t + 1 < t1) load(t + 1);
+#pragma unroll
+        for (int u = 0; u < NR; u++) {
+            const int rr_ = wv + NWV * u, i = i0 + rr_;
+            if (rr_ < 2 || rr_ > RH - 3 || lane < 2 || lane > RW - 3 || !jin || i >= ihi) continue;
+            const int64_t k = G.idx(i, j);
+            const float pcv = sP[rr_][lane];
+            const bool ring = i == ilo || i == ihi - 1 || j == jlo || j == jhi - 1;
+            const float dv = (!ring_only || ring) ? dres[k] : 0.0f;
+            const float qv = sAX[rr_][lane] * (sP[rr_ - 1][lane] - pcv) + sAX[rr_ + 1][lane] * (sP[rr_ + 1][lane] - pcv) +
+                             sAY[rr_][lane] * (sP[rr_][lane - 1] - pcv) + sAY[rr_][lane + 1] * (sP[rr_][lane + 1] - pcv) +
+                             dv * pcv;
+            pnew[k] = pcv;
+            q[k] = qv;
+            acc += static_cast<double>(pcv) * qv;
+        }
+        __syncthreads();
+    }
+    double pq[1] = {acc};
+    put_partials(pq, ps);
+}
+
+// launches k_lp_spmv_mult<c> for the run-time coarsening c
+void launch_lp_spmv_mult(hipStream_t s, int nb, Geom G, Coarse C, const float* ax, const float* ay, const float* dres,
+                         int ring_only, const float* r, const float* dinv, const float* y, Dirs Dd, KrylovState* st,
+                         const double* pb, const double* pm, int nbb, const double* pc, const double* pcm, int nbc,
+                         double* ps, int it, int nwin, double omega, const double* pu, int nbu,
+                         const double* fine_totals) {
+#define ASD2_LP_SPMV_M(CV)                                                                                            \
+    hipLaunchKernelGGL((k_lp_spmv_mult<CV>), dim3(nb), dim3(KB), 0, s, G, C, ax, ay, dres, ring_only, r, dinv, y, Dd, \
+                       st, pb, pm, nbb, pc, pcm, nbc, ps, it, nwin, omega, pu, nbu, fine_totals)
+    switch (C.c) {
+        case 1: ASD2_LP_SPMV_M(1); break;
+        case 2: ASD2_LP_SPMV_M(2); break;
+        case 4: ASD2_LP_SPMV_M(4); break;
+        case 8: ASD2_LP_SPMV_M(8); break;
+        case 16: ASD2_LP_SPMV_M(16); break;
+        default: ASD2_LP_SPMV_M(32); break;
+    }
+#undef ASD2_LP_SPMV_M
+}
+
 // r32 = r64 after a reliable update, partials (r.r, (D^-1 r)^2)
 __global__ __launch_bounds__(KB) void k_fcg_resume(int64_t n, const double* __restrict__ r64, float* __restrict__ r32,
                                                    const float* __restrict__ dinv, const KrylovState* st,
@@ -1468,6 +1694,9 @@ constexpr int ML_SW = 4;            // smoothing steps before and after the coar
 // (LMAX = 2 bounds the spectrum of D^-1 A for these diagonally dominant operators): step 0
 // d = D^-1 r / theta, step k d = a_k d + b_k D^-1 r, each followed by y += d, r = g - A y.
 constexpr double ML_LMAX = 2.0, ML_ALPHA = 32.0;
+// damping of the fine level's Jacobi steps before and after the coarse correction
+// (k_lp_spmv_mult): below 1 so that omega D^-1 A stays inside (0, 2) with a margin
+constexpr double ML_OMEGA = 0.8;
 struct Cheb {
     double c0;          // 1 / theta
     double a[ML_SW];    // a_k = rho_k rho_{k-1} (k >= 1)
@@ -1837,11 +2066,13 @@ __global__ __launch_bounds__(1024) void k_ml_lower(Hierarchy Hh, int first, cons
 // iteration's scalars go to pc (the step along P y is fixed at 1, so y.g and y.A_c y are not
 // needed).  The tile's operator, y1, the coarse values under it and (level 0) the coarse dots'
 // operands are loaded in one batch with the read of the solver state.
-// one tile of k_ml_up (yout, rc, qc: level 0 only; a_r, a_q accumulate the partials' operands)
+// one tile of k_ml_up (yout, rc, qc, aqc: level 0 only; a_r, a_q, a_g, a_aq accumulate the partials'
+// operands y.P^T r, y.P^T q_1, and with aqc the multiplicative level's y.g and y.P^T A D^-1 q_1)
 template <int T>
 __device__ __forceinline__ bool ml_up_tile(const Level& L, const Level& next, float* __restrict__ yout,
-                                           const double* rc, const double* qc, const Cheb& ch, int done, int t,
-                                           MlTileLds<T>& sm, double& a_r, double& a_q) {
+                                           const double* rc, const double* qc, const double* aqc, const Cheb& ch,
+                                           int done, int t, MlTileLds<T>& sm, double& a_r, double& a_q, double& a_g,
+                                           double& a_aq) {
     constexpr int HW = T + 2 * ML_SW, NT = T * T;
     const int tiles_j = ceil_div(L.W, T);
     const int a = threadIdx.x / T, b = threadIdx.x - a * T;
@@ -1865,6 +2096,7 @@ __device__ __forceinline__ bool ml_up_tile(const Level& L, const Level& next, fl
     const bool mine = i < L.H && j < L.W;
     const int64_t e0 = mine ? static_cast<int64_t>(i) * L.W + j : 0;
     const double rcv = yout ? rc[e0] : 0.0, qcv = yout ? qc[e0] : 0.0;
+    const double gcv = (yout && aqc) ? L.g[e0] : 0.0, aqv = (yout && aqc) ? aqc[e0] : 0.0;
     if (done != DONE_RUNNING) return false;
     tr.store(sm.o);
 #pragma unroll
@@ -1884,6 +2116,8 @@ __device__ __forceinline__ bool ml_up_tile(const Level& L, const Level& next, fl
             yout[e0] = yf;
             a_r += static_cast<double>(yf) * rcv;
             a_q += static_cast<double>(yf) * qcv;
+            a_g += static_cast<double>(yf) * gcv;
+            a_aq += static_cast<double>(yf) * aqv;
         } else {
             L.y[e0] = y;
         }
@@ -1898,19 +2132,26 @@ __device__ __forceinline__ bool ml_up_tile(const Level& L, const Level& next, fl
 // iteration's scalars go to pc (the step along P y is fixed at 1, so y.g and y.A_c y are not
 // needed).  The tile's operator, y1, the coarse values under it and (level 0) the coarse dots'
 // operands are loaded in one batch with the read of the solver state.
+// With aqc (the multiplicative fine level) the level-0 launch also writes y.g into row 0 of pc and
+// y.P^T A D^-1 q_1 into pcm.
 template <int T>
 __global__ __launch_bounds__(T * T) void k_ml_up(Level L, Level next, float* __restrict__ yout, const double* rc,
-                                                 const double* qc, const KrylovState* st, double* pc, Cheb ch) {
+                                                 const double* qc, const KrylovState* st, double* pc, Cheb ch,
+                                                 const double* aqc = nullptr, double* pcm = nullptr) {
     __shared__ MlTileLds<T> sm;
     const int done = st->done;
     const int ntiles = ceil_div(L.H, T) * ceil_div(L.W, T);
-    double a_r = 0.0, a_q = 0.0;
+    double a_r = 0.0, a_q = 0.0, a_g = 0.0, a_aq = 0.0;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x)
-        if (!ml_up_tile<T>(L, next, yout, rc, qc, ch, done, t, sm, a_r, a_q)) return;
+        if (!ml_up_tile<T>(L, next, yout, rc, qc, aqc, ch, done, t, sm, a_r, a_q, a_g, a_aq)) return;
     if constexpr (T * T == KB) {  // the partials' block reduction assumes KB threads (level 0 tiles)
         if (pc) {
-            double v[4] = {0.0, 0.0, a_r, a_q};
+            double v[4] = {a_g, 0.0, a_r, a_q};
             put_partials(v, pc);
+        }
+        if (pcm) {
+            double v[1] = {a_aq};
+            put_partials(v, pcm);
         }
     }
 }
@@ -1923,6 +2164,7 @@ struct FineRows {
     const double* pb;
     int nbb;
     double* out;  // nullptr: not summed here
+    const double* pm = nullptr;  // the multiplicative fine level's two rows, summed after the three
 };
 
 // y = A^-1 g on the coarsest level (FP64 inverse): one row per wave, the lanes splitting it; with
@@ -1931,10 +2173,17 @@ __global__ __launch_bounds__(KB) void k_ml_dense_apply(Level L, const double* __
                                                        const KrylovState* st, FineRows F) {
     if (st->done != DONE_RUNNING) return;
     if (F.out && blockIdx.x == gridDim.x - 1) {
-        double v[3];
-        sum_partials3<1, 2, 0>(F.pu, F.nbu, F.pb, F.nbb, nullptr, 0, v);
-        if (threadIdx.x == 0)
-            for (int k = 0; k < 3; k++) F.out[k] = v[k];
+        if (F.pm) {
+            double v[5];
+            sum_partials3<1, 2, 2>(F.pu, F.nbu, F.pb, F.nbb, F.pm, F.nbb, v);
+            if (threadIdx.x == 0)
+                for (int k = 0; k < 5; k++) F.out[k] = v[k];
+        } else {
+            double v[3];
+            sum_partials3<1, 2, 0>(F.pu, F.nbu, F.pb, F.nbb, nullptr, 0, v);
+            if (threadIdx.x == 0)
+                for (int k = 0; k < 3; k++) F.out[k] = v[k];
+        }
         return;
     }
     const int m = L.H * L.W, lane = threadIdx.x & 63;
@@ -2097,7 +2346,7 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
         ASD2_TRY(ws(ctx, ring_names[0][k], n, &R.p[k]));
         ASD2_TRY(ws(ctx, ring_names[1][k], n, &R.q[k]));
     }
-    ASD2_TRY(ws(ctx, "lp_partials", (5 + 2 * FCG_M) * MAX_BLOCKS, &lp_part));
+    ASD2_TRY(ws(ctx, "lp_partials", (8 + 2 * FCG_M) * MAX_BLOCKS, &lp_part));
     Coarse C{};
     C.c = c0;
     C.Hc = G.nx / c0;
@@ -2110,6 +2359,7 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
     ASD2_TRY(ws(ctx, "lp_rc", Pc, &C.rc));
     static const char* qc_names[FCG_M] = {"lp_qc0", "lp_qc1", "lp_qc2", "lp_qc3"};
     for (int j = 0; j < FCG_M; j++) ASD2_TRY(ws(ctx, qc_names[j], Pc, &C.qc[j]));
+    ASD2_TRY(ws(ctx, "lp_aq", Pc, &C.aq));
     // multilevel hierarchy: level 0 shares the first coarsening's arrays
     Hierarchy Hh{};
     double *Minv = nullptr, *Mwork = nullptr;
@@ -2160,9 +2410,11 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
         ASD2_TRY(ws(ctx, "ml_yout", Pc, &yml));
     }
     double* fine_totals = nullptr;  // the fine rows of the SpMV, summed by the dense-apply launch
-    ASD2_TRY(ws(ctx, "lp_fine_totals", 4, &fine_totals));
+    ASD2_TRY(ws(ctx, "lp_fine_totals", 6, &fine_totals));
     double* pb = lp_part;  // fine pass: r.D^-1 r, (D^-1 r).q_j, sum (g / dc)^2
     double* pc = lp_part + (2 + FCG_M) * MAX_BLOCKS;  // coarse dots: y.g, y.A_c y, y.P^T r, y.P^T q_j
+    double* pm = lp_part + (5 + 2 * FCG_M) * MAX_BLOCKS;  // multiplicative level: (A s).s, (A s).D^-1 q_1
+    double* pcm = pm + 2 * MAX_BLOCKS;                    // ... and y.P^T A D^-1 q_1
     const int nbs = spmv_blocks(G), nbf = flat_blocks(n), nbi = interior_blocks(G);
     const int RI = lp_ri(C.c);
     const int nbb = std::max(1, std::min<int>(MAX_BLOCKS, ceil_div(G.nx, RI) * ceil_div(G.ny, LP_RJ)));
@@ -2227,6 +2479,9 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
     // one direction makes the recurrence PCG
     const int mwin = (learned && !hybrid) ? FCG_M : (hybrid ? HYBRID_M : 1);
     const bool many = mwin > 1;  // the kernels' direction count: FCG_M or 1
+    // the multilevel solve smooths the fine level on both sides of its coarse correction
+    // (k_lp_spmv_mult); a Jacobi fallback returns to the additive kernels
+    const bool mult = !learned && !many && Hh.n >= 1;
     int wstart = 0;  // first iteration of the current window of directions
     auto nwin = [&](int k) { return std::max(0, std::min(mwin, k - wstart)); };
     const Cheb ch = ml_cheb();  // the V-cycle's smoother
@@ -2270,8 +2525,13 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
         for (int k = 0; k < count; k++, it++) {
             float* r = rb[it & 1];
             if (fresh) {
-                (many ? launch_lp_pass<false, FCG_M> : launch_lp_pass<false, 1>)(ctx->stream, nbb, G, C, r, nullptr, x, dirs_for(R, it), S.dinv32, S.ax32, S.ay32,
-                                      D.st, D.ps, nbs, nullptr, it, nwin(it), pb, nullptr);
+                if (mult && use_net)
+                    launch_lp_pass<false, 1, true>(ctx->stream, nbb, G, C, r, nullptr, x, dirs_for(R, it), S.dinv32,
+                                                   S.ax32, S.ay32, D.st, D.ps, nbs, nullptr, it, nwin(it), pb, nullptr,
+                                                   ML_OMEGA, pm);
+                else
+                    (many ? launch_lp_pass<false, FCG_M> : launch_lp_pass<false, 1>)(ctx->stream, nbb, G, C, r, nullptr, x, dirs_for(R, it), S.dinv32, S.ax32, S.ay32,
+                                      D.st, D.ps, nbs, nullptr, it, nwin(it), pb, nullptr, 0.0, nullptr);
                 fresh = false;
             }
             if (use_net) {
@@ -2332,7 +2592,8 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
                         hipLaunchKernelGGL(k_ml_dense_apply,
                                            dim3(ceil_div(Hh.L[first].H * Hh.L[first].W, KB / 64) + 1), dim3(KB), 0,
                                            ctx->stream, Hh.L[first], Minv, D.st,
-                                           FineRows{D.pu, nbu, pb, nbb, many ? nullptr : fine_totals});
+                                           FineRows{D.pu, nbu, pb, nbb, many ? nullptr : fine_totals,
+                                                    (mult && use_net) ? pm : nullptr});
                     } else {
                         hipLaunchKernelGGL(k_ml_lower, dim3(1), dim3(1024), 0, ctx->stream, Hh, first, Minv, D.st, ch);
                     }
@@ -2347,25 +2608,39 @@ int coarse_fcg_solve(asd2_ctx* ctx, const asd2_solve_opts* o, double* d_ps, asd2
                                                dim3(l == 0 ? nbc : flat_blocks(static_cast<int64_t>(Hh.L[l].H) * Hh.L[l].W)),
                                                dim3(KB), 0, ctx->stream, Hh.L[l], Hh.L[l + 1],
                                                l == 0 ? yml : static_cast<float*>(nullptr), C.rc, C.qc[0], D.st,
-                                               l == 0 ? pc : static_cast<double*>(nullptr), ch);
+                                               l == 0 ? pc : static_cast<double*>(nullptr), ch,
+                                               (l == 0 && mult && use_net) ? C.aq : static_cast<const double*>(nullptr),
+                                               (l == 0 && mult && use_net) ? pcm : static_cast<double*>(nullptr));
                         }
                     }
                 }
                 if (learned || Hh.n == 1)
                     hipLaunchKernelGGL(k_lp_coarse_dots, dim3(nbc), dim3(KB), 0, ctx->stream, C, ycoarse, D.st,
-                                       nwin(it), pc);
+                                       nwin(it), pc, (mult && use_net) ? pcm : nullptr);
                 prof_end(ctx, ASD2_KID_COARSE);
             }
             prof_begin(ctx, ASD2_KID_FCG_SPMV);
-            (many ? launch_lp_spmv<FCG_M> : launch_lp_spmv<1>)(
-                ctx->stream, nbs, G, C, S.ax32, S.ay32, S.diag32, ctx->dres_ring_only ? 1 : 0, r, S.dinv32,
-                use_net ? ycoarse : nullptr,
-                dirs_for(R, it), D.st, pb, nbb, pc, nbc, D.ps, it, nwin(it), (learned && !hybrid) ? 0.0 : 1.0, D.pu,
-                nbu, (use_net && !learned && Hh.n > 1 && first == Hh.n - 1 && !many) ? fine_totals : nullptr);
+            const double* ft = (use_net && !learned && Hh.n > 1 && first == Hh.n - 1 && !many) ? fine_totals : nullptr;
+            if (mult && use_net)
+                launch_lp_spmv_mult(ctx->stream, nbs, G, C, S.ax32, S.ay32, S.diag32, ctx->dres_ring_only ? 1 : 0, r,
+                                    S.dinv32, ycoarse, dirs_for(R, it), D.st, pb, pm, nbb, pc, pcm, nbc, D.ps, it,
+                                    nwin(it), ML_OMEGA, D.pu, nbu, ft);
+            else
+                (many ? launch_lp_spmv<FCG_M> : launch_lp_spmv<1>)(
+                    ctx->stream, nbs, G, C, S.ax32, S.ay32, S.diag32, ctx->dres_ring_only ? 1 : 0, r, S.dinv32,
+                    use_net ? ycoarse : nullptr,
+                    dirs_for(R, it), D.st, pb, nbb, pc, nbc, D.ps, it, nwin(it), (learned && !hybrid) ? 0.0 : 1.0, D.pu,
+                    nbu, ft);
             prof_end(ctx, ASD2_KID_FCG_SPMV);
             prof_begin(ctx, ASD2_KID_FCG_UPDATE);
-            (many ? launch_lp_pass<true, FCG_M> : launch_lp_pass<true, 1>)(ctx->stream, nbb, G, C, r, rb[(it + 1) & 1], x, dirs_for(R, it + 1), S.dinv32, S.ax32, S.ay32,
-                                 D.st, D.ps, nbs, D.pu, it + 1, nwin(it + 1), pb, k + 1 == count ? snap : nullptr);
+            if (mult && use_net)
+                launch_lp_pass<true, 1, true>(ctx->stream, nbb, G, C, r, rb[(it + 1) & 1], x, dirs_for(R, it + 1),
+                                              S.dinv32, S.ax32, S.ay32, D.st, D.ps, nbs, D.pu, it + 1, nwin(it + 1), pb,
+                                              k + 1 == count ? snap : nullptr, ML_OMEGA, pm);
+            else
+                (many ? launch_lp_pass<true, FCG_M> : launch_lp_pass<true, 1>)(ctx->stream, nbb, G, C, r, rb[(it + 1) & 1], x, dirs_for(R, it + 1), S.dinv32, S.ax32, S.ay32,
+                                     D.st, D.ps, nbs, D.pu, it + 1, nwin(it + 1), pb, k + 1 == count ? snap : nullptr,
+                                     0.0, nullptr);
             prof_end(ctx, ASD2_KID_FCG_UPDATE);
             nbu = nbb;
         }

@@ -11,8 +11,9 @@ algebra can be checked:
   both sides are at most 16; the coarsest system is solved with its inverse;
 * smoothing on every level above the coarsest, before and after the coarse correction: the
   degree-4 Chebyshev polynomial in D^-1 A for the interval [2 / 32, 2];
-* fine level: z = D^-1 r + P y, y = V-cycle(P^T (r - A D^-1 r)) (one Jacobi step, then the coarse
-  correction of what it leaves);
+* fine level: a damped Jacobi step (omega = 0.8) on either side of the coarse correction,
+  z1 = omega D^-1 r + P y with y = V-cycle(P^T (r - omega A D^-1 r)), z = z1 + omega D^-1 (r - A z1)
+  (a symmetric multiplicative two-grid step on the fine level);
 * Krylov: flexible CG with one direction kept, beta = -(z.q_old) / (p_old.q_old),
   alpha = (z.r) / (p.q), stopping at ||r|| <= tol ||b||.
 """
@@ -23,6 +24,7 @@ import scipy.sparse as sp
 
 G = 3
 CHEB_LMAX, CHEB_ALPHA, CHEB_STEPS = 2.0, 32.0, 4
+OMEGA = 0.8  # damping of the fine level's Jacobi steps
 
 
 def stencil_matrix(ax, ay, dg, nx, ny, g=G):
@@ -124,8 +126,24 @@ class Multilevel:
         return self.smooth(A, dc, g, y)
 
     def __call__(self, r):
-        z = self.dinv * r
-        return z + self.P0 @ self.vcycle(self.P0.T @ (r - self.A @ z))
+        z = OMEGA * self.dinv * r
+        z = z + self.P0 @ self.vcycle(self.P0.T @ (r - self.A @ z))
+        return z + OMEGA * self.dinv * (r - self.A @ z)
+
+    def split_dots(self, r, q):
+        """(z.r, z.q) of z = M(r) from the parts the GPU sums separately: with s = D^-1 r,
+        g = P^T (r - omega A s) and y = V-cycle(g),
+        z.r = 2 omega s.r - omega^2 (A s).s + y.g,
+        z.q = 2 omega s.q - omega^2 (A s).D^-1 q + y.P^T q - omega y.P^T A D^-1 q"""
+        w = OMEGA
+        s = self.dinv * r
+        As = self.A @ s
+        g = self.P0.T @ (r - w * As)
+        y = self.vcycle(g)
+        dq = self.dinv * q
+        zr = 2 * w * (s @ r) - w * w * (As @ s) + y @ g
+        zq = 2 * w * (s @ q) - w * w * (As @ dq) + y @ (self.P0.T @ q) - w * (y @ (self.P0.T @ (self.A @ dq)))
+        return zr, zq
 
 
 def fcg(A, b, M, tol=1e-8, maxit=10000):

@@ -1,8 +1,10 @@
 """Pins of the multilevel restatement (oracle/multilevel.py), the CPU statement of the GPU's
-multilevel coarse correction: the V-cycle is a symmetric positive definite linear map, one level is
-the textbook two-level formula, the smoother is the Chebyshev polynomial with its minimax error on
-[2 / 32, 2], and the flexible CG with it converges to the direct solution.  Iteration counts on
-seeded systems are recorded for the GPU comparison (tests/test_gpu_multilevel.py)."""
+multilevel coarse correction: the V-cycle and the whole fine-level map are symmetric positive
+definite linear maps, one level is the textbook symmetric two-grid formula (damped Jacobi, exact
+coarse solve, damped Jacobi), the split dots the GPU sums equal the direct ones, the smoother is the
+Chebyshev polynomial with its minimax error on [2 / 32, 2], and the flexible CG with it converges
+to the direct solution.  Iteration counts on seeded systems are recorded for the GPU comparison
+(tests/test_gpu_multilevel.py)."""
 import numpy as np
 import pytest
 import scipy.sparse.linalg as spla
@@ -36,9 +38,34 @@ def test_one_level_is_the_two_level_formula(oracle):
     M = ML.Multilevel(A, 128, 96, levels=0)
     A0 = (M.P0.T @ A @ M.P0).tocsc()
     r = np.random.default_rng(0).standard_normal(b.size)
-    z = M.dinv * r
-    want = z + M.P0 @ spla.spsolve(A0, M.P0.T @ (r - A @ z))
+    w = 0.8
+    z = w * r / A.diagonal()
+    z = z + M.P0 @ spla.spsolve(A0, M.P0.T @ (r - A @ z))
+    want = z + w * (r - A @ z) / A.diagonal()
     np.testing.assert_allclose(M(r), want, rtol=1e-9, atol=1e-9 * np.abs(want).max())
+
+
+def test_fine_level_map_is_spd(oracle):
+    """r -> z of the whole preconditioner, as a matrix on a small system: symmetric (the two damped
+    Jacobi steps mirror each other around a symmetric coarse correction) and positive definite"""
+    nx, ny = 48, 40
+    A, _ = _system(oracle, nx, ny, 7)
+    M = ML.Multilevel(A, nx, ny)
+    Z = np.stack([M(e) for e in np.eye(nx * ny)], axis=1)
+    assert np.abs(Z - Z.T).max() <= 1e-10 * np.abs(Z).max()
+    assert np.linalg.eigvalsh(0.5 * (Z + Z.T)).min() > 0
+
+
+def test_split_dots_equal_direct(oracle):
+    """the GPU forms z.r and z.q_1 from separately summed fine and coarse parts before z exists"""
+    A, b = _system(oracle, 128, 96, 5)
+    M = ML.Multilevel(A, 128, 96)
+    rng = np.random.default_rng(1)
+    r, q = rng.standard_normal(b.size), rng.standard_normal(b.size)
+    z = M(r)
+    zr, zq = M.split_dots(r, q)
+    assert zr == pytest.approx(z @ r, rel=1e-10)
+    assert zq == pytest.approx(z @ q, rel=1e-9, abs=1e-12 * np.linalg.norm(z) * np.linalg.norm(q))
 
 
 def test_chebyshev_smoother_is_minimax():
