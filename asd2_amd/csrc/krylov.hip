@@ -1490,6 +1490,7 @@ __global__ __launch_bounds__(KB) void k_lp_spmv_mult(Geom G, Coarse C, const flo
     if (st->done != DONE_RUNNING) return;
     // A tile of FI x FJ cells sits in an operand region of RH x RW = 32 x 64 cells (a halo of two):
     // lane = region column, wave w = region rows w, w + 4, ..., so every index is a lane or a row.
+    // (40-row regions, two tiles per block at 2048^2, ran at 3 waves per SIMD and took longer.)
     constexpr int RH = 32, RW = 64, FI = RH - 4, FJ = RW - 4;
     constexpr int NWV = KB / 64, NR = RH / NWV;
     static_assert(RW == 64, "one region row per wave");
@@ -1502,14 +1503,28 @@ __global__ __launch_bounds__(KB) void k_lp_spmv_mult(Geom G, Coarse C, const flo
     // A tile's operands in one batch of loads: r, D^-1, y, p_1 (0 outside the interior), the W-face
     // couplings (rows up to the ghost row below the interior) and the S-face couplings (columns up
     // to the ghost column), which hold the faces the tile and its ring use.
+    // A block's consecutive tiles are vertically adjacent within a column of tiles: the next
+    // region's rows 0 .. 3 are this region's rows FI .. FI + 3, which each wave holds as its last
+    // row (FI is a multiple of the wave count), so they move from u = NR - 1 to u = 0 instead of
+    // being loaded again.
+    static_assert(FI % NWV == 0, "the carried rows stay with their waves");
     float ld_r[NR], ld_d[NR], ld_y[NR], ld_p[NR], ld_ax[NR], ld_ay[NR];
-    auto load = [&](int t) {
+    auto load = [&](int t, bool carry) {
         const int bj = t / tiles_i, bi = t - bj * tiles_i;
         const int i0 = ilo + bi * FI - 2, j = jlo + bj * FJ - 2 + lane;  // the region's first row, this lane's column
         const bool jin = j >= jlo && j < jhi, jiny = j >= jlo && j <= jhi;
         const int jc = (j - jlo) / CC;
 #pragma unroll
         for (int u = 0; u < NR; u++) {
+            if (u == 0 && carry) {
+                ld_r[0] = ld_r[NR - 1];
+                ld_d[0] = ld_d[NR - 1];
+                ld_y[0] = ld_y[NR - 1];
+                ld_p[0] = ld_p[NR - 1];
+                ld_ax[0] = ld_ax[NR - 1];
+                ld_ay[0] = ld_ay[NR - 1];
+                continue;
+            }
             const int i = i0 + wv + NWV * u;
             const bool in = jin && i >= ilo && i < ihi;
             const bool inx = jin && i >= ilo && i <= ihi;
@@ -1529,7 +1544,7 @@ __global__ __launch_bounds__(KB) void k_lp_spmv_mult(Geom G, Coarse C, const flo
             ld_ay[u] = iny ? vya : 0.0f;
         }
     };
-    if (t0 < t1) load(t0);
+    if (t0 < t1) load(t0, false);
     // f: r.r, s.r, s.q_1, (A s).s, (A s).D^-1 q_1; cv: y.g, y.A_c y, y.P^T r, y.P^T q_1, y.P^T A D^-1 q_1
     double f[5], cv[5];
     if (fine_totals) {
@@ -1612,7 +1627,7 @@ __global__ __launch_bounds__(KB) void k_lp_spmv_mult(Geom G, Coarse C, const flo
             sP[rr_][lane] = val;
         }
         __syncthreads();
-        if (t + 1 < t1) load(t + 1);
+        if (t + 1 < t1) load(t + 1, (t + 1) % tiles_i != 0);
 #pragma unroll
         for (int u = 0; u < NR; u++) {
             const int rr_ = wv + NWV * u, i = i0 + rr_;
