@@ -87,7 +87,7 @@ typedef struct asd2_solve_opts {
     int precision;       /* ASD2_PREC_* */
     int precond;         /* ASD2_PRECOND_* */
     double reliable_delta; /* MIXED: FP64 residual replacement when ||r|| drops by this factor
-                              (0 -> 0.01 for the multilevel solver, 0.1 otherwise) */
+                              (0 -> 1e-4 for the multilevel solver, 0.1 otherwise) */
     int check_every;     /* iterations enqueued between host convergence polls (0 -> 64) */
 } asd2_solve_opts;
 

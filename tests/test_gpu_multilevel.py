@@ -79,14 +79,18 @@ def test_multilevel_deterministic(oracle):
 @pytest.mark.parametrize("seed", [901, 902])
 def test_multilevel_counts_equal_restatement(oracle, seed):
     """The GPU solve takes the iterations the FP64 restatement (oracle/multilevel.py) takes on the
-    same seeded-droplet system (512^2, exported from the GPU assembly), to within 3."""
+    same seeded-droplet system (512^2, exported from the GPU assembly), to within 3 (FP32 inner
+    iteration, FP64 residual replacement when the residual drops by 1e-2)."""
     from asd2_amd import learned as LN
     from oracle import multilevel as ML
     n = 512
     ctx = LN.system_context(n)
     LN.build_system(ctx, np.random.default_rng(seed))
     ax, ay, dg, rhs = ctx.gfm_export()
-    st = ctx.poisson_solve(ctx.array(), ctx.opts(tol=1e-8, precond=A.PRECOND_MULTILEVEL, precision=A.PREC_MIXED))
+    # FP64 residual replacement every two decades keeps the FP32 recurrence on the restatement's
+    # path; the default's four decades add a few iterations of FP32 drift on some systems
+    st = ctx.poisson_solve(ctx.array(), ctx.opts(tol=1e-8, precond=A.PRECOND_MULTILEVEL, precision=A.PREC_MIXED,
+                                                 reliable_delta=0.01))
     Am = ML.stencil_matrix(ax, ay, dg, n, n)
     b = rhs[3:3 + n, 3:3 + n].ravel().copy()
     _, it, ok = ML.fcg(Am, b, ML.Multilevel(Am, n, n), tol=1e-8)
