@@ -1694,11 +1694,13 @@ __global__ void k_set_done(KrylovState* st, int from, int to) {
 // ------------------------------------------------------------------ multilevel coarse correction
 // The same flexible PCG with y from a V-cycle instead of the network: the coarse operators of
 // a hierarchy of block aggregations (level 0 = the c x c blocks of the fine grid, each next
-// level factor 4 or 2 coarser, down to at most 16 x 16), one Jacobi sweep before and after the
-// coarse correction on every level, and the coarsest system solved exactly with its inverse,
-// formed once per solve.  Everything is linear in g, so only gamma varies between iterations.
-// Levels of at least ML_WIDE cells run on the whole chip (one thread per cell); the small tail
-// of the hierarchy and the coarsest solve run in one workgroup.
+// level factor 4 or 2 coarser, down to at most 16 x 16), Chebyshev smoothing before and after
+// the coarse correction on every level, and the coarsest system solved exactly with its inverse,
+// formed once per solve.  On the fine level a damped Jacobi step on either side of the coarse
+// correction (k_lp_pass<.., MULT>, k_lp_spmv_mult) makes the preconditioner a symmetric
+// multiplicative two-grid step; the step along P y is 1.  Levels of at least ML_WIDE cells run on
+// the whole chip (tiles of T x T cells); the small tail of the hierarchy and the coarsest solve
+// run in one workgroup.
 
 constexpr int ML_MAX_LEVELS = 8;
 constexpr int ML_MAX_DENSE = 256;   // unknowns of the coarsest level

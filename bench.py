@@ -39,7 +39,8 @@ BYTES_PER_CELL = {
     "pcg64_update": 8 * 8,   # x, p, r, q, dinv -> x, r, z
     "pcg32_spmv": 6 * 4,
     "pcg32_update": 8 * 4,
-    "fcg_spmv": 7 * 4,       # r, dinv, p_old, beta_x, beta_y -> p_new, q (z formed on the fly, diag = 1/dinv)
+    "fcg_spmv": 7 * 4,       # r, dinv, p_old, beta_x, beta_y -> p_new, q (z formed on the fly with the
+                             # fine level's smoothing, diag = 1/dinv)
     "fcg_update": 9 * 4,     # x, p, r, q, dinv, beta_x, beta_y -> x, r' (+ coarse sums of the new r)
     "weno_stage": 112 / 3,   # 32 + 40 + 40 bytes over the 3 stages of an advection step
     "reinit": 56,            # one MinRK2 iteration per launch: substage A 24 B + substage B 32 B
@@ -54,7 +55,7 @@ BYTES_PER_CELL = {
 # asd2_stream_copy of 2^25 doubles); the roofline's `traffic` is this figure x cells.
 MEASURED_BYTES_PER_CELL = {
     "pcg64_spmv": 61.5, "pcg64_update": 64.4, "pcg32_spmv": 31.2, "pcg32_update": 32.2,
-    "fcg_spmv": 31.7, "fcg_update": 41.4, "weno_stage": (40.1 + 50.3 + 50.2) / 3,
+    "fcg_spmv": 42.5, "fcg_update": 41.6, "weno_stage": (40.1 + 50.3 + 50.2) / 3,
     "reinit": 30.6, "gfm": 100.6, "momentum": (59.9 + 58.9) / 2,
 }
 
