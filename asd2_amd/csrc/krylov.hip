@@ -1496,9 +1496,19 @@ __global__ __launch_bounds__(KB) void k_lp_spmv_mult(Geom G, Coarse C, const flo
     static_assert(RW == 64, "one region row per wave");
     const int ilo = G.g, ihi = G.g + G.nx, jlo = G.g, jhi = G.g + G.ny;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int tiles_i = ceil_div(G.nx, FI), ntiles = tiles_i * ceil_div(G.ny, FJ);
-    int t0, t1;
-    tile_range(ntiles, t0, t1);
+    // Each block takes `per` consecutive tile slots of one column of tiles, the column padded to a
+    // multiple of `per` (empty slots are skipped), so that the blocks of horizontally adjacent
+    // columns (logical indices tiles_ip / per apart, on the same XCD) work on the same rows at the
+    // same time and the cache lines their region edges share are fetched once.
+    const int tiles_i = ceil_div(G.nx, FI), tiles_j = ceil_div(G.ny, FJ);
+    int per = ceil_div(tiles_i * tiles_j, gridDim.x), tiles_ip = ceil_div(tiles_i, per) * per;
+    while (ceil_div(tiles_ip * tiles_j, gridDim.x) > per) {
+        per = ceil_div(tiles_ip * tiles_j, gridDim.x);
+        tiles_ip = ceil_div(tiles_i, per) * per;
+    }
+    const int t0 = logical_block() * per;
+    int t1 = min(tiles_ip * tiles_j, t0 + per);
+    if (t0 < t1 && t0 % tiles_ip + (t1 - t0) > tiles_i) t1 = t0 + max(0, tiles_i - t0 % tiles_ip);  // empty slots
     const float* __restrict__ p1 = Dd.p[0];
     // A tile's operands in one batch of loads: r, D^-1, y, p_1 (0 outside the interior), the W-face
     // couplings (rows up to the ghost row below the interior) and the S-face couplings (columns up
@@ -1510,7 +1520,7 @@ __global__ __launch_bounds__(KB) void k_lp_spmv_mult(Geom G, Coarse C, const flo
     static_assert(FI % NWV == 0, "the carried rows stay with their waves");
     float ld_r[NR], ld_d[NR], ld_y[NR], ld_p[NR], ld_ax[NR], ld_ay[NR];
     auto load = [&](int t, bool carry) {
-        const int bj = t / tiles_i, bi = t - bj * tiles_i;
+        const int bj = t / tiles_ip, bi = t - bj * tiles_ip;
         const int i0 = ilo + bi * FI - 2, j = jlo + bj * FJ - 2 + lane;  // the region's first row, this lane's column
         const bool jin = j >= jlo && j < jhi, jiny = j >= jlo && j <= jhi;
         const int jc = (j - jlo) / CC;
@@ -1596,7 +1606,7 @@ __global__ __launch_bounds__(KB) void k_lp_spmv_mult(Geom G, Coarse C, const flo
     __shared__ float sP[RH][RW];   // p on the tile and its ring
     double acc = 0.0;
     for (int t = t0; t < t1; t++) {
-        const int bj = t / tiles_i, bi = t - bj * tiles_i;
+        const int bj = t / tiles_ip, bi = t - bj * tiles_ip;
         const int i0 = ilo + bi * FI - 2, j = jlo + bj * FJ - 2 + lane;
         const bool jin = j >= jlo && j < jhi;
 #pragma unroll
@@ -1627,7 +1637,7 @@ __global__ __launch_bounds__(KB) void k_lp_spmv_mult(Geom G, Coarse C, const flo
             sP[rr_][lane] = val;
         }
         __syncthreads();
-        if (t + 1 < t1) load(t + 1, (t + 1) % tiles_i != 0);
+        if (t + 1 < t1) load(t + 1, true);  // a block's slots never cross a column of tiles
 #pragma unroll
         for (int u = 0; u < NR; u++) {
             const int rr_ = wv + NWV * u, i = i0 + rr_;
