@@ -55,7 +55,7 @@ BYTES_PER_CELL = {
 # asd2_stream_copy of 2^25 doubles); the roofline's `traffic` is this figure x cells.
 MEASURED_BYTES_PER_CELL = {
     "pcg64_spmv": 61.5, "pcg64_update": 64.4, "pcg32_spmv": 31.2, "pcg32_update": 32.2,
-    "fcg_spmv": 33.2, "fcg_update": 41.6, "weno_stage": (40.1 + 50.3 + 50.2) / 3,
+    "fcg_spmv": 32.4, "fcg_update": 41.6, "weno_stage": (40.1 + 50.3 + 50.2) / 3,
     "reinit": 30.6, "gfm": 100.6, "momentum": (59.9 + 58.9) / 2,
 }
 
