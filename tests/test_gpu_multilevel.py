@@ -50,6 +50,26 @@ def test_multilevel_dirichlet_side(oracle):
     assert st.converged == 1 and err <= 1e-8
 
 
+@pytest.mark.parametrize("bc", [("neumann",) * 4, ("neumann", "dirichlet", "neumann", "neumann")])
+def test_multilevel_imported_system(oracle, bc):
+    """a caller-built system (asd2_gfm_import: the oracle's own coefficients) takes the kernels'
+    general FP32 diagonal remainder, which an assembled system confines to the boundary ring"""
+    nx, ny = 160, 96
+    c_p = (0.0, 0.25, 0.0, 0.0)
+    gr, fl, phi, kap, H, div = gfm_case(oracle, nx, ny, seed=11, rho=(1000.0, 1.0), sigma=0.3, bc_p=bc, c_p=c_p)
+    ax, ay, dg, rhs = oracle.gfm_assemble(gr, fl, phi, kap, H, div)
+    x_ref, _, _, _ = oracle.pcg(gr, ax, ay, dg, rhs, 1e-13, 200000, 1)
+    ctx = A.Context(nx, ny, gr.dx, gr.dy)
+    ctx.set_bc(A.GRID_P, bc, c_p)
+    ctx.gfm_import(ax, ay, dg, rhs)
+    ps = ctx.array()
+    st = ctx.poisson_solve(ps, ctx.opts(tol=1e-10, precond=A.PRECOND_MULTILEVEL, precision=A.PREC_MIXED))
+    err = _rel(interior(gr, ps.download()), interior(gr, x_ref))
+    print(f"imported {bc}: multilevel {st.iterations} iters, true res {st.true_rel_residual:.2e}, rel L2 {err:.2e}")
+    assert st.converged == 1 and st.true_rel_residual <= 1e-10 and st.fallbacks == 0
+    assert err <= 1e-8
+
+
 def test_multilevel_rejects_fp64():
     ctx = A.Context(32, 32, 1 / 32, 1 / 32)
     ctx.set_bc(A.GRID_P, ("neumann",) * 4)
